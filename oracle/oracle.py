@@ -1,0 +1,326 @@
+"""ctypes front-end of the FP64 CPU oracle (sfem_oracle.c) and of the compiled
+reference (oracle/_ref/libsfem_ref.so, present only where the reference
+checkout was available at build time).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, as the checker.  The product (solidfem_amd/)
+never imports this module.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+REF_PATH = os.path.join(HERE, "_ref", "libsfem_ref.so")
+
+LINEAR, STVK, COROTATIONAL, NEO_HOOKEAN = 0, 1, 2, 3
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int32)
+
+
+class _Problem(C.Structure):
+    _fields_ = [
+        ("ne", C.c_int64), ("nn", C.c_int64),
+        ("tets", _ip), ("X0", _dp), ("fixed", C.POINTER(C.c_uint8)),
+        ("DmInv", _dp), ("vol", _dp), ("mu", _dp), ("lam", _dp), ("mass", _dp),
+        ("material", C.c_int32), ("nthreads", C.c_int32),
+        ("grav", C.c_double * 3),
+        ("ntri", C.c_int64), ("tris", _ip), ("pressure", C.c_double),
+        ("fext", _dp),
+        ("inc_ptr", C.POINTER(C.c_int64)), ("inc_idx", _ip),
+    ]
+
+
+def build(force=False):
+    """Compile liboracle.so (and the reference library when REF is present)."""
+    if force or not os.path.exists(LIB_PATH) or \
+            os.path.getmtime(LIB_PATH) < os.path.getmtime(os.path.join(HERE, "sfem_oracle.c")):
+        subprocess.check_call(["make", "-s", "-C", HERE, "all"])
+
+
+def build_ref():
+    ref_root = os.environ.get("SFEM_REFERENCE", "/root/reference")
+    if os.path.isdir(os.path.join(ref_root, "Source")):
+        subprocess.check_call(["make", "-s", "-C", HERE, "ref", f"REF={ref_root}"])
+    return os.path.exists(REF_PATH)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        P = C.POINTER(_Problem)
+        for name, args, res in [
+            ("orc_geometry", [P, _dp, _dp], None),
+            ("orc_lumped_mass", [P, C.c_double, _dp], None),
+            ("orc_stress", [C.c_int, C.c_double, C.c_double, _dp, _dp], None),
+            ("orc_dstress", [C.c_int, C.c_double, C.c_double, _dp, _dp, _dp], None),
+            ("orc_energy_density", [C.c_int, C.c_double, C.c_double, _dp], C.c_double),
+            ("orc_internal_forces", [P, _dp, _dp], None),
+            ("orc_energy", [P, _dp], C.c_double),
+            ("orc_matvec", [P, _dp, _dp, _dp], None),
+            ("orc_pressure_forces", [P, _dp, C.c_double, _dp], None),
+            ("orc_external", [P, _dp, C.c_double, _dp], None),
+            ("orc_gradients", [P, _dp, C.c_double, _dp], None),
+            ("orc_block_diag", [P, _dp, _dp], None),
+            ("orc_cg", [P, _dp, _dp, _dp, C.c_int, C.c_float, _dp], C.c_int),
+            ("orc_newton_cg_ref", [P, _dp, C.c_double, C.c_int, C.c_int, _ip], None),
+            ("orc_newton_solve", [P, _dp, C.c_double, C.c_double, C.c_int, C.c_double, C.c_int, _dp], C.c_int),
+        ]:
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = res
+        _lib = L
+    return _lib
+
+
+def _p(a, t=_dp):
+    return a.ctypes.data_as(t) if a is not None else None
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def stress(mat, mu, lam, F):
+    F = _f64(F).reshape(9)
+    P = np.zeros(9)
+    lib().orc_stress(mat, mu, lam, _p(F), _p(P))
+    return P.reshape(3, 3)
+
+
+def dstress(mat, mu, lam, F, dF):
+    F = _f64(F).reshape(9)
+    dF = _f64(dF).reshape(9)
+    dP = np.zeros(9)
+    lib().orc_dstress(mat, mu, lam, _p(F), _p(dF), _p(dP))
+    return dP.reshape(3, 3)
+
+
+def energy_density(mat, mu, lam, F):
+    return lib().orc_energy_density(mat, mu, lam, _p(_f64(F).reshape(9)))
+
+
+class OracleProblem:
+    """One mesh + material + loads, in the caller's original node order."""
+
+    def __init__(self, X0, tets, fixed, material, young=None, poisson=None, density=1070.0,
+                 gravity=(0.0, 0.0, 0.0), tris=None, pressure=0.0, fext=None,
+                 mu=None, lam=None, nthreads=0, incidence=False):
+        self.X0 = _f64(X0).reshape(-1, 3)
+        self.tets = np.ascontiguousarray(tets, dtype=np.int32).reshape(-1, 4)
+        self.fixed = np.ascontiguousarray(fixed, dtype=np.uint8).reshape(-1)
+        self.nn, self.ne = self.X0.shape[0], self.tets.shape[0]
+        if mu is None:
+            E = np.broadcast_to(np.asarray(young, dtype=np.float64), (self.ne,))
+            nu = np.broadcast_to(np.asarray(poisson, dtype=np.float64), (self.ne,))
+            # FemPhysicsBase.cpp:40-41
+            lam = E * nu / (1.0 + nu) / (1.0 - 2.0 * nu)
+            mu = 0.5 * E / (1.0 + nu)
+        self.mu = _f64(np.broadcast_to(mu, (self.ne,)))
+        self.lam = _f64(np.broadcast_to(lam, (self.ne,)))
+        self.material = int(material)
+        self.DmInv = np.zeros((self.ne, 9))
+        self.vol = np.zeros(self.ne)
+        self.mass = np.zeros(self.nn)
+        self.tris = None if tris is None else np.ascontiguousarray(tris, dtype=np.int32).reshape(-1, 3)
+        self.fext = None if fext is None else _f64(fext).reshape(-1, 3)
+        s = _Problem()
+        s.ne, s.nn = self.ne, self.nn
+        s.tets = _p(self.tets, _ip)
+        s.X0 = _p(self.X0)
+        s.fixed = _p(self.fixed, C.POINTER(C.c_uint8))
+        s.DmInv, s.vol, s.mu, s.lam, s.mass = (_p(a) for a in (self.DmInv, self.vol, self.mu, self.lam, self.mass))
+        s.material = self.material
+        s.nthreads = int(nthreads)
+        s.grav[:] = list(gravity)
+        s.ntri = 0 if self.tris is None else self.tris.shape[0]
+        s.tris = _p(self.tris, _ip) if self.tris is not None else None
+        s.pressure = float(pressure)
+        s.fext = _p(self.fext) if self.fext is not None else None
+        self.inc_ptr = self.inc_idx = None
+        if incidence:
+            slots = np.argsort(self.tets.reshape(-1), kind="stable")
+            counts = np.bincount(self.tets.reshape(-1), minlength=self.nn)
+            self.inc_ptr = np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
+            self.inc_idx = slots.astype(np.int32)
+            s.inc_ptr = _p(self.inc_ptr, C.POINTER(C.c_int64))
+            s.inc_idx = _p(self.inc_idx, _ip)
+        self.s = s
+        L = lib()
+        L.orc_geometry(C.byref(s), s.DmInv, s.vol)
+        L.orc_lumped_mass(C.byref(s), float(density), s.mass)
+
+    def _x(self, x):
+        return _f64(self.X0 if x is None else x).reshape(-1, 3)
+
+    def internal_forces(self, x=None):
+        x = self._x(x)
+        f = np.zeros_like(x)
+        lib().orc_internal_forces(C.byref(self.s), _p(x), _p(f))
+        return f
+
+    def energy(self, x=None):
+        return lib().orc_energy(C.byref(self.s), _p(self._x(x)))
+
+    def matvec(self, x, d):
+        x = self._x(x)
+        d = _f64(d).reshape(-1, 3)
+        q = np.zeros_like(x)
+        lib().orc_matvec(C.byref(self.s), _p(x), _p(d), _p(q))
+        return q
+
+    def pressure_forces(self, x=None, frac=1.0):
+        x = self._x(x)
+        f = np.zeros_like(x)
+        lib().orc_pressure_forces(C.byref(self.s), _p(x), frac, _p(f))
+        return f
+
+    def external(self, x=None, frac=1.0):
+        x = self._x(x)
+        r = np.zeros_like(x)
+        lib().orc_external(C.byref(self.s), _p(x), frac, _p(r))
+        return r
+
+    def gradients(self, x=None, frac=1.0):
+        x = self._x(x)
+        r = np.zeros_like(x)
+        lib().orc_gradients(C.byref(self.s), _p(x), frac, _p(r))
+        return r
+
+    def block_diag(self, x=None):
+        x = self._x(x)
+        B = np.zeros((self.nn, 9))
+        lib().orc_block_diag(C.byref(self.s), _p(x), _p(B))
+        return B.reshape(-1, 3, 3)
+
+    def cg(self, x, b, maxit, eps=1e-3, Binv=None, x0=None):
+        x = self._x(x)
+        b = _f64(b).reshape(-1, 3)
+        sol = np.zeros_like(x) if x0 is None else _f64(x0).reshape(-1, 3).copy()
+        Bi = None if Binv is None else _f64(Binv).reshape(-1, 9)
+        k = lib().orc_cg(C.byref(self.s), _p(x), _p(b), _p(sol), int(maxit), float(eps), _p(Bi))
+        return sol, k
+
+    def newton_cg_ref(self, x=None, frac=1.0, outer=10, inner=100):
+        x = self._x(x).copy()
+        its = np.zeros(outer, dtype=np.int32)
+        lib().orc_newton_cg_ref(C.byref(self.s), _p(x), frac, outer, inner, _p(its, _ip))
+        return x, its
+
+    def newton_solve(self, x=None, frac=1.0, tol=1e-10, max_newton=50, cg_rtol=1e-6, max_cg=100000):
+        x = self._x(x).copy()
+        st = np.zeros(4)
+        rc = lib().orc_newton_solve(C.byref(self.s), _p(x), frac, tol, max_newton, cg_rtol, max_cg, _p(st))
+        return x, {"converged": rc == 0, "newton": int(st[0]), "cg": int(st[1]),
+                   "res": st[2], "ext": st[3]}
+
+
+# ------------------------------------------------------- compiled reference
+_ref = None
+
+
+def ref_available():
+    return os.path.exists(REF_PATH)
+
+
+def reflib():
+    global _ref
+    if _ref is None:
+        L = C.CDLL(REF_PATH)
+        L.ref_create.argtypes = [_ip, C.c_int64, _dp, C.c_int64, C.POINTER(C.c_uint8), C.c_int,
+                                 C.c_double, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int,
+                                 C.c_double, C.c_int, C.c_int]
+        L.ref_create.restype = C.c_void_p
+        for name, args, res in [
+            ("ref_destroy", [C.c_void_p], None),
+            ("ref_set_positions", [C.c_void_p, _dp], None),
+            ("ref_get_positions", [C.c_void_p, _dp], None),
+            ("ref_set_pressure_surface", [C.c_void_p, _ip, C.c_int64, C.c_double], None),
+            ("ref_gradients", [C.c_void_p, _dp], None),
+            ("ref_matvec", [C.c_void_p, _dp, _dp], None),
+            ("ref_cg", [C.c_void_p, _dp, _dp, C.c_int], C.c_int),
+            ("ref_step", [C.c_void_p], None),
+            ("ref_newton_cg", [C.c_void_p, C.c_double], None),
+            ("ref_set_force_fraction", [C.c_void_p, C.c_double], None),
+            ("ref_force_fraction", [C.c_void_p], C.c_double),
+        ]:
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = res
+        _ref = L
+    return _ref
+
+
+class RefProblem:
+    """The compiled reference FemPhysicsMatrixFree on the same inputs.
+    Uniform material only (the reference has one mu, lambda); gravity along
+    z when zup (FemPhysicsBase.cpp:30-33)."""
+
+    ST_STATIC, ST_QUASI_STATIC = 0, 1
+
+    def __init__(self, X0, tets, fixed, material, young, poisson, density=1070.0, gravity=-9.8,
+                 zup=True, simtype=0, force_step=0.1, outer=10, inner=100, tris=None, pressure=0.0):
+        self.X0 = _f64(X0).reshape(-1, 3)
+        self.tets = np.ascontiguousarray(tets, dtype=np.int32).reshape(-1, 4)
+        self.fixed = np.ascontiguousarray(fixed, dtype=np.uint8).reshape(-1)
+        L = reflib()
+        # the reference prints its Lame parameters on construction
+        self.h = L.ref_create(_p(self.tets, _ip), self.tets.shape[0], _p(self.X0), self.X0.shape[0],
+                              _p(self.fixed, C.POINTER(C.c_uint8)), int(material), float(young),
+                              float(poisson), float(density), float(gravity), int(zup), int(simtype),
+                              float(force_step), int(outer), int(inner))
+        if tris is not None:
+            self.tris = np.ascontiguousarray(tris, dtype=np.int32).reshape(-1, 3)
+            L.ref_set_pressure_surface(self.h, _p(self.tris, _ip), self.tris.shape[0], float(pressure))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            reflib().ref_destroy(self.h)
+            self.h = None
+
+    def set_positions(self, x):
+        x = _f64(x).reshape(-1, 3)
+        reflib().ref_set_positions(self.h, _p(x))
+
+    def positions(self):
+        x = np.zeros_like(self.X0)
+        reflib().ref_get_positions(self.h, _p(x))
+        return x
+
+    def set_fraction(self, f):
+        reflib().ref_set_force_fraction(self.h, float(f))
+
+    def gradients(self):
+        r = np.zeros_like(self.X0)
+        reflib().ref_gradients(self.h, _p(r))
+        return r
+
+    def matvec(self, d):
+        d = _f64(d).reshape(-1, 3)
+        q = np.zeros_like(self.X0)
+        reflib().ref_matvec(self.h, _p(d), _p(q))
+        return q
+
+    def cg(self, b, iters):
+        b = _f64(b).reshape(-1, 3)
+        x = np.zeros_like(self.X0)
+        k = reflib().ref_cg(self.h, _p(b), _p(x), int(iters))
+        return x, k
+
+    def step(self):
+        reflib().ref_step(self.h)
+
+    def newton_cg(self, frac):
+        reflib().ref_newton_cg(self.h, float(frac))

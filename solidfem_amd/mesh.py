@@ -1,0 +1,171 @@
+"""Structured tetrahedral beam meshes and the loads the beam workloads use.
+
+The hex -> 6 tet split and the node numbering follow the reference's own beam
+fixture ``Models/box2.tet`` (an 8x2x2 grid) as it comes out of
+``IO::LoadFromTet1File`` (Source/FemIO.cpp:506-581, which swaps file y/z on
+read, :548):
+
+* node id ``n = ix*(nz+1)*(ny+1) + iz*(ny+1) + iy``;
+* hexes are visited ``for iy: for iz: for ix`` (x fastest);
+* each hex is cut into 6 tets around one body diagonal.  The diagonal's start
+  corner has offsets ``(ix%2, iy%2, 1 - iz%2)``, so neighbouring hexes mirror
+  and faces match.  The six tets are (D, ring[k], ring[k+1], D') over the ring
+  of the remaining six corners.
+
+Vertex order inside each tet is chosen so the signed volume is positive (the
+reference repairs orientation in FemBody::Prepare, Source/FemBody.cpp:127-156).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+# ring of the six corners around the 000-111 diagonal, as offsets XOR'ed with D
+_RING = np.array([[1, 0, 0], [1, 1, 0], [0, 1, 0], [0, 1, 1], [0, 0, 1], [1, 0, 1]], dtype=np.int64)
+
+
+def _splitmix64(x: np.ndarray) -> np.ndarray:
+    """Counter-based hash (splitmix64 finaliser) -> uint64."""
+    with np.errstate(over="ignore"):
+        z = (x.astype(np.uint64) + np.uint64(0x9E3779B97F4A7C15))
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        return z ^ (z >> np.uint64(31))
+
+
+def uniform01(seed: int, stream: int, n: int) -> np.ndarray:
+    """n deterministic uniforms in [0,1) from (seed, stream, index)."""
+    idx = np.arange(n, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        key = idx + np.uint64((seed * 0x100000001B3 + stream * 0x9E3779B1) & 0xFFFFFFFFFFFFFFFF)
+    h = _splitmix64(key)
+    return (h >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+
+
+class BeamMesh:
+    """Hex grid nx*ny*nz over [0,Lx]x[0,Ly]x[0,Lz], 6 tets per hex."""
+
+    def __init__(self, nx, ny, nz, Lx=1.0, Ly=0.25, Lz=0.25, jitter=0.0, seed=0):
+        self.nx, self.ny, self.nz = int(nx), int(ny), int(nz)
+        self.L = (float(Lx), float(Ly), float(Lz))
+        hx, hy, hz = Lx / nx, Ly / ny, Lz / nz
+        self.h = (hx, hy, hz)
+        NX, NY, NZ = nx + 1, ny + 1, nz + 1
+        ix, iz, iy = np.meshgrid(np.arange(NX), np.arange(NZ), np.arange(NY), indexing="ij")
+        # node id = ix*NZ*NY + iz*NY + iy  <=>  C-order over (ix, iz, iy)
+        X = np.stack([ix * hx, iy * hy, iz * hz], axis=-1).reshape(-1, 3).astype(np.float64)
+        self.grid_index = np.stack([ix, iy, iz], axis=-1).reshape(-1, 3)
+        if jitter > 0:
+            X = self._jitter(X, jitter, seed)
+        self.X = X
+        self.tets = self._tets()
+        self.nn = X.shape[0]
+        self.ne = self.tets.shape[0]
+
+    def node_id(self, ix, iy, iz):
+        return (ix * (self.nz + 1) + iz) * (self.ny + 1) + iy
+
+    def _jitter(self, X, amp, seed):
+        """Move nodes by U(-amp, amp) cells per axis; nodes on a boundary face
+        move only within that face."""
+        g = self.grid_index
+        n = X.shape[0]
+        out = X.copy()
+        dims = (self.nx, self.ny, self.nz)
+        for a in range(3):
+            u = uniform01(seed, 11 + a, n) * 2.0 - 1.0
+            on_face = (g[:, a] == 0) | (g[:, a] == dims[a])
+            out[:, a] += np.where(on_face, 0.0, u * amp * self.h[a])
+        return out
+
+    def _tets(self):
+        nx, ny, nz = self.nx, self.ny, self.nz
+        iy, iz, ix = np.meshgrid(np.arange(ny), np.arange(nz), np.arange(nx), indexing="ij")
+        ix, iy, iz = ix.ravel(), iy.ravel(), iz.ravel()          # hex order: iy, iz, ix(fastest)
+        D = np.stack([ix % 2, iy % 2, 1 - iz % 2], axis=1)
+        Dbar = 1 - D
+        corners = []
+        for k in range(6):
+            a = _RING[k] ^ D
+            b = _RING[(k + 1) % 6] ^ D
+            corners.append((D, a, b, Dbar))
+        out = np.empty((ix.size, 6, 4), dtype=np.int64)
+        for k, quad in enumerate(corners):
+            for v, off in enumerate(quad):
+                out[:, k, v] = self.node_id(ix + off[:, 0], iy + off[:, 1], iz + off[:, 2])
+        tets = out.reshape(-1, 4)
+        # orientation: positive signed volume
+        P = self.X[tets]
+        vol = np.einsum("ij,ij->i", np.cross(P[:, 1] - P[:, 0], P[:, 2] - P[:, 0]), P[:, 3] - P[:, 0])
+        neg = vol < 0
+        tets[neg, 1], tets[neg, 2] = tets[neg, 2].copy(), tets[neg, 1].copy()
+        return np.ascontiguousarray(tets.astype(np.int32))
+
+    # ------------------------------------------------------------ boundary sets
+    def fixed_mask(self):
+        """Nodes on the x = 0 face (the clamped end)."""
+        return (self.grid_index[:, 0] == 0).astype(np.uint8)
+
+    def tip_triangles(self):
+        """Tet faces lying on x = Lx, oriented with outward (+x) normals."""
+        on = self.grid_index[:, 0] == self.nx
+        t = self.tets
+        faces = np.array([[1, 2, 3], [0, 2, 3], [0, 1, 3], [0, 1, 2]])
+        # keep hex order: sort by source tet index
+        tri_all = []
+        for f in faces:
+            tri = t[:, f]
+            sel = np.nonzero(on[tri].all(axis=1))[0]
+            tri_all.append(np.stack([sel, tri[sel][:, 0], tri[sel][:, 1], tri[sel][:, 2]], axis=1))
+        T = np.concatenate(tri_all, axis=0)
+        T = T[np.argsort(T[:, 0], kind="stable")][:, 1:]
+        P = self.X[T]
+        n = np.cross(P[:, 1] - P[:, 0], P[:, 2] - P[:, 0])
+        flip = n[:, 0] < 0
+        T[flip, 1], T[flip, 2] = T[flip, 2].copy(), T[flip, 1].copy()
+        return np.ascontiguousarray(T.astype(np.int32))
+
+
+def traction_nodal_forces(X, tris, traction_fn):
+    """Consistent nodal loads of a dead traction t(p) that is linear over each
+    triangle: f_i = A/12 (2 t_i + t_j + t_k).  Returns an (nn,3) array."""
+    f = np.zeros_like(X)
+    P = X[tris]
+    area = 0.5 * np.linalg.norm(np.cross(P[:, 1] - P[:, 0], P[:, 2] - P[:, 0]), axis=1)
+    T = np.stack([traction_fn(P[:, k]) for k in range(3)], axis=1)   # (nt,3,3)
+    for k in range(3):
+        fk = (area / 12.0)[:, None] * (2 * T[:, k] + T[:, (k + 1) % 3] + T[:, (k + 2) % 3])
+        np.add.at(f, tris[:, k], fk)
+    return f
+
+
+def torsion_traction(center, tau):
+    """Tangential dead traction t(p) = tau * e_x x (p - c) on an x = const face."""
+    c = np.asarray(center, dtype=np.float64)
+
+    def fn(p):
+        r = p - c
+        return tau * np.stack([np.zeros(len(p)), -r[:, 2], r[:, 1]], axis=1)
+    return fn
+
+
+def lame(E, nu):
+    """mu = E/(2(1+nu)), lambda = E nu/((1+nu)(1-2nu))  (Source/FemPhysicsBase.cpp:40-41)."""
+    E = np.asarray(E, dtype=np.float64)
+    nu = np.asarray(nu, dtype=np.float64)
+    lam = E * nu / (1.0 + nu) / (1.0 - 2.0 * nu)
+    mu = 0.5 * E / (1.0 + nu)
+    return mu, lam
+
+
+def load_tet1(path, scale=1.0):
+    """Read a `.tet` file the way IO::LoadFromTet1File does (Source/FemIO.cpp:506-581):
+    'v x y z' lines (y and z swapped on read) and 't a b c d' lines."""
+    X, T = [], []
+    with open(path) as fh:
+        for line in fh:
+            if line.startswith("v"):
+                x, y, z = (float(v) for v in line.split()[1:4])
+                X.append((x * scale, z * scale, y * scale))
+            elif line.startswith("t"):
+                T.append(tuple(int(v) for v in line.split()[1:5]))
+    return np.array(X, dtype=np.float64), np.array(T, dtype=np.int32)
