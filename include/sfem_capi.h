@@ -1,0 +1,164 @@
+/*
+ * sfem_capi.h -- C-ABI of the MI355X matrix-free FEM elasticity library
+ * (libsfem_hip.so).  Drop-in for SolidFEM's FemPhysicsMatrixFree Newton-CG path.
+ *
+ * Conventions
+ *   - Every entry point returns 0 on success and a negative SFEM_E* code on
+ *     failure; sfem_last_error() gives the message (thread-local).  Nothing
+ *     throws across the ABI.
+ *   - Node arrays are nn*3 doubles (x,y,z interleaved) in the caller's ORIGINAL
+ *     node order.  The reference works on free nodes in its own shuffled order
+ *     (FemPhysicsMatrixFree.cpp:229-263); here rows of fixed nodes are present
+ *     and zero wherever the reference drops them.
+ *   - Host buffers are borrowed for the duration of a call.  The context owns
+ *     all device memory; calls are synchronous at return.  One context is not
+ *     thread-safe; distinct contexts may be used from distinct threads.
+ *   - Element kernels and the CG operator run in FP32; residuals, dot products,
+ *     norms, line search and the Newton update run in FP64 on the device.
+ */
+#ifndef SFEM_CAPI_H
+#define SFEM_CAPI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* MaterialModelType values (Include/FemDataStructures.h:84-88) */
+enum { SFEM_LINEAR = 0, SFEM_STVK = 1, SFEM_COROTATIONAL = 2, SFEM_NEO_HOOKEAN = 3 };
+
+/* SimulationType values (Include/FemDataStructures.h:71-77); explicit and
+   implicit dynamics are not on this path */
+enum { SFEM_ST_STATIC = 0, SFEM_ST_QUASI_STATIC = 1 };
+
+/* Solve modes.
+   REFERENCE: SolveNewtonCG exactly (FemPhysicsMatrixFree.cpp:604-628): outer
+     Newton steps, each an unpreconditioned CG from zero to |r.r| <= (1e-3f)^2
+     |r0.r0| or inner_iterations; no residual test, no line search.
+   CONVERGED: Newton to ||r||_2 <= tol * ||r_ext||_2 (FP64), PCG inner solves
+     to cg_rtol, FP64 back-tracking line search, Jacobi fallback judge.       */
+enum { SFEM_SOLVE_REFERENCE = 0, SFEM_SOLVE_CONVERGED = 1 };
+
+/* Preconditioners for the inner CG */
+enum { SFEM_PC_NONE = 0, SFEM_PC_JACOBI = 1, SFEM_PC_LEARNED = 2 };
+
+enum {
+    SFEM_OK = 0,
+    SFEM_EINVAL = -1,     /* bad argument */
+    SFEM_EDEVICE = -2,    /* HIP runtime error */
+    SFEM_ENOMEM = -3,     /* device allocation failed */
+    SFEM_ENOTCONV = -4,   /* solve did not reach its tolerance */
+    SFEM_ESTATE = -5      /* call not valid in the current state */
+};
+
+typedef struct sfem_ctx sfem_ctx;
+
+typedef struct {
+    const int32_t* tets;    /* ne*4 node ids (Tet::idx, FemDataStructures.h:167-170) */
+    int64_t ne;
+    const double* X0;       /* nn*3 rest positions (Node::pos0) */
+    int64_t nn;
+    const uint8_t* fixed;   /* nn; 1 = Dirichlet node (Node::invMass == 0) */
+} sfem_mesh;
+
+typedef struct {
+    int32_t material;       /* SFEM_LINEAR .. SFEM_NEO_HOOKEAN (FemConfig::mMaterial) */
+    double young;           /* FemConfig::mYoungsModulus */
+    double poisson;         /* FemConfig::mPoissonRatio */
+    const double* young_e;  /* optional per-element E (ne), overrides young */
+    const double* poisson_e;/* optional per-element nu (ne), overrides poisson */
+    double density;         /* FemConfig::mDensity */
+    double gravity[3];      /* FemPhysicsBase::mGravity (FemPhysicsBase.cpp:30-33) */
+} sfem_material;
+
+typedef struct {
+    int32_t mode;              /* SFEM_SOLVE_REFERENCE | SFEM_SOLVE_CONVERGED */
+    int32_t precond;           /* SFEM_PC_* (REFERENCE mode ignores it: none) */
+    int32_t outer_iterations;  /* FemConfig::mOuterIterations; CONVERGED: max Newton steps */
+    int32_t inner_iterations;  /* FemConfig::mInnerIterations; max CG iterations per step */
+    double tol;                /* CONVERGED: relative FP64 residual target (e.g. 1e-8) */
+    double cg_rtol;            /* CONVERGED: inner forcing ||r_cg|| <= cg_rtol ||b|| */
+    int32_t line_search;       /* CONVERGED: 1 = FP64 back-tracking on ||r|| */
+    int32_t check_every;       /* CG iterations between host convergence checks (>=1) */
+    int32_t sim_type;          /* sfem_step only: SFEM_ST_STATIC | SFEM_ST_QUASI_STATIC */
+    double force_step;         /* sfem_step only: FemConfig::mForceApplicationStep */
+} sfem_solve_params;
+
+typedef struct {
+    int32_t newton_iters;      /* Newton steps taken */
+    int32_t cg_iters;          /* total CG iterations */
+    int32_t converged;         /* CONVERGED mode: 1 if ||r|| <= tol ||r_ext|| */
+    int32_t fallbacks;         /* learned-PC steps rerun with Jacobi */
+    double res0;               /* ||r||_2 at the start (FP64) */
+    double res;                /* ||r||_2 at the end (FP64) */
+    double ext_norm;           /* ||r_ext||_2 at the end (FP64) */
+    double seconds;            /* device wall time of the solve */
+    int64_t hessvec_evals;     /* element Hessian-vector evaluations (ne per apply) */
+    int64_t force_evals;       /* element internal-force evaluations (ne per residual) */
+} sfem_solve_stats;
+
+/* FemPhysicsMatrixFree(tets, nodes, config) (Include/FemPhysicsMatrixFree.h:61-62,
+   Source/FemPhysicsMatrixFree.cpp:71-213): copies the mesh to the device,
+   computes Dm^-1, volumes and lumped masses, partitions elements. */
+int sfem_create(const sfem_mesh* mesh, const sfem_material* mat, sfem_ctx** out);
+
+/* releases every device buffer of the context */
+void sfem_destroy(sfem_ctx* ctx);
+
+/* message of the last failed call on this thread */
+const char* sfem_last_error(void);
+
+/* SetDeformedPosition for all nodes (Include/FemPhysicsMatrixFree.h:72) */
+int sfem_set_positions(sfem_ctx* ctx, const double* x);
+
+/* GetDeformedPosition for all nodes, original order (Include/FemPhysicsMatrixFree.h:70) */
+int sfem_get_positions(sfem_ctx* ctx, double* x);
+
+/* the load fraction mForceFraction (FemPhysicsBase.h:165; GetForceFraction :108) */
+int sfem_set_load_fraction(sfem_ctx* ctx, double frac);
+int sfem_get_load_fraction(sfem_ctx* ctx, double* frac);
+
+/* SetBoundaryConditionsSurface (FemPhysicsMatrixFree.cpp:1051-1060): follower
+   pressure on a triangle list (original node ids) */
+int sfem_set_pressure_surface(sfem_ctx* ctx, const int32_t* tris, int64_t ntri, double pressure);
+
+/* SetExternalForces (FemPhysicsBase.h:68; asserts for the matrix-free class in
+   the reference): dead nodal loads scaled by the load fraction.  NULL clears. */
+int sfem_set_external_forces(sfem_ctx* ctx, const double* f);
+
+/* ElasticEnergy::ComputeForces (Source/ElasticEnergy.cpp:259-282), FP64, all rows */
+int sfem_internal_forces(sfem_ctx* ctx, double* f);
+
+/* ComputeGradients (FemPhysicsMatrixFree.cpp:536-560), FP64; fixed rows 0 */
+int sfem_gradients(sfem_ctx* ctx, double* r);
+
+/* MatrixVectorMultiply (FemPhysicsMatrixFree.cpp:523-534), static: K d on free
+   rows, FP32 element kernel, fixed rows of d ignored and of Kd zero */
+int sfem_matvec(sfem_ctx* ctx, const double* d, double* Kd);
+
+/* 3x3 diagonal blocks of K per node (nn*9, row-major), FP32 kernel; fixed
+   nodes get the identity.  The block-Jacobi preconditioner is their inverse. */
+int sfem_block_diagonal(sfem_ctx* ctx, double* B);
+
+/* Solve() with NST_NEWTON_CG (FemPhysicsMatrixFree.cpp:570-628) at the current
+   load fraction, from the current positions */
+int sfem_solve(sfem_ctx* ctx, const sfem_solve_params* p, sfem_solve_stats* st);
+
+/* Step(0) (FemPhysicsMatrixFree.cpp:297-342): ST_STATIC solves once at full
+   load; ST_QUASI_STATIC raises the load fraction by force_step and solves */
+int sfem_step(sfem_ctx* ctx, const sfem_solve_params* p, sfem_solve_stats* st);
+
+/* Installs per-node 3x3 inverse preconditioner blocks (nn*9 FP32 row-major,
+   original order) used by SFEM_PC_LEARNED; NULL uninstalls.  No reference
+   counterpart: the reference CG's only preconditioner is a probed scalar
+   diagonal (Math/IterativeSolvers.h:133-152). */
+int sfem_set_precond_blocks(sfem_ctx* ctx, const float* minv9);
+
+/* Number of elements, nodes, fixed nodes, and element blocks of the partition */
+int sfem_info(sfem_ctx* ctx, int64_t* ne, int64_t* nn, int64_t* nfixed, int64_t* nblocks);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SFEM_CAPI_H */

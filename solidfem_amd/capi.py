@@ -1,0 +1,216 @@
+"""ctypes binding of libsfem_hip.so (the C-ABI in include/sfem_capi.h).
+
+The shared library is the product path; there is no fallback.  Importing this
+module does not need a GPU, but creating a context does.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libsfem_hip.so")
+
+LINEAR, STVK, COROTATIONAL, NEO_HOOKEAN = 0, 1, 2, 3
+ST_STATIC, ST_QUASI_STATIC = 0, 1
+SOLVE_REFERENCE, SOLVE_CONVERGED = 0, 1
+PC_NONE, PC_JACOBI, PC_LEARNED = 0, 1, 2
+ENOTCONV = -4
+
+EXPORTS = [
+    "sfem_create", "sfem_destroy", "sfem_last_error", "sfem_set_positions", "sfem_get_positions",
+    "sfem_set_load_fraction", "sfem_get_load_fraction", "sfem_set_pressure_surface",
+    "sfem_set_external_forces", "sfem_internal_forces", "sfem_gradients", "sfem_matvec",
+    "sfem_block_diagonal", "sfem_solve", "sfem_step", "sfem_set_precond_blocks", "sfem_info",
+]
+
+_dp = C.POINTER(C.c_double)
+_fp = C.POINTER(C.c_float)
+_ip = C.POINTER(C.c_int32)
+
+
+class Mesh(C.Structure):
+    _fields_ = [("tets", _ip), ("ne", C.c_int64), ("X0", _dp), ("nn", C.c_int64),
+                ("fixed", C.POINTER(C.c_uint8))]
+
+
+class Material(C.Structure):
+    _fields_ = [("material", C.c_int32), ("young", C.c_double), ("poisson", C.c_double),
+                ("young_e", _dp), ("poisson_e", _dp), ("density", C.c_double),
+                ("gravity", C.c_double * 3)]
+
+
+class SolveParams(C.Structure):
+    _fields_ = [("mode", C.c_int32), ("precond", C.c_int32), ("outer_iterations", C.c_int32),
+                ("inner_iterations", C.c_int32), ("tol", C.c_double), ("cg_rtol", C.c_double),
+                ("line_search", C.c_int32), ("check_every", C.c_int32), ("sim_type", C.c_int32),
+                ("force_step", C.c_double)]
+
+
+class SolveStats(C.Structure):
+    _fields_ = [("newton_iters", C.c_int32), ("cg_iters", C.c_int32), ("converged", C.c_int32),
+                ("fallbacks", C.c_int32), ("res0", C.c_double), ("res", C.c_double),
+                ("ext_norm", C.c_double), ("seconds", C.c_double), ("hessvec_evals", C.c_int64),
+                ("force_evals", C.c_int64)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class SfemError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"sfem error {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def lib():
+    """Load libsfem_hip.so; raises if it is not built (no fallback path)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is not built; run __graft_entry__.build()")
+        L = C.CDLL(LIB_PATH)
+        V = C.c_void_p
+        sig = {
+            "sfem_create": ([C.POINTER(Mesh), C.POINTER(Material), C.POINTER(V)], C.c_int),
+            "sfem_destroy": ([V], None),
+            "sfem_last_error": ([], C.c_char_p),
+            "sfem_set_positions": ([V, _dp], C.c_int),
+            "sfem_get_positions": ([V, _dp], C.c_int),
+            "sfem_set_load_fraction": ([V, C.c_double], C.c_int),
+            "sfem_get_load_fraction": ([V, _dp], C.c_int),
+            "sfem_set_pressure_surface": ([V, _ip, C.c_int64, C.c_double], C.c_int),
+            "sfem_set_external_forces": ([V, _dp], C.c_int),
+            "sfem_internal_forces": ([V, _dp], C.c_int),
+            "sfem_gradients": ([V, _dp], C.c_int),
+            "sfem_matvec": ([V, _dp, _dp], C.c_int),
+            "sfem_block_diagonal": ([V, _dp], C.c_int),
+            "sfem_solve": ([V, C.POINTER(SolveParams), C.POINTER(SolveStats)], C.c_int),
+            "sfem_step": ([V, C.POINTER(SolveParams), C.POINTER(SolveStats)], C.c_int),
+            "sfem_set_precond_blocks": ([V, _fp], C.c_int),
+            "sfem_info": ([V, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64),
+                           C.POINTER(C.c_int64)], C.c_int),
+        }
+        for name, (args, res) in sig.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = res
+        _lib = L
+    return _lib
+
+
+def _check(rc, allow=()):
+    if rc != 0 and rc not in allow:
+        raise SfemError(rc, lib().sfem_last_error().decode())
+    return rc
+
+
+def _p(a, t=_dp):
+    return None if a is None else a.ctypes.data_as(t)
+
+
+def solve_params(mode=SOLVE_CONVERGED, precond=PC_JACOBI, outer=50, inner=20000, tol=1e-8, cg_rtol=1e-2,
+                 line_search=1, check_every=16, sim_type=ST_STATIC, force_step=0.1):
+    return SolveParams(mode, precond, outer, inner, tol, cg_rtol, line_search, check_every, sim_type, force_step)
+
+
+class Solver:
+    """One device context: mesh, material, loads, state (original node order)."""
+
+    def __init__(self, X0, tets, fixed, material, young=1e6, poisson=0.45, density=1070.0,
+                 gravity=(0.0, 0.0, 0.0), young_e=None, poisson_e=None):
+        self.X0 = np.ascontiguousarray(X0, dtype=np.float64).reshape(-1, 3)
+        self.tets = np.ascontiguousarray(tets, dtype=np.int32).reshape(-1, 4)
+        self.fixed = np.ascontiguousarray(fixed, dtype=np.uint8).reshape(-1)
+        self.nn, self.ne = self.X0.shape[0], self.tets.shape[0]
+        self._ye = None if young_e is None else np.ascontiguousarray(young_e, dtype=np.float64)
+        self._pe = None if poisson_e is None else np.ascontiguousarray(poisson_e, dtype=np.float64)
+        mesh = Mesh(_p(self.tets, _ip), self.ne, _p(self.X0), self.nn, _p(self.fixed, C.POINTER(C.c_uint8)))
+        g = (C.c_double * 3)(*gravity)
+        mat = Material(int(material), float(young), float(poisson), _p(self._ye), _p(self._pe), float(density), g)
+        h = C.c_void_p()
+        _check(lib().sfem_create(C.byref(mesh), C.byref(mat), C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().sfem_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def info(self):
+        v = [C.c_int64() for _ in range(4)]
+        _check(lib().sfem_info(self.h, *[C.byref(x) for x in v]))
+        return dict(zip(("ne", "nn", "nfixed", "nblocks"), (x.value for x in v)))
+
+    def set_positions(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, 3)
+        _check(lib().sfem_set_positions(self.h, _p(x)))
+
+    def positions(self):
+        x = np.zeros((self.nn, 3))
+        _check(lib().sfem_get_positions(self.h, _p(x)))
+        return x
+
+    def set_load_fraction(self, f):
+        _check(lib().sfem_set_load_fraction(self.h, float(f)))
+
+    def load_fraction(self):
+        f = C.c_double()
+        _check(lib().sfem_get_load_fraction(self.h, C.byref(f)))
+        return f.value
+
+    def set_pressure_surface(self, tris, pressure):
+        self._tris = np.ascontiguousarray(tris, dtype=np.int32).reshape(-1, 3)
+        _check(lib().sfem_set_pressure_surface(self.h, _p(self._tris, _ip), self._tris.shape[0], float(pressure)))
+
+    def set_external_forces(self, f):
+        self._fext = None if f is None else np.ascontiguousarray(f, dtype=np.float64).reshape(-1, 3)
+        _check(lib().sfem_set_external_forces(self.h, _p(self._fext)))
+
+    def internal_forces(self):
+        f = np.zeros((self.nn, 3))
+        _check(lib().sfem_internal_forces(self.h, _p(f)))
+        return f
+
+    def gradients(self):
+        r = np.zeros((self.nn, 3))
+        _check(lib().sfem_gradients(self.h, _p(r)))
+        return r
+
+    def matvec(self, d):
+        d = np.ascontiguousarray(d, dtype=np.float64).reshape(-1, 3)
+        q = np.zeros_like(d)
+        _check(lib().sfem_matvec(self.h, _p(d), _p(q)))
+        return q
+
+    def block_diagonal(self):
+        B = np.zeros((self.nn, 9))
+        _check(lib().sfem_block_diagonal(self.h, _p(B)))
+        return B.reshape(-1, 3, 3)
+
+    def set_precond_blocks(self, minv):
+        self._minv = None if minv is None else np.ascontiguousarray(minv, dtype=np.float32).reshape(-1, 9)
+        _check(lib().sfem_set_precond_blocks(self.h, _p(self._minv, _fp)))
+
+    def solve(self, params=None, allow_notconv=False, **kw):
+        p = params if params is not None else solve_params(**kw)
+        st = SolveStats()
+        _check(lib().sfem_solve(self.h, C.byref(p), C.byref(st)), allow=(ENOTCONV,) if allow_notconv else ())
+        return st.as_dict()
+
+    def step(self, params=None, allow_notconv=False, **kw):
+        p = params if params is not None else solve_params(**kw)
+        st = SolveStats()
+        _check(lib().sfem_step(self.h, C.byref(p), C.byref(st)), allow=(ENOTCONV,) if allow_notconv else ())
+        return st.as_dict()

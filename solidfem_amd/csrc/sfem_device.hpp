@@ -1,0 +1,294 @@
+// sfem_device.hpp -- per-tetrahedron constitutive math for the MI355X kernels.
+//
+// All element quantities are computed from DISPLACEMENTS: with the rest shape
+// matrix Dm and G = Dm^-1 (row-major g[3k+j] = G(k,j)),
+//     F  = I + [u1-u0 | u2-u0 | u3-u0] G          (= Ds G, FemPhysicsMatrixFree.cpp:354-368)
+//     dF = [d1-d0 | d2-d0 | d3-d0] G              (:426-430)
+// so FP32 evaluation does not lose the small strains to cancellation against
+// the rest positions.  Material coefficients arrive pre-multiplied by the rest
+// volume (cmu = V mu, clam = V lambda): every stress and stress differential
+// below is linear in (mu, lambda), so the result is already V P / V dP.
+//
+// Corner outputs: column k of (V dP) G^T is the contribution to corner k+1,
+// corner 0 gets minus their sum (FemPhysicsMatrixFree.cpp:479-495).
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace sfem {
+
+enum { LINEAR = 0, STVK = 1, COROTATIONAL = 2, NEO_HOOKEAN = 3 };
+
+template <typename T> __device__ __forceinline__ T dsqrt(T x) { return sqrt(x); }
+template <typename T> __device__ __forceinline__ T dlog(T x) { return log(x); }
+
+// C = A B (3x3 row-major)
+template <typename T>
+__device__ __forceinline__ void mul3(const T* A, const T* B, T* C)
+{
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++)
+            C[3 * i + j] = A[3 * i] * B[j] + A[3 * i + 1] * B[3 + j] + A[3 * i + 2] * B[6 + j];
+}
+
+// adjugate inverse (Math/matrix3.h:234-249); returns the determinant
+template <typename T>
+__device__ __forceinline__ T inv3(const T* m, T* v)
+{
+    v[0] = m[4] * m[8] - m[5] * m[7];
+    v[3] = -m[3] * m[8] + m[6] * m[5];
+    v[6] = m[3] * m[7] - m[6] * m[4];
+    v[1] = m[2] * m[7] - m[1] * m[8];
+    v[4] = m[0] * m[8] - m[2] * m[6];
+    v[7] = m[1] * m[6] - m[0] * m[7];
+    v[2] = m[1] * m[5] - m[2] * m[4];
+    v[5] = m[2] * m[3] - m[0] * m[5];
+    v[8] = m[0] * m[4] - m[1] * m[3];
+    T det = m[0] * v[0] + m[1] * v[3] + m[2] * v[6];
+    T s = T(1) / det;
+#pragma unroll
+    for (int k = 0; k < 9; k++) v[k] *= s;
+    return det;
+}
+
+// [a1|a2|a3] G  with a_k = w_k - w_0
+template <typename T>
+__device__ __forceinline__ void grad4(const T* g, const T w[4][3], T* M)
+{
+    T a[3][3];
+#pragma unroll
+    for (int k = 0; k < 3; k++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) a[k][c] = w[k + 1][c] - w[0][c];
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++)
+            M[3 * i + j] = a[0][i] * g[j] + a[1][i] * g[3 + j] + a[2][i] * g[6 + j];
+}
+
+// corner outputs from a (volume-scaled) stress-like matrix S: col k of S G^T
+template <typename T>
+__device__ __forceinline__ void corners(const T* S, const T* g, T sign, T out[4][3])
+{
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        T h0 = sign * (S[3 * c] * g[0] + S[3 * c + 1] * g[1] + S[3 * c + 2] * g[2]);
+        T h1 = sign * (S[3 * c] * g[3] + S[3 * c + 1] * g[4] + S[3 * c + 2] * g[5]);
+        T h2 = sign * (S[3 * c] * g[6] + S[3 * c + 1] * g[7] + S[3 * c + 2] * g[8]);
+        out[1][c] = h0;
+        out[2][c] = h1;
+        out[3][c] = h2;
+        out[0][c] = -(h0 + h1 + h2);
+    }
+}
+
+// Gram-Schmidt rotation of PolarDecomposition::QRPolarDecomposition
+// (Source/PolarDecomposition.cpp:269-280) and S = sym(R^T F) (:37-47)
+template <typename T>
+__device__ __forceinline__ void qr_rot(const T* F, T* R, T* S)
+{
+    T r0[3] = {F[0], F[3], F[6]};
+    T s = T(1) / dsqrt(r0[0] * r0[0] + r0[1] * r0[1] + r0[2] * r0[2]);
+    r0[0] *= s; r0[1] *= s; r0[2] *= s;
+    T a1[3] = {F[1], F[4], F[7]};
+    T dd = a1[0] * r0[0] + a1[1] * r0[1] + a1[2] * r0[2];
+    T r1[3] = {a1[0] - dd * r0[0], a1[1] - dd * r0[1], a1[2] - dd * r0[2]};
+    s = T(1) / dsqrt(r1[0] * r1[0] + r1[1] * r1[1] + r1[2] * r1[2]);
+    r1[0] *= s; r1[1] *= s; r1[2] *= s;
+    T r2[3] = {r0[1] * r1[2] - r1[1] * r0[2], -r0[0] * r1[2] + r0[2] * r1[0], r0[0] * r1[1] - r1[0] * r0[1]};
+    s = T(1) / dsqrt(r2[0] * r2[0] + r2[1] * r2[1] + r2[2] * r2[2]);
+    r2[0] *= s; r2[1] *= s; r2[2] *= s;
+#pragma unroll
+    for (int i = 0; i < 3; i++) { R[3 * i] = r0[i]; R[3 * i + 1] = r1[i]; R[3 * i + 2] = r2[i]; }
+    T V[9];
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++)
+            V[3 * i + j] = R[i] * F[j] + R[3 + i] * F[3 + j] + R[6 + i] * F[6 + j];   // (R^T F)_ij
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) S[3 * i + j] = T(0.5) * (V[3 * i + j] + V[3 * j + i]);
+}
+
+// ---------------------------------------------------------------------------
+// State-dependent part of the tangent, computed once per element per apply.
+template <typename T, int MAT> struct Tangent;
+
+// LINEAR: dP = mu (dF + dF^T) + lam tr(dF) I   (FemPhysicsMatrixFree.cpp:436-440)
+template <typename T> struct Tangent<T, LINEAR> {
+    T cmu, clam;
+    __device__ __forceinline__ void init(const T*, T m, T l) { cmu = m; clam = l; }
+    __device__ __forceinline__ void apply(const T* dF, T* dP) const
+    {
+        T tr = dF[0] + dF[4] + dF[8];
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++) dP[3 * i + j] = cmu * (dF[3 * i + j] + dF[3 * j + i]);
+        dP[0] += clam * tr; dP[4] += clam * tr; dP[8] += clam * tr;
+    }
+};
+
+// STVK: dP = dF (2mu E + lam trE I) + F (2mu dE + lam tr dE I), dE = sym(F^T dF)
+// (:457-463)
+template <typename T> struct Tangent<T, STVK> {
+    T F[9], S2[9], cmu, clam;
+    __device__ __forceinline__ void init(const T* Fin, T m, T l)
+    {
+        cmu = m; clam = l;
+#pragma unroll
+        for (int k = 0; k < 9; k++) F[k] = Fin[k];
+        T E[9];
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++)
+                E[3 * i + j] = T(0.5) * (F[i] * F[j] + F[3 + i] * F[3 + j] + F[6 + i] * F[6 + j] - (i == j ? T(1) : T(0)));
+        T tr = E[0] + E[4] + E[8];
+#pragma unroll
+        for (int k = 0; k < 9; k++) S2[k] = 2 * cmu * E[k];
+        S2[0] += clam * tr; S2[4] += clam * tr; S2[8] += clam * tr;
+    }
+    __device__ __forceinline__ void apply(const T* dF, T* dP) const
+    {
+        T FtdF[9];
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++)
+                FtdF[3 * i + j] = F[i] * dF[j] + F[3 + i] * dF[3 + j] + F[6 + i] * dF[6 + j];
+        T trdE = FtdF[0] + FtdF[4] + FtdF[8];
+        T B[9];
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++) B[3 * i + j] = cmu * (FtdF[3 * i + j] + FtdF[3 * j + i]);
+        B[0] += clam * trdE; B[4] += clam * trdE; B[8] += clam * trdE;
+        T t1[9], t2[9];
+        mul3(dF, S2, t1);
+        mul3(F, B, t2);
+#pragma unroll
+        for (int k = 0; k < 9; k++) dP[k] = t1[k] + t2[k];
+    }
+};
+
+// COROTATIONAL with the reference's QR rotation and its (non-symmetric)
+// tangent (:441-456): dS = R^T dF; dP = 2mu dF + lam tr(dS) R
+//   + (lam (trS-3) - 2mu) R skew((S - trS I)^-1 w(dS))
+template <typename T> struct Tangent<T, COROTATIONAL> {
+    T R[9], Ai[9], cmu, clam, cc;
+    __device__ __forceinline__ void init(const T* F, T m, T l)
+    {
+        cmu = m; clam = l;
+        T S[9];
+        qr_rot(F, R, S);
+        T trS = S[0] + S[4] + S[8];
+        T A[9];
+#pragma unroll
+        for (int k = 0; k < 9; k++) A[k] = S[k];
+        A[0] -= trS; A[4] -= trS; A[8] -= trS;
+        inv3(A, Ai);
+        cc = clam * (trS - 3) - 2 * cmu;
+    }
+    __device__ __forceinline__ void apply(const T* dF, T* dP) const
+    {
+        T dS[9];
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++)
+                dS[3 * i + j] = R[i] * dF[j] + R[3 + i] * dF[3 + j] + R[6 + i] * dF[6 + j];
+        T tr = dS[0] + dS[4] + dS[8];
+        T w0 = dS[5] - dS[7], w1 = dS[2] - dS[6], w2 = dS[1] - dS[3];
+        T r0 = Ai[0] * w0 + Ai[1] * w1 + Ai[2] * w2;
+        T r1 = Ai[3] * w0 + Ai[4] * w1 + Ai[5] * w2;
+        T r2 = Ai[6] * w0 + Ai[7] * w1 + Ai[8] * w2;
+        // dR = R skew(r), skew(r) = [[0,-r2,r1],[r2,0,-r0],[-r1,r0,0]] (matrix3.h:264-269)
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            T d0 = R[3 * i + 1] * r2 - R[3 * i + 2] * r1;
+            T d1 = -R[3 * i] * r2 + R[3 * i + 2] * r0;
+            T d2 = R[3 * i] * r1 - R[3 * i + 1] * r0;
+            dP[3 * i + 0] = 2 * cmu * dF[3 * i + 0] + clam * tr * R[3 * i + 0] + cc * d0;
+            dP[3 * i + 1] = 2 * cmu * dF[3 * i + 1] + clam * tr * R[3 * i + 1] + cc * d1;
+            dP[3 * i + 2] = 2 * cmu * dF[3 * i + 2] + clam * tr * R[3 * i + 2] + cc * d2;
+        }
+    }
+};
+
+// NEO_HOOKEAN: dP = mu dF + (mu - lam ln J) F^-T dF^T F^-T + lam tr(F^-1 dF) F^-T
+// (:464-470); F^-T dF^T F^-T = (F^-1 dF F^-1)^T
+template <typename T> struct Tangent<T, NEO_HOOKEAN> {
+    T Fi[9], cmu, clam, c;
+    __device__ __forceinline__ void init(const T* F, T m, T l)
+    {
+        cmu = m; clam = l;
+        T J = inv3(F, Fi);
+        c = cmu - clam * dlog(J);
+    }
+    __device__ __forceinline__ void apply(const T* dF, T* dP) const
+    {
+        T G[9], H[9];
+        mul3(Fi, dF, G);
+        T tr = G[0] + G[4] + G[8];
+        mul3(G, Fi, H);
+        T lt = clam * tr;
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++)
+                dP[3 * i + j] = cmu * dF[3 * i + j] + c * H[3 * j + i] + lt * Fi[3 * j + i];
+    }
+};
+
+// First Piola-Kirchhoff stress, volume-scaled (ElasticEnergy.cpp:114-187)
+template <typename T, int MAT>
+__device__ __forceinline__ void stress(const T* F, T cmu, T clam, T* P)
+{
+    if (MAT == NEO_HOOKEAN) {
+        T Fi[9];
+        T J = inv3(F, Fi);
+        T lJ = clam * dlog(J);
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++) P[3 * i + j] = cmu * (F[3 * i + j] - Fi[3 * j + i]) + lJ * Fi[3 * j + i];
+    } else if (MAT == STVK) {
+        Tangent<T, STVK> t;
+        t.init(F, cmu, clam);
+        mul3(F, t.S2, P);
+    } else {
+        T R[9], S[9];
+        const T* U = F;
+        if (MAT == COROTATIONAL) { qr_rot(F, R, S); U = S; }
+        T e[9];
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++) e[3 * i + j] = T(0.5) * (U[3 * i + j] + U[3 * j + i]) - (i == j ? T(1) : T(0));
+        T tr = e[0] + e[4] + e[8];
+        T Q[9];
+#pragma unroll
+        for (int k = 0; k < 9; k++) Q[k] = 2 * cmu * e[k];
+        Q[0] += clam * tr; Q[4] += clam * tr; Q[8] += clam * tr;
+        if (MAT == COROTATIONAL) mul3(R, Q, P);
+        else {
+#pragma unroll
+            for (int k = 0; k < 9; k++) P[k] = Q[k];
+        }
+    }
+}
+
+// F = I + grad(u)
+template <typename T>
+__device__ __forceinline__ void defgrad(const T* g, const T u[4][3], T* F)
+{
+    grad4(g, u, F);
+    F[0] += T(1); F[4] += T(1); F[8] += T(1);
+}
+
+}  // namespace sfem

@@ -1,0 +1,65 @@
+// sfem_kernels.hpp -- device data layout shared by the kernels and the host.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace sfem {
+
+// elements per workgroup of the element-block kernels
+constexpr int BLOCK_E = 256;
+// cap on workgroups of the reduction kernels (their partials buffer size)
+constexpr int RED_GRID = 2048;
+
+// FP64 scalars kept on the device during a CG solve
+enum Scal { S_RZ = 0, S_RZ0, S_RR, S_RR0, S_PQ, S_ALPHA, S_BETA, S_ITER, S_NR, S_NE, S_NT, S_NT2, S_COUNT = 16 };
+
+// Element-block partition, all device pointers.
+struct BlockArgs {
+    int64_t ne;            // elements
+    int64_t ne_pad;        // SoA stride (nblk * BLOCK_E)
+    int nblk;              // workgroups
+    int nmax;              // max nodes of any block (LDS sizing)
+    const int* bnode_ptr;  // nblk+1 offsets into bnodes
+    const uint32_t* bnodes;// block node list: node id | FLAG_SHARED | FLAG_FIXED
+    const uint16_t* lptr;  // per block nb+1 offsets into its incidence list
+    const uint16_t* lent;  // per block 4*BLOCK_E incidence entries (element<<2 | corner)
+    const uint2* lidx;     // per element 4 block-local node ids (16 bit each)
+    const float* dminv32;  // 9 x ne_pad, Dm^-1 row-major, FP32
+    const float* coef32;   // 2 x ne_pad: V mu, V lambda, FP32
+    const double* dminv64; // 9 x ne_pad, FP64
+    const double* coef64;  // 2 x ne_pad, FP64
+};
+
+// nodes shared by several blocks: partial rows to sum
+struct SharedArgs {
+    int nshared;
+    const uint32_t* node;  // node id | flags
+    const int* ptr;        // nshared+1
+    const int* idx;        // partial row index (bnode_ptr[b] + local id), block order
+};
+
+hipError_t launch_hessvec(int mat, const BlockArgs& a, const SharedArgs& s, const float* u32, const float* d32, float* out,
+                          float* partial, const int* done, hipStream_t st);
+hipError_t launch_force(int mat, const BlockArgs& a, const SharedArgs& s, const double* u64, double* out, double* partial,
+                        const double* ext, int mask_fixed, hipStream_t st);
+hipError_t launch_blockdiag(int mat, const BlockArgs& a, const SharedArgs& s, const float* u32, float* out, float* partial,
+                            hipStream_t st);
+size_t elem_lds_max(const BlockArgs& a);
+hipError_t set_elem_lds_limits(const BlockArgs& a);
+hipError_t launch_cg_init(int pc, int64_t n3, const float* b, float* x, float* r, float* z, float* p, const float* minv,
+                          const uint8_t* fixed, double* scal, double* partials, unsigned* counter, int* done, hipStream_t st);
+hipError_t launch_cg_iter_tail(int pc, int64_t n3, float* x, float* r, float* z, float* p, const float* q, const float* minv,
+                               double* scal, double* partials, unsigned* counter, int* done, int test_kind, double tol2,
+                               int guard, hipStream_t st);
+hipError_t launch_norm2(int64_t n3, const double* a, const double* b, double* scal, int slot, double* partials,
+                        unsigned* counter, hipStream_t st);
+hipError_t launch_axpy_state(int64_t n3, const double* u, const float* dx, double alpha, double* ut, float* u32, hipStream_t st);
+hipError_t launch_to_f32(int64_t n, const double* a, float* b, hipStream_t st);
+hipError_t launch_external(int64_t nn, const double* mass, const double* fdead, const double* g, double frac, const int* pptr,
+                           const int* pidx, const double* ftri, const uint8_t* fixed, double* ext, hipStream_t st);
+hipError_t launch_pressure(int64_t ntri, const int* tris, const double* X0, const double* u, double pressure, double frac,
+                           double* ftri, hipStream_t st);
+hipError_t launch_invert_blocks(int64_t nn, const float* B, const uint8_t* fixed, float* Minv, hipStream_t st);
+hipError_t launch_identity_blocks(int64_t nn, const uint8_t* fixed, float* Minv, hipStream_t st);
+
+}  // namespace sfem

@@ -1,0 +1,123 @@
+"""GPU parity of the HIP path against the FP64 CPU oracle, through the C-ABI.
+
+Tolerances: FP64 kernels (internal force, residual) agree with the oracle to
+1e-11 relative (max-norm over the vector); FP32 kernels (Hessian-vector
+product, diagonal blocks) to 2e-5 relative in the 2-norm; converged
+displacements to 1e-6 relative L2 (the stated target)."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from solidfem_amd import capi
+from solidfem_amd.mesh import BeamMesh, lame, uniform01
+
+pytestmark = pytest.mark.gpu
+MATS = [capi.LINEAR, capi.STVK, capi.COROTATIONAL, capi.NEO_HOOKEAN]
+
+
+def relmax(a, b):
+    return np.abs(a - b).max() / np.abs(b).max()
+
+
+def rel2(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+def setup(mat, grid=(10, 3, 3), L=(1.0, 0.25, 0.25), per_elem=False, jitter=0.0, seed=0, pressure=1e4):
+    m = BeamMesh(*grid, *L, jitter=jitter, seed=seed)
+    fixed = m.fixed_mask()
+    tris = m.tip_triangles()
+    E = nu = None
+    kw = dict(gravity=(0.0, 0.0, -9.8))
+    if per_elem:
+        E = 10 ** (5 + 2 * uniform01(seed, 1, m.ne))
+        nu = 0.30 + 0.18 * uniform01(seed, 2, m.ne)
+        mu, lam = lame(E, nu)
+        p = O.OracleProblem(m.X, m.tets, fixed, mat, mu=mu, lam=lam, density=1070.0, tris=tris, pressure=pressure, **kw)
+        s = capi.Solver(m.X, m.tets, fixed, mat, 1e6, 0.45, 1070.0, young_e=E, poisson_e=nu, **kw)
+    else:
+        p = O.OracleProblem(m.X, m.tets, fixed, mat, 1e6, 0.45, 1070.0, tris=tris, pressure=pressure, **kw)
+        s = capi.Solver(m.X, m.tets, fixed, mat, 1e6, 0.45, 1070.0, **kw)
+    s.set_pressure_surface(tris, pressure)
+    return m, fixed, p, s
+
+
+@pytest.mark.parametrize("mat", MATS)
+@pytest.mark.parametrize("grid,per_elem", [((10, 3, 3), False), ((24, 6, 8), True)])
+def test_forces_and_hessvec(mat, grid, per_elem):
+    print(f"case mat={mat} grid={grid} per_elem={per_elem}", flush=True)
+    m, fixed, p, s = setup(mat, grid, per_elem=per_elem, seed=3)
+    rng = np.random.default_rng(mat)
+    x = m.X + 0.01 * rng.standard_normal(m.X.shape) * (1 - fixed[:, None])
+    d = rng.standard_normal(m.X.shape)
+    s.set_positions(x)
+    s.set_load_fraction(0.8)
+    assert np.abs(s.positions() - x).max() < 1e-15
+    assert relmax(s.internal_forces(), p.internal_forces(x)) < 1e-11
+    assert relmax(s.gradients(), p.gradients(x, 0.8)) < 1e-11
+    q = s.matvec(d)
+    assert rel2(q, p.matvec(x, d)) < 2e-5
+    assert np.all(q[fixed == 1] == 0)
+    B = s.block_diagonal()
+    Bo = p.block_diag(x)
+    free = fixed == 0
+    assert rel2(B[free], Bo[free]) < 2e-5
+
+
+def test_hessvec_is_deterministic():
+    m, fixed, p, s = setup(capi.NEO_HOOKEAN, (24, 6, 8), seed=5)
+    rng = np.random.default_rng(1)
+    s.set_positions(m.X + 0.01 * rng.standard_normal(m.X.shape) * (1 - fixed[:, None]))
+    d = rng.standard_normal(m.X.shape)
+    a = s.matvec(d)
+    for _ in range(3):
+        assert np.array_equal(a, s.matvec(d))
+    g = s.gradients()
+    assert np.array_equal(g, s.gradients())
+
+
+@pytest.mark.parametrize("mat", MATS)
+def test_reference_mode_solve_cfg1(mat):
+    """SolveNewtonCG semantics (10 steps, CG eps 1e-3f) against the compiled
+    reference's own result (golden vector).  FP32 inner solves: the
+    trajectories differ at the FP32 level, so the bound is looser than the
+    converged-solve bound."""
+    import os
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", f"ref_cfg1_mat{mat}.npz"))
+    m, fixed, p, s = setup(mat)
+    s.set_load_fraction(1.0)
+    st = s.solve(capi.solve_params(mode=capi.SOLVE_REFERENCE, outer=10, inner=1000, check_every=8))
+    u_ref = g["newton_x"] - m.X
+    err = rel2(s.positions() - m.X, u_ref)
+    print(f"mat {mat}: reference-mode rel err {err:.3e}, cg {st['cg_iters']}")
+    assert st["newton_iters"] == 10
+    assert err < (5e-2 if mat == capi.COROTATIONAL else 1e-4)
+
+
+@pytest.mark.parametrize("mat", [capi.LINEAR, capi.STVK, capi.NEO_HOOKEAN])
+@pytest.mark.parametrize("precond", [capi.PC_NONE, capi.PC_JACOBI])
+def test_converged_solve_cfg1(mat, precond):
+    m, fixed, p, s = setup(mat)
+    s.set_load_fraction(1.0)
+    st = s.solve(tol=1e-8, precond=precond)
+    assert st["converged"] == 1
+    xr, ost = p.newton_solve(None, 1.0, tol=1e-10, cg_rtol=1e-8)
+    err = rel2(s.positions() - m.X, xr - m.X)
+    res = np.linalg.norm(p.gradients(s.positions(), 1.0)) / np.linalg.norm(p.external(s.positions(), 1.0))
+    print(f"mat {mat} pc {precond}: err {err:.3e} oracle-res {res:.3e} newton {st['newton_iters']} cg {st['cg_iters']}")
+    assert res < 1e-8
+    assert err < 1e-6
+
+
+def test_quasi_static_nh_per_element_material():
+    """Reduced config-2/3 physics: per-element E, nu; 4 quasi-static steps."""
+    m, fixed, p, s = setup(capi.NEO_HOOKEAN, (16, 4, 4), per_elem=True, seed=9, pressure=0.0)
+    xo = m.X.copy()
+    for k in range(4):
+        st = s.step(tol=1e-8, precond=capi.PC_JACOBI, sim_type=capi.ST_QUASI_STATIC, force_step=0.25)
+        assert st["converged"] == 1
+        xo, _ = p.newton_solve(xo, 0.25 * (k + 1), tol=1e-10, cg_rtol=1e-8)
+    assert abs(s.load_fraction() - 1.0) < 1e-15
+    err = rel2(s.positions() - m.X, xo - m.X)
+    print(f"quasi-static err {err:.3e}")
+    assert err < 1e-6
