@@ -1,0 +1,155 @@
+"""Benchmark: Newton-CG solve of the 786,432-tet Neo-Hookean beam on one MI355X.
+
+Workload (one timed step = one complete quasi-static solve from rest):
+  128x32x32 hex grid, 6 tets/hex (786,432 tets, 140,481 nodes) over
+  [0,1]x[0,0.25]x[0,0.25] m, x = 0 face clamped; Neo-Hookean E = 1e6 Pa,
+  nu = 0.45, rho = 1070 kg/m^3; gravity -9.8 m/s^2 along z plus a dead tip
+  traction of 1e4 Pa along -z on the x = 1 face; 10 load steps, each Newton-PCG
+  to ||r||_2 <= 1e-8 ||r_ext||_2 in FP64 (FP32 element kernels and CG operator).
+
+Prints ONE JSON line.  `value` = element evaluations per second over the timed
+solves (Hessian-vector + internal-force element evaluations / wall time), with
+the mesh and all state resident in HBM before the timed region starts.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from solidfem_amd import capi  # noqa: E402
+from solidfem_amd.mesh import BeamMesh, traction_nodal_forces  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+
+
+def workload(grid=(128, 32, 32)):
+    m = BeamMesh(*grid)
+    fixed = m.fixed_mask()
+    tris = m.tip_triangles()
+    fdead = traction_nodal_forces(m.X, tris, lambda p: np.tile([0.0, 0.0, -1e4], (len(p), 1)))
+    return m, fixed, fdead
+
+
+def hessvec_bytes(ne, nn, per_element=False):
+    """Algorithmic bytes of one Hessian-vector apply: per element 16 B node ids +
+    36 B Dm^-1 + 4 B volume (+ 8 B mu, lambda when per element), per node 36 B
+    (read u, read d, write Kd), FP32."""
+    return ne * (16 + 36 + 4 + (8 if per_element else 0)) + nn * 36
+
+
+def cpu_baseline(m, fixed, fdead, x, budget_s=10.0):
+    """The FP64 CPU restatement of the reference path (oracle, same loop
+    structure: OpenMP element loop, serial scatter) doing Hessian-vector
+    products on the same mesh at the solved state, on this host's cores."""
+    from oracle import oracle as O
+    cores = min(os.cpu_count() or 1, 16)
+    p = O.OracleProblem(m.X, m.tets, fixed, O.NEO_HOOKEAN, 1e6, 0.45, 1070.0, gravity=(0, 0, -9.8),
+                        fext=fdead, nthreads=cores)
+    d = np.random.default_rng(0).standard_normal(m.X.shape)
+    p.matvec(x, d)
+    n, t0 = 0, time.perf_counter()
+    while True:
+        p.matvec(x, d)
+        n += 1
+        dt = time.perf_counter() - t0
+        if dt > budget_s or n >= 1000:
+            break
+    return {"value": n * m.ne / dt, "unit": "elem-evals/s", "cores": cores, "kind": "port",
+            "sample": f"{n} Hessian-vector products of the 786,432-tet NH beam at its solved state "
+                      f"(oracle/sfem_oracle.c orc_matvec, {cores} OpenMP threads, {dt:.1f} s)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--precond", choices=["jacobi", "none"], default="jacobi")
+    ap.add_argument("--load-steps", type=int, default=10)
+    ap.add_argument("--grid", default="128x32x32")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--profile-reps", type=int, default=200)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU sharding is design-only.",
+              file=sys.stderr)
+        sys.exit(2)
+    grid = tuple(int(v) for v in args.grid.split("x"))
+    m, fixed, fdead = workload(grid)
+    s = capi.Solver(m.X, m.tets, fixed, capi.NEO_HOOKEAN, 1e6, 0.45, 1070.0, gravity=(0.0, 0.0, -9.8))
+    s.set_external_forces(fdead)
+    pc = {"jacobi": capi.PC_JACOBI, "none": capi.PC_NONE}[args.precond]
+    params = capi.solve_params(tol=1e-8, precond=pc, cg_rtol=0.1, sim_type=capi.ST_QUASI_STATIC,
+                               force_step=1.0 / args.load_steps, inner=100000, check_every=32)
+
+    def one_solve():
+        s.reset()
+        tot = {"newton_iters": 0, "cg_iters": 0, "hessvec_evals": 0, "force_evals": 0, "converged": 1}
+        last = None
+        for _ in range(args.load_steps):
+            st = s.step(params)
+            for k in ("newton_iters", "cg_iters", "hessvec_evals", "force_evals"):
+                tot[k] += st[k]
+            tot["converged"] &= st["converged"]
+            last = st
+        tot["res"], tot["ext_norm"] = last["res"], last["ext_norm"]
+        return tot
+
+    for _ in range(args.warmup):
+        one_solve()
+    t0 = time.perf_counter()
+    runs = [one_solve() for _ in range(args.steps)]
+    dt = time.perf_counter() - t0
+    evals = sum(r["hessvec_evals"] + r["force_evals"] for r in runs)
+    cg = sum(r["cg_iters"] for r in runs)
+    x = s.positions()
+
+    ms_elem, ms_gather = s.profile_hessvec(args.profile_reps)
+    info = s.info()
+    bytes_alg = hessvec_bytes(info["ne"], info["nn"])
+    achieved = bytes_alg / (ms_elem * 1e-3) / 1e9
+    line = {
+        "metric": "Newton-CG time to 1e-8 rel. residual (learned-PCG vs Jacobi-PCG); elem-evals/s",
+        "value": evals / dt,
+        "unit": "elem-evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * dt / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 element kernels + f64 residual/dots",
+        "data": "synthetic",
+        "config": {"workload": f"nh_beam_{info['ne']}tet_{args.load_steps}loadsteps",
+                   "grid": args.grid, "tets": info["ne"], "nodes": info["nn"], "material": "neo-hookean",
+                   "E": 1e6, "nu": 0.45, "loads": "gravity -z + dead tip traction 1e4 Pa -z",
+                   "load_steps": args.load_steps, "tol": 1e-8, "precond": args.precond, "parallelism": "none"},
+        "newton_solve_s": dt / args.steps,
+        "newton_iters": runs[-1]["newton_iters"],
+        "cg_iters": runs[-1]["cg_iters"],
+        "cg_iters_per_s": cg / dt,
+        "converged": bool(all(r["converged"] for r in runs)),
+        "final_rel_residual": runs[-1]["res"] / runs[-1]["ext_norm"],
+        "tip_deflection_m": float((x - m.X)[:, 2].min()),
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_elem_block<0,NEO_HOOKEAN,256> (Hessian-vector element blocks)",
+                     "ms_per_launch": ms_elem, "gather_ms_per_launch": ms_gather,
+                     "bytes_per_launch": bytes_alg},
+    }
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline(m, fixed, fdead, x)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -155,6 +155,17 @@ int sfem_step(sfem_ctx* ctx, const sfem_solve_params* p, sfem_solve_stats* st);
    diagonal (Math/IterativeSolvers.h:133-152). */
 int sfem_set_precond_blocks(sfem_ctx* ctx, const float* minv9);
 
+/* Returns every node to its rest position and the load fraction to 0 without
+   a host round trip (the state FemPhysicsMatrixFree's constructor leaves). */
+int sfem_reset(sfem_ctx* ctx);
+
+/* Instrumentation: average device time (ms) of one launch of the Hessian-vector
+   element-block kernel and of its shared-node gather kernel over `reps`
+   back-to-back launches at the current state, timed with HIP events on the
+   context's stream.  Corresponds to the reference's PROFILE_SCOPE("Differential")
+   (FemPhysicsMatrixFree.cpp:408). */
+int sfem_profile_hessvec(sfem_ctx* ctx, int reps, double* ms_elem, double* ms_gather);
+
 /* Number of elements, nodes, fixed nodes, and element blocks of the partition */
 int sfem_info(sfem_ctx* ctx, int64_t* ne, int64_t* nn, int64_t* nfixed, int64_t* nblocks);
 

@@ -219,7 +219,7 @@ class OracleProblem:
         lib().orc_newton_cg_ref(C.byref(self.s), _p(x), frac, outer, inner, _p(its, _ip))
         return x, its
 
-    def newton_solve(self, x=None, frac=1.0, tol=1e-10, max_newton=50, cg_rtol=1e-6, max_cg=100000):
+    def newton_solve(self, x=None, frac=1.0, tol=1e-10, max_newton=50, cg_rtol=1e-1, max_cg=100000):
         x = self._x(x).copy()
         st = np.zeros(4)
         rc = lib().orc_newton_solve(C.byref(self.s), _p(x), frac, tol, max_newton, cg_rtol, max_cg, _p(st))

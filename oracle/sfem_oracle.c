@@ -561,8 +561,20 @@ static void apply_blocks(int64_t nn, const double* Binv, const double* r, double
    |delta| <= eps*eps*|delta0| with eps*eps rounded in float.  Binv (nn*9)
    selects the block-preconditioned variant (z = Binv r, delta = r.z); NULL is
    the reference default (usePrecond = false).  Returns the iteration count.   */
+static int cg_impl(const orc_problem* p, const double* xpos, const double* b, double* x,
+                   int maxit, double tol2, int true_res, const double* Binv);
+
 int orc_cg(const orc_problem* p, const double* xpos, const double* b, double* x,
            int maxit, float eps, const double* Binv)
+{
+    float e2 = eps * eps;     /* float product, as the reference evaluates it */
+    return cg_impl(p, xpos, b, x, maxit, (double)e2, 0, Binv);
+}
+
+/* true_res = 0: the reference test |r.z| <= tol2 |r0.z0|;
+   true_res = 1: ||r||^2 <= tol2 ||r0||^2 (the converged solver's test)       */
+static int cg_impl(const orc_problem* p, const double* xpos, const double* b, double* x,
+                   int maxit, double tol2, int true_res, const double* Binv)
 {
     int64_t n = 3 * p->nn;
     double* r = (double*)malloc(sizeof(double) * n);
@@ -577,10 +589,10 @@ int orc_cg(const orc_problem* p, const double* xpos, const double* b, double* x,
     if (Binv) apply_blocks(p->nn, Binv, r, z); else memcpy(z, r, sizeof(double) * n);
     memcpy(d, z, sizeof(double) * n);
     double delta = dotv(n, r, z);
+    double rr0 = dotv(n, r, r);
     int it = 0;
     if (delta != 0) {
         double delta0 = delta;
-        float e2 = eps * eps;
         it = maxit;
         for (int iter = 0; iter < maxit; iter++) {
             orc_matvec(p, xpos, d, q);
@@ -591,7 +603,8 @@ int orc_cg(const orc_problem* p, const double* xpos, const double* b, double* x,
             double delta1 = dotv(n, r, z);
             double beta = delta1 / delta;
             delta = delta1;
-            if (fabs(delta) <= (double)e2 * fabs(delta0)) { it = iter + 1; break; }
+            int conv = true_res ? (dotv(n, r, r) <= tol2 * rr0) : (fabs(delta) <= tol2 * fabs(delta0));
+            if (conv) { it = iter + 1; break; }
             for (int64_t i = 0; i < n; i++) d[i] = z[i] + beta * d[i];
         }
     }
@@ -625,9 +638,12 @@ static void invert_blocks(int64_t nn, const double* B, double* Binv)
 }
 
 /* Converged Newton solve used as the FP64 oracle at scale (an extension: the
-   reference has no residual test).  Block-Jacobi PCG inner solves to relative
-   tolerance cg_rtol, FP64 back-tracking on ||r|| modelled on
-   NewtonSolverBackTrack (Include/NewtonSolver.h:242-293), stop when
+   reference has no residual test), the same algorithm as the device driver:
+   block-Jacobi PCG inner solves with Eisenstat-Walker forcing (choice 2,
+   gamma 0.9, capped at cg_rtol), a line search on the slope
+   R(a) = dx . r(x + a dx) (accept |R(a)| <= R(0)/2 or R(a) R(0) > 0, else the
+   secant root, 8 trials; cf. the bisection on the same quantity in
+   NewtonSolverBackTrack, Include/NewtonSolver.h:229-293), stop when
    ||r|| <= tol ||r_ext||.  stats[0..3] = newton iters, total CG iters,
    final ||r||, ||r_ext||.  Returns 0 on convergence, 1 otherwise.            */
 int orc_newton_solve(const orc_problem* p, double* x, double frac, double tol, int max_newton,
@@ -636,35 +652,56 @@ int orc_newton_solve(const orc_problem* p, double* x, double frac, double tol, i
     int64_t n = 3 * p->nn;
     double* r = (double*)malloc(sizeof(double) * n);
     double* rt = (double*)malloc(sizeof(double) * n);
+    double* ext = (double*)malloc(sizeof(double) * n);
     double* dl = (double*)malloc(sizeof(double) * n);
     double* xt = (double*)malloc(sizeof(double) * n);
     double* B = (double*)malloc(sizeof(double) * 9 * p->nn);
     double* Binv = (double*)malloc(sizeof(double) * 9 * p->nn);
-    int total_cg = 0, it, ok = 1;
-    double rn = 0, en = 0;
-    for (it = 0; it <= max_newton; it++) {
-        orc_external(p, x, frac, rt);
-        en = sqrt(dotv(n, rt, rt));
-        orc_gradients(p, x, frac, r);
-        rn = sqrt(dotv(n, r, r));
-        if (rn <= tol * en || it == max_newton) { ok = rn <= tol * en; break; }
+    int total_cg = 0, it;
+    orc_gradients(p, x, frac, r);
+    orc_external(p, x, frac, ext);
+    double rn = sqrt(dotv(n, r, r)), en = sqrt(dotv(n, ext, ext));
+    double eta = cg_rtol, rn_prev = 0;
+    for (it = 0; it < max_newton; it++) {
+        if (rn <= tol * en) break;
+        if (it > 0) {
+            double ew = 0.9 * (rn / rn_prev) * (rn / rn_prev), safe = 0.9 * eta * eta;
+            eta = safe > 0.1 ? (ew > safe ? ew : safe) : ew;
+            if (eta > cg_rtol) eta = cg_rtol;
+        }
+        double floor = 0.5 * tol * en / rn;
+        if (eta < floor) eta = floor;
+        if (eta < 1e-12) eta = 1e-12;
         orc_block_diag(p, x, B);
         invert_blocks(p->nn, B, Binv);
         memset(dl, 0, sizeof(double) * n);
-        float eps = (float)cg_rtol;
-        /* the preconditioned test |r.z| <= eps^2 |r0.z0| is the reference's
-           criterion applied to the preconditioned residual                     */
-        total_cg += orc_cg(p, x, r, dl, max_cg, eps, Binv);
-        double alpha = 1.0;
-        for (int ls = 0; ls < 12; ls++) {
+        total_cg += cg_impl(p, x, r, dl, max_cg, eta * eta, 1, Binv);
+        double R0 = dotv(n, dl, r), alpha = 1.0, Ra = 0;
+        for (int ls = 0; ls < 8; ls++) {
             for (int64_t i = 0; i < n; i++) xt[i] = x[i] + alpha * dl[i];
             orc_gradients(p, xt, frac, rt);
-            if (sqrt(dotv(n, rt, rt)) < rn || ls == 11) break;
-            alpha *= 0.5;
+            Ra = dotv(n, dl, rt);
+            double rr = dotv(n, rt, rt);
+            int finite = isfinite(Ra) && isfinite(rr);
+            if (ls == 7) break;
+            if (finite && (fabs(Ra) <= 0.5 * fabs(R0) || Ra * R0 > 0)) break;
+            if (finite) {
+                double a = alpha * R0 / (R0 - Ra);
+                if (a < 0.1 * alpha) a = 0.1 * alpha;
+                if (a > 0.9 * alpha) a = 0.9 * alpha;
+                alpha = a;
+            } else alpha *= 0.5;
         }
         memcpy(x, xt, sizeof(double) * n);
+        memcpy(r, rt, sizeof(double) * n);
+        orc_external(p, x, frac, ext);
+        rn_prev = rn;
+        rn = sqrt(dotv(n, r, r));
+        en = sqrt(dotv(n, ext, ext));
+        if (!isfinite(rn)) break;
     }
+    int ok = rn <= tol * en;
     if (stats) { stats[0] = it; stats[1] = total_cg; stats[2] = rn; stats[3] = en; }
-    free(r); free(rt); free(dl); free(xt); free(B); free(Binv);
+    free(r); free(rt); free(ext); free(dl); free(xt); free(B); free(Binv);
     return ok ? 0 : 1;
 }

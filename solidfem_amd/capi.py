@@ -24,6 +24,7 @@ EXPORTS = [
     "sfem_set_load_fraction", "sfem_get_load_fraction", "sfem_set_pressure_surface",
     "sfem_set_external_forces", "sfem_internal_forces", "sfem_gradients", "sfem_matvec",
     "sfem_block_diagonal", "sfem_solve", "sfem_step", "sfem_set_precond_blocks", "sfem_info",
+    "sfem_reset", "sfem_profile_hessvec",
 ]
 
 _dp = C.POINTER(C.c_double)
@@ -95,6 +96,8 @@ def lib():
             "sfem_set_precond_blocks": ([V, _fp], C.c_int),
             "sfem_info": ([V, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64),
                            C.POINTER(C.c_int64)], C.c_int),
+            "sfem_reset": ([V], C.c_int),
+            "sfem_profile_hessvec": ([V, C.c_int, _dp, _dp], C.c_int),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -152,6 +155,14 @@ class Solver:
         v = [C.c_int64() for _ in range(4)]
         _check(lib().sfem_info(self.h, *[C.byref(x) for x in v]))
         return dict(zip(("ne", "nn", "nfixed", "nblocks"), (x.value for x in v)))
+
+    def reset(self):
+        _check(lib().sfem_reset(self.h))
+
+    def profile_hessvec(self, reps=100):
+        a, b = C.c_double(), C.c_double()
+        _check(lib().sfem_profile_hessvec(self.h, int(reps), C.byref(a), C.byref(b)))
+        return a.value, b.value
 
     def set_positions(self, x):
         x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, 3)

@@ -309,16 +309,18 @@ void hessvec(sfem_ctx* c, const float* d, float* out, const int* done)
     HIPCK(launch_hessvec(c->material, c->blk, c->shr, c->d_u32.as<float>(), d, out, c->d_partial.as<float>(), done, c->stream));
 }
 
-// ||a||^2, ||b||^2 -> host
-void norms2(sfem_ctx* c, const double* a, const double* b, double* na, double* nb)
+// a.a, b.b, d.a (FP64) -> host
+struct Dots {
+    double aa, bb, da;
+};
+Dots dots3(sfem_ctx* c, const double* a, const double* b, const float* d)
 {
-    HIPCK(launch_norm2(3 * c->nn, a, b, c->d_scal.as<double>(), S_NT, c->d_partials.as<double>(),
+    HIPCK(launch_dots3(3 * c->nn, a, b, d, c->d_scal.as<double>(), S_NT, c->d_partials.as<double>(),
                        c->d_counter.as<unsigned>(), c->stream));
-    double h[2];
+    double h[3];
     HIPCK(hipMemcpyAsync(h, c->d_scal.as<double>() + S_NT, sizeof h, hipMemcpyDeviceToHost, c->stream));
     HIPCK(hipStreamSynchronize(c->stream));
-    *na = h[0];
-    if (nb) *nb = h[1];
+    return Dots{h[0], h[1], h[2]};
 }
 
 void build_jacobi(sfem_ctx* c)
@@ -372,18 +374,6 @@ CgResult pcg(sfem_ctx* c, const float* minv, int pc, int maxit, int test_kind, d
     return res;
 }
 
-int guarded(const char* where, int (*fn)(void*), void* arg)
-{
-    try {
-        return fn(arg);
-    } catch (const HipError& e) {
-        return fail(e.e == hipErrorOutOfMemory ? SFEM_ENOMEM : SFEM_EDEVICE,
-                    std::string(where) + ": " + e.what + ": " + hipGetErrorString(e.e));
-    } catch (const std::exception& e) {
-        return fail(SFEM_EDEVICE, std::string(where) + ": " + e.what());
-    }
-}
-
 template <typename F> int guard_call(const char* where, F&& f)
 {
     try {
@@ -406,14 +396,10 @@ int solve_reference(sfem_ctx* c, const sfem_solve_params* p, sfem_solve_stats* s
     HIPCK(launch_identity_blocks(c->nn, c->d_fixed.as<uint8_t>(), c->d_minv.as<float>(), c->stream));
     const float eps = 1e-3f;
     const float e2 = eps * eps;   // float product, as the reference evaluates eps * eps
-    double rn2, en2;
     for (int it = 0; it < p->outer_iterations; it++) {
         residual(c, c->d_u64.as<double>(), c->d_r64.as<double>());
         st->force_evals += c->ne;
-        if (it == 0) {
-            norms2(c, c->d_r64.as<double>(), c->d_ext64.as<double>(), &rn2, &en2);
-            st->res0 = std::sqrt(rn2);
-        }
+        if (it == 0) st->res0 = std::sqrt(dots3(c, c->d_r64.as<double>(), nullptr, nullptr).aa);
         HIPCK(launch_to_f32(n3, c->d_r64.as<double>(), c->d_b32.as<float>(), c->stream));
         CgResult cg = pcg(c, c->d_minv.as<float>(), 0, p->inner_iterations, 0, (double)e2, 0, p->check_every,
                           &st->hessvec_evals);
@@ -425,14 +411,20 @@ int solve_reference(sfem_ctx* c, const sfem_solve_params* p, sfem_solve_stats* s
     }
     residual(c, c->d_u64.as<double>(), c->d_r64.as<double>());
     st->force_evals += c->ne;
-    norms2(c, c->d_r64.as<double>(), c->d_ext64.as<double>(), &rn2, &en2);
-    st->res = std::sqrt(rn2);
-    st->ext_norm = std::sqrt(en2);
+    Dots dn = dots3(c, c->d_r64.as<double>(), c->d_ext64.as<double>(), nullptr);
+    st->res = std::sqrt(dn.aa);
+    st->ext_norm = std::sqrt(dn.bb);
     st->converged = 0;
     return SFEM_OK;
 }
 
 // Converged Newton-PCG: FP64 residual test and line search, FP32 inner PCG.
+//  - forcing: Eisenstat-Walker choice 2 (gamma 0.9, exponent 2) capped by
+//    cg_rtol, floored where the outer target no longer needs more;
+//  - line search on the slope R(a) = dx . r(u + a dx) (Bonet & Wood): accept
+//    when |R(a)| <= 0.5 R(0) or R(a) > 0 (still descending), otherwise move to
+//    the secant root of R, at most 8 trials (the reference's back-tracking
+//    Newton, Include/NewtonSolver.h:229-293, bisects on the same quantity).
 int solve_converged(sfem_ctx* c, const sfem_solve_params* p, sfem_solve_stats* st)
 {
     const int64_t n3 = 3 * c->nn;
@@ -440,54 +432,58 @@ int solve_converged(sfem_ctx* c, const sfem_solve_params* p, sfem_solve_stats* s
     if (pc_kind == SFEM_PC_LEARNED && !c->has_learned) return fail(SFEM_ESTATE, "learned preconditioner not installed");
     if (pc_kind == SFEM_PC_NONE)
         HIPCK(launch_identity_blocks(c->nn, c->d_fixed.as<uint8_t>(), c->d_minv.as<float>(), c->stream));
-    double rn2, en2;
     residual(c, c->d_u64.as<double>(), c->d_r64.as<double>());
     st->force_evals += c->ne;
-    norms2(c, c->d_r64.as<double>(), c->d_ext64.as<double>(), &rn2, &en2);
-    double rn = std::sqrt(rn2), en = std::sqrt(en2);
+    Dots d0 = dots3(c, c->d_r64.as<double>(), c->d_ext64.as<double>(), nullptr);
+    double rn = std::sqrt(d0.aa), en = std::sqrt(d0.bb);
     st->res0 = rn;
     const int maxn = p->outer_iterations > 0 ? p->outer_iterations : 50;
-    const double cg_rtol = p->cg_rtol > 0 ? p->cg_rtol : 1e-3;
+    const double eta_max = p->cg_rtol > 0 ? p->cg_rtol : 1e-1;
+    double eta = eta_max, rn_prev = 0;
     for (int it = 0; it < maxn; it++) {
         if (rn <= p->tol * en) break;
+        if (it > 0) {
+            const double ew = 0.9 * (rn / rn_prev) * (rn / rn_prev);
+            const double safe = 0.9 * eta * eta;
+            eta = std::min(eta_max, safe > 0.1 ? std::max(ew, safe) : ew);
+        }
+        eta = std::max(eta, std::max(0.5 * p->tol * en / rn, 1e-6));
         if (pc_kind == SFEM_PC_JACOBI) build_jacobi(c);
         HIPCK(launch_to_f32(n3, c->d_r64.as<double>(), c->d_b32.as<float>(), c->stream));
         const float* minv = pc_kind == SFEM_PC_LEARNED ? c->d_minv_learned.as<float>() : c->d_minv.as<float>();
-        // forcing: do not ask the FP32 inner solve for more than the outer
-        // residual needs (Eisenstat-Walker style floor at the target)
-        double eta = std::max(cg_rtol, 0.5 * p->tol * en / rn);
         CgResult cg = pcg(c, minv, pc_kind == SFEM_PC_NONE ? 0 : 1, p->inner_iterations, 1, eta * eta, 1, p->check_every,
                           &st->hessvec_evals);
         st->cg_iters += cg.iters;
-        if (pc_kind == SFEM_PC_LEARNED && (cg.status != 1)) {
-            // judge: the learned PCG failed to converge -> redo with Jacobi
+        if (pc_kind == SFEM_PC_LEARNED && cg.status != 1) {
+            // judge: the learned PCG did not reach its forcing -> redo with Jacobi
             st->fallbacks++;
             build_jacobi(c);
             cg = pcg(c, c->d_minv.as<float>(), 1, p->inner_iterations, 1, eta * eta, 1, p->check_every, &st->hessvec_evals);
             st->cg_iters += cg.iters;
         }
-        // FP64 back-tracking on ||r|| (NewtonSolver.h:242-293 style)
-        double alpha = 1.0, rt = 0, et = 0;
-        bool accepted = false;
-        for (int ls = 0; ls < (p->line_search ? 12 : 1); ls++) {
+        const double R0 = dots3(c, c->d_r64.as<double>(), nullptr, c->d_x32.as<float>()).da;
+        double alpha = 1.0;
+        Dots dt{0, 0, 0};
+        const int ntrial = p->line_search ? 8 : 1;
+        for (int ls = 0; ls < ntrial; ls++) {
             HIPCK(launch_axpy_state(n3, c->d_u64.as<double>(), c->d_x32.as<float>(), alpha, c->d_ut64.as<double>(),
                                     c->d_u32.as<float>(), c->stream));
             residual(c, c->d_ut64.as<double>(), c->d_f64tmp.as<double>());
             st->force_evals += c->ne;
-            norms2(c, c->d_f64tmp.as<double>(), c->d_ext64.as<double>(), &rt, &et);
-            rt = std::sqrt(rt);
-            et = std::sqrt(et);
-            if (!p->line_search || (rt == rt && rt < rn)) { accepted = true; break; }
-            alpha *= 0.5;
-        }
-        if (!accepted) {
-            // keep the last trial anyway (smallest step); the residual test decides
+            dt = dots3(c, c->d_f64tmp.as<double>(), c->d_ext64.as<double>(), c->d_x32.as<float>());
+            const double Ra = dt.da;
+            const bool finite = std::isfinite(Ra) && std::isfinite(dt.aa);
+            if (!p->line_search || ls == ntrial - 1) break;
+            if (finite && (std::fabs(Ra) <= 0.5 * std::fabs(R0) || Ra * R0 > 0)) break;
+            alpha = finite ? std::min(std::max(alpha * R0 / (R0 - Ra), 0.1 * alpha), 0.9 * alpha) : 0.5 * alpha;
         }
         std::swap(c->d_u64.p, c->d_ut64.p);
         std::swap(c->d_r64.p, c->d_f64tmp.p);
-        rn = rt;
-        en = et;
+        rn_prev = rn;
+        rn = std::sqrt(dt.aa);
+        en = std::sqrt(dt.bb);
         st->newton_iters++;
+        if (!std::isfinite(rn)) break;
     }
     // u32 must match the accepted state
     HIPCK(launch_to_f32(n3, c->d_u64.as<double>(), c->d_u32.as<float>(), c->stream));
@@ -580,7 +576,7 @@ int sfem_create(const sfem_mesh* m, const sfem_material* mat, sfem_ctx** out)
         zero<float>(c->d_minv, 9 * (size_t)nn, s);
         zero<float>(c->d_bd32, 9 * (size_t)nn, s);
         zero<double>(c->d_scal, S_COUNT, s);
-        zero<double>(c->d_partials, 2 * (size_t)RED_GRID, s);
+        zero<double>(c->d_partials, 3 * (size_t)RED_GRID, s);
         zero<unsigned>(c->d_counter, 4, s);
         zero<int>(c->d_done, 4, s);
         HIPCK(hipStreamSynchronize(s));
@@ -738,6 +734,54 @@ int sfem_block_diagonal(sfem_ctx* c, double* B)
         HIPCK(hipMemcpyAsync(o.data(), c->d_bd32.p, n9 * sizeof(float), hipMemcpyDeviceToHost, c->stream));
         HIPCK(hipStreamSynchronize(c->stream));
         for (size_t i = 0; i < n9; i++) B[i] = o[i];
+        return SFEM_OK;
+    });
+}
+
+int sfem_reset(sfem_ctx* c)
+{
+    if (!c) return fail(SFEM_EINVAL, "sfem_reset: null context");
+    return guard_call("sfem_reset", [&]() -> int {
+        HIPCK(hipMemsetAsync(c->d_u64.p, 0, 3 * c->nn * sizeof(double), c->stream));
+        HIPCK(hipMemsetAsync(c->d_u32.p, 0, 3 * c->nn * sizeof(float), c->stream));
+        HIPCK(hipStreamSynchronize(c->stream));
+        c->frac = 0.0;
+        return SFEM_OK;
+    });
+}
+
+int sfem_profile_hessvec(sfem_ctx* c, int reps, double* ms_elem, double* ms_gather)
+{
+    if (!c || reps <= 0 || !ms_elem || !ms_gather) return fail(SFEM_EINVAL, "sfem_profile_hessvec: bad argument");
+    return guard_call("sfem_profile_hessvec", [&]() -> int {
+        // direction: the current displacement (a smooth, realistic field)
+        HIPCK(hipMemcpyAsync(c->d_d32.p, c->d_u32.p, 3 * c->nn * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+        hipEvent_t e0, e1, e2;
+        HIPCK(hipEventCreate(&e0));
+        HIPCK(hipEventCreate(&e1));
+        HIPCK(hipEventCreate(&e2));
+        SharedArgs none = c->shr;
+        none.nshared = 0;
+        // warm
+        HIPCK(launch_hessvec(c->material, c->blk, c->shr, c->d_u32.as<float>(), c->d_d32.as<float>(), c->d_q32.as<float>(),
+                             c->d_partial.as<float>(), nullptr, c->stream));
+        HIPCK(hipEventRecord(e0, c->stream));
+        for (int i = 0; i < reps; i++)
+            HIPCK(launch_hessvec(c->material, c->blk, none, c->d_u32.as<float>(), c->d_d32.as<float>(),
+                                 c->d_q32.as<float>(), c->d_partial.as<float>(), nullptr, c->stream));
+        HIPCK(hipEventRecord(e1, c->stream));
+        for (int i = 0; i < reps; i++)
+            HIPCK(launch_shared_only(c->shr, c->d_partial.as<float>(), c->d_q32.as<float>(), c->stream));
+        HIPCK(hipEventRecord(e2, c->stream));
+        HIPCK(hipEventSynchronize(e2));
+        float t1 = 0, t2 = 0;
+        HIPCK(hipEventElapsedTime(&t1, e0, e1));
+        HIPCK(hipEventElapsedTime(&t2, e1, e2));
+        *ms_elem = t1 / reps;
+        *ms_gather = t2 / reps;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        (void)hipEventDestroy(e2);
         return SFEM_OK;
     });
 }

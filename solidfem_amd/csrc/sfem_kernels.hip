@@ -394,19 +394,23 @@ __global__ __launch_bounds__(256) void k_cg_p(int64_t n3, const float* __restric
 }
 
 // ---------------------------------------------------------------- FP64 vectors
-// sums of squares of two FP64 node vectors -> scal[slot], scal[slot+1]
-__global__ __launch_bounds__(256) void k_norm2(int64_t n3, const double* __restrict__ a, const double* __restrict__ b,
-                                               double* scal, int slot, double* partials, unsigned* counter)
+// a.a, b.b and d.a for FP64 node vectors a, b and an FP32 direction d
+// -> scal[slot], scal[slot+1], scal[slot+2]  (b, d may be null)
+__global__ __launch_bounds__(256) void k_dots3(int64_t n3, const double* __restrict__ a, const double* __restrict__ b,
+                                               const float* __restrict__ d, double* scal, int slot, double* partials,
+                                               unsigned* counter)
 {
-    double v[2] = {0.0, 0.0};
+    double v[3] = {0.0, 0.0, 0.0};
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n3 / 3; i += (int64_t)gridDim.x * blockDim.x) {
         v[0] += a[3 * i] * a[3 * i] + a[3 * i + 1] * a[3 * i + 1] + a[3 * i + 2] * a[3 * i + 2];
         if (b) v[1] += b[3 * i] * b[3 * i] + b[3 * i + 1] * b[3 * i + 1] + b[3 * i + 2] * b[3 * i + 2];
+        if (d) v[2] += (double)d[3 * i] * a[3 * i] + (double)d[3 * i + 1] * a[3 * i + 1] + (double)d[3 * i + 2] * a[3 * i + 2];
     }
-    double tot[2];
-    if (grid_sum<2>(v, partials, counter, tot) && threadIdx.x == 0) {
+    double tot[3];
+    if (grid_sum<3>(v, partials, counter, tot) && threadIdx.x == 0) {
         scal[slot] = tot[0];
         scal[slot + 1] = tot[1];
+        scal[slot + 2] = tot[2];
     }
 }
 
@@ -560,6 +564,11 @@ hipError_t launch_hessvec(int mat, const BlockArgs& a, const SharedArgs& s, cons
     return launch_shared<float, 3>(s, partial, out, nullptr, 1, done, st);
 }
 
+hipError_t launch_shared_only(const SharedArgs& s, const float* partial, float* out, hipStream_t st)
+{
+    return launch_shared<float, 3>(s, partial, out, nullptr, 1, nullptr, st);
+}
+
 hipError_t launch_force(int mat, const BlockArgs& a, const SharedArgs& s, const double* u64, double* out, double* partial,
                         const double* ext, int mask_fixed, hipStream_t st)
 {
@@ -627,11 +636,11 @@ hipError_t launch_cg_iter_tail(int pc, int64_t n3, float* x, float* r, float* z,
     return hipGetLastError();
 }
 
-hipError_t launch_norm2(int64_t n3, const double* a, const double* b, double* scal, int slot, double* partials,
-                        unsigned* counter, hipStream_t st)
+hipError_t launch_dots3(int64_t n3, const double* a, const double* b, const float* d, double* scal, int slot,
+                        double* partials, unsigned* counter, hipStream_t st)
 {
     const int g = grid_for(n3 / 3, 256, RED_GRID);
-    hipLaunchKernelGGL(k_norm2, dim3(g), dim3(256), 0, st, n3, a, b, scal, slot, partials, counter);
+    hipLaunchKernelGGL(k_dots3, dim3(g), dim3(256), 0, st, n3, a, b, d, scal, slot, partials, counter);
     return hipGetLastError();
 }
 

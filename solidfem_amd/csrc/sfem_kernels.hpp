@@ -8,10 +8,10 @@ namespace sfem {
 // elements per workgroup of the element-block kernels
 constexpr int BLOCK_E = 256;
 // cap on workgroups of the reduction kernels (their partials buffer size)
-constexpr int RED_GRID = 2048;
+constexpr int RED_GRID = 2048;   // partials buffer holds 3 * RED_GRID doubles
 
 // FP64 scalars kept on the device during a CG solve
-enum Scal { S_RZ = 0, S_RZ0, S_RR, S_RR0, S_PQ, S_ALPHA, S_BETA, S_ITER, S_NR, S_NE, S_NT, S_NT2, S_COUNT = 16 };
+enum Scal { S_RZ = 0, S_RZ0, S_RR, S_RR0, S_PQ, S_ALPHA, S_BETA, S_ITER, S_NT, S_NT2, S_NT3, S_COUNT = 16 };
 
 // Element-block partition, all device pointers.
 struct BlockArgs {
@@ -40,6 +40,7 @@ struct SharedArgs {
 
 hipError_t launch_hessvec(int mat, const BlockArgs& a, const SharedArgs& s, const float* u32, const float* d32, float* out,
                           float* partial, const int* done, hipStream_t st);
+hipError_t launch_shared_only(const SharedArgs& s, const float* partial, float* out, hipStream_t st);
 hipError_t launch_force(int mat, const BlockArgs& a, const SharedArgs& s, const double* u64, double* out, double* partial,
                         const double* ext, int mask_fixed, hipStream_t st);
 hipError_t launch_blockdiag(int mat, const BlockArgs& a, const SharedArgs& s, const float* u32, float* out, float* partial,
@@ -51,8 +52,8 @@ hipError_t launch_cg_init(int pc, int64_t n3, const float* b, float* x, float* r
 hipError_t launch_cg_iter_tail(int pc, int64_t n3, float* x, float* r, float* z, float* p, const float* q, const float* minv,
                                double* scal, double* partials, unsigned* counter, int* done, int test_kind, double tol2,
                                int guard, hipStream_t st);
-hipError_t launch_norm2(int64_t n3, const double* a, const double* b, double* scal, int slot, double* partials,
-                        unsigned* counter, hipStream_t st);
+hipError_t launch_dots3(int64_t n3, const double* a, const double* b, const float* d, double* scal, int slot,
+                        double* partials, unsigned* counter, hipStream_t st);
 hipError_t launch_axpy_state(int64_t n3, const double* u, const float* dx, double alpha, double* ut, float* u32, hipStream_t st);
 hipError_t launch_to_f32(int64_t n, const double* a, float* b, hipStream_t st);
 hipError_t launch_external(int64_t nn, const double* mass, const double* fdead, const double* g, double frac, const int* pptr,
