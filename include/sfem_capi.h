@@ -83,12 +83,14 @@ typedef struct {
     int32_t check_every;       /* CG iterations between host convergence checks (>=1) */
     int32_t sim_type;          /* sfem_step only: SFEM_ST_STATIC | SFEM_ST_QUASI_STATIC */
     double force_step;         /* sfem_step only: FemConfig::mForceApplicationStep */
+    double abs_tol;            /* CONVERGED: also stop when ||r||_2 <= abs_tol
+                                  (FemConfig::mAbsNewtonRsidualThreshold); 0 = off */
 } sfem_solve_params;
 
 typedef struct {
     int32_t newton_iters;      /* Newton steps taken */
     int32_t cg_iters;          /* total CG iterations */
-    int32_t converged;         /* CONVERGED mode: 1 if ||r|| <= tol ||r_ext|| */
+    int32_t converged;         /* CONVERGED mode: 1 if ||r|| <= max(tol ||r_ext||, abs_tol) */
     int32_t fallbacks;         /* learned-PC steps rerun with Jacobi */
     double res0;               /* ||r||_2 at the start (FP64) */
     double res;                /* ||r||_2 at the end (FP64) */

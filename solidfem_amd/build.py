@@ -36,8 +36,24 @@ def build_lib():
     return LIB
 
 
+def build_pysolidfem():
+    """pybind11 module `pysolidfem` (host C++ only, links libsfem_hip.so)."""
+    import pybind11
+    suffix = sysconfig.get_config_var("EXT_SUFFIX")
+    out = os.path.join(HERE, "pysolidfem" + suffix)
+    host = os.path.join(CSRC, "host")
+    cmd = ["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-fvisibility=hidden",
+           f"-I{pybind11.get_include()}", f"-I{sysconfig.get_paths()['include']}", f"-I{INCLUDE}", f"-I{host}",
+           os.path.join(host, "FemPhysicsMatrixFree.cpp"), os.path.join(host, "pysolidfem.cpp"),
+           f"-L{HERE}", "-lsfem_hip", "-Wl,-rpath,$ORIGIN", "-o", out + ".tmp"]
+    _run(cmd)
+    os.replace(out + ".tmp", out)
+    return out
+
+
 def build_all():
     build_lib()
+    build_pysolidfem()
 
 
 if __name__ == "__main__":

@@ -1,0 +1,225 @@
+// pysolidfem.cpp -- the `pysolidfem` Python module (PySolidFEM/Module.cpp:90-111,
+// PyNonlinearFEM.cpp) on top of the MI355X FemPhysicsMatrixFree.
+//
+// NonlinearFEM(tets, nodes, fixed_nodes, config) keeps the reference's
+// constructor, `step`, `simulate_static`, `get_nodes`, `get_tets`,
+// `set_lame_params`, `get_shear_modulus`, `get_lame_lambda` and its config keys
+// (young, poisson, density, simtype, substeps, maxiter, tol, mixed, optimizer;
+// PyNonlinearFEM.cpp:240-292).  New keys select what the reference hard-wires
+// (NH + NST_NEWTON_LS, :280-291): solver ("newton", "newton_ls", "newton_cg"),
+// material ("linear", "stvk", "corotational", "neo-hookean" or the enum value),
+// gravity, zup, inner_iters, rel_tol, cg_tol, precond ("none", "jacobi",
+// "learned"), force_step, young_e / poisson_e (per element arrays).
+// Unlike the reference's FemBody::BuildBoundaryMesh, no uint16 node indices
+// are involved anywhere (meshes above 65,535 nodes work).
+#include <pybind11/numpy.h>
+#include <pybind11/pybind11.h>
+#include <pybind11/stl.h>
+
+#include <memory>
+#include <stdexcept>
+#include <string>
+
+#include "FemPhysicsMatrixFree.h"
+
+namespace py = pybind11;
+using namespace FEM_SYSTEM;
+
+namespace {
+
+const real DT = 0.016;   // PyNonlinearFEM.h:51
+
+MaterialModelType parse_material(py::handle h)
+{
+    if (py::isinstance<py::int_>(h)) return (MaterialModelType)py::cast<int>(h);
+    std::string s = py::cast<std::string>(h);
+    if (s == "linear") return MMT_LINEAR;
+    if (s == "stvk") return MMT_STVK;
+    if (s == "corotational" || s == "corot") return MMT_COROTATIONAL;
+    if (s == "neo-hookean" || s == "neohookean" || s == "nh") return MMT_NEO_HOOKEAN;
+    throw std::invalid_argument("unknown material '" + s + "'");
+}
+
+NonlinearSolverType parse_solver(py::handle h)
+{
+    if (py::isinstance<py::int_>(h)) return (NonlinearSolverType)py::cast<int>(h);
+    std::string s = py::cast<std::string>(h);
+    if (s == "newton") return NST_NEWTON;
+    if (s == "newton_ls") return NST_NEWTON_LS;
+    if (s == "newton_cg") return NST_NEWTON_CG;
+    throw std::invalid_argument("unknown solver '" + s + "'");
+}
+
+class PyNonlinearFEM {
+public:
+    PyNonlinearFEM(py::array_t<int, py::array::c_style | py::array::forcecast> tets,
+                   py::array_t<double, py::array::c_style | py::array::forcecast> nodes,
+                   py::array_t<int, py::array::c_style | py::array::forcecast> fixed, py::dict config)
+    {
+        if (nodes.ndim() != 2 || nodes.shape(1) != 3) throw std::invalid_argument("nodes should have 3 columns");
+        if (tets.size() % 4) throw std::invalid_argument("tets should have 4 indices per element");
+        const size_t nn = nodes.shape(0), ne = tets.size() / 4;
+        std::vector<Node> n(nn);
+        const double* pn = nodes.data();
+        for (size_t i = 0; i < nn; i++) {
+            n[i].pos = Vector3R(pn[3 * i], pn[3 * i + 1], pn[3 * i + 2]);
+            n[i].pos0 = n[i].pos;
+        }
+        const int* pf = fixed.data();
+        for (ssize_t i = 0; i < fixed.size(); i++) {
+            if (pf[i] < 0 || (size_t)pf[i] >= nn) throw std::invalid_argument("fixed node index out of range");
+            n[pf[i]].invMass = 0;
+        }
+        std::vector<Tet> t(ne);
+        const int* pt = tets.data();
+        for (size_t e = 0; e < ne; e++)
+            for (int k = 0; k < 4; k++) {
+                if (pt[4 * e + k] < 0 || (size_t)pt[4 * e + k] >= nn) throw std::invalid_argument("tet index out of range");
+                t[e].idx[k] = (uint32)pt[4 * e + k];
+            }
+        FemConfig cfg = ParseConfig(config);
+        mPhys.reset(new FemPhysicsMatrixFree(t, n, cfg));
+        if (config.contains("young_e") || config.contains("poisson_e")) {
+            auto E = py::cast<std::vector<real>>(config.contains("young_e") ? config["young_e"] : py::cast(std::vector<real>(ne, cfg.mYoungsModulus)));
+            auto nu = py::cast<std::vector<real>>(config.contains("poisson_e") ? config["poisson_e"] : py::cast(std::vector<real>(ne, cfg.mPoissonRatio)));
+            mPhys->SetPerElementMaterial(E, nu);
+        }
+    }
+
+    // Step (PyNonlinearFEM.cpp:126-132)
+    void Step(real dt) { mPhys->Step(dt); }
+
+    // SimulateStatic (:134-143): step until the load fraction reaches 1
+    void SimulateStatic()
+    {
+        int guard = 0;
+        while (mPhys->GetForceFraction() < 1 && guard++ < 1000000) mPhys->Step(0);
+    }
+
+    // GetNodes (:145-159): positions in original order, (nn, 3)
+    py::array_t<double> GetNodes() const
+    {
+        const auto& nodes = mPhys->GetNodes();
+        py::array_t<double> out({(py::ssize_t)nodes.size(), (py::ssize_t)3});
+        double* p = out.mutable_data();
+        for (size_t i = 0; i < nodes.size(); i++)
+            for (int c = 0; c < 3; c++) p[3 * i + c] = nodes[i].pos[c];
+        return out;
+    }
+
+    py::array_t<int> GetTets() const
+    {
+        const uint32 ne = mPhys->GetNumElements();
+        py::array_t<int> out({(py::ssize_t)ne, (py::ssize_t)4});
+        int* p = out.mutable_data();
+        for (uint32 e = 0; e < ne; e++)
+            for (int k = 0; k < 4; k++) p[4 * e + k] = (int)mPhys->GetGlobalOriginalIndex(e, k);
+        return out;
+    }
+
+    void SetLameParams(real mu, real lambda) { mPhys->SetLameParams(mu, lambda); }
+    real GetShearModulus() const { return mPhys->GetShearModulus(); }
+    real GetLameLambda() const { return mPhys->GetLameFirstParam(); }
+
+    void SetPressure(py::array_t<int, py::array::c_style | py::array::forcecast> tris, real p)
+    {
+        std::vector<uint32> t(tris.data(), tris.data() + tris.size());
+        mPhys->SetBoundaryConditionsSurface(t, p);
+    }
+
+    void SetExternalForces(py::array_t<double, py::array::c_style | py::array::forcecast> f)
+    {
+        std::vector<Vector3R> v(f.size() / 3);
+        const double* p = f.data();
+        for (size_t i = 0; i < v.size(); i++) v[i] = Vector3R(p[3 * i], p[3 * i + 1], p[3 * i + 2]);
+        mPhys->SetExternalForces(v);
+    }
+
+    void SetPreconditionerBlocks(py::array_t<float, py::array::c_style | py::array::forcecast> m)
+    {
+        mPhys->SetPreconditionerBlocks(std::vector<float>(m.data(), m.data() + m.size()));
+    }
+
+    py::dict LastStats() const
+    {
+        const sfem_solve_stats& s = mPhys->GetLastSolveStats();
+        py::dict d;
+        d["newton_iters"] = s.newton_iters;
+        d["cg_iters"] = s.cg_iters;
+        d["converged"] = s.converged;
+        d["fallbacks"] = s.fallbacks;
+        d["res0"] = s.res0;
+        d["res"] = s.res;
+        d["ext_norm"] = s.ext_norm;
+        d["seconds"] = s.seconds;
+        d["hessvec_evals"] = s.hessvec_evals;
+        d["force_evals"] = s.force_evals;
+        return d;
+    }
+
+    real GetForceFraction() const { return mPhys->GetForceFraction(); }
+    bool CheckForInversion() const { return mPhys->CheckForInversion(0); }
+
+private:
+    FemConfig ParseConfig(py::dict config)
+    {
+        FemConfig c;
+        c.mVerbose = false;
+        if (config.contains("young")) c.mYoungsModulus = py::cast<real>(config["young"]);
+        if (config.contains("poisson")) c.mPoissonRatio = py::cast<real>(config["poisson"]);
+        if (config.contains("density")) c.mDensity = py::cast<real>(config["density"]);
+        if (config.contains("simtype")) c.mSimType = (SimulationType)py::cast<int>(config["simtype"]);
+        if (config.contains("substeps")) c.mNumSubsteps = py::cast<int>(config["substeps"]);
+        if (config.contains("maxiter")) c.mOuterIterations = py::cast<int>(config["maxiter"]);
+        if (config.contains("tol")) c.mAbsNewtonRsidualThreshold = py::cast<real>(config["tol"]);
+        if (config.contains("mixed") && py::cast<bool>(config["mixed"]))
+            throw std::invalid_argument("the mixed (pressure) formulation is not on the matrix-free path");
+        if (config.contains("optimizer")) mCfg.mOptimizer = py::cast<bool>(config["optimizer"]);
+        // the reference bindings hard-wire Neo-Hookean + NST_NEWTON_LS
+        c.mMaterial = config.contains("material") ? parse_material(config["material"]) : MMT_NEO_HOOKEAN;
+        mCfg.mSolver = config.contains("solver") ? parse_solver(config["solver"]) : NST_NEWTON_LS;
+        if (config.contains("gravity")) c.mGravity = py::cast<real>(config["gravity"]);
+        if (config.contains("zup")) c.mZUpAxis = py::cast<bool>(config["zup"]);
+        if (config.contains("inner_iters")) c.mInnerIterations = py::cast<int>(config["inner_iters"]);
+        else if (mCfg.mSolver != NST_NEWTON_CG) c.mInnerIterations = 100000;
+        if (config.contains("force_step")) c.mForceApplicationStep = py::cast<real>(config["force_step"]);
+        if (config.contains("rel_tol")) mCfg.mTolerance = py::cast<real>(config["rel_tol"]);
+        if (config.contains("cg_tol")) mCfg.mCGTolerance = py::cast<real>(config["cg_tol"]);
+        if (config.contains("verbose")) c.mVerbose = py::cast<bool>(config["verbose"]);
+        if (config.contains("precond")) {
+            std::string p = py::cast<std::string>(config["precond"]);
+            mCfg.mPreconditioner = p == "none" ? FemPhysicsMatrixFree::PC_NONE
+                                   : p == "learned" ? FemPhysicsMatrixFree::PC_LEARNED
+                                                    : FemPhysicsMatrixFree::PC_JACOBI;
+        }
+        c.mCustomConfig = &mCfg;
+        return c;
+    }
+
+    FemPhysicsMatrixFree::Config mCfg;
+    std::unique_ptr<FemPhysicsMatrixFree> mPhys;
+};
+
+}  // namespace
+
+PYBIND11_MODULE(pysolidfem, m)
+{
+    m.doc() = "SolidFEM-compatible bindings of the MI355X matrix-free Newton-CG solver";
+    py::class_<PyNonlinearFEM>(m, "NonlinearFEM")
+        .def(py::init<py::array_t<int, py::array::c_style | py::array::forcecast>,
+                      py::array_t<double, py::array::c_style | py::array::forcecast>,
+                      py::array_t<int, py::array::c_style | py::array::forcecast>, py::dict>())
+        .def("step", &PyNonlinearFEM::Step, py::arg("dt") = DT)
+        .def("simulate_static", &PyNonlinearFEM::SimulateStatic)
+        .def("get_nodes", &PyNonlinearFEM::GetNodes)
+        .def("get_tets", &PyNonlinearFEM::GetTets)
+        .def("set_lame_params", &PyNonlinearFEM::SetLameParams)
+        .def("get_shear_modulus", &PyNonlinearFEM::GetShearModulus)
+        .def("get_lame_lambda", &PyNonlinearFEM::GetLameLambda)
+        .def("set_pressure", &PyNonlinearFEM::SetPressure)
+        .def("set_external_forces", &PyNonlinearFEM::SetExternalForces)
+        .def("set_preconditioner_blocks", &PyNonlinearFEM::SetPreconditionerBlocks)
+        .def("last_stats", &PyNonlinearFEM::LastStats)
+        .def("get_force_fraction", &PyNonlinearFEM::GetForceFraction)
+        .def("check_for_inversion", &PyNonlinearFEM::CheckForInversion);
+}

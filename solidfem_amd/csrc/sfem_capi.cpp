@@ -440,14 +440,16 @@ int solve_converged(sfem_ctx* c, const sfem_solve_params* p, sfem_solve_stats* s
     const int maxn = p->outer_iterations > 0 ? p->outer_iterations : 50;
     const double eta_max = p->cg_rtol > 0 ? p->cg_rtol : 1e-1;
     double eta = eta_max, rn_prev = 0;
+    const double abs_tol = p->abs_tol > 0 ? p->abs_tol : 0.0;
+    auto target = [&](double e) { return std::max(p->tol * e, abs_tol); };
     for (int it = 0; it < maxn; it++) {
-        if (rn <= p->tol * en) break;
+        if (rn <= target(en)) break;
         if (it > 0) {
             const double ew = 0.9 * (rn / rn_prev) * (rn / rn_prev);
             const double safe = 0.9 * eta * eta;
             eta = std::min(eta_max, safe > 0.1 ? std::max(ew, safe) : ew);
         }
-        eta = std::max(eta, std::max(0.5 * p->tol * en / rn, 1e-6));
+        eta = std::max(eta, std::max(0.5 * target(en) / rn, 1e-6));
         if (pc_kind == SFEM_PC_JACOBI) build_jacobi(c);
         HIPCK(launch_to_f32(n3, c->d_r64.as<double>(), c->d_b32.as<float>(), c->stream));
         const float* minv = pc_kind == SFEM_PC_LEARNED ? c->d_minv_learned.as<float>() : c->d_minv.as<float>();
@@ -489,7 +491,7 @@ int solve_converged(sfem_ctx* c, const sfem_solve_params* p, sfem_solve_stats* s
     HIPCK(launch_to_f32(n3, c->d_u64.as<double>(), c->d_u32.as<float>(), c->stream));
     st->res = rn;
     st->ext_norm = en;
-    st->converged = rn <= p->tol * en;
+    st->converged = rn <= target(en);
     return st->converged ? SFEM_OK : SFEM_ENOTCONV;
 }
 
