@@ -168,6 +168,55 @@ int sfem_reset(sfem_ctx* ctx);
    (FemPhysicsMatrixFree.cpp:408). */
 int sfem_profile_hessvec(sfem_ctx* ctx, int reps, double* ms_elem, double* ms_gather);
 
+/* ---------------------------------------------------------------------------
+ * Learned block preconditioner (no reference counterpart).  A message-passing
+ * network over the node graph (3 rounds, width 32, BF16 MFMA GEMMs with FP32
+ * accumulation) maps per-node features of the block-Jacobi blocks D_i to SPD
+ * blocks Minv_i = C_i (L_i L_i^T + 0.01 I) C_i^T with C_i C_i^T = D_i^-1; the
+ * untrained network is block Jacobi.  Trained on the device on residual /
+ * solution pairs (r, e), K e = r, that the solver collects.
+ * ------------------------------------------------------------------------- */
+typedef struct sfem_gnn sfem_gnn;
+
+int sfem_gnn_create(uint64_t seed, sfem_gnn** out);
+void sfem_gnn_destroy(sfem_gnn* gnn);
+int sfem_gnn_num_params(sfem_gnn* gnn, int64_t* n);
+int sfem_gnn_get_params(sfem_gnn* gnn, float* w);
+int sfem_gnn_set_params(sfem_gnn* gnn, const float* w);
+
+/* Attaches the network to a context: SFEM_PC_LEARNED solves then rebuild the
+   blocks from it at every Newton step (Jacobi judge and fallback unchanged).
+   NULL detaches.  The network must outlive its attachment. */
+int sfem_set_learned_preconditioner(sfem_ctx* ctx, sfem_gnn* gnn);
+
+/* Solves K e = load at the current state (block-Jacobi PCG to rtol) and keeps
+   nsnap pairs (r_k, r_k . e_k): k = 0, snap_every, 2 snap_every, ... of a
+   second identical PCG run, with e_k = e - x_k so that K e_k = r_k.  The
+   pairs belong to the current state: change it only after training. */
+int sfem_collect_pairs(sfem_ctx* ctx, const double* load, int nsnap, int snap_every, double rtol, int* collected);
+int sfem_num_pairs(sfem_ctx* ctx, int64_t* n);
+int sfem_get_pair(sfem_ctx* ctx, int64_t k, float* r, double* er);
+int sfem_clear_pairs(sfem_ctx* ctx);
+
+/* Mean loss 1 - (z.r)^2 / ((z.Kz)(r.e)), z = Minv r, over pairs
+   [first, first+count) (count <= 16), and its parameter gradient when grad is
+   non-null (nparams floats). */
+int sfem_gnn_loss(sfem_gnn* gnn, sfem_ctx* ctx, int64_t first, int count, double* loss, float* grad);
+
+/* Mean loss over all pairs of the context for the network and for block Jacobi */
+int sfem_gnn_eval(sfem_gnn* gnn, sfem_ctx* ctx, double* loss_learned, double* loss_jacobi);
+
+/* `steps` Adam steps (lr, beta 0.9 / 0.999) on minibatches of `batch` (<= 16)
+   pairs drawn with `seed`; losses[steps] receives each step's loss (nullable) */
+int sfem_gnn_train(sfem_gnn* gnn, sfem_ctx* ctx, int steps, int batch, float lr, uint64_t seed, double* losses);
+
+/* The network's inputs and outputs at the current state: features (nn*16),
+   Cholesky factors of the inverse Jacobi blocks (nn*6), blocks (nn*9) */
+int sfem_gnn_inputs(sfem_gnn* gnn, sfem_ctx* ctx, float* feat, float* cbase, float* minv);
+
+/* Y = X W + b through the BF16 MFMA GEMM of layer `layer` (N rows) */
+int sfem_gnn_gemm_check(sfem_gnn* gnn, int layer, int64_t N, const float* X, float* Y);
+
 /* Number of elements, nodes, fixed nodes, and element blocks of the partition */
 int sfem_info(sfem_ctx* ctx, int64_t* ne, int64_t* nn, int64_t* nfixed, int64_t* nblocks);
 

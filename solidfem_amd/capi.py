@@ -25,6 +25,9 @@ EXPORTS = [
     "sfem_set_external_forces", "sfem_internal_forces", "sfem_gradients", "sfem_matvec",
     "sfem_block_diagonal", "sfem_solve", "sfem_step", "sfem_set_precond_blocks", "sfem_info",
     "sfem_reset", "sfem_profile_hessvec",
+    "sfem_gnn_create", "sfem_gnn_destroy", "sfem_gnn_num_params", "sfem_gnn_get_params", "sfem_gnn_set_params",
+    "sfem_set_learned_preconditioner", "sfem_collect_pairs", "sfem_num_pairs", "sfem_get_pair", "sfem_clear_pairs",
+    "sfem_gnn_loss", "sfem_gnn_eval", "sfem_gnn_train", "sfem_gnn_inputs", "sfem_gnn_gemm_check",
 ]
 
 _dp = C.POINTER(C.c_double)
@@ -98,6 +101,21 @@ def lib():
                            C.POINTER(C.c_int64)], C.c_int),
             "sfem_reset": ([V], C.c_int),
             "sfem_profile_hessvec": ([V, C.c_int, _dp, _dp], C.c_int),
+            "sfem_gnn_create": ([C.c_uint64, C.POINTER(V)], C.c_int),
+            "sfem_gnn_destroy": ([V], None),
+            "sfem_gnn_num_params": ([V, C.POINTER(C.c_int64)], C.c_int),
+            "sfem_gnn_get_params": ([V, _fp], C.c_int),
+            "sfem_gnn_set_params": ([V, _fp], C.c_int),
+            "sfem_set_learned_preconditioner": ([V, V], C.c_int),
+            "sfem_collect_pairs": ([V, _dp, C.c_int, C.c_int, C.c_double, C.POINTER(C.c_int)], C.c_int),
+            "sfem_num_pairs": ([V, C.POINTER(C.c_int64)], C.c_int),
+            "sfem_get_pair": ([V, C.c_int64, _fp, _dp], C.c_int),
+            "sfem_clear_pairs": ([V], C.c_int),
+            "sfem_gnn_loss": ([V, V, C.c_int64, C.c_int, _dp, _fp], C.c_int),
+            "sfem_gnn_eval": ([V, V, _dp, _dp], C.c_int),
+            "sfem_gnn_train": ([V, V, C.c_int, C.c_int, C.c_float, C.c_uint64, _dp], C.c_int),
+            "sfem_gnn_inputs": ([V, V, _fp, _fp, _fp], C.c_int),
+            "sfem_gnn_gemm_check": ([V, C.c_int, C.c_int64, _fp, _fp], C.c_int),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -226,3 +244,103 @@ class Solver:
         st = SolveStats()
         _check(lib().sfem_step(self.h, C.byref(p), C.byref(st)), allow=(ENOTCONV,) if allow_notconv else ())
         return st.as_dict()
+
+
+class LearnedPreconditioner:
+    """The message-passing network that predicts per-node 3x3 SPD blocks
+    (sfem_gnn_* in include/sfem_capi.h); weights live on the device."""
+
+    LAYERS = [(16, 32), (32, 32), (64, 32), (32, 32), (64, 32), (32, 32), (64, 32), (32, 32), (32, 32), (32, 16)]
+
+    def __init__(self, seed=1):
+        h = C.c_void_p()
+        _check(lib().sfem_gnn_create(int(seed), C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().sfem_gnn_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def num_params(self):
+        n = C.c_int64()
+        _check(lib().sfem_gnn_num_params(self.h, C.byref(n)))
+        return n.value
+
+    def params(self):
+        w = np.zeros(self.num_params(), dtype=np.float32)
+        _check(lib().sfem_gnn_get_params(self.h, _p(w, _fp)))
+        return w
+
+    def set_params(self, w):
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        assert w.size == self.num_params()
+        _check(lib().sfem_gnn_set_params(self.h, _p(w, _fp)))
+
+    def attach(self, solver):
+        _check(lib().sfem_set_learned_preconditioner(solver.h, self.h))
+
+    @staticmethod
+    def detach(solver):
+        _check(lib().sfem_set_learned_preconditioner(solver.h, None))
+
+    @staticmethod
+    def collect(solver, load, nsnap=8, snap_every=10, rtol=1e-6):
+        load = np.ascontiguousarray(load, dtype=np.float64).reshape(-1, 3)
+        n = C.c_int()
+        _check(lib().sfem_collect_pairs(solver.h, _p(load), int(nsnap), int(snap_every), float(rtol), C.byref(n)))
+        return n.value
+
+    @staticmethod
+    def num_pairs(solver):
+        n = C.c_int64()
+        _check(lib().sfem_num_pairs(solver.h, C.byref(n)))
+        return n.value
+
+    @staticmethod
+    def pair(solver, k):
+        r = np.zeros((solver.nn, 3), dtype=np.float32)
+        er = C.c_double()
+        _check(lib().sfem_get_pair(solver.h, int(k), _p(r, _fp), C.byref(er)))
+        return r, er.value
+
+    @staticmethod
+    def clear_pairs(solver):
+        _check(lib().sfem_clear_pairs(solver.h))
+
+    def loss(self, solver, first, count, grad=False):
+        l = C.c_double()
+        g = np.zeros(self.num_params(), dtype=np.float32) if grad else None
+        _check(lib().sfem_gnn_loss(self.h, solver.h, int(first), int(count), C.byref(l), _p(g, _fp)))
+        return (l.value, g) if grad else l.value
+
+    def evaluate(self, solver):
+        a, b = C.c_double(), C.c_double()
+        _check(lib().sfem_gnn_eval(self.h, solver.h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def train(self, solver, steps, batch=8, lr=1e-3, seed=0):
+        losses = np.zeros(steps)
+        _check(lib().sfem_gnn_train(self.h, solver.h, int(steps), int(batch), float(lr), int(seed), _p(losses)))
+        return losses
+
+    def inputs(self, solver):
+        f = np.zeros((solver.nn, 16), dtype=np.float32)
+        c = np.zeros((solver.nn, 6), dtype=np.float32)
+        m = np.zeros((solver.nn, 9), dtype=np.float32)
+        _check(lib().sfem_gnn_inputs(self.h, solver.h, _p(f, _fp), _p(c, _fp), _p(m, _fp)))
+        return f, c, m
+
+    def gemm_check(self, layer, X):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        K, NO = self.LAYERS[layer]
+        assert X.shape[1] == K
+        Y = np.zeros((X.shape[0], NO), dtype=np.float32)
+        _check(lib().sfem_gnn_gemm_check(self.h, int(layer), X.shape[0], _p(X, _fp), _p(Y, _fp)))
+        return Y

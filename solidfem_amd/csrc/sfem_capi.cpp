@@ -23,115 +23,18 @@
 #include "sfem_capi.h"
 #include "sfem_kernels.hpp"
 
+#include "sfem_internal.hpp"
+
 using namespace sfem;
 
-namespace {
-
+namespace sfem {
 thread_local std::string g_err;
-
 int fail(int code, const std::string& msg)
 {
     g_err = msg;
     return code;
 }
-
-struct HipError {
-    hipError_t e;
-    const char* what;
-};
-
-#define HIPCK(x)                                            \
-    do {                                                    \
-        hipError_t e_ = (x);                                \
-        if (e_ != hipSuccess) throw HipError{e_, #x};       \
-    } while (0)
-
-struct DevBuf {
-    void* p = nullptr;
-    size_t n = 0;
-    DevBuf() = default;
-    DevBuf(const DevBuf&) = delete;
-    DevBuf& operator=(const DevBuf&) = delete;
-    ~DevBuf() { release(); }
-    void release()
-    {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        n = 0;
-    }
-    void alloc(size_t bytes)
-    {
-        release();
-        if (bytes == 0) bytes = 16;
-        if (hipMalloc(&p, bytes) != hipSuccess) {
-            p = nullptr;
-            throw HipError{hipErrorOutOfMemory, "hipMalloc"};
-        }
-        n = bytes;
-    }
-    template <typename T> void upload(const std::vector<T>& v)
-    {
-        alloc(v.size() * sizeof(T));
-        if (!v.empty()) HIPCK(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
-    }
-    template <typename T> T* as() const { return static_cast<T*>(p); }
-};
-
-constexpr uint32_t F_SHARED = 0x80000000u;
-constexpr uint32_t F_FIXED = 0x40000000u;
-
-// 21-bit-per-axis Morton code
-uint64_t spread21(uint64_t v)
-{
-    v &= 0x1fffff;
-    v = (v | v << 32) & 0x1f00000000ffffULL;
-    v = (v | v << 16) & 0x1f0000ff0000ffULL;
-    v = (v | v << 8) & 0x100f00f00f00f00fULL;
-    v = (v | v << 4) & 0x10c30c30c30c30c3ULL;
-    v = (v | v << 2) & 0x1249249249249249ULL;
-    return v;
-}
-
-}  // namespace
-
-struct sfem_ctx {
-    int64_t ne = 0, nn = 0, nfixed = 0;
-    int material = 0;
-    double gravity[3] = {0, 0, 0};
-    double frac = 0.0;
-    double pressure = 0.0;
-    int64_t ntri = 0;
-    bool has_fdead = false;
-    bool has_learned = false;
-    hipStream_t stream = nullptr;
-
-    // host copies kept for preprocessing
-    std::vector<uint8_t> fixed_h;
-
-    // partition
-    BlockArgs blk{};
-    SharedArgs shr{};
-    int nblk = 0;
-    int64_t nrows_partial = 0;
-    DevBuf d_bnode_ptr, d_bnodes, d_lptr, d_lent, d_lidx, d_dminv32, d_coef32, d_dminv64, d_coef64;
-    DevBuf d_sh_node, d_sh_ptr, d_sh_idx;
-    DevBuf d_partial;        // nrows_partial * 9 floats (also used as FP64 x3)
-
-    // node data
-    DevBuf d_X0, d_mass, d_fixed, d_fdead;
-    DevBuf d_u64, d_ut64, d_u32;
-    DevBuf d_r64, d_ext64, d_f64tmp;
-    DevBuf d_b32, d_x32, d_r32, d_z32, d_p32, d_q32, d_d32, d_minv, d_minv_learned, d_bd32;
-    DevBuf d_tris, d_pptr, d_pidx, d_ftri;
-
-    // reductions
-    DevBuf d_scal, d_partials, d_counter, d_done;
-
-    ~sfem_ctx()
-    {
-        if (stream) (void)hipStreamDestroy(stream);
-    }
-};
+}  // namespace sfem
 
 namespace {
 
@@ -304,15 +207,16 @@ void residual(sfem_ctx* c, const double* u, double* r)
     HIPCK(launch_force(c->material, c->blk, c->shr, u, r, part, c->d_ext64.as<double>(), 1, c->stream));
 }
 
+}  // namespace
+
+namespace sfem {
+
 void hessvec(sfem_ctx* c, const float* d, float* out, const int* done)
 {
     HIPCK(launch_hessvec(c->material, c->blk, c->shr, c->d_u32.as<float>(), d, out, c->d_partial.as<float>(), done, c->stream));
 }
 
 // a.a, b.b, d.a (FP64) -> host
-struct Dots {
-    double aa, bb, da;
-};
 Dots dots3(sfem_ctx* c, const double* a, const double* b, const float* d)
 {
     HIPCK(launch_dots3(3 * c->nn, a, b, d, c->d_scal.as<double>(), S_NT, c->d_partials.as<double>(),
@@ -330,15 +234,9 @@ void build_jacobi(sfem_ctx* c)
     HIPCK(launch_invert_blocks(c->nn, c->d_bd32.as<float>(), c->d_fixed.as<uint8_t>(), c->d_minv.as<float>(), c->stream));
 }
 
-struct CgResult {
-    int iters;
-    int status;   // 1 converged, 0 max iterations, 3 breakdown, 4 NaN
-    double rr0, rr;
-};
-
 // PCG on K x = b (b already in d_b32); x in d_x32.
 CgResult pcg(sfem_ctx* c, const float* minv, int pc, int maxit, int test_kind, double tol2, int guard, int check_every,
-             int64_t* hv_evals)
+             int64_t* hv_evals, CgHook hook, void* user)
 {
     const int64_t n3 = 3 * c->nn;
     double* scal = c->d_scal.as<double>();
@@ -363,6 +261,7 @@ CgResult pcg(sfem_ctx* c, const float* minv, int pc, int maxit, int test_kind, d
         HIPCK(hipMemcpyAsync(&dflag, done, sizeof(int), hipMemcpyDeviceToHost, c->stream));
         HIPCK(hipStreamSynchronize(c->stream));
         if (dflag) break;
+        if (hook) hook(c, launched, user);
     }
     HIPCK(hipMemcpy(hs, scal, sizeof hs, hipMemcpyDeviceToHost));
     CgResult res;
@@ -374,17 +273,9 @@ CgResult pcg(sfem_ctx* c, const float* minv, int pc, int maxit, int test_kind, d
     return res;
 }
 
-template <typename F> int guard_call(const char* where, F&& f)
-{
-    try {
-        return f();
-    } catch (const HipError& e) {
-        return fail(e.e == hipErrorOutOfMemory ? SFEM_ENOMEM : SFEM_EDEVICE,
-                    std::string(where) + ": " + e.what + ": " + hipGetErrorString(e.e));
-    } catch (const std::exception& e) {
-        return fail(SFEM_EDEVICE, std::string(where) + ": " + e.what());
-    }
-}
+}  // namespace sfem
+
+namespace {
 
 // ----------------------------------------------------------- Newton drivers
 
@@ -429,7 +320,8 @@ int solve_converged(sfem_ctx* c, const sfem_solve_params* p, sfem_solve_stats* s
 {
     const int64_t n3 = 3 * c->nn;
     int pc_kind = p->precond;
-    if (pc_kind == SFEM_PC_LEARNED && !c->has_learned) return fail(SFEM_ESTATE, "learned preconditioner not installed");
+    if (pc_kind == SFEM_PC_LEARNED && !c->has_learned && !c->gnn)
+        return fail(SFEM_ESTATE, "learned preconditioner not installed");
     if (pc_kind == SFEM_PC_NONE)
         HIPCK(launch_identity_blocks(c->nn, c->d_fixed.as<uint8_t>(), c->d_minv.as<float>(), c->stream));
     residual(c, c->d_u64.as<double>(), c->d_r64.as<double>());
@@ -451,6 +343,10 @@ int solve_converged(sfem_ctx* c, const sfem_solve_params* p, sfem_solve_stats* s
         }
         eta = std::max(eta, std::max(0.5 * target(en) / rn, 1e-6));
         if (pc_kind == SFEM_PC_JACOBI) build_jacobi(c);
+        if (pc_kind == SFEM_PC_LEARNED && c->gnn) {
+            build_jacobi(c);   // network input and the blocks it corrects
+            gnn_install(c);
+        }
         HIPCK(launch_to_f32(n3, c->d_r64.as<double>(), c->d_b32.as<float>(), c->stream));
         const float* minv = pc_kind == SFEM_PC_LEARNED ? c->d_minv_learned.as<float>() : c->d_minv.as<float>();
         CgResult cg = pcg(c, minv, pc_kind == SFEM_PC_NONE ? 0 : 1, p->inner_iterations, 1, eta * eta, 1, p->check_every,
@@ -554,6 +450,7 @@ int sfem_create(const sfem_mesh* m, const sfem_material* mat, sfem_ctx** out)
         }
         std::vector<double> mass(nn, 0.0);
         c->fixed_h.assign(m->fixed, m->fixed + nn);
+        c->tets_h.assign(m->tets, m->tets + 4 * ne);
         for (int64_t i = 0; i < nn; i++) {
             if (m->fixed[i]) { c->nfixed++; continue; }
             const double inv = msum[i] != 0 ? 1.0 / msum[i] : 1.0;   // Node::invMass default 1

@@ -1,0 +1,175 @@
+// sfem_internal.hpp -- context layout and helpers shared by the host sources
+// of libsfem_hip.so (not part of the public C-ABI).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "sfem_capi.h"
+#include "sfem_kernels.hpp"
+
+
+namespace sfem {
+
+// records the message returned by sfem_last_error() on this thread
+int fail(int code, const std::string& msg);
+
+struct HipError {
+    hipError_t e;
+    const char* what;
+};
+
+#define HIPCK(x)                                            \
+    do {                                                    \
+        hipError_t e_ = (x);                                \
+        if (e_ != hipSuccess) throw HipError{e_, #x};       \
+    } while (0)
+
+struct DevBuf {   // owning device allocation
+    void* p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { release(); }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    void alloc(size_t bytes)
+    {
+        release();
+        if (bytes == 0) bytes = 16;
+        if (hipMalloc(&p, bytes) != hipSuccess) {
+            p = nullptr;
+            throw HipError{hipErrorOutOfMemory, "hipMalloc"};
+        }
+        n = bytes;
+    }
+    template <typename T> void upload(const std::vector<T>& v)
+    {
+        alloc(v.size() * sizeof(T));
+        if (!v.empty()) HIPCK(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    }
+    template <typename T> T* as() const { return static_cast<T*>(p); }
+};
+
+constexpr uint32_t F_SHARED = 0x80000000u;
+constexpr uint32_t F_FIXED = 0x40000000u;
+
+// 21-bit-per-axis Morton code
+inline uint64_t spread21(uint64_t v)
+{
+    v &= 0x1fffff;
+    v = (v | v << 32) & 0x1f00000000ffffULL;
+    v = (v | v << 16) & 0x1f0000ff0000ffULL;
+    v = (v | v << 8) & 0x100f00f00f00f00fULL;
+    v = (v | v << 4) & 0x10c30c30c30c30c3ULL;
+    v = (v | v << 2) & 0x1249249249249249ULL;
+    return v;
+}
+
+}  // namespace sfem
+
+struct sfem_gnn;
+
+using sfem::BlockArgs;
+using sfem::DevBuf;
+using sfem::SharedArgs;
+
+// residual/solution training pairs collected by the solver at one state
+struct SfemPairs {
+    int64_t n = 0;                                  // number of pairs
+    std::vector<std::unique_ptr<sfem::DevBuf>> r;   // FP32 residual vectors (nn*3)
+    std::vector<double> er;                         // r . e per pair (FP64)
+};
+
+struct sfem_ctx {
+    int64_t ne = 0, nn = 0, nfixed = 0;
+    int material = 0;
+    double gravity[3] = {0, 0, 0};
+    double frac = 0.0;
+    double pressure = 0.0;
+    int64_t ntri = 0;
+    bool has_fdead = false;
+    bool has_learned = false;
+    hipStream_t stream = nullptr;
+
+    // host copies kept for preprocessing
+    std::vector<uint8_t> fixed_h;
+    std::vector<int32_t> tets_h;   // element table (node graph of the learned preconditioner)
+
+    // partition
+    BlockArgs blk{};
+    SharedArgs shr{};
+    int nblk = 0;
+    int64_t nrows_partial = 0;
+    DevBuf d_bnode_ptr, d_bnodes, d_lptr, d_lent, d_lidx, d_dminv32, d_coef32, d_dminv64, d_coef64;
+    DevBuf d_sh_node, d_sh_ptr, d_sh_idx;
+    DevBuf d_partial;        // nrows_partial * 9 floats (also used as FP64 x3)
+
+    // node data
+    DevBuf d_X0, d_mass, d_fixed, d_fdead;
+    DevBuf d_u64, d_ut64, d_u32;
+    DevBuf d_r64, d_ext64, d_f64tmp;
+    DevBuf d_b32, d_x32, d_r32, d_z32, d_p32, d_q32, d_d32, d_minv, d_minv_learned, d_bd32;
+    DevBuf d_tris, d_pptr, d_pidx, d_ftri;
+
+    // reductions
+    DevBuf d_scal, d_partials, d_counter, d_done;
+
+    // learned preconditioner: node graph (CSR, no self loops), network, pairs
+    DevBuf d_adj_ptr, d_adj_idx;
+    int64_t nadj = 0;
+    sfem_gnn* gnn = nullptr;
+    SfemPairs pairs;
+
+    ~sfem_ctx()
+    {
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+
+namespace sfem {
+
+template <typename F> int guard_call(const char* where, F&& f)
+{
+    try {
+        return f();
+    } catch (const HipError& e) {
+        return fail(e.e == hipErrorOutOfMemory ? SFEM_ENOMEM : SFEM_EDEVICE,
+                    std::string(where) + ": " + e.what + ": " + hipGetErrorString(e.e));
+    } catch (const std::exception& e) {
+        return fail(SFEM_EDEVICE, std::string(where) + ": " + e.what());
+    }
+}
+
+// operations on a context (sfem_capi.cpp)
+void hessvec(sfem_ctx* c, const float* d, float* out, const int* done);
+void build_jacobi(sfem_ctx* c);
+struct Dots {
+    double aa, bb, da;
+};
+Dots dots3(sfem_ctx* c, const double* a, const double* b, const float* d);
+struct CgResult {
+    int iters;
+    int status;   // 1 converged, 0 max iterations, 3 breakdown, 4 NaN
+    double rr0, rr;
+};
+// on_check(iters) runs after every check_every iterations while not done
+typedef void (*CgHook)(sfem_ctx* c, int iters, void* user);
+CgResult pcg(sfem_ctx* c, const float* minv, int pc, int maxit, int test_kind, double tol2, int guard, int check_every,
+             int64_t* hv_evals, CgHook hook = nullptr, void* user = nullptr);
+
+// learned preconditioner (sfem_gnn_host.cpp): minv_learned from the network at
+// the current state (needs the Jacobi blocks in d_bd32)
+void gnn_install(sfem_ctx* c);
+void ensure_graph(sfem_ctx* c);
+
+}  // namespace sfem
