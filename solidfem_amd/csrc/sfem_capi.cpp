@@ -141,11 +141,12 @@ void build_partition(sfem_ctx* c, const sfem_mesh* m, const std::vector<double>&
         }
     }
     for (size_t i = 0; i < sh_node.size(); i++) sh_ptr[i + 1] += sh_ptr[i];
-    std::vector<int> sh_idx(sh_ptr.back());
+    // each shared node's partial rows get contiguous slots, in block order
+    std::vector<int> bslot(bnodes.size(), -1);
     std::vector<int> fill(sh_ptr.begin(), sh_ptr.end() - 1);
     for (size_t r = 0; r < bnodes.size(); r++) {   // rows are in block order
         const uint32_t id = bnodes[r] & 0x3fffffffu;
-        if (sh_of[id] >= 0) sh_idx[fill[sh_of[id]]++] = (int)r;
+        if (sh_of[id] >= 0) bslot[r] = fill[sh_of[id]]++;
     }
 
     c->d_bnode_ptr.upload(bnode_ptr);
@@ -159,7 +160,7 @@ void build_partition(sfem_ctx* c, const sfem_mesh* m, const std::vector<double>&
     c->d_coef64.upload(cf64);
     c->d_sh_node.upload(sh_node);
     c->d_sh_ptr.upload(sh_ptr);
-    c->d_sh_idx.upload(sh_idx);
+    c->d_bslot.upload(bslot);
     c->nrows_partial = (int64_t)bnodes.size();
     c->d_partial.alloc((size_t)c->nrows_partial * 9 * sizeof(float) + (size_t)c->nrows_partial * 3 * sizeof(double));
 
@@ -170,6 +171,7 @@ void build_partition(sfem_ctx* c, const sfem_mesh* m, const std::vector<double>&
     a.nmax = nmax;
     a.bnode_ptr = c->d_bnode_ptr.as<int>();
     a.bnodes = c->d_bnodes.as<uint32_t>();
+    a.bslot = c->d_bslot.as<int>();
     a.lptr = c->d_lptr.as<uint16_t>();
     a.lent = c->d_lent.as<uint16_t>();
     a.lidx = c->d_lidx.as<uint2>();
@@ -181,7 +183,6 @@ void build_partition(sfem_ctx* c, const sfem_mesh* m, const std::vector<double>&
     s.nshared = (int)sh_node.size();
     s.node = c->d_sh_node.as<uint32_t>();
     s.ptr = c->d_sh_ptr.as<int>();
-    s.idx = c->d_sh_idx.as<int>();
 }
 
 template <typename T> void zero(DevBuf& b, size_t n, hipStream_t s)
@@ -219,8 +220,7 @@ void hessvec(sfem_ctx* c, const float* d, float* out, const int* done)
 // a.a, b.b, d.a (FP64) -> host
 Dots dots3(sfem_ctx* c, const double* a, const double* b, const float* d)
 {
-    HIPCK(launch_dots3(3 * c->nn, a, b, d, c->d_scal.as<double>(), S_NT, c->d_partials.as<double>(),
-                       c->d_counter.as<unsigned>(), c->stream));
+    HIPCK(launch_dots3(3 * c->nn, a, b, d, c->d_scal.as<double>(), S_NT, c->d_partials.as<double>(), c->stream));
     double h[3];
     HIPCK(hipMemcpyAsync(h, c->d_scal.as<double>() + S_NT, sizeof h, hipMemcpyDeviceToHost, c->stream));
     HIPCK(hipStreamSynchronize(c->stream));
@@ -240,11 +240,13 @@ CgResult pcg(sfem_ctx* c, const float* minv, int pc, int maxit, int test_kind, d
 {
     const int64_t n3 = 3 * c->nn;
     double* scal = c->d_scal.as<double>();
-    double* parts = c->d_partials.as<double>();
-    unsigned* cnt = c->d_counter.as<unsigned>();
+    double* parts = c->d_partials.as<double>();          // CG update partials (2 x CG_GRID)
+    double* pq_elem = parts + 2 * CG_GRID;               // p.q per element block (nblk)
+    double* pq_part = pq_elem + c->nblk;                 // p.q per gather workgroup
+    const int n_pq = gather_grid(c->shr);
     int* done = c->d_done.as<int>();
     HIPCK(launch_cg_init(pc, n3, c->d_b32.as<float>(), c->d_x32.as<float>(), c->d_r32.as<float>(), c->d_z32.as<float>(),
-                         c->d_p32.as<float>(), minv, c->d_fixed.as<uint8_t>(), scal, parts, cnt, done, c->stream));
+                         c->d_p32.as<float>(), minv, c->d_fixed.as<uint8_t>(), scal, parts, done, c->stream));
     if (check_every < 1) check_every = 1;
     int launched = 0;
     int dflag = 0;
@@ -252,10 +254,11 @@ CgResult pcg(sfem_ctx* c, const float* minv, int pc, int maxit, int test_kind, d
     while (launched < maxit) {
         const int k = std::min(check_every, maxit - launched);
         for (int i = 0; i < k; i++) {
-            hessvec(c, c->d_p32.as<float>(), c->d_q32.as<float>(), done);
+            HIPCK(launch_hessvec(c->material, c->blk, c->shr, c->d_u32.as<float>(), c->d_p32.as<float>(),
+                                 c->d_q32.as<float>(), c->d_partial.as<float>(), done, c->stream, pq_elem, pq_part));
             HIPCK(launch_cg_iter_tail(pc, n3, c->d_x32.as<float>(), c->d_r32.as<float>(), c->d_z32.as<float>(),
-                                      c->d_p32.as<float>(), c->d_q32.as<float>(), minv, scal, parts, cnt, done, test_kind,
-                                      tol2, guard, c->stream));
+                                      c->d_p32.as<float>(), c->d_q32.as<float>(), minv, scal, pq_part, n_pq, parts, done,
+                                      test_kind, tol2, guard, c->stream));
         }
         launched += k;
         HIPCK(hipMemcpyAsync(&dflag, done, sizeof(int), hipMemcpyDeviceToHost, c->stream));
@@ -475,8 +478,7 @@ int sfem_create(const sfem_mesh* m, const sfem_material* mat, sfem_ctx** out)
         zero<float>(c->d_minv, 9 * (size_t)nn, s);
         zero<float>(c->d_bd32, 9 * (size_t)nn, s);
         zero<double>(c->d_scal, S_COUNT, s);
-        zero<double>(c->d_partials, 3 * (size_t)RED_GRID, s);
-        zero<unsigned>(c->d_counter, 4, s);
+        zero<double>(c->d_partials, std::max<size_t>(3 * (size_t)RED_GRID, 2 * (size_t)CG_GRID + c->nblk + gather_grid(c->shr)), s);
         zero<int>(c->d_done, 4, s);
         HIPCK(hipStreamSynchronize(s));
         return SFEM_OK;

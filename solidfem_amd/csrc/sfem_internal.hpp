@@ -110,7 +110,7 @@ struct sfem_ctx {
     int nblk = 0;
     int64_t nrows_partial = 0;
     DevBuf d_bnode_ptr, d_bnodes, d_lptr, d_lent, d_lidx, d_dminv32, d_coef32, d_dminv64, d_coef64;
-    DevBuf d_sh_node, d_sh_ptr, d_sh_idx;
+    DevBuf d_sh_node, d_sh_ptr, d_bslot;
     DevBuf d_partial;        // nrows_partial * 9 floats (also used as FP64 x3)
 
     // node data
@@ -121,7 +121,7 @@ struct sfem_ctx {
     DevBuf d_tris, d_pptr, d_pidx, d_ftri;
 
     // reductions
-    DevBuf d_scal, d_partials, d_counter, d_done;
+    DevBuf d_scal, d_partials, d_done;
 
     // learned preconditioner: node graph (CSR, no self loops), network, pairs
     DevBuf d_adj_ptr, d_adj_idx;

@@ -33,6 +33,73 @@ constexpr uint32_t FLAG_FIXED = 0x40000000u;
 // slot of (component q, element el) in the corner-output array
 __device__ __forceinline__ int sfo(int q, int el, int BE) { return q * BE + el; }
 
+// ------------------------------------------------------------ reductions
+template <int NV>
+__device__ __forceinline__ void block_sum(double (&v)[NV], double* lds /* >= NV*16 */)
+{
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+#pragma unroll
+    for (int q = 0; q < NV; q++)
+        for (int off = 32; off > 0; off >>= 1) v[q] += __shfl_down(v[q], off, 64);
+    if (lane == 0)
+#pragma unroll
+        for (int q = 0; q < NV; q++) lds[q * 16 + w] = v[q];
+    __syncthreads();
+    if (w == 0) {
+#pragma unroll
+        for (int q = 0; q < NV; q++) {
+            double t = lane < nw ? lds[q * 16 + lane] : 0.0;
+            for (int off = 8; off > 0; off >>= 1) t += __shfl_down(t, off, 64);
+            v[q] = t;
+        }
+    }
+    __syncthreads();
+}
+
+// this workgroup's NV sums -> part[q * gridDim.x + blockIdx.x]
+template <int NV>
+__device__ __forceinline__ void block_partials(double (&v)[NV], double* part)
+{
+    __shared__ double lds[NV * 16];
+    block_sum<NV>(v, lds);
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int q = 0; q < NV; q++) part[(size_t)q * gridDim.x + blockIdx.x] = v[q];
+}
+
+// every thread of the workgroup gets sum_i part[q * stride + i], i < n, in a
+// fixed order (thread-strided partial sums, then the fixed block tree)
+template <int NV>
+__device__ __forceinline__ void sum_partials(const double* part, int n, int stride, double (&tot)[NV])
+{
+    __shared__ double lds[NV * 16];
+    __shared__ double res[NV];
+    double acc[NV];
+#pragma unroll
+    for (int q = 0; q < NV; q++) {
+        acc[q] = 0.0;
+        for (int i = threadIdx.x; i < n; i += blockDim.x) acc[q] += part[(size_t)q * stride + i];
+    }
+    block_sum<NV>(acc, lds);
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int q = 0; q < NV; q++) res[q] = acc[q];
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < NV; q++) tot[q] = res[q];
+}
+
+// one workgroup: scal[slot + q] = sum of the NV partial arrays
+template <int NV>
+__global__ __launch_bounds__(256) void k_finish(const double* __restrict__ part, int n, double* scal, int slot)
+{
+    double tot[NV];
+    sum_partials<NV>(part, n, n, tot);
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int q = 0; q < NV; q++) scal[slot + q] = tot[q];
+}
+
 // ------------------------------------------------------- element block kernel
 // KIND 0: Hessian-vector (FP32): out = K d            (u, d staged)
 // KIND 1: internal force (FP64):  out = f_int (+ ext) (u staged)
@@ -42,13 +109,21 @@ template <> struct KindTraits<0> { typedef float T; static constexpr int W = 3; 
 template <> struct KindTraits<1> { typedef double T; static constexpr int W = 3; static constexpr bool HAS_D = false; };
 template <> struct KindTraits<2> { typedef float T; static constexpr int W = 9; static constexpr bool HAS_D = false; };
 
+// LDS: corner outputs (4 W BE of T), staged u (and d) rows (3 nmax each), the
+// block's incidence entries (4 BE uint16) and offsets (nmax + 1 uint16)
+template <int KIND>
+__host__ __device__ constexpr size_t lds_corner_bytes(int BE)
+{
+    return sizeof(typename KindTraits<KIND>::T) * 4 * KindTraits<KIND>::W * BE;
+}
+
 template <int KIND, int MAT, int BE>
 __global__ __launch_bounds__(BE) void k_elem_block(BlockArgs a, const typename KindTraits<KIND>::T* __restrict__ u,
                                                     const typename KindTraits<KIND>::T* __restrict__ d,
                                                     typename KindTraits<KIND>::T* __restrict__ out,
                                                     typename KindTraits<KIND>::T* __restrict__ partial,
                                                     const double* __restrict__ ext, int mask_fixed,
-                                                    const int* __restrict__ done)
+                                                    const int* __restrict__ done, double* __restrict__ dot_part)
 {
     typedef typename KindTraits<KIND>::T T;
     constexpr int W = KindTraits<KIND>::W;
@@ -58,28 +133,23 @@ __global__ __launch_bounds__(BE) void k_elem_block(BlockArgs a, const typename K
     T* sf = reinterpret_cast<T*>(smem_raw);          // 4*W*BE corner outputs
     T* su = sf + 4 * W * BE;                          // 3*nmax
     T* sd = su + 3 * a.nmax;                          // 3*nmax (HAS_D)
+    uint16_t* s_ent = reinterpret_cast<uint16_t*>(sd + (HAS_D ? 3 * a.nmax : 0));   // 4*BE
+    uint16_t* s_ptr = s_ent + 4 * BE;                                                // nmax+1
 
     const int b = blockIdx.x;
     const int tid = threadIdx.x;
     const int n0 = a.bnode_ptr[b];
     const int nb = a.bnode_ptr[b + 1] - n0;
 
-    for (int j = tid; j < nb; j += BE) {
-        const uint32_t node = a.bnodes[n0 + j] & NODE_MASK;
-        const T* up = u + 3 * (size_t)node;
-        su[3 * j] = up[0]; su[3 * j + 1] = up[1]; su[3 * j + 2] = up[2];
-        if (HAS_D) {
-            const T* dp = d + 3 * (size_t)node;
-            sd[3 * j] = dp[0]; sd[3 * j + 1] = dp[1]; sd[3 * j + 2] = dp[2];
-        }
-    }
-    __syncthreads();
-
+    // element data first: independent of the node staging, so its latency
+    // overlaps the gathers below
     const int64_t e = (int64_t)b * BE + tid;
-    if (e < a.ne) {
+    const bool live = e < a.ne;
+    T g[9];
+    T cmu = T(0), clam = T(0);
+    uint2 li = make_uint2(0, 0);
+    if (live) {
         const size_t st = (size_t)a.ne_pad;
-        T g[9];
-        T cmu, clam;
         if (sizeof(T) == 4) {
 #pragma unroll
             for (int k = 0; k < 9; k++) g[k] = (T)a.dminv32[k * st + e];
@@ -91,7 +161,28 @@ __global__ __launch_bounds__(BE) void k_elem_block(BlockArgs a, const typename K
             cmu = (T)a.coef64[e];
             clam = (T)a.coef64[st + e];
         }
-        const uint2 li = a.lidx[e];
+        li = a.lidx[e];
+    }
+    // stage block nodes and the block's incidence lists
+    for (int j = tid; j < nb; j += BE) {
+        const uint32_t node = a.bnodes[n0 + j] & NODE_MASK;
+        const T* up = u + 3 * (size_t)node;
+        su[3 * j] = up[0]; su[3 * j + 1] = up[1]; su[3 * j + 2] = up[2];
+        if (HAS_D) {
+            const T* dp = d + 3 * (size_t)node;
+            sd[3 * j] = dp[0]; sd[3 * j + 1] = dp[1]; sd[3 * j + 2] = dp[2];
+        }
+    }
+    {
+        const uint32_t* le = reinterpret_cast<const uint32_t*>(a.lent + (size_t)b * 4 * BE);
+        uint32_t* se = reinterpret_cast<uint32_t*>(s_ent);
+        for (int k = tid; k < 2 * BE; k += BE) se[k] = le[k];
+        const uint16_t* lp = a.lptr + (n0 + b);
+        for (int j = tid; j <= nb; j += BE) s_ptr[j] = lp[j];
+    }
+    __syncthreads();
+
+    if (live) {
         const int l[4] = {(int)(li.x & 0xffffu), (int)(li.x >> 16), (int)(li.y & 0xffffu), (int)(li.y >> 16)};
         T uu[4][3];
 #pragma unroll
@@ -153,15 +244,14 @@ __global__ __launch_bounds__(BE) void k_elem_block(BlockArgs a, const typename K
     }
     __syncthreads();
 
-    const uint16_t* lp = a.lptr + (n0 + b);
-    const uint16_t* le = a.lent + (size_t)b * 4 * BE;
+    double dot = 0.0;
     for (int j = tid; j < nb; j += BE) {
-        const int s0 = lp[j], s1 = lp[j + 1];
+        const int s0 = s_ptr[j], s1 = s_ptr[j + 1];
         T acc[W];
 #pragma unroll
         for (int q = 0; q < W; q++) acc[q] = T(0);
         for (int k = s0; k < s1; k++) {
-            const int ent = le[k];
+            const int ent = s_ent[k];
             const int el = ent >> 2, cn = ent & 3;
 #pragma unroll
             for (int q = 0; q < W; q++) acc[q] += sf[sfo(cn * W + q, el, BE)];
@@ -169,7 +259,7 @@ __global__ __launch_bounds__(BE) void k_elem_block(BlockArgs a, const typename K
         const uint32_t fl = a.bnodes[n0 + j];
         const uint32_t node = fl & NODE_MASK;
         if (fl & FLAG_SHARED) {
-            T* pp = partial + (size_t)W * (n0 + j);
+            T* pp = partial + (size_t)W * a.bslot[n0 + j];
 #pragma unroll
             for (int q = 0; q < W; q++) pp[q] = acc[q];
         } else {
@@ -180,110 +270,74 @@ __global__ __launch_bounds__(BE) void k_elem_block(BlockArgs a, const typename K
             } else {
 #pragma unroll
                 for (int q = 0; q < W; q++) op[q] = acc[q] + (ext ? (T)ext[3 * (size_t)node + q % 3] : T(0));
+                if (KIND == 0) dot += (double)sd[3 * j] * acc[0] + (double)sd[3 * j + 1] * acc[1] + (double)sd[3 * j + 2] * acc[2];
             }
         }
     }
+    if (KIND == 0 && dot_part) {
+        double v[1] = {dot};
+        block_partials<1>(v, dot_part);
+    }
 }
 
-// shared nodes: sum the partial rows in block order
-template <typename T, int W>
+// Shared nodes: sum each node's contiguous partial rows (block order).  With
+// DOT, each workgroup also folds a slice of the element kernel's p.q
+// partials and its own nodes' p.q into dot_out[blockIdx.x].
+template <typename T, int W, bool DOT>
 __global__ __launch_bounds__(256) void k_shared_gather(SharedArgs s, const T* __restrict__ partial, T* __restrict__ out,
                                                        const double* __restrict__ ext, int mask_fixed,
-                                                       const int* __restrict__ done)
+                                                       const int* __restrict__ done, const float* __restrict__ pvec,
+                                                       const double* __restrict__ dot_in, int n_in,
+                                                       double* __restrict__ dot_out)
 {
     if (done && *done) return;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= s.nshared) return;
-    const uint32_t fl = s.node[i];
-    const uint32_t node = fl & NODE_MASK;
-    T acc[W];
+    double dot = 0.0;
+    if (i < s.nshared) {
+        const uint32_t fl = s.node[i];
+        const uint32_t node = fl & NODE_MASK;
+        T acc[W];
 #pragma unroll
-    for (int q = 0; q < W; q++) acc[q] = T(0);
-    for (int k = s.ptr[i]; k < s.ptr[i + 1]; k++) {
-        const T* pp = partial + (size_t)W * s.idx[k];
+        for (int q = 0; q < W; q++) acc[q] = T(0);
+        for (int k = s.ptr[i]; k < s.ptr[i + 1]; k++) {
+            const T* pp = partial + (size_t)W * k;
 #pragma unroll
-        for (int q = 0; q < W; q++) acc[q] += pp[q];
-    }
-    T* op = out + (size_t)W * node;
-    if ((fl & FLAG_FIXED) && mask_fixed) {
+            for (int q = 0; q < W; q++) acc[q] += pp[q];
+        }
+        T* op = out + (size_t)W * node;
+        if ((fl & FLAG_FIXED) && mask_fixed) {
 #pragma unroll
-        for (int q = 0; q < W; q++) op[q] = (W == 9 && (q == 0 || q == 4 || q == 8)) ? T(1) : T(0);
-    } else {
+            for (int q = 0; q < W; q++) op[q] = (W == 9 && (q == 0 || q == 4 || q == 8)) ? T(1) : T(0);
+        } else {
 #pragma unroll
-        for (int q = 0; q < W; q++) op[q] = acc[q] + (ext ? (T)ext[3 * (size_t)node + q % 3] : T(0));
-    }
-}
-
-// ------------------------------------------------------------ reductions
-template <int NV>
-__device__ __forceinline__ void block_sum(double (&v)[NV], double* lds /* >= NV*16 */)
-{
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
-#pragma unroll
-    for (int q = 0; q < NV; q++)
-        for (int off = 32; off > 0; off >>= 1) v[q] += __shfl_down(v[q], off, 64);
-    if (lane == 0)
-#pragma unroll
-        for (int q = 0; q < NV; q++) lds[q * 16 + w] = v[q];
-    __syncthreads();
-    if (w == 0) {
-#pragma unroll
-        for (int q = 0; q < NV; q++) {
-            double t = lane < nw ? lds[q * 16 + lane] : 0.0;
-            for (int off = 8; off > 0; off >>= 1) t += __shfl_down(t, off, 64);
-            v[q] = t;
+            for (int q = 0; q < W; q++) op[q] = acc[q] + (ext ? (T)ext[3 * (size_t)node + q % 3] : T(0));
+            if (DOT) dot += (double)pvec[3 * (size_t)node] * acc[0] + (double)pvec[3 * (size_t)node + 1] * acc[1] +
+                            (double)pvec[3 * (size_t)node + 2] * acc[2];
         }
     }
-    __syncthreads();
-}
-
-// Writes this workgroup's NV sums; returns true in the one workgroup that
-// finishes last, after which `tot` holds the totals (all threads).
-template <int NV>
-__device__ __forceinline__ bool grid_sum(double (&v)[NV], double* partials, unsigned* counter, double (&tot)[NV])
-{
-    __shared__ double lds[NV * 16];
-    __shared__ int is_last;
-    block_sum<NV>(v, lds);
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int q = 0; q < NV; q++) partials[(size_t)q * gridDim.x + blockIdx.x] = v[q];
-        __threadfence();
-        unsigned t = atomicAdd(counter, 1u);
-        is_last = (t == gridDim.x - 1);
+    if (DOT) {
+        const int chunk = (n_in + gridDim.x - 1) / gridDim.x;
+        const int b0 = blockIdx.x * chunk;
+        const int b1 = b0 + chunk < n_in ? b0 + chunk : n_in;
+        for (int k = b0 + threadIdx.x; k < b1; k += blockDim.x) dot += dot_in[k];
+        double v[1] = {dot};
+        block_partials<1>(v, dot_out);
     }
-    __syncthreads();
-    if (!is_last) return false;
-    __threadfence();
-    double acc[NV];
-#pragma unroll
-    for (int q = 0; q < NV; q++) {
-        acc[q] = 0.0;
-        double* pv = partials + (size_t)q * gridDim.x;
-        for (int i = threadIdx.x; i < (int)gridDim.x; i += blockDim.x)
-            acc[q] += __hip_atomic_load(pv + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    block_sum<NV>(acc, lds);
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int q = 0; q < NV; q++) lds[q] = acc[q];
-        *counter = 0u;     // ready for the next launch
-    }
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < NV; q++) tot[q] = lds[q];
-    return true;
 }
 
 // ---------------------------------------------------------------- CG kernels
-// scalars (double) layout: see Scal in sfem_kernels.hpp
+// Per iteration: hessvec (element blocks + shared gather, which also leave
+// p.q partials), k_cg_update (alpha, x, r, z; r.z and r.r partials), k_cg_p
+// (beta, convergence test, p).  Each workgroup reduces the partials it needs
+// itself in a fixed order: no atomics, no extra launch.  Scalars that a kernel
+// reads are never written by the same kernel (S_RZ_PREV holds the previous r.z).
 
-// x = 0, r = b, z = M r, p = z; rz = r.z, rr = r.r
+// x = 0, r = b, z = M r, p = z; partials of r.z, r.r
 template <int PC>
 __global__ __launch_bounds__(256) void k_cg_init(int64_t n3, const float* __restrict__ b, float* __restrict__ x,
                                                  float* __restrict__ r, float* __restrict__ z, float* __restrict__ p,
                                                  const float* __restrict__ minv, const uint8_t* __restrict__ fixed,
-                                                 double* scal, double* partials, unsigned* counter, int* done)
+                                                 double* __restrict__ part)
 {
     const int64_t nn = n3 / 3;
     double v[2] = {0.0, 0.0};
@@ -304,8 +358,14 @@ __global__ __launch_bounds__(256) void k_cg_init(int64_t n3, const float* __rest
         v[0] += (double)rv[0] * zv[0] + (double)rv[1] * zv[1] + (double)rv[2] * zv[2];
         v[1] += (double)rv[0] * rv[0] + (double)rv[1] * rv[1] + (double)rv[2] * rv[2];
     }
+    block_partials<2>(v, part);
+}
+
+__global__ __launch_bounds__(256) void k_cg_init_finish(const double* __restrict__ part, int n, double* scal, int* done)
+{
     double tot[2];
-    if (grid_sum<2>(v, partials, counter, tot) && threadIdx.x == 0) {
+    sum_partials<2>(part, n, n, tot);
+    if (threadIdx.x == 0) {
         scal[S_RZ] = tot[0];
         scal[S_RZ0] = tot[0];
         scal[S_RR] = tot[1];
@@ -315,35 +375,30 @@ __global__ __launch_bounds__(256) void k_cg_init(int64_t n3, const float* __rest
     }
 }
 
-// pq = p.q ; alpha = rz / pq
-__global__ __launch_bounds__(256) void k_cg_alpha(int64_t n3, const float* __restrict__ p, const float* __restrict__ q,
-                                                  double* scal, double* partials, unsigned* counter, int* done,
-                                                  int guard)
-{
-    if (*done) return;
-    const int64_t nn = n3 / 3;
-    double v[1] = {0.0};
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nn; i += (int64_t)gridDim.x * blockDim.x)
-        v[0] += (double)p[3 * i] * q[3 * i] + (double)p[3 * i + 1] * q[3 * i + 1] + (double)p[3 * i + 2] * q[3 * i + 2];
-    double tot[1];
-    if (grid_sum<1>(v, partials, counter, tot) && threadIdx.x == 0) {
-        scal[S_PQ] = tot[0];
-        scal[S_ALPHA] = scal[S_RZ] / tot[0];
-        if (guard && !(tot[0] > 0.0)) *done = 3;   // breakdown: not positive definite
-    }
-}
-
-// x += a p; r -= a q; z = M r; rz' = r.z; rr = r.r; beta = rz'/rz; test
+// alpha = rz / p.q (every workgroup); x += a p; r -= a q; z = M r; partials
 template <int PC>
 __global__ __launch_bounds__(256) void k_cg_update(int64_t n3, float* __restrict__ x, float* __restrict__ r,
                                                    float* __restrict__ z, const float* __restrict__ p,
                                                    const float* __restrict__ q, const float* __restrict__ minv,
-                                                   double* scal, double* partials, unsigned* counter, int* done,
-                                                   int test_kind, double tol2)
+                                                   double* scal, const double* __restrict__ pq_part, int n_pq,
+                                                   double* __restrict__ part, int* done, int guard)
 {
     if (*done) return;
+    double pq[1];
+    sum_partials<1>(pq_part, n_pq, n_pq, pq);
+    const double rz = scal[S_RZ];
+    if (guard && !(pq[0] > 0.0)) {          // breakdown: not positive definite
+        if (blockIdx.x == 0 && threadIdx.x == 0) *done = 3;
+        return;
+    }
+    const double alpha_d = rz / pq[0];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        scal[S_PQ] = pq[0];
+        scal[S_ALPHA] = alpha_d;
+        scal[S_RZ_PREV] = rz;
+    }
+    const float alpha = (float)alpha_d;
     const int64_t nn = n3 / 3;
-    const float alpha = (float)scal[S_ALPHA];
     double v[2] = {0.0, 0.0};
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nn; i += (int64_t)gridDim.x * blockDim.x) {
         float rv[3], zv[3];
@@ -366,39 +421,44 @@ __global__ __launch_bounds__(256) void k_cg_update(int64_t n3, float* __restrict
         v[0] += (double)rv[0] * zv[0] + (double)rv[1] * zv[1] + (double)rv[2] * zv[2];
         v[1] += (double)rv[0] * rv[0] + (double)rv[1] * rv[1] + (double)rv[2] * rv[2];
     }
-    double tot[2];
-    if (grid_sum<2>(v, partials, counter, tot) && threadIdx.x == 0) {
-        const double rz1 = tot[0];
-        scal[S_BETA] = rz1 / scal[S_RZ];
-        scal[S_RZ] = rz1;
-        scal[S_RR] = tot[1];
-        scal[S_ITER] += 1.0;
-        // test 0: the reference criterion |r.z| <= eps^2 |r0.z0| (IterativeSolvers.h:180-185)
-        // test 1: true residual ||r||^2 <= tol^2 ||r0||^2
-        const bool conv = test_kind == 0 ? (fabs(rz1) <= tol2 * fabs(scal[S_RZ0])) : (tot[1] <= tol2 * scal[S_RR0]);
-        if (conv) *done = 1;
-        else if (!(tot[1] == tot[1]) || !(rz1 == rz1)) *done = 4;   // NaN
-    }
+    block_partials<2>(v, part);
 }
 
-// p = z + beta p
+// beta = rz' / rz (every workgroup); test; p = z + beta p
+// test 0: the reference criterion |r.z| <= eps^2 |r0.z0| (IterativeSolvers.h:180-185)
+// test 1: true residual ||r||^2 <= tol^2 ||r0||^2
 template <int PC>
 __global__ __launch_bounds__(256) void k_cg_p(int64_t n3, const float* __restrict__ z, const float* __restrict__ r,
-                                              float* __restrict__ p, const double* scal, const int* done)
+                                              float* __restrict__ p, double* scal, const double* __restrict__ part, int n_part,
+                                              int* done, int test_kind, double tol2)
 {
     if (*done) return;
-    const float beta = (float)scal[S_BETA];
+    double t[2];
+    sum_partials<2>(part, n_part, n_part, t);
+    const double rz1 = t[0], rr = t[1];
+    const bool conv = test_kind == 0 ? (fabs(rz1) <= tol2 * fabs(scal[S_RZ0])) : (rr <= tol2 * scal[S_RR0]);
+    const bool bad = !(rr == rr) || !(rz1 == rz1);
+    const double beta = rz1 / scal[S_RZ_PREV];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        scal[S_RZ] = rz1;
+        scal[S_RR] = rr;
+        scal[S_BETA] = beta;
+        scal[S_ITER] += 1.0;
+        if (conv) *done = 1;
+        else if (bad) *done = 4;
+    }
+    if (conv || bad) return;
+    const float fb = (float)beta;
     const float* zz = PC ? z : r;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n3; i += (int64_t)gridDim.x * blockDim.x)
-        p[i] = zz[i] + beta * p[i];
+        p[i] = zz[i] + fb * p[i];
 }
 
 // ---------------------------------------------------------------- FP64 vectors
 // a.a, b.b and d.a for FP64 node vectors a, b and an FP32 direction d
 // -> scal[slot], scal[slot+1], scal[slot+2]  (b, d may be null)
 __global__ __launch_bounds__(256) void k_dots3(int64_t n3, const double* __restrict__ a, const double* __restrict__ b,
-                                               const float* __restrict__ d, double* scal, int slot, double* partials,
-                                               unsigned* counter)
+                                               const float* __restrict__ d, double* __restrict__ part)
 {
     double v[3] = {0.0, 0.0, 0.0};
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n3 / 3; i += (int64_t)gridDim.x * blockDim.x) {
@@ -406,12 +466,7 @@ __global__ __launch_bounds__(256) void k_dots3(int64_t n3, const double* __restr
         if (b) v[1] += b[3 * i] * b[3 * i] + b[3 * i + 1] * b[3 * i + 1] + b[3 * i + 2] * b[3 * i + 2];
         if (d) v[2] += (double)d[3 * i] * a[3 * i] + (double)d[3 * i + 1] * a[3 * i + 1] + (double)d[3 * i + 2] * a[3 * i + 2];
     }
-    double tot[3];
-    if (grid_sum<3>(v, partials, counter, tot) && threadIdx.x == 0) {
-        scal[slot] = tot[0];
-        scal[slot + 1] = tot[1];
-        scal[slot + 2] = tot[2];
-    }
+    block_partials<3>(v, part);
 }
 
 // ut = u + alpha * dx (free rows); u32 = (float) ut
@@ -518,24 +573,20 @@ static int grid_for(int64_t n, int bs, int cap)
 
 template <int KIND, int BE>
 static hipError_t launch_elem_mat(int mat, const BlockArgs& a, const void* u, const void* d, void* out, void* partial,
-                                  const double* ext, int mask_fixed, const int* done, size_t lds, hipStream_t s)
+                                  const double* ext, int mask_fixed, const int* done, double* dot, size_t lds, hipStream_t s)
 {
     typedef typename KindTraits<KIND>::T T;
     dim3 grid(a.nblk), block(BE);
+#define SFEM_ELEM_CASE(M)                                                                                            \
+    hipLaunchKernelGGL((k_elem_block<KIND, M, BE>), grid, block, lds, s, a, (const T*)u, (const T*)d, (T*)out,      \
+                       (T*)partial, ext, mask_fixed, done, dot)
     switch (mat) {
-    case LINEAR:
-        hipLaunchKernelGGL((k_elem_block<KIND, LINEAR, BE>), grid, block, lds, s, a, (const T*)u, (const T*)d, (T*)out, (T*)partial, ext, mask_fixed, done);
-        break;
-    case STVK:
-        hipLaunchKernelGGL((k_elem_block<KIND, STVK, BE>), grid, block, lds, s, a, (const T*)u, (const T*)d, (T*)out, (T*)partial, ext, mask_fixed, done);
-        break;
-    case COROTATIONAL:
-        hipLaunchKernelGGL((k_elem_block<KIND, COROTATIONAL, BE>), grid, block, lds, s, a, (const T*)u, (const T*)d, (T*)out, (T*)partial, ext, mask_fixed, done);
-        break;
-    default:
-        hipLaunchKernelGGL((k_elem_block<KIND, NEO_HOOKEAN, BE>), grid, block, lds, s, a, (const T*)u, (const T*)d, (T*)out, (T*)partial, ext, mask_fixed, done);
-        break;
+    case LINEAR: SFEM_ELEM_CASE(LINEAR); break;
+    case STVK: SFEM_ELEM_CASE(STVK); break;
+    case COROTATIONAL: SFEM_ELEM_CASE(COROTATIONAL); break;
+    default: SFEM_ELEM_CASE(NEO_HOOKEAN); break;
     }
+#undef SFEM_ELEM_CASE
     return hipGetLastError();
 }
 
@@ -543,8 +594,8 @@ template <int KIND>
 static size_t lds_bytes(const BlockArgs& a)
 {
     typedef typename KindTraits<KIND>::T T;
-    constexpr int W = KindTraits<KIND>::W;
-    return sizeof(T) * ((size_t)4 * W * BLOCK_E + (size_t)3 * a.nmax * (KindTraits<KIND>::HAS_D ? 2 : 1));
+    const size_t rows = (size_t)3 * a.nmax * (KindTraits<KIND>::HAS_D ? 2 : 1);
+    return lds_corner_bytes<KIND>(BLOCK_E) + sizeof(T) * rows + sizeof(uint16_t) * (4 * BLOCK_E + a.nmax + 1) + 16;
 }
 
 template <typename T, int W>
@@ -552,16 +603,23 @@ static hipError_t launch_shared(const SharedArgs& s, const void* partial, void* 
                                 const int* done, hipStream_t st)
 {
     if (s.nshared == 0) return hipSuccess;
-    hipLaunchKernelGGL((k_shared_gather<T, W>), dim3((s.nshared + 255) / 256), dim3(256), 0, st, s, (const T*)partial, (T*)out, ext, mask_fixed, done);
+    hipLaunchKernelGGL((k_shared_gather<T, W, false>), dim3((s.nshared + 255) / 256), dim3(256), 0, st, s, (const T*)partial,
+                       (T*)out, ext, mask_fixed, done, nullptr, nullptr, 0, nullptr);
     return hipGetLastError();
 }
 
+int gather_grid(const SharedArgs& s) { return s.nshared > 0 ? (s.nshared + 255) / 256 : 1; }
+
 hipError_t launch_hessvec(int mat, const BlockArgs& a, const SharedArgs& s, const float* u32, const float* d32, float* out,
-                          float* partial, const int* done, hipStream_t st)
+                          float* partial, const int* done, hipStream_t st, double* dot_elem, double* dot_out)
 {
-    hipError_t e = launch_elem_mat<0, BLOCK_E>(mat, a, u32, d32, out, partial, nullptr, 1, done, lds_bytes<0>(a), st);
+    hipError_t e = launch_elem_mat<0, BLOCK_E>(mat, a, u32, d32, out, partial, nullptr, 1, done, dot_out ? dot_elem : nullptr,
+                                               lds_bytes<0>(a), st);
     if (e != hipSuccess) return e;
-    return launch_shared<float, 3>(s, partial, out, nullptr, 1, done, st);
+    if (!dot_out) return launch_shared<float, 3>(s, partial, out, nullptr, 1, done, st);
+    hipLaunchKernelGGL((k_shared_gather<float, 3, true>), dim3(gather_grid(s)), dim3(256), 0, st, s, (const float*)partial, out,
+                       nullptr, 1, done, d32, dot_elem, a.nblk, dot_out);
+    return hipGetLastError();
 }
 
 hipError_t launch_shared_only(const SharedArgs& s, const float* partial, float* out, hipStream_t st)
@@ -572,7 +630,8 @@ hipError_t launch_shared_only(const SharedArgs& s, const float* partial, float* 
 hipError_t launch_force(int mat, const BlockArgs& a, const SharedArgs& s, const double* u64, double* out, double* partial,
                         const double* ext, int mask_fixed, hipStream_t st)
 {
-    hipError_t e = launch_elem_mat<1, BLOCK_E>(mat, a, u64, nullptr, out, partial, ext, mask_fixed, nullptr, lds_bytes<1>(a), st);
+    hipError_t e = launch_elem_mat<1, BLOCK_E>(mat, a, u64, nullptr, out, partial, ext, mask_fixed, nullptr, nullptr,
+                                               lds_bytes<1>(a), st);
     if (e != hipSuccess) return e;
     return launch_shared<double, 3>(s, partial, out, ext, mask_fixed, nullptr, st);
 }
@@ -580,7 +639,7 @@ hipError_t launch_force(int mat, const BlockArgs& a, const SharedArgs& s, const 
 hipError_t launch_blockdiag(int mat, const BlockArgs& a, const SharedArgs& s, const float* u32, float* out, float* partial,
                             hipStream_t st)
 {
-    hipError_t e = launch_elem_mat<2, BLOCK_E>(mat, a, u32, nullptr, out, partial, nullptr, 1, nullptr, lds_bytes<2>(a), st);
+    hipError_t e = launch_elem_mat<2, BLOCK_E>(mat, a, u32, nullptr, out, partial, nullptr, 1, nullptr, nullptr, lds_bytes<2>(a), st);
     if (e != hipSuccess) return e;
     return launch_shared<float, 9>(s, partial, out, nullptr, 1, nullptr, st);
 }
@@ -613,34 +672,36 @@ hipError_t set_elem_lds_limits(const BlockArgs& a)
     return hipSuccess;
 }
 
+int cg_grid(int64_t nn) { return grid_for(nn, 256, CG_GRID); }
+
 hipError_t launch_cg_init(int pc, int64_t n3, const float* b, float* x, float* r, float* z, float* p, const float* minv,
-                          const uint8_t* fixed, double* scal, double* partials, unsigned* counter, int* done, hipStream_t st)
+                          const uint8_t* fixed, double* scal, double* part, int* done, hipStream_t st)
 {
-    const int g = grid_for(n3 / 3, 256, RED_GRID);
-    if (pc) hipLaunchKernelGGL(k_cg_init<1>, dim3(g), dim3(256), 0, st, n3, b, x, r, z, p, minv, fixed, scal, partials, counter, done);
-    else hipLaunchKernelGGL(k_cg_init<0>, dim3(g), dim3(256), 0, st, n3, b, x, r, z, p, minv, fixed, scal, partials, counter, done);
+    const int g = cg_grid(n3 / 3);
+    if (pc) hipLaunchKernelGGL(k_cg_init<1>, dim3(g), dim3(256), 0, st, n3, b, x, r, z, p, minv, fixed, part);
+    else hipLaunchKernelGGL(k_cg_init<0>, dim3(g), dim3(256), 0, st, n3, b, x, r, z, p, minv, fixed, part);
+    hipLaunchKernelGGL(k_cg_init_finish, dim3(1), dim3(256), 0, st, part, g, scal, done);
     return hipGetLastError();
 }
 
 hipError_t launch_cg_iter_tail(int pc, int64_t n3, float* x, float* r, float* z, float* p, const float* q, const float* minv,
-                               double* scal, double* partials, unsigned* counter, int* done, int test_kind, double tol2,
-                               int guard, hipStream_t st)
+                               double* scal, const double* pq_part, int n_pq, double* part, int* done, int test_kind,
+                               double tol2, int guard, hipStream_t st)
 {
-    const int g = grid_for(n3 / 3, 256, RED_GRID);
-    hipLaunchKernelGGL(k_cg_alpha, dim3(g), dim3(256), 0, st, n3, p, q, scal, partials, counter, done, guard);
-    if (pc) hipLaunchKernelGGL(k_cg_update<1>, dim3(g), dim3(256), 0, st, n3, x, r, z, p, q, minv, scal, partials, counter, done, test_kind, tol2);
-    else hipLaunchKernelGGL(k_cg_update<0>, dim3(g), dim3(256), 0, st, n3, x, r, z, p, q, minv, scal, partials, counter, done, test_kind, tol2);
-    const int gp = grid_for(n3, 256, 4096);
-    if (pc) hipLaunchKernelGGL(k_cg_p<1>, dim3(gp), dim3(256), 0, st, n3, z, r, p, scal, done);
-    else hipLaunchKernelGGL(k_cg_p<0>, dim3(gp), dim3(256), 0, st, n3, z, r, p, scal, done);
+    const int g = cg_grid(n3 / 3);
+    if (pc) hipLaunchKernelGGL(k_cg_update<1>, dim3(g), dim3(256), 0, st, n3, x, r, z, p, q, minv, scal, pq_part, n_pq, part, done, guard);
+    else hipLaunchKernelGGL(k_cg_update<0>, dim3(g), dim3(256), 0, st, n3, x, r, z, p, q, minv, scal, pq_part, n_pq, part, done, guard);
+    if (pc) hipLaunchKernelGGL(k_cg_p<1>, dim3(g), dim3(256), 0, st, n3, z, r, p, scal, part, g, done, test_kind, tol2);
+    else hipLaunchKernelGGL(k_cg_p<0>, dim3(g), dim3(256), 0, st, n3, z, r, p, scal, part, g, done, test_kind, tol2);
     return hipGetLastError();
 }
 
-hipError_t launch_dots3(int64_t n3, const double* a, const double* b, const float* d, double* scal, int slot,
-                        double* partials, unsigned* counter, hipStream_t st)
+hipError_t launch_dots3(int64_t n3, const double* a, const double* b, const float* d, double* scal, int slot, double* part,
+                        hipStream_t st)
 {
     const int g = grid_for(n3 / 3, 256, RED_GRID);
-    hipLaunchKernelGGL(k_dots3, dim3(g), dim3(256), 0, st, n3, a, b, d, scal, slot, partials, counter);
+    hipLaunchKernelGGL(k_dots3, dim3(g), dim3(256), 0, st, n3, a, b, d, part);
+    hipLaunchKernelGGL(k_finish<3>, dim3(1), dim3(256), 0, st, part, g, scal, slot);
     return hipGetLastError();
 }
 

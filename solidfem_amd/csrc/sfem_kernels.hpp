@@ -8,10 +8,13 @@ namespace sfem {
 // elements per workgroup of the element-block kernels
 constexpr int BLOCK_E = 256;
 // cap on workgroups of the reduction kernels (their partials buffer size)
-constexpr int RED_GRID = 2048;   // partials buffer holds 3 * RED_GRID doubles
+constexpr int RED_GRID = 2048;   // reduction partials buffer holds 3 * RED_GRID doubles
+// workgroups of the CG vector kernels; each of them re-reduces the previous
+// kernel's partials, so this stays small
+constexpr int CG_GRID = 256;
 
 // FP64 scalars kept on the device during a CG solve
-enum Scal { S_RZ = 0, S_RZ0, S_RR, S_RR0, S_PQ, S_ALPHA, S_BETA, S_ITER, S_NT, S_NT2, S_NT3, S_COUNT = 16 };
+enum Scal { S_RZ = 0, S_RZ0, S_RR, S_RR0, S_PQ, S_ALPHA, S_BETA, S_ITER, S_NT, S_NT2, S_NT3, S_RZ_PREV, S_COUNT = 16 };
 
 // Element-block partition, all device pointers.
 struct BlockArgs {
@@ -21,6 +24,7 @@ struct BlockArgs {
     int nmax;              // max nodes of any block (LDS sizing)
     const int* bnode_ptr;  // nblk+1 offsets into bnodes
     const uint32_t* bnodes;// block node list: node id | FLAG_SHARED | FLAG_FIXED
+    const int* bslot;      // per block node row: its partial slot (shared rows only)
     const uint16_t* lptr;  // per block nb+1 offsets into its incidence list
     const uint16_t* lent;  // per block 4*BLOCK_E incidence entries (element<<2 | corner)
     const uint2* lidx;     // per element 4 block-local node ids (16 bit each)
@@ -34,12 +38,16 @@ struct BlockArgs {
 struct SharedArgs {
     int nshared;
     const uint32_t* node;  // node id | flags
-    const int* ptr;        // nshared+1
-    const int* idx;        // partial row index (bnode_ptr[b] + local id), block order
+    const int* ptr;        // nshared+1: node i owns partial slots [ptr[i], ptr[i+1]), block order
 };
 
+// q = K d; with dot_out, also p.q partials: dot_elem (nblk doubles) then
+// dot_out (gather_grid(s) doubles)
 hipError_t launch_hessvec(int mat, const BlockArgs& a, const SharedArgs& s, const float* u32, const float* d32, float* out,
-                          float* partial, const int* done, hipStream_t st);
+                          float* partial, const int* done, hipStream_t st, double* dot_elem = nullptr,
+                          double* dot_out = nullptr);
+int gather_grid(const SharedArgs& s);
+int cg_grid(int64_t nn);
 hipError_t launch_shared_only(const SharedArgs& s, const float* partial, float* out, hipStream_t st);
 hipError_t launch_force(int mat, const BlockArgs& a, const SharedArgs& s, const double* u64, double* out, double* partial,
                         const double* ext, int mask_fixed, hipStream_t st);
@@ -48,12 +56,12 @@ hipError_t launch_blockdiag(int mat, const BlockArgs& a, const SharedArgs& s, co
 size_t elem_lds_max(const BlockArgs& a);
 hipError_t set_elem_lds_limits(const BlockArgs& a);
 hipError_t launch_cg_init(int pc, int64_t n3, const float* b, float* x, float* r, float* z, float* p, const float* minv,
-                          const uint8_t* fixed, double* scal, double* partials, unsigned* counter, int* done, hipStream_t st);
+                          const uint8_t* fixed, double* scal, double* part, int* done, hipStream_t st);
 hipError_t launch_cg_iter_tail(int pc, int64_t n3, float* x, float* r, float* z, float* p, const float* q, const float* minv,
-                               double* scal, double* partials, unsigned* counter, int* done, int test_kind, double tol2,
-                               int guard, hipStream_t st);
-hipError_t launch_dots3(int64_t n3, const double* a, const double* b, const float* d, double* scal, int slot,
-                        double* partials, unsigned* counter, hipStream_t st);
+                               double* scal, const double* pq_part, int n_pq, double* part, int* done, int test_kind,
+                               double tol2, int guard, hipStream_t st);
+hipError_t launch_dots3(int64_t n3, const double* a, const double* b, const float* d, double* scal, int slot, double* part,
+                        hipStream_t st);
 hipError_t launch_axpy_state(int64_t n3, const double* u, const float* dx, double alpha, double* ut, float* u32, hipStream_t st);
 hipError_t launch_to_f32(int64_t n, const double* a, float* b, hipStream_t st);
 hipError_t launch_external(int64_t nn, const double* mass, const double* fdead, const double* g, double frac, const int* pptr,

@@ -1,0 +1,167 @@
+"""The learned block preconditioner on the GPU against its FP64 CPU reference
+(oracle/gnn_ref.py, itself pinned by finite differences): MFMA GEMM operand
+layout, input features, residual/solution pairs, loss and parameter gradient;
+then training on the device and what it gains over block Jacobi."""
+import numpy as np
+import pytest
+
+from oracle import gnn_ref as G
+from oracle import oracle as O
+from solidfem_amd import capi
+from solidfem_amd.mesh import BeamMesh, traction_nodal_forces, uniform01
+
+pytestmark = pytest.mark.gpu
+
+
+def rel2(a, b):
+    return np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(np.asarray(b))
+
+
+def beam(grid=(8, 3, 3), seed=0, per_elem=False):
+    m = BeamMesh(*grid)
+    fixed = m.fixed_mask()
+    kw = {}
+    if per_elem:
+        kw = dict(young_e=10 ** (5 + 2 * uniform01(seed, 1, m.ne)), poisson_e=0.30 + 0.18 * uniform01(seed, 2, m.ne))
+    s = capi.Solver(m.X, m.tets, fixed, capi.NEO_HOOKEAN, 1e6, 0.45, 1070.0, gravity=(0, 0, -9.8), **kw)
+    return m, fixed, s, kw
+
+
+def random_load(m, fixed, seed):
+    rng = np.random.default_rng(seed)
+    t = rng.standard_normal(3) * 1e4
+    f = traction_nodal_forces(m.X, m.tip_triangles(), lambda p: np.tile(t, (len(p), 1)))
+    f += rng.standard_normal(m.X.shape) * 0.1 * np.abs(f).max()
+    return f * (1 - fixed[:, None])
+
+
+@pytest.mark.parametrize("layer", [0, 1, 2, 9])
+def test_mfma_gemm_layout(layer):
+    net = capi.LearnedPreconditioner(seed=11)
+    w = net.params()
+    rng = np.random.default_rng(layer)
+    off = 0
+    for li, (K, NO) in enumerate(G.LAYERS):
+        if li == layer:
+            break
+        off += K * NO + NO
+    K, NO = G.LAYERS[layer]
+    W = rng.standard_normal((K, NO)).astype(np.float32)
+    b = rng.standard_normal(NO).astype(np.float32)
+    w[off:off + K * NO] = W.reshape(-1)
+    w[off + K * NO:off + K * NO + NO] = b
+    net.set_params(w)
+    X = rng.standard_normal((203, K)).astype(np.float32)   # not a multiple of the 16-row tiles
+    Y = net.gemm_check(layer, X)
+    ref = G.bf16(X) @ G.bf16(W) + b
+    assert rel2(Y, ref) < 1e-6
+
+
+def test_features_and_blocks_match_reference():
+    m, fixed, s, _ = beam()
+    net = capi.LearnedPreconditioner(seed=3)
+    feat, cb, minv = net.inputs(s)
+    p = O.OracleProblem(m.X, m.tets, fixed, O.NEO_HOOKEAN, 1e6, 0.45, 1070.0)
+    ptr, idx = G.node_graph(m.tets, m.nn)
+    f_ref, cb_ref = G.features(p.block_diag(m.X), fixed, ptr, idx)
+    assert np.abs(feat - f_ref).max() < 1e-5
+    assert rel2(cb, cb_ref) < 1e-5
+    # untrained: Minv = (1 + eps) * block-Jacobi inverse
+    J = np.linalg.inv(p.block_diag(m.X))
+    free = fixed == 0
+    assert rel2(minv.reshape(-1, 3, 3)[free], (1 + G.EPS) * J[free]) < 1e-5
+
+
+def test_pairs_satisfy_K_e_equals_r():
+    m, fixed, s, _ = beam()
+    load = random_load(m, fixed, 5)
+    n = capi.LearnedPreconditioner.collect(s, load, nsnap=4, snap_every=5, rtol=1e-7)
+    assert n == 4 and capi.LearnedPreconditioner.num_pairs(s) == 4
+    p = O.OracleProblem(m.X, m.tets, fixed, O.NEO_HOOKEAN, 1e6, 0.45, 1070.0)
+    r0, er0 = capi.LearnedPreconditioner.pair(s, 0)
+    assert rel2(r0, load) < 1e-6
+    e_ref, _ = p.cg(m.X, load, 20000, 1e-7)
+    assert abs(er0 - np.sum(load * e_ref)) / abs(er0) < 1e-4
+    for k in range(1, 4):   # later snapshots: r.K^-1 r in (0, r0.e0)
+        rk, erk = capi.LearnedPreconditioner.pair(s, k)
+        ek, _ = p.cg(m.X, rk.astype(np.float64), 20000, 1e-7)
+        assert abs(erk - np.sum(rk * ek)) / abs(erk) < 2e-3, k
+        assert 0 < erk < er0
+
+
+def test_loss_and_gradient_match_reference():
+    m, fixed, s, _ = beam()
+    net = capi.LearnedPreconditioner(seed=5)
+    # give the zero-initialised last layer small weights so every layer has a gradient
+    w = net.params()
+    rng = np.random.default_rng(9)
+    w[-(32 * 16 + 16):] = rng.standard_normal(32 * 16 + 16).astype(np.float32) * 0.05
+    net.set_params(w)
+    for seed in (1, 2):
+        capi.LearnedPreconditioner.collect(s, random_load(m, fixed, seed), nsnap=3, snap_every=4, rtol=1e-7)
+    npairs = capi.LearnedPreconditioner.num_pairs(s)
+    loss, grad = net.loss(s, 0, npairs, grad=True)
+    feat, cb, _ = net.inputs(s)
+    R = np.stack([capi.LearnedPreconditioner.pair(s, k)[0] for k in range(npairs)]).astype(np.float64)
+    er = np.array([capi.LearnedPreconditioner.pair(s, k)[1] for k in range(npairs)])
+    p = O.OracleProblem(m.X, m.tets, fixed, O.NEO_HOOKEAN, 1e6, 0.45, 1070.0)
+    ptr, idx = G.node_graph(m.tets, m.nn)
+    ref = G.GnnRef(net.params(), ptr, idx, use_bf16=True)
+    l_ref, g_ref = ref.loss(feat.astype(np.float64), cb.astype(np.float64), R, er, lambda z: p.matvec(m.X, z))
+    print(f"loss gpu {loss:.8f} ref {l_ref:.8f}; grad rel err {rel2(grad, g_ref):.3e}")
+    assert abs(loss - l_ref) < 1e-4 * abs(l_ref)
+    assert rel2(grad, g_ref) < 2e-2
+    cos = np.dot(grad, g_ref) / np.linalg.norm(grad) / np.linalg.norm(g_ref)
+    assert cos > 0.999
+    # every layer receives gradient
+    off = 0
+    for K, NO in G.LAYERS:
+        assert np.abs(grad[off:off + K * NO]).max() > 0
+        off += K * NO + NO
+
+
+def test_training_improves_on_block_jacobi_and_is_deterministic():
+    """Train on pairs from two random-material beams, evaluate on a third
+    (held out): the learned blocks must lower the one-step energy-error loss
+    below block Jacobi's, and two identical runs give identical weights."""
+    runs = []
+    for rep in range(2):
+        net = capi.LearnedPreconditioner(seed=1)
+        train = []
+        for k in range(2):
+            m, fixed, s, _ = beam((12, 4, 4), seed=20 + k, per_elem=True)
+            for j in range(3):
+                capi.LearnedPreconditioner.collect(s, random_load(m, fixed, 100 * k + j), nsnap=6, snap_every=8, rtol=1e-6)
+            train.append(s)
+        for epoch in range(6):
+            for s in train:
+                net.train(s, 10, batch=8, lr=2e-3, seed=epoch)
+        runs.append(net.params())
+        if rep == 0:
+            m, fixed, s, _ = beam((12, 4, 4), seed=99, per_elem=True)
+            for j in range(2):
+                capi.LearnedPreconditioner.collect(s, random_load(m, fixed, 999 + j), nsnap=6, snap_every=8, rtol=1e-6)
+            ll, lj = net.evaluate(s)
+            print(f"held-out loss: learned {ll:.4f} jacobi {lj:.4f}")
+            assert ll < lj
+    assert np.array_equal(runs[0], runs[1])
+
+
+def test_learned_pcg_newton_solve_converges_with_judge():
+    m, fixed, s, _ = beam((12, 4, 4), seed=7, per_elem=True)
+    net = capi.LearnedPreconditioner(seed=2)
+    for j in range(3):
+        capi.LearnedPreconditioner.collect(s, random_load(m, fixed, j), nsnap=6, snap_every=8, rtol=1e-6)
+    net.train(s, 40, batch=8, lr=2e-3, seed=0)
+    net.attach(s)
+    s.set_load_fraction(0.5)
+    st_l = s.solve(tol=1e-8, precond=capi.PC_LEARNED)
+    xl = s.positions()
+    s.reset()
+    s.set_load_fraction(0.5)
+    st_j = s.solve(tol=1e-8, precond=capi.PC_JACOBI)
+    xj = s.positions()
+    print(f"learned: newton {st_l['newton_iters']} cg {st_l['cg_iters']} fallbacks {st_l['fallbacks']}; "
+          f"jacobi: newton {st_j['newton_iters']} cg {st_j['cg_iters']}")
+    assert st_l["converged"] == 1 and st_j["converged"] == 1
+    assert rel2(xl - m.X, xj - m.X) < 1e-6
