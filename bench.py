@@ -67,6 +67,34 @@ def cpu_baseline(m, fixed, fdead, x, budget_s=10.0):
                       f"(oracle/sfem_oracle.c orc_matvec, {cores} OpenMP threads, {dt:.1f} s)"}
 
 
+def hessvec_only(s, m, args):
+    """Kernel-level timing at a smooth deformed state (bending-like field), no
+    solve: the run that hardware-counter passes profile."""
+    x = m.X.copy()
+    xi = m.X[:, 0]
+    x[:, 2] += -0.3 * xi ** 2
+    x[:, 0] += 0.01 * np.sin(3 * m.X[:, 1]) * xi
+    s.set_positions(x)
+    ms_elem, ms_gather = s.profile_hessvec(args.hessvec_only)
+    info = s.info()
+    bytes_alg = hessvec_bytes(info["ne"], info["nn"])
+    out = {"kernel": "k_elem_block<0,NEO_HOOKEAN,256>", "reps": args.hessvec_only, "ms_elem": ms_elem,
+           "ms_gather": ms_gather, "bytes_alg": bytes_alg, "gbs_alg": bytes_alg / (ms_elem * 1e-3) / 1e9}
+    if args.calib_mib:
+        import ctypes as C
+        hip = C.CDLL("libamdhip64.so")
+        n = args.calib_mib << 20
+        a, b = C.c_void_p(), C.c_void_p()
+        assert hip.hipMalloc(C.byref(a), C.c_size_t(n)) == 0 and hip.hipMalloc(C.byref(b), C.c_size_t(n)) == 0
+        assert hip.hipMemset(a, 1, C.c_size_t(n)) == 0
+        assert hip.hipMemcpy(b, a, C.c_size_t(n), 3) == 0        # device to device: one copy kernel
+        assert hip.hipDeviceSynchronize() == 0
+        hip.hipFree(a)
+        hip.hipFree(b)
+        out["calib_copy_bytes"] = n
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -77,6 +105,10 @@ def main():
     ap.add_argument("--grid", default="128x32x32")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--profile-reps", type=int, default=200)
+    ap.add_argument("--hessvec-only", type=int, default=0, metavar="REPS",
+                    help="only time REPS Hessian-vector launches at a deformed state (for counter runs)")
+    ap.add_argument("--calib-mib", type=int, default=0,
+                    help="with --hessvec-only: also run one device-to-device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU sharding is design-only.",
@@ -86,6 +118,9 @@ def main():
     m, fixed, fdead = workload(grid)
     s = capi.Solver(m.X, m.tets, fixed, capi.NEO_HOOKEAN, 1e6, 0.45, 1070.0, gravity=(0.0, 0.0, -9.8))
     s.set_external_forces(fdead)
+    if args.hessvec_only:
+        hessvec_only(s, m, args)
+        return
     pc = {"jacobi": capi.PC_JACOBI, "none": capi.PC_NONE}[args.precond]
     params = capi.solve_params(tol=1e-8, precond=pc, cg_rtol=0.1, sim_type=capi.ST_QUASI_STATIC,
                                force_step=1.0 / args.load_steps, inner=100000, check_every=32)
