@@ -121,7 +121,7 @@ void build_partition(sfem_ctx* c, const sfem_mesh* m, const std::vector<double>&
         for (int64_t s = e0; s < e1; s++) {
             const uint32_t* li = &lidx[2 * s];
             const int l[4] = {(int)(li[0] & 0xffff), (int)(li[0] >> 16), (int)(li[1] & 0xffff), (int)(li[1] >> 16)};
-            for (int v = 0; v < 4; v++) lent[(size_t)b * 4 * BE + fill[l[v]]++] = (uint16_t)(((s - e0) << 2) | v);
+            for (int v = 0; v < 4; v++) lent[(size_t)b * 4 * BE + fill[l[v]]++] = (uint16_t)(v * BE + (s - e0));
         }
     }
     // shared nodes: flag, and list their partial rows in block order

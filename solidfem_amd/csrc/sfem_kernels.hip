@@ -29,10 +29,6 @@ constexpr uint32_t NODE_MASK = 0x3fffffffu;
 constexpr uint32_t FLAG_SHARED = 0x80000000u;
 constexpr uint32_t FLAG_FIXED = 0x40000000u;
 
-// --------------------------------------------------------------------- LDS
-// slot of (component q, element el) in the corner-output array
-__device__ __forceinline__ int sfo(int q, int el, int BE) { return q * BE + el; }
-
 // ------------------------------------------------------------ reductions
 template <int NV>
 __device__ __forceinline__ void block_sum(double (&v)[NV], double* lds /* >= NV*16 */)
@@ -109,12 +105,29 @@ template <> struct KindTraits<0> { typedef float T; static constexpr int W = 3; 
 template <> struct KindTraits<1> { typedef double T; static constexpr int W = 3; static constexpr bool HAS_D = false; };
 template <> struct KindTraits<2> { typedef float T; static constexpr int W = 9; static constexpr bool HAS_D = false; };
 
-// LDS: corner outputs (4 W BE of T), staged u (and d) rows (3 nmax each), the
-// block's incidence entries (4 BE uint16) and offsets (nmax + 1 uint16)
+// LDS layout (vec4 = 4 x T, one ds_read/ds_write_b128 per row):
+//   corner outputs: CW vec4 per (element, corner), component-major
+//       sf4[(corner * CW + j) * BE + element]   (CW = 1 for W = 3, 3 for W = 9)
+//   staged node rows: su4[nmax] (and sd4[nmax]) = (x, y, z, 0)
+//   incidence entries s_ent[4 BE] = corner * BE + element, offsets s_ptr[nmax+1]
+template <typename T> struct Vec4;
+template <> struct Vec4<float> { typedef float4 V; };
+template <> struct Vec4<double> { typedef double4 V; };
+
 template <int KIND>
-__host__ __device__ constexpr size_t lds_corner_bytes(int BE)
+__host__ __device__ constexpr int corner_vec4(int W_)
 {
-    return sizeof(typename KindTraits<KIND>::T) * 4 * KindTraits<KIND>::W * BE;
+    return (W_ + 3) / 4;
+}
+
+template <int KIND>
+size_t lds_layout_bytes(int BE, int nmax)
+{
+    typedef typename KindTraits<KIND>::T T;
+    constexpr int W = KindTraits<KIND>::W;
+    const size_t v4 = 4 * sizeof(T);
+    const size_t rows = (size_t)nmax * (KindTraits<KIND>::HAS_D ? 2 : 1);
+    return v4 * ((size_t)4 * corner_vec4<KIND>(W) * BE + rows) + sizeof(uint16_t) * (4 * (size_t)BE + nmax + 1) + 16;
 }
 
 template <int KIND, int MAT, int BE>
@@ -126,15 +139,17 @@ __global__ __launch_bounds__(BE) void k_elem_block(BlockArgs a, const typename K
                                                     const int* __restrict__ done, double* __restrict__ dot_part)
 {
     typedef typename KindTraits<KIND>::T T;
+    typedef typename Vec4<T>::V V4;
     constexpr int W = KindTraits<KIND>::W;
+    constexpr int CW = corner_vec4<KIND>(W);
     constexpr bool HAS_D = KindTraits<KIND>::HAS_D;
     if (done && *done) return;
-    extern __shared__ __align__(16) unsigned char smem_raw[];
-    T* sf = reinterpret_cast<T*>(smem_raw);          // 4*W*BE corner outputs
-    T* su = sf + 4 * W * BE;                          // 3*nmax
-    T* sd = su + 3 * a.nmax;                          // 3*nmax (HAS_D)
-    uint16_t* s_ent = reinterpret_cast<uint16_t*>(sd + (HAS_D ? 3 * a.nmax : 0));   // 4*BE
-    uint16_t* s_ptr = s_ent + 4 * BE;                                                // nmax+1
+    extern __shared__ __align__(32) unsigned char smem_raw[];
+    V4* sf4 = reinterpret_cast<V4*>(smem_raw);        // 4*CW*BE corner rows
+    V4* su4 = sf4 + 4 * CW * BE;                      // nmax
+    V4* sd4 = su4 + a.nmax;                           // nmax (HAS_D)
+    uint16_t* s_ent = reinterpret_cast<uint16_t*>(sd4 + (HAS_D ? a.nmax : 0));   // 4*BE
+    uint16_t* s_ptr = s_ent + 4 * BE;                                             // nmax+1
 
     const int b = blockIdx.x;
     const int tid = threadIdx.x;
@@ -167,10 +182,14 @@ __global__ __launch_bounds__(BE) void k_elem_block(BlockArgs a, const typename K
     for (int j = tid; j < nb; j += BE) {
         const uint32_t node = a.bnodes[n0 + j] & NODE_MASK;
         const T* up = u + 3 * (size_t)node;
-        su[3 * j] = up[0]; su[3 * j + 1] = up[1]; su[3 * j + 2] = up[2];
+        V4 uv;
+        uv.x = up[0]; uv.y = up[1]; uv.z = up[2]; uv.w = T(0);
+        su4[j] = uv;
         if (HAS_D) {
             const T* dp = d + 3 * (size_t)node;
-            sd[3 * j] = dp[0]; sd[3 * j + 1] = dp[1]; sd[3 * j + 2] = dp[2];
+            V4 dv;
+            dv.x = dp[0]; dv.y = dp[1]; dv.z = dp[2]; dv.w = T(0);
+            sd4[j] = dv;
         }
     }
     {
@@ -186,35 +205,38 @@ __global__ __launch_bounds__(BE) void k_elem_block(BlockArgs a, const typename K
         const int l[4] = {(int)(li.x & 0xffffu), (int)(li.x >> 16), (int)(li.y & 0xffffu), (int)(li.y >> 16)};
         T uu[4][3];
 #pragma unroll
-        for (int k = 0; k < 4; k++)
-#pragma unroll
-            for (int c = 0; c < 3; c++) uu[k][c] = su[3 * l[k] + c];
+        for (int k = 0; k < 4; k++) {
+            const V4 v = su4[l[k]];
+            uu[k][0] = v.x; uu[k][1] = v.y; uu[k][2] = v.z;
+        }
         T F[9];
         defgrad(g, uu, F);
-        if (KIND == 0) {
-            T dd[4][3];
+        if (KIND == 0 || KIND == 1) {
+            T o[4][3];
+            if (KIND == 0) {
+                T dd[4][3];
 #pragma unroll
-            for (int k = 0; k < 4; k++)
+                for (int k = 0; k < 4; k++) {
+                    const V4 v = sd4[l[k]];
+                    dd[k][0] = v.x; dd[k][1] = v.y; dd[k][2] = v.z;
+                }
+                T dF[9], dP[9];
+                grad4(g, dd, dF);
+                Tangent<T, MAT> tg;
+                tg.init(F, cmu, clam);
+                tg.apply(dF, dP);
+                corners(dP, g, T(1), o);
+            } else {
+                T P[9];
+                stress<T, MAT>(F, cmu, clam, P);
+                corners(P, g, T(-1), o);
+            }
 #pragma unroll
-                for (int c = 0; c < 3; c++) dd[k][c] = sd[3 * l[k] + c];
-            T dF[9], dP[9], o[4][3];
-            grad4(g, dd, dF);
-            Tangent<T, MAT> tg;
-            tg.init(F, cmu, clam);
-            tg.apply(dF, dP);
-            corners(dP, g, T(1), o);
-#pragma unroll
-            for (int k = 0; k < 4; k++)
-#pragma unroll
-                for (int c = 0; c < 3; c++) sf[sfo(3 * k + c, tid, BE)] = o[k][c];
-        } else if (KIND == 1) {
-            T P[9], o[4][3];
-            stress<T, MAT>(F, cmu, clam, P);
-            corners(P, g, T(-1), o);
-#pragma unroll
-            for (int k = 0; k < 4; k++)
-#pragma unroll
-                for (int c = 0; c < 3; c++) sf[sfo(3 * k + c, tid, BE)] = o[k][c];
+            for (int k = 0; k < 4; k++) {
+                V4 v;
+                v.x = o[k][0]; v.y = o[k][1]; v.z = o[k][2]; v.w = T(0);
+                sf4[k * BE + tid] = v;
+            }
         } else {
             // K_aa(:,k) = V dP(e_k (x) g_a) g_a,  g_a = gradient of shape function a
             Tangent<T, MAT> tg;
@@ -227,6 +249,7 @@ __global__ __launch_bounds__(BE) void k_elem_block(BlockArgs a, const typename K
             }
 #pragma unroll
             for (int aa = 0; aa < 4; aa++) {
+                T blk[12];
 #pragma unroll
                 for (int k = 0; k < 3; k++) {
                     T dF[9], dP[9];
@@ -236,8 +259,14 @@ __global__ __launch_bounds__(BE) void k_elem_block(BlockArgs a, const typename K
                     tg.apply(dF, dP);
 #pragma unroll
                     for (int c = 0; c < 3; c++)
-                        sf[sfo(aa * 9 + 3 * c + k, tid, BE)] =
-                            dP[3 * c] * ga[aa][0] + dP[3 * c + 1] * ga[aa][1] + dP[3 * c + 2] * ga[aa][2];
+                        blk[3 * c + k] = dP[3 * c] * ga[aa][0] + dP[3 * c + 1] * ga[aa][1] + dP[3 * c + 2] * ga[aa][2];
+                }
+                blk[9] = blk[10] = blk[11] = T(0);
+#pragma unroll
+                for (int jv = 0; jv < CW; jv++) {
+                    V4 v;
+                    v.x = blk[4 * jv]; v.y = blk[4 * jv + 1]; v.z = blk[4 * jv + 2]; v.w = blk[4 * jv + 3];
+                    sf4[(aa * CW + jv) * BE + tid] = v;
                 }
             }
         }
@@ -247,14 +276,17 @@ __global__ __launch_bounds__(BE) void k_elem_block(BlockArgs a, const typename K
     double dot = 0.0;
     for (int j = tid; j < nb; j += BE) {
         const int s0 = s_ptr[j], s1 = s_ptr[j + 1];
-        T acc[W];
+        T acc[4 * CW];
 #pragma unroll
-        for (int q = 0; q < W; q++) acc[q] = T(0);
+        for (int q = 0; q < 4 * CW; q++) acc[q] = T(0);
         for (int k = s0; k < s1; k++) {
-            const int ent = s_ent[k];
-            const int el = ent >> 2, cn = ent & 3;
+            const int ent = s_ent[k];             // corner * BE + element
+            const int base = CW == 1 ? ent : ent + (ent / BE) * (CW - 1) * BE;
 #pragma unroll
-            for (int q = 0; q < W; q++) acc[q] += sf[sfo(cn * W + q, el, BE)];
+            for (int jv = 0; jv < CW; jv++) {
+                const V4 v = sf4[base + jv * BE];
+                acc[4 * jv] += v.x; acc[4 * jv + 1] += v.y; acc[4 * jv + 2] += v.z; acc[4 * jv + 3] += v.w;
+            }
         }
         const uint32_t fl = a.bnodes[n0 + j];
         const uint32_t node = fl & NODE_MASK;
@@ -270,7 +302,10 @@ __global__ __launch_bounds__(BE) void k_elem_block(BlockArgs a, const typename K
             } else {
 #pragma unroll
                 for (int q = 0; q < W; q++) op[q] = acc[q] + (ext ? (T)ext[3 * (size_t)node + q % 3] : T(0));
-                if (KIND == 0) dot += (double)sd[3 * j] * acc[0] + (double)sd[3 * j + 1] * acc[1] + (double)sd[3 * j + 2] * acc[2];
+                if (KIND == 0) {
+                    const V4 pv = sd4[j];
+                    dot += (double)pv.x * acc[0] + (double)pv.y * acc[1] + (double)pv.z * acc[2];
+                }
             }
         }
     }
@@ -375,7 +410,9 @@ __global__ __launch_bounds__(256) void k_cg_init_finish(const double* __restrict
     }
 }
 
-// alpha = rz / p.q (every workgroup); x += a p; r -= a q; z = M r; partials
+// alpha = rz / p.q (every workgroup); x += a p; r -= a q; z = M r; partials.
+// Each thread first loads its first node's operands, then reduces the p.q
+// partials, so the two latencies overlap.
 template <int PC>
 __global__ __launch_bounds__(256) void k_cg_update(int64_t n3, float* __restrict__ x, float* __restrict__ r,
                                                    float* __restrict__ z, const float* __restrict__ p,
@@ -384,6 +421,21 @@ __global__ __launch_bounds__(256) void k_cg_update(int64_t n3, float* __restrict
                                                    double* __restrict__ part, int* done, int guard)
 {
     if (*done) return;
+    const int64_t nn = n3 / 3;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    float pv[3], qv[3], rv[3], xv[3], M[9];
+    auto load = [&](int64_t n) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            pv[c] = p[3 * n + c]; qv[c] = q[3 * n + c]; rv[c] = r[3 * n + c]; xv[c] = x[3 * n + c];
+        }
+        if (PC) {
+#pragma unroll
+            for (int k = 0; k < 9; k++) M[k] = minv[9 * n + k];
+        }
+    };
+    if (i < nn) load(i);
     double pq[1];
     sum_partials<1>(pq_part, n_pq, n_pq, pq);
     const double rz = scal[S_RZ];
@@ -398,18 +450,17 @@ __global__ __launch_bounds__(256) void k_cg_update(int64_t n3, float* __restrict
         scal[S_RZ_PREV] = rz;
     }
     const float alpha = (float)alpha_d;
-    const int64_t nn = n3 / 3;
     double v[2] = {0.0, 0.0};
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nn; i += (int64_t)gridDim.x * blockDim.x) {
-        float rv[3], zv[3];
+    for (; i < nn; i += stride) {
+        float zv[3];
 #pragma unroll
         for (int c = 0; c < 3; c++) {
-            x[3 * i + c] += alpha * p[3 * i + c];
-            rv[c] = r[3 * i + c] - alpha * q[3 * i + c];
+            xv[c] += alpha * pv[c];
+            rv[c] -= alpha * qv[c];
+            x[3 * i + c] = xv[c];
             r[3 * i + c] = rv[c];
         }
         if (PC) {
-            const float* M = minv + 9 * i;
 #pragma unroll
             for (int c = 0; c < 3; c++) zv[c] = M[3 * c] * rv[0] + M[3 * c + 1] * rv[1] + M[3 * c + 2] * rv[2];
 #pragma unroll
@@ -420,11 +471,12 @@ __global__ __launch_bounds__(256) void k_cg_update(int64_t n3, float* __restrict
         }
         v[0] += (double)rv[0] * zv[0] + (double)rv[1] * zv[1] + (double)rv[2] * zv[2];
         v[1] += (double)rv[0] * rv[0] + (double)rv[1] * rv[1] + (double)rv[2] * rv[2];
+        if (i + stride < nn) load(i + stride);
     }
     block_partials<2>(v, part);
 }
 
-// beta = rz' / rz (every workgroup); test; p = z + beta p
+// beta = rz' / rz (every workgroup); test; p = z + beta p, one node per step
 // test 0: the reference criterion |r.z| <= eps^2 |r0.z0| (IterativeSolvers.h:180-185)
 // test 1: true residual ||r||^2 <= tol^2 ||r0||^2
 template <int PC>
@@ -433,6 +485,15 @@ __global__ __launch_bounds__(256) void k_cg_p(int64_t n3, const float* __restric
                                               int* done, int test_kind, double tol2)
 {
     if (*done) return;
+    const int64_t nn = n3 / 3;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    const float* zz = PC ? z : r;
+    float zv[3], pv[3];
+    if (i < nn) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) { zv[c] = zz[3 * i + c]; pv[c] = p[3 * i + c]; }
+    }
     double t[2];
     sum_partials<2>(part, n_part, n_part, t);
     const double rz1 = t[0], rr = t[1];
@@ -449,9 +510,14 @@ __global__ __launch_bounds__(256) void k_cg_p(int64_t n3, const float* __restric
     }
     if (conv || bad) return;
     const float fb = (float)beta;
-    const float* zz = PC ? z : r;
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n3; i += (int64_t)gridDim.x * blockDim.x)
-        p[i] = zz[i] + fb * p[i];
+    for (; i < nn; i += stride) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) p[3 * i + c] = zv[c] + fb * pv[c];
+        if (i + stride < nn) {
+#pragma unroll
+            for (int c = 0; c < 3; c++) { zv[c] = zz[3 * (i + stride) + c]; pv[c] = p[3 * (i + stride) + c]; }
+        }
+    }
 }
 
 // ---------------------------------------------------------------- FP64 vectors
@@ -594,8 +660,7 @@ template <int KIND>
 static size_t lds_bytes(const BlockArgs& a)
 {
     typedef typename KindTraits<KIND>::T T;
-    const size_t rows = (size_t)3 * a.nmax * (KindTraits<KIND>::HAS_D ? 2 : 1);
-    return lds_corner_bytes<KIND>(BLOCK_E) + sizeof(T) * rows + sizeof(uint16_t) * (4 * BLOCK_E + a.nmax + 1) + 16;
+    return lds_layout_bytes<KIND>(BLOCK_E, a.nmax);
 }
 
 template <typename T, int W>

@@ -11,7 +11,7 @@ constexpr int BLOCK_E = 256;
 constexpr int RED_GRID = 2048;   // reduction partials buffer holds 3 * RED_GRID doubles
 // workgroups of the CG vector kernels; each of them re-reduces the previous
 // kernel's partials, so this stays small
-constexpr int CG_GRID = 256;
+constexpr int CG_GRID = 512;
 
 // FP64 scalars kept on the device during a CG solve
 enum Scal { S_RZ = 0, S_RZ0, S_RR, S_RR0, S_PQ, S_ALPHA, S_BETA, S_ITER, S_NT, S_NT2, S_NT3, S_RZ_PREV, S_COUNT = 16 };
@@ -26,7 +26,7 @@ struct BlockArgs {
     const uint32_t* bnodes;// block node list: node id | FLAG_SHARED | FLAG_FIXED
     const int* bslot;      // per block node row: its partial slot (shared rows only)
     const uint16_t* lptr;  // per block nb+1 offsets into its incidence list
-    const uint16_t* lent;  // per block 4*BLOCK_E incidence entries (element<<2 | corner)
+    const uint16_t* lent;  // per block 4*BLOCK_E incidence entries (corner * BLOCK_E + element)
     const uint2* lidx;     // per element 4 block-local node ids (16 bit each)
     const float* dminv32;  // 9 x ne_pad, Dm^-1 row-major, FP32
     const float* coef32;   // 2 x ne_pad: V mu, V lambda, FP32
