@@ -161,6 +161,10 @@ void build_partition(sfem_ctx* c, const sfem_mesh* m, const std::vector<double>&
     c->d_sh_node.upload(sh_node);
     c->d_sh_ptr.upload(sh_ptr);
     c->d_bslot.upload(bslot);
+    // per node: shared index, -1 for a free node of one block, -2 for fixed
+    std::vector<int> node_info(nn);
+    for (int64_t i = 0; i < nn; i++) node_info[i] = m->fixed[i] ? -2 : sh_of[i];
+    c->d_node_info.upload(node_info);
     c->nrows_partial = (int64_t)bnodes.size();
     c->d_partial.alloc((size_t)c->nrows_partial * 9 * sizeof(float) + (size_t)c->nrows_partial * 3 * sizeof(double));
 
@@ -242,8 +246,6 @@ CgResult pcg(sfem_ctx* c, const float* minv, int pc, int maxit, int test_kind, d
     double* scal = c->d_scal.as<double>();
     double* parts = c->d_partials.as<double>();          // CG update partials (2 x CG_GRID)
     double* pq_elem = parts + 2 * CG_GRID;               // p.q per element block (nblk)
-    double* pq_part = pq_elem + c->nblk;                 // p.q per gather workgroup
-    const int n_pq = gather_grid(c->shr);
     int* done = c->d_done.as<int>();
     HIPCK(launch_cg_init(pc, n3, c->d_b32.as<float>(), c->d_x32.as<float>(), c->d_r32.as<float>(), c->d_z32.as<float>(),
                          c->d_p32.as<float>(), minv, c->d_fixed.as<uint8_t>(), scal, parts, done, c->stream));
@@ -254,10 +256,11 @@ CgResult pcg(sfem_ctx* c, const float* minv, int pc, int maxit, int test_kind, d
     while (launched < maxit) {
         const int k = std::min(check_every, maxit - launched);
         for (int i = 0; i < k; i++) {
-            HIPCK(launch_hessvec(c->material, c->blk, c->shr, c->d_u32.as<float>(), c->d_p32.as<float>(),
-                                 c->d_q32.as<float>(), c->d_partial.as<float>(), done, c->stream, pq_elem, pq_part));
+            HIPCK(launch_hessvec_cg(c->material, c->blk, c->d_u32.as<float>(), c->d_p32.as<float>(), c->d_q32.as<float>(),
+                                    c->d_partial.as<float>(), done, pq_elem, c->stream));
             HIPCK(launch_cg_iter_tail(pc, n3, c->d_x32.as<float>(), c->d_r32.as<float>(), c->d_z32.as<float>(),
-                                      c->d_p32.as<float>(), c->d_q32.as<float>(), minv, scal, pq_part, n_pq, parts, done,
+                                      c->d_p32.as<float>(), c->d_q32.as<float>(), minv, c->d_node_info.as<int>(),
+                                      c->shr.ptr, c->d_partial.as<float>(), scal, pq_elem, c->nblk, parts, done,
                                       test_kind, tol2, guard, c->stream));
         }
         launched += k;
@@ -478,7 +481,7 @@ int sfem_create(const sfem_mesh* m, const sfem_material* mat, sfem_ctx** out)
         zero<float>(c->d_minv, 9 * (size_t)nn, s);
         zero<float>(c->d_bd32, 9 * (size_t)nn, s);
         zero<double>(c->d_scal, S_COUNT, s);
-        zero<double>(c->d_partials, std::max<size_t>(3 * (size_t)RED_GRID, 2 * (size_t)CG_GRID + c->nblk + gather_grid(c->shr)), s);
+        zero<double>(c->d_partials, std::max<size_t>(3 * (size_t)RED_GRID, 2 * (size_t)CG_GRID + c->nblk), s);
         zero<int>(c->d_done, 4, s);
         HIPCK(hipStreamSynchronize(s));
         return SFEM_OK;

@@ -302,11 +302,13 @@ __global__ __launch_bounds__(BE) void k_elem_block(BlockArgs a, const typename K
             } else {
 #pragma unroll
                 for (int q = 0; q < W; q++) op[q] = acc[q] + (ext ? (T)ext[3 * (size_t)node + q % 3] : T(0));
-                if (KIND == 0) {
-                    const V4 pv = sd4[j];
-                    dot += (double)pv.x * acc[0] + (double)pv.y * acc[1] + (double)pv.z * acc[2];
-                }
             }
+        }
+        // d.(K d) is linear in the node sums, so every row -- shared rows with
+        // their partial sums too -- adds its share here; fixed rows have d = 0
+        if (KIND == 0 && dot_part) {
+            const V4 pv = sd4[j];
+            dot += (double)pv.x * acc[0] + (double)pv.y * acc[1] + (double)pv.z * acc[2];
         }
     }
     if (KIND == 0 && dot_part) {
@@ -413,10 +415,15 @@ __global__ __launch_bounds__(256) void k_cg_init_finish(const double* __restrict
 // alpha = rz / p.q (every workgroup); x += a p; r -= a q; z = M r; partials.
 // Each thread first loads its first node's operands, then reduces the p.q
 // partials, so the two latencies overlap.
+// q of a node: written by the element kernel for nodes of one block, the sum
+// of the node's contiguous partial slots for shared nodes, 0 for fixed nodes
+// (node_info: >= 0 shared index, -1 single-block free node, -2 fixed)
 template <int PC>
 __global__ __launch_bounds__(256) void k_cg_update(int64_t n3, float* __restrict__ x, float* __restrict__ r,
                                                    float* __restrict__ z, const float* __restrict__ p,
                                                    const float* __restrict__ q, const float* __restrict__ minv,
+                                                   const int* __restrict__ node_info, const int* __restrict__ sh_ptr,
+                                                   const float* __restrict__ partial,
                                                    double* scal, const double* __restrict__ pq_part, int n_pq,
                                                    double* __restrict__ part, int* done, int guard)
 {
@@ -426,10 +433,17 @@ __global__ __launch_bounds__(256) void k_cg_update(int64_t n3, float* __restrict
     int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     float pv[3], qv[3], rv[3], xv[3], M[9];
     auto load = [&](int64_t n) {
+        const int info = node_info[n];
 #pragma unroll
         for (int c = 0; c < 3; c++) {
-            pv[c] = p[3 * n + c]; qv[c] = q[3 * n + c]; rv[c] = r[3 * n + c]; xv[c] = x[3 * n + c];
+            pv[c] = p[3 * n + c]; rv[c] = r[3 * n + c]; xv[c] = x[3 * n + c];
+            qv[c] = info == -1 ? q[3 * n + c] : 0.f;
         }
+        if (info >= 0)
+            for (int k = sh_ptr[info]; k < sh_ptr[info + 1]; k++) {
+#pragma unroll
+                for (int c = 0; c < 3; c++) qv[c] += partial[3 * (size_t)k + c];
+            }
         if (PC) {
 #pragma unroll
             for (int k = 0; k < 9; k++) M[k] = minv[9 * n + k];
@@ -749,13 +763,20 @@ hipError_t launch_cg_init(int pc, int64_t n3, const float* b, float* x, float* r
     return hipGetLastError();
 }
 
+hipError_t launch_hessvec_cg(int mat, const BlockArgs& a, const float* u32, const float* d32, float* out, float* partial,
+                             const int* done, double* dot_elem, hipStream_t st)
+{
+    return launch_elem_mat<0, BLOCK_E>(mat, a, u32, d32, out, partial, nullptr, 1, done, dot_elem, lds_bytes<0>(a), st);
+}
+
 hipError_t launch_cg_iter_tail(int pc, int64_t n3, float* x, float* r, float* z, float* p, const float* q, const float* minv,
+                               const int* node_info, const int* sh_ptr, const float* partial,
                                double* scal, const double* pq_part, int n_pq, double* part, int* done, int test_kind,
                                double tol2, int guard, hipStream_t st)
 {
     const int g = cg_grid(n3 / 3);
-    if (pc) hipLaunchKernelGGL(k_cg_update<1>, dim3(g), dim3(256), 0, st, n3, x, r, z, p, q, minv, scal, pq_part, n_pq, part, done, guard);
-    else hipLaunchKernelGGL(k_cg_update<0>, dim3(g), dim3(256), 0, st, n3, x, r, z, p, q, minv, scal, pq_part, n_pq, part, done, guard);
+    if (pc) hipLaunchKernelGGL(k_cg_update<1>, dim3(g), dim3(256), 0, st, n3, x, r, z, p, q, minv, node_info, sh_ptr, partial, scal, pq_part, n_pq, part, done, guard);
+    else hipLaunchKernelGGL(k_cg_update<0>, dim3(g), dim3(256), 0, st, n3, x, r, z, p, q, minv, node_info, sh_ptr, partial, scal, pq_part, n_pq, part, done, guard);
     if (pc) hipLaunchKernelGGL(k_cg_p<1>, dim3(g), dim3(256), 0, st, n3, z, r, p, scal, part, g, done, test_kind, tol2);
     else hipLaunchKernelGGL(k_cg_p<0>, dim3(g), dim3(256), 0, st, n3, z, r, p, scal, part, g, done, test_kind, tol2);
     return hipGetLastError();

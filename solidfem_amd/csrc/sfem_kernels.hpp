@@ -57,7 +57,13 @@ size_t elem_lds_max(const BlockArgs& a);
 hipError_t set_elem_lds_limits(const BlockArgs& a);
 hipError_t launch_cg_init(int pc, int64_t n3, const float* b, float* x, float* r, float* z, float* p, const float* minv,
                           const uint8_t* fixed, double* scal, double* part, int* done, hipStream_t st);
+// CG operator apply: element blocks only (single-block rows -> out, shared
+// rows -> partial slots) plus one d.(K d) partial per element block
+hipError_t launch_hessvec_cg(int mat, const BlockArgs& a, const float* u32, const float* d32, float* out, float* partial,
+                             const int* done, double* dot_elem, hipStream_t st);
+// alpha, x, r, z (summing shared nodes' partial slots), then beta and p
 hipError_t launch_cg_iter_tail(int pc, int64_t n3, float* x, float* r, float* z, float* p, const float* q, const float* minv,
+                               const int* node_info, const int* sh_ptr, const float* partial,
                                double* scal, const double* pq_part, int n_pq, double* part, int* done, int test_kind,
                                double tol2, int guard, hipStream_t st);
 hipError_t launch_dots3(int64_t n3, const double* a, const double* b, const float* d, double* scal, int slot, double* part,
