@@ -273,42 +273,67 @@ __global__ __launch_bounds__(BE) void k_elem_block(BlockArgs a, const typename K
     }
     __syncthreads();
 
+    // node phase: two adjacent lanes per node split its incidence list
+    // (even / odd entries, two loads in flight each) and combine by a lane swap
     double dot = 0.0;
-    for (int j = tid; j < nb; j += BE) {
-        const int s0 = s_ptr[j], s1 = s_ptr[j + 1];
+    const int half = tid & 1;
+    for (int jb = 0; jb < nb; jb += BE / 2) {
+        const int j = jb + (tid >> 1);
+        const bool has = j < nb;
         T acc[4 * CW];
 #pragma unroll
         for (int q = 0; q < 4 * CW; q++) acc[q] = T(0);
-        for (int k = s0; k < s1; k++) {
-            const int ent = s_ent[k];             // corner * BE + element
-            const int base = CW == 1 ? ent : ent + (ent / BE) * (CW - 1) * BE;
+        if (has) {
+            const int s1 = s_ptr[j + 1];
+            int k = s_ptr[j] + half;
+            for (; k + 2 < s1; k += 4) {
+                const int e0 = s_ent[k], e1 = s_ent[k + 2];
+                const int b0 = CW == 1 ? e0 : e0 + (e0 / BE) * (CW - 1) * BE;
+                const int b1 = CW == 1 ? e1 : e1 + (e1 / BE) * (CW - 1) * BE;
 #pragma unroll
-            for (int jv = 0; jv < CW; jv++) {
-                const V4 v = sf4[base + jv * BE];
-                acc[4 * jv] += v.x; acc[4 * jv + 1] += v.y; acc[4 * jv + 2] += v.z; acc[4 * jv + 3] += v.w;
+                for (int jv = 0; jv < CW; jv++) {
+                    const V4 v0 = sf4[b0 + jv * BE];
+                    const V4 v1 = sf4[b1 + jv * BE];
+                    acc[4 * jv] += v0.x + v1.x; acc[4 * jv + 1] += v0.y + v1.y;
+                    acc[4 * jv + 2] += v0.z + v1.z; acc[4 * jv + 3] += v0.w + v1.w;
+                }
+            }
+            if (k < s1) {
+                const int e0 = s_ent[k];
+                const int b0 = CW == 1 ? e0 : e0 + (e0 / BE) * (CW - 1) * BE;
+#pragma unroll
+                for (int jv = 0; jv < CW; jv++) {
+                    const V4 v0 = sf4[b0 + jv * BE];
+                    acc[4 * jv] += v0.x; acc[4 * jv + 1] += v0.y; acc[4 * jv + 2] += v0.z; acc[4 * jv + 3] += v0.w;
+                }
             }
         }
-        const uint32_t fl = a.bnodes[n0 + j];
-        const uint32_t node = fl & NODE_MASK;
-        if (fl & FLAG_SHARED) {
-            T* pp = partial + (size_t)W * a.bslot[n0 + j];
+        // both lanes of the pair are in the same wave: add the partner's sums
 #pragma unroll
-            for (int q = 0; q < W; q++) pp[q] = acc[q];
-        } else {
-            T* op = out + (size_t)W * node;
-            if ((fl & FLAG_FIXED) && mask_fixed) {
+        for (int q = 0; q < 4 * CW; q++) acc[q] += __shfl_xor(acc[q], 1, 64);
+        if (has && half == 0) {
+            const uint32_t fl = a.bnodes[n0 + j];
+            const uint32_t node = fl & NODE_MASK;
+            if (fl & FLAG_SHARED) {
+                T* pp = partial + (size_t)W * a.bslot[n0 + j];
 #pragma unroll
-                for (int q = 0; q < W; q++) op[q] = (KIND == 2 && (q == 0 || q == 4 || q == 8)) ? T(1) : T(0);
+                for (int q = 0; q < W; q++) pp[q] = acc[q];
             } else {
+                T* op = out + (size_t)W * node;
+                if ((fl & FLAG_FIXED) && mask_fixed) {
 #pragma unroll
-                for (int q = 0; q < W; q++) op[q] = acc[q] + (ext ? (T)ext[3 * (size_t)node + q % 3] : T(0));
+                    for (int q = 0; q < W; q++) op[q] = (KIND == 2 && (q == 0 || q == 4 || q == 8)) ? T(1) : T(0);
+                } else {
+#pragma unroll
+                    for (int q = 0; q < W; q++) op[q] = acc[q] + (ext ? (T)ext[3 * (size_t)node + q % 3] : T(0));
+                }
             }
-        }
-        // d.(K d) is linear in the node sums, so every row -- shared rows with
-        // their partial sums too -- adds its share here; fixed rows have d = 0
-        if (KIND == 0 && dot_part) {
-            const V4 pv = sd4[j];
-            dot += (double)pv.x * acc[0] + (double)pv.y * acc[1] + (double)pv.z * acc[2];
+            // d.(K d) is linear in the node sums, so every row -- shared rows with
+            // their partial sums too -- adds its share here; fixed rows have d = 0
+            if (KIND == 0 && dot_part) {
+                const V4 pv = sd4[j];
+                dot += (double)pv.x * acc[0] + (double)pv.y * acc[1] + (double)pv.z * acc[2];
+            }
         }
     }
     if (KIND == 0 && dot_part) {

@@ -1,0 +1,77 @@
+"""The five workloads (solidfem_amd/configs.py) on the GPU against the FP64
+oracle.  Default sizes are reduced grids with the full physics of each
+workload; SFEM_FULL=1 runs the full grids (minutes), checking the FP64
+residual of the device solution with the oracle and, where the CPU can solve
+it, the displacement error."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from solidfem_amd import capi, configs
+from tests.cfg_util import oracle_problem
+
+pytestmark = pytest.mark.gpu
+FULL = os.environ.get("SFEM_FULL") == "1"
+
+
+def rel2(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+def oracle_solution(d, tol=1e-10, max_newton=60):
+    p = oracle_problem(d, nthreads=16)
+    x = d["mesh"].X.copy()
+    steps = d["load_steps"]
+    for k in range(steps):
+        x, st = p.newton_solve(x, (k + 1) / steps, tol=tol, cg_rtol=0.1, max_newton=max_newton, max_cg=200000)
+    return x, st, p
+
+
+@pytest.mark.parametrize("cfg,grid", [(1, None), (2, (16, 4, 4)), (4, (16, 4, 4)), (5, (16, 4, 4))])
+def test_config_converged_against_oracle(cfg, grid):
+    out, d, s = configs.run(cfg, grid=grid)
+    x = s.positions()
+    xo, st, p = oracle_solution(d)
+    err = rel2(x - d["mesh"].X, xo - d["mesh"].X)
+    res = np.linalg.norm(p.gradients(x, 1.0)) / np.linalg.norm(p.external(x, 1.0))
+    print(f"config {cfg} grid {grid}: newton {out['newton_iters']} cg {out['cg_iters']} "
+          f"oracle-res {res:.2e} err vs FP64 oracle {err:.2e}")
+    assert out["converged"] and st["converged"]
+    assert res < 1e-8
+    assert err < 1e-6
+
+
+def test_config3_corotational_reference_semantics():
+    """The reference's corotational tangent is not the Jacobian of its force,
+    so Newton-CG does not converge quadratically; the device and the oracle run
+    the same fixed-step algorithm (reference mode, 4 Newton steps per load
+    step) on a reduced grid and must agree."""
+    d = configs.build(3, (16, 4, 4))
+    s = configs.solver(d)
+    p = oracle_problem(d)
+    x = d["mesh"].X.copy()
+    for k in range(4):
+        s.step(capi.solve_params(mode=capi.SOLVE_REFERENCE, outer=4, inner=2000, check_every=8,
+                                 sim_type=capi.ST_QUASI_STATIC, force_step=0.05))
+        x, _ = p.newton_cg_ref(x, 0.05 * (k + 1), 4, 2000)
+    err = rel2(s.positions() - d["mesh"].X, x - d["mesh"].X)
+    print(f"config 3 reduced, reference semantics: err {err:.2e}")
+    assert err < 1e-4
+
+
+@pytest.mark.skipif(not FULL, reason="full-size workloads run with SFEM_FULL=1")
+@pytest.mark.parametrize("cfg", [1, 2, 3, 4, 5])
+def test_full_size_config(cfg):
+    out, d, s = configs.run(cfg, max_newton=60 if cfg != 3 else 12)
+    x = s.positions()
+    p = oracle_problem(d, nthreads=16)
+    out["oracle_rel_residual"] = float(np.linalg.norm(p.gradients(x, 1.0)) / np.linalg.norm(p.external(x, 1.0)))
+    if cfg in (1, 2):
+        xo, st, _ = oracle_solution(d)
+        out["err_vs_fp64_oracle"] = float(rel2(x - d["mesh"].X, xo - d["mesh"].X))
+    out.pop("steps")
+    print("FULLCONFIG " + json.dumps(out), flush=True)
+    if cfg != 3:
+        assert out["converged"] and out["oracle_rel_residual"] < 1e-8
