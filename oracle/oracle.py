@@ -276,11 +276,23 @@ class RefProblem:
         self.tets = np.ascontiguousarray(tets, dtype=np.int32).reshape(-1, 4)
         self.fixed = np.ascontiguousarray(fixed, dtype=np.uint8).reshape(-1)
         L = reflib()
-        # the reference prints its Lame parameters on construction
-        self.h = L.ref_create(_p(self.tets, _ip), self.tets.shape[0], _p(self.X0), self.X0.shape[0],
-                              _p(self.fixed, C.POINTER(C.c_uint8)), int(material), float(young),
-                              float(poisson), float(density), float(gravity), int(zup), int(simtype),
-                              float(force_step), int(outer), int(inner))
+        # the reference prints its Lame parameters on construction (printf to
+        # fd 1): keep stdout clean for callers that emit JSON
+        import sys
+        sys.stdout.flush()
+        saved = os.dup(1)
+        devnull = os.open(os.devnull, os.O_WRONLY)
+        os.dup2(devnull, 1)
+        try:
+            self.h = L.ref_create(_p(self.tets, _ip), self.tets.shape[0], _p(self.X0), self.X0.shape[0],
+                                  _p(self.fixed, C.POINTER(C.c_uint8)), int(material), float(young),
+                                  float(poisson), float(density), float(gravity), int(zup), int(simtype),
+                                  float(force_step), int(outer), int(inner))
+            C.CDLL(None).fflush(None)
+        finally:
+            os.dup2(saved, 1)
+            os.close(saved)
+            os.close(devnull)
         if tris is not None:
             self.tris = np.ascontiguousarray(tris, dtype=np.int32).reshape(-1, 3)
             L.ref_set_pressure_surface(self.h, _p(self.tris, _ip), self.tris.shape[0], float(pressure))

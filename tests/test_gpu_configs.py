@@ -64,13 +64,24 @@ def test_config3_corotational_reference_semantics():
 @pytest.mark.skipif(not FULL, reason="full-size workloads run with SFEM_FULL=1")
 @pytest.mark.parametrize("cfg", [1, 2, 3, 4, 5])
 def test_full_size_config(cfg):
+    """Full grids.  The oracle checks the FP64 residual of the device solution;
+    for 1 it solves the problem itself, for 2 and 4 it continues Newton in FP64
+    from the device solution to 1e-10 (same equilibrium branch) and the
+    displacement error is measured against that.  3 uses the reference's
+    corotational tangent, with which Newton stalls (no 1e-8 claim); 5 is
+    checked by residual only (an FP64 CPU solve at nu = 0.495 takes hours)."""
     out, d, s = configs.run(cfg, max_newton=60 if cfg != 3 else 12)
     x = s.positions()
     p = oracle_problem(d, nthreads=16)
     out["oracle_rel_residual"] = float(np.linalg.norm(p.gradients(x, 1.0)) / np.linalg.norm(p.external(x, 1.0)))
-    if cfg in (1, 2):
+    X = d["mesh"].X
+    if cfg == 1:
         xo, st, _ = oracle_solution(d)
-        out["err_vs_fp64_oracle"] = float(rel2(x - d["mesh"].X, xo - d["mesh"].X))
+        out["err_vs_fp64_oracle"] = float(rel2(x - X, xo - X))
+    elif cfg in (2, 4):
+        xo, st = p.newton_solve(x, 1.0, tol=1e-10, cg_rtol=1e-2, max_newton=10, max_cg=200000)
+        out["err_vs_fp64_oracle"] = float(rel2(x - X, xo - X))
+        out["oracle_polish"] = {"newton": st["newton"], "cg": st["cg"], "converged": st["converged"]}
     out.pop("steps")
     print("FULLCONFIG " + json.dumps(out), flush=True)
     if cfg != 3:
