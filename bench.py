@@ -114,7 +114,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--precond", choices=["jacobi", "none"], default="jacobi")
+    ap.add_argument("--precond", choices=["both", "jacobi", "none", "learned"], default="both",
+                    help="both: the Jacobi-PCG solve is `value`, the learned-PCG solve is reported beside it")
     ap.add_argument("--load-steps", type=int, default=10)
     ap.add_argument("--grid", default="128x32x32")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -135,31 +136,59 @@ def main():
     if args.hessvec_only:
         hessvec_only(s, m, args)
         return
-    pc = {"jacobi": capi.PC_JACOBI, "none": capi.PC_NONE}[args.precond]
-    params = capi.solve_params(tol=1e-8, precond=pc, cg_rtol=0.1, sim_type=capi.ST_QUASI_STATIC,
-                               force_step=1.0 / args.load_steps, inner=100000, check_every=32)
+    def leg(pc):
+        """warmup + timed solves with preconditioner pc -> (wall s, runs, final positions)"""
+        params = capi.solve_params(tol=1e-8, precond=pc, cg_rtol=0.1, sim_type=capi.ST_QUASI_STATIC,
+                                   force_step=1.0 / args.load_steps, inner=100000, check_every=32)
 
-    def one_solve():
-        s.reset()
-        tot = {"newton_iters": 0, "cg_iters": 0, "hessvec_evals": 0, "force_evals": 0, "converged": 1}
-        last = None
-        for _ in range(args.load_steps):
-            st = s.step(params)
-            for k in ("newton_iters", "cg_iters", "hessvec_evals", "force_evals"):
-                tot[k] += st[k]
-            tot["converged"] &= st["converged"]
-            last = st
-        tot["res"], tot["ext_norm"] = last["res"], last["ext_norm"]
-        return tot
+        def one_solve():
+            s.reset()
+            tot = {"newton_iters": 0, "cg_iters": 0, "hessvec_evals": 0, "force_evals": 0, "converged": 1,
+                   "fallbacks": 0}
+            last = None
+            for _ in range(args.load_steps):
+                st = s.step(params)
+                for k in ("newton_iters", "cg_iters", "hessvec_evals", "force_evals", "fallbacks"):
+                    tot[k] += st[k]
+                tot["converged"] &= st["converged"]
+                last = st
+            tot["res"], tot["ext_norm"] = last["res"], last["ext_norm"]
+            return tot
 
-    for _ in range(args.warmup):
-        one_solve()
-    t0 = time.perf_counter()
-    runs = [one_solve() for _ in range(args.steps)]
-    dt = time.perf_counter() - t0
+        for _ in range(args.warmup):
+            one_solve()
+        t0 = time.perf_counter()
+        runs = [one_solve() for _ in range(args.steps)]
+        return time.perf_counter() - t0, runs, s.positions()
+
+    main_pc = {"jacobi": capi.PC_JACOBI, "both": capi.PC_JACOBI, "none": capi.PC_NONE,
+               "learned": capi.PC_LEARNED}[args.precond]
+    net = None
+    if args.precond in ("learned", "both"):
+        from solidfem_amd import learned
+        if os.path.exists(learned.WEIGHTS):
+            net = learned.load()
+            net.attach(s)
+        elif args.precond == "learned":
+            raise SystemExit(f"no trained weights at {learned.WEIGHTS}")
+    dt, runs, x = leg(main_pc)
     evals = sum(r["hessvec_evals"] + r["force_evals"] for r in runs)
     cg = sum(r["cg_iters"] for r in runs)
-    x = s.positions()
+    learned_leg = None
+    if args.precond == "both":
+        if net is None:
+            learned_leg = {"skipped": "no trained weights in the package"}
+        else:
+            dtl, runs_l, xl = leg(capi.PC_LEARNED)
+            ev_l = sum(r["hessvec_evals"] + r["force_evals"] for r in runs_l)
+            learned_leg = {
+                "newton_solve_s": dtl / args.steps, "value": ev_l / dtl, "unit": "elem-evals/s",
+                "newton_iters": runs_l[-1]["newton_iters"], "cg_iters": runs_l[-1]["cg_iters"],
+                "fallbacks": runs_l[-1]["fallbacks"], "converged": bool(all(r["converged"] for r in runs_l)),
+                "final_rel_residual": runs_l[-1]["res"] / runs_l[-1]["ext_norm"],
+                "speedup_vs_jacobi": dt / dtl,
+                "displacement_rel_diff_vs_jacobi": float(np.linalg.norm(xl - x) / np.linalg.norm(x - m.X)),
+            }
 
     ms_elem, ms_gather = s.profile_hessvec(args.profile_reps)
     info = s.info()
@@ -181,7 +210,8 @@ def main():
         "config": {"workload": f"nh_beam_{info['ne']}tet_{args.load_steps}loadsteps",
                    "grid": args.grid, "tets": info["ne"], "nodes": info["nn"], "material": "neo-hookean",
                    "E": 1e6, "nu": 0.45, "loads": "gravity -z + dead tip traction 1e4 Pa -z",
-                   "load_steps": args.load_steps, "tol": 1e-8, "precond": args.precond, "parallelism": "none"},
+                   "load_steps": args.load_steps, "tol": 1e-8,
+                   "precond": "jacobi" if args.precond == "both" else args.precond, "parallelism": "none"},
         "newton_solve_s": dt / args.steps,
         "newton_iters": runs[-1]["newton_iters"],
         "cg_iters": runs[-1]["cg_iters"],
@@ -195,6 +225,8 @@ def main():
                      "ms_per_launch": ms_elem, "gather_ms_per_launch": ms_gather,
                      "bytes_per_launch": bytes_alg},
     }
+    if learned_leg is not None:
+        line["learned_pcg"] = learned_leg
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(m, fixed, fdead, x)
     print(json.dumps(line), flush=True)

@@ -161,10 +161,19 @@ void build_partition(sfem_ctx* c, const sfem_mesh* m, const std::vector<double>&
     c->d_sh_node.upload(sh_node);
     c->d_sh_ptr.upload(sh_ptr);
     c->d_bslot.upload(bslot);
-    // per node: shared index, -1 for a free node of one block, -2 for fixed
-    std::vector<int> node_info(nn);
-    for (int64_t i = 0; i < nn; i++) node_info[i] = m->fixed[i] ? -2 : sh_of[i];
-    c->d_node_info.upload(node_info);
+    // per node, for the CG update: a shared node's partial slot range
+    // {first, end}; {-1, 0} for a free node of one block; {-2, 0} for a fixed
+    // node and for a node that no element references (its row of K is 0)
+    std::vector<int> node_q(2 * nn, 0);
+    for (int64_t i = 0; i < nn; i++) {
+        if (m->fixed[i] || node_blocks[i] == 0) node_q[2 * i] = -2;
+        else if (sh_of[i] < 0) node_q[2 * i] = -1;
+        else {
+            node_q[2 * i] = sh_ptr[sh_of[i]];
+            node_q[2 * i + 1] = sh_ptr[sh_of[i] + 1];
+        }
+    }
+    c->d_node_info.upload(node_q);
     c->nrows_partial = (int64_t)bnodes.size();
     c->d_partial.alloc((size_t)c->nrows_partial * 9 * sizeof(float) + (size_t)c->nrows_partial * 3 * sizeof(double));
 
@@ -259,8 +268,8 @@ CgResult pcg(sfem_ctx* c, const float* minv, int pc, int maxit, int test_kind, d
             HIPCK(launch_hessvec_cg(c->material, c->blk, c->d_u32.as<float>(), c->d_p32.as<float>(), c->d_q32.as<float>(),
                                     c->d_partial.as<float>(), done, pq_elem, c->stream));
             HIPCK(launch_cg_iter_tail(pc, n3, c->d_x32.as<float>(), c->d_r32.as<float>(), c->d_z32.as<float>(),
-                                      c->d_p32.as<float>(), c->d_q32.as<float>(), minv, c->d_node_info.as<int>(),
-                                      c->shr.ptr, c->d_partial.as<float>(), scal, pq_elem, c->nblk, parts, done,
+                                      c->d_p32.as<float>(), c->d_q32.as<float>(), minv, c->d_node_info.as<int2>(),
+                                      c->d_partial.as<float>(), scal, pq_elem, c->nblk, parts, done,
                                       test_kind, tol2, guard, c->stream));
         }
         launched += k;

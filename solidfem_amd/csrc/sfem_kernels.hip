@@ -14,9 +14,11 @@
 // so every output is bitwise reproducible run to run.
 //
 // Reductions (dot products, norms) are FP64: a wave-shuffle tree per
-// workgroup, per-workgroup partials, and the last workgroup to finish (ticket
-// counter) sums the partials in index order and derives the CG scalars on
-// the device.
+// workgroup, per-workgroup partials, and every workgroup of the next kernel
+// that needs the total sums the partials itself, in the same fixed order, and
+// derives the CG scalars on the device.  (A last-workgroup ticket scheme needs
+// a device-scope fence per workgroup, which writes back the XCD's L2: far
+// slower here than the redundant sums.)
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -64,17 +66,31 @@ __device__ __forceinline__ void block_partials(double (&v)[NV], double* part)
 }
 
 // every thread of the workgroup gets sum_i part[q * stride + i], i < n, in a
-// fixed order (thread-strided partial sums, then the fixed block tree)
-template <int NV>
+// fixed order (thread-strided partial sums, then the fixed block tree).
+// The partials were written by workgroups on every XCD, so each load misses
+// this XCD's L2: U loads per thread are issued before the first add, so a
+// thread waits for ceil(n / (U * blockDim)) round trips, not n / blockDim.
+template <int NV, int U = 16>
 __device__ __forceinline__ void sum_partials(const double* part, int n, int stride, double (&tot)[NV])
 {
     __shared__ double lds[NV * 16];
     __shared__ double res[NV];
     double acc[NV];
 #pragma unroll
-    for (int q = 0; q < NV; q++) {
-        acc[q] = 0.0;
-        for (int i = threadIdx.x; i < n; i += blockDim.x) acc[q] += part[(size_t)q * stride + i];
+    for (int q = 0; q < NV; q++) acc[q] = 0.0;
+    for (int base = threadIdx.x; base < n; base += blockDim.x * U) {
+        double v[NV][U];
+#pragma unroll
+        for (int q = 0; q < NV; q++)
+#pragma unroll
+            for (int k = 0; k < U; k++) {
+                const int i = base + k * blockDim.x;
+                v[q][k] = i < n ? part[(size_t)q * stride + i] : 0.0;
+            }
+#pragma unroll
+        for (int q = 0; q < NV; q++)
+#pragma unroll
+            for (int k = 0; k < U; k++) acc[q] += v[q][k];
     }
     block_sum<NV>(acc, lds);
     if (threadIdx.x == 0)
@@ -109,6 +125,7 @@ template <> struct KindTraits<2> { typedef float T; static constexpr int W = 9; 
 //   corner outputs: CW vec4 per (element, corner), component-major
 //       sf4[(corner * CW + j) * BE + element]   (CW = 1 for W = 3, 3 for W = 9)
 //   staged node rows: su4[nmax] (and sd4[nmax]) = (x, y, z, 0)
+//   node flags s_flag[nmax] (bnodes entry) and partial slots s_slot[nmax]
 //   incidence entries s_ent[4 BE] = corner * BE + element, offsets s_ptr[nmax+1]
 template <typename T> struct Vec4;
 template <> struct Vec4<float> { typedef float4 V; };
@@ -127,7 +144,8 @@ size_t lds_layout_bytes(int BE, int nmax)
     constexpr int W = KindTraits<KIND>::W;
     const size_t v4 = 4 * sizeof(T);
     const size_t rows = (size_t)nmax * (KindTraits<KIND>::HAS_D ? 2 : 1);
-    return v4 * ((size_t)4 * corner_vec4<KIND>(W) * BE + rows) + sizeof(uint16_t) * (4 * (size_t)BE + nmax + 1) + 16;
+    return v4 * ((size_t)4 * corner_vec4<KIND>(W) * BE + rows) + 2 * sizeof(uint32_t) * (size_t)nmax +
+           sizeof(uint16_t) * (4 * (size_t)BE + nmax + 1) + 16;
 }
 
 template <int KIND, int MAT, int BE>
@@ -148,7 +166,9 @@ __global__ __launch_bounds__(BE) void k_elem_block(BlockArgs a, const typename K
     V4* sf4 = reinterpret_cast<V4*>(smem_raw);        // 4*CW*BE corner rows
     V4* su4 = sf4 + 4 * CW * BE;                      // nmax
     V4* sd4 = su4 + a.nmax;                           // nmax (HAS_D)
-    uint16_t* s_ent = reinterpret_cast<uint16_t*>(sd4 + (HAS_D ? a.nmax : 0));   // 4*BE
+    uint32_t* s_flag = reinterpret_cast<uint32_t*>(sd4 + (HAS_D ? a.nmax : 0)); // nmax
+    int* s_slot = reinterpret_cast<int*>(s_flag + a.nmax);                        // nmax
+    uint16_t* s_ent = reinterpret_cast<uint16_t*>(s_slot + a.nmax);               // 4*BE
     uint16_t* s_ptr = s_ent + 4 * BE;                                             // nmax+1
 
     const int b = blockIdx.x;
@@ -178,26 +198,36 @@ __global__ __launch_bounds__(BE) void k_elem_block(BlockArgs a, const typename K
         }
         li = a.lidx[e];
     }
-    // stage block nodes and the block's incidence lists
+    // the block's incidence lists: this thread's share (2 words of entries,
+    // one offset) is loaded now and stored to LDS after the node staging
+    const uint32_t* le = reinterpret_cast<const uint32_t*>(a.lent + (size_t)b * 4 * BE);
+    const uint16_t* lp = a.lptr + (n0 + b);
+    const uint32_t le0 = le[tid], le1 = le[tid + BE];
+    const uint16_t lp0 = tid <= nb ? lp[tid] : uint16_t(0);
+    // stage block nodes (rows, flags, partial slots); all of a row's loads are
+    // issued before its LDS stores
     for (int j = tid; j < nb; j += BE) {
-        const uint32_t node = a.bnodes[n0 + j] & NODE_MASK;
+        const uint32_t fl = a.bnodes[n0 + j];
+        const uint32_t node = fl & NODE_MASK;
         const T* up = u + 3 * (size_t)node;
-        V4 uv;
+        V4 uv, dv;
         uv.x = up[0]; uv.y = up[1]; uv.z = up[2]; uv.w = T(0);
-        su4[j] = uv;
         if (HAS_D) {
             const T* dp = d + 3 * (size_t)node;
-            V4 dv;
             dv.x = dp[0]; dv.y = dp[1]; dv.z = dp[2]; dv.w = T(0);
-            sd4[j] = dv;
         }
+        const int slot = (fl & FLAG_SHARED) ? a.bslot[n0 + j] : -1;
+        su4[j] = uv;
+        if (HAS_D) sd4[j] = dv;
+        s_flag[j] = fl;
+        s_slot[j] = slot;
     }
     {
-        const uint32_t* le = reinterpret_cast<const uint32_t*>(a.lent + (size_t)b * 4 * BE);
         uint32_t* se = reinterpret_cast<uint32_t*>(s_ent);
-        for (int k = tid; k < 2 * BE; k += BE) se[k] = le[k];
-        const uint16_t* lp = a.lptr + (n0 + b);
-        for (int j = tid; j <= nb; j += BE) s_ptr[j] = lp[j];
+        se[tid] = le0;
+        se[tid + BE] = le1;
+        if (tid <= nb) s_ptr[tid] = lp0;
+        for (int j = tid + BE; j <= nb; j += BE) s_ptr[j] = lp[j];
     }
     __syncthreads();
 
@@ -312,10 +342,10 @@ __global__ __launch_bounds__(BE) void k_elem_block(BlockArgs a, const typename K
 #pragma unroll
         for (int q = 0; q < 4 * CW; q++) acc[q] += __shfl_xor(acc[q], 1, 64);
         if (has && half == 0) {
-            const uint32_t fl = a.bnodes[n0 + j];
+            const uint32_t fl = s_flag[j];
             const uint32_t node = fl & NODE_MASK;
             if (fl & FLAG_SHARED) {
-                T* pp = partial + (size_t)W * a.bslot[n0 + j];
+                T* pp = partial + (size_t)W * s_slot[j];
 #pragma unroll
                 for (int q = 0; q < W; q++) pp[q] = acc[q];
             } else {
@@ -361,10 +391,18 @@ __global__ __launch_bounds__(256) void k_shared_gather(SharedArgs s, const T* __
         T acc[W];
 #pragma unroll
         for (int q = 0; q < W; q++) acc[q] = T(0);
-        for (int k = s.ptr[i]; k < s.ptr[i + 1]; k++) {
-            const T* pp = partial + (size_t)W * k;
+        // up to 4 partial rows in flight per round trip, summed in slot order
+        const int k1 = s.ptr[i + 1];
+        for (int k = s.ptr[i]; k < k1; k += 4) {
+            T t[4][W];
 #pragma unroll
-            for (int q = 0; q < W; q++) acc[q] += pp[q];
+            for (int j = 0; j < 4; j++)
+#pragma unroll
+                for (int q = 0; q < W; q++) t[j][q] = k + j < k1 ? partial[(size_t)W * (k + j) + q] : T(0);
+#pragma unroll
+            for (int j = 0; j < 4; j++)
+#pragma unroll
+                for (int q = 0; q < W; q++) acc[q] += t[j][q];
         }
         T* op = out + (size_t)W * node;
         if ((fl & FLAG_FIXED) && mask_fixed) {
@@ -442,13 +480,13 @@ __global__ __launch_bounds__(256) void k_cg_init_finish(const double* __restrict
 // partials, so the two latencies overlap.
 // q of a node: written by the element kernel for nodes of one block, the sum
 // of the node's contiguous partial slots for shared nodes, 0 for fixed nodes
-// (node_info: >= 0 shared index, -1 single-block free node, -2 fixed)
+// and nodes of no element (node_q: {first slot, end slot} of a shared node;
+// {-1, 0} free node of one block, {-2, 0} fixed or of no element)
 template <int PC>
 __global__ __launch_bounds__(256) void k_cg_update(int64_t n3, float* __restrict__ x, float* __restrict__ r,
                                                    float* __restrict__ z, const float* __restrict__ p,
                                                    const float* __restrict__ q, const float* __restrict__ minv,
-                                                   const int* __restrict__ node_info, const int* __restrict__ sh_ptr,
-                                                   const float* __restrict__ partial,
+                                                   const int2* __restrict__ node_q, const float* __restrict__ partial,
                                                    double* scal, const double* __restrict__ pq_part, int n_pq,
                                                    double* __restrict__ part, int* done, int guard)
 {
@@ -458,17 +496,27 @@ __global__ __launch_bounds__(256) void k_cg_update(int64_t n3, float* __restrict
     int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     float pv[3], qv[3], rv[3], xv[3], M[9];
     auto load = [&](int64_t n) {
-        const int info = node_info[n];
+        const int2 nq = node_q[n];
 #pragma unroll
         for (int c = 0; c < 3; c++) {
             pv[c] = p[3 * n + c]; rv[c] = r[3 * n + c]; xv[c] = x[3 * n + c];
-            qv[c] = info == -1 ? q[3 * n + c] : 0.f;
+            qv[c] = nq.x == -1 ? q[3 * n + c] : 0.f;
         }
-        if (info >= 0)
-            for (int k = sh_ptr[info]; k < sh_ptr[info + 1]; k++) {
+        if (nq.x >= 0) {
+            // up to 4 partial rows in flight per round trip, summed in slot order
+            const int k1 = nq.y;
+            for (int k = nq.x; k < k1; k += 4) {
+                float t[4][3];
 #pragma unroll
-                for (int c = 0; c < 3; c++) qv[c] += partial[3 * (size_t)k + c];
+                for (int j = 0; j < 4; j++)
+#pragma unroll
+                    for (int c = 0; c < 3; c++) t[j][c] = k + j < k1 ? partial[3 * (size_t)(k + j) + c] : 0.f;
+#pragma unroll
+                for (int j = 0; j < 4; j++)
+#pragma unroll
+                    for (int c = 0; c < 3; c++) qv[c] += t[j][c];
             }
+        }
         if (PC) {
 #pragma unroll
             for (int k = 0; k < 9; k++) M[k] = minv[9 * n + k];
@@ -795,13 +843,13 @@ hipError_t launch_hessvec_cg(int mat, const BlockArgs& a, const float* u32, cons
 }
 
 hipError_t launch_cg_iter_tail(int pc, int64_t n3, float* x, float* r, float* z, float* p, const float* q, const float* minv,
-                               const int* node_info, const int* sh_ptr, const float* partial,
+                               const int2* node_q, const float* partial,
                                double* scal, const double* pq_part, int n_pq, double* part, int* done, int test_kind,
                                double tol2, int guard, hipStream_t st)
 {
     const int g = cg_grid(n3 / 3);
-    if (pc) hipLaunchKernelGGL(k_cg_update<1>, dim3(g), dim3(256), 0, st, n3, x, r, z, p, q, minv, node_info, sh_ptr, partial, scal, pq_part, n_pq, part, done, guard);
-    else hipLaunchKernelGGL(k_cg_update<0>, dim3(g), dim3(256), 0, st, n3, x, r, z, p, q, minv, node_info, sh_ptr, partial, scal, pq_part, n_pq, part, done, guard);
+    if (pc) hipLaunchKernelGGL(k_cg_update<1>, dim3(g), dim3(256), 0, st, n3, x, r, z, p, q, minv, node_q, partial, scal, pq_part, n_pq, part, done, guard);
+    else hipLaunchKernelGGL(k_cg_update<0>, dim3(g), dim3(256), 0, st, n3, x, r, z, p, q, minv, node_q, partial, scal, pq_part, n_pq, part, done, guard);
     if (pc) hipLaunchKernelGGL(k_cg_p<1>, dim3(g), dim3(256), 0, st, n3, z, r, p, scal, part, g, done, test_kind, tol2);
     else hipLaunchKernelGGL(k_cg_p<0>, dim3(g), dim3(256), 0, st, n3, z, r, p, scal, part, g, done, test_kind, tol2);
     return hipGetLastError();

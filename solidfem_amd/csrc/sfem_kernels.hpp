@@ -11,7 +11,7 @@ constexpr int BLOCK_E = 256;
 constexpr int RED_GRID = 2048;   // reduction partials buffer holds 3 * RED_GRID doubles
 // workgroups of the CG vector kernels; each of them re-reduces the previous
 // kernel's partials, so this stays small
-constexpr int CG_GRID = 512;
+constexpr int CG_GRID = 1024;
 
 // FP64 scalars kept on the device during a CG solve
 enum Scal { S_RZ = 0, S_RZ0, S_RR, S_RR0, S_PQ, S_ALPHA, S_BETA, S_ITER, S_NT, S_NT2, S_NT3, S_RZ_PREV, S_COUNT = 16 };
@@ -63,7 +63,7 @@ hipError_t launch_hessvec_cg(int mat, const BlockArgs& a, const float* u32, cons
                              const int* done, double* dot_elem, hipStream_t st);
 // alpha, x, r, z (summing shared nodes' partial slots), then beta and p
 hipError_t launch_cg_iter_tail(int pc, int64_t n3, float* x, float* r, float* z, float* p, const float* q, const float* minv,
-                               const int* node_info, const int* sh_ptr, const float* partial,
+                               const int2* node_q, const float* partial,
                                double* scal, const double* pq_part, int n_pq, double* part, int* done, int test_kind,
                                double tol2, int guard, hipStream_t st);
 hipError_t launch_dots3(int64_t n3, const double* a, const double* b, const float* d, double* scal, int slot, double* part,
