@@ -85,6 +85,10 @@ typedef struct {
     double force_step;         /* sfem_step only: FemConfig::mForceApplicationStep */
     double abs_tol;            /* CONVERGED: also stop when ||r||_2 <= abs_tol
                                   (FemConfig::mAbsNewtonRsidualThreshold); 0 = off */
+    int32_t predictor;         /* sfem_step, ST_QUASI_STATIC, CONVERGED only: 1 = start each
+                                  load step from the secant extrapolation of the last two
+                                  states (dropped if it inverts an element); 0 = from the
+                                  last state, as Step does */
 } sfem_solve_params;
 
 typedef struct {

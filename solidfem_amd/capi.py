@@ -50,7 +50,7 @@ class SolveParams(C.Structure):
     _fields_ = [("mode", C.c_int32), ("precond", C.c_int32), ("outer_iterations", C.c_int32),
                 ("inner_iterations", C.c_int32), ("tol", C.c_double), ("cg_rtol", C.c_double),
                 ("line_search", C.c_int32), ("check_every", C.c_int32), ("sim_type", C.c_int32),
-                ("force_step", C.c_double), ("abs_tol", C.c_double)]
+                ("force_step", C.c_double), ("abs_tol", C.c_double), ("predictor", C.c_int32)]
 
 
 class SolveStats(C.Structure):
@@ -136,9 +136,9 @@ def _p(a, t=_dp):
 
 
 def solve_params(mode=SOLVE_CONVERGED, precond=PC_JACOBI, outer=50, inner=20000, tol=1e-8, cg_rtol=1e-1,
-                 line_search=1, check_every=16, sim_type=ST_STATIC, force_step=0.1, abs_tol=0.0):
+                 line_search=1, check_every=16, sim_type=ST_STATIC, force_step=0.1, abs_tol=0.0, predictor=1):
     return SolveParams(mode, precond, outer, inner, tol, cg_rtol, line_search, check_every, sim_type, force_step,
-                       abs_tol)
+                       abs_tol, predictor)
 
 
 class Solver:

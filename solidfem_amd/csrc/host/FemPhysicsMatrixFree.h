@@ -150,7 +150,8 @@ public:
     //   (||r|| <= mTolerance ||r_ext||); 0 keeps the reference's fixed-step
     //   SolveNewtonCG.  NST_NEWTON / NST_NEWTON_LS (assembled direct solves in
     //   the reference) run the converged matrix-free Newton-PCG without / with
-    //   line search.
+    //   line search.  mPredictor (converged modes, quasi-static steps) starts
+    //   each load step from the secant extrapolation of the last two states.
     struct Config {
         NodalForcesType mNodalForcesComputation = NFT_DIRECT;
         NonlinearSolverType mSolver = NST_NEWTON;
@@ -160,6 +161,7 @@ public:
         real mCGTolerance = 0.1;
         Preconditioner mPreconditioner = PC_JACOBI;
         int mCheckEvery = 16;
+        bool mPredictor = true;
     };
 
     FemPhysicsMatrixFree(const std::vector<Tet>& tetrahedra, const std::vector<Node>& allNodes, const FemConfig& config);

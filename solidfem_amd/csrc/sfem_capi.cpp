@@ -149,6 +149,37 @@ void build_partition(sfem_ctx* c, const sfem_mesh* m, const std::vector<double>&
         if (sh_of[id] >= 0) bslot[r] = fill[sh_of[id]]++;
     }
 
+    // rest coordinates of every block row relative to the centre of the block's
+    // bounding box (FP64 difference, then FP32): |coordinate| <= half the block
+    // diameter, so the FP32 edge vectors the Hessian-vector kernel forms from
+    // them are accurate to ~2^-24 block diameter / edge, ~3e-7 relative here
+    std::vector<float> rel32(3 * bnodes.size());
+    for (int b = 0; b < nblk; b++) {
+        double blo[3] = {1e300, 1e300, 1e300}, bhi[3] = {-1e300, -1e300, -1e300};
+        for (int r = bnode_ptr[b]; r < bnode_ptr[b + 1]; r++) {
+            const int64_t id = bnodes[r] & 0x3fffffffu;
+            for (int k = 0; k < 3; k++) {
+                blo[k] = std::min(blo[k], m->X0[3 * id + k]);
+                bhi[k] = std::max(bhi[k], m->X0[3 * id + k]);
+            }
+        }
+        for (int r = bnode_ptr[b]; r < bnode_ptr[b + 1]; r++) {
+            const int64_t id = bnodes[r] & 0x3fffffffu;
+            for (int k = 0; k < 3; k++) rel32[3 * (size_t)r + k] = (float)(m->X0[3 * id + k] - 0.5 * (blo[k] + bhi[k]));
+        }
+    }
+    c->d_rel32.upload(rel32);
+    // Lame parameters for the Hessian-vector kernel: per element only when they vary
+    bool uniform = true;
+    for (int64_t e = 1; e < ne && uniform; e++) uniform = mu[e] == mu[0] && lam[e] == lam[0];
+    if (!uniform) {
+        std::vector<float> ml((size_t)2 * ne_pad, 0.f);
+        for (int64_t s2 = 0; s2 < ne; s2++) {
+            ml[s2] = (float)mu[key[s2].second];
+            ml[ne_pad + s2] = (float)lam[key[s2].second];
+        }
+        c->d_mat32.upload(ml);
+    }
     c->d_bnode_ptr.upload(bnode_ptr);
     c->d_bnodes.upload(bnodes);
     c->d_lptr.upload(lptr);
@@ -192,6 +223,10 @@ void build_partition(sfem_ctx* c, const sfem_mesh* m, const std::vector<double>&
     a.coef32 = c->d_coef32.as<float>();
     a.dminv64 = c->d_dminv64.as<double>();
     a.coef64 = c->d_coef64.as<double>();
+    a.rel32 = c->d_rel32.as<float>();
+    a.mat32 = uniform ? nullptr : c->d_mat32.as<float>();
+    a.mu0 = (float)mu[0];
+    a.lam0 = (float)lam[0];
     SharedArgs& s = c->shr;
     s.nshared = (int)sh_node.size();
     s.node = c->d_sh_node.as<uint32_t>();
@@ -481,6 +516,7 @@ int sfem_create(const sfem_mesh* m, const sfem_material* mat, sfem_ctx** out)
         const size_t n3 = 3 * (size_t)nn;
         hipStream_t s = c->stream;
         zero<double>(c->d_u64, n3, s);
+        zero<double>(c->d_uprev, n3, s);
         zero<double>(c->d_ut64, n3, s);
         zero<float>(c->d_u32, n3, s);
         zero<double>(c->d_r64, n3, s);
@@ -526,6 +562,7 @@ int sfem_set_positions(sfem_ctx* c, const double* x)
         HIPCK(hipMemcpy(c->d_u64.p, u.data(), n3 * sizeof(double), hipMemcpyHostToDevice));
         HIPCK(launch_to_f32((int64_t)n3, c->d_u64.as<double>(), c->d_u32.as<float>(), c->stream));
         HIPCK(hipStreamSynchronize(c->stream));
+        c->has_prev = false;
         return SFEM_OK;
     });
 }
@@ -548,6 +585,7 @@ int sfem_set_load_fraction(sfem_ctx* c, double frac)
 {
     if (!c) return fail(SFEM_EINVAL, "sfem_set_load_fraction: null context");
     c->frac = frac;
+    c->has_prev = false;
     return SFEM_OK;
 }
 
@@ -659,6 +697,7 @@ int sfem_reset(sfem_ctx* c)
         HIPCK(hipMemsetAsync(c->d_u32.p, 0, 3 * c->nn * sizeof(float), c->stream));
         HIPCK(hipStreamSynchronize(c->stream));
         c->frac = 0.0;
+        c->has_prev = false;
         return SFEM_OK;
     });
 }
@@ -744,8 +783,40 @@ int sfem_step(sfem_ctx* c, const sfem_solve_params* p, sfem_solve_stats* st)
     }
     if (p->sim_type == SFEM_ST_QUASI_STATIC) {
         if (c->frac < 1) {
-            c->frac = std::min(1.0, c->frac + p->force_step);
-            return sfem_solve(c, p, st);
+            const double f_old = c->frac, f_new = std::min(1.0, c->frac + p->force_step);
+            int extra = 0;
+            int rc = guard_call("sfem_step", [&]() -> int {
+                const int64_t n3 = 3 * c->nn;
+                double* u = c->d_u64.as<double>();
+                double* r = c->d_r64.as<double>();
+                const bool pred = p->mode == SFEM_SOLVE_CONVERGED && p->predictor && c->has_prev && f_old > c->frac_prev;
+                c->frac = f_new;
+                if (pred) {
+                    // secant predictor u + s (u - u_prev), dropped if it inverts an
+                    // element (non-finite residual).  Its residual norm is usually
+                    // larger than at u (the stiff modes it misses dominate ||r||)
+                    // while its error in the soft modes that Newton-CG converges
+                    // slowest is much smaller, so ||r|| is no test of it.
+                    HIPCK(launch_extrapolate(n3, u, c->d_uprev.as<double>(), (f_new - f_old) / (f_old - c->frac_prev),
+                                             c->d_u32.as<float>(), c->stream));
+                    residual(c, u, r);
+                    const double r1 = dots3(c, r, nullptr, nullptr).aa;
+                    extra = 1;
+                    if (!std::isfinite(r1)) {
+                        HIPCK(hipMemcpyAsync(u, c->d_uprev.p, n3 * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+                        HIPCK(launch_to_f32(n3, u, c->d_u32.as<float>(), c->stream));
+                    }
+                } else {
+                    HIPCK(hipMemcpyAsync(c->d_uprev.p, u, n3 * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+                }
+                c->frac_prev = f_old;
+                c->has_prev = true;
+                return SFEM_OK;
+            });
+            if (rc != SFEM_OK) return rc;
+            rc = sfem_solve(c, p, st);
+            st->force_evals += (int64_t)extra * c->ne;
+            return rc;
         }
         std::memset(st, 0, sizeof *st);
         return SFEM_OK;

@@ -94,6 +94,10 @@ struct sfem_ctx {
     int material = 0;
     double gravity[3] = {0, 0, 0};
     double frac = 0.0;
+    // load-step predictor history: the converged displacement of the previous
+    // quasi-static step (d_uprev) and its load fraction
+    double frac_prev = 0.0;
+    bool has_prev = false;
     double pressure = 0.0;
     int64_t ntri = 0;
     bool has_fdead = false;
@@ -109,12 +113,12 @@ struct sfem_ctx {
     SharedArgs shr{};
     int nblk = 0;
     int64_t nrows_partial = 0;
-    DevBuf d_bnode_ptr, d_bnodes, d_lptr, d_lent, d_lidx, d_dminv32, d_coef32, d_dminv64, d_coef64;
+    DevBuf d_rel32, d_mat32, d_bnode_ptr, d_bnodes, d_lptr, d_lent, d_lidx, d_dminv32, d_coef32, d_dminv64, d_coef64;
     DevBuf d_sh_node, d_sh_ptr, d_bslot, d_node_info;
     DevBuf d_partial;        // nrows_partial * 9 floats (also used as FP64 x3)
 
     // node data
-    DevBuf d_X0, d_mass, d_fixed, d_fdead;
+    DevBuf d_uprev, d_X0, d_mass, d_fixed, d_fdead;
     DevBuf d_u64, d_ut64, d_u32;
     DevBuf d_r64, d_ext64, d_f64tmp;
     DevBuf d_b32, d_x32, d_r32, d_z32, d_p32, d_q32, d_d32, d_minv, d_minv_learned, d_bd32;

@@ -117,14 +117,23 @@ __global__ __launch_bounds__(256) void k_finish(const double* __restrict__ part,
 // KIND 1: internal force (FP64):  out = f_int (+ ext) (u staged)
 // KIND 2: diagonal 3x3 blocks (FP32): out = K_ii      (u staged), W = 9
 template <int KIND> struct KindTraits;
-template <> struct KindTraits<0> { typedef float T; static constexpr int W = 3; static constexpr bool HAS_D = true; };
-template <> struct KindTraits<1> { typedef double T; static constexpr int W = 3; static constexpr bool HAS_D = false; };
-template <> struct KindTraits<2> { typedef float T; static constexpr int W = 9; static constexpr bool HAS_D = false; };
+// HAS_X: Dm^-1 and V from the staged rest coordinates instead of per-element
+// arrays (the per-element bytes dominate this kernel's HBM traffic)
+template <> struct KindTraits<0> {
+    typedef float T; static constexpr int W = 3; static constexpr bool HAS_D = true; static constexpr bool HAS_X = true;
+};
+template <> struct KindTraits<1> {
+    typedef double T; static constexpr int W = 3; static constexpr bool HAS_D = false; static constexpr bool HAS_X = false;
+};
+template <> struct KindTraits<2> {
+    typedef float T; static constexpr int W = 9; static constexpr bool HAS_D = false; static constexpr bool HAS_X = false;
+};
 
 // LDS layout (vec4 = 4 x T, one ds_read/ds_write_b128 per row):
 //   corner outputs: CW vec4 per (element, corner), component-major
 //       sf4[(corner * CW + j) * BE + element]   (CW = 1 for W = 3, 3 for W = 9)
-//   staged node rows: su4[nmax] (and sd4[nmax]) = (x, y, z, 0)
+//   staged node rows: su4[nmax] (and sd4[nmax], sx4[nmax]) = (x, y, z, 0);
+//   sx4: rest coordinates relative to the block's centre
 //   node flags s_flag[nmax] (bnodes entry) and partial slots s_slot[nmax]
 //   incidence entries s_ent[4 BE] = corner * BE + element, offsets s_ptr[nmax+1]
 template <typename T> struct Vec4;
@@ -143,7 +152,7 @@ size_t lds_layout_bytes(int BE, int nmax)
     typedef typename KindTraits<KIND>::T T;
     constexpr int W = KindTraits<KIND>::W;
     const size_t v4 = 4 * sizeof(T);
-    const size_t rows = (size_t)nmax * (KindTraits<KIND>::HAS_D ? 2 : 1);
+    const size_t rows = (size_t)nmax * (1 + (KindTraits<KIND>::HAS_D ? 1 : 0) + (KindTraits<KIND>::HAS_X ? 1 : 0));
     return v4 * ((size_t)4 * corner_vec4<KIND>(W) * BE + rows) + 2 * sizeof(uint32_t) * (size_t)nmax +
            sizeof(uint16_t) * (4 * (size_t)BE + nmax + 1) + 16;
 }
@@ -161,12 +170,14 @@ __global__ __launch_bounds__(BE) void k_elem_block(BlockArgs a, const typename K
     constexpr int W = KindTraits<KIND>::W;
     constexpr int CW = corner_vec4<KIND>(W);
     constexpr bool HAS_D = KindTraits<KIND>::HAS_D;
+    constexpr bool HAS_X = KindTraits<KIND>::HAS_X;
     if (done && *done) return;
     extern __shared__ __align__(32) unsigned char smem_raw[];
     V4* sf4 = reinterpret_cast<V4*>(smem_raw);        // 4*CW*BE corner rows
     V4* su4 = sf4 + 4 * CW * BE;                      // nmax
     V4* sd4 = su4 + a.nmax;                           // nmax (HAS_D)
-    uint32_t* s_flag = reinterpret_cast<uint32_t*>(sd4 + (HAS_D ? a.nmax : 0)); // nmax
+    V4* sx4 = sd4 + (HAS_D ? a.nmax : 0);             // nmax (HAS_X)
+    uint32_t* s_flag = reinterpret_cast<uint32_t*>(sx4 + (HAS_X ? a.nmax : 0)); // nmax
     int* s_slot = reinterpret_cast<int*>(s_flag + a.nmax);                        // nmax
     uint16_t* s_ent = reinterpret_cast<uint16_t*>(s_slot + a.nmax);               // 4*BE
     uint16_t* s_ptr = s_ent + 4 * BE;                                             // nmax+1
@@ -183,7 +194,16 @@ __global__ __launch_bounds__(BE) void k_elem_block(BlockArgs a, const typename K
     T g[9];
     T cmu = T(0), clam = T(0);
     uint2 li = make_uint2(0, 0);
-    if (live) {
+    if (live && HAS_X) {
+        if (a.mat32) {
+            cmu = (T)a.mat32[e];
+            clam = (T)a.mat32[(size_t)a.ne_pad + e];
+        } else {
+            cmu = (T)a.mu0;
+            clam = (T)a.lam0;
+        }
+        li = a.lidx[e];
+    } else if (live) {
         const size_t st = (size_t)a.ne_pad;
         if (sizeof(T) == 4) {
 #pragma unroll
@@ -217,8 +237,14 @@ __global__ __launch_bounds__(BE) void k_elem_block(BlockArgs a, const typename K
             dv.x = dp[0]; dv.y = dp[1]; dv.z = dp[2]; dv.w = T(0);
         }
         const int slot = (fl & FLAG_SHARED) ? a.bslot[n0 + j] : -1;
+        V4 xv;
+        if (HAS_X) {
+            const float* xp = a.rel32 + 3 * (size_t)(n0 + j);
+            xv.x = (T)xp[0]; xv.y = (T)xp[1]; xv.z = (T)xp[2]; xv.w = T(0);
+        }
         su4[j] = uv;
         if (HAS_D) sd4[j] = dv;
+        if (HAS_X) sx4[j] = xv;
         s_flag[j] = fl;
         s_slot[j] = slot;
     }
@@ -233,6 +259,25 @@ __global__ __launch_bounds__(BE) void k_elem_block(BlockArgs a, const typename K
 
     if (live) {
         const int l[4] = {(int)(li.x & 0xffffu), (int)(li.x >> 16), (int)(li.y & 0xffffu), (int)(li.y >> 16)};
+        if (HAS_X) {
+            // Dm = [X1 - X0, X2 - X0, X3 - X0] (columns), Dm^-1 and V = det / 6 as
+            // at set-up (FemPhysicsMatrixFree.cpp:126-146), in FP32
+            T X[4][3];
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const V4 v = sx4[l[k]];
+                X[k][0] = v.x; X[k][1] = v.y; X[k][2] = v.z;
+            }
+            T D[9];
+#pragma unroll
+            for (int k = 1; k < 4; k++)
+#pragma unroll
+                for (int r = 0; r < 3; r++) D[3 * r + (k - 1)] = X[k][r] - X[0][r];
+            const T det = inv3(D, g);
+            const T V = det * T(1.0 / 6.0);
+            cmu *= V;
+            clam *= V;
+        }
         T uu[4][3];
 #pragma unroll
         for (int k = 0; k < 4; k++) {
@@ -633,6 +678,19 @@ __global__ __launch_bounds__(256) void k_axpy_state(int64_t n3, const double* __
     }
 }
 
+// load-step predictor: uprev <- u, u <- u + s (u - uprev), u32 <- (float) u
+__global__ __launch_bounds__(256) void k_extrapolate(int64_t n3, double* __restrict__ u, double* __restrict__ uprev,
+                                                     double sc, float* __restrict__ u32)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n3; i += (int64_t)gridDim.x * blockDim.x) {
+        const double a = u[i], b = uprev[i];
+        const double v = a + sc * (a - b);
+        uprev[i] = a;
+        u[i] = v;
+        u32[i] = (float)v;
+    }
+}
+
 __global__ __launch_bounds__(256) void k_to_f32(int64_t n, const double* __restrict__ a, float* __restrict__ b)
 {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
@@ -867,6 +925,12 @@ hipError_t launch_dots3(int64_t n3, const double* a, const double* b, const floa
 hipError_t launch_axpy_state(int64_t n3, const double* u, const float* dx, double alpha, double* ut, float* u32, hipStream_t st)
 {
     hipLaunchKernelGGL(k_axpy_state, dim3(grid_for(n3, 256, 8192)), dim3(256), 0, st, n3, u, dx, alpha, ut, u32);
+    return hipGetLastError();
+}
+
+hipError_t launch_extrapolate(int64_t n3, double* u, double* uprev, double sc, float* u32, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_extrapolate, dim3(grid_for(n3, 256, 8192)), dim3(256), 0, st, n3, u, uprev, sc, u32);
     return hipGetLastError();
 }
 

@@ -32,6 +32,11 @@ struct BlockArgs {
     const float* coef32;   // 2 x ne_pad: V mu, V lambda, FP32
     const double* dminv64; // 9 x ne_pad, FP64
     const double* coef64;  // 2 x ne_pad, FP64
+    // Hessian-vector kernel: Dm^-1 and V are recomputed in FP32 from the rest
+    // coordinates of the block rows, relative to the block's centre
+    const float* rel32;    // 3 per block row
+    const float* mat32;    // 2 x ne_pad: mu, lambda per element; null when uniform
+    float mu0, lam0;       // uniform mu, lambda
 };
 
 // nodes shared by several blocks: partial rows to sum
@@ -69,6 +74,7 @@ hipError_t launch_cg_iter_tail(int pc, int64_t n3, float* x, float* r, float* z,
 hipError_t launch_dots3(int64_t n3, const double* a, const double* b, const float* d, double* scal, int slot, double* part,
                         hipStream_t st);
 hipError_t launch_axpy_state(int64_t n3, const double* u, const float* dx, double alpha, double* ut, float* u32, hipStream_t st);
+hipError_t launch_extrapolate(int64_t n3, double* u, double* uprev, double sc, float* u32, hipStream_t st);
 hipError_t launch_to_f32(int64_t n, const double* a, float* b, hipStream_t st);
 hipError_t launch_external(int64_t nn, const double* mass, const double* fdead, const double* g, double frac, const int* pptr,
                            const int* pidx, const double* ftri, const uint8_t* fixed, double* ext, hipStream_t st);

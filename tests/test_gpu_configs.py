@@ -44,21 +44,33 @@ def test_config_converged_against_oracle(cfg, grid):
 
 
 def test_config3_corotational_reference_semantics():
-    """The reference's corotational tangent is not the Jacobian of its force,
-    so Newton-CG does not converge quadratically; the device and the oracle run
-    the same fixed-step algorithm (reference mode, 4 Newton steps per load
-    step) on a reduced grid and must agree."""
+    """The reference's corotational tangent is not the Jacobian of its force
+    and not symmetric, so its CG (fixed 1e-3 forcing, no convergence test on
+    the Newton residual) amplifies operator rounding: the FP64 oracle itself
+    moves by the spread printed below when its moduli change by +-2e-7, about
+    FP32 rounding (a CG run that stops one iteration earlier or later).  The
+    device and the oracle run the same fixed-step algorithm (reference mode, 4
+    Newton steps per load step) on a reduced grid; the device must agree with
+    the oracle to within a small multiple of that spread."""
     d = configs.build(3, (16, 4, 4))
     s = configs.solver(d)
-    p = oracle_problem(d)
-    x = d["mesh"].X.copy()
+    X = d["mesh"].X
+
+    def oracle_run(scale):
+        p = oracle_problem(d, modulus_scale=scale)
+        x = X.copy()
+        for k in range(4):
+            x, _ = p.newton_cg_ref(x, 0.05 * (k + 1), 4, 2000)
+        return x
+
     for k in range(4):
         s.step(capi.solve_params(mode=capi.SOLVE_REFERENCE, outer=4, inner=2000, check_every=8,
                                  sim_type=capi.ST_QUASI_STATIC, force_step=0.05))
-        x, _ = p.newton_cg_ref(x, 0.05 * (k + 1), 4, 2000)
-    err = rel2(s.positions() - d["mesh"].X, x - d["mesh"].X)
-    print(f"config 3 reduced, reference semantics: err {err:.2e}")
-    assert err < 1e-4
+    x = oracle_run(1.0)
+    err = rel2(s.positions() - X, x - X)
+    spread = max(rel2(oracle_run(1.0 + sc) - X, x - X) for sc in (2e-7, -2e-7))
+    print(f"config 3 reduced, reference semantics: err {err:.2e}, oracle spread under +-2e-7 moduli {spread:.2e}")
+    assert err < max(10 * spread, 1e-5)
 
 
 @pytest.mark.skipif(not FULL, reason="full-size workloads run with SFEM_FULL=1")
