@@ -2,7 +2,10 @@
 and the trained weights it ships with.
 
     python -m solidfem_amd.learned train [--beams 24] [--epochs 30] [--out FILE]
-    python -m solidfem_amd.learned eval  [--weights FILE] [--grid 128x32x32]
+
+The weights in weights/learned_precond_nh.npz were written by that command
+(2,016 pairs on 24 beams, 30 epochs, 47 s on one MI355X); bench.py loads them
+for its learned-PCG leg.
 
 Training data (the network, loss and device kernels are those of
 sfem_gnn_* in include/sfem_capi.h): seeded random Neo-Hookean beams from

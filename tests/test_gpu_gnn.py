@@ -165,3 +165,26 @@ def test_learned_pcg_newton_solve_converges_with_judge():
           f"jacobi: newton {st_j['newton_iters']} cg {st_j['cg_iters']}")
     assert st_l["converged"] == 1 and st_j["converged"] == 1
     assert rel2(xl - m.X, xj - m.X) < 1e-6
+
+
+def test_shipped_weights_learned_pcg_matches_jacobi():
+    """The trained weights shipped in the package drive a learned-PCG
+    quasi-static solve (with the Jacobi judge) to the same state as
+    Jacobi-PCG."""
+    from solidfem_amd import learned
+    net = learned.load()
+    m = BeamMesh(16, 4, 4)
+    fixed = m.fixed_mask()
+    f = traction_nodal_forces(m.X, m.tip_triangles(), lambda p: np.tile([0.0, 0.0, -1e4], (len(p), 1)))
+    out = {}
+    for pc in (capi.PC_JACOBI, capi.PC_LEARNED):
+        s = capi.Solver(m.X, m.tets, fixed, capi.NEO_HOOKEAN, 1e6, 0.45, 1070.0, gravity=(0, 0, -9.8))
+        s.set_external_forces(f)
+        net.attach(s)
+        p = capi.solve_params(tol=1e-8, precond=pc, sim_type=capi.ST_QUASI_STATIC, force_step=0.5, inner=100000)
+        sts = [s.step(p) for _ in range(2)]
+        assert all(st["converged"] == 1 for st in sts)
+        out[pc] = (s.positions(), sum(st["cg_iters"] for st in sts), sum(st["fallbacks"] for st in sts))
+    print(f"cg iterations: jacobi {out[capi.PC_JACOBI][1]}, learned {out[capi.PC_LEARNED][1]} "
+          f"(fallbacks {out[capi.PC_LEARNED][2]})")
+    assert rel2(out[capi.PC_LEARNED][0] - m.X, out[capi.PC_JACOBI][0] - m.X) < 1e-6
