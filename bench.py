@@ -28,6 +28,23 @@ from solidfem_amd import capi  # noqa: E402
 from solidfem_amd.mesh import BeamMesh, traction_nodal_forces  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+# every TIME_EVERY-th Hessian-vector launch of the timed solves is bracketed by
+# HIP events on the solver's stream (the roofline's average launch duration)
+TIME_EVERY = 16
+COUNTERS = os.path.join(ROOT, "profiles", "hessvec_nh786k_counters.md")
+
+
+def measured_traffic(grid):
+    """HBM bytes per launch of the Hessian-vector kernel on the 128x32x32 beam
+    from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (its line
+    `traffic_bytes_per_launch: N`); None for other grids or without it."""
+    if grid != (128, 32, 32) or not os.path.exists(COUNTERS):
+        return None
+    with open(COUNTERS) as f:
+        for line in f:
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":")[1])
+    return None
 
 
 def workload(grid=(128, 32, 32)):
@@ -119,7 +136,6 @@ def main():
     ap.add_argument("--load-steps", type=int, default=10)
     ap.add_argument("--grid", default="128x32x32")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--profile-reps", type=int, default=200)
     ap.add_argument("--hessvec-only", type=int, default=0, metavar="REPS",
                     help="only time REPS Hessian-vector launches at a deformed state (for counter runs)")
     ap.add_argument("--calib-mib", type=int, default=0,
@@ -157,9 +173,12 @@ def main():
 
         for _ in range(args.warmup):
             one_solve()
+        s.time_hessvec(TIME_EVERY)
         t0 = time.perf_counter()
         runs = [one_solve() for _ in range(args.steps)]
-        return time.perf_counter() - t0, runs, s.positions()
+        dt = time.perf_counter() - t0
+        ms, n = s.time_hessvec(0)
+        return dt, runs, s.positions(), (ms, n)
 
     main_pc = {"jacobi": capi.PC_JACOBI, "both": capi.PC_JACOBI, "none": capi.PC_NONE,
                "learned": capi.PC_LEARNED}[args.precond]
@@ -171,7 +190,7 @@ def main():
             net.attach(s)
         elif args.precond == "learned":
             raise SystemExit(f"no trained weights at {learned.WEIGHTS}")
-    dt, runs, x = leg(main_pc)
+    dt, runs, x, (ms_elem, n_timed) = leg(main_pc)
     evals = sum(r["hessvec_evals"] + r["force_evals"] for r in runs)
     cg = sum(r["cg_iters"] for r in runs)
     learned_leg = None
@@ -179,7 +198,7 @@ def main():
         if net is None:
             learned_leg = {"skipped": "no trained weights in the package"}
         else:
-            dtl, runs_l, xl = leg(capi.PC_LEARNED)
+            dtl, runs_l, xl, _ = leg(capi.PC_LEARNED)
             ev_l = sum(r["hessvec_evals"] + r["force_evals"] for r in runs_l)
             learned_leg = {
                 "newton_solve_s": dtl / args.steps, "value": ev_l / dtl, "unit": "elem-evals/s",
@@ -190,7 +209,6 @@ def main():
                 "displacement_rel_diff_vs_jacobi": float(np.linalg.norm(xl - x) / np.linalg.norm(x - m.X)),
             }
 
-    ms_elem, ms_gather = s.profile_hessvec(args.profile_reps)
     info = s.info()
     bytes_alg = hessvec_bytes(info["ne"], info["nn"])
     achieved = bytes_alg / (ms_elem * 1e-3) / 1e9
@@ -220,9 +238,9 @@ def main():
         "final_rel_residual": runs[-1]["res"] / runs[-1]["ext_norm"],
         "tip_deflection_m": float((x - m.X)[:, 2].min()),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(grid),
                      "kernel": "k_elem_block<0,NEO_HOOKEAN,256> (Hessian-vector element blocks)",
-                     "ms_per_launch": ms_elem, "gather_ms_per_launch": ms_gather,
+                     "ms_per_launch": ms_elem, "timed_launches": n_timed,
                      "bytes_per_launch": bytes_alg},
     }
     if learned_leg is not None:

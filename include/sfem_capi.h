@@ -172,6 +172,14 @@ int sfem_reset(sfem_ctx* ctx);
    (FemPhysicsMatrixFree.cpp:408). */
 int sfem_profile_hessvec(sfem_ctx* ctx, int reps, double* ms_elem, double* ms_gather);
 
+/* Instrumentation inside solves: every > 0 starts timing every `every`-th
+   Hessian-vector element-block launch of the CG loops with a pair of HIP
+   events that the dispatch itself records at its start and end, on the
+   context's stream (accumulators reset); every == 0 stops.
+   Returns the average device time (ms) and the number of timed launches since
+   the last start.  Same counterpart as sfem_profile_hessvec. */
+int sfem_time_hessvec(sfem_ctx* ctx, int every, double* ms_avg, int64_t* samples);
+
 /* ---------------------------------------------------------------------------
  * Learned block preconditioner (no reference counterpart).  A message-passing
  * network over the node graph (3 rounds, width 32, BF16 MFMA GEMMs with FP32

@@ -24,7 +24,7 @@ EXPORTS = [
     "sfem_set_load_fraction", "sfem_get_load_fraction", "sfem_set_pressure_surface",
     "sfem_set_external_forces", "sfem_internal_forces", "sfem_gradients", "sfem_matvec",
     "sfem_block_diagonal", "sfem_solve", "sfem_step", "sfem_set_precond_blocks", "sfem_info",
-    "sfem_reset", "sfem_profile_hessvec",
+    "sfem_reset", "sfem_profile_hessvec", "sfem_time_hessvec",
     "sfem_gnn_create", "sfem_gnn_destroy", "sfem_gnn_num_params", "sfem_gnn_get_params", "sfem_gnn_set_params",
     "sfem_set_learned_preconditioner", "sfem_collect_pairs", "sfem_num_pairs", "sfem_get_pair", "sfem_clear_pairs",
     "sfem_gnn_loss", "sfem_gnn_eval", "sfem_gnn_train", "sfem_gnn_inputs", "sfem_gnn_gemm_check",
@@ -101,6 +101,7 @@ def lib():
                            C.POINTER(C.c_int64)], C.c_int),
             "sfem_reset": ([V], C.c_int),
             "sfem_profile_hessvec": ([V, C.c_int, _dp, _dp], C.c_int),
+            "sfem_time_hessvec": ([V, C.c_int, _dp, C.POINTER(C.c_int64)], C.c_int),
             "sfem_gnn_create": ([C.c_uint64, C.POINTER(V)], C.c_int),
             "sfem_gnn_destroy": ([V], None),
             "sfem_gnn_num_params": ([V, C.POINTER(C.c_int64)], C.c_int),
@@ -182,6 +183,13 @@ class Solver:
         a, b = C.c_double(), C.c_double()
         _check(lib().sfem_profile_hessvec(self.h, int(reps), C.byref(a), C.byref(b)))
         return a.value, b.value
+
+    def time_hessvec(self, every):
+        """start (every > 0) / stop (0) sampled event timing of the CG loops'
+        Hessian-vector launches; -> (average ms, samples) since the last start"""
+        a, n = C.c_double(), C.c_int64()
+        _check(lib().sfem_time_hessvec(self.h, int(every), C.byref(a), C.byref(n)))
+        return a.value, n.value
 
     def set_positions(self, x):
         x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, 3)

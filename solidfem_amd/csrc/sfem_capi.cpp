@@ -300,8 +300,12 @@ CgResult pcg(sfem_ctx* c, const float* minv, int pc, int maxit, int test_kind, d
     while (launched < maxit) {
         const int k = std::min(check_every, maxit - launched);
         for (int i = 0; i < k; i++) {
+            const bool timed = c->tm_every > 0 && c->tm_counter++ % c->tm_every == 0 && c->tm_pending < sfem_ctx::TM_POOL;
+            hipEvent_t ev0 = timed ? c->tm_ev[2 * c->tm_pending] : nullptr;
+            hipEvent_t ev1 = timed ? c->tm_ev[2 * c->tm_pending + 1] : nullptr;
             HIPCK(launch_hessvec_cg(c->material, c->blk, c->d_u32.as<float>(), c->d_p32.as<float>(), c->d_q32.as<float>(),
-                                    c->d_partial.as<float>(), done, pq_elem, c->stream));
+                                    c->d_partial.as<float>(), done, pq_elem, c->stream, ev0, ev1));
+            if (timed) c->tm_iter[c->tm_pending++] = launched + i;
             HIPCK(launch_cg_iter_tail(pc, n3, c->d_x32.as<float>(), c->d_r32.as<float>(), c->d_z32.as<float>(),
                                       c->d_p32.as<float>(), c->d_q32.as<float>(), minv, c->d_node_info.as<int2>(),
                                       c->d_partial.as<float>(), scal, pq_elem, c->nblk, parts, done,
@@ -310,6 +314,20 @@ CgResult pcg(sfem_ctx* c, const float* minv, int pc, int maxit, int test_kind, d
         launched += k;
         HIPCK(hipMemcpyAsync(&dflag, done, sizeof(int), hipMemcpyDeviceToHost, c->stream));
         HIPCK(hipStreamSynchronize(c->stream));
+        if (c->tm_pending) {
+            // launches after the converged iteration return at once: keep those that ran
+            double it_done = 1e300;
+            if (dflag) HIPCK(hipMemcpy(&it_done, scal + S_ITER, sizeof(double), hipMemcpyDeviceToHost));
+            for (int k = 0; k < c->tm_pending; k++) {
+                float ms = 0.f;
+                HIPCK(hipEventElapsedTime(&ms, c->tm_ev[2 * k], c->tm_ev[2 * k + 1]));
+                if ((double)c->tm_iter[k] < it_done) {
+                    c->tm_ms += ms;
+                    c->tm_samples++;
+                }
+            }
+            c->tm_pending = 0;
+        }
         if (dflag) break;
         if (hook) hook(c, launched, user);
     }
@@ -540,6 +558,24 @@ int sfem_create(const sfem_mesh* m, const sfem_material* mat, sfem_ctx** out)
 }
 
 void sfem_destroy(sfem_ctx* c) { delete c; }
+
+int sfem_time_hessvec(sfem_ctx* c, int every, double* ms_avg, int64_t* samples)
+{
+    if (!c || every < 0) return fail(SFEM_EINVAL, "sfem_time_hessvec: bad argument");
+    return guard_call("sfem_time_hessvec", [&]() -> int {
+        if (ms_avg) *ms_avg = c->tm_samples ? c->tm_ms / (double)c->tm_samples : 0.0;
+        if (samples) *samples = c->tm_samples;
+        if (every > 0 && !c->tm_ev[0])
+            for (hipEvent_t& e : c->tm_ev) HIPCK(hipEventCreate(&e));
+        if (every != c->tm_every || every > 0) {
+            c->tm_every = every;
+            c->tm_counter = 0;
+            c->tm_samples = 0;
+            c->tm_ms = 0.0;
+        }
+        return SFEM_OK;
+    });
+}
 
 int sfem_info(sfem_ctx* c, int64_t* ne, int64_t* nn, int64_t* nfixed, int64_t* nblocks)
 {

@@ -133,8 +133,18 @@ struct sfem_ctx {
     sfem_gnn* gnn = nullptr;
     SfemPairs pairs;
 
+    // sampled timing of the CG loops' Hessian-vector launches (sfem_time_hessvec)
+    static constexpr int TM_POOL = 64;
+    hipEvent_t tm_ev[2 * TM_POOL] = {};
+    int tm_iter[TM_POOL] = {};
+    int tm_every = 0, tm_pending = 0;
+    int64_t tm_counter = 0, tm_samples = 0;
+    double tm_ms = 0.0;
+
     ~sfem_ctx()
     {
+        for (hipEvent_t e : tm_ev)
+            if (e) (void)hipEventDestroy(e);
         if (stream) (void)hipStreamDestroy(stream);
     }
 };

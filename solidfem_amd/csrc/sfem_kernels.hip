@@ -19,6 +19,7 @@
 // derives the CG scalars on the device.  (A last-workgroup ticket scheme needs
 // a device-scope fence per workgroup, which writes back the XCD's L2: far
 // slower here than the redundant sums.)
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -784,13 +785,20 @@ static int grid_for(int64_t n, int bs, int cap)
 
 template <int KIND, int BE>
 static hipError_t launch_elem_mat(int mat, const BlockArgs& a, const void* u, const void* d, void* out, void* partial,
-                                  const double* ext, int mask_fixed, const int* done, double* dot, size_t lds, hipStream_t s)
+                                  const double* ext, int mask_fixed, const int* done, double* dot, size_t lds, hipStream_t s,
+                                  hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr)
 {
     typedef typename KindTraits<KIND>::T T;
     dim3 grid(a.nblk), block(BE);
+    // with events: hipExtLaunchKernelGGL records them at the dispatch's own start
+    // and end, so their interval is the kernel's execution time
 #define SFEM_ELEM_CASE(M)                                                                                            \
-    hipLaunchKernelGGL((k_elem_block<KIND, M, BE>), grid, block, lds, s, a, (const T*)u, (const T*)d, (T*)out,      \
-                       (T*)partial, ext, mask_fixed, done, dot)
+    if (ev0)                                                                                                         \
+        hipExtLaunchKernelGGL((k_elem_block<KIND, M, BE>), grid, block, (uint32_t)lds, s, ev0, ev1, 0u, a,          \
+                              (const T*)u, (const T*)d, (T*)out, (T*)partial, ext, mask_fixed, done, dot);           \
+    else                                                                                                             \
+        hipLaunchKernelGGL((k_elem_block<KIND, M, BE>), grid, block, lds, s, a, (const T*)u, (const T*)d, (T*)out,  \
+                           (T*)partial, ext, mask_fixed, done, dot)
     switch (mat) {
     case LINEAR: SFEM_ELEM_CASE(LINEAR); break;
     case STVK: SFEM_ELEM_CASE(STVK); break;
@@ -864,19 +872,31 @@ size_t elem_lds_max(const BlockArgs& a)
 
 hipError_t set_elem_lds_limits(const BlockArgs& a)
 {
-    // allow dynamic LDS above the 64 KiB default for unstructured meshes
-    const int lim = 160 * 1024;
+    // dynamic LDS above the 64 KiB default (unstructured meshes with many nodes
+    // per block): raise each kernel's limit to what it needs, within the
+    // device's opt-in maximum less the kernel's static LDS
     const void* fns[] = {
-        (const void*)k_elem_block<0, LINEAR, BLOCK_E>, (const void*)k_elem_block<0, STVK, BLOCK_E>,
-        (const void*)k_elem_block<0, COROTATIONAL, BLOCK_E>, (const void*)k_elem_block<0, NEO_HOOKEAN, BLOCK_E>,
-        (const void*)k_elem_block<1, LINEAR, BLOCK_E>, (const void*)k_elem_block<1, STVK, BLOCK_E>,
-        (const void*)k_elem_block<1, COROTATIONAL, BLOCK_E>, (const void*)k_elem_block<1, NEO_HOOKEAN, BLOCK_E>,
-        (const void*)k_elem_block<2, LINEAR, BLOCK_E>, (const void*)k_elem_block<2, STVK, BLOCK_E>,
-        (const void*)k_elem_block<2, COROTATIONAL, BLOCK_E>, (const void*)k_elem_block<2, NEO_HOOKEAN, BLOCK_E>,
+#define SFEM_FN(K, M) (const void*)k_elem_block<K, M, BLOCK_E>
+        SFEM_FN(0, LINEAR), SFEM_FN(0, STVK), SFEM_FN(0, COROTATIONAL), SFEM_FN(0, NEO_HOOKEAN),
+        SFEM_FN(1, LINEAR), SFEM_FN(1, STVK), SFEM_FN(1, COROTATIONAL), SFEM_FN(1, NEO_HOOKEAN),
+        SFEM_FN(2, LINEAR), SFEM_FN(2, STVK), SFEM_FN(2, COROTATIONAL), SFEM_FN(2, NEO_HOOKEAN),
+#undef SFEM_FN
     };
+    const size_t need[] = {lds_bytes<0>(a), lds_bytes<1>(a), lds_bytes<2>(a)};
     if (elem_lds_max(a) <= 64 * 1024) return hipSuccess;
-    for (const void* f : fns) {
-        hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
+    int dev = 0, optin = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    e = hipDeviceGetAttribute(&optin, hipDeviceAttributeSharedMemPerBlockOptin, dev);
+    if (e != hipSuccess) return e;
+    for (int k = 0; k < 12; k++) {
+        hipFuncAttributes at;
+        e = hipFuncGetAttributes(&at, fns[k]);
+        if (e != hipSuccess) return e;
+        const size_t n = need[k / 4];
+        if (n <= 64 * 1024) continue;
+        if (n + at.sharedSizeBytes > (size_t)optin) return hipErrorInvalidValue;   // mesh too irregular for one block
+        e = hipFuncSetAttribute(fns[k], hipFuncAttributeMaxDynamicSharedMemorySize, (int)n);
         if (e != hipSuccess) return e;
     }
     return hipSuccess;
@@ -895,9 +915,10 @@ hipError_t launch_cg_init(int pc, int64_t n3, const float* b, float* x, float* r
 }
 
 hipError_t launch_hessvec_cg(int mat, const BlockArgs& a, const float* u32, const float* d32, float* out, float* partial,
-                             const int* done, double* dot_elem, hipStream_t st)
+                             const int* done, double* dot_elem, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1)
 {
-    return launch_elem_mat<0, BLOCK_E>(mat, a, u32, d32, out, partial, nullptr, 1, done, dot_elem, lds_bytes<0>(a), st);
+    return launch_elem_mat<0, BLOCK_E>(mat, a, u32, d32, out, partial, nullptr, 1, done, dot_elem, lds_bytes<0>(a), st,
+                                       ev0, ev1);
 }
 
 hipError_t launch_cg_iter_tail(int pc, int64_t n3, float* x, float* r, float* z, float* p, const float* q, const float* minv,

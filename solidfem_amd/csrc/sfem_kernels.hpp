@@ -64,8 +64,10 @@ hipError_t launch_cg_init(int pc, int64_t n3, const float* b, float* x, float* r
                           const uint8_t* fixed, double* scal, double* part, int* done, hipStream_t st);
 // CG operator apply: element blocks only (single-block rows -> out, shared
 // rows -> partial slots) plus one d.(K d) partial per element block
+// with events ev0/ev1: they record the dispatch's start and end (timing)
 hipError_t launch_hessvec_cg(int mat, const BlockArgs& a, const float* u32, const float* d32, float* out, float* partial,
-                             const int* done, double* dot_elem, hipStream_t st);
+                             const int* done, double* dot_elem, hipStream_t st, hipEvent_t ev0 = nullptr,
+                             hipEvent_t ev1 = nullptr);
 // alpha, x, r, z (summing shared nodes' partial slots), then beta and p
 hipError_t launch_cg_iter_tail(int pc, int64_t n3, float* x, float* r, float* z, float* p, const float* q, const float* minv,
                                const int2* node_q, const float* partial,
