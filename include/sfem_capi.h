@@ -28,9 +28,9 @@ extern "C" {
 /* MaterialModelType values (Include/FemDataStructures.h:84-88) */
 enum { SFEM_LINEAR = 0, SFEM_STVK = 1, SFEM_COROTATIONAL = 2, SFEM_NEO_HOOKEAN = 3 };
 
-/* SimulationType values (Include/FemDataStructures.h:71-77); explicit and
-   implicit dynamics are not on this path */
-enum { SFEM_ST_STATIC = 0, SFEM_ST_QUASI_STATIC = 1 };
+/* SimulationType values (Include/FemDataStructures.h:71-77); implicit
+   dynamics is not on this path */
+enum { SFEM_ST_STATIC = 0, SFEM_ST_QUASI_STATIC = 1, SFEM_ST_EXPLICIT = 2 };
 
 /* Solve modes.
    REFERENCE: SolveNewtonCG exactly (FemPhysicsMatrixFree.cpp:604-628): outer
@@ -87,15 +87,19 @@ typedef struct {
                                   (FemConfig::mAbsNewtonRsidualThreshold); 0 = off */
     int32_t predictor;         /* sfem_step, ST_QUASI_STATIC, CONVERGED only: 1 = start each
                                   load step from the secant extrapolation of the last two
-                                  states (dropped if it inverts an element); 0 = from the
-                                  last state, as Step does */
+                                  states (dropped if it inverts an element; the step is
+                                  redone from the last state if that solve fails); 0 = from
+                                  the last state, as Step does */
+    double time_step;          /* sfem_step, ST_EXPLICIT: Step(dt)'s dt */
+    int32_t substeps;          /* sfem_step, ST_EXPLICIT: FemConfig::mNumSubsteps (>= 1) */
 } sfem_solve_params;
 
 typedef struct {
     int32_t newton_iters;      /* Newton steps taken */
     int32_t cg_iters;          /* total CG iterations */
     int32_t converged;         /* CONVERGED mode: 1 if ||r|| <= max(tol ||r_ext||, abs_tol) */
-    int32_t fallbacks;         /* learned-PC steps rerun with Jacobi */
+    int32_t fallbacks;         /* learned-PC steps rerun with Jacobi, and quasi-static steps
+                                  redone from the last state after the predicted start failed */
     double res0;               /* ||r||_2 at the start (FP64) */
     double res;                /* ||r||_2 at the end (FP64) */
     double ext_norm;           /* ||r_ext||_2 at the end (FP64) */
@@ -136,6 +140,15 @@ int sfem_set_external_forces(sfem_ctx* ctx, const double* f);
 /* ElasticEnergy::ComputeForces (Source/ElasticEnergy.cpp:259-282), FP64, all rows */
 int sfem_internal_forces(sfem_ctx* ctx, double* f);
 
+/* ElasticEnergy::ComputeElementStress (ElasticEnergy.cpp:114-257): first
+   Piola-Kirchhoff stress of every element at the current positions, ne*9
+   doubles row-major, original element order (FP64; the VTK heat map's cell
+   data, FemIO.cpp:317-349) */
+int sfem_element_stress(sfem_ctx* ctx, double* P);
+
+/* Node velocities (Node::vel) of the explicit integrator, nn*3 doubles */
+int sfem_get_velocities(sfem_ctx* ctx, double* v);
+
 /* ComputeGradients (FemPhysicsMatrixFree.cpp:536-560), FP64; fixed rows 0 */
 int sfem_gradients(sfem_ctx* ctx, double* r);
 
@@ -151,8 +164,11 @@ int sfem_block_diagonal(sfem_ctx* ctx, double* B);
    load fraction, from the current positions */
 int sfem_solve(sfem_ctx* ctx, const sfem_solve_params* p, sfem_solve_stats* st);
 
-/* Step(0) (FemPhysicsMatrixFree.cpp:297-342): ST_STATIC solves once at full
-   load; ST_QUASI_STATIC raises the load fraction by force_step and solves */
+/* Step(dt) (FemPhysicsMatrixFree.cpp:297-342): ST_STATIC solves once at full
+   load; ST_QUASI_STATIC raises the load fraction by force_step and solves;
+   ST_EXPLICIT runs `substeps` symplectic Euler substeps of time_step/substeps
+   (SubStep, FemPhysicsMatrixFree.cpp:371-387: v += h f_int / m + h g,
+   x += h v on free nodes, elastic forces only) */
 int sfem_step(sfem_ctx* ctx, const sfem_solve_params* p, sfem_solve_stats* st);
 
 /* Installs per-node 3x3 inverse preconditioner blocks (nn*9 FP32 row-major,

@@ -93,6 +93,21 @@ def _f64(a):
     return np.ascontiguousarray(a, dtype=np.float64)
 
 
+def explicit_steps(problem, x, v, h, nsub, gravity):
+    """SubStep (FemPhysicsMatrixFree.cpp:371-387) restated: nsub symplectic Euler
+    substeps v += h f_int / m + h g, x += h v on free nodes (elastic forces
+    only), with the oracle's FP64 internal forces and lumped masses."""
+    x = _f64(x).reshape(-1, 3).copy()
+    v = _f64(v).reshape(-1, 3).copy()
+    free = problem.fixed == 0
+    g = np.asarray(gravity, dtype=np.float64)
+    for _ in range(nsub):
+        f = problem.internal_forces(x)
+        v[free] += (h / problem.mass[free])[:, None] * f[free] + h * g
+        x[free] += h * v[free]
+    return x, v
+
+
 def stress(mat, mu, lam, F):
     F = _f64(F).reshape(9)
     P = np.zeros(9)
@@ -252,6 +267,7 @@ def reflib():
             ("ref_matvec", [C.c_void_p, _dp, _dp], None),
             ("ref_cg", [C.c_void_p, _dp, _dp, C.c_int], C.c_int),
             ("ref_step", [C.c_void_p], None),
+            ("ref_step_dt", [C.c_void_p, C.c_double], None),
             ("ref_newton_cg", [C.c_void_p, C.c_double], None),
             ("ref_set_force_fraction", [C.c_void_p, C.c_double], None),
             ("ref_force_fraction", [C.c_void_p], C.c_double),
@@ -268,7 +284,7 @@ class RefProblem:
     Uniform material only (the reference has one mu, lambda); gravity along
     z when zup (FemPhysicsBase.cpp:30-33)."""
 
-    ST_STATIC, ST_QUASI_STATIC = 0, 1
+    ST_STATIC, ST_QUASI_STATIC, ST_EXPLICIT = 0, 1, 2
 
     def __init__(self, X0, tets, fixed, material, young, poisson, density=1070.0, gravity=-9.8,
                  zup=True, simtype=0, force_step=0.1, outer=10, inner=100, tris=None, pressure=0.0):
@@ -333,6 +349,9 @@ class RefProblem:
 
     def step(self):
         reflib().ref_step(self.h)
+
+    def step_dt(self, dt):
+        reflib().ref_step_dt(self.h, float(dt))
 
     def newton_cg(self, frac):
         reflib().ref_newton_cg(self.h, float(frac))

@@ -165,6 +165,13 @@ void ref_step(void* h)
     c->phys->Step(0);
 }
 
+// Step(dt): for ST_EXPLICIT, mNumSubsteps symplectic Euler substeps (SubStep, :371-387)
+void ref_step_dt(void* h, double dt)
+{
+    RefCtx* c = (RefCtx*)h;
+    c->phys->Step(dt);
+}
+
 // SolveNewtonCG (:604-628) at load fraction frac
 void ref_newton_cg(void* h, double frac)
 {

@@ -677,12 +677,13 @@ int orc_newton_solve(const orc_problem* p, double* x, double frac, double tol, i
         memset(dl, 0, sizeof(double) * n);
         total_cg += cg_impl(p, x, r, dl, max_cg, eta * eta, 1, Binv);
         double R0 = dotv(n, dl, r), alpha = 1.0, Ra = 0;
+        int finite = 0;
         for (int ls = 0; ls < 8; ls++) {
             for (int64_t i = 0; i < n; i++) xt[i] = x[i] + alpha * dl[i];
             orc_gradients(p, xt, frac, rt);
             Ra = dotv(n, dl, rt);
             double rr = dotv(n, rt, rt);
-            int finite = isfinite(Ra) && isfinite(rr);
+            finite = isfinite(Ra) && isfinite(rr);
             if (ls == 7) break;
             if (finite && (fabs(Ra) <= 0.5 * fabs(R0) || Ra * R0 > 0)) break;
             if (finite) {
@@ -692,6 +693,15 @@ int orc_newton_solve(const orc_problem* p, double* x, double frac, double tol, i
                 alpha = a;
             } else alpha *= 0.5;
         }
+        /* never accept a state with inverted elements: shorten the step until
+           the residual is finite, or stop at the current state */
+        for (int k = 0; !finite && k < 40; k++) {
+            alpha *= 0.5;
+            for (int64_t i = 0; i < n; i++) xt[i] = x[i] + alpha * dl[i];
+            orc_gradients(p, xt, frac, rt);
+            finite = isfinite(dotv(n, dl, rt)) && isfinite(dotv(n, rt, rt));
+        }
+        if (!finite) break;
         memcpy(x, xt, sizeof(double) * n);
         memcpy(r, rt, sizeof(double) * n);
         orc_external(p, x, frac, ext);

@@ -44,7 +44,8 @@ def build_pysolidfem():
     host = os.path.join(CSRC, "host")
     cmd = ["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-fvisibility=hidden",
            f"-I{pybind11.get_include()}", f"-I{sysconfig.get_paths()['include']}", f"-I{INCLUDE}", f"-I{host}",
-           os.path.join(host, "FemPhysicsMatrixFree.cpp"), os.path.join(host, "pysolidfem.cpp"),
+           os.path.join(host, "FemPhysicsMatrixFree.cpp"), os.path.join(host, "FemIO.cpp"),
+           os.path.join(host, "pysolidfem.cpp"),
            f"-L{HERE}", "-lsfem_hip", "-Wl,-rpath,$ORIGIN", "-o", out + ".tmp"]
     _run(cmd)
     os.replace(out + ".tmp", out)

@@ -14,7 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libsfem_hip.so")
 
 LINEAR, STVK, COROTATIONAL, NEO_HOOKEAN = 0, 1, 2, 3
-ST_STATIC, ST_QUASI_STATIC = 0, 1
+ST_STATIC, ST_QUASI_STATIC, ST_EXPLICIT = 0, 1, 2
 SOLVE_REFERENCE, SOLVE_CONVERGED = 0, 1
 PC_NONE, PC_JACOBI, PC_LEARNED = 0, 1, 2
 ENOTCONV = -4
@@ -22,7 +22,8 @@ ENOTCONV = -4
 EXPORTS = [
     "sfem_create", "sfem_destroy", "sfem_last_error", "sfem_set_positions", "sfem_get_positions",
     "sfem_set_load_fraction", "sfem_get_load_fraction", "sfem_set_pressure_surface",
-    "sfem_set_external_forces", "sfem_internal_forces", "sfem_gradients", "sfem_matvec",
+    "sfem_set_external_forces", "sfem_internal_forces", "sfem_element_stress", "sfem_get_velocities",
+    "sfem_gradients", "sfem_matvec",
     "sfem_block_diagonal", "sfem_solve", "sfem_step", "sfem_set_precond_blocks", "sfem_info",
     "sfem_reset", "sfem_profile_hessvec", "sfem_time_hessvec",
     "sfem_gnn_create", "sfem_gnn_destroy", "sfem_gnn_num_params", "sfem_gnn_get_params", "sfem_gnn_set_params",
@@ -50,7 +51,8 @@ class SolveParams(C.Structure):
     _fields_ = [("mode", C.c_int32), ("precond", C.c_int32), ("outer_iterations", C.c_int32),
                 ("inner_iterations", C.c_int32), ("tol", C.c_double), ("cg_rtol", C.c_double),
                 ("line_search", C.c_int32), ("check_every", C.c_int32), ("sim_type", C.c_int32),
-                ("force_step", C.c_double), ("abs_tol", C.c_double), ("predictor", C.c_int32)]
+                ("force_step", C.c_double), ("abs_tol", C.c_double), ("predictor", C.c_int32),
+                ("time_step", C.c_double), ("substeps", C.c_int32)]
 
 
 class SolveStats(C.Structure):
@@ -91,6 +93,8 @@ def lib():
             "sfem_set_pressure_surface": ([V, _ip, C.c_int64, C.c_double], C.c_int),
             "sfem_set_external_forces": ([V, _dp], C.c_int),
             "sfem_internal_forces": ([V, _dp], C.c_int),
+            "sfem_element_stress": ([V, _dp], C.c_int),
+            "sfem_get_velocities": ([V, _dp], C.c_int),
             "sfem_gradients": ([V, _dp], C.c_int),
             "sfem_matvec": ([V, _dp, _dp], C.c_int),
             "sfem_block_diagonal": ([V, _dp], C.c_int),
@@ -137,9 +141,10 @@ def _p(a, t=_dp):
 
 
 def solve_params(mode=SOLVE_CONVERGED, precond=PC_JACOBI, outer=50, inner=20000, tol=1e-8, cg_rtol=1e-1,
-                 line_search=1, check_every=16, sim_type=ST_STATIC, force_step=0.1, abs_tol=0.0, predictor=1):
+                 line_search=1, check_every=16, sim_type=ST_STATIC, force_step=0.1, abs_tol=0.0, predictor=1,
+                 time_step=0.016, substeps=1):
     return SolveParams(mode, precond, outer, inner, tol, cg_rtol, line_search, check_every, sim_type, force_step,
-                       abs_tol, predictor)
+                       abs_tol, predictor, time_step, substeps)
 
 
 class Solver:
@@ -220,6 +225,17 @@ class Solver:
         f = np.zeros((self.nn, 3))
         _check(lib().sfem_internal_forces(self.h, _p(f)))
         return f
+
+    def element_stress(self):
+        """first Piola-Kirchhoff stress per element (ne, 3, 3), original element order"""
+        P = np.zeros((self.ne, 3, 3))
+        _check(lib().sfem_element_stress(self.h, _p(P)))
+        return P
+
+    def velocities(self):
+        v = np.zeros((self.nn, 3))
+        _check(lib().sfem_get_velocities(self.h, _p(v)))
+        return v
 
     def gradients(self):
         r = np.zeros((self.nn, 3))

@@ -79,7 +79,8 @@ def solver(d):
     return s
 
 
-def run(cfg, precond=capi.PC_JACOBI, tol=1e-8, grid=None, max_newton=50):
+def run(cfg, precond=capi.PC_JACOBI, tol=1e-8, grid=None, max_newton=50, log=None):
+    """Solves workload `cfg`; `log(str)` (optional) gets one line per load step."""
     d = build(cfg, grid)
     s = solver(d)
     steps = d["load_steps"]
@@ -88,9 +89,12 @@ def run(cfg, precond=capi.PC_JACOBI, tol=1e-8, grid=None, max_newton=50):
     out = {"config": cfg, "description": DESCRIPTIONS[cfg], "tets": d["mesh"].ne, "nodes": d["mesh"].nn,
            "steps": []}
     t0 = time.perf_counter()
-    for _ in range(steps):
+    for k in range(steps):
         st = s.step(p, allow_notconv=True)
         out["steps"].append({k: st[k] for k in ("newton_iters", "cg_iters", "converged", "res", "ext_norm", "seconds")})
+        if log:
+            log(f"workload {cfg} load step {k + 1}/{steps}: newton {st['newton_iters']} cg {st['cg_iters']} "
+                f"rel residual {st['res'] / st['ext_norm']:.2e} ({st['seconds']:.2f} s)")
     out["wall_s"] = time.perf_counter() - t0
     out["newton_iters"] = sum(x["newton_iters"] for x in out["steps"])
     out["cg_iters"] = sum(x["cg_iters"] for x in out["steps"])

@@ -20,7 +20,8 @@ FemPhysicsBase::FemPhysicsBase(const FemConfig& config)
     : mYoungsModulus(config.mYoungsModulus), mPoissonRatio(config.mPoissonRatio), mDensity(config.mDensity),
       mSimType(config.mSimType), mMaterial(config.mMaterial), mOuterIterations(config.mOuterIterations),
       mInnerIterations(config.mInnerIterations), mForceStep(config.mForceApplicationStep),
-      mAbsNewtonResidualThreshold(config.mAbsNewtonRsidualThreshold), mVerbose(config.mVerbose)
+      mAbsNewtonResidualThreshold(config.mAbsNewtonRsidualThreshold), mVerbose(config.mVerbose),
+      mNumSteps(config.mNumSubsteps)
 {
     mGravity = config.mZUpAxis ? Vector3R(0, 0, config.mGravity) : Vector3R(0, config.mGravity, 0);
     mLameLambda = real(mYoungsModulus * mPoissonRatio / (1.0 + mPoissonRatio) / (1.0 - 2.0 * mPoissonRatio));
@@ -35,8 +36,8 @@ FemPhysicsMatrixFree::FemPhysicsMatrixFree(const std::vector<Tet>& tetrahedra, c
     if (mMaterial > MMT_NEO_HOOKEAN)
         throw std::runtime_error("FemPhysicsMatrixFree: only LINEAR, STVK, COROTATIONAL and NEO_HOOKEAN are on the "
                                  "matrix-free Newton-CG path");
-    if (mSimType != ST_STATIC && mSimType != ST_QUASI_STATIC)
-        throw std::runtime_error("FemPhysicsMatrixFree: only static and quasi-static simulation are on this path");
+    if (mSimType == ST_IMPLICIT)
+        throw std::runtime_error("FemPhysicsMatrixFree: implicit dynamics is not on this path");
     const size_t nn = mNodes.size();
     // ReshuffleFixedNodes (FemPhysicsMatrixFree.cpp:229-263): fixed nodes first
     mFixed.resize(nn);
@@ -165,6 +166,17 @@ Vector3R FemPhysicsMatrixFree::GetDeformedPosition(uint32 idx, bool shuffle) con
 }
 
 Vector3R FemPhysicsMatrixFree::GetInitialPosition(uint32 idx) const { return mNodes[idx].pos0; }
+
+void FemPhysicsMatrixFree::GetElementStresses(std::vector<Matrix3R>& P) const
+{
+    PushPositions();
+    std::vector<double> buf(9 * mTets.size());
+    Check(sfem_element_stress(mCtx, buf.data()), "sfem_element_stress");
+    P.resize(mTets.size());
+    for (size_t e = 0; e < mTets.size(); e++)
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) P[e].m[i][j] = buf[9 * e + 3 * i + j];
+}
 
 void FemPhysicsMatrixFree::SetDeformedPosition(uint32 idx, Vector3R val, bool shuffle)
 {
@@ -311,9 +323,18 @@ sfem_solve_params FemPhysicsMatrixFree::Params() const
 }
 
 // Step (FemPhysicsMatrixFree.cpp:297-342), static and quasi-static branches
-void FemPhysicsMatrixFree::Step(real)
+void FemPhysicsMatrixFree::Step(real dt)
 {
-    if (mSimType == ST_STATIC) {
+    if (mSimType == ST_EXPLICIT) {
+        // mNumSteps substeps of symplectic Euler (SubStep, FemPhysicsMatrixFree.cpp:371-387), on the device
+        PushPositions();
+        sfem_solve_params p = Params();
+        p.time_step = dt;
+        p.substeps = std::max(1, mNumSteps);
+        mForceFraction = 1;
+        Check(sfem_step(mCtx, &p, &mStats), "sfem_step");
+        mDeviceDirty = true;
+    } else if (mSimType == ST_STATIC) {
         if (mForceFraction == 0) {
             mForceFraction = 1;
             Solve();

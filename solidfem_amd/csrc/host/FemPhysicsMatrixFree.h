@@ -129,6 +129,7 @@ protected:
     Vector3R mGravity;
     real mDensity;
     SimulationType mSimType;
+    int mNumSteps = 1;   // FemConfig::mNumSubsteps
     MaterialModelType mMaterial;
     uint32 mOuterIterations, mInnerIterations;
     real mForceFraction = 0;
@@ -185,6 +186,9 @@ public:
     uint32 GetGlobalOriginalIndex(uint32 e, uint32 l) const { return mTets[e].idx[l]; }
     real GetTotalVolume() const override;
     bool CheckForInversion(int verbose = 0) const;
+    // first Piola-Kirchhoff stress per element (ElasticEnergy::ComputeElementStress,
+    // ElasticEnergy.cpp:114-257), original element order, computed on the device
+    void GetElementStresses(std::vector<Matrix3R>& P) const;
 
     // Krylov solver helpers (FemPhysicsMatrixFree.h:85-89)
     void MatrixVectorMultiply(const std::vector<Vector3R>& in, std::vector<Vector3R>& out) const;

@@ -17,9 +17,13 @@
 #include <pybind11/stl.h>
 
 #include <memory>
+#include <sstream>
 #include <stdexcept>
 #include <string>
 
+#include <fstream>
+
+#include "FemIO.h"
 #include "FemPhysicsMatrixFree.h"
 
 namespace py = pybind11;
@@ -50,6 +54,21 @@ NonlinearSolverType parse_solver(py::handle h)
     throw std::invalid_argument("unknown solver '" + s + "'");
 }
 
+// FemBody::Prepare (FemBody.cpp:127-156), which the reference's bindings run
+// on every mesh: rest positions, and tetrahedra with negative signed volume get
+// their first two indices swapped
+void Prepare(std::vector<Node>& nodes, std::vector<Tet>& tets)
+{
+    for (Node& n : nodes) n.pos0 = n.pos;
+    for (Tet& t : tets) {
+        const Vector3R x0 = nodes[t.idx[0]].pos0;
+        const Vector3R d1 = nodes[t.idx[1]].pos0 - x0, d2 = nodes[t.idx[2]].pos0 - x0, d3 = nodes[t.idx[3]].pos0 - x0;
+        const real det = d1[0] * (d2[1] * d3[2] - d2[2] * d3[1]) - d2[0] * (d1[1] * d3[2] - d1[2] * d3[1]) +
+                         d3[0] * (d1[1] * d2[2] - d1[2] * d2[1]);
+        if (det < 0) std::swap(t.idx[0], t.idx[1]);
+    }
+}
+
 class PyNonlinearFEM {
 public:
     PyNonlinearFEM(py::array_t<int, py::array::c_style | py::array::forcecast> tets,
@@ -77,6 +96,7 @@ public:
                 if (pt[4 * e + k] < 0 || (size_t)pt[4 * e + k] >= nn) throw std::invalid_argument("tet index out of range");
                 t[e].idx[k] = (uint32)pt[4 * e + k];
             }
+        Prepare(n, t);
         FemConfig cfg = ParseConfig(config);
         mPhys.reset(new FemPhysicsMatrixFree(t, n, cfg));
         if (config.contains("young_e") || config.contains("poisson_e")) {
@@ -84,6 +104,23 @@ public:
             auto nu = py::cast<std::vector<real>>(config.contains("poisson_e") ? config["poisson_e"] : py::cast(std::vector<real>(ne, cfg.mPoissonRatio)));
             mPhys->SetPerElementMaterial(E, nu);
         }
+    }
+
+    // PyNonlinearFEM(path) (PyNonlinearFEM.cpp:68-92): the <solidfem> scene's
+    // mesh, Dirichlet nodes and FemConfig (IO::LoadFromXmlFile, FemIO.cpp:583-847),
+    // then Neo-Hookean + NST_NEWTON_LS as the reference sets them
+    explicit PyNonlinearFEM(const std::string& path) { InitFromXml(path, nullptr); }
+
+    // PyNonlinearFEM(path, config) (:94-121): the scene's mesh and Dirichlet
+    // nodes; the config dict replaces the scene's FemConfig
+    PyNonlinearFEM(const std::string& path, py::dict config) { InitFromXml(path, &config); }
+
+    // SaveToVTK (:176-184): IO::ExportToVTKHeatMap of the current state
+    void SaveToVTK(const std::string& path) const
+    {
+        std::ofstream out(path);
+        if (!out) throw std::runtime_error("cannot write " + path);
+        IO::ExportToVTKHeatMap(out, *mPhys);
     }
 
     // Step (PyNonlinearFEM.cpp:126-132)
@@ -161,6 +198,100 @@ public:
     bool CheckForInversion() const { return mPhys->CheckForInversion(0); }
 
 private:
+    static std::vector<int> Ints(py::handle text)
+    {
+        std::vector<int> v;
+        if (text.is_none()) return v;
+        std::string s = py::cast<std::string>(text);
+        for (char& ch : s)
+            if (ch == ',') ch = ' ';
+        std::istringstream in(s);
+        int k;
+        while (in >> k) v.push_back(k);
+        return v;
+    }
+
+    void InitFromXml(const std::string& path, py::dict* config)
+    {
+        py::object ET = py::module_::import("xml.etree.ElementTree");
+        py::object root = ET.attr("parse")(path).attr("getroot")();
+        if (py::cast<std::string>(root.attr("tag")) != "solidfem") throw std::invalid_argument(path + ": not a <solidfem> scene");
+        auto attr = [](py::handle el, const char* name) -> py::object { return el.attr("get")(name); };
+        const std::string dir = path.substr(0, path.find_last_of("/\\") + 1);
+        std::vector<Node> nodes;
+        std::vector<Tet> tets;
+        std::vector<int> fixed, tris;
+        FemConfig c;
+        c.mVerbose = false;
+        py::object model = root.attr("find")("model");
+        if (model.is_none()) throw std::invalid_argument(path + ": no <model>");
+        py::object mpath = attr(model, "path"), type = attr(model, "type");
+        if (mpath.is_none() || type.is_none() || py::cast<std::string>(type) != "tet1")
+            throw std::invalid_argument(path + ": only tet1 models are supported");
+        const real scale = attr(model, "scale").is_none() ? 1 : std::stod(py::cast<std::string>(attr(model, "scale")));
+        if (!IO::LoadFromTet1File((dir + py::cast<std::string>(mpath)).c_str(), scale, Vector3R(), nodes, tets))
+            throw std::invalid_argument(path + ": cannot read the model");
+        if (!attr(model, "swap").is_none()) {
+            const std::string sw = py::cast<std::string>(attr(model, "swap"));
+            const int a = sw == "yz" ? 1 : 0, b = sw == "xy" ? 1 : 2;
+            if (sw == "xy" || sw == "xz" || sw == "yz")
+                for (Node& n : nodes) {
+                    std::swap(n.pos[a], n.pos[b]);
+                    n.pos0 = n.pos;
+                }
+        }
+        py::object dir_el = model.attr("find")("dirichlet");
+        if (!dir_el.is_none()) fixed = Ints(dir_el.attr("text"));
+        py::object mat = root.attr("find")("material");
+        if (!mat.is_none()) {
+            if (!attr(mat, "young").is_none()) c.mYoungsModulus = std::stod(py::cast<std::string>(attr(mat, "young")));
+            if (!attr(mat, "poisson").is_none()) c.mPoissonRatio = std::stod(py::cast<std::string>(attr(mat, "poisson")));
+            if (!attr(mat, "density").is_none()) c.mDensity = std::stod(py::cast<std::string>(attr(mat, "density")));
+        }
+        real pressure = 0;
+        py::object sim = root.attr("find")("simulation");
+        if (!sim.is_none()) {
+            const std::string st = attr(sim, "type").is_none() ? "" : py::cast<std::string>(attr(sim, "type"));
+            if (st == "quasi-static") c.mSimType = ST_QUASI_STATIC;
+            else if (st == "explicit") c.mSimType = ST_EXPLICIT;
+            if (!attr(sim, "method").is_none() && py::cast<std::string>(attr(sim, "method")) == "mixed")
+                throw std::invalid_argument("the mixed (pressure) formulation is not on the matrix-free path");
+            if (!attr(sim, "steps").is_none()) {
+                const int n = std::stoi(py::cast<std::string>(attr(sim, "steps")));
+                if (c.mSimType < ST_EXPLICIT) c.mForceApplicationStep = real(1) / n;
+                else c.mNumSubsteps = n;
+            }
+            if (!attr(sim, "gravity").is_none()) c.mGravity = std::stod(py::cast<std::string>(attr(sim, "gravity")));
+            py::object pr = sim.attr("find")("pressure");
+            if (!pr.is_none()) {
+                if (!attr(pr, "p").is_none()) pressure = std::stod(py::cast<std::string>(attr(pr, "p")));
+                tris = Ints(pr.attr("text"));
+            }
+        }
+        py::object sol = root.attr("find")("solver");
+        if (!sol.is_none()) {
+            if (!attr(sol, "iterations").is_none()) c.mOuterIterations = std::stoi(py::cast<std::string>(attr(sol, "iterations")));
+            if (!attr(sol, "tolerance").is_none())
+                c.mAbsNewtonRsidualThreshold = std::stod(py::cast<std::string>(attr(sol, "tolerance")));
+        }
+        if (!root.attr("find")("cable").is_none()) throw std::invalid_argument("cables are not on the matrix-free path");
+        for (int i : fixed) {
+            if (i < 0 || (size_t)i >= nodes.size()) throw std::invalid_argument("fixed node index out of range");
+            nodes[i].invMass = 0;
+        }
+        if (config) {
+            c = ParseConfig(*config);
+        } else {
+            c.mMaterial = MMT_NEO_HOOKEAN;
+            mCfg.mSolver = NST_NEWTON_LS;
+            c.mInnerIterations = 100000;
+            c.mCustomConfig = &mCfg;
+        }
+        Prepare(nodes, tets);
+        mPhys.reset(new FemPhysicsMatrixFree(tets, nodes, c));
+        if (!tris.empty()) mPhys->SetBoundaryConditionsSurface(std::vector<uint32>(tris.begin(), tris.end()), pressure);
+    }
+
     FemConfig ParseConfig(py::dict config)
     {
         FemConfig c;
@@ -210,6 +341,9 @@ PYBIND11_MODULE(pysolidfem, m)
         .def(py::init<py::array_t<int, py::array::c_style | py::array::forcecast>,
                       py::array_t<double, py::array::c_style | py::array::forcecast>,
                       py::array_t<int, py::array::c_style | py::array::forcecast>, py::dict>())
+        .def(py::init<const std::string&>())
+        .def(py::init<const std::string&, py::dict>())
+        .def("save_to_vtk", &PyNonlinearFEM::SaveToVTK)
         .def("step", &PyNonlinearFEM::Step, py::arg("dt") = DT)
         .def("simulate_static", &PyNonlinearFEM::SimulateStatic)
         .def("get_nodes", &PyNonlinearFEM::GetNodes)

@@ -180,6 +180,10 @@ void build_partition(sfem_ctx* c, const sfem_mesh* m, const std::vector<double>&
         }
         c->d_mat32.upload(ml);
     }
+    // sorted element -> original element (per-element outputs in caller order)
+    std::vector<int> eorig(ne);
+    for (int64_t s2 = 0; s2 < ne; s2++) eorig[s2] = (int)key[s2].second;
+    c->d_eorig.upload(eorig);
     c->d_bnode_ptr.upload(bnode_ptr);
     c->d_bnodes.upload(bnodes);
     c->d_lptr.upload(lptr);
@@ -430,18 +434,32 @@ int solve_converged(sfem_ctx* c, const sfem_solve_params* p, sfem_solve_stats* s
         const double R0 = dots3(c, c->d_r64.as<double>(), nullptr, c->d_x32.as<float>()).da;
         double alpha = 1.0;
         Dots dt{0, 0, 0};
+        bool finite = false;
         const int ntrial = p->line_search ? 8 : 1;
-        for (int ls = 0; ls < ntrial; ls++) {
+        auto trial = [&]() {
             HIPCK(launch_axpy_state(n3, c->d_u64.as<double>(), c->d_x32.as<float>(), alpha, c->d_ut64.as<double>(),
                                     c->d_u32.as<float>(), c->stream));
             residual(c, c->d_ut64.as<double>(), c->d_f64tmp.as<double>());
             st->force_evals += c->ne;
             dt = dots3(c, c->d_f64tmp.as<double>(), c->d_ext64.as<double>(), c->d_x32.as<float>());
+            finite = std::isfinite(dt.da) && std::isfinite(dt.aa);
+        };
+        for (int ls = 0; ls < ntrial; ls++) {
+            trial();
             const double Ra = dt.da;
-            const bool finite = std::isfinite(Ra) && std::isfinite(dt.aa);
             if (!p->line_search || ls == ntrial - 1) break;
             if (finite && (std::fabs(Ra) <= 0.5 * std::fabs(R0) || Ra * R0 > 0)) break;
             alpha = finite ? std::min(std::max(alpha * R0 / (R0 - Ra), 0.1 * alpha), 0.9 * alpha) : 0.5 * alpha;
+        }
+        // never accept a state with inverted elements (non-finite residual):
+        // shorten the step until it is finite, or stop at the current state
+        for (int k = 0; !finite && k < 40; k++) {
+            alpha *= 0.5;
+            trial();
+        }
+        if (!finite) {
+            HIPCK(launch_to_f32(n3, c->d_u64.as<double>(), c->d_u32.as<float>(), c->stream));
+            break;
         }
         std::swap(c->d_u64.p, c->d_ut64.p);
         std::swap(c->d_r64.p, c->d_f64tmp.p);
@@ -535,6 +553,7 @@ int sfem_create(const sfem_mesh* m, const sfem_material* mat, sfem_ctx** out)
         hipStream_t s = c->stream;
         zero<double>(c->d_u64, n3, s);
         zero<double>(c->d_uprev, n3, s);
+        zero<double>(c->d_vel, n3, s);
         zero<double>(c->d_ut64, n3, s);
         zero<float>(c->d_u32, n3, s);
         zero<double>(c->d_r64, n3, s);
@@ -682,6 +701,28 @@ int sfem_internal_forces(sfem_ctx* c, double* f)
     });
 }
 
+int sfem_element_stress(sfem_ctx* c, double* P)
+{
+    if (!c || !P) return fail(SFEM_EINVAL, "sfem_element_stress: null argument");
+    return guard_call("sfem_element_stress", [&]() -> int {
+        DevBuf out;
+        out.alloc((size_t)9 * c->ne * sizeof(double));
+        HIPCK(launch_elem_stress(c->material, c->blk, c->d_u64.as<double>(), c->d_eorig.as<int>(), out.as<double>(), c->stream));
+        HIPCK(hipMemcpyAsync(P, out.p, (size_t)9 * c->ne * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPCK(hipStreamSynchronize(c->stream));
+        return SFEM_OK;
+    });
+}
+
+int sfem_get_velocities(sfem_ctx* c, double* v)
+{
+    if (!c || !v) return fail(SFEM_EINVAL, "sfem_get_velocities: null argument");
+    return guard_call("sfem_get_velocities", [&]() -> int {
+        HIPCK(hipMemcpy(v, c->d_vel.p, 3 * c->nn * sizeof(double), hipMemcpyDeviceToHost));
+        return SFEM_OK;
+    });
+}
+
 int sfem_gradients(sfem_ctx* c, double* r)
 {
     if (!c || !r) return fail(SFEM_EINVAL, "sfem_gradients: null argument");
@@ -731,6 +772,7 @@ int sfem_reset(sfem_ctx* c)
     return guard_call("sfem_reset", [&]() -> int {
         HIPCK(hipMemsetAsync(c->d_u64.p, 0, 3 * c->nn * sizeof(double), c->stream));
         HIPCK(hipMemsetAsync(c->d_u32.p, 0, 3 * c->nn * sizeof(float), c->stream));
+        HIPCK(hipMemsetAsync(c->d_vel.p, 0, 3 * c->nn * sizeof(double), c->stream));
         HIPCK(hipStreamSynchronize(c->stream));
         c->frac = 0.0;
         c->has_prev = false;
@@ -821,6 +863,7 @@ int sfem_step(sfem_ctx* c, const sfem_solve_params* p, sfem_solve_stats* st)
         if (c->frac < 1) {
             const double f_old = c->frac, f_new = std::min(1.0, c->frac + p->force_step);
             int extra = 0;
+            bool predicted = false;
             int rc = guard_call("sfem_step", [&]() -> int {
                 const int64_t n3 = 3 * c->nn;
                 double* u = c->d_u64.as<double>();
@@ -838,7 +881,8 @@ int sfem_step(sfem_ctx* c, const sfem_solve_params* p, sfem_solve_stats* st)
                     residual(c, u, r);
                     const double r1 = dots3(c, r, nullptr, nullptr).aa;
                     extra = 1;
-                    if (!std::isfinite(r1)) {
+                    predicted = std::isfinite(r1);
+                    if (!predicted) {
                         HIPCK(hipMemcpyAsync(u, c->d_uprev.p, n3 * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
                         HIPCK(launch_to_f32(n3, u, c->d_u32.as<float>(), c->stream));
                     }
@@ -851,13 +895,57 @@ int sfem_step(sfem_ctx* c, const sfem_solve_params* p, sfem_solve_stats* st)
             });
             if (rc != SFEM_OK) return rc;
             rc = sfem_solve(c, p, st);
+            if (predicted && rc == SFEM_ENOTCONV) {
+                // the solve from the predicted state failed: redo the step from the
+                // last state (what Step does), adding up the work of both attempts
+                const sfem_solve_stats first = *st;
+                rc = guard_call("sfem_step", [&]() -> int {
+                    HIPCK(hipMemcpyAsync(c->d_u64.p, c->d_uprev.p, 3 * c->nn * sizeof(double), hipMemcpyDeviceToDevice,
+                                         c->stream));
+                    HIPCK(launch_to_f32(3 * c->nn, c->d_u64.as<double>(), c->d_u32.as<float>(), c->stream));
+                    return SFEM_OK;
+                });
+                if (rc != SFEM_OK) return rc;
+                rc = sfem_solve(c, p, st);
+                st->newton_iters += first.newton_iters;
+                st->cg_iters += first.cg_iters;
+                st->fallbacks += first.fallbacks + 1;
+                st->seconds += first.seconds;
+                st->hessvec_evals += first.hessvec_evals;
+                st->force_evals += first.force_evals;
+                st->res0 = first.res0;
+            }
             st->force_evals += (int64_t)extra * c->ne;
             return rc;
         }
         std::memset(st, 0, sizeof *st);
         return SFEM_OK;
     }
-    return fail(SFEM_EINVAL, "sfem_step: only static and quasi-static simulation types are on this path");
+    if (p->sim_type == SFEM_ST_EXPLICIT) {
+        // Step for ST_EXPLICIT (FemPhysicsMatrixFree.cpp:316-323) and SubStep (:371-387)
+        if (!(p->time_step > 0) || p->substeps < 1) return fail(SFEM_EINVAL, "sfem_step: explicit needs time_step > 0, substeps >= 1");
+        std::memset(st, 0, sizeof *st);
+        return guard_call("sfem_step", [&]() -> int {
+            HIPCK(hipStreamSynchronize(c->stream));
+            auto t0 = std::chrono::steady_clock::now();
+            c->frac = 1.0;
+            c->has_prev = false;
+            const double h = p->time_step / p->substeps;
+            double* part = reinterpret_cast<double*>(c->d_partial.as<char>() + (size_t)c->nrows_partial * 9 * sizeof(float));
+            for (int k = 0; k < p->substeps; k++) {
+                HIPCK(launch_force(c->material, c->blk, c->shr, c->d_u64.as<double>(), c->d_f64tmp.as<double>(), part,
+                                   nullptr, 0, c->stream));
+                HIPCK(launch_explicit(c->nn, c->d_u64.as<double>(), c->d_vel.as<double>(), c->d_f64tmp.as<double>(),
+                                      c->d_mass.as<double>(), c->d_fixed.as<uint8_t>(), c->gravity, h, c->d_u32.as<float>(),
+                                      c->stream));
+            }
+            HIPCK(hipStreamSynchronize(c->stream));
+            st->force_evals = (int64_t)p->substeps * c->ne;
+            st->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+            return SFEM_OK;
+        });
+    }
+    return fail(SFEM_EINVAL, "sfem_step: implicit dynamics is not on this path");
 }
 
 }  // extern "C"

@@ -118,7 +118,7 @@ struct sfem_ctx {
     DevBuf d_partial;        // nrows_partial * 9 floats (also used as FP64 x3)
 
     // node data
-    DevBuf d_uprev, d_X0, d_mass, d_fixed, d_fdead;
+    DevBuf d_uprev, d_vel, d_eorig, d_X0, d_mass, d_fixed, d_fdead;
     DevBuf d_u64, d_ut64, d_u32;
     DevBuf d_r64, d_ext64, d_f64tmp;
     DevBuf d_b32, d_x32, d_r32, d_z32, d_p32, d_q32, d_d32, d_minv, d_minv_learned, d_bd32;

@@ -480,7 +480,7 @@ __global__ __launch_bounds__(256) void k_shared_gather(SharedArgs s, const T* __
 
 // x = 0, r = b, z = M r, p = z; partials of r.z, r.r
 template <int PC>
-__global__ __launch_bounds__(256) void k_cg_init(int64_t n3, const float* __restrict__ b, float* __restrict__ x,
+__global__ __launch_bounds__(CG_BLOCK) void k_cg_init(int64_t n3, const float* __restrict__ b, float* __restrict__ x,
                                                  float* __restrict__ r, float* __restrict__ z, float* __restrict__ p,
                                                  const float* __restrict__ minv, const uint8_t* __restrict__ fixed,
                                                  double* __restrict__ part)
@@ -529,7 +529,7 @@ __global__ __launch_bounds__(256) void k_cg_init_finish(const double* __restrict
 // and nodes of no element (node_q: {first slot, end slot} of a shared node;
 // {-1, 0} free node of one block, {-2, 0} fixed or of no element)
 template <int PC>
-__global__ __launch_bounds__(256) void k_cg_update(int64_t n3, float* __restrict__ x, float* __restrict__ r,
+__global__ __launch_bounds__(CG_BLOCK) void k_cg_update(int64_t n3, float* __restrict__ x, float* __restrict__ r,
                                                    float* __restrict__ z, const float* __restrict__ p,
                                                    const float* __restrict__ q, const float* __restrict__ minv,
                                                    const int2* __restrict__ node_q, const float* __restrict__ partial,
@@ -613,7 +613,7 @@ __global__ __launch_bounds__(256) void k_cg_update(int64_t n3, float* __restrict
 // test 0: the reference criterion |r.z| <= eps^2 |r0.z0| (IterativeSolvers.h:180-185)
 // test 1: true residual ||r||^2 <= tol^2 ||r0||^2
 template <int PC>
-__global__ __launch_bounds__(256) void k_cg_p(int64_t n3, const float* __restrict__ z, const float* __restrict__ r,
+__global__ __launch_bounds__(CG_BLOCK) void k_cg_p(int64_t n3, const float* __restrict__ z, const float* __restrict__ r,
                                               float* __restrict__ p, double* scal, const double* __restrict__ part, int n_part,
                                               int* done, int test_kind, double tol2)
 {
@@ -677,6 +677,61 @@ __global__ __launch_bounds__(256) void k_axpy_state(int64_t n3, const double* __
         ut[i] = v;
         u32[i] = (float)v;
     }
+}
+
+// explicit substep on free nodes: v += h f / m + h g; u += h v; u32 = (float) u
+__global__ __launch_bounds__(256) void k_explicit(int64_t nn, double* __restrict__ u, double* __restrict__ vel,
+                                                  const double* __restrict__ f, const double* __restrict__ mass,
+                                                  const uint8_t* __restrict__ fixed, double gx, double gy, double gz,
+                                                  double h, float* __restrict__ u32)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nn; i += (int64_t)gridDim.x * blockDim.x) {
+        if (fixed[i]) continue;
+        const double g[3] = {gx, gy, gz};
+        const double hm = h / mass[i];
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const double v = vel[3 * i + c] + hm * f[3 * i + c] + h * g[c];
+            vel[3 * i + c] = v;
+            const double x = u[3 * i + c] + h * v;
+            u[3 * i + c] = x;
+            u32[3 * i + c] = (float)x;
+        }
+    }
+}
+
+// first Piola-Kirchhoff stress per element (FP64), written at the element's
+// original index: the coefficients hold V mu, V lambda and P is linear in
+// (mu, lambda), so P = stress(F, V mu, V lambda) / V, V = 1 / (6 det Dm^-1)
+template <int MAT>
+__global__ __launch_bounds__(256) void k_elem_stress(BlockArgs a, const double* __restrict__ u,
+                                                     const int* __restrict__ eorig, double* __restrict__ out)
+{
+    const int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (s >= a.ne) return;
+    const size_t st = (size_t)a.ne_pad;
+    double g[9];
+#pragma unroll
+    for (int k = 0; k < 9; k++) g[k] = a.dminv64[k * st + s];
+    const uint2 li = a.lidx[s];
+    const int n0 = a.bnode_ptr[s / BLOCK_E];
+    const int l[4] = {(int)(li.x & 0xffffu), (int)(li.x >> 16), (int)(li.y & 0xffffu), (int)(li.y >> 16)};
+    double uu[4][3];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const uint32_t node = a.bnodes[n0 + l[k]] & NODE_MASK;
+#pragma unroll
+        for (int c = 0; c < 3; c++) uu[k][c] = u[3 * (size_t)node + c];
+    }
+    double F[9], P[9];
+    defgrad(g, uu, F);
+    stress<double, MAT>(F, a.coef64[s], a.coef64[st + s], P);
+    const double detg = g[0] * (g[4] * g[8] - g[5] * g[7]) - g[1] * (g[3] * g[8] - g[5] * g[6]) +
+                        g[2] * (g[3] * g[7] - g[4] * g[6]);
+    const double invV = 6.0 * detg;
+    double* o = out + 9 * (size_t)eorig[s];
+#pragma unroll
+    for (int k = 0; k < 9; k++) o[k] = P[k] * invV;
 }
 
 // load-step predictor: uprev <- u, u <- u + s (u - uprev), u32 <- (float) u
@@ -902,14 +957,14 @@ hipError_t set_elem_lds_limits(const BlockArgs& a)
     return hipSuccess;
 }
 
-int cg_grid(int64_t nn) { return grid_for(nn, 256, CG_GRID); }
+int cg_grid(int64_t nn) { return grid_for(nn, CG_BLOCK, CG_GRID); }
 
 hipError_t launch_cg_init(int pc, int64_t n3, const float* b, float* x, float* r, float* z, float* p, const float* minv,
                           const uint8_t* fixed, double* scal, double* part, int* done, hipStream_t st)
 {
     const int g = cg_grid(n3 / 3);
-    if (pc) hipLaunchKernelGGL(k_cg_init<1>, dim3(g), dim3(256), 0, st, n3, b, x, r, z, p, minv, fixed, part);
-    else hipLaunchKernelGGL(k_cg_init<0>, dim3(g), dim3(256), 0, st, n3, b, x, r, z, p, minv, fixed, part);
+    if (pc) hipLaunchKernelGGL(k_cg_init<1>, dim3(g), dim3(CG_BLOCK), 0, st, n3, b, x, r, z, p, minv, fixed, part);
+    else hipLaunchKernelGGL(k_cg_init<0>, dim3(g), dim3(CG_BLOCK), 0, st, n3, b, x, r, z, p, minv, fixed, part);
     hipLaunchKernelGGL(k_cg_init_finish, dim3(1), dim3(256), 0, st, part, g, scal, done);
     return hipGetLastError();
 }
@@ -927,10 +982,10 @@ hipError_t launch_cg_iter_tail(int pc, int64_t n3, float* x, float* r, float* z,
                                double tol2, int guard, hipStream_t st)
 {
     const int g = cg_grid(n3 / 3);
-    if (pc) hipLaunchKernelGGL(k_cg_update<1>, dim3(g), dim3(256), 0, st, n3, x, r, z, p, q, minv, node_q, partial, scal, pq_part, n_pq, part, done, guard);
-    else hipLaunchKernelGGL(k_cg_update<0>, dim3(g), dim3(256), 0, st, n3, x, r, z, p, q, minv, node_q, partial, scal, pq_part, n_pq, part, done, guard);
-    if (pc) hipLaunchKernelGGL(k_cg_p<1>, dim3(g), dim3(256), 0, st, n3, z, r, p, scal, part, g, done, test_kind, tol2);
-    else hipLaunchKernelGGL(k_cg_p<0>, dim3(g), dim3(256), 0, st, n3, z, r, p, scal, part, g, done, test_kind, tol2);
+    if (pc) hipLaunchKernelGGL(k_cg_update<1>, dim3(g), dim3(CG_BLOCK), 0, st, n3, x, r, z, p, q, minv, node_q, partial, scal, pq_part, n_pq, part, done, guard);
+    else hipLaunchKernelGGL(k_cg_update<0>, dim3(g), dim3(CG_BLOCK), 0, st, n3, x, r, z, p, q, minv, node_q, partial, scal, pq_part, n_pq, part, done, guard);
+    if (pc) hipLaunchKernelGGL(k_cg_p<1>, dim3(g), dim3(CG_BLOCK), 0, st, n3, z, r, p, scal, part, g, done, test_kind, tol2);
+    else hipLaunchKernelGGL(k_cg_p<0>, dim3(g), dim3(CG_BLOCK), 0, st, n3, z, r, p, scal, part, g, done, test_kind, tol2);
     return hipGetLastError();
 }
 
@@ -946,6 +1001,26 @@ hipError_t launch_dots3(int64_t n3, const double* a, const double* b, const floa
 hipError_t launch_axpy_state(int64_t n3, const double* u, const float* dx, double alpha, double* ut, float* u32, hipStream_t st)
 {
     hipLaunchKernelGGL(k_axpy_state, dim3(grid_for(n3, 256, 8192)), dim3(256), 0, st, n3, u, dx, alpha, ut, u32);
+    return hipGetLastError();
+}
+
+hipError_t launch_explicit(int64_t nn, double* u, double* vel, const double* f, const double* mass, const uint8_t* fixed,
+                           const double* g, double h, float* u32, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_explicit, dim3(grid_for(nn, 256, 8192)), dim3(256), 0, st, nn, u, vel, f, mass, fixed, g[0], g[1],
+                       g[2], h, u32);
+    return hipGetLastError();
+}
+
+hipError_t launch_elem_stress(int mat, const BlockArgs& a, const double* u, const int* eorig, double* out, hipStream_t st)
+{
+    const dim3 grid((unsigned)((a.ne + 255) / 256));
+    switch (mat) {
+    case LINEAR: hipLaunchKernelGGL(k_elem_stress<LINEAR>, grid, dim3(256), 0, st, a, u, eorig, out); break;
+    case STVK: hipLaunchKernelGGL(k_elem_stress<STVK>, grid, dim3(256), 0, st, a, u, eorig, out); break;
+    case COROTATIONAL: hipLaunchKernelGGL(k_elem_stress<COROTATIONAL>, grid, dim3(256), 0, st, a, u, eorig, out); break;
+    default: hipLaunchKernelGGL(k_elem_stress<NEO_HOOKEAN>, grid, dim3(256), 0, st, a, u, eorig, out); break;
+    }
     return hipGetLastError();
 }
 

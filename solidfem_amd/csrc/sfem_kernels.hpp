@@ -12,6 +12,11 @@ constexpr int RED_GRID = 2048;   // reduction partials buffer holds 3 * RED_GRID
 // workgroups of the CG vector kernels; each of them re-reduces the previous
 // kernel's partials, so this stays small
 constexpr int CG_GRID = 1024;
+// threads per workgroup of the CG vector kernels (one node per thread)
+#ifndef SFEM_CG_BLOCK
+#define SFEM_CG_BLOCK 256
+#endif
+constexpr int CG_BLOCK = SFEM_CG_BLOCK;
 
 // FP64 scalars kept on the device during a CG solve
 enum Scal { S_RZ = 0, S_RZ0, S_RR, S_RR0, S_PQ, S_ALPHA, S_BETA, S_ITER, S_NT, S_NT2, S_NT3, S_RZ_PREV, S_COUNT = 16 };
@@ -76,6 +81,10 @@ hipError_t launch_cg_iter_tail(int pc, int64_t n3, float* x, float* r, float* z,
 hipError_t launch_dots3(int64_t n3, const double* a, const double* b, const float* d, double* scal, int slot, double* part,
                         hipStream_t st);
 hipError_t launch_axpy_state(int64_t n3, const double* u, const float* dx, double alpha, double* ut, float* u32, hipStream_t st);
+hipError_t launch_explicit(int64_t nn, double* u, double* vel, const double* f, const double* mass, const uint8_t* fixed,
+                           const double* g, double h, float* u32, hipStream_t st);
+// P per element at displacement u, 9 doubles at the element's original index
+hipError_t launch_elem_stress(int mat, const BlockArgs& a, const double* u, const int* eorig, double* out, hipStream_t st);
 hipError_t launch_extrapolate(int64_t n3, double* u, double* uprev, double sc, float* u32, hipStream_t st);
 hipError_t launch_to_f32(int64_t n, const double* a, float* b, hipStream_t st);
 hipError_t launch_external(int64_t nn, const double* mass, const double* fdead, const double* g, double frac, const int* pptr,

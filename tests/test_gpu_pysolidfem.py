@@ -66,3 +66,51 @@ def test_simulate_static_quasi_static_and_lame():
     assert abs(mu - 0.5e6 / 1.45) < 1e-6 and abs(lam - 1e6 * 0.45 / 1.45 / 0.1) < 1e-4
     fem.set_lame_params(2 * mu, 2 * lam)
     assert abs(fem.get_shear_modulus() - 2 * mu) < 1e-6
+
+
+def test_xml_scene_through_bindings(tmp_path):
+    """NonlinearFEM(path) on the reference's own cantilever.xml / box2.tet:
+    Neo-Hookean + NST_NEWTON_LS as the reference's path constructor sets them,
+    gravity along y (mZUpAxis false), |r| <= 0.1 N (the scene's tolerance);
+    the result against the oracle's FP64 solve, then save_to_vtk."""
+    import pysolidfem
+    from solidfem_amd import io
+    fem = pysolidfem.NonlinearFEM(os.path.join(GOLD, "cantilever.xml"))
+    fem.simulate_static()
+    st = fem.last_stats()
+    assert st["converged"] == 1 and st["res"] <= 0.1
+    sc = io.load_xml(os.path.join(GOLD, "cantilever.xml"))
+    X, T = sc["nodes"], io.prepare(sc["nodes"], sc["tets"])
+    fixed = np.zeros(len(X), dtype=np.uint8)
+    fixed[sc["fixed"]] = 1
+    c = sc["config"]
+    p = O.OracleProblem(X, T, fixed, O.NEO_HOOKEAN, c["young"], c["poisson"], c["density"], gravity=(0, -9.8, 0))
+    x = fem.get_nodes()
+    # the device stopped at |r| <= 0.1 N: the FP64 oracle's residual at the
+    # device state says the same, and the state is near the oracle's converged
+    # one (stopping at 0.1 N leaves ~2e-3 relative, as the oracle itself shows)
+    r_oracle = np.linalg.norm(p.gradients(x, 1.0))
+    xo, sto = p.newton_solve(X, 1.0, tol=1e-10)
+    err = np.linalg.norm(x - xo) / np.linalg.norm(xo - X)
+    print(f"cantilever.xml via bindings: sag {(x - X)[:, 1].min():.4e} m, err {err:.2e}, |r| device "
+          f"{st['res']:.3e} oracle {r_oracle:.3e}")
+    assert abs(r_oracle - st["res"]) <= 1e-6 * max(st["res"], 1e-3) and r_oracle <= 0.1
+    assert err < 2e-2
+    out = tmp_path / "c.vtk"
+    fem.save_to_vtk(str(out))
+    r = io.read_vtk(str(out))
+    assert np.allclose(r["points"], x, rtol=0, atol=1e-15) and np.array_equal(r["cells"], T)
+    assert np.allclose(r["displacement"], x - X, rtol=1e-6, atol=1e-12)
+    assert r["stress"].shape == (len(T), 3, 3) and np.isfinite(r["stress"]).all()
+
+
+def test_explicit_through_bindings():
+    """config simtype 2 (ST_EXPLICIT) with substeps: step(dt) runs the device
+    SubStep; against the oracle's restatement."""
+    m = BeamMesh(6, 2, 3, 0.6, 0.2, 0.3)
+    fem = _fem(m, dict(simtype=2, substeps=4, material="stvk", young=2e5, poisson=0.3, density=900.0))
+    for _ in range(3):
+        fem.step(8e-3)
+    p = O.OracleProblem(m.X, m.tets, m.fixed_mask(), O.STVK, 2e5, 0.3, 900.0, gravity=(0, 0, -9.8))
+    x, _ = O.explicit_steps(p, m.X, np.zeros_like(m.X), 2e-3, 12, (0.0, 0.0, -9.8))
+    assert np.linalg.norm(fem.get_nodes() - x) / np.linalg.norm(x - m.X) < 1e-11

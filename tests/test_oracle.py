@@ -111,6 +111,38 @@ def test_live_reference_random_state(oracle_lib, mat):
     assert rel(s2, s1) < 1e-5
 
 
+@pytest.mark.skipif(not O.ref_available(), reason="compiled reference not built here")
+@pytest.mark.parametrize("mat", MATS)
+def test_explicit_substeps_match_live_reference(oracle_lib, mat):
+    """SubStep (symplectic Euler, FemPhysicsMatrixFree.cpp:371-387): the
+    restatement O.explicit_steps against the compiled reference's own Step(dt)
+    in ST_EXPLICIT, over several steps from rest."""
+    m = BeamMesh(6, 2, 3, 0.6, 0.2, 0.3)
+    fixed = m.fixed_mask()
+    ref = O.RefProblem(m.X, m.tets, fixed, mat, 2e5, 0.3, 900.0, -9.8, zup=True, simtype=O.RefProblem.ST_EXPLICIT)
+    p = O.OracleProblem(m.X, m.tets, fixed, mat, 2e5, 0.3, 900.0, gravity=(0, 0, -9.8))
+    x, v = m.X.copy(), np.zeros_like(m.X)
+    for k in range(5):
+        ref.step_dt(2e-3)
+        x, v = O.explicit_steps(p, x, v, 2e-3, 1, (0.0, 0.0, -9.8))
+    xr = ref.positions()
+    assert rel(x - m.X, xr - m.X) < 1e-12
+    # the beam has moved: not a trivial comparison (gravity alone gives 5 h^2 g.. per node)
+    assert np.abs(xr - m.X).max() > 1e-5
+
+
+def test_explicit_free_fall_closed_form(oracle_lib):
+    """Without elastic forces (zero moduli) every free node falls as
+    x_n = x_0 + h^2 g n(n+1)/2 (symplectic Euler on a constant acceleration)."""
+    m = BeamMesh(3, 2, 2)
+    fixed = np.zeros(m.nn, dtype=np.uint8)
+    p = O.OracleProblem(m.X, m.tets, fixed, O.LINEAR, mu=np.zeros(m.ne), lam=np.zeros(m.ne), density=1000.0)
+    h, n = 1e-2, 7
+    x, v = O.explicit_steps(p, m.X, np.zeros_like(m.X), h, n, (0.0, -2.0, 0.0))
+    assert np.allclose(x[:, 1] - m.X[:, 1], -2.0 * h * h * n * (n + 1) / 2, rtol=0, atol=1e-15)
+    assert np.allclose(v[:, 1], -2.0 * h * n)
+
+
 # ------------------------------------------------------------ mathematics
 def _rand_F(rng, scale=0.2):
     return np.eye(3) + scale * rng.standard_normal((3, 3))
