@@ -28,9 +28,8 @@ extern "C" {
 /* MaterialModelType values (Include/FemDataStructures.h:84-88) */
 enum { SFEM_LINEAR = 0, SFEM_STVK = 1, SFEM_COROTATIONAL = 2, SFEM_NEO_HOOKEAN = 3 };
 
-/* SimulationType values (Include/FemDataStructures.h:71-77); implicit
-   dynamics is not on this path */
-enum { SFEM_ST_STATIC = 0, SFEM_ST_QUASI_STATIC = 1, SFEM_ST_EXPLICIT = 2 };
+/* SimulationType values (Include/FemDataStructures.h:71-77) */
+enum { SFEM_ST_STATIC = 0, SFEM_ST_QUASI_STATIC = 1, SFEM_ST_EXPLICIT = 2, SFEM_ST_IMPLICIT = 3 };
 
 /* Solve modes.
    REFERENCE: SolveNewtonCG exactly (FemPhysicsMatrixFree.cpp:604-628): outer
@@ -90,8 +89,8 @@ typedef struct {
                                   states (dropped if it inverts an element; the step is
                                   redone from the last state if that solve fails); 0 = from
                                   the last state, as Step does */
-    double time_step;          /* sfem_step, ST_EXPLICIT: Step(dt)'s dt */
-    int32_t substeps;          /* sfem_step, ST_EXPLICIT: FemConfig::mNumSubsteps (>= 1) */
+    double time_step;          /* sfem_step, ST_EXPLICIT / ST_IMPLICIT: Step(dt)'s dt */
+    int32_t substeps;          /* sfem_step, ST_EXPLICIT / ST_IMPLICIT: FemConfig::mNumSubsteps (>= 1) */
 } sfem_solve_params;
 
 typedef struct {
@@ -168,7 +167,12 @@ int sfem_solve(sfem_ctx* ctx, const sfem_solve_params* p, sfem_solve_stats* st);
    load; ST_QUASI_STATIC raises the load fraction by force_step and solves;
    ST_EXPLICIT runs `substeps` symplectic Euler substeps of time_step/substeps
    (SubStep, FemPhysicsMatrixFree.cpp:371-387: v += h f_int / m + h g,
-   x += h v on free nodes, elastic forces only) */
+   x += h v on free nodes, elastic forces only); ST_IMPLICIT (:299-314) saves
+   x0 = x, then runs `substeps` solves with h = time_step/substeps: operator
+   M/h^2 + K (MatrixVectorMultiply :523-534), residual + m/h v - m/h^2 (x - x0)
+   (ComputeGradients :536-560), v = (x - x0)/h after each (:589-597).  After an
+   implicit step h stays set, as mTimeStep does (sfem_matvec and
+   sfem_gradients include the dynamic terms); sfem_reset clears it. */
 int sfem_step(sfem_ctx* ctx, const sfem_solve_params* p, sfem_solve_stats* st);
 
 /* Installs per-node 3x3 inverse preconditioner blocks (nn*9 FP32 row-major,

@@ -108,6 +108,51 @@ def explicit_steps(problem, x, v, h, nsub, gravity):
     return x, v
 
 
+class ImplicitStep:
+    """Backward-Euler terms of FemPhysicsMatrixFree with mTimeStep = h
+    restated around an OracleProblem: residual r(x) = r_static(x) + m/h v -
+    m/h^2 (x - x0) and operator K d + m/h^2 d on free nodes
+    (ComputeGradients / MatrixVectorMultiply, FemPhysicsMatrixFree.cpp:523-560),
+    and a dense FP64 Newton solve of r(x) = 0 for small meshes."""
+
+    def __init__(self, problem, x0, v, h, frac=1.0):
+        self.p, self.h, self.frac = problem, float(h), frac
+        self.x0 = _f64(x0).reshape(-1, 3).copy()
+        self.v = _f64(v).reshape(-1, 3).copy()
+        self.free = problem.fixed == 0
+        self.m = problem.mass
+
+    def residual(self, x):
+        r = self.p.gradients(x, self.frac)
+        f = self.free
+        r[f] += (self.m[f] / self.h)[:, None] * self.v[f] - (self.m[f] / self.h ** 2)[:, None] * (x[f] - self.x0[f])
+        return r
+
+    def matvec(self, x, d):
+        q = self.p.matvec(x, d)
+        f = self.free
+        q[f] += (self.m[f] / self.h ** 2)[:, None] * d[f]
+        return q
+
+    def solve(self, x, tol=1e-12, max_newton=30):
+        """Newton with the operator assembled column by column (tiny meshes)"""
+        x = _f64(x).reshape(-1, 3).copy()
+        dofs = np.flatnonzero(np.repeat(self.free, 3))
+        for it in range(max_newton):
+            r = self.residual(x)
+            if np.linalg.norm(r) <= tol * max(np.linalg.norm(self.residual(self.x0)), 1e-300):
+                return x, it
+            K = np.zeros((dofs.size, dofs.size))
+            for j, dof in enumerate(dofs):
+                e = np.zeros(x.size)
+                e[dof] = 1.0
+                K[:, j] = self.matvec(x, e.reshape(-1, 3)).reshape(-1)[dofs]
+            dx = np.zeros(x.size)
+            dx[dofs] = np.linalg.solve(K, r.reshape(-1)[dofs])
+            x += dx.reshape(-1, 3)
+        return x, max_newton
+
+
 def stress(mat, mu, lam, F):
     F = _f64(F).reshape(9)
     P = np.zeros(9)
@@ -284,7 +329,7 @@ class RefProblem:
     Uniform material only (the reference has one mu, lambda); gravity along
     z when zup (FemPhysicsBase.cpp:30-33)."""
 
-    ST_STATIC, ST_QUASI_STATIC, ST_EXPLICIT = 0, 1, 2
+    ST_STATIC, ST_QUASI_STATIC, ST_EXPLICIT, ST_IMPLICIT = 0, 1, 2, 3
 
     def __init__(self, X0, tets, fixed, material, young, poisson, density=1070.0, gravity=-9.8,
                  zup=True, simtype=0, force_step=0.1, outer=10, inner=100, tris=None, pressure=0.0):

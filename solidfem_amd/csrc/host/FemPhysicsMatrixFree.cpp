@@ -36,8 +36,6 @@ FemPhysicsMatrixFree::FemPhysicsMatrixFree(const std::vector<Tet>& tetrahedra, c
     if (mMaterial > MMT_NEO_HOOKEAN)
         throw std::runtime_error("FemPhysicsMatrixFree: only LINEAR, STVK, COROTATIONAL and NEO_HOOKEAN are on the "
                                  "matrix-free Newton-CG path");
-    if (mSimType == ST_IMPLICIT)
-        throw std::runtime_error("FemPhysicsMatrixFree: implicit dynamics is not on this path");
     const size_t nn = mNodes.size();
     // ReshuffleFixedNodes (FemPhysicsMatrixFree.cpp:229-263): fixed nodes first
     mFixed.resize(nn);
@@ -325,7 +323,19 @@ sfem_solve_params FemPhysicsMatrixFree::Params() const
 // Step (FemPhysicsMatrixFree.cpp:297-342), static and quasi-static branches
 void FemPhysicsMatrixFree::Step(real dt)
 {
-    if (mSimType == ST_EXPLICIT) {
+    if (mSimType == ST_IMPLICIT) {
+        // mNumSteps backward-Euler solves on the device (Step :299-314, Solve :589-597).
+        // Unlike the reference, the nodes' pos0 keep the rest positions (the
+        // reference overwrites them with the step's start positions).
+        PushPositions();
+        sfem_solve_params p = Params();
+        p.time_step = dt;
+        p.substeps = std::max(1, mNumSteps);
+        mForceFraction = 1;
+        const int rc = sfem_step(mCtx, &p, &mStats);
+        if (rc != SFEM_OK && rc != SFEM_ENOTCONV) Check(rc, "sfem_step");
+        mDeviceDirty = true;
+    } else if (mSimType == ST_EXPLICIT) {
         // mNumSteps substeps of symplectic Euler (SubStep, FemPhysicsMatrixFree.cpp:371-387), on the device
         PushPositions();
         sfem_solve_params p = Params();

@@ -228,6 +228,9 @@ void build_partition(sfem_ctx* c, const sfem_mesh* m, const std::vector<double>&
     a.dminv64 = c->d_dminv64.as<double>();
     a.coef64 = c->d_coef64.as<double>();
     a.rel32 = c->d_rel32.as<float>();
+    a.mdiag32 = nullptr;
+    a.mdiag64 = nullptr;
+    a.node_q = c->d_node_info.as<int2>();
     a.mat32 = uniform ? nullptr : c->d_mat32.as<float>();
     a.mu0 = (float)mu[0];
     a.lam0 = (float)lam[0];
@@ -243,6 +246,26 @@ template <typename T> void zero(DevBuf& b, size_t n, hipStream_t s)
     HIPCK(hipMemsetAsync(b.p, 0, n * sizeof(T), s));
 }
 
+// Implicit dynamics time step h (0: static): m / h^2 per node for the element
+// kernels (operator diagonal, residual inertia) and the load kernel
+void set_time_step(sfem_ctx* c, double h)
+{
+    c->tstep = h;
+    if (h > 0) {
+        HIPCK(launch_mass_diag(c->nn, c->d_mass.as<double>(), c->d_fixed.as<uint8_t>(), 1.0 / (h * h), c->d_mdiag32.as<float>(),
+                               c->d_mdiag64.as<double>(), c->stream));
+        c->blk.mdiag32 = c->d_mdiag32.as<float>();
+        c->blk.mdiag64 = c->d_mdiag64.as<double>();
+    } else {
+        c->blk.mdiag32 = nullptr;
+        c->blk.mdiag64 = nullptr;
+    }
+}
+
+// Newton steps a predicted load step gets to fall below the residual of the
+// unpredicted start before it is abandoned
+constexpr int PRED_PATIENCE = 4;
+
 // ---------------------------------------------------------------- operations
 
 // residual r = f_int + ext on free rows (0 on fixed) at displacement u (device)
@@ -252,10 +275,12 @@ void residual(sfem_ctx* c, const double* u, double* r)
     if (c->ntri)
         HIPCK(launch_pressure(c->ntri, c->d_tris.as<int>(), c->d_X0.as<double>(), u, c->pressure, c->frac,
                               c->d_ftri.as<double>(), c->stream));
+    const bool dyn = c->tstep > 0;
     HIPCK(launch_external(nn, c->d_mass.as<double>(), c->has_fdead ? c->d_fdead.as<double>() : nullptr, c->gravity, c->frac,
                           c->ntri ? c->d_pptr.as<int>() : nullptr, c->ntri ? c->d_pidx.as<int>() : nullptr,
-                          c->ntri ? c->d_ftri.as<double>() : nullptr, c->d_fixed.as<uint8_t>(), c->d_ext64.as<double>(),
-                          c->stream));
+                          c->ntri ? c->d_ftri.as<double>() : nullptr, c->d_fixed.as<uint8_t>(),
+                          dyn ? c->d_vel.as<double>() : nullptr, dyn ? c->d_u0step.as<double>() : nullptr,
+                          dyn ? 1.0 / c->tstep : 0.0, c->d_ext64.as<double>(), c->stream));
     double* part = reinterpret_cast<double*>(c->d_partial.as<char>() + (size_t)c->nrows_partial * 9 * sizeof(float));
     HIPCK(launch_force(c->material, c->blk, c->shr, u, r, part, c->d_ext64.as<double>(), 1, c->stream));
 }
@@ -468,6 +493,8 @@ int solve_converged(sfem_ctx* c, const sfem_solve_params* p, sfem_solve_stats* s
         en = std::sqrt(dt.bb);
         st->newton_iters++;
         if (!std::isfinite(rn)) break;
+        // a predicted start that is not paying off (see sfem_step)
+        if (c->abort_after > 0 && st->newton_iters >= c->abort_after && rn > c->abort_rn && rn > target(en)) break;
     }
     // u32 must match the accepted state
     HIPCK(launch_to_f32(n3, c->d_u64.as<double>(), c->d_u32.as<float>(), c->stream));
@@ -554,6 +581,9 @@ int sfem_create(const sfem_mesh* m, const sfem_material* mat, sfem_ctx** out)
         zero<double>(c->d_u64, n3, s);
         zero<double>(c->d_uprev, n3, s);
         zero<double>(c->d_vel, n3, s);
+        zero<double>(c->d_u0step, n3, s);
+        zero<float>(c->d_mdiag32, (size_t)nn, s);
+        zero<double>(c->d_mdiag64, (size_t)nn, s);
         zero<double>(c->d_ut64, n3, s);
         zero<float>(c->d_u32, n3, s);
         zero<double>(c->d_r64, n3, s);
@@ -693,7 +723,10 @@ int sfem_internal_forces(sfem_ctx* c, double* f)
     if (!c || !f) return fail(SFEM_EINVAL, "sfem_internal_forces: null argument");
     return guard_call("sfem_internal_forces", [&]() -> int {
         double* part = reinterpret_cast<double*>(c->d_partial.as<char>() + (size_t)c->nrows_partial * 9 * sizeof(float));
-        HIPCK(launch_force(c->material, c->blk, c->shr, c->d_u64.as<double>(), c->d_f64tmp.as<double>(), part, nullptr, 0,
+        BlockArgs a = c->blk;   // elastic forces only, whatever the time step
+        a.mdiag32 = nullptr;
+        a.mdiag64 = nullptr;
+        HIPCK(launch_force(c->material, a, c->shr, c->d_u64.as<double>(), c->d_f64tmp.as<double>(), part, nullptr, 0,
                            c->stream));
         HIPCK(hipMemcpyAsync(f, c->d_f64tmp.p, 3 * c->nn * sizeof(double), hipMemcpyDeviceToHost, c->stream));
         HIPCK(hipStreamSynchronize(c->stream));
@@ -776,6 +809,7 @@ int sfem_reset(sfem_ctx* c)
         HIPCK(hipStreamSynchronize(c->stream));
         c->frac = 0.0;
         c->has_prev = false;
+        set_time_step(c, 0.0);
         return SFEM_OK;
     });
 }
@@ -875,13 +909,18 @@ int sfem_step(sfem_ctx* c, const sfem_solve_params* p, sfem_solve_stats* st)
                     // element (non-finite residual).  Its residual norm is usually
                     // larger than at u (the stiff modes it misses dominate ||r||)
                     // while its error in the soft modes that Newton-CG converges
-                    // slowest is much smaller, so ||r|| is no test of it.
+                    // slowest is much smaller, so ||r|| is no test of it at the
+                    // start; Newton from it must get below ||r(u)|| within
+                    // PRED_PATIENCE steps, else the step is redone from u.
+                    residual(c, u, r);
+                    c->abort_rn = std::sqrt(dots3(c, r, nullptr, nullptr).aa);
                     HIPCK(launch_extrapolate(n3, u, c->d_uprev.as<double>(), (f_new - f_old) / (f_old - c->frac_prev),
                                              c->d_u32.as<float>(), c->stream));
                     residual(c, u, r);
                     const double r1 = dots3(c, r, nullptr, nullptr).aa;
-                    extra = 1;
+                    extra = 2;
                     predicted = std::isfinite(r1);
+                    c->abort_after = predicted ? PRED_PATIENCE : 0;
                     if (!predicted) {
                         HIPCK(hipMemcpyAsync(u, c->d_uprev.p, n3 * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
                         HIPCK(launch_to_f32(n3, u, c->d_u32.as<float>(), c->stream));
@@ -895,6 +934,7 @@ int sfem_step(sfem_ctx* c, const sfem_solve_params* p, sfem_solve_stats* st)
             });
             if (rc != SFEM_OK) return rc;
             rc = sfem_solve(c, p, st);
+            c->abort_after = 0;
             if (predicted && rc == SFEM_ENOTCONV) {
                 // the solve from the predicted state failed: redo the step from the
                 // last state (what Step does), adding up the work of both attempts
@@ -932,8 +972,11 @@ int sfem_step(sfem_ctx* c, const sfem_solve_params* p, sfem_solve_stats* st)
             c->has_prev = false;
             const double h = p->time_step / p->substeps;
             double* part = reinterpret_cast<double*>(c->d_partial.as<char>() + (size_t)c->nrows_partial * 9 * sizeof(float));
+            BlockArgs a = c->blk;   // elastic forces only (ElasticEnergy::ComputeForces)
+            a.mdiag32 = nullptr;
+            a.mdiag64 = nullptr;
             for (int k = 0; k < p->substeps; k++) {
-                HIPCK(launch_force(c->material, c->blk, c->shr, c->d_u64.as<double>(), c->d_f64tmp.as<double>(), part,
+                HIPCK(launch_force(c->material, a, c->shr, c->d_u64.as<double>(), c->d_f64tmp.as<double>(), part,
                                    nullptr, 0, c->stream));
                 HIPCK(launch_explicit(c->nn, c->d_u64.as<double>(), c->d_vel.as<double>(), c->d_f64tmp.as<double>(),
                                       c->d_mass.as<double>(), c->d_fixed.as<uint8_t>(), c->gravity, h, c->d_u32.as<float>(),
@@ -945,7 +988,47 @@ int sfem_step(sfem_ctx* c, const sfem_solve_params* p, sfem_solve_stats* st)
             return SFEM_OK;
         });
     }
-    return fail(SFEM_EINVAL, "sfem_step: implicit dynamics is not on this path");
+    if (p->sim_type == SFEM_ST_IMPLICIT) {
+        // Step for ST_IMPLICIT (FemPhysicsMatrixFree.cpp:299-314): x0 = x at the start
+        // of the step; `substeps` solves with M / h^2 in the operator and
+        // m/h v - m/h^2 (x - x0) in the residual; v = (x - x0) / h after each
+        if (!(p->time_step > 0) || p->substeps < 1) return fail(SFEM_EINVAL, "sfem_step: implicit needs time_step > 0, substeps >= 1");
+        const double h = p->time_step / p->substeps;
+        int rc = guard_call("sfem_step", [&]() -> int {
+            const int64_t n3 = 3 * c->nn;
+            HIPCK(hipMemcpyAsync(c->d_u0step.p, c->d_u64.p, n3 * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+            set_time_step(c, h);
+            c->frac = 1.0;
+            c->has_prev = false;
+            return SFEM_OK;
+        });
+        if (rc != SFEM_OK) return rc;
+        sfem_solve_stats tot{};
+        for (int k = 0; k < p->substeps; k++) {
+            rc = sfem_solve(c, p, st);
+            if (rc != SFEM_OK && rc != SFEM_ENOTCONV) return rc;
+            const int rc2 = guard_call("sfem_step", [&]() -> int {
+                HIPCK(launch_implicit_vel(c->nn, c->d_u64.as<double>(), c->d_u0step.as<double>(), c->d_fixed.as<uint8_t>(),
+                                          1.0 / h, c->d_vel.as<double>(), c->stream));
+                HIPCK(hipStreamSynchronize(c->stream));
+                return SFEM_OK;
+            });
+            if (rc2 != SFEM_OK) return rc2;
+            tot.newton_iters += st->newton_iters;
+            tot.cg_iters += st->cg_iters;
+            tot.fallbacks += st->fallbacks;
+            tot.seconds += st->seconds;
+            tot.hessvec_evals += st->hessvec_evals;
+            tot.force_evals += st->force_evals;
+            if (k == 0) tot.res0 = st->res0;
+            tot.res = st->res;
+            tot.ext_norm = st->ext_norm;
+            tot.converged = st->converged;
+        }
+        *st = tot;
+        return rc;
+    }
+    return fail(SFEM_EINVAL, "sfem_step: unknown simulation type");
 }
 
 }  // extern "C"

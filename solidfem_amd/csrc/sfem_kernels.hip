@@ -135,7 +135,8 @@ template <> struct KindTraits<2> {
 //       sf4[(corner * CW + j) * BE + element]   (CW = 1 for W = 3, 3 for W = 9)
 //   staged node rows: su4[nmax] (and sd4[nmax], sx4[nmax]) = (x, y, z, 0);
 //   sx4: rest coordinates relative to the block's centre
-//   node flags s_flag[nmax] (bnodes entry) and partial slots s_slot[nmax]
+//   per row m / h^2 for dynamics s_md[nmax] (T), node flags s_flag[nmax]
+//   (bnodes entry) and partial slots s_slot[nmax]
 //   incidence entries s_ent[4 BE] = corner * BE + element, offsets s_ptr[nmax+1]
 template <typename T> struct Vec4;
 template <> struct Vec4<float> { typedef float4 V; };
@@ -154,7 +155,7 @@ size_t lds_layout_bytes(int BE, int nmax)
     constexpr int W = KindTraits<KIND>::W;
     const size_t v4 = 4 * sizeof(T);
     const size_t rows = (size_t)nmax * (1 + (KindTraits<KIND>::HAS_D ? 1 : 0) + (KindTraits<KIND>::HAS_X ? 1 : 0));
-    return v4 * ((size_t)4 * corner_vec4<KIND>(W) * BE + rows) + 2 * sizeof(uint32_t) * (size_t)nmax +
+    return v4 * ((size_t)4 * corner_vec4<KIND>(W) * BE + rows) + (2 * sizeof(uint32_t) + sizeof(T)) * (size_t)nmax +
            sizeof(uint16_t) * (4 * (size_t)BE + nmax + 1) + 16;
 }
 
@@ -164,7 +165,9 @@ __global__ __launch_bounds__(BE) void k_elem_block(BlockArgs a, const typename K
                                                     typename KindTraits<KIND>::T* __restrict__ out,
                                                     typename KindTraits<KIND>::T* __restrict__ partial,
                                                     const double* __restrict__ ext, int mask_fixed,
-                                                    const int* __restrict__ done, double* __restrict__ dot_part)
+                                                    const int* __restrict__ done, double* __restrict__ dot_part,
+                                                    const typename KindTraits<KIND>::T* __restrict__ mdiag,
+                                                    const int2* __restrict__ node_q)
 {
     typedef typename KindTraits<KIND>::T T;
     typedef typename Vec4<T>::V V4;
@@ -178,7 +181,8 @@ __global__ __launch_bounds__(BE) void k_elem_block(BlockArgs a, const typename K
     V4* su4 = sf4 + 4 * CW * BE;                      // nmax
     V4* sd4 = su4 + a.nmax;                           // nmax (HAS_D)
     V4* sx4 = sd4 + (HAS_D ? a.nmax : 0);             // nmax (HAS_X)
-    uint32_t* s_flag = reinterpret_cast<uint32_t*>(sx4 + (HAS_X ? a.nmax : 0)); // nmax
+    T* s_md = reinterpret_cast<T*>(sx4 + (HAS_X ? a.nmax : 0));                  // nmax (mdiag)
+    uint32_t* s_flag = reinterpret_cast<uint32_t*>(s_md + a.nmax);                // nmax
     int* s_slot = reinterpret_cast<int*>(s_flag + a.nmax);                        // nmax
     uint16_t* s_ent = reinterpret_cast<uint16_t*>(s_slot + a.nmax);               // 4*BE
     uint16_t* s_ptr = s_ent + 4 * BE;                                             // nmax+1
@@ -238,6 +242,10 @@ __global__ __launch_bounds__(BE) void k_elem_block(BlockArgs a, const typename K
             dv.x = dp[0]; dv.y = dp[1]; dv.z = dp[2]; dv.w = T(0);
         }
         const int slot = (fl & FLAG_SHARED) ? a.bslot[n0 + j] : -1;
+        // dynamics: the row of the node's first block carries m / h^2 (the
+        // node's partial rows add up to one mass term)
+        T md = T(0);
+        if (mdiag && (!(fl & FLAG_SHARED) || node_q[node].x == slot)) md = mdiag[node];
         V4 xv;
         if (HAS_X) {
             const float* xp = a.rel32 + 3 * (size_t)(n0 + j);
@@ -246,6 +254,7 @@ __global__ __launch_bounds__(BE) void k_elem_block(BlockArgs a, const typename K
         su4[j] = uv;
         if (HAS_D) sd4[j] = dv;
         if (HAS_X) sx4[j] = xv;
+        s_md[j] = md;
         s_flag[j] = fl;
         s_slot[j] = slot;
     }
@@ -387,6 +396,20 @@ __global__ __launch_bounds__(BE) void k_elem_block(BlockArgs a, const typename K
         // both lanes of the pair are in the same wave: add the partner's sums
 #pragma unroll
         for (int q = 0; q < 4 * CW; q++) acc[q] += __shfl_xor(acc[q], 1, 64);
+        if (has && half == 0 && mdiag) {
+            // M / h^2 (implicit dynamics, MatrixVectorMultiply FemPhysicsMatrixFree.cpp:523-534;
+            // the residual's -M / h^2 (x - x0), ComputeGradients :553-557)
+            const T md = s_md[j];
+            if (KIND == 0) {
+                const V4 dv = sd4[j];
+                acc[0] += md * dv.x; acc[1] += md * dv.y; acc[2] += md * dv.z;
+            } else if (KIND == 1) {
+                const V4 uv = su4[j];
+                acc[0] -= md * uv.x; acc[1] -= md * uv.y; acc[2] -= md * uv.z;
+            } else {
+                acc[0] += md; acc[4] += md; acc[8] += md;
+            }
+        }
         if (has && half == 0) {
             const uint32_t fl = s_flag[j];
             const uint32_t node = fl & NODE_MASK;
@@ -734,6 +757,27 @@ __global__ __launch_bounds__(256) void k_elem_stress(BlockArgs a, const double* 
     for (int k = 0; k < 9; k++) o[k] = P[k] * invV;
 }
 
+// implicit dynamics: m / h^2 per free node (0 on fixed), FP32 and FP64
+__global__ __launch_bounds__(256) void k_mass_diag(int64_t nn, const double* __restrict__ mass, const uint8_t* __restrict__ fixed,
+                                                   double invh2, float* __restrict__ m32, double* __restrict__ m64)
+{
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= nn) return;
+    const double v = fixed[i] ? 0.0 : mass[i] * invh2;
+    m32[i] = (float)v;
+    m64[i] = v;
+}
+
+// after an implicit solve (Solve, FemPhysicsMatrixFree.cpp:589-597): v = (x - x0) / h on free nodes
+__global__ __launch_bounds__(256) void k_implicit_vel(int64_t nn, const double* __restrict__ u, const double* __restrict__ u0,
+                                                      const uint8_t* __restrict__ fixed, double invh, double* __restrict__ vel)
+{
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= nn || fixed[i]) return;
+#pragma unroll
+    for (int c = 0; c < 3; c++) vel[3 * i + c] = invh * (u[3 * i + c] - u0[3 * i + c]);
+}
+
 // load-step predictor: uprev <- u, u <- u + s (u - uprev), u32 <- (float) u
 __global__ __launch_bounds__(256) void k_extrapolate(int64_t n3, double* __restrict__ u, double* __restrict__ uprev,
                                                      double sc, float* __restrict__ u32)
@@ -758,15 +802,22 @@ __global__ __launch_bounds__(256) void k_external(int64_t nn, const double* __re
                                                   double gx, double gy, double gz, double frac,
                                                   const int* __restrict__ pptr, const int* __restrict__ pidx,
                                                   const double* __restrict__ ftri, const uint8_t* __restrict__ fixed,
-                                                  double* __restrict__ ext)
+                                                  const double* __restrict__ vel, const double* __restrict__ u0,
+                                                  double invh, double* __restrict__ ext)
 {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= nn) return;
     double e[3] = {0.0, 0.0, 0.0};
+    if (!fixed[i] && vel) {
+        // implicit dynamics (ComputeGradients :553-557): m/h v + m/h^2 u0; the
+        // element kernel subtracts m/h^2 u at the row of the node's first block
+#pragma unroll
+        for (int c = 0; c < 3; c++) e[c] = mass[i] * invh * (vel[3 * i + c] + invh * u0[3 * i + c]);
+    }
     if (!fixed[i]) {
         // ComputeGradients (FemPhysicsMatrixFree.cpp:547-552): m frac g
         const double m = mass[i] * frac;
-        e[0] = m * gx; e[1] = m * gy; e[2] = m * gz;
+        e[0] += m * gx; e[1] += m * gy; e[2] += m * gz;
         if (fdead) {
 #pragma unroll
             for (int c = 0; c < 3; c++) e[c] += frac * fdead[3 * i + c];
@@ -845,15 +896,17 @@ static hipError_t launch_elem_mat(int mat, const BlockArgs& a, const void* u, co
 {
     typedef typename KindTraits<KIND>::T T;
     dim3 grid(a.nblk), block(BE);
+    const T* md = (const T*)(sizeof(T) == 8 ? (const void*)a.mdiag64 : (const void*)a.mdiag32);
+    const int2* nq = a.node_q;
     // with events: hipExtLaunchKernelGGL records them at the dispatch's own start
     // and end, so their interval is the kernel's execution time
 #define SFEM_ELEM_CASE(M)                                                                                            \
     if (ev0)                                                                                                         \
         hipExtLaunchKernelGGL((k_elem_block<KIND, M, BE>), grid, block, (uint32_t)lds, s, ev0, ev1, 0u, a,          \
-                              (const T*)u, (const T*)d, (T*)out, (T*)partial, ext, mask_fixed, done, dot);           \
+                              (const T*)u, (const T*)d, (T*)out, (T*)partial, ext, mask_fixed, done, dot, md, nq);   \
     else                                                                                                             \
         hipLaunchKernelGGL((k_elem_block<KIND, M, BE>), grid, block, lds, s, a, (const T*)u, (const T*)d, (T*)out,  \
-                           (T*)partial, ext, mask_fixed, done, dot)
+                           (T*)partial, ext, mask_fixed, done, dot, md, nq)
     switch (mat) {
     case LINEAR: SFEM_ELEM_CASE(LINEAR); break;
     case STVK: SFEM_ELEM_CASE(STVK); break;
@@ -1024,6 +1077,20 @@ hipError_t launch_elem_stress(int mat, const BlockArgs& a, const double* u, cons
     return hipGetLastError();
 }
 
+hipError_t launch_mass_diag(int64_t nn, const double* mass, const uint8_t* fixed, double invh2, float* m32, double* m64,
+                            hipStream_t st)
+{
+    hipLaunchKernelGGL(k_mass_diag, dim3((nn + 255) / 256), dim3(256), 0, st, nn, mass, fixed, invh2, m32, m64);
+    return hipGetLastError();
+}
+
+hipError_t launch_implicit_vel(int64_t nn, const double* u, const double* u0, const uint8_t* fixed, double invh, double* vel,
+                               hipStream_t st)
+{
+    hipLaunchKernelGGL(k_implicit_vel, dim3((nn + 255) / 256), dim3(256), 0, st, nn, u, u0, fixed, invh, vel);
+    return hipGetLastError();
+}
+
 hipError_t launch_extrapolate(int64_t n3, double* u, double* uprev, double sc, float* u32, hipStream_t st)
 {
     hipLaunchKernelGGL(k_extrapolate, dim3(grid_for(n3, 256, 8192)), dim3(256), 0, st, n3, u, uprev, sc, u32);
@@ -1037,9 +1104,11 @@ hipError_t launch_to_f32(int64_t n, const double* a, float* b, hipStream_t st)
 }
 
 hipError_t launch_external(int64_t nn, const double* mass, const double* fdead, const double* g, double frac, const int* pptr,
-                           const int* pidx, const double* ftri, const uint8_t* fixed, double* ext, hipStream_t st)
+                           const int* pidx, const double* ftri, const uint8_t* fixed, const double* vel, const double* u0,
+                           double invh, double* ext, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_external, dim3((nn + 255) / 256), dim3(256), 0, st, nn, mass, fdead, g[0], g[1], g[2], frac, pptr, pidx, ftri, fixed, ext);
+    hipLaunchKernelGGL(k_external, dim3((nn + 255) / 256), dim3(256), 0, st, nn, mass, fdead, g[0], g[1], g[2], frac, pptr, pidx,
+                       ftri, fixed, vel, u0, invh, ext);
     return hipGetLastError();
 }
 

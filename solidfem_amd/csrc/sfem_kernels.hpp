@@ -42,6 +42,11 @@ struct BlockArgs {
     const float* rel32;    // 3 per block row
     const float* mat32;    // 2 x ne_pad: mu, lambda per element; null when uniform
     float mu0, lam0;       // uniform mu, lambda
+    // implicit dynamics: m / h^2 per node (null when static) and each node's
+    // partial-slot range (the first slot's block adds the mass term)
+    const float* mdiag32;
+    const double* mdiag64;
+    const int2* node_q;
 };
 
 // nodes shared by several blocks: partial rows to sum
@@ -87,8 +92,14 @@ hipError_t launch_explicit(int64_t nn, double* u, double* vel, const double* f, 
 hipError_t launch_elem_stress(int mat, const BlockArgs& a, const double* u, const int* eorig, double* out, hipStream_t st);
 hipError_t launch_extrapolate(int64_t n3, double* u, double* uprev, double sc, float* u32, hipStream_t st);
 hipError_t launch_to_f32(int64_t n, const double* a, float* b, hipStream_t st);
+// ext = frac (m g + f_dead) + pressure (+ m/h v + m/h^2 u0 with vel, u0: implicit dynamics)
 hipError_t launch_external(int64_t nn, const double* mass, const double* fdead, const double* g, double frac, const int* pptr,
-                           const int* pidx, const double* ftri, const uint8_t* fixed, double* ext, hipStream_t st);
+                           const int* pidx, const double* ftri, const uint8_t* fixed, const double* vel, const double* u0,
+                           double invh, double* ext, hipStream_t st);
+hipError_t launch_mass_diag(int64_t nn, const double* mass, const uint8_t* fixed, double invh2, float* m32, double* m64,
+                            hipStream_t st);
+hipError_t launch_implicit_vel(int64_t nn, const double* u, const double* u0, const uint8_t* fixed, double invh, double* vel,
+                               hipStream_t st);
 hipError_t launch_pressure(int64_t ntri, const int* tris, const double* X0, const double* u, double pressure, double frac,
                            double* ftri, hipStream_t st);
 hipError_t launch_invert_blocks(int64_t nn, const float* B, const uint8_t* fixed, float* Minv, hipStream_t st);

@@ -43,6 +43,61 @@ def test_explicit_substeps_match_oracle(mat):
     assert np.all(s.velocities() == 0)
 
 
+@pytest.mark.parametrize("mat", [capi.LINEAR, capi.STVK, capi.NEO_HOOKEAN])
+def test_implicit_step_matches_dense_oracle(mat):
+    """Converged backward-Euler steps (ST_IMPLICIT) against O.ImplicitStep's
+    dense FP64 Newton (pinned to the compiled reference's residual and
+    operator in test_oracle.py); then the dynamic terms stay in sfem_gradients
+    / sfem_matvec, as mTimeStep stays set in the reference."""
+    print(f"case implicit mat={mat}", flush=True)
+    m = BeamMesh(4, 2, 2)
+    fixed = m.fixed_mask()
+    g = (0.0, 0.0, -9.8)
+    h = 4e-3
+    s = capi.Solver(m.X, m.tets, fixed, mat, 1e5, 0.3, 1000.0, gravity=g)
+    p = O.OracleProblem(m.X, m.tets, fixed, mat, 1e5, 0.3, 1000.0, gravity=g)
+    prm = capi.solve_params(tol=1e-11, sim_type=capi.ST_IMPLICIT, time_step=2 * h, substeps=1, inner=100000)
+    x, v = m.X.copy(), np.zeros_like(m.X)
+    for k in range(3):
+        st = s.step(prm)
+        assert st["converged"] == 1
+        imp = O.ImplicitStep(p, x, v, 2 * h)
+        xn, _ = imp.solve(x)
+        v = (xn - x) / (2 * h)
+        x = xn
+        assert rel(s.positions() - m.X, x - m.X) < 1e-8, k
+        assert rel(s.velocities(), v) < 1e-7, k
+    # state after the last step: x0 = its start, v = its velocity update
+    x0 = x - 2 * h * v
+    imp = O.ImplicitStep(p, x0, v, 2 * h)
+    xs = s.positions()
+    assert rel(s.gradients(), imp.residual(xs)) < 1e-10
+    d = np.random.default_rng(mat).standard_normal(m.X.shape)
+    assert rel(s.matvec(d), imp.matvec(xs, d)) < 2e-5
+    s.reset()
+    assert rel(s.matvec(d), p.matvec(m.X, d)) < 2e-5      # static operator again
+
+
+@pytest.mark.skipif(not O.ref_available(), reason="compiled reference not shipped")
+def test_implicit_reference_mode_matches_compiled_reference():
+    """REFERENCE mode (SolveNewtonCG: fixed Newton steps, CG to 1e-3) inside
+    implicit Step(dt), against the compiled reference doing the same."""
+    m = BeamMesh(6, 2, 3, 0.6, 0.2, 0.3)
+    fixed = m.fixed_mask()
+    ref = O.RefProblem(m.X, m.tets, fixed, capi.NEO_HOOKEAN, 2e5, 0.3, 900.0, -9.8, zup=True,
+                       simtype=O.RefProblem.ST_IMPLICIT, outer=10, inner=100)
+    s = capi.Solver(m.X, m.tets, fixed, capi.NEO_HOOKEAN, 2e5, 0.3, 900.0, gravity=(0, 0, -9.8))
+    prm = capi.solve_params(mode=capi.SOLVE_REFERENCE, outer=10, inner=100, sim_type=capi.ST_IMPLICIT, time_step=5e-3,
+                            substeps=1)
+    for _ in range(3):
+        ref.step_dt(5e-3)
+        s.step(prm)
+    xr = ref.positions()
+    err = rel(s.positions() - m.X, xr - m.X)
+    print(f"implicit reference mode vs compiled reference: {err:.2e}")
+    assert err < 1e-5
+
+
 @pytest.mark.parametrize("mat", MATS)
 @pytest.mark.parametrize("per_elem", [False, True])
 def test_element_stress_matches_oracle(mat, per_elem):

@@ -131,6 +131,53 @@ def test_explicit_substeps_match_live_reference(oracle_lib, mat):
     assert np.abs(xr - m.X).max() > 1e-5
 
 
+@pytest.mark.skipif(not O.ref_available(), reason="compiled reference not built here")
+@pytest.mark.parametrize("mat", MATS)
+def test_implicit_terms_match_live_reference(oracle_lib, mat):
+    """Backward-Euler residual and operator (mTimeStep = h): O.ImplicitStep
+    against the compiled reference's ComputeGradients / MatrixVectorMultiply
+    after its own implicit Step(dt), at the state and velocities it reached."""
+    m = BeamMesh(6, 2, 3, 0.6, 0.2, 0.3)
+    fixed = m.fixed_mask()
+    h = 5e-3
+    ref = O.RefProblem(m.X, m.tets, fixed, mat, 2e5, 0.3, 900.0, -9.8, zup=True, simtype=O.RefProblem.ST_IMPLICIT,
+                       outer=3, inner=50)
+    p = O.OracleProblem(m.X, m.tets, fixed, mat, 2e5, 0.3, 900.0, gravity=(0, 0, -9.8))
+    ref.step_dt(h)                       # from rest: x0 = X, v = 0 -> x1, v1 = (x1 - X) / h
+    x0 = ref.positions()
+    v0 = (x0 - m.X) / h
+    ref.step_dt(h)                       # the second step starts at x0 with v0
+    x1 = ref.positions()
+    assert np.abs(x1 - x0).max() > 1e-7
+    imp = O.ImplicitStep(p, x0, v0, h)
+    v1 = (x1 - x0) / h
+    # the reference's state after its step: residual and operator at x1 with
+    # x0 = start of that step and v = v1 (its velocity update ran after the solve)
+    imp1 = O.ImplicitStep(p, x0, v1, h)
+    assert rel(imp1.residual(x1), ref.gradients()) < 1e-11
+    d = np.random.default_rng(mat).standard_normal(m.X.shape)
+    assert rel(imp1.matvec(x1, d), ref.matvec(d)) < 1e-12
+    # and the dynamic terms are not negligible in either
+    assert rel(imp.residual(x1), p.gradients(x1, 1.0)) > 1e-3
+    assert rel(imp1.matvec(x1, d), p.matvec(x1, d)) > 1e-3
+
+
+def test_implicit_dense_newton_solves_its_equations(oracle_lib):
+    m = BeamMesh(3, 2, 2)
+    fixed = m.fixed_mask()
+    p = O.OracleProblem(m.X, m.tets, fixed, O.NEO_HOOKEAN, 1e5, 0.3, 1000.0, gravity=(0, 0, -9.8))
+    imp = O.ImplicitStep(p, m.X, np.zeros_like(m.X), 1e-2)
+    x, it = imp.solve(m.X)
+    assert it < 10 and np.linalg.norm(imp.residual(x)) < 1e-10 * np.linalg.norm(imp.residual(m.X))
+    # h -> large: the static equilibrium; h small: barely moves
+    x_s, _ = O.ImplicitStep(p, m.X, np.zeros_like(m.X), 1e3).solve(m.X)
+    xs, st = p.newton_solve(m.X, 1.0, tol=1e-12)
+    assert rel(x_s - m.X, xs - m.X) < 1e-5
+    x_t, _ = O.ImplicitStep(p, m.X, np.zeros_like(m.X), 1e-5).solve(m.X)
+    free = fixed == 0
+    assert np.allclose((x_t - m.X)[free], (1e-5) ** 2 * np.array([0, 0, -9.8]), rtol=1e-2, atol=1e-14)
+
+
 def test_explicit_free_fall_closed_form(oracle_lib):
     """Without elastic forces (zero moduli) every free node falls as
     x_n = x_0 + h^2 g n(n+1)/2 (symplectic Euler on a constant acceleration)."""
