@@ -20,6 +20,10 @@ enum { LINEAR = 0, STVK = 1, COROTATIONAL = 2, NEO_HOOKEAN = 3 };
 
 template <typename T> __device__ __forceinline__ T dsqrt(T x) { return sqrt(x); }
 template <typename T> __device__ __forceinline__ T dlog(T x) { return log(x); }
+// 1/x: FP32 kernels (the CG operator) take the hardware reciprocal (1 ulp);
+// the FP64 kernels divide exactly
+__device__ __forceinline__ float drcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ double drcp(double x) { return 1.0 / x; }
 
 // C = A B (3x3 row-major)
 template <typename T>
@@ -46,7 +50,7 @@ __device__ __forceinline__ T inv3(const T* m, T* v)
     v[5] = m[2] * m[3] - m[0] * m[5];
     v[8] = m[0] * m[4] - m[1] * m[3];
     T det = m[0] * v[0] + m[1] * v[3] + m[2] * v[6];
-    T s = T(1) / det;
+    T s = drcp(det);
 #pragma unroll
     for (int k = 0; k < 9; k++) v[k] *= s;
     return det;
