@@ -71,8 +71,7 @@ void build_partition(sfem_ctx* c, const sfem_mesh* m, const std::vector<double>&
     std::vector<int> bnode_ptr(nblk + 1, 0);
     std::vector<uint32_t> bnodes;
     std::vector<uint16_t> lptr;
-    std::vector<uint16_t> lent((size_t)nblk * 4 * BE, 0);
-    std::vector<uint32_t> lidx((size_t)ne_pad * 2, 0);
+    std::vector<uint32_t> lidx((size_t)ne_pad * 2, 0), lperm((size_t)ne_pad * 2, 0);
     std::vector<float> dm32((size_t)9 * ne_pad, 0.f), cf32((size_t)2 * ne_pad, 0.f);
     std::vector<double> dm64((size_t)9 * ne_pad, 0.0), cf64((size_t)2 * ne_pad, 0.0);
     std::vector<int> node_blocks(nn, 0);
@@ -121,7 +120,12 @@ void build_partition(sfem_ctx* c, const sfem_mesh* m, const std::vector<double>&
         for (int64_t s = e0; s < e1; s++) {
             const uint32_t* li = &lidx[2 * s];
             const int l[4] = {(int)(li[0] & 0xffff), (int)(li[0] >> 16), (int)(li[1] & 0xffff), (int)(li[1] >> 16)};
-            for (int v = 0; v < 4; v++) lent[(size_t)b * 4 * BE + fill[l[v]]++] = (uint16_t)(v * BE + (s - e0));
+            // corner v of element s -> its slot in the node-sorted incidence order
+            // (counting sort in element order: a node's slots list its elements in order)
+            uint16_t k[4];
+            for (int v = 0; v < 4; v++) k[v] = (uint16_t)fill[l[v]]++;
+            lperm[2 * s] = (uint32_t)k[0] | (uint32_t)k[1] << 16;
+            lperm[2 * s + 1] = (uint32_t)k[2] | (uint32_t)k[3] << 16;
         }
     }
     // shared nodes: flag, and list their partial rows in block order
@@ -187,7 +191,7 @@ void build_partition(sfem_ctx* c, const sfem_mesh* m, const std::vector<double>&
     c->d_bnode_ptr.upload(bnode_ptr);
     c->d_bnodes.upload(bnodes);
     c->d_lptr.upload(lptr);
-    c->d_lent.upload(lent);
+    c->d_lperm.upload(lperm);
     c->d_lidx.upload(lidx);
     c->d_dminv32.upload(dm32);
     c->d_coef32.upload(cf32);
@@ -221,7 +225,7 @@ void build_partition(sfem_ctx* c, const sfem_mesh* m, const std::vector<double>&
     a.bnodes = c->d_bnodes.as<uint32_t>();
     a.bslot = c->d_bslot.as<int>();
     a.lptr = c->d_lptr.as<uint16_t>();
-    a.lent = c->d_lent.as<uint16_t>();
+    a.lperm = c->d_lperm.as<uint2>();
     a.lidx = c->d_lidx.as<uint2>();
     a.dminv32 = c->d_dminv32.as<float>();
     a.coef32 = c->d_coef32.as<float>();

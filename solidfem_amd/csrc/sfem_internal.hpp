@@ -117,7 +117,7 @@ struct sfem_ctx {
     SharedArgs shr{};
     int nblk = 0;
     int64_t nrows_partial = 0;
-    DevBuf d_rel32, d_mat32, d_bnode_ptr, d_bnodes, d_lptr, d_lent, d_lidx, d_dminv32, d_coef32, d_dminv64, d_coef64;
+    DevBuf d_rel32, d_mat32, d_bnode_ptr, d_bnodes, d_lptr, d_lperm, d_lidx, d_dminv32, d_coef32, d_dminv64, d_coef64;
     DevBuf d_sh_node, d_sh_ptr, d_bslot, d_node_info;
     DevBuf d_partial;        // nrows_partial * 9 floats (also used as FP64 x3)
 

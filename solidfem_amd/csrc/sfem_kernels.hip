@@ -131,13 +131,15 @@ template <> struct KindTraits<2> {
 };
 
 // LDS layout (vec4 = 4 x T, one ds_read/ds_write_b128 per row):
-//   corner outputs: CW vec4 per (element, corner), component-major
-//       sf4[(corner * CW + j) * BE + element]   (CW = 1 for W = 3, 3 for W = 9)
+//   corner outputs: CW vec4 per incidence slot k, in the block's node-sorted
+//       incidence order, sf4[k * CW + j]  (CW = 1 for W = 3, 3 for W = 9); an
+//       element writes its corner c to the slot lperm[element] gives it, so a
+//       node's incidences are the contiguous slots [s_ptr[j], s_ptr[j+1])
 //   staged node rows: su4[nmax] (and sd4[nmax], sx4[nmax]) = (x, y, z, 0);
 //   sx4: rest coordinates relative to the block's centre
 //   per row m / h^2 for dynamics s_md[nmax] (T), node flags s_flag[nmax]
 //   (bnodes entry) and partial slots s_slot[nmax]
-//   incidence entries s_ent[4 BE] = corner * BE + element, offsets s_ptr[nmax+1]
+//   incidence offsets s_ptr[nmax+1]
 template <typename T> struct Vec4;
 template <> struct Vec4<float> { typedef float4 V; };
 template <> struct Vec4<double> { typedef double4 V; };
@@ -156,10 +158,12 @@ size_t lds_layout_bytes(int BE, int nmax)
     const size_t v4 = 4 * sizeof(T);
     const size_t rows = (size_t)nmax * (1 + (KindTraits<KIND>::HAS_D ? 1 : 0) + (KindTraits<KIND>::HAS_X ? 1 : 0));
     return v4 * ((size_t)4 * corner_vec4<KIND>(W) * BE + rows) + (2 * sizeof(uint32_t) + sizeof(T)) * (size_t)nmax +
-           sizeof(uint16_t) * (4 * (size_t)BE + nmax + 1) + 16;
+           sizeof(uint16_t) * ((size_t)nmax + 1) + 16;
 }
 
-template <int KIND, int MAT, int BE>
+// DYN: implicit dynamics (mdiag non-null); a separate instance so that static
+// solves carry none of its code
+template <int KIND, int MAT, int BE, bool DYN>
 __global__ __launch_bounds__(BE) void k_elem_block(BlockArgs a, const typename KindTraits<KIND>::T* __restrict__ u,
                                                     const typename KindTraits<KIND>::T* __restrict__ d,
                                                     typename KindTraits<KIND>::T* __restrict__ out,
@@ -184,8 +188,7 @@ __global__ __launch_bounds__(BE) void k_elem_block(BlockArgs a, const typename K
     T* s_md = reinterpret_cast<T*>(sx4 + (HAS_X ? a.nmax : 0));                  // nmax (mdiag)
     uint32_t* s_flag = reinterpret_cast<uint32_t*>(s_md + a.nmax);                // nmax
     int* s_slot = reinterpret_cast<int*>(s_flag + a.nmax);                        // nmax
-    uint16_t* s_ent = reinterpret_cast<uint16_t*>(s_slot + a.nmax);               // 4*BE
-    uint16_t* s_ptr = s_ent + 4 * BE;                                             // nmax+1
+    uint16_t* s_ptr = reinterpret_cast<uint16_t*>(s_slot + a.nmax);               // nmax+1
 
     const int b = blockIdx.x;
     const int tid = threadIdx.x;
@@ -198,7 +201,7 @@ __global__ __launch_bounds__(BE) void k_elem_block(BlockArgs a, const typename K
     const bool live = e < a.ne;
     T g[9];
     T cmu = T(0), clam = T(0);
-    uint2 li = make_uint2(0, 0);
+    uint2 li = make_uint2(0, 0), lp4 = make_uint2(0, 0);
     if (live && HAS_X) {
         if (a.mat32) {
             cmu = (T)a.mat32[e];
@@ -208,6 +211,7 @@ __global__ __launch_bounds__(BE) void k_elem_block(BlockArgs a, const typename K
             clam = (T)a.lam0;
         }
         li = a.lidx[e];
+        lp4 = a.lperm[e];
     } else if (live) {
         const size_t st = (size_t)a.ne_pad;
         if (sizeof(T) == 4) {
@@ -222,12 +226,11 @@ __global__ __launch_bounds__(BE) void k_elem_block(BlockArgs a, const typename K
             clam = (T)a.coef64[st + e];
         }
         li = a.lidx[e];
+        lp4 = a.lperm[e];
     }
-    // the block's incidence lists: this thread's share (2 words of entries,
-    // one offset) is loaded now and stored to LDS after the node staging
-    const uint32_t* le = reinterpret_cast<const uint32_t*>(a.lent + (size_t)b * 4 * BE);
+    // the block's incidence offsets: this thread's first one is loaded now and
+    // stored to LDS after the node staging
     const uint16_t* lp = a.lptr + (n0 + b);
-    const uint32_t le0 = le[tid], le1 = le[tid + BE];
     const uint16_t lp0 = tid <= nb ? lp[tid] : uint16_t(0);
     // stage block nodes (rows, flags, partial slots); all of a row's loads are
     // issued before its LDS stores
@@ -245,7 +248,7 @@ __global__ __launch_bounds__(BE) void k_elem_block(BlockArgs a, const typename K
         // dynamics: the row of the node's first block carries m / h^2 (the
         // node's partial rows add up to one mass term)
         T md = T(0);
-        if (mdiag && (!(fl & FLAG_SHARED) || node_q[node].x == slot)) md = mdiag[node];
+        if (DYN && (!(fl & FLAG_SHARED) || node_q[node].x == slot)) md = mdiag[node];
         V4 xv;
         if (HAS_X) {
             const float* xp = a.rel32 + 3 * (size_t)(n0 + j);
@@ -254,17 +257,12 @@ __global__ __launch_bounds__(BE) void k_elem_block(BlockArgs a, const typename K
         su4[j] = uv;
         if (HAS_D) sd4[j] = dv;
         if (HAS_X) sx4[j] = xv;
-        s_md[j] = md;
+        if (DYN) s_md[j] = md;
         s_flag[j] = fl;
         s_slot[j] = slot;
     }
-    {
-        uint32_t* se = reinterpret_cast<uint32_t*>(s_ent);
-        se[tid] = le0;
-        se[tid + BE] = le1;
-        if (tid <= nb) s_ptr[tid] = lp0;
-        for (int j = tid + BE; j <= nb; j += BE) s_ptr[j] = lp[j];
-    }
+    if (tid <= nb) s_ptr[tid] = lp0;
+    for (int j = tid + BE; j <= nb; j += BE) s_ptr[j] = lp[j];
     __syncthreads();
 
     if (live) {
@@ -316,16 +314,18 @@ __global__ __launch_bounds__(BE) void k_elem_block(BlockArgs a, const typename K
                 stress<T, MAT>(F, cmu, clam, P);
                 corners(P, g, T(-1), o);
             }
+            const int sl[4] = {(int)(lp4.x & 0xffffu), (int)(lp4.x >> 16), (int)(lp4.y & 0xffffu), (int)(lp4.y >> 16)};
 #pragma unroll
             for (int k = 0; k < 4; k++) {
                 V4 v;
                 v.x = o[k][0]; v.y = o[k][1]; v.z = o[k][2]; v.w = T(0);
-                sf4[k * BE + tid] = v;
+                sf4[sl[k]] = v;
             }
         } else {
             // K_aa(:,k) = V dP(e_k (x) g_a) g_a,  g_a = gradient of shape function a
             Tangent<T, MAT> tg;
             tg.init(F, cmu, clam);
+            const int sl[4] = {(int)(lp4.x & 0xffffu), (int)(lp4.x >> 16), (int)(lp4.y & 0xffffu), (int)(lp4.y >> 16)};
             T ga[4][3];
 #pragma unroll
             for (int c = 0; c < 3; c++) {
@@ -351,15 +351,15 @@ __global__ __launch_bounds__(BE) void k_elem_block(BlockArgs a, const typename K
                 for (int jv = 0; jv < CW; jv++) {
                     V4 v;
                     v.x = blk[4 * jv]; v.y = blk[4 * jv + 1]; v.z = blk[4 * jv + 2]; v.w = blk[4 * jv + 3];
-                    sf4[(aa * CW + jv) * BE + tid] = v;
+                    sf4[sl[aa] * CW + jv] = v;
                 }
             }
         }
     }
     __syncthreads();
 
-    // node phase: two adjacent lanes per node split its incidence list
-    // (even / odd entries, two loads in flight each) and combine by a lane swap
+    // node phase: two adjacent lanes per node split its contiguous incidence
+    // slots (even / odd, two loads in flight each) and combine by a lane swap
     double dot = 0.0;
     const int half = tid & 1;
     for (int jb = 0; jb < nb; jb += BE / 2) {
@@ -372,23 +372,18 @@ __global__ __launch_bounds__(BE) void k_elem_block(BlockArgs a, const typename K
             const int s1 = s_ptr[j + 1];
             int k = s_ptr[j] + half;
             for (; k + 2 < s1; k += 4) {
-                const int e0 = s_ent[k], e1 = s_ent[k + 2];
-                const int b0 = CW == 1 ? e0 : e0 + (e0 / BE) * (CW - 1) * BE;
-                const int b1 = CW == 1 ? e1 : e1 + (e1 / BE) * (CW - 1) * BE;
 #pragma unroll
                 for (int jv = 0; jv < CW; jv++) {
-                    const V4 v0 = sf4[b0 + jv * BE];
-                    const V4 v1 = sf4[b1 + jv * BE];
+                    const V4 v0 = sf4[k * CW + jv];
+                    const V4 v1 = sf4[(k + 2) * CW + jv];
                     acc[4 * jv] += v0.x + v1.x; acc[4 * jv + 1] += v0.y + v1.y;
                     acc[4 * jv + 2] += v0.z + v1.z; acc[4 * jv + 3] += v0.w + v1.w;
                 }
             }
             if (k < s1) {
-                const int e0 = s_ent[k];
-                const int b0 = CW == 1 ? e0 : e0 + (e0 / BE) * (CW - 1) * BE;
 #pragma unroll
                 for (int jv = 0; jv < CW; jv++) {
-                    const V4 v0 = sf4[b0 + jv * BE];
+                    const V4 v0 = sf4[k * CW + jv];
                     acc[4 * jv] += v0.x; acc[4 * jv + 1] += v0.y; acc[4 * jv + 2] += v0.z; acc[4 * jv + 3] += v0.w;
                 }
             }
@@ -396,7 +391,7 @@ __global__ __launch_bounds__(BE) void k_elem_block(BlockArgs a, const typename K
         // both lanes of the pair are in the same wave: add the partner's sums
 #pragma unroll
         for (int q = 0; q < 4 * CW; q++) acc[q] += __shfl_xor(acc[q], 1, 64);
-        if (has && half == 0 && mdiag) {
+        if (DYN && has && half == 0) {
             // M / h^2 (implicit dynamics, MatrixVectorMultiply FemPhysicsMatrixFree.cpp:523-534;
             // the residual's -M / h^2 (x - x0), ComputeGradients :553-557)
             const T md = s_md[j];
@@ -636,7 +631,7 @@ __global__ __launch_bounds__(CG_BLOCK) void k_cg_update(int64_t n3, float* __res
 // test 0: the reference criterion |r.z| <= eps^2 |r0.z0| (IterativeSolvers.h:180-185)
 // test 1: true residual ||r||^2 <= tol^2 ||r0||^2
 template <int PC>
-__global__ __launch_bounds__(CG_BLOCK) void k_cg_p(int64_t n3, const float* __restrict__ z, const float* __restrict__ r,
+__global__ __launch_bounds__(CG_P_BLOCK) void k_cg_p(int64_t n3, const float* __restrict__ z, const float* __restrict__ r,
                                               float* __restrict__ p, double* scal, const double* __restrict__ part, int n_part,
                                               int* done, int test_kind, double tol2)
 {
@@ -900,13 +895,19 @@ static hipError_t launch_elem_mat(int mat, const BlockArgs& a, const void* u, co
     const int2* nq = a.node_q;
     // with events: hipExtLaunchKernelGGL records them at the dispatch's own start
     // and end, so their interval is the kernel's execution time
-#define SFEM_ELEM_CASE(M)                                                                                            \
+#define SFEM_ELEM_DYN(M, D)                                                                                          \
     if (ev0)                                                                                                         \
-        hipExtLaunchKernelGGL((k_elem_block<KIND, M, BE>), grid, block, (uint32_t)lds, s, ev0, ev1, 0u, a,          \
+        hipExtLaunchKernelGGL((k_elem_block<KIND, M, BE, D>), grid, block, (uint32_t)lds, s, ev0, ev1, 0u, a,       \
                               (const T*)u, (const T*)d, (T*)out, (T*)partial, ext, mask_fixed, done, dot, md, nq);   \
     else                                                                                                             \
-        hipLaunchKernelGGL((k_elem_block<KIND, M, BE>), grid, block, lds, s, a, (const T*)u, (const T*)d, (T*)out,  \
+        hipLaunchKernelGGL((k_elem_block<KIND, M, BE, D>), grid, block, lds, s, a, (const T*)u, (const T*)d, (T*)out, \
                            (T*)partial, ext, mask_fixed, done, dot, md, nq)
+#define SFEM_ELEM_CASE(M)                                                                                            \
+    if (md) {                                                                                                        \
+        SFEM_ELEM_DYN(M, true);                                                                                      \
+    } else {                                                                                                         \
+        SFEM_ELEM_DYN(M, false);                                                                                     \
+    }
     switch (mat) {
     case LINEAR: SFEM_ELEM_CASE(LINEAR); break;
     case STVK: SFEM_ELEM_CASE(STVK); break;
@@ -914,6 +915,7 @@ static hipError_t launch_elem_mat(int mat, const BlockArgs& a, const void* u, co
     default: SFEM_ELEM_CASE(NEO_HOOKEAN); break;
     }
 #undef SFEM_ELEM_CASE
+#undef SFEM_ELEM_DYN
     return hipGetLastError();
 }
 
@@ -984,7 +986,7 @@ hipError_t set_elem_lds_limits(const BlockArgs& a)
     // per block): raise each kernel's limit to what it needs, within the
     // device's opt-in maximum less the kernel's static LDS
     const void* fns[] = {
-#define SFEM_FN(K, M) (const void*)k_elem_block<K, M, BLOCK_E>
+#define SFEM_FN(K, M) (const void*)k_elem_block<K, M, BLOCK_E, false>, (const void*)k_elem_block<K, M, BLOCK_E, true>
         SFEM_FN(0, LINEAR), SFEM_FN(0, STVK), SFEM_FN(0, COROTATIONAL), SFEM_FN(0, NEO_HOOKEAN),
         SFEM_FN(1, LINEAR), SFEM_FN(1, STVK), SFEM_FN(1, COROTATIONAL), SFEM_FN(1, NEO_HOOKEAN),
         SFEM_FN(2, LINEAR), SFEM_FN(2, STVK), SFEM_FN(2, COROTATIONAL), SFEM_FN(2, NEO_HOOKEAN),
@@ -997,11 +999,11 @@ hipError_t set_elem_lds_limits(const BlockArgs& a)
     if (e != hipSuccess) return e;
     e = hipDeviceGetAttribute(&optin, hipDeviceAttributeSharedMemPerBlockOptin, dev);
     if (e != hipSuccess) return e;
-    for (int k = 0; k < 12; k++) {
+    for (int k = 0; k < 24; k++) {
         hipFuncAttributes at;
         e = hipFuncGetAttributes(&at, fns[k]);
         if (e != hipSuccess) return e;
-        const size_t n = need[k / 4];
+        const size_t n = need[k / 8];
         if (n <= 64 * 1024) continue;
         if (n + at.sharedSizeBytes > (size_t)optin) return hipErrorInvalidValue;   // mesh too irregular for one block
         e = hipFuncSetAttribute(fns[k], hipFuncAttributeMaxDynamicSharedMemorySize, (int)n);
@@ -1037,8 +1039,11 @@ hipError_t launch_cg_iter_tail(int pc, int64_t n3, float* x, float* r, float* z,
     const int g = cg_grid(n3 / 3);
     if (pc) hipLaunchKernelGGL(k_cg_update<1>, dim3(g), dim3(CG_BLOCK), 0, st, n3, x, r, z, p, q, minv, node_q, partial, scal, pq_part, n_pq, part, done, guard);
     else hipLaunchKernelGGL(k_cg_update<0>, dim3(g), dim3(CG_BLOCK), 0, st, n3, x, r, z, p, q, minv, node_q, partial, scal, pq_part, n_pq, part, done, guard);
-    if (pc) hipLaunchKernelGGL(k_cg_p<1>, dim3(g), dim3(CG_BLOCK), 0, st, n3, z, r, p, scal, part, g, done, test_kind, tol2);
-    else hipLaunchKernelGGL(k_cg_p<0>, dim3(g), dim3(CG_BLOCK), 0, st, n3, z, r, p, scal, part, g, done, test_kind, tol2);
+    // k_cg_p re-reduces the update's g partials in every one of its workgroups:
+    // fewer, larger workgroups read them fewer times
+    const int gp = grid_for(n3 / 3, CG_P_BLOCK, CG_GRID);
+    if (pc) hipLaunchKernelGGL(k_cg_p<1>, dim3(gp), dim3(CG_P_BLOCK), 0, st, n3, z, r, p, scal, part, g, done, test_kind, tol2);
+    else hipLaunchKernelGGL(k_cg_p<0>, dim3(gp), dim3(CG_P_BLOCK), 0, st, n3, z, r, p, scal, part, g, done, test_kind, tol2);
     return hipGetLastError();
 }
 

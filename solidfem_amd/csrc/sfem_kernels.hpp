@@ -17,6 +17,10 @@ constexpr int CG_GRID = 1024;
 #define SFEM_CG_BLOCK 256
 #endif
 constexpr int CG_BLOCK = SFEM_CG_BLOCK;
+#ifndef SFEM_CG_P_BLOCK
+#define SFEM_CG_P_BLOCK 256
+#endif
+constexpr int CG_P_BLOCK = SFEM_CG_P_BLOCK;
 
 // FP64 scalars kept on the device during a CG solve
 enum Scal { S_RZ = 0, S_RZ0, S_RR, S_RR0, S_PQ, S_ALPHA, S_BETA, S_ITER, S_NT, S_NT2, S_NT3, S_RZ_PREV, S_COUNT = 16 };
@@ -31,8 +35,8 @@ struct BlockArgs {
     const uint32_t* bnodes;// block node list: node id | FLAG_SHARED | FLAG_FIXED
     const int* bslot;      // per block node row: its partial slot (shared rows only)
     const uint16_t* lptr;  // per block nb+1 offsets into its incidence list
-    const uint16_t* lent;  // per block 4*BLOCK_E incidence entries (corner * BLOCK_E + element)
     const uint2* lidx;     // per element 4 block-local node ids (16 bit each)
+    const uint2* lperm;    // per element, per corner its slot in the block's node-sorted incidences (16 bit each)
     const float* dminv32;  // 9 x ne_pad, Dm^-1 row-major, FP32
     const float* coef32;   // 2 x ne_pad: V mu, V lambda, FP32
     const double* dminv64; // 9 x ne_pad, FP64
