@@ -202,8 +202,9 @@ void build_partition(sfem_ctx* c, const sfem_mesh* m, const std::vector<double>&
     c->d_bslot.upload(bslot);
     // per node, for the CG update: a shared node's partial slot range
     // {first, end}; {-1, 0} for a free node of one block; {-2, 0} for a fixed
-    // node and for a node that no element references (its row of K is 0)
-    std::vector<int> node_q(2 * nn, 0);
+    // node and for a node that no element references (its row of K is 0);
+    // after the nn pairs, one more pair that stays 0 (BlockArgs::zero_flag)
+    std::vector<int> node_q(2 * nn + 2, 0);
     for (int64_t i = 0; i < nn; i++) {
         if (m->fixed[i] || node_blocks[i] == 0) node_q[2 * i] = -2;
         else if (sh_of[i] < 0) node_q[2 * i] = -1;
@@ -235,6 +236,7 @@ void build_partition(sfem_ctx* c, const sfem_mesh* m, const std::vector<double>&
     a.mdiag32 = nullptr;
     a.mdiag64 = nullptr;
     a.node_q = c->d_node_info.as<int2>();
+    a.zero_flag = c->d_node_info.as<int>() + 2 * nn;
     a.mat32 = uniform ? nullptr : c->d_mat32.as<float>();
     a.mu0 = (float)mu[0];
     a.lam0 = (float)lam[0];

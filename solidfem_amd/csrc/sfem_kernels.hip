@@ -32,11 +32,23 @@ constexpr uint32_t NODE_MASK = 0x3fffffffu;
 constexpr uint32_t FLAG_SHARED = 0x80000000u;
 constexpr uint32_t FLAG_FIXED = 0x40000000u;
 
+// an int read through the vector memory path even where its address is
+// uniform: a scalar load would be waited for with every scalar kernel-argument
+// load after it (scalar loads complete out of order, so the compiler waits for
+// all of them), which puts its round trip in front of unrelated loads
+__device__ __forceinline__ int vload(const int* f)
+{
+    return __hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
 // ------------------------------------------------------------ reductions
-template <int NV>
+// BS: the workgroup size the kernel is launched with (a constant, so that no
+// dispatch-packet load and no wait on it precede the reduction's loads)
+template <int NV, int BS>
 __device__ __forceinline__ void block_sum(double (&v)[NV], double* lds /* >= NV*16 */)
 {
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    static_assert(BS % 64 == 0 && BS <= 1024, "whole waves");
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = BS >> 6;
 #pragma unroll
     for (int q = 0; q < NV; q++)
         for (int off = 32; off > 0; off >>= 1) v[q] += __shfl_down(v[q], off, 64);
@@ -56,11 +68,11 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double* lds /* >= NV*
 }
 
 // this workgroup's NV sums -> part[q * gridDim.x + blockIdx.x]
-template <int NV>
+template <int NV, int BS>
 __device__ __forceinline__ void block_partials(double (&v)[NV], double* part)
 {
     __shared__ double lds[NV * 16];
-    block_sum<NV>(v, lds);
+    block_sum<NV, BS>(v, lds);
     if (threadIdx.x == 0)
 #pragma unroll
         for (int q = 0; q < NV; q++) part[(size_t)q * gridDim.x + blockIdx.x] = v[q];
@@ -70,22 +82,45 @@ __device__ __forceinline__ void block_partials(double (&v)[NV], double* part)
 // fixed order (thread-strided partial sums, then the fixed block tree).
 // The partials were written by workgroups on every XCD, so each load misses
 // this XCD's L2: U loads per thread are issued before the first add, so a
-// thread waits for ceil(n / (U * blockDim)) round trips, not n / blockDim.
-template <int NV, int U = 16>
-__device__ __forceinline__ void sum_partials(const double* part, int n, int stride, double (&tot)[NV])
+// thread waits for ceil(n / (U * BS)) round trips, not n / BS.
+// In two steps, so that a kernel can issue other loads between the first
+// round of partial loads and the wait for them: partials_issue loads the first
+// U * BS partials, partials_finish adds them, sums the rest and the block tree.
+template <int NV, int BS, int U>
+__device__ __forceinline__ void partials_issue(const double* part, int n, int stride, double (&v)[NV][U])
+{
+#pragma unroll
+    for (int q = 0; q < NV; q++)
+#pragma unroll
+        for (int k = 0; k < U; k++) {
+            const int i = threadIdx.x + k * BS;
+            v[q][k] = i < n ? part[(size_t)q * stride + i] : 0.0;
+        }
+}
+
+template <int NV, int BS, int U>
+__device__ __forceinline__ void partials_finish(const double* part, int n, int stride, const double (&v0)[NV][U],
+                                                double (&tot)[NV])
 {
     __shared__ double lds[NV * 16];
     __shared__ double res[NV];
+    // starting from v0[q][0] rather than 0.0 + v0[q][0]: an add of a constant
+    // to a conditional load is folded into the load's branch, which then waits
+    // for the load where it is issued
     double acc[NV];
 #pragma unroll
-    for (int q = 0; q < NV; q++) acc[q] = 0.0;
-    for (int base = threadIdx.x; base < n; base += blockDim.x * U) {
+    for (int q = 0; q < NV; q++) {
+        acc[q] = v0[q][0];
+#pragma unroll
+        for (int k = 1; k < U; k++) acc[q] += v0[q][k];
+    }
+    for (int base = threadIdx.x + BS * U; base < n; base += BS * U) {
         double v[NV][U];
 #pragma unroll
         for (int q = 0; q < NV; q++)
 #pragma unroll
             for (int k = 0; k < U; k++) {
-                const int i = base + k * blockDim.x;
+                const int i = base + k * BS;
                 v[q][k] = i < n ? part[(size_t)q * stride + i] : 0.0;
             }
 #pragma unroll
@@ -93,7 +128,7 @@ __device__ __forceinline__ void sum_partials(const double* part, int n, int stri
 #pragma unroll
             for (int k = 0; k < U; k++) acc[q] += v[q][k];
     }
-    block_sum<NV>(acc, lds);
+    block_sum<NV, BS>(acc, lds);
     if (threadIdx.x == 0)
 #pragma unroll
         for (int q = 0; q < NV; q++) res[q] = acc[q];
@@ -102,12 +137,20 @@ __device__ __forceinline__ void sum_partials(const double* part, int n, int stri
     for (int q = 0; q < NV; q++) tot[q] = res[q];
 }
 
+template <int NV, int BS, int U = 16>
+__device__ __forceinline__ void sum_partials(const double* part, int n, int stride, double (&tot)[NV])
+{
+    double v[NV][U];
+    partials_issue<NV, BS, U>(part, n, stride, v);
+    partials_finish<NV, BS, U>(part, n, stride, v, tot);
+}
+
 // one workgroup: scal[slot + q] = sum of the NV partial arrays
 template <int NV>
 __global__ __launch_bounds__(256) void k_finish(const double* __restrict__ part, int n, double* scal, int slot)
 {
     double tot[NV];
-    sum_partials<NV>(part, n, n, tot);
+    sum_partials<NV, 256>(part, n, n, tot);
     if (threadIdx.x == 0)
 #pragma unroll
         for (int q = 0; q < NV; q++) scal[slot + q] = tot[q];
@@ -179,7 +222,10 @@ __global__ __launch_bounds__(BE) void k_elem_block(BlockArgs a, const typename K
     constexpr int CW = corner_vec4<KIND>(W);
     constexpr bool HAS_D = KindTraits<KIND>::HAS_D;
     constexpr bool HAS_X = KindTraits<KIND>::HAS_X;
-    if (done && *done) return;
+    // the done flag (never null: launches outside CG pass a flag that stays 0)
+    // is read with the first-round loads and tested before the staging loop,
+    // so its round trip does not precede theirs
+    const int stop = vload(done);
     extern __shared__ __align__(32) unsigned char smem_raw[];
     V4* sf4 = reinterpret_cast<V4*>(smem_raw);        // 4*CW*BE corner rows
     V4* su4 = sf4 + 4 * CW * BE;                      // nmax
@@ -192,8 +238,8 @@ __global__ __launch_bounds__(BE) void k_elem_block(BlockArgs a, const typename K
 
     const int b = blockIdx.x;
     const int tid = threadIdx.x;
-    const int n0 = a.bnode_ptr[b];
-    const int nb = a.bnode_ptr[b + 1] - n0;
+    const int n0 = vload(a.bnode_ptr + b);
+    const int nb = vload(a.bnode_ptr + b + 1) - n0;
 
     // element data first: independent of the node staging, so its latency
     // overlaps the gathers below
@@ -232,6 +278,7 @@ __global__ __launch_bounds__(BE) void k_elem_block(BlockArgs a, const typename K
     // stored to LDS after the node staging
     const uint16_t* lp = a.lptr + (n0 + b);
     const uint16_t lp0 = tid <= nb ? lp[tid] : uint16_t(0);
+    if (stop) return;
     // stage block nodes (rows, flags, partial slots); all of a row's loads are
     // issued before its LDS stores
     for (int j = tid; j < nb; j += BE) {
@@ -432,7 +479,7 @@ __global__ __launch_bounds__(BE) void k_elem_block(BlockArgs a, const typename K
     }
     if (KIND == 0 && dot_part) {
         double v[1] = {dot};
-        block_partials<1>(v, dot_part);
+        block_partials<1, BE>(v, dot_part);
     }
 }
 
@@ -485,7 +532,7 @@ __global__ __launch_bounds__(256) void k_shared_gather(SharedArgs s, const T* __
         const int b1 = b0 + chunk < n_in ? b0 + chunk : n_in;
         for (int k = b0 + threadIdx.x; k < b1; k += blockDim.x) dot += dot_in[k];
         double v[1] = {dot};
-        block_partials<1>(v, dot_out);
+        block_partials<1, 256>(v, dot_out);
     }
 }
 
@@ -505,7 +552,7 @@ __global__ __launch_bounds__(CG_BLOCK) void k_cg_init(int64_t n3, const float* _
 {
     const int64_t nn = n3 / 3;
     double v[2] = {0.0, 0.0};
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nn; i += (int64_t)gridDim.x * blockDim.x) {
+    for (int64_t i = blockIdx.x * (int64_t)CG_BLOCK + threadIdx.x; i < nn; i += (int64_t)gridDim.x * CG_BLOCK) {
         float rv[3], zv[3];
 #pragma unroll
         for (int c = 0; c < 3; c++) { rv[c] = fixed[i] ? 0.f : b[3 * i + c]; x[3 * i + c] = 0.f; r[3 * i + c] = rv[c]; }
@@ -522,13 +569,13 @@ __global__ __launch_bounds__(CG_BLOCK) void k_cg_init(int64_t n3, const float* _
         v[0] += (double)rv[0] * zv[0] + (double)rv[1] * zv[1] + (double)rv[2] * zv[2];
         v[1] += (double)rv[0] * rv[0] + (double)rv[1] * rv[1] + (double)rv[2] * rv[2];
     }
-    block_partials<2>(v, part);
+    block_partials<2, CG_BLOCK>(v, part);
 }
 
 __global__ __launch_bounds__(256) void k_cg_init_finish(const double* __restrict__ part, int n, double* scal, int* done)
 {
     double tot[2];
-    sum_partials<2>(part, n, n, tot);
+    sum_partials<2, 256>(part, n, n, tot);
     if (threadIdx.x == 0) {
         scal[S_RZ] = tot[0];
         scal[S_RZ0] = tot[0];
@@ -554,18 +601,33 @@ __global__ __launch_bounds__(CG_BLOCK) void k_cg_update(int64_t n3, float* __res
                                                    double* scal, const double* __restrict__ pq_part, int n_pq,
                                                    double* __restrict__ part, int* done, int guard)
 {
-    if (*done) return;
+    // the done flag is read with the node loads and the p.q reduction and
+    // tested before anything is stored (a vector load: as a scalar load its
+    // miss would hold back the reduction's loads, which wait for the scalar
+    // kernel arguments)
+    const int stop = vload(done);
     const int64_t nn = n3 / 3;
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    const int64_t stride = (int64_t)gridDim.x * CG_BLOCK;
+    int64_t i = blockIdx.x * (int64_t)CG_BLOCK + threadIdx.x;
     float pv[3], qv[3], rv[3], xv[3], M[9];
-    auto load = [&](int64_t n) {
-        const int2 nq = node_q[n];
+    int2 nq = make_int2(-2, 0);
+    auto load_rows = [&](int64_t n) {
+        nq = node_q[n];
+        // every row is loaded in the first round (q is selected, not loaded
+        // conditionally on node_q); the partial rows of a shared node follow
 #pragma unroll
         for (int c = 0; c < 3; c++) {
             pv[c] = p[3 * n + c]; rv[c] = r[3 * n + c]; xv[c] = x[3 * n + c];
-            qv[c] = nq.x == -1 ? q[3 * n + c] : 0.f;
+            qv[c] = q[3 * n + c];
         }
+        if (PC) {
+#pragma unroll
+            for (int k = 0; k < 9; k++) M[k] = minv[9 * n + k];
+        }
+    };
+    auto load_partial_rows = [&]() {
+#pragma unroll
+        for (int c = 0; c < 3; c++) qv[c] = nq.x == -1 ? qv[c] : 0.f;
         if (nq.x >= 0) {
             // up to 4 partial rows in flight per round trip, summed in slot order
             const int k1 = nq.y;
@@ -581,14 +643,15 @@ __global__ __launch_bounds__(CG_BLOCK) void k_cg_update(int64_t n3, float* __res
                     for (int c = 0; c < 3; c++) qv[c] += t[j][c];
             }
         }
-        if (PC) {
-#pragma unroll
-            for (int k = 0; k < 9; k++) M[k] = minv[9 * n + k];
-        }
     };
-    if (i < nn) load(i);
-    double pq[1];
-    sum_partials<1>(pq_part, n_pq, n_pq, pq);
+    // first round: this thread's first node rows and the first p.q partials;
+    // then the node's partial rows (they need node_q); then the reduction
+    double pq_v[1][16], pq[1];
+    if (i < nn) load_rows(i);
+    partials_issue<1, CG_BLOCK, 16>(pq_part, n_pq, n_pq, pq_v);
+    if (i < nn) load_partial_rows();
+    partials_finish<1, CG_BLOCK, 16>(pq_part, n_pq, n_pq, pq_v, pq);
+    if (stop) return;
     const double rz = scal[S_RZ];
     if (guard && !(pq[0] > 0.0)) {          // breakdown: not positive definite
         if (blockIdx.x == 0 && threadIdx.x == 0) *done = 3;
@@ -622,9 +685,12 @@ __global__ __launch_bounds__(CG_BLOCK) void k_cg_update(int64_t n3, float* __res
         }
         v[0] += (double)rv[0] * zv[0] + (double)rv[1] * zv[1] + (double)rv[2] * zv[2];
         v[1] += (double)rv[0] * rv[0] + (double)rv[1] * rv[1] + (double)rv[2] * rv[2];
-        if (i + stride < nn) load(i + stride);
+        if (i + stride < nn) {
+            load_rows(i + stride);
+            load_partial_rows();
+        }
     }
-    block_partials<2>(v, part);
+    block_partials<2, CG_BLOCK>(v, part);
 }
 
 // beta = rz' / rz (every workgroup); test; p = z + beta p, one node per step
@@ -635,10 +701,10 @@ __global__ __launch_bounds__(CG_P_BLOCK) void k_cg_p(int64_t n3, const float* __
                                               float* __restrict__ p, double* scal, const double* __restrict__ part, int n_part,
                                               int* done, int test_kind, double tol2)
 {
-    if (*done) return;
+    const int stop = vload(done);    // tested after the loads and the reduction are issued
     const int64_t nn = n3 / 3;
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    const int64_t stride = (int64_t)gridDim.x * CG_P_BLOCK;
+    int64_t i = blockIdx.x * (int64_t)CG_P_BLOCK + threadIdx.x;
     const float* zz = PC ? z : r;
     float zv[3], pv[3];
     if (i < nn) {
@@ -646,7 +712,8 @@ __global__ __launch_bounds__(CG_P_BLOCK) void k_cg_p(int64_t n3, const float* __
         for (int c = 0; c < 3; c++) { zv[c] = zz[3 * i + c]; pv[c] = p[3 * i + c]; }
     }
     double t[2];
-    sum_partials<2>(part, n_part, n_part, t);
+    sum_partials<2, CG_P_BLOCK>(part, n_part, n_part, t);
+    if (stop) return;
     const double rz1 = t[0], rr = t[1];
     const bool conv = test_kind == 0 ? (fabs(rz1) <= tol2 * fabs(scal[S_RZ0])) : (rr <= tol2 * scal[S_RR0]);
     const bool bad = !(rr == rr) || !(rz1 == rz1);
@@ -683,7 +750,7 @@ __global__ __launch_bounds__(256) void k_dots3(int64_t n3, const double* __restr
         if (b) v[1] += b[3 * i] * b[3 * i] + b[3 * i + 1] * b[3 * i + 1] + b[3 * i + 2] * b[3 * i + 2];
         if (d) v[2] += (double)d[3 * i] * a[3 * i] + (double)d[3 * i + 1] * a[3 * i + 1] + (double)d[3 * i + 2] * a[3 * i + 2];
     }
-    block_partials<3>(v, part);
+    block_partials<3, 256>(v, part);
 }
 
 // ut = u + alpha * dx (free rows); u32 = (float) ut
@@ -893,6 +960,7 @@ static hipError_t launch_elem_mat(int mat, const BlockArgs& a, const void* u, co
     dim3 grid(a.nblk), block(BE);
     const T* md = (const T*)(sizeof(T) == 8 ? (const void*)a.mdiag64 : (const void*)a.mdiag32);
     const int2* nq = a.node_q;
+    if (!done) done = a.zero_flag;
     // with events: hipExtLaunchKernelGGL records them at the dispatch's own start
     // and end, so their interval is the kernel's execution time
 #define SFEM_ELEM_DYN(M, D)                                                                                          \

@@ -51,6 +51,7 @@ struct BlockArgs {
     const float* mdiag32;
     const double* mdiag64;
     const int2* node_q;
+    const int* zero_flag;  // a device int that stays 0: the done flag of launches outside CG
 };
 
 // nodes shared by several blocks: partial rows to sum
