@@ -39,8 +39,12 @@ enum { SFEM_ST_STATIC = 0, SFEM_ST_QUASI_STATIC = 1, SFEM_ST_EXPLICIT = 2, SFEM_
      to cg_rtol, FP64 back-tracking line search, Jacobi fallback judge.       */
 enum { SFEM_SOLVE_REFERENCE = 0, SFEM_SOLVE_CONVERGED = 1 };
 
-/* Preconditioners for the inner CG */
-enum { SFEM_PC_NONE = 0, SFEM_PC_JACOBI = 1, SFEM_PC_LEARNED = 2 };
+/* Preconditioners for the inner CG.  MULTIGRID: one V-cycle over the levels
+   added by sfem_mg_add_level per CG iteration, Chebyshev smoothing on block
+   Jacobi; MULTIGRID_LEARNED: the same with the learned blocks in the finest
+   level's smoother.  Both fall back to block-Jacobi PCG when they do not reach
+   the forcing tolerance. */
+enum { SFEM_PC_NONE = 0, SFEM_PC_JACOBI = 1, SFEM_PC_LEARNED = 2, SFEM_PC_MULTIGRID = 3, SFEM_PC_MULTIGRID_LEARNED = 4 };
 
 enum {
     SFEM_OK = 0,
@@ -199,6 +203,37 @@ int sfem_profile_hessvec(sfem_ctx* ctx, int reps, double* ms_elem, double* ms_ga
    Returns the average device time (ms) and the number of timed launches since
    the last start.  Same counterpart as sfem_profile_hessvec. */
 int sfem_time_hessvec(sfem_ctx* ctx, int every, double* ms_avg, int64_t* samples);
+
+/* ---------------------------------------------------------------------------
+ * Multigrid preconditioner (no reference counterpart: the reference CG is
+ * unpreconditioned, Math/IterativeSolvers.h:115-195).  A hierarchy of nested
+ * tetrahedral meshes, finest (the context's) first; each coarse level runs the
+ * same element kernels on its own mesh at the state injected from the level
+ * above, so its operator is the tangent of its mesh (rediscretisation).
+ * ------------------------------------------------------------------------- */
+
+/* Adds a level below the current coarsest one (the context's mesh, before the
+   first call).  coarse: the level's mesh (positive volumes; fixed flags of its
+   Dirichlet nodes).  inject[k] (coarse nn): the node of the level above at
+   coarse node k's position; the coarse displacement is sampled there.
+   Prolongation from this level to the one above, CSR over the nodes of the
+   level above: row i holds entries p_ptr[i] .. p_ptr[i+1]-1 of
+   (p_col = coarse node, p_val = weight); node values, the same for x, y, z.
+   The level inherits the context's material (uniform moduli only). */
+int sfem_mg_add_level(sfem_ctx* ctx, const sfem_mesh* coarse, const int32_t* inject, const int64_t* p_ptr,
+                      const int32_t* p_col, const double* p_val);
+int sfem_mg_clear(sfem_ctx* ctx);
+/* levels (1 + coarse levels) and, when lmax is non-null, the largest-eigenvalue
+   estimates of M A per level from the last set-up */
+int sfem_mg_info(sfem_ctx* ctx, int32_t* nlevels, double* lmax);
+/* z = V(r): one V-cycle (block-Jacobi smoothers) at the current state;
+   r, z nn*3 in original order, fixed rows ignored / zero */
+int sfem_mg_apply(sfem_ctx* ctx, const double* r, double* z);
+/* One linear solve K x = b at the current state by PCG with the V-cycle
+   (block-Jacobi smoothers) to ||r|| <= rtol ||b|| in at most maxit iterations;
+   b, x nn*3 in original order; status 1 converged, 0 iteration limit,
+   3 breakdown (p.Kp <= 0), 4 not finite */
+int sfem_mg_pcg(sfem_ctx* ctx, const double* b, double* x, double rtol, int32_t maxit, int32_t* iters, int32_t* status);
 
 /* ---------------------------------------------------------------------------
  * Learned block preconditioner (no reference counterpart).  A message-passing

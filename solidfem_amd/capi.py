@@ -16,7 +16,7 @@ LIB_PATH = os.path.join(HERE, "libsfem_hip.so")
 LINEAR, STVK, COROTATIONAL, NEO_HOOKEAN = 0, 1, 2, 3
 ST_STATIC, ST_QUASI_STATIC, ST_EXPLICIT, ST_IMPLICIT = 0, 1, 2, 3
 SOLVE_REFERENCE, SOLVE_CONVERGED = 0, 1
-PC_NONE, PC_JACOBI, PC_LEARNED = 0, 1, 2
+PC_NONE, PC_JACOBI, PC_LEARNED, PC_MULTIGRID, PC_MULTIGRID_LEARNED = 0, 1, 2, 3, 4
 ENOTCONV = -4
 
 EXPORTS = [
@@ -26,6 +26,7 @@ EXPORTS = [
     "sfem_gradients", "sfem_matvec",
     "sfem_block_diagonal", "sfem_solve", "sfem_step", "sfem_set_precond_blocks", "sfem_info",
     "sfem_reset", "sfem_profile_hessvec", "sfem_time_hessvec",
+    "sfem_mg_add_level", "sfem_mg_clear", "sfem_mg_info", "sfem_mg_apply", "sfem_mg_pcg",
     "sfem_gnn_create", "sfem_gnn_destroy", "sfem_gnn_num_params", "sfem_gnn_get_params", "sfem_gnn_set_params",
     "sfem_set_learned_preconditioner", "sfem_collect_pairs", "sfem_num_pairs", "sfem_get_pair", "sfem_clear_pairs",
     "sfem_gnn_loss", "sfem_gnn_eval", "sfem_gnn_train", "sfem_gnn_inputs", "sfem_gnn_gemm_check",
@@ -106,6 +107,11 @@ def lib():
             "sfem_reset": ([V], C.c_int),
             "sfem_profile_hessvec": ([V, C.c_int, _dp, _dp], C.c_int),
             "sfem_time_hessvec": ([V, C.c_int, _dp, C.POINTER(C.c_int64)], C.c_int),
+            "sfem_mg_add_level": ([V, C.POINTER(Mesh), _ip, C.POINTER(C.c_int64), _ip, _dp], C.c_int),
+            "sfem_mg_clear": ([V], C.c_int),
+            "sfem_mg_info": ([V, C.POINTER(C.c_int32), _dp], C.c_int),
+            "sfem_mg_apply": ([V, _dp, _dp], C.c_int),
+            "sfem_mg_pcg": ([V, _dp, _dp, C.c_double, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32)], C.c_int),
             "sfem_gnn_create": ([C.c_uint64, C.POINTER(V)], C.c_int),
             "sfem_gnn_destroy": ([V], None),
             "sfem_gnn_num_params": ([V, C.POINTER(C.c_int64)], C.c_int),
@@ -256,6 +262,46 @@ class Solver:
     def set_precond_blocks(self, minv):
         self._minv = None if minv is None else np.ascontiguousarray(minv, dtype=np.float32).reshape(-1, 9)
         _check(lib().sfem_set_precond_blocks(self.h, _p(self._minv, _fp)))
+
+    def add_mg_level(self, X0, tets, fixed, inject, p_ptr, p_col, p_val):
+        """one coarser level of the multigrid preconditioner (sfem_mg_add_level)"""
+        X0 = np.ascontiguousarray(X0, dtype=np.float64).reshape(-1, 3)
+        tets = np.ascontiguousarray(tets, dtype=np.int32).reshape(-1, 4)
+        fixed = np.ascontiguousarray(fixed, dtype=np.uint8).reshape(-1)
+        inject = np.ascontiguousarray(inject, dtype=np.int32)
+        p_ptr = np.ascontiguousarray(p_ptr, dtype=np.int64)
+        p_col = np.ascontiguousarray(p_col, dtype=np.int32)
+        p_val = np.ascontiguousarray(p_val, dtype=np.float64)
+        m = Mesh(_p(tets, _ip), tets.shape[0], _p(X0), X0.shape[0], fixed.ctypes.data_as(C.POINTER(C.c_uint8)))
+        _check(lib().sfem_mg_add_level(self.h, C.byref(m), _p(inject, _ip), p_ptr.ctypes.data_as(C.POINTER(C.c_int64)),
+                                       _p(p_col, _ip), _p(p_val)))
+
+    def set_multigrid(self, levels):
+        """levels: [(X0, tets, fixed, inject, p_ptr, p_col, p_val)] finest coarse level first"""
+        _check(lib().sfem_mg_clear(self.h))
+        for lv in levels:
+            self.add_mg_level(*lv)
+
+    def mg_info(self):
+        n = C.c_int32()
+        _check(lib().sfem_mg_info(self.h, C.byref(n), None))
+        lm = np.zeros(n.value)
+        _check(lib().sfem_mg_info(self.h, C.byref(n), _p(lm)))
+        return lm
+
+    def mg_apply(self, r):
+        r = np.ascontiguousarray(r, dtype=np.float64).reshape(-1, 3)
+        z = np.zeros_like(r)
+        _check(lib().sfem_mg_apply(self.h, _p(r), _p(z)))
+        return z
+
+    def mg_pcg(self, b, rtol=1e-6, maxit=500):
+        """-> (x, iterations, status) of one multigrid-PCG solve K x = b"""
+        b = np.ascontiguousarray(b, dtype=np.float64).reshape(-1, 3)
+        x = np.zeros_like(b)
+        it, stt = C.c_int32(), C.c_int32()
+        _check(lib().sfem_mg_pcg(self.h, _p(b), _p(x), rtol, maxit, C.byref(it), C.byref(stt)))
+        return x, it.value, stt.value
 
     def solve(self, params=None, allow_notconv=False, **kw):
         p = params if params is not None else solve_params(**kw)

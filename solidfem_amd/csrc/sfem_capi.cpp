@@ -423,8 +423,12 @@ int solve_converged(sfem_ctx* c, const sfem_solve_params* p, sfem_solve_stats* s
 {
     const int64_t n3 = 3 * c->nn;
     int pc_kind = p->precond;
-    if (pc_kind == SFEM_PC_LEARNED && !c->has_learned && !c->gnn)
+    if ((pc_kind == SFEM_PC_LEARNED || pc_kind == SFEM_PC_MULTIGRID_LEARNED) && !c->has_learned && !c->gnn)
         return fail(SFEM_ESTATE, "learned preconditioner not installed");
+    const bool multigrid = pc_kind == SFEM_PC_MULTIGRID || pc_kind == SFEM_PC_MULTIGRID_LEARNED;
+    if (multigrid && c->mg.empty()) return fail(SFEM_ESTATE, "multigrid preconditioner without coarse levels");
+    if (multigrid && c->tstep > 0)
+        return fail(SFEM_ESTATE, "multigrid preconditioner: static and quasi-static solves only");
     if (pc_kind == SFEM_PC_NONE)
         HIPCK(launch_identity_blocks(c->nn, c->d_fixed.as<uint8_t>(), c->d_minv.as<float>(), c->stream));
     residual(c, c->d_u64.as<double>(), c->d_r64.as<double>());
@@ -445,18 +449,25 @@ int solve_converged(sfem_ctx* c, const sfem_solve_params* p, sfem_solve_stats* s
             eta = std::min(eta_max, safe > 0.1 ? std::max(ew, safe) : ew);
         }
         eta = std::max(eta, std::max(0.5 * target(en) / rn, 1e-6));
-        if (pc_kind == SFEM_PC_JACOBI) build_jacobi(c);
-        if (pc_kind == SFEM_PC_LEARNED && c->gnn) {
-            build_jacobi(c);   // network input and the blocks it corrects
+        const bool learned = pc_kind == SFEM_PC_LEARNED || pc_kind == SFEM_PC_MULTIGRID_LEARNED;
+        if (pc_kind == SFEM_PC_JACOBI || multigrid) build_jacobi(c);
+        if (learned && c->gnn) {
+            if (!multigrid) build_jacobi(c);   // network input and the blocks it corrects
             gnn_install(c);
         }
         HIPCK(launch_to_f32(n3, c->d_r64.as<double>(), c->d_b32.as<float>(), c->stream));
-        const float* minv = pc_kind == SFEM_PC_LEARNED ? c->d_minv_learned.as<float>() : c->d_minv.as<float>();
-        CgResult cg = pcg(c, minv, pc_kind == SFEM_PC_NONE ? 0 : 1, p->inner_iterations, 1, eta * eta, 1, p->check_every,
-                          &st->hessvec_evals);
+        const float* minv = learned ? c->d_minv_learned.as<float>() : c->d_minv.as<float>();
+        CgResult cg;
+        if (multigrid) {
+            mg_prepare(c, minv);
+            cg = mg_pcg(c, p->inner_iterations, eta * eta, &st->hessvec_evals);
+        } else {
+            cg = pcg(c, minv, pc_kind == SFEM_PC_NONE ? 0 : 1, p->inner_iterations, 1, eta * eta, 1, p->check_every,
+                     &st->hessvec_evals);
+        }
         st->cg_iters += cg.iters;
-        if (pc_kind == SFEM_PC_LEARNED && cg.status != 1) {
-            // judge: the learned PCG did not reach its forcing -> redo with Jacobi
+        if ((learned || multigrid) && cg.status != 1) {
+            // judge: the learned / multigrid PCG did not reach its forcing -> redo with Jacobi
             st->fallbacks++;
             build_jacobi(c);
             cg = pcg(c, c->d_minv.as<float>(), 1, p->inner_iterations, 1, eta * eta, 1, p->check_every, &st->hessvec_evals);
@@ -535,6 +546,10 @@ int sfem_create(const sfem_mesh* m, const sfem_material* mat, sfem_ctx** out)
         c->ne = m->ne;
         c->nn = m->nn;
         c->material = mat->material;
+        c->young = mat->young;
+        c->poisson = mat->poisson;
+        c->density = mat->density;
+        c->per_element = mat->young_e != nullptr || mat->poisson_e != nullptr;
         for (int k = 0; k < 3; k++) c->gravity[k] = mat->gravity[k];
         const int64_t ne = m->ne, nn = m->nn;
         // --- geometry (FemPhysicsMatrixFree.cpp:126-146): Dm^-1, V, lumped masses

@@ -89,9 +89,24 @@ struct SfemPairs {
     std::vector<double> er;                         // r . e per pair (FP64)
 };
 
+// a coarse level of the multigrid preconditioner: its own context (mesh,
+// partition, state, block preconditioner), which runs on the stream of the
+// finest level's context, and the transfer operators to the level above
+struct MgLevel {
+    sfem_ctx* c = nullptr;
+    sfem::DevBuf inj;                 // per node: the node of the level above at its position
+    sfem::DevBuf p_ptr, p_col, p_val; // prolongation rows (nodes above) -> nodes here, FP32 weights
+    sfem::DevBuf t_ptr, t_row, t_val; // its transpose (restriction), rows = nodes here
+    double lmax = 0.0;                // largest eigenvalue estimate of M A on this level
+    ~MgLevel();
+};
+
 struct sfem_ctx {
     int64_t ne = 0, nn = 0, nfixed = 0;
     int material = 0;
+    // the material as given (coarse multigrid levels inherit a uniform one)
+    double young = 0.0, poisson = 0.0, density = 0.0;
+    bool per_element = false;
     double gravity[3] = {0, 0, 0};
     double frac = 0.0;
     double tstep = 0.0;   // implicit dynamics substep h; 0 = static (FemPhysicsMatrixFree::mTimeStep)
@@ -107,6 +122,7 @@ struct sfem_ctx {
     bool has_fdead = false;
     bool has_learned = false;
     hipStream_t stream = nullptr;
+    bool owns_stream = true;   // false for multigrid levels (the finest level's stream)
 
     // host copies kept for preprocessing
     std::vector<uint8_t> fixed_h;
@@ -137,6 +153,14 @@ struct sfem_ctx {
     sfem_gnn* gnn = nullptr;
     SfemPairs pairs;
 
+    // multigrid preconditioner: coarse levels (finest first), the finest
+    // level's smoother blocks and work vectors, power-iteration norms
+    std::vector<std::unique_ptr<MgLevel>> mg;
+    const float* mg_minv0 = nullptr;
+    double mg_lmax0 = 0.0;
+    int64_t mg_evals = 0;      // element evaluations of one V-cycle
+    DevBuf d_mg_r, d_mg_d, d_mg_q, d_mg_part, d_mg_norm;
+
     // sampled timing of the CG loops' Hessian-vector launches (sfem_time_hessvec)
     static constexpr int TM_POOL = 64;
     hipEvent_t tm_ev[2 * TM_POOL] = {};
@@ -149,7 +173,8 @@ struct sfem_ctx {
     {
         for (hipEvent_t e : tm_ev)
             if (e) (void)hipEventDestroy(e);
-        if (stream) (void)hipStreamDestroy(stream);
+        mg.clear();
+        if (stream && owns_stream) (void)hipStreamDestroy(stream);
     }
 };
 
@@ -189,5 +214,13 @@ CgResult pcg(sfem_ctx* c, const float* minv, int pc, int maxit, int test_kind, d
 // the current state (needs the Jacobi blocks in d_bd32)
 void gnn_install(sfem_ctx* c);
 void ensure_graph(sfem_ctx* c);
+
+// multigrid preconditioner (sfem_mg.cpp): level operators at the current
+// state with minv0 as the finest level's smoother blocks, then PCG on
+// K x = b (b in d_b32, x in d_x32) with one V-cycle per iteration
+void mg_prepare(sfem_ctx* c, const float* minv0);
+CgResult mg_pcg(sfem_ctx* c, int maxit, double tol2, int64_t* hv_evals);
+// z = V(r) on the finest level (r in d_r32, z in d_z32), after mg_prepare
+void mg_vcycle(sfem_ctx* c);
 
 }  // namespace sfem

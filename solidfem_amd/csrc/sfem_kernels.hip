@@ -25,125 +25,13 @@
 
 #include "sfem_device.hpp"
 #include "sfem_kernels.hpp"
+#include "sfem_reduce.hpp"
 
 namespace sfem {
 
 constexpr uint32_t NODE_MASK = 0x3fffffffu;
 constexpr uint32_t FLAG_SHARED = 0x80000000u;
 constexpr uint32_t FLAG_FIXED = 0x40000000u;
-
-// an int read through the vector memory path even where its address is
-// uniform: a scalar load would be waited for with every scalar kernel-argument
-// load after it (scalar loads complete out of order, so the compiler waits for
-// all of them), which puts its round trip in front of unrelated loads
-__device__ __forceinline__ int vload(const int* f)
-{
-    return __hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-}
-
-// ------------------------------------------------------------ reductions
-// BS: the workgroup size the kernel is launched with (a constant, so that no
-// dispatch-packet load and no wait on it precede the reduction's loads)
-template <int NV, int BS>
-__device__ __forceinline__ void block_sum(double (&v)[NV], double* lds /* >= NV*16 */)
-{
-    static_assert(BS % 64 == 0 && BS <= 1024, "whole waves");
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = BS >> 6;
-#pragma unroll
-    for (int q = 0; q < NV; q++)
-        for (int off = 32; off > 0; off >>= 1) v[q] += __shfl_down(v[q], off, 64);
-    if (lane == 0)
-#pragma unroll
-        for (int q = 0; q < NV; q++) lds[q * 16 + w] = v[q];
-    __syncthreads();
-    if (w == 0) {
-#pragma unroll
-        for (int q = 0; q < NV; q++) {
-            double t = lane < nw ? lds[q * 16 + lane] : 0.0;
-            for (int off = 8; off > 0; off >>= 1) t += __shfl_down(t, off, 64);
-            v[q] = t;
-        }
-    }
-    __syncthreads();
-}
-
-// this workgroup's NV sums -> part[q * gridDim.x + blockIdx.x]
-template <int NV, int BS>
-__device__ __forceinline__ void block_partials(double (&v)[NV], double* part)
-{
-    __shared__ double lds[NV * 16];
-    block_sum<NV, BS>(v, lds);
-    if (threadIdx.x == 0)
-#pragma unroll
-        for (int q = 0; q < NV; q++) part[(size_t)q * gridDim.x + blockIdx.x] = v[q];
-}
-
-// every thread of the workgroup gets sum_i part[q * stride + i], i < n, in a
-// fixed order (thread-strided partial sums, then the fixed block tree).
-// The partials were written by workgroups on every XCD, so each load misses
-// this XCD's L2: U loads per thread are issued before the first add, so a
-// thread waits for ceil(n / (U * BS)) round trips, not n / BS.
-// In two steps, so that a kernel can issue other loads between the first
-// round of partial loads and the wait for them: partials_issue loads the first
-// U * BS partials, partials_finish adds them, sums the rest and the block tree.
-template <int NV, int BS, int U>
-__device__ __forceinline__ void partials_issue(const double* part, int n, int stride, double (&v)[NV][U])
-{
-#pragma unroll
-    for (int q = 0; q < NV; q++)
-#pragma unroll
-        for (int k = 0; k < U; k++) {
-            const int i = threadIdx.x + k * BS;
-            v[q][k] = i < n ? part[(size_t)q * stride + i] : 0.0;
-        }
-}
-
-template <int NV, int BS, int U>
-__device__ __forceinline__ void partials_finish(const double* part, int n, int stride, const double (&v0)[NV][U],
-                                                double (&tot)[NV])
-{
-    __shared__ double lds[NV * 16];
-    __shared__ double res[NV];
-    // starting from v0[q][0] rather than 0.0 + v0[q][0]: an add of a constant
-    // to a conditional load is folded into the load's branch, which then waits
-    // for the load where it is issued
-    double acc[NV];
-#pragma unroll
-    for (int q = 0; q < NV; q++) {
-        acc[q] = v0[q][0];
-#pragma unroll
-        for (int k = 1; k < U; k++) acc[q] += v0[q][k];
-    }
-    for (int base = threadIdx.x + BS * U; base < n; base += BS * U) {
-        double v[NV][U];
-#pragma unroll
-        for (int q = 0; q < NV; q++)
-#pragma unroll
-            for (int k = 0; k < U; k++) {
-                const int i = base + k * BS;
-                v[q][k] = i < n ? part[(size_t)q * stride + i] : 0.0;
-            }
-#pragma unroll
-        for (int q = 0; q < NV; q++)
-#pragma unroll
-            for (int k = 0; k < U; k++) acc[q] += v[q][k];
-    }
-    block_sum<NV, BS>(acc, lds);
-    if (threadIdx.x == 0)
-#pragma unroll
-        for (int q = 0; q < NV; q++) res[q] = acc[q];
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < NV; q++) tot[q] = res[q];
-}
-
-template <int NV, int BS, int U = 16>
-__device__ __forceinline__ void sum_partials(const double* part, int n, int stride, double (&tot)[NV])
-{
-    double v[NV][U];
-    partials_issue<NV, BS, U>(part, n, stride, v);
-    partials_finish<NV, BS, U>(part, n, stride, v, tot);
-}
 
 // one workgroup: scal[slot + q] = sum of the NV partial arrays
 template <int NV>
@@ -1112,6 +1000,35 @@ hipError_t launch_cg_iter_tail(int pc, int64_t n3, float* x, float* r, float* z,
     const int gp = grid_for(n3 / 3, CG_P_BLOCK, CG_GRID);
     if (pc) hipLaunchKernelGGL(k_cg_p<1>, dim3(gp), dim3(CG_P_BLOCK), 0, st, n3, z, r, p, scal, part, g, done, test_kind, tol2);
     else hipLaunchKernelGGL(k_cg_p<0>, dim3(gp), dim3(CG_P_BLOCK), 0, st, n3, z, r, p, scal, part, g, done, test_kind, tol2);
+    return hipGetLastError();
+}
+
+// the two halves of the CG tail for a preconditioner applied between them
+// (multigrid): the update without z, then p from z and r.z partials the
+// caller wrote in the update's layout (2 x cg_grid)
+hipError_t launch_cg_update_only(int64_t n3, float* x, float* r, const float* p, const float* q, const int2* node_q,
+                                 const float* partial, double* scal, const double* pq_part, int n_pq, double* part,
+                                 int* done, int guard, hipStream_t st)
+{
+    const int g = cg_grid(n3 / 3);
+    hipLaunchKernelGGL(k_cg_update<0>, dim3(g), dim3(CG_BLOCK), 0, st, n3, x, r, nullptr, p, q, nullptr, node_q, partial,
+                       scal, pq_part, n_pq, part, done, guard);
+    return hipGetLastError();
+}
+
+hipError_t launch_cg_p_only(int64_t n3, const float* z, const float* r, float* p, double* scal, const double* part,
+                            int* done, int test_kind, double tol2, hipStream_t st)
+{
+    const int gp = grid_for(n3 / 3, CG_P_BLOCK, CG_GRID);
+    hipLaunchKernelGGL(k_cg_p<1>, dim3(gp), dim3(CG_P_BLOCK), 0, st, n3, z, r, p, scal, part, cg_grid(n3 / 3), done,
+                       test_kind, tol2);
+    return hipGetLastError();
+}
+
+// CG scalars from r.z, r.r partials (2 x n): rz, rz0, rr, rr0, iteration 0
+hipError_t launch_cg_scalars(const double* part, int n, double* scal, int* done, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_cg_init_finish, dim3(1), dim3(256), 0, st, part, n, scal, done);
     return hipGetLastError();
 }
 

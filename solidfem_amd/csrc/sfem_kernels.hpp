@@ -88,6 +88,12 @@ hipError_t launch_cg_iter_tail(int pc, int64_t n3, float* x, float* r, float* z,
                                const int2* node_q, const float* partial,
                                double* scal, const double* pq_part, int n_pq, double* part, int* done, int test_kind,
                                double tol2, int guard, hipStream_t st);
+hipError_t launch_cg_update_only(int64_t n3, float* x, float* r, const float* p, const float* q, const int2* node_q,
+                                 const float* partial, double* scal, const double* pq_part, int n_pq, double* part,
+                                 int* done, int guard, hipStream_t st);
+hipError_t launch_cg_p_only(int64_t n3, const float* z, const float* r, float* p, double* scal, const double* part,
+                            int* done, int test_kind, double tol2, hipStream_t st);
+hipError_t launch_cg_scalars(const double* part, int n, double* scal, int* done, hipStream_t st);
 hipError_t launch_dots3(int64_t n3, const double* a, const double* b, const float* d, double* scal, int slot, double* part,
                         hipStream_t st);
 hipError_t launch_axpy_state(int64_t n3, const double* u, const float* dx, double alpha, double* ut, float* u32, hipStream_t st);

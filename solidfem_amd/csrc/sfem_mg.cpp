@@ -1,0 +1,396 @@
+// sfem_mg.cpp -- the multigrid preconditioner of the converged Newton-PCG:
+// hierarchy set-up (sfem_mg_add_level), the level operators at each Newton
+// iterate, the V-cycle, and the PCG loop that applies it.
+//
+// Each coarse level is a full context of its own mesh (partition, element
+// kernels, block preconditioner) whose state is the level above's
+// displacement at the coinciding nodes, so every level's operator is the
+// matrix-free tangent of its own mesh at (an injection of) the current state
+// -- a rediscretisation, not a Galerkin product.  Smoothing is a Chebyshev
+// polynomial in M A (M the level's inverse 3x3 diagonal blocks, or the
+// learned blocks on the finest level) on [lmax / ratio, lmax], with lmax from
+// a power iteration at each Newton iterate.  Pre- and post-smoothing use the
+// same polynomial, so with a symmetric coarsest solve (Chebyshev again) the
+// V-cycle is a symmetric operator, as PCG needs.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <string>
+#include <vector>
+
+#include "sfem_capi.h"
+#include "sfem_internal.hpp"
+#include "sfem_kernels.hpp"
+#include "sfem_mg.hpp"
+
+using namespace sfem;
+
+MgLevel::~MgLevel() { delete c; }
+
+namespace {
+
+constexpr int MG_DEGREE = 2;            // Chebyshev degree of pre- and post-smoothing
+constexpr int MG_COARSE_DEGREE = 16;    // Chebyshev degree of the coarsest-level solve
+constexpr double MG_RATIO = 10.0;       // smoothed band [lmax / ratio, lmax]
+constexpr double MG_COARSE_RATIO = 100.0;
+constexpr double MG_SAFETY = 1.15;      // the power iteration estimates lmax from below
+constexpr int MG_POWER_ITERS = 12;
+
+struct LevelView {
+    sfem_ctx* c;           // mesh, partition, state of the level
+    const float* minv;     // smoother blocks
+    const float* b;        // right-hand side
+    float *x, *r, *d, *q;  // solution, residual, Chebyshev direction, operator image
+    double lmax;
+};
+
+LevelView level(sfem_ctx* f, int l)
+{
+    LevelView L;
+    if (l == 0) {
+        L.c = f;
+        L.minv = f->mg_minv0;
+        L.b = f->d_r32.as<float>();
+        L.x = f->d_z32.as<float>();
+        L.r = f->d_mg_r.as<float>();
+        L.d = f->d_mg_d.as<float>();
+        L.q = f->d_mg_q.as<float>();
+        L.lmax = f->mg_lmax0;
+    } else {
+        MgLevel& m = *f->mg[l - 1];
+        sfem_ctx* c = m.c;
+        L.c = c;
+        L.minv = c->d_minv.as<float>();
+        L.b = c->d_b32.as<float>();
+        L.x = c->d_x32.as<float>();
+        L.r = c->d_r32.as<float>();
+        L.d = c->d_p32.as<float>();
+        L.q = c->d_q32.as<float>();
+        L.lmax = m.lmax;
+    }
+    return L;
+}
+
+// out = K_l v (element rows and partial rows; operator_row assembles them)
+void matvec(const LevelView& L, const float* v, float* out)
+{
+    sfem_ctx* c = L.c;
+    HIPCK(launch_hessvec_cg(c->material, c->blk, c->d_u32.as<float>(), v, out, c->d_partial.as<float>(), nullptr, nullptr,
+                            c->stream, nullptr, nullptr));
+}
+
+struct Cheb {
+    double theta, delta, sigma;
+};
+
+Cheb cheb_band(double lmax, double ratio)
+{
+    const double hi = MG_SAFETY * lmax, lo = hi / ratio;
+    return Cheb{0.5 * (hi + lo), 0.5 * (hi - lo), (hi + lo) / (hi - lo)};
+}
+
+// Chebyshev smoothing of K x = b (Saad, Iterative Methods for Sparse Linear
+// Systems, 2nd ed., Alg. 12.1, with the preconditioner M): from x = 0, or from
+// the current x when r already holds b - K x_old and q = K e for x = x_old + e
+void smooth(const LevelView& L, int degree, bool from_zero, const Cheb& ch)
+{
+    sfem_ctx* c = L.c;
+    const int64_t nn = c->nn;
+    const int2* nq = c->d_node_info.as<int2>();
+    const float* part = c->d_partial.as<float>();
+    HIPCK(launch_mg_cheb(from_zero ? 0 : 2, nn, L.b, L.r, L.d, L.x, L.q, nq, part, L.minv, 0.f, (float)(1.0 / ch.theta),
+                         c->stream));
+    double rho = 1.0 / ch.sigma;
+    for (int i = 1; i < degree; i++) {
+        matvec(L, L.d, L.q);
+        const double rho1 = 1.0 / (2.0 * ch.sigma - rho);
+        HIPCK(launch_mg_cheb(1, nn, L.b, L.r, L.d, L.x, L.q, nq, part, L.minv, (float)(rho1 * rho),
+                             (float)(2.0 * rho1 / ch.delta), c->stream));
+        rho = rho1;
+    }
+}
+
+void vcycle(sfem_ctx* f, int l)
+{
+    const LevelView L = level(f, l);
+    sfem_ctx* c = L.c;
+    const int nlev = (int)f->mg.size();
+    if (l == nlev) {
+        smooth(L, MG_COARSE_DEGREE, true, cheb_band(L.lmax, MG_COARSE_RATIO));
+        return;
+    }
+    const Cheb ch = cheb_band(L.lmax, MG_RATIO);
+    smooth(L, MG_DEGREE, true, ch);
+    matvec(L, L.d, L.q);
+    HIPCK(launch_mg_cheb(3, c->nn, L.b, L.r, L.d, L.x, L.q, c->d_node_info.as<int2>(), c->d_partial.as<float>(), L.minv,
+                         0.f, 0.f, c->stream));
+    MgLevel& m = *f->mg[l];
+    const LevelView C = level(f, l + 1);
+    HIPCK(launch_mg_restrict(C.c->nn, m.t_ptr.as<int>(), m.t_row.as<int>(), m.t_val.as<float>(), L.r,
+                             C.c->d_fixed.as<uint8_t>(), const_cast<float*>(C.b), c->stream));
+    vcycle(f, l + 1);
+    HIPCK(launch_mg_prolong(c->nn, m.p_ptr.as<int>(), m.p_col.as<int>(), m.p_val.as<float>(), C.x,
+                            c->d_fixed.as<uint8_t>(), L.d, L.x, c->stream));
+    matvec(L, L.d, L.q);
+    smooth(L, MG_DEGREE, false, ch);
+}
+
+// element evaluations of one V-cycle (matvecs per level x its elements)
+int64_t vcycle_evals(sfem_ctx* f)
+{
+    const int nlev = (int)f->mg.size();
+    int64_t n = 0;
+    for (int l = 0; l <= nlev; l++) {
+        const LevelView L = level(f, l);
+        const int mv = l == nlev ? MG_COARSE_DEGREE - 1 : 2 * MG_DEGREE;
+        n += (int64_t)mv * L.c->ne;
+    }
+    return n;
+}
+
+}  // namespace
+
+namespace sfem {
+
+void mg_prepare(sfem_ctx* f, const float* minv0)
+{
+    hipStream_t s = f->stream;
+    const size_t n3 = 3 * (size_t)f->nn;
+    if (!f->d_mg_r.p) {
+        f->d_mg_r.alloc(n3 * sizeof(float));
+        f->d_mg_d.alloc(n3 * sizeof(float));
+        f->d_mg_q.alloc(n3 * sizeof(float));
+        f->d_mg_part.alloc(1024 * sizeof(double));
+    }
+    const int nlev = (int)f->mg.size();
+    if (!f->d_mg_norm.p || f->d_mg_norm.n < 2 * (size_t)(nlev + 1) * sizeof(double))
+        f->d_mg_norm.alloc(2 * (size_t)(nlev + 1) * sizeof(double));
+    f->mg_minv0 = minv0;
+    // coarse states and block preconditioners, finest to coarsest
+    const double* ua = f->d_u64.as<double>();
+    for (int l = 0; l < nlev; l++) {
+        sfem_ctx* c = f->mg[l]->c;
+        HIPCK(launch_mg_inject(c->nn, f->mg[l]->inj.as<int>(), ua, c->d_u64.as<double>(), c->d_u32.as<float>(), s));
+        build_jacobi(c);
+        ua = c->d_u64.as<double>();
+    }
+    // lmax(M A) per level: K unnormalised power steps from a hashed start
+    // (the growth of K steps stays far inside FP32 range for lmax of a few
+    // units), lmax ~ |v_K| / |v_{K-1}|
+    double* norms = f->d_mg_norm.as<double>();
+    for (int l = 0; l <= nlev; l++) {
+        const LevelView L = level(f, l);
+        sfem_ctx* c = L.c;
+        float* v = L.d;
+        float* y = L.r;
+        HIPCK(launch_mg_start(c->nn, c->d_fixed.as<uint8_t>(), v, s));
+        for (int it = 0; it < MG_POWER_ITERS; it++) {
+            matvec(L, v, L.q);
+            HIPCK(launch_mg_power(c->nn, L.q, c->d_node_info.as<int2>(), c->d_partial.as<float>(), L.minv, y, s));
+            std::swap(v, y);
+        }
+        HIPCK(launch_mg_sqnorm(3 * c->nn, v, f->d_mg_part.as<double>(), norms + 2 * l, s));
+        HIPCK(launch_mg_sqnorm(3 * c->nn, y, f->d_mg_part.as<double>(), norms + 2 * l + 1, s));
+    }
+    std::vector<double> h(2 * (nlev + 1));
+    HIPCK(hipMemcpyAsync(h.data(), norms, h.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCK(hipStreamSynchronize(s));
+    for (int l = 0; l <= nlev; l++) {
+        const double lm = h[2 * l + 1] > 0 ? std::sqrt(h[2 * l] / h[2 * l + 1]) : 0.0;
+        if (!(lm > 0) || !std::isfinite(lm)) throw std::runtime_error("multigrid: no eigenvalue estimate on level " +
+                                                                      std::to_string(l));
+        if (l == 0) f->mg_lmax0 = lm;
+        else f->mg[l - 1]->lmax = lm;
+    }
+    f->mg_evals = vcycle_evals(f);
+}
+
+void mg_vcycle(sfem_ctx* f) { vcycle(f, 0); }
+
+// PCG with one V-cycle per iteration: the element kernel (q = K p, p.q
+// partials), the CG update without z, z = V(r), r.z partials, then beta,
+// the convergence test and p.  The host reads the done flag every iteration:
+// an iteration past convergence would cost a whole V-cycle.
+CgResult mg_pcg(sfem_ctx* c, int maxit, double tol2, int64_t* hv_evals)
+{
+    const int64_t n3 = 3 * c->nn;
+    hipStream_t s = c->stream;
+    double* scal = c->d_scal.as<double>();
+    double* parts = c->d_partials.as<double>();
+    double* pq_elem = parts + 2 * CG_GRID;
+    int* done = c->d_done.as<int>();
+    float* x = c->d_x32.as<float>();
+    float* r = c->d_r32.as<float>();
+    float* z = c->d_z32.as<float>();
+    float* p = c->d_p32.as<float>();
+    float* q = c->d_q32.as<float>();
+    HIPCK(launch_cg_init(0, n3, c->d_b32.as<float>(), x, r, z, p, c->d_minv.as<float>(), c->d_fixed.as<uint8_t>(), scal,
+                         parts, done, s));
+    mg_vcycle(c);
+    HIPCK(launch_mg_rz(c->nn, r, z, parts, s));
+    HIPCK(launch_cg_scalars(parts, cg_grid(c->nn), scal, done, s));
+    HIPCK(launch_mg_copy(n3, z, p, s));
+    int dflag = 0, launched = 0;
+    while (launched < maxit) {
+        HIPCK(hipMemcpyAsync(&dflag, done, sizeof(int), hipMemcpyDeviceToHost, s));
+        HIPCK(hipStreamSynchronize(s));
+        if (dflag) break;
+        HIPCK(launch_hessvec_cg(c->material, c->blk, c->d_u32.as<float>(), p, q, c->d_partial.as<float>(), done, pq_elem, s,
+                                nullptr, nullptr));
+        HIPCK(launch_cg_update_only(n3, x, r, p, q, c->d_node_info.as<int2>(), c->d_partial.as<float>(), scal, pq_elem,
+                                    c->nblk, parts, done, 1, s));
+        mg_vcycle(c);
+        HIPCK(launch_mg_rz(c->nn, r, z, parts, s));
+        HIPCK(launch_cg_p_only(n3, z, r, p, scal, parts, done, 1, tol2, s));
+        launched++;
+    }
+    if (!dflag) {
+        HIPCK(hipMemcpyAsync(&dflag, done, sizeof(int), hipMemcpyDeviceToHost, s));
+        HIPCK(hipStreamSynchronize(s));
+    }
+    double hs[S_COUNT];
+    HIPCK(hipMemcpy(hs, scal, sizeof hs, hipMemcpyDeviceToHost));
+    CgResult res;
+    res.iters = (int)hs[S_ITER];
+    res.status = dflag;
+    res.rr0 = hs[S_RR0];
+    res.rr = hs[S_RR];
+    if (hv_evals) *hv_evals += (int64_t)res.iters * (c->ne + c->mg_evals) + c->mg_evals;
+    return res;
+}
+
+}  // namespace sfem
+
+// ======================================================================= C-ABI
+extern "C" {
+
+int sfem_mg_add_level(sfem_ctx* f, const sfem_mesh* coarse, const int32_t* inject, const int64_t* p_ptr,
+                      const int32_t* p_col, const double* p_val)
+{
+    if (!f || !coarse || !inject || !p_ptr || !p_col || !p_val) return fail(SFEM_EINVAL, "sfem_mg_add_level: null argument");
+    if (f->per_element)
+        return fail(SFEM_EINVAL, "sfem_mg_add_level: coarse levels inherit a uniform material; this context has per-element "
+                                 "moduli");
+    sfem_ctx* above = f->mg.empty() ? f : f->mg.back()->c;
+    const int64_t na = above->nn, nc = coarse->nn;
+    if (nc <= 0 || nc >= na) return fail(SFEM_EINVAL, "sfem_mg_add_level: a coarse level has fewer nodes than the one above");
+    for (int64_t k = 0; k < nc; k++)
+        if (inject[k] < 0 || inject[k] >= na) return fail(SFEM_EINVAL, "sfem_mg_add_level: injection index out of range");
+    if (p_ptr[0] != 0) return fail(SFEM_EINVAL, "sfem_mg_add_level: p_ptr[0] != 0");
+    for (int64_t i = 0; i < na; i++)
+        if (p_ptr[i + 1] < p_ptr[i]) return fail(SFEM_EINVAL, "sfem_mg_add_level: p_ptr not monotone");
+    const int64_t nnz = p_ptr[na];
+    if (nnz >= (int64_t)1 << 31) return fail(SFEM_EINVAL, "sfem_mg_add_level: prolongation too large");
+    for (int64_t k = 0; k < nnz; k++)
+        if (p_col[k] < 0 || p_col[k] >= nc) return fail(SFEM_EINVAL, "sfem_mg_add_level: prolongation column out of range");
+    sfem_material mat{};
+    mat.material = f->material;
+    mat.young = f->young;
+    mat.poisson = f->poisson;
+    mat.density = f->density;
+    sfem_ctx* lc = nullptr;
+    int rc = sfem_create(coarse, &mat, &lc);
+    if (rc != SFEM_OK) return rc;
+    return guard_call("sfem_mg_add_level", [&]() -> int {
+        std::unique_ptr<MgLevel> m(new MgLevel);
+        m->c = lc;
+        HIPCK(hipStreamSynchronize(lc->stream));
+        HIPCK(hipStreamDestroy(lc->stream));
+        lc->stream = f->stream;
+        lc->owns_stream = false;
+        std::vector<int> inj(inject, inject + nc);
+        m->inj.upload(inj);
+        std::vector<int> ptr(na + 1), col(nnz);
+        std::vector<float> val(nnz);
+        for (int64_t i = 0; i <= na; i++) ptr[i] = (int)p_ptr[i];
+        for (int64_t k = 0; k < nnz; k++) {
+            col[k] = p_col[k];
+            val[k] = (float)p_val[k];
+        }
+        // transpose: for each coarse node its (fine row, weight) entries in
+        // increasing fine-row order (the restriction's fixed summation order)
+        std::vector<int> tptr(nc + 1, 0), trow(nnz);
+        std::vector<float> tval(nnz);
+        for (int64_t k = 0; k < nnz; k++) tptr[col[k] + 1]++;
+        for (int64_t j = 0; j < nc; j++) tptr[j + 1] += tptr[j];
+        std::vector<int> fill(tptr.begin(), tptr.end() - 1);
+        for (int64_t i = 0; i < na; i++)
+            for (int k = ptr[i]; k < ptr[i + 1]; k++) {
+                const int slot = fill[col[k]]++;
+                trow[slot] = (int)i;
+                tval[slot] = val[k];
+            }
+        m->p_ptr.upload(ptr);
+        m->p_col.upload(col);
+        m->p_val.upload(val);
+        m->t_ptr.upload(tptr);
+        m->t_row.upload(trow);
+        m->t_val.upload(tval);
+        f->mg.push_back(std::move(m));
+        return SFEM_OK;
+    });
+}
+
+int sfem_mg_clear(sfem_ctx* f)
+{
+    if (!f) return fail(SFEM_EINVAL, "sfem_mg_clear: null context");
+    return guard_call("sfem_mg_clear", [&]() -> int {
+        HIPCK(hipStreamSynchronize(f->stream));
+        f->mg.clear();
+        return SFEM_OK;
+    });
+}
+
+int sfem_mg_info(sfem_ctx* f, int32_t* nlevels, double* lmax)
+{
+    if (!f || !nlevels) return fail(SFEM_EINVAL, "sfem_mg_info: null argument");
+    *nlevels = (int32_t)f->mg.size() + 1;
+    if (lmax) {
+        lmax[0] = f->mg_lmax0;
+        for (size_t l = 0; l < f->mg.size(); l++) lmax[l + 1] = f->mg[l]->lmax;
+    }
+    return SFEM_OK;
+}
+
+int sfem_mg_apply(sfem_ctx* f, const double* r, double* z)
+{
+    if (!f || !r || !z) return fail(SFEM_EINVAL, "sfem_mg_apply: null argument");
+    if (f->mg.empty()) return fail(SFEM_ESTATE, "sfem_mg_apply: no coarse levels (sfem_mg_add_level)");
+    return guard_call("sfem_mg_apply", [&]() -> int {
+        const int64_t n3 = 3 * f->nn;
+        std::vector<float> h(n3);
+        for (int64_t i = 0; i < n3; i++) h[i] = f->fixed_h[i / 3] ? 0.f : (float)r[i];
+        build_jacobi(f);
+        mg_prepare(f, f->d_minv.as<float>());
+        HIPCK(hipMemcpyAsync(f->d_r32.p, h.data(), n3 * sizeof(float), hipMemcpyHostToDevice, f->stream));
+        mg_vcycle(f);
+        HIPCK(hipMemcpyAsync(h.data(), f->d_z32.p, n3 * sizeof(float), hipMemcpyDeviceToHost, f->stream));
+        HIPCK(hipStreamSynchronize(f->stream));
+        for (int64_t i = 0; i < n3; i++) z[i] = h[i];
+        return SFEM_OK;
+    });
+}
+
+int sfem_mg_pcg(sfem_ctx* f, const double* b, double* x, double rtol, int32_t maxit, int32_t* iters, int32_t* status)
+{
+    if (!f || !b || !x || !iters || !status || maxit <= 0) return fail(SFEM_EINVAL, "sfem_mg_pcg: bad argument");
+    if (f->mg.empty()) return fail(SFEM_ESTATE, "sfem_mg_pcg: no coarse levels (sfem_mg_add_level)");
+    return guard_call("sfem_mg_pcg", [&]() -> int {
+        const int64_t n3 = 3 * f->nn;
+        std::vector<float> h(n3);
+        for (int64_t i = 0; i < n3; i++) h[i] = f->fixed_h[i / 3] ? 0.f : (float)b[i];
+        build_jacobi(f);
+        mg_prepare(f, f->d_minv.as<float>());
+        HIPCK(hipMemcpyAsync(f->d_b32.p, h.data(), n3 * sizeof(float), hipMemcpyHostToDevice, f->stream));
+        CgResult cg = mg_pcg(f, maxit, rtol * rtol, nullptr);
+        HIPCK(hipMemcpyAsync(h.data(), f->d_x32.p, n3 * sizeof(float), hipMemcpyDeviceToHost, f->stream));
+        HIPCK(hipStreamSynchronize(f->stream));
+        for (int64_t i = 0; i < n3; i++) x[i] = h[i];
+        *iters = cg.iters;
+        *status = cg.status;
+        return SFEM_OK;
+    });
+}
+
+}  // extern "C"

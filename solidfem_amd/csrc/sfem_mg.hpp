@@ -1,0 +1,27 @@
+// sfem_mg.hpp -- launchers of the multigrid preconditioner kernels (sfem_mg.hip)
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace sfem {
+
+// Chebyshev step: mode 0 first step from x = 0, 1 next step, 2 first step from
+// x after a coarse correction, 3 residual only (see k_mg_cheb)
+hipError_t launch_mg_cheb(int mode, int64_t nn, const float* b, float* r, float* d, float* x, const float* q,
+                          const int2* node_q, const float* partial, const float* minv, float c1, float c2, hipStream_t st);
+hipError_t launch_mg_prolong(int64_t nn, const int* ptr, const int* col, const float* val, const float* xc,
+                             const uint8_t* fixed, float* e, float* x, hipStream_t st);
+hipError_t launch_mg_restrict(int64_t nnc, const int* ptr, const int* row, const float* val, const float* r,
+                              const uint8_t* fixed, float* bc, hipStream_t st);
+hipError_t launch_mg_inject(int64_t nnc, const int* inj, const double* ua, double* u64, float* u32, hipStream_t st);
+hipError_t launch_mg_power(int64_t nn, const float* q, const int2* node_q, const float* partial, const float* minv, float* y,
+                           hipStream_t st);
+hipError_t launch_mg_start(int64_t nn, const uint8_t* fixed, float* v, hipStream_t st);
+// out[0] = a.a (FP64, fixed order); part holds mg_sqnorm_grid(n3) doubles
+int mg_sqnorm_grid(int64_t n3);
+hipError_t launch_mg_sqnorm(int64_t n3, const float* a, double* part, double* out, hipStream_t st);
+// r.z, r.r partials in the CG update's layout (2 x cg_grid(nn))
+hipError_t launch_mg_rz(int64_t nn, const float* r, const float* z, double* part, hipStream_t st);
+hipError_t launch_mg_copy(int64_t n, const float* a, float* b, hipStream_t st);
+
+}  // namespace sfem

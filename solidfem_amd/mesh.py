@@ -169,3 +169,54 @@ def load_tet1(path, scale=1.0):
             elif line.startswith("t"):
                 T.append(tuple(int(v) for v in line.split()[1:5]))
     return np.array(X, dtype=np.float64), np.array(T, dtype=np.int32)
+
+
+def beam_hierarchy(m, min_cells=2, fixed=None):
+    """Nested coarse levels of a beam mesh for the multigrid preconditioner:
+    each level halves the hex grid in every direction while all three cell
+    counts stay even and the thinner cross-section keeps >= min_cells cells.
+    Coarse node (i, j, k) sits at fine node (2i, 2j, 2k) (its position, so a
+    jittered mesh stays nested), and the prolongation interpolates node
+    values trilinearly in the grid indices.
+    Returns [(X0, tets, fixed, inject, p_ptr, p_col, p_val)], finest coarse
+    level first (the arguments of capi.Solver.add_mg_level)."""
+    levels = []
+    fine, ffix = m, (m.fixed_mask() if fixed is None else np.asarray(fixed, dtype=np.uint8))
+    while (fine.nx % 2 == 0 and fine.ny % 2 == 0 and fine.nz % 2 == 0
+           and min(fine.ny, fine.nz) // 2 >= min_cells):
+        c = BeamMesh(fine.nx // 2, fine.ny // 2, fine.nz // 2, *fine.L)
+        g = c.grid_index
+        inject = fine.node_id(2 * g[:, 0], 2 * g[:, 1], 2 * g[:, 2]).astype(np.int32)
+        c.X = fine.X[inject].copy()
+        tets = c.tets.astype(np.int64)
+        P = c.X[tets]
+        vol = np.einsum("ij,ij->i", np.cross(P[:, 1] - P[:, 0], P[:, 2] - P[:, 0]), P[:, 3] - P[:, 0])
+        neg = vol < 0
+        tets[neg, 1], tets[neg, 2] = tets[neg, 2].copy(), tets[neg, 1].copy()
+        c.tets = np.ascontiguousarray(tets.astype(np.int32))
+        cfix = ffix[inject].astype(np.uint8)
+        # trilinear prolongation rows (fine nodes) over the coarse grid
+        gf = fine.grid_index
+        lo = gf // 2
+        odd = gf % 2
+        rows, cols, vals = [], [], []
+        for cx in (0, 1):
+            for cy in (0, 1):
+                for cz in (0, 1):
+                    sel = np.array([cx, cy, cz])
+                    ok = np.all((sel == 0) | (odd == 1), axis=1)
+                    w = np.prod(np.where(odd == 1, 0.5, 1.0), axis=1)
+                    idx = lo + sel * odd
+                    n = np.nonzero(ok)[0]
+                    rows.append(n)
+                    cols.append(c.node_id(idx[n, 0], idx[n, 1], idx[n, 2]))
+                    vals.append(w[n])
+        rows, cols, vals = np.concatenate(rows), np.concatenate(cols), np.concatenate(vals)
+        order = np.lexsort((cols, rows))
+        rows, cols, vals = rows[order], cols[order], vals[order]
+        p_ptr = np.zeros(fine.nn + 1, dtype=np.int64)
+        np.add.at(p_ptr, rows + 1, 1)
+        p_ptr = np.cumsum(p_ptr)
+        levels.append((c.X, c.tets, cfix, inject, p_ptr, cols.astype(np.int32), vals.astype(np.float64)))
+        fine, ffix = c, cfix
+    return levels

@@ -1,0 +1,143 @@
+"""GPU tests of the multigrid preconditioner (SFEM_PC_MULTIGRID*), through the
+C-ABI.
+
+The V-cycle is a preconditioner, not a reference function: what is pinned is
+(a) that it is a symmetric positive operator (PCG needs one), bitwise
+reproducible, (b) that PCG with it reaches the same FP64-converged
+displacements as the FP64 oracle's Newton solve (1e-6 relative L2, the stated
+target) with the oracle's own residual below the tolerance, and (c) that it
+needs far fewer CG iterations than block-Jacobi PCG."""
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from solidfem_amd import capi
+from solidfem_amd.mesh import BeamMesh, beam_hierarchy, traction_nodal_forces
+
+pytestmark = pytest.mark.gpu
+
+
+def rel2(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+def beam(grid, mg=True):
+    m = BeamMesh(*grid)
+    fixed = m.fixed_mask()
+    s = capi.Solver(m.X, m.tets, fixed, capi.NEO_HOOKEAN, 1e6, 0.45, 1070.0, gravity=(0.0, 0.0, -9.8))
+    tris = m.tip_triangles()
+    f = traction_nodal_forces(m.X, tris, lambda p: np.tile([0.0, 0.0, -1e4], (len(p), 1)))
+    s.set_external_forces(f)
+    if mg:
+        s.set_multigrid(beam_hierarchy(m))
+    return m, fixed, f, s
+
+
+def bent(m, fixed, amp=0.05):
+    """a smooth bending state (no inverted elements)"""
+    x = m.X.copy()
+    x[:, 2] -= amp * m.X[:, 0] ** 2
+    return x
+
+
+def test_vcycle_symmetric_positive_deterministic():
+    print("case 32x8x8 V-cycle", flush=True)
+    m, fixed, f, s = beam((32, 8, 8))
+    s.set_positions(bent(m, fixed))
+    rng = np.random.default_rng(3)
+    free = (1 - fixed)[:, None]
+    r1 = rng.standard_normal(m.X.shape) * free
+    r2 = rng.standard_normal(m.X.shape) * free
+    z1, z2 = s.mg_apply(r1), s.mg_apply(r2)
+    lm = s.mg_info()
+    print(f"levels {len(lm)} lmax {lm}", flush=True)
+    assert len(lm) == 3 and np.all(lm > 0)
+    assert np.array_equal(z1, s.mg_apply(r1))
+    assert np.all(z1[fixed == 1] == 0)
+    a, b = np.vdot(r2, z1), np.vdot(r1, z2)
+    scale = np.sqrt(np.vdot(r1, z1) * np.vdot(r2, z2))
+    print(f"asymmetry {abs(a - b) / scale:.2e}", flush=True)
+    assert abs(a - b) <= 1e-4 * scale
+    for k in range(4):
+        r = rng.standard_normal(m.X.shape) * free
+        assert np.vdot(r, s.mg_apply(r)) > 0
+
+
+def _weights_or_skip():
+    from solidfem_amd import learned
+    if not os.path.exists(learned.WEIGHTS):
+        pytest.skip("no trained weights")
+    return learned.load()
+
+
+@pytest.mark.parametrize("precond", [capi.PC_MULTIGRID, capi.PC_MULTIGRID_LEARNED])
+def test_multigrid_pcg_matches_oracle(precond):
+    print(f"case 32x8x8 NH two load steps precond={precond}", flush=True)
+    m, fixed, f, s = beam((32, 8, 8))
+    if precond == capi.PC_MULTIGRID_LEARNED:
+        net = _weights_or_skip()
+        net.attach(s)
+    p = O.OracleProblem(m.X, m.tets, fixed, capi.NEO_HOOKEAN, 1e6, 0.45, 1070.0, gravity=(0.0, 0.0, -9.8), fext=f)
+    xo = m.X.copy()
+    it_mg = 0
+    for k in range(2):
+        # no load-step predictor: a predicted start far from equilibrium can
+        # make the tangent indefinite, and the breakdown falls back to Jacobi
+        st = s.step(tol=1e-8, precond=precond, sim_type=capi.ST_QUASI_STATIC, force_step=0.5, predictor=0)
+        assert st["converged"] == 1 and st["fallbacks"] == 0
+        it_mg += st["cg_iters"]
+        xo, _ = p.newton_solve(xo, 0.5 * (k + 1), tol=1e-10, cg_rtol=1e-8)
+    x = s.positions()
+    err = rel2(x - m.X, xo - m.X)
+    res = np.linalg.norm(p.gradients(x, 1.0)) / np.linalg.norm(p.external(x, 1.0))
+    # the same solve with block-Jacobi PCG
+    _, _, _, sj = beam((32, 8, 8), mg=False)
+    it_j = 0
+    for k in range(2):
+        st = sj.step(tol=1e-8, precond=capi.PC_JACOBI, sim_type=capi.ST_QUASI_STATIC, force_step=0.5, predictor=0)
+        it_j += st["cg_iters"]
+    print(f"err {err:.3e} oracle-res {res:.3e} cg mg {it_mg} jacobi {it_j}", flush=True)
+    assert res < 1e-8
+    assert err < 1e-6
+    assert it_mg < 0.2 * it_j
+
+
+def test_multigrid_linear_solve():
+    """one PCG solve K x = b: the FP32 recursion reaches the tolerance in a
+    mesh-independent number of iterations, and K x matches b to the accuracy
+    an FP32 operator can show (its rounding on the smooth solution)"""
+    its = []
+    for grid in ((16, 4, 4), (32, 8, 8), (64, 16, 16)):
+        print(f"case {grid} linear solve", flush=True)
+        m, fixed, f, s = beam(grid)     # at rest: the tangent is linear elasticity (SPD)
+        b = np.random.default_rng(1).standard_normal(m.X.shape) * (1 - fixed[:, None])
+        x, it, status = s.mg_pcg(b, rtol=1e-5, maxit=200)
+        res = np.linalg.norm(b - s.matvec(x)) / np.linalg.norm(b)
+        print(f"  iters {it} status {status} rel res {res:.2e}", flush=True)
+        assert status == 1 and res < 5e-3
+        its.append(it)
+    assert max(its) <= 60 and its[-1] <= 1.5 * its[0]
+
+
+def test_multigrid_errors():
+    m, fixed, f, s = beam((16, 4, 4), mg=False)
+    with pytest.raises(capi.SfemError):
+        s.solve(tol=1e-8, precond=capi.PC_MULTIGRID)          # no coarse levels
+    lv = beam_hierarchy(m)
+    X0, tets, fx, inj, ptr, col, val = lv[0]
+    bad = inj.copy()
+    bad[0] = m.nn
+    with pytest.raises(capi.SfemError):
+        s.add_mg_level(X0, tets, fx, bad, ptr, col, val)
+    bad = col.copy()
+    bad[0] = X0.shape[0]
+    with pytest.raises(capi.SfemError):
+        s.add_mg_level(X0, tets, fx, inj, ptr, bad, val)
+    E = np.full(m.ne, 1e6)
+    sp = capi.Solver(m.X, m.tets, fixed, capi.NEO_HOOKEAN, 1e6, 0.45, 1070.0, young_e=E)
+    with pytest.raises(capi.SfemError):
+        sp.add_mg_level(*lv[0])                                 # per-element moduli
+    s.set_multigrid(lv)
+    assert len(s.mg_info()) == 2

@@ -1,0 +1,52 @@
+"""CPU checks of the nested beam hierarchy the multigrid preconditioner is
+built on (solidfem_amd.mesh.beam_hierarchy): level sizes, nesting (coarse
+nodes at fine nodes), the prolongation's exactness on fields trilinear in the
+grid indices (so constants and linear fields in the unjittered geometry), and
+element orientation."""
+import numpy as np
+
+from solidfem_amd.mesh import BeamMesh, beam_hierarchy
+
+
+def dense_p(nf, nc, ptr, col, val):
+    P = np.zeros((nf, nc))
+    for i in range(nf):
+        P[i, col[ptr[i]:ptr[i + 1]]] = val[ptr[i]:ptr[i + 1]]
+    return P
+
+
+def test_levels_of_the_bench_beam():
+    m = BeamMesh(128, 32, 32)
+    lv = beam_hierarchy(m)
+    assert [l[1].shape[0] for l in lv] == [98304, 12288, 1536, 192]
+    assert [l[0].shape[0] for l in lv] == [65 * 17 * 17, 33 * 9 * 9, 17 * 5 * 5, 9 * 3 * 3]
+
+
+def test_prolongation_nesting_and_orientation():
+    m = BeamMesh(16, 8, 4, jitter=0.1, seed=7)
+    lv = beam_hierarchy(m)
+    assert len(lv) == 1
+    X0, tets, fix, inj, ptr, col, val = lv[0]
+    c = BeamMesh(8, 4, 2)
+    P = dense_p(m.nn, X0.shape[0], ptr, col, val)
+    # coarse nodes coincide with fine nodes, where P is the identity row
+    assert np.array_equal(X0, m.X[inj])
+    assert np.allclose(P[inj], np.eye(X0.shape[0]))
+    assert np.allclose(P.sum(axis=1), 1.0)
+    # exact on fields trilinear in the grid indices
+    gf, gc = m.grid_index.astype(float), 2.0 * c.grid_index.astype(float)
+    for f in (lambda g: 1.0 + 0 * g[:, 0], lambda g: 2 * g[:, 0] - g[:, 1] + 0.5 * g[:, 2],
+              lambda g: g[:, 0] * g[:, 1] * g[:, 2]):
+        assert np.abs(P @ f(gc) - f(gf)).max() < 1e-12
+    # fixed flags are the fine flags at the coinciding nodes; positive volumes
+    assert np.array_equal(fix, m.fixed_mask()[inj])
+    Pt = X0[tets]
+    vol = np.einsum("ij,ij->i", np.cross(Pt[:, 1] - Pt[:, 0], Pt[:, 2] - Pt[:, 0]), Pt[:, 3] - Pt[:, 0])
+    assert np.all(vol > 0)
+    # every coarse element lies in the fine mesh's box
+    assert np.all(X0.min(axis=0) >= m.X.min(axis=0)) and np.all(X0.max(axis=0) <= m.X.max(axis=0))
+
+
+def test_no_level_for_odd_or_thin_grids():
+    assert beam_hierarchy(BeamMesh(10, 3, 3)) == []
+    assert beam_hierarchy(BeamMesh(16, 2, 2)) == []
