@@ -25,7 +25,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 from solidfem_amd import capi  # noqa: E402
-from solidfem_amd.mesh import BeamMesh, traction_nodal_forces  # noqa: E402
+from solidfem_amd.mesh import BeamMesh, beam_hierarchy, traction_nodal_forces  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 # every TIME_EVERY-th Hessian-vector launch of the timed solves is bracketed by
@@ -131,8 +131,11 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--precond", choices=["both", "jacobi", "none", "learned"], default="both",
-                    help="both: the Jacobi-PCG solve is `value`, the learned-PCG solve is reported beside it")
+    ap.add_argument("--precond", choices=["all", "both", "jacobi", "none", "learned", "multigrid", "multigrid_learned"],
+                    default="all",
+                    help="all: the Jacobi-PCG solve is `value`; the learned-PCG solves (learned blocks alone, and "
+                         "as the finest smoother of the multigrid preconditioner) and the multigrid-PCG solve with "
+                         "block-Jacobi smoothers are reported beside it; both: Jacobi and learned blocks only")
     ap.add_argument("--load-steps", type=int, default=10)
     ap.add_argument("--grid", default="128x32x32")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -180,37 +183,53 @@ def main():
         ms, n = s.time_hessvec(0)
         return dt, runs, s.positions(), (ms, n)
 
-    main_pc = {"jacobi": capi.PC_JACOBI, "both": capi.PC_JACOBI, "none": capi.PC_NONE,
-               "learned": capi.PC_LEARNED}[args.precond]
+    main_pc = {"jacobi": capi.PC_JACOBI, "both": capi.PC_JACOBI, "all": capi.PC_JACOBI, "none": capi.PC_NONE,
+               "learned": capi.PC_LEARNED, "multigrid": capi.PC_MULTIGRID,
+               "multigrid_learned": capi.PC_MULTIGRID_LEARNED}[args.precond]
     net = None
-    if args.precond in ("learned", "both"):
+    if args.precond in ("learned", "multigrid_learned", "both", "all"):
         from solidfem_amd import learned
         if os.path.exists(learned.WEIGHTS):
             net = learned.load()
             net.attach(s)
-        elif args.precond == "learned":
+        elif args.precond in ("learned", "multigrid_learned"):
             raise SystemExit(f"no trained weights at {learned.WEIGHTS}")
+    if args.precond in ("multigrid", "multigrid_learned", "all"):
+        s.set_multigrid(beam_hierarchy(m))
     dt, runs, x, (ms_elem, n_timed) = leg(main_pc)
     evals = sum(r["hessvec_evals"] + r["force_evals"] for r in runs)
     cg = sum(r["cg_iters"] for r in runs)
-    learned_leg = None
-    if args.precond == "both":
+
+    def side_leg(pc):
+        dtl, runs_l, xl, _ = leg(pc)
+        ev_l = sum(r["hessvec_evals"] + r["force_evals"] for r in runs_l)
+        return {
+            "newton_solve_s": dtl / args.steps, "value": ev_l / dtl, "unit": "elem-evals/s",
+            "newton_iters": runs_l[-1]["newton_iters"], "cg_iters": runs_l[-1]["cg_iters"],
+            "fallbacks": runs_l[-1]["fallbacks"], "converged": bool(all(r["converged"] for r in runs_l)),
+            "final_rel_residual": runs_l[-1]["res"] / runs_l[-1]["ext_norm"],
+            "speedup_vs_jacobi": dt / dtl,
+            "displacement_rel_diff_vs_jacobi": float(np.linalg.norm(xl - x) / np.linalg.norm(x - m.X)),
+        }
+
+    extra = {}
+    if args.precond in ("both", "all"):
         if net is None:
-            learned_leg = {"skipped": "no trained weights in the package"}
+            extra["learned_pcg"] = {"skipped": "no trained weights in the package"}
         else:
-            dtl, runs_l, xl, _ = leg(capi.PC_LEARNED)
-            ev_l = sum(r["hessvec_evals"] + r["force_evals"] for r in runs_l)
-            learned_leg = {
-                "newton_solve_s": dtl / args.steps, "value": ev_l / dtl, "unit": "elem-evals/s",
-                "newton_iters": runs_l[-1]["newton_iters"], "cg_iters": runs_l[-1]["cg_iters"],
-                "fallbacks": runs_l[-1]["fallbacks"], "converged": bool(all(r["converged"] for r in runs_l)),
-                "final_rel_residual": runs_l[-1]["res"] / runs_l[-1]["ext_norm"],
-                "speedup_vs_jacobi": dt / dtl,
-                "displacement_rel_diff_vs_jacobi": float(np.linalg.norm(xl - x) / np.linalg.norm(x - m.X)),
-            }
+            extra["learned_pcg"] = {"block": side_leg(capi.PC_LEARNED)}
+            if args.precond == "all":
+                extra["learned_pcg"]["multigrid"] = side_leg(capi.PC_MULTIGRID_LEARNED)
+        if args.precond == "all":
+            extra["multigrid_pcg"] = side_leg(capi.PC_MULTIGRID)
 
     info = s.info()
     bytes_alg = hessvec_bytes(info["ne"], info["nn"])
+    timing = "in-solve"
+    if n_timed == 0:
+        # the multigrid loops launch the kernel without events: time it back to back
+        ms_elem, _ = s.profile_hessvec(200)
+        timing = "back-to-back"
     achieved = bytes_alg / (ms_elem * 1e-3) / 1e9
     line = {
         "metric": "Newton-CG time to 1e-8 rel. residual (learned-PCG vs Jacobi-PCG); elem-evals/s",
@@ -229,7 +248,8 @@ def main():
                    "grid": args.grid, "tets": info["ne"], "nodes": info["nn"], "material": "neo-hookean",
                    "E": 1e6, "nu": 0.45, "loads": "gravity -z + dead tip traction 1e4 Pa -z",
                    "load_steps": args.load_steps, "tol": 1e-8,
-                   "precond": "jacobi" if args.precond == "both" else args.precond, "parallelism": "none"},
+                   "precond": "jacobi" if args.precond in ("both", "all") else args.precond,
+                   "parallelism": "none"},
         "newton_solve_s": dt / args.steps,
         "newton_iters": runs[-1]["newton_iters"],
         "cg_iters": runs[-1]["cg_iters"],
@@ -240,11 +260,10 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(grid),
                      "kernel": "k_elem_block<0,NEO_HOOKEAN,256> (Hessian-vector element blocks)",
-                     "ms_per_launch": ms_elem, "timed_launches": n_timed,
+                     "ms_per_launch": ms_elem, "timed_launches": n_timed, "timing": timing,
                      "bytes_per_launch": bytes_alg},
     }
-    if learned_leg is not None:
-        line["learned_pcg"] = learned_leg
+    line.update(extra)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(m, fixed, fdead, x)
     print(json.dumps(line), flush=True)

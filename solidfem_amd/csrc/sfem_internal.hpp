@@ -98,6 +98,12 @@ struct MgLevel {
     sfem::DevBuf p_ptr, p_col, p_val; // prolongation rows (nodes above) -> nodes here, FP32 weights
     sfem::DevBuf t_ptr, t_row, t_val; // its transpose (restriction), rows = nodes here
     double lmax = 0.0;                // largest eigenvalue estimate of M A on this level
+    // direct solve when this level is the coarsest and 3 nn <= MG_DENSE_MAX:
+    // element table, Dm^-1 (9 per element) and V mu, V lambda (2 per element)
+    // in element order, node -> (element, corner) incidences, element
+    // matrices, the dense operator and its inverse
+    bool dense = false;
+    sfem::DevBuf d_tets4, d_dminv, d_cvol, d_inc_ptr, d_inc, d_Ke, d_A, d_Ainv;
     ~MgLevel();
 };
 

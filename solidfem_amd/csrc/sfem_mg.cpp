@@ -117,7 +117,9 @@ void vcycle(sfem_ctx* f, int l)
     sfem_ctx* c = L.c;
     const int nlev = (int)f->mg.size();
     if (l == nlev) {
-        smooth(L, MG_COARSE_DEGREE, true, cheb_band(L.lmax, MG_COARSE_RATIO));
+        const MgLevel* m = l > 0 ? f->mg[l - 1].get() : nullptr;
+        if (m && m->dense) HIPCK(launch_mg_dense_solve(3 * (int)c->nn, m->d_Ainv.as<float>(), L.b, L.x, c->stream));
+        else smooth(L, MG_COARSE_DEGREE, true, cheb_band(L.lmax, MG_COARSE_RATIO));
         return;
     }
     const Cheb ch = cheb_band(L.lmax, MG_RATIO);
@@ -143,7 +145,8 @@ int64_t vcycle_evals(sfem_ctx* f)
     int64_t n = 0;
     for (int l = 0; l <= nlev; l++) {
         const LevelView L = level(f, l);
-        const int mv = l == nlev ? MG_COARSE_DEGREE - 1 : 2 * MG_DEGREE;
+        const bool dense = l == nlev && l > 0 && f->mg[l - 1]->dense;
+        const int mv = l == nlev ? (dense ? 0 : MG_COARSE_DEGREE - 1) : 2 * MG_DEGREE;
         n += (int64_t)mv * L.c->ne;
     }
     return n;
@@ -169,17 +172,30 @@ void mg_prepare(sfem_ctx* f, const float* minv0)
     f->mg_minv0 = minv0;
     // coarse states and block preconditioners, finest to coarsest
     const double* ua = f->d_u64.as<double>();
+    const bool dense = nlev > 0 && f->mg[nlev - 1]->dense;
     for (int l = 0; l < nlev; l++) {
-        sfem_ctx* c = f->mg[l]->c;
-        HIPCK(launch_mg_inject(c->nn, f->mg[l]->inj.as<int>(), ua, c->d_u64.as<double>(), c->d_u32.as<float>(), s));
-        build_jacobi(c);
+        MgLevel& m = *f->mg[l];
+        sfem_ctx* c = m.c;
+        HIPCK(launch_mg_inject(c->nn, m.inj.as<int>(), ua, c->d_u64.as<double>(), c->d_u32.as<float>(), s));
         ua = c->d_u64.as<double>();
+        if (l == nlev - 1 && dense) {
+            // the coarsest operator, assembled and inverted
+            const int n = 3 * (int)c->nn;
+            HIPCK(launch_mg_elem_matrix(c->material, c->ne, m.d_tets4.as<int4>(), c->d_u32.as<float>(), m.d_dminv.as<float>(),
+                                        m.d_cvol.as<float>(), m.d_Ke.as<float>(), s));
+            HIPCK(launch_mg_dense_assemble((int)c->nn, m.d_inc_ptr.as<int>(), m.d_inc.as<int2>(), m.d_tets4.as<int4>(),
+                                           m.d_Ke.as<float>(), c->d_fixed.as<uint8_t>(), m.d_A.as<float>(), s));
+            HIPCK(launch_mg_sweep_inverse(n, m.d_A.as<float>(), m.d_Ainv.as<float>(), s));
+        } else {
+            build_jacobi(c);
+        }
     }
     // lmax(M A) per level: K unnormalised power steps from a hashed start
     // (the growth of K steps stays far inside FP32 range for lmax of a few
     // units), lmax ~ |v_K| / |v_{K-1}|
     double* norms = f->d_mg_norm.as<double>();
-    for (int l = 0; l <= nlev; l++) {
+    const int nest = dense ? nlev : nlev + 1;   // levels that smooth
+    for (int l = 0; l < nest; l++) {
         const LevelView L = level(f, l);
         sfem_ctx* c = L.c;
         float* v = L.d;
@@ -193,10 +209,10 @@ void mg_prepare(sfem_ctx* f, const float* minv0)
         HIPCK(launch_mg_sqnorm(3 * c->nn, v, f->d_mg_part.as<double>(), norms + 2 * l, s));
         HIPCK(launch_mg_sqnorm(3 * c->nn, y, f->d_mg_part.as<double>(), norms + 2 * l + 1, s));
     }
-    std::vector<double> h(2 * (nlev + 1));
+    std::vector<double> h(2 * nest);
     HIPCK(hipMemcpyAsync(h.data(), norms, h.size() * sizeof(double), hipMemcpyDeviceToHost, s));
     HIPCK(hipStreamSynchronize(s));
-    for (int l = 0; l <= nlev; l++) {
+    for (int l = 0; l < nest; l++) {
         const double lm = h[2 * l + 1] > 0 ? std::sqrt(h[2 * l] / h[2 * l + 1]) : 0.0;
         if (!(lm > 0) || !std::isfinite(lm)) throw std::runtime_error("multigrid: no eigenvalue estimate on level " +
                                                                       std::to_string(l));
@@ -262,6 +278,65 @@ CgResult mg_pcg(sfem_ctx* c, int maxit, double tol2, int64_t* hv_evals)
 
 }  // namespace sfem
 
+namespace {
+
+// the element data of a small level for its direct solve (element order):
+// Dm^-1 and V mu, V lambda as the context computes them at set-up
+// (FemPhysicsMatrixFree.cpp:126-146, FemPhysicsBase.cpp:40-41)
+void dense_setup(MgLevel& m, const sfem_ctx* f, const sfem_mesh* mesh)
+{
+    const int64_t ne = mesh->ne, nn = mesh->nn;
+    const double E = f->young, nu = f->poisson;
+    const double lam = E * nu / (1.0 + nu) / (1.0 - 2.0 * nu), mu = 0.5 * E / (1.0 + nu);
+    std::vector<int> tets4(4 * ne);
+    std::vector<float> dminv(9 * ne), cvol(2 * ne);
+    std::vector<std::vector<std::pair<int, int>>> inc(nn);
+    for (int64_t e = 0; e < ne; e++) {
+        const int32_t* t = mesh->tets + 4 * e;
+        double D[9], v[9];
+        for (int k = 1; k < 4; k++)
+            for (int r = 0; r < 3; r++) D[3 * r + (k - 1)] = mesh->X0[3 * (int64_t)t[k] + r] - mesh->X0[3 * (int64_t)t[0] + r];
+        v[0] = D[4] * D[8] - D[5] * D[7];
+        v[3] = -D[3] * D[8] + D[6] * D[5];
+        v[6] = D[3] * D[7] - D[6] * D[4];
+        v[1] = D[2] * D[7] - D[1] * D[8];
+        v[4] = D[0] * D[8] - D[2] * D[6];
+        v[7] = D[1] * D[6] - D[0] * D[7];
+        v[2] = D[1] * D[5] - D[2] * D[4];
+        v[5] = D[2] * D[3] - D[0] * D[5];
+        v[8] = D[0] * D[4] - D[1] * D[3];
+        const double det = D[0] * v[0] + D[1] * v[3] + D[2] * v[6];
+        for (int k = 0; k < 9; k++) dminv[9 * e + k] = (float)(v[k] / det);
+        cvol[2 * e] = (float)(det / 6.0 * mu);
+        cvol[2 * e + 1] = (float)(det / 6.0 * lam);
+        for (int a = 0; a < 4; a++) {
+            tets4[4 * e + a] = t[a];
+            inc[t[a]].push_back({(int)e, a});
+        }
+    }
+    std::vector<int> inc_ptr(nn + 1, 0), incv;
+    for (int64_t i = 0; i < nn; i++) {
+        inc_ptr[i + 1] = inc_ptr[i] + (int)inc[i].size();
+        for (auto& p : inc[i]) {
+            incv.push_back(p.first);
+            incv.push_back(p.second);
+        }
+    }
+    m.d_tets4.upload(tets4);
+    m.d_dminv.upload(dminv);
+    m.d_cvol.upload(cvol);
+    m.d_inc_ptr.upload(inc_ptr);
+    m.d_inc.upload(incv);
+    m.d_Ke.alloc(144 * (size_t)ne * sizeof(float));
+    const size_t n = 3 * (size_t)nn;
+    m.d_A.alloc(n * n * sizeof(float));
+    m.d_Ainv.alloc(n * n * sizeof(float));
+    HIPCK(mg_sweep_init());
+    m.dense = true;
+}
+
+}  // namespace
+
 // ======================================================================= C-ABI
 extern "C" {
 
@@ -321,6 +396,7 @@ int sfem_mg_add_level(sfem_ctx* f, const sfem_mesh* coarse, const int32_t* injec
                 trow[slot] = (int)i;
                 tval[slot] = val[k];
             }
+        if (3 * nc <= MG_DENSE_MAX) dense_setup(*m, f, coarse);
         m->p_ptr.upload(ptr);
         m->p_col.upload(col);
         m->p_val.upload(val);

@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "sfem_device.hpp"
 #include "sfem_kernels.hpp"
 #include "sfem_mg.hpp"
 #include "sfem_reduce.hpp"
@@ -95,6 +96,31 @@ __global__ __launch_bounds__(MG_BLOCK) void k_mg_cheb(int64_t nn, const float* _
     }
 }
 
+// acc += sum over entries j0 .. j1-1 (in order) of val[j] * v[idx[j]] (3-vectors);
+// eight entries' index loads, then their row loads, are in flight at once
+__device__ __forceinline__ void gather_rows(int j0, int j1, const int* __restrict__ idx, const float* __restrict__ val,
+                                            const float* __restrict__ v, float (&acc)[3])
+{
+    for (int j = j0; j < j1; j += 8) {
+        int id[8];
+        float w[8], t[8][3];
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            const bool ok = j + q < j1;
+            id[q] = ok ? idx[j + q] : -1;
+            w[q] = ok ? val[j + q] : 0.f;
+        }
+#pragma unroll
+        for (int q = 0; q < 8; q++)
+#pragma unroll
+            for (int c = 0; c < 3; c++) t[q][c] = id[q] >= 0 ? v[3 * (size_t)id[q] + c] : 0.f;
+#pragma unroll
+        for (int q = 0; q < 8; q++)
+#pragma unroll
+            for (int c = 0; c < 3; c++) acc[c] += w[q] * t[q][c];
+    }
+}
+
 // e = P xc on the level above (0 at its fixed nodes); x += e
 __global__ __launch_bounds__(MG_BLOCK) void k_mg_prolong(int64_t nn, const int* __restrict__ ptr, const int* __restrict__ col,
                                                          const float* __restrict__ val, const float* __restrict__ xc,
@@ -104,13 +130,7 @@ __global__ __launch_bounds__(MG_BLOCK) void k_mg_prolong(int64_t nn, const int* 
     const int64_t i = blockIdx.x * (int64_t)MG_BLOCK + threadIdx.x;
     if (i >= nn) return;
     float ev[3] = {0.f, 0.f, 0.f};
-    const int k1 = ptr[i + 1];
-    for (int k = ptr[i]; k < k1; k++) {
-        const float w = val[k];
-        const int j = col[k];
-#pragma unroll
-        for (int c = 0; c < 3; c++) ev[c] += w * xc[3 * (size_t)j + c];
-    }
+    gather_rows(ptr[i], ptr[i + 1], col, val, xc, ev);
     const bool f = fixed[i] != 0;
 #pragma unroll
     for (int c = 0; c < 3; c++) {
@@ -129,13 +149,7 @@ __global__ __launch_bounds__(MG_BLOCK) void k_mg_restrict(int64_t nnc, const int
     const int64_t k = blockIdx.x * (int64_t)MG_BLOCK + threadIdx.x;
     if (k >= nnc) return;
     float s[3] = {0.f, 0.f, 0.f};
-    const int j1 = ptr[k + 1];
-    for (int j = ptr[k]; j < j1; j++) {
-        const float w = val[j];
-        const int i = row[j];
-#pragma unroll
-        for (int c = 0; c < 3; c++) s[c] += w * r[3 * (size_t)i + c];
-    }
+    gather_rows(ptr[k], ptr[k + 1], row, val, r, s);
     const bool f = fixed[k] != 0;
 #pragma unroll
     for (int c = 0; c < 3; c++) bc[3 * k + c] = f ? 0.f : s[c];
@@ -226,6 +240,152 @@ __global__ __launch_bounds__(MG_BLOCK) void k_mg_copy(int64_t n, const float* __
     if (i < n) b[i] = a[i];
 }
 
+// ------------------------------------------- direct solve on the coarsest level
+// Its dense operator (n = 3 nn dofs, n <= MG_DENSE_MAX) at the current state:
+// element matrices, a fixed-order gather into the dense matrix, and an
+// in-LDS inverse; each V-cycle then applies the inverse (one dense product).
+
+// Ke[e] (12 x 12, row (3a + c), column (3b + k)) = V dP(e_k (x) g_b) g_a, with
+// dP the material tangent at F = I + grad u (the Hessian-vector kernel's
+// element operator applied to unit corner displacements)
+template <int MAT>
+__global__ __launch_bounds__(MG_BLOCK) void k_mg_elem_matrix(int64_t ne, const int4* __restrict__ tets,
+                                                             const float* __restrict__ u, const float* __restrict__ dminv,
+                                                             const float* __restrict__ cvol, float* __restrict__ Ke)
+{
+    const int64_t e = blockIdx.x * (int64_t)MG_BLOCK + threadIdx.x;
+    if (e >= ne) return;
+    const int4 t = tets[e];
+    const int id[4] = {t.x, t.y, t.z, t.w};
+    float g[9], uu[4][3], F[9];
+#pragma unroll
+    for (int k = 0; k < 9; k++) g[k] = dminv[9 * e + k];
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) uu[a][c] = u[3 * (int64_t)id[a] + c];
+    defgrad(g, uu, F);
+    Tangent<float, MAT> tg;
+    tg.init(F, cvol[2 * e], cvol[2 * e + 1]);
+    float ga[4][3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        ga[1][c] = g[c]; ga[2][c] = g[3 + c]; ga[3][c] = g[6 + c];
+        ga[0][c] = -(g[c] + g[3 + c] + g[6 + c]);
+    }
+    float* out = Ke + 144 * e;
+    for (int b = 0; b < 4; b++)
+        for (int k = 0; k < 3; k++) {
+            float dF[9], dP[9];
+#pragma unroll
+            for (int q = 0; q < 9; q++) dF[q] = 0.f;
+            dF[3 * k] = ga[b][0]; dF[3 * k + 1] = ga[b][1]; dF[3 * k + 2] = ga[b][2];
+            tg.apply(dF, dP);
+#pragma unroll
+            for (int a = 0; a < 4; a++)
+#pragma unroll
+                for (int c = 0; c < 3; c++)
+                    out[(3 * a + c) * 12 + 3 * b + k] =
+                        dP[3 * c] * ga[a][0] + dP[3 * c + 1] * ga[a][1] + dP[3 * c + 2] * ga[a][2];
+        }
+}
+
+// A[i][j] (n x n): the sum over the elements of node I = i / 3 (in element
+// order) that also hold node J = j / 3; identity rows and columns at fixed nodes
+__global__ __launch_bounds__(MG_BLOCK) void k_mg_dense_assemble(int nn, const int* __restrict__ inc_ptr,
+                                                                const int2* __restrict__ inc, const int4* __restrict__ tets,
+                                                                const float* __restrict__ Ke,
+                                                                const uint8_t* __restrict__ fixed, float* __restrict__ A)
+{
+    const int n = 3 * nn;
+    const int64_t t = blockIdx.x * (int64_t)MG_BLOCK + threadIdx.x;
+    if (t >= (int64_t)n * n) return;
+    const int i = (int)(t / n), j = (int)(t % n);
+    const int I = i / 3, c = i % 3, J = j / 3, k = j % 3;
+    float s = 0.f;
+    if (fixed[I] || fixed[J]) {
+        s = i == j ? 1.f : 0.f;
+    } else {
+        for (int q = inc_ptr[I]; q < inc_ptr[I + 1]; q++) {
+            const int2 ea = inc[q];   // (element, corner of I)
+            const int4 tt = tets[ea.x];
+            const int id[4] = {tt.x, tt.y, tt.z, tt.w};
+#pragma unroll
+            for (int b = 0; b < 4; b++)
+                if (id[b] == J) s += Ke[144 * (int64_t)ea.x + (3 * ea.y + c) * 12 + 3 * b + k];
+        }
+    }
+    A[t] = s;
+}
+
+// Ainv = A^-1 for a symmetric positive definite A by the sweep operator
+// (Goodnight, The American Statistician 33 (1979) 149-158): sweeping every
+// pivot k in turn (a_kk <- -1/d, a_ik <- a_ik/d, a_ij <- a_ij - a_ik a_kj/d,
+// d = a_kk) turns A into -A^-1.  The upper triangle (packed, symmetrised from
+// A) stays in LDS; one workgroup; every thread owns up to MG_SWEEP_PER entries.
+constexpr int MG_SWEEP_BLOCK = 1024;
+constexpr int MG_SWEEP_PER = 32;   // ceil(n (n + 1) / 2 / 1024) for n <= MG_DENSE_MAX
+
+__global__ __launch_bounds__(MG_SWEEP_BLOCK) void k_mg_sweep_inverse(int n, const float* __restrict__ A,
+                                                                     float* __restrict__ Ainv)
+{
+    extern __shared__ float sm[];
+    const int np = n * (n + 1) / 2;
+    float* P = sm;          // packed upper triangle: (i, j), i <= j, at j (j + 1) / 2 + i
+    float* col = sm + np;   // column k before the sweep of k
+    const int tid = threadIdx.x;
+    // this thread's entries, decoded once
+    uint32_t ij[MG_SWEEP_PER];
+    int cnt = 0;
+    for (int e = tid; e < np; e += MG_SWEEP_BLOCK) {
+        int j = (int)((sqrtf(8.f * (float)e + 1.f) - 1.f) * 0.5f);
+        while (j * (j + 1) / 2 > e) j--;
+        while ((j + 1) * (j + 2) / 2 <= e) j++;
+        const int i = e - j * (j + 1) / 2;
+        ij[cnt++] = (uint32_t)i | ((uint32_t)j << 16);
+        P[e] = 0.5f * (A[(int64_t)i * n + j] + A[(int64_t)j * n + i]);
+    }
+    __syncthreads();
+    for (int k = 0; k < n; k++) {
+        for (int i = tid; i < n; i += MG_SWEEP_BLOCK) col[i] = i <= k ? P[k * (k + 1) / 2 + i] : P[i * (i + 1) / 2 + k];
+        __syncthreads();
+        const float d = col[k];
+        const float rd = 1.f / d;
+#pragma unroll 4
+        for (int q = 0; q < cnt; q++) {
+            const int i = (int)(ij[q] & 0xffffu), j = (int)(ij[q] >> 16);
+            const int e = j * (j + 1) / 2 + i;
+            float v;
+            if (i == k && j == k) v = -rd;
+            else if (i == k) v = col[j] * rd;
+            else if (j == k) v = col[i] * rd;
+            else v = P[e] - col[i] * col[j] * rd;
+            P[e] = v;
+        }
+        __syncthreads();
+    }
+    for (int q = 0; q < cnt; q++) {
+        const int i = (int)(ij[q] & 0xffffu), j = (int)(ij[q] >> 16);
+        const float v = -P[j * (j + 1) / 2 + i];
+        Ainv[(int64_t)i * n + j] = v;
+        Ainv[(int64_t)j * n + i] = v;
+    }
+}
+
+// x = Ainv b (Ainv symmetric: thread i reads column i, coalesced across threads)
+__global__ __launch_bounds__(MG_BLOCK) void k_mg_dense_solve(int n, const float* __restrict__ Ainv,
+                                                             const float* __restrict__ b, float* __restrict__ x)
+{
+    __shared__ float sb[MG_DENSE_MAX];
+    for (int j = threadIdx.x; j < n; j += MG_BLOCK) sb[j] = b[j];
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += MG_BLOCK) {
+        float s = 0.f;
+        for (int j = 0; j < n; j++) s += Ainv[(int64_t)j * n + i] * sb[j];
+        x[i] = s;
+    }
+}
+
 // ------------------------------------------------------------------ launchers
 
 hipError_t launch_mg_cheb(int mode, int64_t nn, const float* b, float* r, float* d, float* x, const float* q,
@@ -291,6 +451,49 @@ hipError_t launch_mg_sqnorm(int64_t n3, const float* a, double* part, double* ou
 hipError_t launch_mg_rz(int64_t nn, const float* r, const float* z, double* part, hipStream_t st)
 {
     hipLaunchKernelGGL(k_mg_rz, dim3(cg_grid(nn)), dim3(CG_BLOCK), 0, st, nn, r, z, part);
+    return hipGetLastError();
+}
+
+hipError_t launch_mg_elem_matrix(int mat, int64_t ne, const int4* tets, const float* u, const float* dminv,
+                                 const float* cvol, float* Ke, hipStream_t st)
+{
+    const dim3 g(mg_grid(ne)), bl(MG_BLOCK);
+    switch (mat) {
+    case LINEAR: hipLaunchKernelGGL(k_mg_elem_matrix<LINEAR>, g, bl, 0, st, ne, tets, u, dminv, cvol, Ke); break;
+    case STVK: hipLaunchKernelGGL(k_mg_elem_matrix<STVK>, g, bl, 0, st, ne, tets, u, dminv, cvol, Ke); break;
+    case COROTATIONAL: hipLaunchKernelGGL(k_mg_elem_matrix<COROTATIONAL>, g, bl, 0, st, ne, tets, u, dminv, cvol, Ke); break;
+    default: hipLaunchKernelGGL(k_mg_elem_matrix<NEO_HOOKEAN>, g, bl, 0, st, ne, tets, u, dminv, cvol, Ke); break;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_mg_dense_assemble(int nn, const int* inc_ptr, const int2* inc, const int4* tets, const float* Ke,
+                                    const uint8_t* fixed, float* A, hipStream_t st)
+{
+    const int64_t n = 3 * (int64_t)nn;
+    hipLaunchKernelGGL(k_mg_dense_assemble, dim3(mg_grid(n * n)), dim3(MG_BLOCK), 0, st, nn, inc_ptr, inc, tets, Ke, fixed, A);
+    return hipGetLastError();
+}
+
+static size_t sweep_lds(int n) { return ((size_t)n * (n + 1) / 2 + n) * sizeof(float); }
+
+hipError_t mg_sweep_init()
+{
+    return hipFuncSetAttribute(reinterpret_cast<const void*>(&k_mg_sweep_inverse),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)sweep_lds(MG_DENSE_MAX));
+}
+
+hipError_t launch_mg_sweep_inverse(int n, const float* A, float* Ainv, hipStream_t st)
+{
+    if (n > MG_DENSE_MAX) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_mg_sweep_inverse, dim3(1), dim3(MG_SWEEP_BLOCK), sweep_lds(n), st, n, A, Ainv);
+    return hipGetLastError();
+}
+
+hipError_t launch_mg_dense_solve(int n, const float* Ainv, const float* b, float* x, hipStream_t st)
+{
+    if (n > MG_DENSE_MAX) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_mg_dense_solve, dim3(1), dim3(MG_BLOCK), 0, st, n, Ainv, b, x);
     return hipGetLastError();
 }
 

@@ -5,6 +5,10 @@
 
 namespace sfem {
 
+// the coarsest level is solved directly when its dofs (3 x nodes) are at most
+// this many: its dense inverse's packed upper triangle fits one workgroup's LDS
+constexpr int MG_DENSE_MAX = 255;
+
 // Chebyshev step: mode 0 first step from x = 0, 1 next step, 2 first step from
 // x after a coarse correction, 3 residual only (see k_mg_cheb)
 hipError_t launch_mg_cheb(int mode, int64_t nn, const float* b, float* r, float* d, float* x, const float* q,
@@ -23,5 +27,14 @@ hipError_t launch_mg_sqnorm(int64_t n3, const float* a, double* part, double* ou
 // r.z, r.r partials in the CG update's layout (2 x cg_grid(nn))
 hipError_t launch_mg_rz(int64_t nn, const float* r, const float* z, double* part, hipStream_t st);
 hipError_t launch_mg_copy(int64_t n, const float* a, float* b, hipStream_t st);
+// dense direct solve of the coarsest level: element matrices (12 x 12 per
+// element), the dense operator (n = 3 nn), its inverse, x = Ainv b
+hipError_t launch_mg_elem_matrix(int mat, int64_t ne, const int4* tets, const float* u, const float* dminv,
+                                 const float* cvol, float* Ke, hipStream_t st);
+hipError_t launch_mg_dense_assemble(int nn, const int* inc_ptr, const int2* inc, const int4* tets, const float* Ke,
+                                    const uint8_t* fixed, float* A, hipStream_t st);
+hipError_t mg_sweep_init();   // opts the inverse kernel in to its LDS size (once per device)
+hipError_t launch_mg_sweep_inverse(int n, const float* A, float* Ainv, hipStream_t st);
+hipError_t launch_mg_dense_solve(int n, const float* Ainv, const float* b, float* x, hipStream_t st);
 
 }  // namespace sfem
