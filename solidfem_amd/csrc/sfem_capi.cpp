@@ -459,7 +459,7 @@ int solve_converged(sfem_ctx* c, const sfem_solve_params* p, sfem_solve_stats* s
         const float* minv = learned ? c->d_minv_learned.as<float>() : c->d_minv.as<float>();
         CgResult cg;
         if (multigrid) {
-            mg_prepare(c, minv);
+            mg_prepare(c, minv, it == 0);
             cg = mg_pcg(c, p->inner_iterations, eta * eta, &st->hessvec_evals);
         } else {
             cg = pcg(c, minv, pc_kind == SFEM_PC_NONE ? 0 : 1, p->inner_iterations, 1, eta * eta, 1, p->check_every,

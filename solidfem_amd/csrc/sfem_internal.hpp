@@ -164,6 +164,7 @@ struct sfem_ctx {
     std::vector<std::unique_ptr<MgLevel>> mg;
     const float* mg_minv0 = nullptr;
     double mg_lmax0 = 0.0;
+    bool mg_ready = false;     // eigenvalue estimates and coarsest inverse exist
     int64_t mg_evals = 0;      // element evaluations of one V-cycle
     DevBuf d_mg_r, d_mg_d, d_mg_q, d_mg_part, d_mg_norm;
 
@@ -223,8 +224,11 @@ void ensure_graph(sfem_ctx* c);
 
 // multigrid preconditioner (sfem_mg.cpp): level operators at the current
 // state with minv0 as the finest level's smoother blocks, then PCG on
-// K x = b (b in d_b32, x in d_x32) with one V-cycle per iteration
-void mg_prepare(sfem_ctx* c, const float* minv0);
+// K x = b (b in d_b32, x in d_x32) with one V-cycle per iteration.  Without
+// `full`, the Chebyshev bands and the coarsest level's inverse of the last
+// full set-up are kept (the Newton steps of one load step share them); the
+// states and block preconditioners are always the current ones.
+void mg_prepare(sfem_ctx* c, const float* minv0, bool full);
 CgResult mg_pcg(sfem_ctx* c, int maxit, double tol2, int64_t* hv_evals);
 // z = V(r) on the finest level (r in d_r32, z in d_z32), after mg_prepare
 void mg_vcycle(sfem_ctx* c);

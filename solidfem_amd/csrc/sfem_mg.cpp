@@ -156,7 +156,7 @@ int64_t vcycle_evals(sfem_ctx* f)
 
 namespace sfem {
 
-void mg_prepare(sfem_ctx* f, const float* minv0)
+void mg_prepare(sfem_ctx* f, const float* minv0, bool full)
 {
     hipStream_t s = f->stream;
     const size_t n3 = 3 * (size_t)f->nn;
@@ -170,6 +170,7 @@ void mg_prepare(sfem_ctx* f, const float* minv0)
     if (!f->d_mg_norm.p || f->d_mg_norm.n < 2 * (size_t)(nlev + 1) * sizeof(double))
         f->d_mg_norm.alloc(2 * (size_t)(nlev + 1) * sizeof(double));
     f->mg_minv0 = minv0;
+    full = full || !f->mg_ready;
     // coarse states and block preconditioners, finest to coarsest
     const double* ua = f->d_u64.as<double>();
     const bool dense = nlev > 0 && f->mg[nlev - 1]->dense;
@@ -179,6 +180,7 @@ void mg_prepare(sfem_ctx* f, const float* minv0)
         HIPCK(launch_mg_inject(c->nn, m.inj.as<int>(), ua, c->d_u64.as<double>(), c->d_u32.as<float>(), s));
         ua = c->d_u64.as<double>();
         if (l == nlev - 1 && dense) {
+            if (!full) continue;
             // the coarsest operator, assembled and inverted
             const int n = 3 * (int)c->nn;
             HIPCK(launch_mg_elem_matrix(c->material, c->ne, m.d_tets4.as<int4>(), c->d_u32.as<float>(), m.d_dminv.as<float>(),
@@ -193,6 +195,8 @@ void mg_prepare(sfem_ctx* f, const float* minv0)
     // lmax(M A) per level: K unnormalised power steps from a hashed start
     // (the growth of K steps stays far inside FP32 range for lmax of a few
     // units), lmax ~ |v_K| / |v_{K-1}|
+    f->mg_evals = vcycle_evals(f);
+    if (!full) return;
     double* norms = f->d_mg_norm.as<double>();
     const int nest = dense ? nlev : nlev + 1;   // levels that smooth
     for (int l = 0; l < nest; l++) {
@@ -219,7 +223,7 @@ void mg_prepare(sfem_ctx* f, const float* minv0)
         if (l == 0) f->mg_lmax0 = lm;
         else f->mg[l - 1]->lmax = lm;
     }
-    f->mg_evals = vcycle_evals(f);
+    f->mg_ready = true;
 }
 
 void mg_vcycle(sfem_ctx* f) { vcycle(f, 0); }
@@ -404,6 +408,7 @@ int sfem_mg_add_level(sfem_ctx* f, const sfem_mesh* coarse, const int32_t* injec
         m->t_row.upload(trow);
         m->t_val.upload(tval);
         f->mg.push_back(std::move(m));
+        f->mg_ready = false;
         return SFEM_OK;
     });
 }
@@ -414,6 +419,7 @@ int sfem_mg_clear(sfem_ctx* f)
     return guard_call("sfem_mg_clear", [&]() -> int {
         HIPCK(hipStreamSynchronize(f->stream));
         f->mg.clear();
+        f->mg_ready = false;
         return SFEM_OK;
     });
 }
@@ -438,7 +444,7 @@ int sfem_mg_apply(sfem_ctx* f, const double* r, double* z)
         std::vector<float> h(n3);
         for (int64_t i = 0; i < n3; i++) h[i] = f->fixed_h[i / 3] ? 0.f : (float)r[i];
         build_jacobi(f);
-        mg_prepare(f, f->d_minv.as<float>());
+        mg_prepare(f, f->d_minv.as<float>(), true);
         HIPCK(hipMemcpyAsync(f->d_r32.p, h.data(), n3 * sizeof(float), hipMemcpyHostToDevice, f->stream));
         mg_vcycle(f);
         HIPCK(hipMemcpyAsync(h.data(), f->d_z32.p, n3 * sizeof(float), hipMemcpyDeviceToHost, f->stream));
@@ -457,7 +463,7 @@ int sfem_mg_pcg(sfem_ctx* f, const double* b, double* x, double rtol, int32_t ma
         std::vector<float> h(n3);
         for (int64_t i = 0; i < n3; i++) h[i] = f->fixed_h[i / 3] ? 0.f : (float)b[i];
         build_jacobi(f);
-        mg_prepare(f, f->d_minv.as<float>());
+        mg_prepare(f, f->d_minv.as<float>(), true);
         HIPCK(hipMemcpyAsync(f->d_b32.p, h.data(), n3 * sizeof(float), hipMemcpyHostToDevice, f->stream));
         CgResult cg = mg_pcg(f, maxit, rtol * rtol, nullptr);
         HIPCK(hipMemcpyAsync(h.data(), f->d_x32.p, n3 * sizeof(float), hipMemcpyDeviceToHost, f->stream));

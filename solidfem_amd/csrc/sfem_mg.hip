@@ -141,18 +141,32 @@ __global__ __launch_bounds__(MG_BLOCK) void k_mg_prolong(int64_t nn, const int* 
 }
 
 // bc = P^T r: rows of the transpose (coarse nodes) in increasing fine-node
-// order; 0 at fixed coarse nodes
+// order; 0 at fixed coarse nodes.  Four lanes per coarse node: each sums a
+// contiguous quarter of the row, then (s0 + s1) + (s2 + s3).
 __global__ __launch_bounds__(MG_BLOCK) void k_mg_restrict(int64_t nnc, const int* __restrict__ ptr, const int* __restrict__ row,
                                                           const float* __restrict__ val, const float* __restrict__ r,
                                                           const uint8_t* __restrict__ fixed, float* __restrict__ bc)
 {
-    const int64_t k = blockIdx.x * (int64_t)MG_BLOCK + threadIdx.x;
-    if (k >= nnc) return;
+    const int64_t t = blockIdx.x * (int64_t)MG_BLOCK + threadIdx.x;
+    const int64_t k = t >> 2;
+    const int lane = (int)(t & 3);
     float s[3] = {0.f, 0.f, 0.f};
-    gather_rows(ptr[k], ptr[k + 1], row, val, r, s);
-    const bool f = fixed[k] != 0;
+    if (k < nnc) {
+        const int p0 = ptr[k], p1 = ptr[k + 1];
+        const int cs = (p1 - p0 + 3) >> 2;
+        const int a = p0 + lane * cs, e = a + cs < p1 ? a + cs : p1;
+        gather_rows(a, e, row, val, r, s);
+    }
 #pragma unroll
-    for (int c = 0; c < 3; c++) bc[3 * k + c] = f ? 0.f : s[c];
+    for (int c = 0; c < 3; c++) {
+        s[c] += __shfl_xor(s[c], 1, 64);
+        s[c] += __shfl_xor(s[c], 2, 64);
+    }
+    if (k < nnc && lane == 0) {
+        const bool f = fixed[k] != 0;
+#pragma unroll
+        for (int c = 0; c < 3; c++) bc[3 * k + c] = f ? 0.f : s[c];
+    }
 }
 
 // the coarse level's state: the displacement of the level above at the node
@@ -321,68 +335,84 @@ __global__ __launch_bounds__(MG_BLOCK) void k_mg_dense_assemble(int nn, const in
 // Ainv = A^-1 for a symmetric positive definite A by the sweep operator
 // (Goodnight, The American Statistician 33 (1979) 149-158): sweeping every
 // pivot k in turn (a_kk <- -1/d, a_ik <- a_ik/d, a_ij <- a_ij - a_ik a_kj/d,
-// d = a_kk) turns A into -A^-1.  The upper triangle (packed, symmetrised from
-// A) stays in LDS; one workgroup; every thread owns up to MG_SWEEP_PER entries.
+// d = a_kk) turns A into -A^-1.  One workgroup; the packed upper triangle
+// (symmetrised from A) lives in LDS, every thread owning up to MG_SWEEP_PER
+// entries (decoded once); column k is double-buffered in LDS: the owners of
+// column k+1's entries publish them while sweeping k, so each pivot costs one
+// barrier.
 constexpr int MG_SWEEP_BLOCK = 1024;
 constexpr int MG_SWEEP_PER = 32;   // ceil(n (n + 1) / 2 / 1024) for n <= MG_DENSE_MAX
 
 __global__ __launch_bounds__(MG_SWEEP_BLOCK) void k_mg_sweep_inverse(int n, const float* __restrict__ A,
                                                                      float* __restrict__ Ainv)
 {
-    extern __shared__ float sm[];
+    extern __shared__ float P[];   // (i, j), i <= j, at j (j + 1) / 2 + i
+    __shared__ float col[2][MG_DENSE_MAX];
     const int np = n * (n + 1) / 2;
-    float* P = sm;          // packed upper triangle: (i, j), i <= j, at j (j + 1) / 2 + i
-    float* col = sm + np;   // column k before the sweep of k
     const int tid = threadIdx.x;
-    // this thread's entries, decoded once
-    uint32_t ij[MG_SWEEP_PER];
-    int cnt = 0;
-    for (int e = tid; e < np; e += MG_SWEEP_BLOCK) {
-        int j = (int)((sqrtf(8.f * (float)e + 1.f) - 1.f) * 0.5f);
-        while (j * (j + 1) / 2 > e) j--;
-        while ((j + 1) * (j + 2) / 2 <= e) j++;
-        const int i = e - j * (j + 1) / 2;
-        ij[cnt++] = (uint32_t)i | ((uint32_t)j << 16);
-        P[e] = 0.5f * (A[(int64_t)i * n + j] + A[(int64_t)j * n + i]);
+    uint32_t ij[MG_SWEEP_PER];     // i | j << 16; j = 0xffff: no entry
+#pragma unroll
+    for (int q = 0; q < MG_SWEEP_PER; q++) {
+        const int e = tid + q * MG_SWEEP_BLOCK;
+        int i = 0, j = 0xffff;
+        if (e < np) {
+            j = (int)((sqrtf(8.f * (float)e + 1.f) - 1.f) * 0.5f);
+            while (j * (j + 1) / 2 > e) j--;
+            while ((j + 1) * (j + 2) / 2 <= e) j++;
+            i = e - j * (j + 1) / 2;
+            const float a = 0.5f * (A[(int64_t)i * n + j] + A[(int64_t)j * n + i]);
+            P[e] = a;
+            if (i == 0) col[0][j] = a;   // column 0 = row 0 of the upper triangle
+        }
+        ij[q] = (uint32_t)i | ((uint32_t)j << 16);
     }
     __syncthreads();
     for (int k = 0; k < n; k++) {
-        for (int i = tid; i < n; i += MG_SWEEP_BLOCK) col[i] = i <= k ? P[k * (k + 1) / 2 + i] : P[i * (i + 1) / 2 + k];
-        __syncthreads();
-        const float d = col[k];
-        const float rd = 1.f / d;
-#pragma unroll 4
-        for (int q = 0; q < cnt; q++) {
+        const float* c = col[k & 1];
+        float* cn = col[(k + 1) & 1];
+        const float rd = 1.f / c[k];
+#pragma unroll 8
+        for (int q = 0; q < MG_SWEEP_PER; q++) {
             const int i = (int)(ij[q] & 0xffffu), j = (int)(ij[q] >> 16);
-            const int e = j * (j + 1) / 2 + i;
-            float v;
-            if (i == k && j == k) v = -rd;
-            else if (i == k) v = col[j] * rd;
-            else if (j == k) v = col[i] * rd;
-            else v = P[e] - col[i] * col[j] * rd;
-            P[e] = v;
+            if (j == 0xffff) break;
+            const int e = tid + q * MG_SWEEP_BLOCK;
+            const float ci = c[i], cj = c[j];
+            const float t = ci * rd;
+            float nv = fmaf(-t, cj, P[e]);
+            const bool ik = i == k, jk = j == k;
+            if (ik || jk) nv = (ik && jk) ? -rd : (ik ? cj * rd : t);
+            P[e] = nv;
+            // publish column k + 1: entries (i, k+1) and (k+1, j)
+            if (j == k + 1) cn[i] = nv;
+            else if (i == k + 1) cn[j] = nv;
         }
         __syncthreads();
     }
-    for (int q = 0; q < cnt; q++) {
+#pragma unroll
+    for (int q = 0; q < MG_SWEEP_PER; q++) {
         const int i = (int)(ij[q] & 0xffffu), j = (int)(ij[q] >> 16);
-        const float v = -P[j * (j + 1) / 2 + i];
+        if (j == 0xffff) continue;
+        const float v = -P[tid + q * MG_SWEEP_BLOCK];
         Ainv[(int64_t)i * n + j] = v;
         Ainv[(int64_t)j * n + i] = v;
     }
 }
 
-// x = Ainv b (Ainv symmetric: thread i reads column i, coalesced across threads)
+// x = Ainv b: one workgroup per row, a fixed-order tree over its n products
 __global__ __launch_bounds__(MG_BLOCK) void k_mg_dense_solve(int n, const float* __restrict__ Ainv,
                                                              const float* __restrict__ b, float* __restrict__ x)
 {
-    __shared__ float sb[MG_DENSE_MAX];
-    for (int j = threadIdx.x; j < n; j += MG_BLOCK) sb[j] = b[j];
+    __shared__ float red[MG_BLOCK / 64];
+    const int i = blockIdx.x, j = threadIdx.x;
+    float s = j < n ? Ainv[(int64_t)i * n + j] * b[j] : 0.f;
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+    if ((j & 63) == 0) red[j >> 6] = s;
     __syncthreads();
-    for (int i = threadIdx.x; i < n; i += MG_BLOCK) {
-        float s = 0.f;
-        for (int j = 0; j < n; j++) s += Ainv[(int64_t)j * n + i] * sb[j];
-        x[i] = s;
+    if (j == 0) {
+        float t = 0.f;
+#pragma unroll
+        for (int w = 0; w < MG_BLOCK / 64; w++) t += red[w];
+        x[i] = t;
     }
 }
 
@@ -411,7 +441,7 @@ hipError_t launch_mg_prolong(int64_t nn, const int* ptr, const int* col, const f
 hipError_t launch_mg_restrict(int64_t nnc, const int* ptr, const int* row, const float* val, const float* r,
                               const uint8_t* fixed, float* bc, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_mg_restrict, dim3(mg_grid(nnc)), dim3(MG_BLOCK), 0, st, nnc, ptr, row, val, r, fixed, bc);
+    hipLaunchKernelGGL(k_mg_restrict, dim3(mg_grid(4 * nnc)), dim3(MG_BLOCK), 0, st, nnc, ptr, row, val, r, fixed, bc);
     return hipGetLastError();
 }
 
@@ -475,7 +505,7 @@ hipError_t launch_mg_dense_assemble(int nn, const int* inc_ptr, const int2* inc,
     return hipGetLastError();
 }
 
-static size_t sweep_lds(int n) { return ((size_t)n * (n + 1) / 2 + n) * sizeof(float); }
+static size_t sweep_lds(int n) { return (size_t)n * (n + 1) / 2 * sizeof(float); }
 
 hipError_t mg_sweep_init()
 {
@@ -492,8 +522,8 @@ hipError_t launch_mg_sweep_inverse(int n, const float* A, float* Ainv, hipStream
 
 hipError_t launch_mg_dense_solve(int n, const float* Ainv, const float* b, float* x, hipStream_t st)
 {
-    if (n > MG_DENSE_MAX) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_mg_dense_solve, dim3(1), dim3(MG_BLOCK), 0, st, n, Ainv, b, x);
+    if (n > MG_DENSE_MAX || n > MG_BLOCK) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_mg_dense_solve, dim3(n), dim3(MG_BLOCK), 0, st, n, Ainv, b, x);
     return hipGetLastError();
 }
 
