@@ -457,10 +457,9 @@ int solve_converged(sfem_ctx* c, const sfem_solve_params* p, sfem_solve_stats* s
         }
         HIPCK(launch_to_f32(n3, c->d_r64.as<double>(), c->d_b32.as<float>(), c->stream));
         const float* minv = learned ? c->d_minv_learned.as<float>() : c->d_minv.as<float>();
-        CgResult cg;
+        CgResult cg{0, 0, 0.0, 0.0};
         if (multigrid) {
-            mg_prepare(c, minv, it == 0);
-            cg = mg_pcg(c, p->inner_iterations, eta * eta, &st->hessvec_evals);
+            if (mg_prepare(c, minv, it == 0)) cg = mg_pcg(c, p->inner_iterations, eta * eta, &st->hessvec_evals);
         } else {
             cg = pcg(c, minv, pc_kind == SFEM_PC_NONE ? 0 : 1, p->inner_iterations, 1, eta * eta, 1, p->check_every,
                      &st->hessvec_evals);

@@ -227,8 +227,9 @@ void ensure_graph(sfem_ctx* c);
 // K x = b (b in d_b32, x in d_x32) with one V-cycle per iteration.  Without
 // `full`, the Chebyshev bands and the coarsest level's inverse of the last
 // full set-up are kept (the Newton steps of one load step share them); the
-// states and block preconditioners are always the current ones.
-void mg_prepare(sfem_ctx* c, const float* minv0, bool full);
+// states and block preconditioners are always the current ones.  False when
+// a level's operator is not finite at this state (no V-cycle possible).
+bool mg_prepare(sfem_ctx* c, const float* minv0, bool full);
 CgResult mg_pcg(sfem_ctx* c, int maxit, double tol2, int64_t* hv_evals);
 // z = V(r) on the finest level (r in d_r32, z in d_z32), after mg_prepare
 void mg_vcycle(sfem_ctx* c);

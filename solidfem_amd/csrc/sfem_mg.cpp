@@ -36,6 +36,11 @@ constexpr double MG_RATIO = 10.0;       // smoothed band [lmax / ratio, lmax]
 constexpr double MG_COARSE_RATIO = 100.0;
 constexpr double MG_SAFETY = 1.15;      // the power iteration estimates lmax from below
 constexpr int MG_POWER_ITERS = 12;
+// a multigrid-PCG solve that has not met its forcing after this many
+// iterations stops (and the Newton driver redoes it with block Jacobi): one
+// iteration costs about six Jacobi-PCG iterations, and where the V-cycle
+// works it needs tens to a few hundred
+constexpr int MG_MAX_ITERS = 2500;
 
 struct LevelView {
     sfem_ctx* c;           // mesh, partition, state of the level
@@ -156,7 +161,7 @@ int64_t vcycle_evals(sfem_ctx* f)
 
 namespace sfem {
 
-void mg_prepare(sfem_ctx* f, const float* minv0, bool full)
+bool mg_prepare(sfem_ctx* f, const float* minv0, bool full)
 {
     hipStream_t s = f->stream;
     const size_t n3 = 3 * (size_t)f->nn;
@@ -196,7 +201,7 @@ void mg_prepare(sfem_ctx* f, const float* minv0, bool full)
     // (the growth of K steps stays far inside FP32 range for lmax of a few
     // units), lmax ~ |v_K| / |v_{K-1}|
     f->mg_evals = vcycle_evals(f);
-    if (!full) return;
+    if (!full) return true;
     double* norms = f->d_mg_norm.as<double>();
     const int nest = dense ? nlev : nlev + 1;   // levels that smooth
     for (int l = 0; l < nest; l++) {
@@ -216,14 +221,22 @@ void mg_prepare(sfem_ctx* f, const float* minv0, bool full)
     std::vector<double> h(2 * nest);
     HIPCK(hipMemcpyAsync(h.data(), norms, h.size() * sizeof(double), hipMemcpyDeviceToHost, s));
     HIPCK(hipStreamSynchronize(s));
+    // a level whose operator is not finite at this state (an element inverted
+    // by the injected coarse displacement) gives no estimate: no V-cycle here
+    std::vector<double> lm(nest);
     for (int l = 0; l < nest; l++) {
-        const double lm = h[2 * l + 1] > 0 ? std::sqrt(h[2 * l] / h[2 * l + 1]) : 0.0;
-        if (!(lm > 0) || !std::isfinite(lm)) throw std::runtime_error("multigrid: no eigenvalue estimate on level " +
-                                                                      std::to_string(l));
-        if (l == 0) f->mg_lmax0 = lm;
-        else f->mg[l - 1]->lmax = lm;
+        lm[l] = h[2 * l + 1] > 0 ? std::sqrt(h[2 * l] / h[2 * l + 1]) : 0.0;
+        if (!(lm[l] > 0) || !std::isfinite(lm[l])) {
+            f->mg_ready = false;
+            return false;
+        }
+    }
+    for (int l = 0; l < nest; l++) {
+        if (l == 0) f->mg_lmax0 = lm[l];
+        else f->mg[l - 1]->lmax = lm[l];
     }
     f->mg_ready = true;
+    return true;
 }
 
 void mg_vcycle(sfem_ctx* f) { vcycle(f, 0); }
@@ -234,6 +247,7 @@ void mg_vcycle(sfem_ctx* f) { vcycle(f, 0); }
 // an iteration past convergence would cost a whole V-cycle.
 CgResult mg_pcg(sfem_ctx* c, int maxit, double tol2, int64_t* hv_evals)
 {
+    maxit = std::min(maxit, MG_MAX_ITERS);
     const int64_t n3 = 3 * c->nn;
     hipStream_t s = c->stream;
     double* scal = c->d_scal.as<double>();
@@ -262,6 +276,11 @@ CgResult mg_pcg(sfem_ctx* c, int maxit, double tol2, int64_t* hv_evals)
                                     c->nblk, parts, done, 1, s));
         mg_vcycle(c);
         HIPCK(launch_mg_rz(c->nn, r, z, parts, s));
+        // the test is on the true residual norm (r.r), which stays a valid stop
+        // even where the V-cycle is not positive definite (tangents of
+        // near-incompressible material at strongly deformed Newton iterates);
+        // a V-cycle that stalls reaches MG_MAX_ITERS and the Newton driver
+        // falls back to block Jacobi
         HIPCK(launch_cg_p_only(n3, z, r, p, scal, parts, done, 1, tol2, s));
         launched++;
     }
@@ -444,7 +463,8 @@ int sfem_mg_apply(sfem_ctx* f, const double* r, double* z)
         std::vector<float> h(n3);
         for (int64_t i = 0; i < n3; i++) h[i] = f->fixed_h[i / 3] ? 0.f : (float)r[i];
         build_jacobi(f);
-        mg_prepare(f, f->d_minv.as<float>(), true);
+        if (!mg_prepare(f, f->d_minv.as<float>(), true))
+            return fail(SFEM_ESTATE, "multigrid: a level's operator is not finite at this state");
         HIPCK(hipMemcpyAsync(f->d_r32.p, h.data(), n3 * sizeof(float), hipMemcpyHostToDevice, f->stream));
         mg_vcycle(f);
         HIPCK(hipMemcpyAsync(h.data(), f->d_z32.p, n3 * sizeof(float), hipMemcpyDeviceToHost, f->stream));
@@ -463,7 +483,8 @@ int sfem_mg_pcg(sfem_ctx* f, const double* b, double* x, double rtol, int32_t ma
         std::vector<float> h(n3);
         for (int64_t i = 0; i < n3; i++) h[i] = f->fixed_h[i / 3] ? 0.f : (float)b[i];
         build_jacobi(f);
-        mg_prepare(f, f->d_minv.as<float>(), true);
+        if (!mg_prepare(f, f->d_minv.as<float>(), true))
+            return fail(SFEM_ESTATE, "multigrid: a level's operator is not finite at this state");
         HIPCK(hipMemcpyAsync(f->d_b32.p, h.data(), n3 * sizeof(float), hipMemcpyHostToDevice, f->stream));
         CgResult cg = mg_pcg(f, maxit, rtol * rtol, nullptr);
         HIPCK(hipMemcpyAsync(h.data(), f->d_x32.p, n3 * sizeof(float), hipMemcpyDeviceToHost, f->stream));
