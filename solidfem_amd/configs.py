@@ -2,7 +2,7 @@
 generator, and a runner that solves one on the device and reports iterations,
 timings and FP64 residuals.
 
-    python -m solidfem_amd.configs 1 [jacobi|multigrid|multigrid_learned|learned|none]
+    python -m solidfem_amd.configs 1 [jacobi|multigrid|multigrid_learned|learned|none] [nopredictor]
 
 All beams span [0, 1] x [0, 0.25] x [0, 0.25] m with the x = 0 face clamped
 and 6 tets per hex (the Models/box2.tet pattern, see mesh.py).
@@ -83,7 +83,7 @@ PRECONDS = {"none": capi.PC_NONE, "jacobi": capi.PC_JACOBI, "learned": capi.PC_L
             "multigrid": capi.PC_MULTIGRID, "multigrid_learned": capi.PC_MULTIGRID_LEARNED}
 
 
-def run(cfg, precond=capi.PC_JACOBI, tol=1e-8, grid=None, max_newton=50, log=None):
+def run(cfg, precond=capi.PC_JACOBI, tol=1e-8, grid=None, max_newton=50, log=None, predictor=1):
     """Solves workload `cfg`; `log(str)` (optional) gets one line per load step.
     The multigrid preconditioners get the beam's nested hierarchy, the learned
     ones the shipped weights."""
@@ -98,7 +98,8 @@ def run(cfg, precond=capi.PC_JACOBI, tol=1e-8, grid=None, max_newton=50, log=Non
         s._net = net   # keep the network alive with the solver
     steps = d["load_steps"]
     p = capi.solve_params(tol=tol, precond=precond, cg_rtol=0.1, outer=max_newton, inner=200000, check_every=32,
-                          sim_type=capi.ST_QUASI_STATIC if steps > 1 else capi.ST_STATIC, force_step=1.0 / steps)
+                          sim_type=capi.ST_QUASI_STATIC if steps > 1 else capi.ST_STATIC, force_step=1.0 / steps,
+                          predictor=predictor)
     out = {"config": cfg, "description": DESCRIPTIONS[cfg], "tets": d["mesh"].ne, "nodes": d["mesh"].nn,
            "steps": []}
     t0 = time.perf_counter()
@@ -121,6 +122,8 @@ def run(cfg, precond=capi.PC_JACOBI, tol=1e-8, grid=None, max_newton=50, log=Non
 if __name__ == "__main__":
     cfg = int(sys.argv[1])
     pc = sys.argv[2] if len(sys.argv) > 2 else "jacobi"
-    out, d, s = run(cfg, PRECONDS[pc], log=lambda line: print(line, file=sys.stderr, flush=True))
+    pred = 0 if "nopredictor" in sys.argv[3:] else 1
+    out, d, s = run(cfg, PRECONDS[pc], log=lambda line: print(line, file=sys.stderr, flush=True), predictor=pred)
     out["precond"] = pc
+    out["predictor"] = pred
     print(json.dumps(out), flush=True)

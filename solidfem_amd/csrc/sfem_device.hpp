@@ -16,7 +16,8 @@
 
 namespace sfem {
 
-enum { LINEAR = 0, STVK = 1, COROTATIONAL = 2, NEO_HOOKEAN = 3 };
+enum { LINEAR = 0, STVK = 1, COROTATIONAL = 2, NEO_HOOKEAN = 3,
+       NEO_HOOKEAN_COARSE = 4 };   // internal: coarse multigrid levels (MAT_NEO_HOOKEAN_COARSE)
 
 template <typename T> __device__ __forceinline__ T dsqrt(T x) { return sqrt(x); }
 template <typename T> __device__ __forceinline__ T dlog(T x) { return log(x); }
@@ -249,11 +250,35 @@ template <typename T> struct Tangent<T, NEO_HOOKEAN> {
     }
 };
 
+// The Neo-Hookean tangent on the coarse levels of the multigrid
+// preconditioner, whose states are injected from the level above and need not
+// be valid deformations of the coarse mesh: an element with J <= 0.1 (or not
+// finite) takes the tangent at F = I, and the coefficient mu - lam ln J of the
+// F^-T dF^T F^-T term is kept >= mu / 10 (it turns negative for
+// J > exp(mu / lam), J > 1.01 at nu = 0.495, where an expanded
+// near-incompressible element's tangent is indefinite).  A preconditioner
+// operator, not the element's tangent.
+template <typename T> struct Tangent<T, NEO_HOOKEAN_COARSE> : Tangent<T, NEO_HOOKEAN> {
+    __device__ __forceinline__ void init(const T* F, T m, T l)
+    {
+        this->cmu = m;
+        this->clam = l;
+        T J = inv3(F, this->Fi);
+        if (!(J > T(0.1))) {
+#pragma unroll
+            for (int k = 0; k < 9; k++) this->Fi[k] = (k % 4 == 0) ? T(1) : T(0);
+            J = T(1);
+        }
+        const T c = m - l * dlog(J);
+        this->c = c > T(0.1) * m ? c : T(0.1) * m;
+    }
+};
+
 // First Piola-Kirchhoff stress, volume-scaled (ElasticEnergy.cpp:114-187)
 template <typename T, int MAT>
 __device__ __forceinline__ void stress(const T* F, T cmu, T clam, T* P)
 {
-    if (MAT == NEO_HOOKEAN) {
+    if (MAT == NEO_HOOKEAN || MAT == NEO_HOOKEAN_COARSE) {
         T Fi[9];
         T J = inv3(F, Fi);
         T lJ = clam * dlog(J);

@@ -868,6 +868,14 @@ static hipError_t launch_elem_mat(int mat, const BlockArgs& a, const void* u, co
     case LINEAR: SFEM_ELEM_CASE(LINEAR); break;
     case STVK: SFEM_ELEM_CASE(STVK); break;
     case COROTATIONAL: SFEM_ELEM_CASE(COROTATIONAL); break;
+    case NEO_HOOKEAN_COARSE:
+        // coarse multigrid levels: operator and diagonal blocks only
+        if constexpr (KIND != 1) {
+            SFEM_ELEM_CASE(NEO_HOOKEAN_COARSE);
+        } else {
+            SFEM_ELEM_CASE(NEO_HOOKEAN);
+        }
+        break;
     default: SFEM_ELEM_CASE(NEO_HOOKEAN); break;
     }
 #undef SFEM_ELEM_CASE
@@ -941,11 +949,17 @@ hipError_t set_elem_lds_limits(const BlockArgs& a)
     // dynamic LDS above the 64 KiB default (unstructured meshes with many nodes
     // per block): raise each kernel's limit to what it needs, within the
     // device's opt-in maximum less the kernel's static LDS
-    const void* fns[] = {
-#define SFEM_FN(K, M) (const void*)k_elem_block<K, M, BLOCK_E, false>, (const void*)k_elem_block<K, M, BLOCK_E, true>
+    struct Fn {
+        const void* f;
+        int kind;
+    };
+    const Fn fns[] = {
+#define SFEM_FN(K, M) {(const void*)k_elem_block<K, M, BLOCK_E, false>, K}, {(const void*)k_elem_block<K, M, BLOCK_E, true>, K}
         SFEM_FN(0, LINEAR), SFEM_FN(0, STVK), SFEM_FN(0, COROTATIONAL), SFEM_FN(0, NEO_HOOKEAN),
+        SFEM_FN(0, NEO_HOOKEAN_COARSE),
         SFEM_FN(1, LINEAR), SFEM_FN(1, STVK), SFEM_FN(1, COROTATIONAL), SFEM_FN(1, NEO_HOOKEAN),
         SFEM_FN(2, LINEAR), SFEM_FN(2, STVK), SFEM_FN(2, COROTATIONAL), SFEM_FN(2, NEO_HOOKEAN),
+        SFEM_FN(2, NEO_HOOKEAN_COARSE),
 #undef SFEM_FN
     };
     const size_t need[] = {lds_bytes<0>(a), lds_bytes<1>(a), lds_bytes<2>(a)};
@@ -955,14 +969,14 @@ hipError_t set_elem_lds_limits(const BlockArgs& a)
     if (e != hipSuccess) return e;
     e = hipDeviceGetAttribute(&optin, hipDeviceAttributeSharedMemPerBlockOptin, dev);
     if (e != hipSuccess) return e;
-    for (int k = 0; k < 24; k++) {
+    for (const Fn& fn : fns) {
         hipFuncAttributes at;
-        e = hipFuncGetAttributes(&at, fns[k]);
+        e = hipFuncGetAttributes(&at, fn.f);
         if (e != hipSuccess) return e;
-        const size_t n = need[k / 8];
+        const size_t n = need[fn.kind];
         if (n <= 64 * 1024) continue;
         if (n + at.sharedSizeBytes > (size_t)optin) return hipErrorInvalidValue;   // mesh too irregular for one block
-        e = hipFuncSetAttribute(fns[k], hipFuncAttributeMaxDynamicSharedMemorySize, (int)n);
+        e = hipFuncSetAttribute(fn.f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)n);
         if (e != hipSuccess) return e;
     }
     return hipSuccess;

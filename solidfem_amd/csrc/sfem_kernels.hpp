@@ -26,6 +26,10 @@ constexpr int CG_P_BLOCK = SFEM_CG_P_BLOCK;
 enum Scal { S_RZ = 0, S_RZ0, S_RR, S_RR0, S_PQ, S_ALPHA, S_BETA, S_ITER, S_NT, S_NT2, S_NT3, S_RZ_PREV, S_COUNT = 16 };
 
 // Element-block partition, all device pointers.
+// the material id of coarse multigrid levels of a Neo-Hookean context (the
+// safeguarded tangent, sfem_device.hpp); the C-ABI ids are 0 .. 3
+constexpr int MAT_NEO_HOOKEAN_COARSE = 4;
+
 struct BlockArgs {
     int64_t ne;            // elements
     int64_t ne_pad;        // SoA stride (nblk * BLOCK_E)

@@ -397,6 +397,8 @@ int sfem_mg_add_level(sfem_ctx* f, const sfem_mesh* coarse, const int32_t* injec
         HIPCK(hipStreamDestroy(lc->stream));
         lc->stream = f->stream;
         lc->owns_stream = false;
+        // coarse Neo-Hookean levels: the safeguarded tangent
+        if (lc->material == SFEM_NEO_HOOKEAN) lc->material = MAT_NEO_HOOKEAN_COARSE;
         std::vector<int> inj(inject, inject + nc);
         m->inj.upload(inj);
         std::vector<int> ptr(na + 1), col(nnz);
