@@ -189,7 +189,10 @@ def main():
     net = None
     if args.precond in ("learned", "multigrid_learned", "both", "all"):
         from solidfem_amd import learned
-        if os.path.exists(learned.WEIGHTS):
+        if args.precond == "multigrid_learned" and os.path.exists(learned.SMOOTHER_WEIGHTS):
+            net = learned.load(smoother=True)
+            net.attach(s)
+        elif os.path.exists(learned.WEIGHTS):
             net = learned.load()
             net.attach(s)
         elif args.precond in ("learned", "multigrid_learned"):
@@ -219,6 +222,10 @@ def main():
         else:
             extra["learned_pcg"] = {"block": side_leg(capi.PC_LEARNED)}
             if args.precond == "all":
+                # the network trained as the smoother (white-noise pairs), when shipped
+                if os.path.exists(learned.SMOOTHER_WEIGHTS):
+                    smoother = learned.load(smoother=True)
+                    smoother.attach(s)
                 extra["learned_pcg"]["multigrid"] = side_leg(capi.PC_MULTIGRID_LEARNED)
         if args.precond == "all":
             extra["multigrid_pcg"] = side_leg(capi.PC_MULTIGRID)

@@ -261,6 +261,10 @@ int sfem_set_learned_preconditioner(sfem_ctx* ctx, sfem_gnn* gnn);
    second identical PCG run, with e_k = e - x_k so that K e_k = r_k.  The
    pairs belong to the current state: change it only after training. */
 int sfem_collect_pairs(sfem_ctx* ctx, const double* load, int nsnap, int snap_every, double rtol, int* collected);
+/* Keeps `count` pairs (K e, e . K e) at the current state with e white noise
+   on the free dofs (seeded): the high-frequency errors a multigrid smoother
+   has to remove, for training the network as a smoother. */
+int sfem_collect_rough_pairs(sfem_ctx* ctx, int count, uint64_t seed, int* collected);
 int sfem_num_pairs(sfem_ctx* ctx, int64_t* n);
 int sfem_get_pair(sfem_ctx* ctx, int64_t k, float* r, double* er);
 int sfem_clear_pairs(sfem_ctx* ctx);

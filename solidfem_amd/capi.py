@@ -28,7 +28,7 @@ EXPORTS = [
     "sfem_reset", "sfem_profile_hessvec", "sfem_time_hessvec",
     "sfem_mg_add_level", "sfem_mg_clear", "sfem_mg_info", "sfem_mg_apply", "sfem_mg_pcg",
     "sfem_gnn_create", "sfem_gnn_destroy", "sfem_gnn_num_params", "sfem_gnn_get_params", "sfem_gnn_set_params",
-    "sfem_set_learned_preconditioner", "sfem_collect_pairs", "sfem_num_pairs", "sfem_get_pair", "sfem_clear_pairs",
+    "sfem_set_learned_preconditioner", "sfem_collect_pairs", "sfem_collect_rough_pairs", "sfem_num_pairs", "sfem_get_pair", "sfem_clear_pairs",
     "sfem_gnn_loss", "sfem_gnn_eval", "sfem_gnn_train", "sfem_gnn_inputs", "sfem_gnn_gemm_check",
 ]
 
@@ -119,6 +119,7 @@ def lib():
             "sfem_gnn_set_params": ([V, _fp], C.c_int),
             "sfem_set_learned_preconditioner": ([V, V], C.c_int),
             "sfem_collect_pairs": ([V, _dp, C.c_int, C.c_int, C.c_double, C.POINTER(C.c_int)], C.c_int),
+            "sfem_collect_rough_pairs": ([V, C.c_int, C.c_uint64, C.POINTER(C.c_int)], C.c_int),
             "sfem_num_pairs": ([V, C.POINTER(C.c_int64)], C.c_int),
             "sfem_get_pair": ([V, C.c_int64, _fp, _dp], C.c_int),
             "sfem_clear_pairs": ([V], C.c_int),
@@ -365,6 +366,13 @@ class LearnedPreconditioner:
         load = np.ascontiguousarray(load, dtype=np.float64).reshape(-1, 3)
         n = C.c_int()
         _check(lib().sfem_collect_pairs(solver.h, _p(load), int(nsnap), int(snap_every), float(rtol), C.byref(n)))
+        return n.value
+
+    @staticmethod
+    def collect_rough(solver, count, seed=0):
+        """count (K e, e.Ke) pairs with white-noise e at the solver's state"""
+        n = C.c_int()
+        _check(lib().sfem_collect_rough_pairs(solver.h, int(count), int(seed), C.byref(n)))
         return n.value
 
     @staticmethod

@@ -25,6 +25,7 @@ and 6 tets per hex (the Models/box2.tet pattern, see mesh.py).
 from __future__ import annotations
 
 import json
+import os
 import sys
 import time
 
@@ -93,7 +94,8 @@ def run(cfg, precond=capi.PC_JACOBI, tol=1e-8, grid=None, max_newton=50, log=Non
         s.set_multigrid(beam_hierarchy(d["mesh"]))
     if precond in (capi.PC_LEARNED, capi.PC_MULTIGRID_LEARNED):
         from . import learned
-        net = learned.load()
+        smoother = precond == capi.PC_MULTIGRID_LEARNED and os.path.exists(learned.SMOOTHER_WEIGHTS)
+        net = learned.load(smoother=smoother)
         net.attach(s)
         s._net = net   # keep the network alive with the solver
     steps = d["load_steps"]

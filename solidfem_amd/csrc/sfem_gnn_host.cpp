@@ -442,6 +442,31 @@ int sfem_collect_pairs(sfem_ctx* c, const double* load, int nsnap, int snap_ever
     });
 }
 
+int sfem_collect_rough_pairs(sfem_ctx* c, int count, uint64_t seed, int* collected)
+{
+    if (!c || count <= 0) return fail(SFEM_EINVAL, "sfem_collect_rough_pairs: bad argument");
+    return guard_call("sfem_collect_rough_pairs", [&]() -> int {
+        const int64_t n3 = 3 * c->nn;
+        std::vector<float> e(n3);
+        HIPCK(hipMemsetAsync(c->d_x32.p, 0, n3 * sizeof(float), c->stream));
+        for (int k = 0; k < count; k++) {
+            // e: independent uniform [-1, 1) per free dof (splitmix64 of seed, pair, dof), 0 at fixed nodes
+            for (int64_t i = 0; i < n3; i++) {
+                uint64_t z = seed * 0x9E3779B97F4A7C15ull + (uint64_t)k * 0xBF58476D1CE4E5B9ull + (uint64_t)i;
+                z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+                z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+                z ^= z >> 31;
+                e[i] = c->fixed_h[i / 3] ? 0.f : (float)((double)(z >> 11) * (2.0 / 9007199254740992.0) - 1.0);
+            }
+            HIPCK(hipMemcpyAsync(c->d_d32.p, e.data(), n3 * sizeof(float), hipMemcpyHostToDevice, c->stream));
+            hessvec(c, c->d_d32.as<float>(), c->d_q32.as<float>(), nullptr);
+            record_pair(c, c->d_d32.as<float>(), c->d_x32.as<float>(), c->d_q32.as<float>());
+        }
+        if (collected) *collected = count;
+        return SFEM_OK;
+    });
+}
+
 int sfem_num_pairs(sfem_ctx* c, int64_t* n)
 {
     if (!c || !n) return fail(SFEM_EINVAL, "sfem_num_pairs: null argument");

@@ -2,10 +2,16 @@
 and the trained weights it ships with.
 
     python -m solidfem_amd.learned train [--beams 24] [--epochs 30] [--out FILE]
+    python -m solidfem_amd.learned train-smoother [--rough 84]
 
-The weights in weights/learned_precond_nh.npz were written by that command
+The weights in weights/learned_precond_nh.npz were written by `train`
 (2,016 pairs on 24 beams, 30 epochs, 47 s on one MI355X); bench.py loads them
-for its learned-PCG leg.
+for its learned-PCG leg (the learned blocks as the preconditioner).  Those in
+weights/learned_smoother_nh.npz were written by `train-smoother` (84
+white-noise pairs per beam, the same 24 beams and schedule, 42 s): the
+network as the finest-level smoother of the multigrid preconditioner, whose
+job is the high-frequency error (held-out one-step loss 0.108 vs 0.123 for
+block Jacobi).
 
 Training data (the network, loss and device kernels are those of
 sfem_gnn_* in include/sfem_capi.h): seeded random Neo-Hookean beams from
@@ -33,6 +39,8 @@ from . import capi
 from .mesh import BeamMesh, traction_nodal_forces, uniform01
 
 WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "weights", "learned_precond_nh.npz")
+# the network trained as the finest-level smoother of the multigrid preconditioner
+SMOOTHER_WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "weights", "learned_smoother_nh.npz")
 GRIDS = [(32, 8, 8), (48, 12, 12), (64, 16, 16), (96, 24, 24), (128, 32, 32)]
 
 
@@ -69,8 +77,13 @@ def random_load(m, fixed, rng):
     return f * (1 - fixed[:, None])
 
 
-def collect(seed, grid, loads=3, nsnap=28):
+def collect(seed, grid, loads=3, nsnap=28, rough=0):
+    """pairs on training beam `seed`: PCG snapshots under `loads` random
+    loads, or (rough > 0) that many white-noise errors (smoother training)"""
     m, fixed, s = random_beam(seed, grid)
+    if rough:
+        capi.LearnedPreconditioner.collect_rough(s, rough, seed=3000 + seed)
+        return s
     rng = np.random.default_rng(2000 + seed)
     # snapshot spacing grows with the beam: the run spans most of a solve
     every = max(4, grid[0] // 2)
@@ -79,11 +92,12 @@ def collect(seed, grid, loads=3, nsnap=28):
     return s
 
 
-def train(beams=24, epochs=30, steps=20, batch=8, lr=2e-3, seed=1, grids=GRIDS, held_out=3, log=print):
-    """-> (trained LearnedPreconditioner, summary dict)"""
+def train(beams=24, epochs=30, steps=20, batch=8, lr=2e-3, seed=1, grids=GRIDS, held_out=3, log=print, rough=0):
+    """-> (trained LearnedPreconditioner, summary dict); rough > 0 trains the
+    smoother on that many white-noise pairs per beam"""
     t0 = time.perf_counter()
-    train_set = [collect(k, grids[k % len(grids)]) for k in range(beams)]
-    held = [collect(10000 + k, grids[(k + 2) % len(grids)]) for k in range(held_out)]
+    train_set = [collect(k, grids[k % len(grids)], rough=rough) for k in range(beams)]
+    held = [collect(10000 + k, grids[(k + 2) % len(grids)], rough=rough) for k in range(held_out)]
     npairs = sum(capi.LearnedPreconditioner.num_pairs(s) for s in train_set)
     log(f"collected {npairs} pairs on {beams} beams in {time.perf_counter() - t0:.1f} s")
     net = capi.LearnedPreconditioner(seed=seed)
@@ -104,7 +118,14 @@ def save(net, path=WEIGHTS, meta=None):
     np.savez(path, params=net.params(), meta=json.dumps(meta or {}))
 
 
-def load(path=WEIGHTS, seed=1):
+def load(path=None, seed=1, smoother=False):
+    """-> LearnedPreconditioner with the weights of `path` (default: the
+    shipped preconditioner weights, or the smoother weights)"""
+    path = path or (SMOOTHER_WEIGHTS if smoother else WEIGHTS)
+    return _load(path, seed)
+
+
+def _load(path, seed=1):
     """-> LearnedPreconditioner with the weights of `path`"""
     if not os.path.exists(path):
         raise FileNotFoundError(f"no trained weights at {path}; run python -m solidfem_amd.learned train")
@@ -117,15 +138,18 @@ def load(path=WEIGHTS, seed=1):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("cmd", choices=["train"])
+    ap.add_argument("cmd", choices=["train", "train-smoother"])
     ap.add_argument("--beams", type=int, default=24)
     ap.add_argument("--epochs", type=int, default=30)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--lr", type=float, default=2e-3)
-    ap.add_argument("--out", default=WEIGHTS)
+    ap.add_argument("--out", default=None)
+    ap.add_argument("--rough", type=int, default=84, help="white-noise pairs per beam (train-smoother)")
     a = ap.parse_args()
-    net, meta = train(a.beams, a.epochs, a.steps, lr=a.lr, log=lambda s: print(s, flush=True))
-    save(net, a.out, meta)
+    smoother = a.cmd == "train-smoother"
+    net, meta = train(a.beams, a.epochs, a.steps, lr=a.lr, log=lambda s: print(s, flush=True),
+                      rough=a.rough if smoother else 0)
+    save(net, a.out or (SMOOTHER_WEIGHTS if smoother else WEIGHTS), meta)
     print(json.dumps({k: v for k, v in meta.items() if k != "history"} | {"final": meta["history"][-1]}), flush=True)
 
 

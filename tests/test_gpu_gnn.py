@@ -89,6 +89,26 @@ def test_pairs_satisfy_K_e_equals_r():
         assert 0 < erk < er0
 
 
+def test_rough_pairs_satisfy_K_e_equals_r():
+    """white-noise pairs (smoother training): r = K e with e.Ke = r.K^-1 r
+    (checked against an FP64 oracle solve), reproducible from the seed"""
+    m, fixed, s, _ = beam()
+    assert capi.LearnedPreconditioner.collect_rough(s, 3, seed=7) == 3
+    p = O.OracleProblem(m.X, m.tets, fixed, O.NEO_HOOKEAN, 1e6, 0.45, 1070.0)
+    for k in range(3):
+        r, er = capi.LearnedPreconditioner.pair(s, k)
+        assert np.all(r[fixed == 1] == 0) and er > 0
+        e, _ = p.cg(m.X, r.astype(np.float64), 20000, 1e-8)
+        assert abs(er - np.sum(r * e)) / er < 2e-3, k
+        # white noise: no smooth component dominates (|e| spread over all free nodes)
+        mag = np.linalg.norm(e, axis=1)[fixed == 0]
+        assert np.median(mag) > 0.2 * mag.max()
+    r0, _ = capi.LearnedPreconditioner.pair(s, 0)
+    capi.LearnedPreconditioner.clear_pairs(s)
+    capi.LearnedPreconditioner.collect_rough(s, 1, seed=7)
+    assert np.array_equal(capi.LearnedPreconditioner.pair(s, 0)[0], r0)
+
+
 def test_loss_and_gradient_match_reference():
     m, fixed, s, _ = beam()
     net = capi.LearnedPreconditioner(seed=5)

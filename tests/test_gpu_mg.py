@@ -67,7 +67,10 @@ def test_vcycle_symmetric_positive_deterministic():
 
 
 def _weights_or_skip():
+    """the network trained as the smoother (else as the preconditioner)"""
     from solidfem_amd import learned
+    if os.path.exists(learned.SMOOTHER_WEIGHTS):
+        return learned.load(smoother=True)
     if not os.path.exists(learned.WEIGHTS):
         pytest.skip("no trained weights")
     return learned.load()
