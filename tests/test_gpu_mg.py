@@ -23,10 +23,10 @@ def rel2(a, b):
     return np.linalg.norm(a - b) / np.linalg.norm(b)
 
 
-def beam(grid, mg=True):
-    m = BeamMesh(*grid)
+def beam(grid, mg=True, nu=0.45, jitter=0.0):
+    m = BeamMesh(*grid, jitter=jitter, seed=5)
     fixed = m.fixed_mask()
-    s = capi.Solver(m.X, m.tets, fixed, capi.NEO_HOOKEAN, 1e6, 0.45, 1070.0, gravity=(0.0, 0.0, -9.8))
+    s = capi.Solver(m.X, m.tets, fixed, capi.NEO_HOOKEAN, 1e6, nu, 1070.0, gravity=(0.0, 0.0, -9.8))
     tris = m.tip_triangles()
     f = traction_nodal_forces(m.X, tris, lambda p: np.tile([0.0, 0.0, -1e4], (len(p), 1)))
     s.set_external_forces(f)
@@ -106,6 +106,26 @@ def test_multigrid_pcg_matches_oracle(precond):
     assert res < 1e-8
     assert err < 1e-6
     assert it_mg < 0.2 * it_j
+
+
+def test_multigrid_near_incompressible_jittered_matches_oracle():
+    """workload-5 physics at 32x8x8: nu = 0.495, nodes jittered by +-0.1 cell
+    (the hierarchy stays nested), two load steps with the predictor"""
+    print("case 32x8x8 jittered nu=0.495", flush=True)
+    m, fixed, f, s = beam((32, 8, 8), nu=0.495, jitter=0.1)
+    p = O.OracleProblem(m.X, m.tets, fixed, capi.NEO_HOOKEAN, 1e6, 0.495, 1070.0, gravity=(0.0, 0.0, -9.8), fext=f)
+    xo = m.X.copy()
+    for k in range(2):
+        st = s.step(tol=1e-8, precond=capi.PC_MULTIGRID, sim_type=capi.ST_QUASI_STATIC, force_step=0.5)
+        assert st["converged"] == 1
+        xo, _ = p.newton_solve(xo, 0.5 * (k + 1), tol=1e-10, cg_rtol=1e-8)
+        print(f"  step {k}: newton {st['newton_iters']} cg {st['cg_iters']} fallbacks {st['fallbacks']}", flush=True)
+    x = s.positions()
+    err = rel2(x - m.X, xo - m.X)
+    res = np.linalg.norm(p.gradients(x, 1.0)) / np.linalg.norm(p.external(x, 1.0))
+    print(f"err {err:.3e} oracle-res {res:.3e}", flush=True)
+    assert res < 1e-8
+    assert err < 1e-6
 
 
 def test_multigrid_linear_solve():
