@@ -219,9 +219,11 @@ int sfem_time_hessvec(sfem_ctx* ctx, int every, double* ms_avg, int64_t* samples
    Prolongation from this level to the one above, CSR over the nodes of the
    level above: row i holds entries p_ptr[i] .. p_ptr[i+1]-1 of
    (p_col = coarse node, p_val = weight); node values, the same for x, y, z.
-   The level inherits the context's material (uniform moduli only). */
+   The level has the context's material model; its moduli are the context's
+   uniform ones, or per coarse element young_e / poisson_e (required when the
+   context has per-element moduli, e.g. averages over each coarse element). */
 int sfem_mg_add_level(sfem_ctx* ctx, const sfem_mesh* coarse, const int32_t* inject, const int64_t* p_ptr,
-                      const int32_t* p_col, const double* p_val);
+                      const int32_t* p_col, const double* p_val, const double* young_e, const double* poisson_e);
 int sfem_mg_clear(sfem_ctx* ctx);
 /* levels (1 + coarse levels) and, when lmax is non-null, the largest-eigenvalue
    estimates of M A per level from the last set-up */

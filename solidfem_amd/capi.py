@@ -107,7 +107,7 @@ def lib():
             "sfem_reset": ([V], C.c_int),
             "sfem_profile_hessvec": ([V, C.c_int, _dp, _dp], C.c_int),
             "sfem_time_hessvec": ([V, C.c_int, _dp, C.POINTER(C.c_int64)], C.c_int),
-            "sfem_mg_add_level": ([V, C.POINTER(Mesh), _ip, C.POINTER(C.c_int64), _ip, _dp], C.c_int),
+            "sfem_mg_add_level": ([V, C.POINTER(Mesh), _ip, C.POINTER(C.c_int64), _ip, _dp, _dp, _dp], C.c_int),
             "sfem_mg_clear": ([V], C.c_int),
             "sfem_mg_info": ([V, C.POINTER(C.c_int32), _dp], C.c_int),
             "sfem_mg_apply": ([V, _dp, _dp], C.c_int),
@@ -264,8 +264,10 @@ class Solver:
         self._minv = None if minv is None else np.ascontiguousarray(minv, dtype=np.float32).reshape(-1, 9)
         _check(lib().sfem_set_precond_blocks(self.h, _p(self._minv, _fp)))
 
-    def add_mg_level(self, X0, tets, fixed, inject, p_ptr, p_col, p_val):
+    def add_mg_level(self, X0, tets, fixed, inject, p_ptr, p_col, p_val, young_e=None, poisson_e=None):
         """one coarser level of the multigrid preconditioner (sfem_mg_add_level)"""
+        ye = None if young_e is None else np.ascontiguousarray(young_e, dtype=np.float64)
+        pe = None if poisson_e is None else np.ascontiguousarray(poisson_e, dtype=np.float64)
         X0 = np.ascontiguousarray(X0, dtype=np.float64).reshape(-1, 3)
         tets = np.ascontiguousarray(tets, dtype=np.int32).reshape(-1, 4)
         fixed = np.ascontiguousarray(fixed, dtype=np.uint8).reshape(-1)
@@ -275,10 +277,11 @@ class Solver:
         p_val = np.ascontiguousarray(p_val, dtype=np.float64)
         m = Mesh(_p(tets, _ip), tets.shape[0], _p(X0), X0.shape[0], fixed.ctypes.data_as(C.POINTER(C.c_uint8)))
         _check(lib().sfem_mg_add_level(self.h, C.byref(m), _p(inject, _ip), p_ptr.ctypes.data_as(C.POINTER(C.c_int64)),
-                                       _p(p_col, _ip), _p(p_val)))
+                                       _p(p_col, _ip), _p(p_val), _p(ye), _p(pe)))
 
     def set_multigrid(self, levels):
-        """levels: [(X0, tets, fixed, inject, p_ptr, p_col, p_val)] finest coarse level first"""
+        """levels: [(X0, tets, fixed, inject, p_ptr, p_col, p_val[, young_e, poisson_e])], finest
+        coarse level first (solidfem_amd.mesh.beam_hierarchy)"""
         _check(lib().sfem_mg_clear(self.h))
         for lv in levels:
             self.add_mg_level(*lv)

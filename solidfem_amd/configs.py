@@ -91,7 +91,7 @@ def run(cfg, precond=capi.PC_JACOBI, tol=1e-8, grid=None, max_newton=50, log=Non
     d = build(cfg, grid)
     s = solver(d)
     if precond in (capi.PC_MULTIGRID, capi.PC_MULTIGRID_LEARNED):
-        s.set_multigrid(beam_hierarchy(d["mesh"]))
+        s.set_multigrid(beam_hierarchy(d["mesh"], young_e=d["young_e"], poisson_e=d["poisson_e"]))
     if precond in (capi.PC_LEARNED, capi.PC_MULTIGRID_LEARNED):
         from . import learned
         smoother = precond == capi.PC_MULTIGRID_LEARNED and os.path.exists(learned.SMOOTHER_WEIGHTS)

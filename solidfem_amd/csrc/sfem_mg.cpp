@@ -306,11 +306,9 @@ namespace {
 // the element data of a small level for its direct solve (element order):
 // Dm^-1 and V mu, V lambda as the context computes them at set-up
 // (FemPhysicsMatrixFree.cpp:126-146, FemPhysicsBase.cpp:40-41)
-void dense_setup(MgLevel& m, const sfem_ctx* f, const sfem_mesh* mesh)
+void dense_setup(MgLevel& m, const sfem_ctx* f, const sfem_mesh* mesh, const double* young_e, const double* poisson_e)
 {
     const int64_t ne = mesh->ne, nn = mesh->nn;
-    const double E = f->young, nu = f->poisson;
-    const double lam = E * nu / (1.0 + nu) / (1.0 - 2.0 * nu), mu = 0.5 * E / (1.0 + nu);
     std::vector<int> tets4(4 * ne);
     std::vector<float> dminv(9 * ne), cvol(2 * ne);
     std::vector<std::vector<std::pair<int, int>>> inc(nn);
@@ -329,6 +327,8 @@ void dense_setup(MgLevel& m, const sfem_ctx* f, const sfem_mesh* mesh)
         v[5] = D[2] * D[3] - D[0] * D[5];
         v[8] = D[0] * D[4] - D[1] * D[3];
         const double det = D[0] * v[0] + D[1] * v[3] + D[2] * v[6];
+        const double E = young_e ? young_e[e] : f->young, nu = poisson_e ? poisson_e[e] : f->poisson;
+        const double lam = E * nu / (1.0 + nu) / (1.0 - 2.0 * nu), mu = 0.5 * E / (1.0 + nu);
         for (int k = 0; k < 9; k++) dminv[9 * e + k] = (float)(v[k] / det);
         cvol[2 * e] = (float)(det / 6.0 * mu);
         cvol[2 * e + 1] = (float)(det / 6.0 * lam);
@@ -364,12 +364,11 @@ void dense_setup(MgLevel& m, const sfem_ctx* f, const sfem_mesh* mesh)
 extern "C" {
 
 int sfem_mg_add_level(sfem_ctx* f, const sfem_mesh* coarse, const int32_t* inject, const int64_t* p_ptr,
-                      const int32_t* p_col, const double* p_val)
+                      const int32_t* p_col, const double* p_val, const double* young_e, const double* poisson_e)
 {
     if (!f || !coarse || !inject || !p_ptr || !p_col || !p_val) return fail(SFEM_EINVAL, "sfem_mg_add_level: null argument");
-    if (f->per_element)
-        return fail(SFEM_EINVAL, "sfem_mg_add_level: coarse levels inherit a uniform material; this context has per-element "
-                                 "moduli");
+    if (f->per_element && (!young_e || !poisson_e))
+        return fail(SFEM_EINVAL, "sfem_mg_add_level: this context has per-element moduli; give the level's own");
     sfem_ctx* above = f->mg.empty() ? f : f->mg.back()->c;
     const int64_t na = above->nn, nc = coarse->nn;
     if (nc <= 0 || nc >= na) return fail(SFEM_EINVAL, "sfem_mg_add_level: a coarse level has fewer nodes than the one above");
@@ -387,6 +386,8 @@ int sfem_mg_add_level(sfem_ctx* f, const sfem_mesh* coarse, const int32_t* injec
     mat.young = f->young;
     mat.poisson = f->poisson;
     mat.density = f->density;
+    mat.young_e = young_e;
+    mat.poisson_e = poisson_e;
     sfem_ctx* lc = nullptr;
     int rc = sfem_create(coarse, &mat, &lc);
     if (rc != SFEM_OK) return rc;
@@ -421,7 +422,7 @@ int sfem_mg_add_level(sfem_ctx* f, const sfem_mesh* coarse, const int32_t* injec
                 trow[slot] = (int)i;
                 tval[slot] = val[k];
             }
-        if (3 * nc <= MG_DENSE_MAX) dense_setup(*m, f, coarse);
+        if (3 * nc <= MG_DENSE_MAX) dense_setup(*m, f, coarse, young_e, poisson_e);
         m->p_ptr.upload(ptr);
         m->p_col.upload(col);
         m->p_val.upload(val);

@@ -171,17 +171,33 @@ def load_tet1(path, scale=1.0):
     return np.array(X, dtype=np.float64), np.array(T, dtype=np.int32)
 
 
-def beam_hierarchy(m, min_cells=2, fixed=None):
+def coarse_cell_mean(fine, values):
+    """per-element values of `fine` (a BeamMesh, 6 tets per hex in hex order
+    iy, iz, ix) -> per-element values of the beam with half the cells in every
+    direction: the mean over the 48 fine tets inside each coarse hex, given
+    to its 6 tets"""
+    v = np.asarray(values, dtype=np.float64).reshape(fine.ny, fine.nz, fine.nx, 6).mean(axis=3)
+    v = v.reshape(fine.ny // 2, 2, fine.nz // 2, 2, fine.nx // 2, 2).mean(axis=(1, 3, 5))
+    return np.repeat(v.reshape(-1), 6)
+
+
+def beam_hierarchy(m, min_cells=2, fixed=None, young_e=None, poisson_e=None):
     """Nested coarse levels of a beam mesh for the multigrid preconditioner:
     each level halves the hex grid in every direction while all three cell
     counts stay even and the thinner cross-section keeps >= min_cells cells.
     Coarse node (i, j, k) sits at fine node (2i, 2j, 2k) (its position, so a
     jittered mesh stays nested), and the prolongation interpolates node
-    values trilinearly in the grid indices.
-    Returns [(X0, tets, fixed, inject, p_ptr, p_col, p_val)], finest coarse
-    level first (the arguments of capi.Solver.add_mg_level)."""
+    values trilinearly in the grid indices.  Per-element moduli (young_e,
+    poisson_e) are averaged over each coarse hex (coarse_cell_mean).
+    Returns [(X0, tets, fixed, inject, p_ptr, p_col, p_val[, young_e,
+    poisson_e])], finest coarse level first (the arguments of
+    capi.Solver.add_mg_level)."""
     levels = []
     fine, ffix = m, (m.fixed_mask() if fixed is None else np.asarray(fixed, dtype=np.uint8))
+    if (young_e is None) != (poisson_e is None):
+        raise ValueError("beam_hierarchy: give both young_e and poisson_e, or neither")
+    fE = None if young_e is None else np.asarray(young_e, dtype=np.float64)
+    fnu = None if poisson_e is None else np.asarray(poisson_e, dtype=np.float64)
     while (fine.nx % 2 == 0 and fine.ny % 2 == 0 and fine.nz % 2 == 0
            and min(fine.ny, fine.nz) // 2 >= min_cells):
         c = BeamMesh(fine.nx // 2, fine.ny // 2, fine.nz // 2, *fine.L)
@@ -217,6 +233,10 @@ def beam_hierarchy(m, min_cells=2, fixed=None):
         p_ptr = np.zeros(fine.nn + 1, dtype=np.int64)
         np.add.at(p_ptr, rows + 1, 1)
         p_ptr = np.cumsum(p_ptr)
-        levels.append((c.X, c.tets, cfix, inject, p_ptr, cols.astype(np.int32), vals.astype(np.float64)))
+        lv = (c.X, c.tets, cfix, inject, p_ptr, cols.astype(np.int32), vals.astype(np.float64))
+        if fE is not None:
+            fE, fnu = coarse_cell_mean(fine, fE), coarse_cell_mean(fine, fnu)
+            lv = lv + (fE, fnu)
+        levels.append(lv)
         fine, ffix = c, cfix
     return levels

@@ -14,7 +14,7 @@ import pytest
 
 from oracle import oracle as O
 from solidfem_amd import capi
-from solidfem_amd.mesh import BeamMesh, beam_hierarchy, traction_nodal_forces
+from solidfem_amd.mesh import BeamMesh, beam_hierarchy, lame, traction_nodal_forces, uniform01
 
 pytestmark = pytest.mark.gpu
 
@@ -126,6 +126,45 @@ def test_multigrid_near_incompressible_jittered_matches_oracle():
     print(f"err {err:.3e} oracle-res {res:.3e}", flush=True)
     assert res < 1e-8
     assert err < 1e-6
+
+
+def test_multigrid_per_element_moduli_matches_oracle():
+    """E log-uniform in [1e5, 1e7] and nu in [0.30, 0.48] per element; coarse
+    levels take the means over their cells"""
+    print("case 32x8x8 per-element moduli", flush=True)
+    m = BeamMesh(32, 8, 8)
+    fixed = m.fixed_mask()
+    E = 10 ** (5 + 2 * uniform01(4, 1, m.ne))
+    nu = 0.30 + 0.18 * uniform01(4, 2, m.ne)
+    f = traction_nodal_forces(m.X, m.tip_triangles(), lambda p: np.tile([0.0, 0.0, -1e3], (len(p), 1)))
+
+    def solver(mg):
+        s = capi.Solver(m.X, m.tets, fixed, capi.NEO_HOOKEAN, 1e6, 0.45, 1070.0, gravity=(0.0, 0.0, -9.8),
+                        young_e=E, poisson_e=nu)
+        s.set_external_forces(f)
+        if mg:
+            s.set_multigrid(beam_hierarchy(m, young_e=E, poisson_e=nu))
+        return s
+
+    s, sj = solver(True), solver(False)
+    mu, lam = lame(E, nu)
+    p = O.OracleProblem(m.X, m.tets, fixed, capi.NEO_HOOKEAN, mu=mu, lam=lam, density=1070.0, gravity=(0.0, 0.0, -9.8),
+                        fext=f)
+    xo = m.X.copy()
+    it_mg = it_j = 0
+    for k in range(2):
+        st = s.step(tol=1e-8, precond=capi.PC_MULTIGRID, sim_type=capi.ST_QUASI_STATIC, force_step=0.5, predictor=0)
+        assert st["converged"] == 1 and st["fallbacks"] == 0
+        it_mg += st["cg_iters"]
+        it_j += sj.step(tol=1e-8, precond=capi.PC_JACOBI, sim_type=capi.ST_QUASI_STATIC, force_step=0.5,
+                        predictor=0)["cg_iters"]
+        xo, _ = p.newton_solve(xo, 0.5 * (k + 1), tol=1e-10, cg_rtol=1e-8)
+    x = s.positions()
+    err = rel2(x - m.X, xo - m.X)
+    res = np.linalg.norm(p.gradients(x, 1.0)) / np.linalg.norm(p.external(x, 1.0))
+    print(f"err {err:.3e} oracle-res {res:.3e} cg mg {it_mg} jacobi {it_j}", flush=True)
+    assert res < 1e-8 and err < 1e-6
+    assert it_mg < 0.5 * it_j
 
 
 def test_multigrid_linear_solve():

@@ -5,7 +5,9 @@ grid indices (so constants and linear fields in the unjittered geometry), and
 element orientation."""
 import numpy as np
 
-from solidfem_amd.mesh import BeamMesh, beam_hierarchy
+import pytest
+
+from solidfem_amd.mesh import BeamMesh, beam_hierarchy, coarse_cell_mean
 
 
 def dense_p(nf, nc, ptr, col, val):
@@ -50,3 +52,27 @@ def test_prolongation_nesting_and_orientation():
 def test_no_level_for_odd_or_thin_grids():
     assert beam_hierarchy(BeamMesh(10, 3, 3)) == []
     assert beam_hierarchy(BeamMesh(16, 2, 2)) == []
+
+
+def test_coarse_moduli_are_cell_means():
+    """per-element moduli of a coarse level: the mean over the fine tets whose
+    centroids lie in each coarse hex (brute force over centroids)"""
+    m = BeamMesh(8, 4, 4)
+    rng = np.random.default_rng(3)
+    E = 10 ** (5 + 2 * rng.random(m.ne))
+    nu = 0.3 + 0.18 * rng.random(m.ne)
+    lv = beam_hierarchy(m, young_e=E, poisson_e=nu)
+    X0, tets, fx, inj, ptr, col, val, Ec, nuc = lv[0]
+    c = BeamMesh(4, 2, 2)
+    cf = m.X[m.tets].mean(axis=1)                 # fine centroids
+    cc = X0[tets].mean(axis=1)                    # coarse centroids
+    H = np.array(c.h)
+    fcell = np.floor(cf / H).astype(int)
+    ccell = np.floor(cc / H).astype(int)
+    for t in range(c.ne):
+        inside = np.all(fcell == ccell[t], axis=1)
+        assert inside.sum() == 48
+        assert np.isclose(Ec[t], E[inside].mean()) and np.isclose(nuc[t], nu[inside].mean())
+    assert np.allclose(coarse_cell_mean(m, np.full(m.ne, 2.5)), 2.5)
+    with pytest.raises(ValueError):
+        beam_hierarchy(m, young_e=E)
