@@ -36,7 +36,7 @@ enum { SFEM_ST_STATIC = 0, SFEM_ST_QUASI_STATIC = 1, SFEM_ST_EXPLICIT = 2, SFEM_
      Newton steps, each an unpreconditioned CG from zero to |r.r| <= (1e-3f)^2
      |r0.r0| or inner_iterations; no residual test, no line search.
    CONVERGED: Newton to ||r||_2 <= tol * ||r_ext||_2 (FP64), PCG inner solves
-     to cg_rtol, FP64 back-tracking line search, Jacobi fallback judge.       */
+     to cg_rtol, FP64 back-tracking line search, block-Jacobi fallback.       */
 enum { SFEM_SOLVE_REFERENCE = 0, SFEM_SOLVE_CONVERGED = 1 };
 
 /* Preconditioners for the inner CG.  MULTIGRID: one V-cycle over the levels
@@ -254,7 +254,7 @@ int sfem_gnn_get_params(sfem_gnn* gnn, float* w);
 int sfem_gnn_set_params(sfem_gnn* gnn, const float* w);
 
 /* Attaches the network to a context: SFEM_PC_LEARNED solves then rebuild the
-   blocks from it at every Newton step (Jacobi judge and fallback unchanged).
+   blocks from it at every Newton step (block-Jacobi fallback unchanged).
    NULL detaches.  The network must outlive its attachment. */
 int sfem_set_learned_preconditioner(sfem_ctx* ctx, sfem_gnn* gnn);
 

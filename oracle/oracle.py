@@ -3,7 +3,7 @@ reference (oracle/_ref/libsfem_ref.so, present only where the reference
 checkout was available at build time).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
-cpu_baseline leg of bench.py, as the checker.  The product (solidfem_amd/)
+cpu_baseline leg of bench.py, to compare against.  The product (solidfem_amd/)
 never imports this module.
 """
 from __future__ import annotations

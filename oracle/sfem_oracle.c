@@ -1,8 +1,8 @@
 /*
  * sfem_oracle.c -- CPU FP64 restatement of SolidFEM's matrix-free Newton-CG path.
  *
- * TEST INFRASTRUCTURE ONLY.  This file is the checker that the HIP product path
- * is compared against; it is never linked into, loaded by, or called from the
+ * TEST INFRASTRUCTURE ONLY.  This file is the FP64 reference that the HIP product
+ * path is compared against; it is never linked into, loaded by, or called from the
  * product library (solidfem_amd/).  Only tests/, __graft_entry__.smoke() and the
  * cpu_baseline leg of bench.py use it.
  *

@@ -466,7 +466,7 @@ int solve_converged(sfem_ctx* c, const sfem_solve_params* p, sfem_solve_stats* s
         }
         st->cg_iters += cg.iters;
         if ((learned || multigrid) && cg.status != 1) {
-            // judge: the learned / multigrid PCG did not reach its forcing -> redo with Jacobi
+            // the learned / multigrid PCG did not reach its forcing: redo with block Jacobi
             st->fallbacks++;
             build_jacobi(c);
             cg = pcg(c, c->d_minv.as<float>(), 1, p->inner_iterations, 1, eta * eta, 1, p->check_every, &st->hessvec_evals);

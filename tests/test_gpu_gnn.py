@@ -167,7 +167,7 @@ def test_training_improves_on_block_jacobi_and_is_deterministic():
     assert np.array_equal(runs[0], runs[1])
 
 
-def test_learned_pcg_newton_solve_converges_with_judge():
+def test_learned_pcg_newton_solve_converges_with_jacobi_fallback():
     m, fixed, s, _ = beam((12, 4, 4), seed=7, per_elem=True)
     net = capi.LearnedPreconditioner(seed=2)
     for j in range(3):
@@ -189,7 +189,7 @@ def test_learned_pcg_newton_solve_converges_with_judge():
 
 def test_shipped_weights_learned_pcg_matches_jacobi():
     """The trained weights shipped in the package drive a learned-PCG
-    quasi-static solve (with the Jacobi judge) to the same state as
+    quasi-static solve (with the block-Jacobi fallback) to the same state as
     Jacobi-PCG."""
     from solidfem_amd import learned
     net = learned.load()
