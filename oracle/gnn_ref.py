@@ -1,7 +1,7 @@
 """FP64 CPU reference of the learned block preconditioner (the network, its
 block construction, the training loss and the parameter gradient).
 
-TEST INFRASTRUCTURE ONLY: the checker for the device kernels of
+TEST INFRASTRUCTURE ONLY: the FP64 reference for the device kernels of
 solidfem_amd/csrc/sfem_gnn.hip and sfem_gnn_host.cpp.  The reference has no
 counterpart (SolidFEM's only preconditioner is a probed scalar diagonal,
 Math/IterativeSolvers.h:133-152); this restates the network defined in
