@@ -105,6 +105,14 @@ void FemPhysicsMatrixFree::Create(const double* youngE, const double* poissonE)
     mHostDirty = true;
     PushPositions();
     Check(sfem_set_load_fraction(mCtx, mForceFraction), "sfem_set_load_fraction");
+    // the multigrid levels follow the new material: uniform moduli re-add them
+    // without coarse moduli; per-element moduli need the caller's coarse means
+    if (youngE) mMultigrid.clear();
+    for (size_t k = 0; k < mMultigrid.size(); k++) {
+        mMultigrid[k].mYoung.clear();
+        mMultigrid[k].mPoisson.clear();
+        PushMultigridLevel(k);
+    }
 }
 
 // FemPhysicsBase::SetLameParams (FemPhysicsBase.h:98-106)
@@ -295,6 +303,46 @@ void FemPhysicsMatrixFree::SetPreconditionerBlocks(const std::vector<float>& min
     }
     if (minv9.size() != 9 * mNodes.size()) throw std::runtime_error("SetPreconditionerBlocks: nn*9 values expected");
     Check(sfem_set_precond_blocks(mCtx, minv9.data()), "sfem_set_precond_blocks");
+}
+
+void FemPhysicsMatrixFree::PushMultigridLevel(size_t k)
+{
+    const MultigridLevel& l = mMultigrid[k];
+    const size_t nc = l.mNodes.size(), nec = l.mTets.size() / 4;
+    if (l.mTets.size() != 4 * nec || l.mFixed.size() != nc || l.mInject.size() != nc)
+        throw std::runtime_error("AddMultigridLevel: 4 node ids per tet and one fixed flag and injection index per node expected");
+    const size_t nf = k == 0 ? mNodes.size() : mMultigrid[k - 1].mNodes.size();
+    if (l.mProlongPtr.size() != nf + 1 || l.mProlongCol.size() != l.mProlongVal.size() ||
+        (int64_t)l.mProlongCol.size() != l.mProlongPtr.back())
+        throw std::runtime_error("AddMultigridLevel: prolongation rows must cover the level above");
+    const bool moduli = !l.mYoung.empty() || !l.mPoisson.empty();
+    if (moduli && (l.mYoung.size() != nec || l.mPoisson.size() != nec))
+        throw std::runtime_error("AddMultigridLevel: one Young's modulus and Poisson ratio per coarse element expected");
+    std::vector<double> X(3 * nc);
+    for (size_t i = 0; i < nc; i++)
+        for (int c = 0; c < 3; c++) X[3 * i + c] = l.mNodes[i][c];
+    sfem_mesh m{l.mTets.data(), (int64_t)nec, X.data(), (int64_t)nc, l.mFixed.data()};
+    Check(sfem_mg_add_level(mCtx, &m, l.mInject.data(), l.mProlongPtr.data(), l.mProlongCol.data(),
+                            l.mProlongVal.data(), moduli ? l.mYoung.data() : nullptr,
+                            moduli ? l.mPoisson.data() : nullptr),
+          "sfem_mg_add_level");
+}
+
+void FemPhysicsMatrixFree::AddMultigridLevel(const MultigridLevel& level)
+{
+    mMultigrid.push_back(level);
+    try {
+        PushMultigridLevel(mMultigrid.size() - 1);
+    } catch (...) {
+        mMultigrid.pop_back();
+        throw;
+    }
+}
+
+void FemPhysicsMatrixFree::ClearMultigrid()
+{
+    Check(sfem_mg_clear(mCtx), "sfem_mg_clear");
+    mMultigrid.clear();
 }
 
 sfem_solve_params FemPhysicsMatrixFree::Params() const

@@ -143,7 +143,29 @@ protected:
 class FemPhysicsMatrixFree : public FemPhysicsBase {
 public:
     enum NodalForcesType { NFT_DIRECT, NFT_FINITE_VOLUME };
-    enum Preconditioner { PC_NONE = SFEM_PC_NONE, PC_JACOBI = SFEM_PC_JACOBI, PC_LEARNED = SFEM_PC_LEARNED };
+    enum Preconditioner {
+        PC_NONE = SFEM_PC_NONE,
+        PC_JACOBI = SFEM_PC_JACOBI,
+        PC_LEARNED = SFEM_PC_LEARNED,
+        PC_MULTIGRID = SFEM_PC_MULTIGRID,                  // needs AddMultigridLevel
+        PC_MULTIGRID_LEARNED = SFEM_PC_MULTIGRID_LEARNED   // + SetPreconditionerBlocks
+    };
+
+    // One coarser nested mesh of the multigrid hierarchy (no reference
+    // counterpart; sfem_mg_add_level): coarse rest positions, tets and fixed
+    // flags, the fine node each coarse node injects from, the prolongation
+    // (CSR, one row per node of the level above, columns coarse nodes) and,
+    // when the fine mesh has per-element moduli, the coarse cells' moduli.
+    struct MultigridLevel {
+        std::vector<Vector3R> mNodes;
+        std::vector<int32_t> mTets;      // 4 per element
+        std::vector<uint8_t> mFixed;
+        std::vector<int32_t> mInject;
+        std::vector<int64_t> mProlongPtr;
+        std::vector<int32_t> mProlongCol;
+        std::vector<double> mProlongVal;
+        std::vector<real> mYoung, mPoisson;
+    };
 
     // FemPhysicsMatrixFree::Config (FemPhysicsMatrixFree.h:52-58) plus the
     // solver controls of this implementation (no reference counterpart):
@@ -201,6 +223,12 @@ public:
     void SetLameParams(real mu, real lambda);
     void SetPerElementMaterial(const std::vector<real>& young, const std::vector<real>& poisson);
     void SetPreconditionerBlocks(const std::vector<float>& minv9);   // original node order, nn*9
+    // appends the next coarser level (finest first); SetLameParams keeps the
+    // levels (with the new uniform moduli), SetPerElementMaterial drops them:
+    // add them again with the coarse cells' moduli.
+    void AddMultigridLevel(const MultigridLevel& level);
+    void ClearMultigrid();
+    uint32 GetNumMultigridLevels() const { return 1 + (uint32)mMultigrid.size(); }
 
     const sfem_solve_stats& GetLastSolveStats() const { return mStats; }
     const Config& GetConfig() const { return mConfig; }
@@ -215,6 +243,7 @@ private:
     void PullPositions() const;
     sfem_solve_params Params() const;
     void Check(int rc, const char* what) const;
+    void PushMultigridLevel(size_t k);
 
     sfem_ctx* mCtx = nullptr;
     Config mConfig;
@@ -224,6 +253,7 @@ private:
     std::vector<real> mVol, mDmInv;
     mutable bool mHostDirty = false;    // host positions newer than the device
     mutable bool mDeviceDirty = false;  // device positions newer than the host
+    std::vector<MultigridLevel> mMultigrid;
     sfem_solve_stats mStats{};
 };
 

@@ -177,6 +177,42 @@ public:
         mPhys->SetPreconditionerBlocks(std::vector<float>(m.data(), m.data() + m.size()));
     }
 
+    // one coarser nested level of the multigrid hierarchy, finest first (the
+    // tuples solidfem_amd.mesh.beam_hierarchy returns unpack into it)
+    void AddMultigridLevel(py::array_t<double, py::array::c_style | py::array::forcecast> nodes,
+                           py::array_t<int, py::array::c_style | py::array::forcecast> tets,
+                           py::array_t<uint8_t, py::array::c_style | py::array::forcecast> fixed,
+                           py::array_t<int, py::array::c_style | py::array::forcecast> inject,
+                           py::array_t<int64_t, py::array::c_style | py::array::forcecast> pPtr,
+                           py::array_t<int, py::array::c_style | py::array::forcecast> pCol,
+                           py::array_t<double, py::array::c_style | py::array::forcecast> pVal, py::object young,
+                           py::object poisson)
+    {
+        if (nodes.size() % 3 != 0) throw std::runtime_error("add_multigrid_level: nodes must be n x 3");
+        FemPhysicsMatrixFree::MultigridLevel l;
+        l.mNodes.resize(nodes.size() / 3);
+        const double* x = nodes.data();
+        for (size_t i = 0; i < l.mNodes.size(); i++) l.mNodes[i] = Vector3R(x[3 * i], x[3 * i + 1], x[3 * i + 2]);
+        l.mTets.assign(tets.data(), tets.data() + tets.size());
+        l.mFixed.assign(fixed.data(), fixed.data() + fixed.size());
+        l.mInject.assign(inject.data(), inject.data() + inject.size());
+        l.mProlongPtr.assign(pPtr.data(), pPtr.data() + pPtr.size());
+        l.mProlongCol.assign(pCol.data(), pCol.data() + pCol.size());
+        l.mProlongVal.assign(pVal.data(), pVal.data() + pVal.size());
+        if (young.is_none() != poisson.is_none())
+            throw std::runtime_error("add_multigrid_level: give both coarse moduli arrays or neither");
+        if (!young.is_none()) {
+            auto e = py::cast<py::array_t<double, py::array::c_style | py::array::forcecast>>(young);
+            auto n = py::cast<py::array_t<double, py::array::c_style | py::array::forcecast>>(poisson);
+            l.mYoung.assign(e.data(), e.data() + e.size());
+            l.mPoisson.assign(n.data(), n.data() + n.size());
+        }
+        mPhys->AddMultigridLevel(l);
+    }
+
+    void ClearMultigrid() { mPhys->ClearMultigrid(); }
+    int NumMultigridLevels() const { return (int)mPhys->GetNumMultigridLevels(); }
+
     py::dict LastStats() const
     {
         const sfem_solve_stats& s = mPhys->GetLastSolveStats();
@@ -320,9 +356,12 @@ private:
         if (config.contains("predictor")) mCfg.mPredictor = py::cast<bool>(config["predictor"]);
         if (config.contains("precond")) {
             std::string p = py::cast<std::string>(config["precond"]);
-            mCfg.mPreconditioner = p == "none" ? FemPhysicsMatrixFree::PC_NONE
-                                   : p == "learned" ? FemPhysicsMatrixFree::PC_LEARNED
-                                                    : FemPhysicsMatrixFree::PC_JACOBI;
+            if (p == "none") mCfg.mPreconditioner = FemPhysicsMatrixFree::PC_NONE;
+            else if (p == "jacobi") mCfg.mPreconditioner = FemPhysicsMatrixFree::PC_JACOBI;
+            else if (p == "learned") mCfg.mPreconditioner = FemPhysicsMatrixFree::PC_LEARNED;
+            else if (p == "multigrid") mCfg.mPreconditioner = FemPhysicsMatrixFree::PC_MULTIGRID;
+            else if (p == "multigrid_learned") mCfg.mPreconditioner = FemPhysicsMatrixFree::PC_MULTIGRID_LEARNED;
+            else throw std::runtime_error("precond: none, jacobi, learned, multigrid or multigrid_learned expected");
         }
         c.mCustomConfig = &mCfg;
         return c;
@@ -354,6 +393,11 @@ PYBIND11_MODULE(pysolidfem, m)
         .def("set_pressure", &PyNonlinearFEM::SetPressure)
         .def("set_external_forces", &PyNonlinearFEM::SetExternalForces)
         .def("set_preconditioner_blocks", &PyNonlinearFEM::SetPreconditionerBlocks)
+        .def("add_multigrid_level", &PyNonlinearFEM::AddMultigridLevel, py::arg("nodes"), py::arg("tets"),
+             py::arg("fixed"), py::arg("inject"), py::arg("p_ptr"), py::arg("p_col"), py::arg("p_val"),
+             py::arg("young_e") = py::none(), py::arg("poisson_e") = py::none())
+        .def("clear_multigrid", &PyNonlinearFEM::ClearMultigrid)
+        .def("num_multigrid_levels", &PyNonlinearFEM::NumMultigridLevels)
         .def("last_stats", &PyNonlinearFEM::LastStats)
         .def("get_force_fraction", &PyNonlinearFEM::GetForceFraction)
         .def("check_for_inversion", &PyNonlinearFEM::CheckForInversion);
