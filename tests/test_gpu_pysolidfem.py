@@ -121,12 +121,13 @@ def test_multigrid_through_bindings():
     the converged Newton-LS solve against the FP64 oracle, fewer CG iterations
     than block Jacobi, and the levels kept across set_lame_params"""
     from solidfem_amd.mesh import beam_hierarchy
-    m = BeamMesh(16, 4, 4)
+    m = BeamMesh(16, 8, 8)
     cfg = dict(material="neo-hookean", solver="newton_ls", rel_tol=1e-8, tol=0.0, maxiter=50)
     fem = _fem(m, dict(cfg, precond="multigrid"))
-    for lv in beam_hierarchy(m):
+    levels = beam_hierarchy(m)          # 8x4x4 -> 4x2x2 (solved directly)
+    for lv in levels:
         fem.add_multigrid_level(*lv)
-    assert fem.num_multigrid_levels() == 3
+    assert fem.num_multigrid_levels() == 1 + len(levels) == 3
     fem.step()
     st = fem.last_stats()
     fj = _fem(m, dict(cfg, precond="jacobi"))
