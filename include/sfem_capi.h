@@ -25,8 +25,15 @@
 extern "C" {
 #endif
 
-/* MaterialModelType values (Include/FemDataStructures.h:84-88) */
-enum { SFEM_LINEAR = 0, SFEM_STVK = 1, SFEM_COROTATIONAL = 2, SFEM_NEO_HOOKEAN = 3 };
+/* MaterialModelType values (Include/FemDataStructures.h:84-88).
+   SFEM_COROTATIONAL is the reference's model: the Gram-Schmidt rotation of
+   QRPolarDecomposition (PolarDecomposition.cpp:269-280) and its tangent
+   (FemPhysicsMatrixFree.cpp:441-456), which is not the derivative of its force.
+   SFEM_COROTATIONAL_POLAR (no reference counterpart; a value past the
+   reference's MaterialModelType range) is the corotational model with the
+   rotation of the polar decomposition F = R S, an energy
+   Psi = mu |F - R|^2 + lam/2 tr(S - I)^2 and its consistent, symmetric tangent. */
+enum { SFEM_LINEAR = 0, SFEM_STVK = 1, SFEM_COROTATIONAL = 2, SFEM_NEO_HOOKEAN = 3, SFEM_COROTATIONAL_POLAR = 16 };
 
 /* SimulationType values (Include/FemDataStructures.h:71-77) */
 enum { SFEM_ST_STATIC = 0, SFEM_ST_QUASI_STATIC = 1, SFEM_ST_EXPLICIT = 2, SFEM_ST_IMPLICIT = 3 };

@@ -19,6 +19,7 @@ LIB_PATH = os.path.join(HERE, "liboracle.so")
 REF_PATH = os.path.join(HERE, "_ref", "libsfem_ref.so")
 
 LINEAR, STVK, COROTATIONAL, NEO_HOOKEAN = 0, 1, 2, 3
+COROTATIONAL_POLAR = 16     # no reference counterpart: the polar rotation, consistent tangent
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int32)

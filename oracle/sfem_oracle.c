@@ -26,6 +26,11 @@
  *                         compiled reference trajectories (iteration counts, x)
  *   loads               : closed-form totals (rho V g, p * sum(area n), torque)
  *   block_diag          : brute-force K e_j probes
+ *   COROTATIONAL_POLAR  : no reference counterpart (the corotational model
+ *                         with the polar rotation).  The rotation against
+ *                         scipy.linalg.polar (SVD), P against -dPsi/dx by
+ *                         central differences, dP against central differences
+ *                         of P and K against its transpose.
  */
 #include <math.h>
 #include <stdint.h>
@@ -35,7 +40,7 @@
 #include <omp.h>
 #endif
 
-enum { MAT_LINEAR = 0, MAT_STVK = 1, MAT_COROTATIONAL = 2, MAT_NEO_HOOKEAN = 3 };
+enum { MAT_LINEAR = 0, MAT_STVK = 1, MAT_COROTATIONAL = 2, MAT_NEO_HOOKEAN = 3, MAT_COROTATIONAL_POLAR = 16 };
 
 typedef struct {
     int64_t ne, nn;
@@ -136,6 +141,37 @@ static void qr_rotation(const double* F, double* R, double* S)
     }
 }
 
+/* Orthogonal factor of the polar decomposition F = R U by Newton's iteration
+   X <- (X + X^-T)/2 (Higham 1986), scaled by g = sqrt(|X^-1|_F / |X|_F)
+   while the update is large, run until the update stops shrinking; then
+   S = (R^T F + F^T R)/2.  COROTATIONAL_POLAR only (no reference counterpart). */
+static void polar_rotation(const double* F, double* R, double* S)
+{
+    double X[9], prev = INFINITY;
+    memcpy(X, F, sizeof X);
+    for (int it = 0; it < 100; it++) {
+        double Xi[9], nx = 0, ni = 0, du = 0;
+        m3_inv(X, Xi);
+        for (int k = 0; k < 9; k++) { nx += X[k] * X[k]; ni += Xi[k] * Xi[k]; }
+        const double g = prev > 1e-2 ? sqrt(sqrt(ni / nx)) : 1.0;
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) {
+                const double xn = 0.5 * (g * X[3*i+j] + Xi[3*j+i] / g);
+                du += (xn - X[3*i+j]) * (xn - X[3*i+j]);
+                X[3*i+j] = xn;
+            }
+        du = sqrt(du);
+        if (du >= prev || du < 1e-15) break;
+        prev = du;
+    }
+    memcpy(R, X, sizeof X);
+    double Rt[9], V[9], Vt[9];
+    m3_tr(R, Rt);
+    m3_mul(Rt, F, V);
+    m3_tr(V, Vt);
+    for (int k = 0; k < 9; k++) S[k] = 0.5 * (V[k] + Vt[k]);
+}
+
 /* ------------------------------------------------------ constitutive laws */
 
 /* First Piola-Kirchhoff stress P(F): ElasticEnergy::ComputeElementStress
@@ -145,12 +181,13 @@ void orc_stress(int mat, double mu, double lam, const double* F_in, double* P)
 {
     double F[9], R[9];
     memcpy(F, F_in, sizeof F);
-    if (mat == MAT_COROTATIONAL) {
+    if (mat == MAT_COROTATIONAL || mat == MAT_COROTATIONAL_POLAR) {
         double S[9];
-        qr_rotation(F_in, R, S);
+        if (mat == MAT_COROTATIONAL) qr_rotation(F_in, R, S);
+        else polar_rotation(F_in, R, S);
         memcpy(F, S, sizeof F);     /* F = U (:133-136) */
     }
-    if (mat == MAT_LINEAR || mat == MAT_COROTATIONAL) {
+    if (mat == MAT_LINEAR || mat == MAT_COROTATIONAL || mat == MAT_COROTATIONAL_POLAR) {
         double e[9], Ft[9];
         m3_tr(F, Ft);
         for (int k = 0; k < 9; k++) e[k] = 0.5 * (F[k] + Ft[k]);
@@ -208,6 +245,31 @@ void orc_dstress(int mat, double mu, double lam, const double* F, const double* 
         m3_mul(R, X, dR);
         double c = lam * (trS - 3) - 2 * mu;
         for (int k = 0; k < 9; k++) dP[k] += c * dR[k];
+    } else if (mat == MAT_COROTATIONAL_POLAR) {
+        /* d/dF of R (2 mu (S - I) + lam tr(S - I) I) with F = R S: R^T dR = [w]x
+           where (S - tr(S) I) w = -axial(R^T dF - dF^T R), so that
+           dP = 2 mu dF + lam tr(R^T dF) R + (lam (tr S - 3) - 2 mu) dR.  An
+           element with det F <= 0 takes the tangent at F = I.               */
+        double R[9], S[9], Rt[9], K[9];
+        if (m3_det(F) > 0) polar_rotation(F, R, S);
+        else {
+            for (int k = 0; k < 9; k++) R[k] = S[k] = (k % 4 == 0) ? 1.0 : 0.0;
+        }
+        m3_tr(R, Rt);
+        m3_mul(Rt, dF, K);
+        const double trS = m3_trace(S), trK = m3_trace(K);
+        double A[9], Ai[9];
+        memcpy(A, S, sizeof A);
+        A[0] -= trS; A[4] -= trS; A[8] -= trS;
+        m3_inv(A, Ai);
+        const double ax[3] = { K[7] - K[5], K[2] - K[6], K[3] - K[1] };
+        double w[3];
+        for (int i = 0; i < 3; i++) w[i] = -(Ai[3*i+0]*ax[0] + Ai[3*i+1]*ax[1] + Ai[3*i+2]*ax[2]);
+        double W[9] = { 0, -w[2], w[1],  w[2], 0, -w[0],  -w[1], w[0], 0 };
+        double dR[9];
+        m3_mul(R, W, dR);
+        const double c = lam * (trS - 3) - 2 * mu;
+        for (int k = 0; k < 9; k++) dP[k] = 2 * mu * dF[k] + lam * trK * R[k] + c * dR[k];
     } else if (mat == MAT_STVK) {                              /* :457-463 */
         double Ft[9], dFt[9], E[9], dE[9], a[9], b[9], T1[9], T2[9];
         m3_tr(F, Ft); m3_tr(dF, dFt);
@@ -247,9 +309,10 @@ double orc_energy_density(int mat, double mu, double lam, const double* F_in)
 {
     double F[9];
     memcpy(F, F_in, sizeof F);
-    if (mat == MAT_COROTATIONAL) {
+    if (mat == MAT_COROTATIONAL || mat == MAT_COROTATIONAL_POLAR) {
         double R[9], S[9];
-        qr_rotation(F_in, R, S);
+        if (mat == MAT_COROTATIONAL) qr_rotation(F_in, R, S);
+        else polar_rotation(F_in, R, S);
         memcpy(F, S, sizeof F);
     }
     if (mat == MAT_NEO_HOOKEAN) {

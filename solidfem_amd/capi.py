@@ -14,6 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libsfem_hip.so")
 
 LINEAR, STVK, COROTATIONAL, NEO_HOOKEAN = 0, 1, 2, 3
+COROTATIONAL_POLAR = 16     # the polar rotation and its consistent tangent (no reference counterpart)
 ST_STATIC, ST_QUASI_STATIC, ST_EXPLICIT, ST_IMPLICIT = 0, 1, 2, 3
 SOLVE_REFERENCE, SOLVE_CONVERGED = 0, 1
 PC_NONE, PC_JACOBI, PC_LEARNED, PC_MULTIGRID, PC_MULTIGRID_LEARNED = 0, 1, 2, 3, 4

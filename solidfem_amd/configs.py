@@ -2,7 +2,7 @@
 generator, and a runner that solves one on the device and reports iterations,
 timings and FP64 residuals.
 
-    python -m solidfem_amd.configs 1 [jacobi|multigrid|multigrid_learned|learned|none] [nopredictor]
+    python -m solidfem_amd.configs 1 [jacobi|multigrid|multigrid_learned|learned|none] [nopredictor] [polar]
 
 All beams span [0, 1] x [0, 0.25] x [0, 0.25] m with the x = 0 face clamped
 and 6 tets per hex (the Models/box2.tet pattern, see mesh.py).
@@ -16,7 +16,10 @@ and 6 tets per hex (the Models/box2.tet pattern, see mesh.py).
 3  128x32x32 (786,432 tets, 140,481 nodes), COROTATIONAL (the reference's QR
    rotation and tangent), per-element E = 10^U(5,7), nu = U(0.30,0.48) from
    seed 0, 20 load steps of a dead tip torsion t = tau e_x x (p - c),
-   tau = 1e5 Pa/m about the face centroid c; Jacobi-PCG.
+   tau = 1e5 Pa/m about the face centroid c; Jacobi-PCG.  `polar` runs it
+   with COROTATIONAL_POLAR (the polar rotation and its consistent tangent):
+   the reference's tangent is not the derivative of its force, and Newton
+   stalls under the torsion's large rotations.
 4  learned preconditioner evaluation beam: 128x32x32 NEO_HOOKEAN as bench.py.
 5  256x64x64 (6,291,456 tets, 1,085,825 nodes), nodes jittered U(-0.1,0.1)
    cell (boundary nodes in-plane), NEO_HOOKEAN E 1e6, nu 0.495, gravity plus a
@@ -43,9 +46,10 @@ DESCRIPTIONS = {
 }
 
 
-def build(cfg, grid=None):
+def build(cfg, grid=None, material=None):
     """-> dict(mesh, fixed, material, young, poisson, young_e, poisson_e,
-    gravity, tris, pressure, fdead, load_steps)"""
+    gravity, tris, pressure, fdead, load_steps); `material` replaces the
+    workload's material model"""
     g = {1: (10, 3, 3), 2: (64, 16, 16), 3: (128, 32, 32), 4: (128, 32, 32), 5: (256, 64, 64)}[cfg]
     grid = grid or g
     m = BeamMesh(*grid, jitter=0.1 if cfg == 5 else 0.0, seed=5 if cfg == 5 else 0)
@@ -66,6 +70,8 @@ def build(cfg, grid=None):
     else:
         d.update(material=capi.NEO_HOOKEAN, poisson=0.495, load_steps=4,
                  fdead=traction_nodal_forces(m.X, tris, lambda p: np.tile([0.0, 0.0, -1e4], (len(p), 1))))
+    if material is not None:
+        d["material"] = material
     return d
 
 
@@ -84,11 +90,11 @@ PRECONDS = {"none": capi.PC_NONE, "jacobi": capi.PC_JACOBI, "learned": capi.PC_L
             "multigrid": capi.PC_MULTIGRID, "multigrid_learned": capi.PC_MULTIGRID_LEARNED}
 
 
-def run(cfg, precond=capi.PC_JACOBI, tol=1e-8, grid=None, max_newton=50, log=None, predictor=1):
+def run(cfg, precond=capi.PC_JACOBI, tol=1e-8, grid=None, max_newton=50, log=None, predictor=1, material=None):
     """Solves workload `cfg`; `log(str)` (optional) gets one line per load step.
     The multigrid preconditioners get the beam's nested hierarchy, the learned
     ones the shipped weights."""
-    d = build(cfg, grid)
+    d = build(cfg, grid, material)
     s = solver(d)
     if precond in (capi.PC_MULTIGRID, capi.PC_MULTIGRID_LEARNED):
         s.set_multigrid(beam_hierarchy(d["mesh"], young_e=d["young_e"], poisson_e=d["poisson_e"]))
@@ -103,7 +109,7 @@ def run(cfg, precond=capi.PC_JACOBI, tol=1e-8, grid=None, max_newton=50, log=Non
                           sim_type=capi.ST_QUASI_STATIC if steps > 1 else capi.ST_STATIC, force_step=1.0 / steps,
                           predictor=predictor)
     out = {"config": cfg, "description": DESCRIPTIONS[cfg], "tets": d["mesh"].ne, "nodes": d["mesh"].nn,
-           "steps": []}
+           "material": d["material"], "steps": []}
     t0 = time.perf_counter()
     for k in range(steps):
         st = s.step(p, allow_notconv=True)
@@ -125,7 +131,9 @@ if __name__ == "__main__":
     cfg = int(sys.argv[1])
     pc = sys.argv[2] if len(sys.argv) > 2 else "jacobi"
     pred = 0 if "nopredictor" in sys.argv[3:] else 1
-    out, d, s = run(cfg, PRECONDS[pc], log=lambda line: print(line, file=sys.stderr, flush=True), predictor=pred)
+    mat = capi.COROTATIONAL_POLAR if "polar" in sys.argv[3:] else None
+    out, d, s = run(cfg, PRECONDS[pc], log=lambda line: print(line, file=sys.stderr, flush=True), predictor=pred,
+                    material=mat)
     out["precond"] = pc
     out["predictor"] = pred
     print(json.dumps(out), flush=True)

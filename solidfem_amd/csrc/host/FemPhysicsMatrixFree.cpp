@@ -33,9 +33,9 @@ FemPhysicsMatrixFree::FemPhysicsMatrixFree(const std::vector<Tet>& tetrahedra, c
     : FemPhysicsBase(config), mTets(tetrahedra), mNodes(allNodes)
 {
     if (config.mCustomConfig) mConfig = *static_cast<const Config*>(config.mCustomConfig);
-    if (mMaterial > MMT_NEO_HOOKEAN)
-        throw std::runtime_error("FemPhysicsMatrixFree: only LINEAR, STVK, COROTATIONAL and NEO_HOOKEAN are on the "
-                                 "matrix-free Newton-CG path");
+    if (mMaterial > MMT_NEO_HOOKEAN && mMaterial != MMT_COROTATIONAL_POLAR)
+        throw std::runtime_error("FemPhysicsMatrixFree: only LINEAR, STVK, COROTATIONAL, NEO_HOOKEAN and "
+                                 "COROTATIONAL_POLAR are on the matrix-free Newton-CG path");
     const size_t nn = mNodes.size();
     // ReshuffleFixedNodes (FemPhysicsMatrixFree.cpp:229-263): fixed nodes first
     mFixed.resize(nn);

@@ -59,6 +59,9 @@ enum MaterialModelType {
     MMT_LINEAR, MMT_STVK, MMT_COROTATIONAL, MMT_NEO_HOOKEAN, MMT_NEO_HOOKEAN_OGDEN, MMT_DISTORTIONAL_LINEAR,
     MMT_DISTORTIONAL_MOONEY, MMT_DISTORTIONAL_NH2, MMT_DISTORTIONAL_NH3, MMT_DISTORTIONAL_BW, MMT_DISTORTIONAL_OGDEN,
     MMT_DISTORTIONAL_NH6, MMT_DISTORTIONAL_NH7,
+    // no reference counterpart: corotational with the polar rotation and its
+    // consistent tangent (SFEM_COROTATIONAL_POLAR)
+    MMT_COROTATIONAL_POLAR = SFEM_COROTATIONAL_POLAR,
 };
 enum NonlinearSolverType {
     NST_NEWTON, NST_NEWTON_LS, NST_NEWTON_CG, NST_NONLINEAR_CG, NST_GRADIENT_DESCENT, NST_STEEPEST_DESCENT, NST_NLOPT,

@@ -7,9 +7,10 @@
 // (young, poisson, density, simtype, substeps, maxiter, tol, mixed, optimizer;
 // PyNonlinearFEM.cpp:240-292).  New keys select what the reference hard-wires
 // (NH + NST_NEWTON_LS, :280-291): solver ("newton", "newton_ls", "newton_cg"),
-// material ("linear", "stvk", "corotational", "neo-hookean" or the enum value),
-// gravity, zup, inner_iters, rel_tol, cg_tol, precond ("none", "jacobi",
-// "learned"), force_step, young_e / poisson_e (per element arrays).
+// material ("linear", "stvk", "corotational", "neo-hookean", "corotational_polar"
+// or the enum value), gravity, zup, inner_iters, rel_tol, cg_tol, precond
+// ("none", "jacobi", "learned", "multigrid", "multigrid_learned"), force_step,
+// young_e / poisson_e (per element arrays).
 // Unlike the reference's FemBody::BuildBoundaryMesh, no uint16 node indices
 // are involved anywhere (meshes above 65,535 nodes work).
 #include <pybind11/numpy.h>
@@ -40,6 +41,7 @@ MaterialModelType parse_material(py::handle h)
     if (s == "linear") return MMT_LINEAR;
     if (s == "stvk") return MMT_STVK;
     if (s == "corotational" || s == "corot") return MMT_COROTATIONAL;
+    if (s == "corotational_polar" || s == "corot_polar") return MMT_COROTATIONAL_POLAR;
     if (s == "neo-hookean" || s == "neohookean" || s == "nh") return MMT_NEO_HOOKEAN;
     throw std::invalid_argument("unknown material '" + s + "'");
 }

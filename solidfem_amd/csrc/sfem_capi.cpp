@@ -532,7 +532,7 @@ int sfem_create(const sfem_mesh* m, const sfem_material* mat, sfem_ctx** out)
     if (!m || !mat || !out || !m->tets || !m->X0 || !m->fixed || m->ne <= 0 || m->nn <= 0)
         return fail(SFEM_EINVAL, "sfem_create: null or empty mesh");
     if (m->nn >= (1 << 30)) return fail(SFEM_EINVAL, "sfem_create: more than 2^30 nodes");
-    if (mat->material < SFEM_LINEAR || mat->material > SFEM_NEO_HOOKEAN)
+    if ((mat->material < SFEM_LINEAR || mat->material > SFEM_NEO_HOOKEAN) && mat->material != SFEM_COROTATIONAL_POLAR)
         return fail(SFEM_EINVAL, "sfem_create: unknown material");
     for (int64_t i = 0; i < 4 * m->ne; i++)
         if (m->tets[i] < 0 || m->tets[i] >= m->nn) return fail(SFEM_EINVAL, "sfem_create: node index out of range");
@@ -544,7 +544,7 @@ int sfem_create(const sfem_mesh* m, const sfem_material* mat, sfem_ctx** out)
         HIPCK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
         c->ne = m->ne;
         c->nn = m->nn;
-        c->material = mat->material;
+        c->material = mat->material == SFEM_COROTATIONAL_POLAR ? MAT_COROTATIONAL_POLAR : mat->material;
         c->young = mat->young;
         c->poisson = mat->poisson;
         c->density = mat->density;

@@ -17,7 +17,8 @@
 namespace sfem {
 
 enum { LINEAR = 0, STVK = 1, COROTATIONAL = 2, NEO_HOOKEAN = 3,
-       NEO_HOOKEAN_COARSE = 4 };   // internal: coarse multigrid levels (MAT_NEO_HOOKEAN_COARSE)
+       NEO_HOOKEAN_COARSE = 4,     // internal: coarse multigrid levels (MAT_NEO_HOOKEAN_COARSE)
+       COROTATIONAL_POLAR = 5 };   // internal kind of SFEM_COROTATIONAL_POLAR
 
 template <typename T> __device__ __forceinline__ T dsqrt(T x) { return sqrt(x); }
 template <typename T> __device__ __forceinline__ T dlog(T x) { return log(x); }
@@ -107,6 +108,54 @@ __device__ __forceinline__ void qr_rot(const T* F, T* R, T* S)
     r2[0] *= s; r2[1] *= s; r2[2] *= s;
 #pragma unroll
     for (int i = 0; i < 3; i++) { R[3 * i] = r0[i]; R[3 * i + 1] = r1[i]; R[3 * i + 2] = r2[i]; }
+    T V[9];
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++)
+            V[3 * i + j] = R[i] * F[j] + R[3 + i] * F[3 + j] + R[6 + i] * F[6 + j];   // (R^T F)_ij
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) S[3 * i + j] = T(0.5) * (V[3 * i + j] + V[3 * j + i]);
+}
+
+// Rotation of the polar decomposition F = R U (U symmetric, positive definite
+// for det F > 0) by Newton's iteration X <- (g X + X^-T / g) / 2 with the
+// Frobenius-norm scaling g = (|X^-1|_F / |X|_F)^(1/2) (N. J. Higham, SIAM J.
+// Sci. Stat. Comput. 7 (1986) 1160-1174), until the update is below
+// 2^-20 (FP32) / 2^-45 (FP64) of |X|_F: the iteration converges
+// quadratically, so the returned R is then exact to the working precision.
+// S = sym(R^T F).  An inverted F (det < 0) converges to the orthogonal factor
+// with det R = -1.
+template <typename T>
+__device__ __forceinline__ void polar_rot(const T* F, T* R, T* S)
+{
+    constexpr int maxit = sizeof(T) == 4 ? 12 : 20;
+    const T tol2 = sizeof(T) == 4 ? T(3.0 * 0x1p-40) : T(3.0 * 0x1p-90);
+    T X[9];
+#pragma unroll
+    for (int k = 0; k < 9; k++) X[k] = F[k];
+    for (int it = 0; it < maxit; it++) {
+        T Xi[9];
+        inv3(X, Xi);
+        T nx = 0, ni = 0;
+#pragma unroll
+        for (int k = 0; k < 9; k++) { nx += X[k] * X[k]; ni += Xi[k] * Xi[k]; }
+        const T g = dsqrt(dsqrt(ni / nx)), h = T(0.5) / g, g2 = T(0.5) * g;
+        T du = 0;
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++) {
+                const T xn = g2 * X[3 * i + j] + h * Xi[3 * j + i];
+                du += (xn - X[3 * i + j]) * (xn - X[3 * i + j]);
+                X[3 * i + j] = xn;
+            }
+        if (!(du > tol2)) break;
+    }
+#pragma unroll
+    for (int k = 0; k < 9; k++) R[k] = X[k];
     T V[9];
 #pragma unroll
     for (int i = 0; i < 3; i++)
@@ -225,6 +274,63 @@ template <typename T> struct Tangent<T, COROTATIONAL> {
     }
 };
 
+// COROTATIONAL_POLAR (no reference counterpart): the corotational model with
+// the rotation of the polar decomposition, P = 2mu (F - R) + lam tr(S - I) R =
+// dPsi/dF for Psi = mu |F - R|^2 + lam/2 tr(S - I)^2, and its consistent
+// tangent.  With K = R^T dF, the rotation rate dR = R [w]x solves
+// (tr S I - S) w = axial(K - K^T) (F = R S, S symmetric), so
+//   dP = 2mu dF + lam tr(K) R + (lam (trS - 3) - 2mu) dR,
+// the COROTATIONAL formula with the polar R and the axial vector
+// (K21 - K12, K02 - K20, K10 - K01).  An inverted element (det F <= 0) takes
+// the tangent at F = I.
+template <typename T> struct Tangent<T, COROTATIONAL_POLAR> {
+    T R[9], Ai[9], cmu, clam, cc;
+    __device__ __forceinline__ void init(const T* F, T m, T l)
+    {
+        cmu = m; clam = l;
+        const T J = F[0] * (F[4] * F[8] - F[5] * F[7]) - F[1] * (F[3] * F[8] - F[5] * F[6]) +
+                    F[2] * (F[3] * F[7] - F[4] * F[6]);
+        T S[9];
+        if (J > T(0)) {
+            polar_rot(F, R, S);
+        } else {
+#pragma unroll
+            for (int k = 0; k < 9; k++) R[k] = S[k] = (k % 4 == 0) ? T(1) : T(0);
+        }
+        const T trS = S[0] + S[4] + S[8];
+        T A[9];
+#pragma unroll
+        for (int k = 0; k < 9; k++) A[k] = -S[k];
+        A[0] += trS; A[4] += trS; A[8] += trS;
+        inv3(A, Ai);
+        cc = clam * (trS - 3) - 2 * cmu;
+    }
+    __device__ __forceinline__ void apply(const T* dF, T* dP) const
+    {
+        T K[9];
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++)
+                K[3 * i + j] = R[i] * dF[j] + R[3 + i] * dF[3 + j] + R[6 + i] * dF[6 + j];
+        const T tr = K[0] + K[4] + K[8];
+        const T a0 = K[7] - K[5], a1 = K[2] - K[6], a2 = K[3] - K[1];
+        const T w0 = Ai[0] * a0 + Ai[1] * a1 + Ai[2] * a2;
+        const T w1 = Ai[3] * a0 + Ai[4] * a1 + Ai[5] * a2;
+        const T w2 = Ai[6] * a0 + Ai[7] * a1 + Ai[8] * a2;
+        // dR = R [w]x, [w]x = [[0,-w2,w1],[w2,0,-w0],[-w1,w0,0]]
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            const T d0 = R[3 * i + 1] * w2 - R[3 * i + 2] * w1;
+            const T d1 = -R[3 * i] * w2 + R[3 * i + 2] * w0;
+            const T d2 = R[3 * i] * w1 - R[3 * i + 1] * w0;
+            dP[3 * i + 0] = 2 * cmu * dF[3 * i + 0] + clam * tr * R[3 * i + 0] + cc * d0;
+            dP[3 * i + 1] = 2 * cmu * dF[3 * i + 1] + clam * tr * R[3 * i + 1] + cc * d1;
+            dP[3 * i + 2] = 2 * cmu * dF[3 * i + 2] + clam * tr * R[3 * i + 2] + cc * d2;
+        }
+    }
+};
+
 // NEO_HOOKEAN: dP = mu dF + (mu - lam ln J) F^-T dF^T F^-T + lam tr(F^-1 dF) F^-T
 // (:464-470); F^-T dF^T F^-T = (F^-1 dF F^-1)^T
 template <typename T> struct Tangent<T, NEO_HOOKEAN> {
@@ -294,6 +400,7 @@ __device__ __forceinline__ void stress(const T* F, T cmu, T clam, T* P)
         T R[9], S[9];
         const T* U = F;
         if (MAT == COROTATIONAL) { qr_rot(F, R, S); U = S; }
+        if (MAT == COROTATIONAL_POLAR) { polar_rot(F, R, S); U = S; }
         T e[9];
 #pragma unroll
         for (int i = 0; i < 3; i++)
@@ -304,7 +411,7 @@ __device__ __forceinline__ void stress(const T* F, T cmu, T clam, T* P)
 #pragma unroll
         for (int k = 0; k < 9; k++) Q[k] = 2 * cmu * e[k];
         Q[0] += clam * tr; Q[4] += clam * tr; Q[8] += clam * tr;
-        if (MAT == COROTATIONAL) mul3(R, Q, P);
+        if (MAT == COROTATIONAL || MAT == COROTATIONAL_POLAR) mul3(R, Q, P);
         else {
 #pragma unroll
             for (int k = 0; k < 9; k++) P[k] = Q[k];

@@ -868,6 +868,7 @@ static hipError_t launch_elem_mat(int mat, const BlockArgs& a, const void* u, co
     case LINEAR: SFEM_ELEM_CASE(LINEAR); break;
     case STVK: SFEM_ELEM_CASE(STVK); break;
     case COROTATIONAL: SFEM_ELEM_CASE(COROTATIONAL); break;
+    case COROTATIONAL_POLAR: SFEM_ELEM_CASE(COROTATIONAL_POLAR); break;
     case NEO_HOOKEAN_COARSE:
         // coarse multigrid levels: operator and diagonal blocks only
         if constexpr (KIND != 1) {
@@ -956,10 +957,11 @@ hipError_t set_elem_lds_limits(const BlockArgs& a)
     const Fn fns[] = {
 #define SFEM_FN(K, M) {(const void*)k_elem_block<K, M, BLOCK_E, false>, K}, {(const void*)k_elem_block<K, M, BLOCK_E, true>, K}
         SFEM_FN(0, LINEAR), SFEM_FN(0, STVK), SFEM_FN(0, COROTATIONAL), SFEM_FN(0, NEO_HOOKEAN),
-        SFEM_FN(0, NEO_HOOKEAN_COARSE),
+        SFEM_FN(0, NEO_HOOKEAN_COARSE), SFEM_FN(0, COROTATIONAL_POLAR),
         SFEM_FN(1, LINEAR), SFEM_FN(1, STVK), SFEM_FN(1, COROTATIONAL), SFEM_FN(1, NEO_HOOKEAN),
+        SFEM_FN(1, COROTATIONAL_POLAR),
         SFEM_FN(2, LINEAR), SFEM_FN(2, STVK), SFEM_FN(2, COROTATIONAL), SFEM_FN(2, NEO_HOOKEAN),
-        SFEM_FN(2, NEO_HOOKEAN_COARSE),
+        SFEM_FN(2, NEO_HOOKEAN_COARSE), SFEM_FN(2, COROTATIONAL_POLAR),
 #undef SFEM_FN
     };
     const size_t need[] = {lds_bytes<0>(a), lds_bytes<1>(a), lds_bytes<2>(a)};
@@ -1069,6 +1071,9 @@ hipError_t launch_explicit(int64_t nn, double* u, double* vel, const double* f, 
     return hipGetLastError();
 }
 
+static_assert(MAT_NEO_HOOKEAN_COARSE == NEO_HOOKEAN_COARSE && MAT_COROTATIONAL_POLAR == COROTATIONAL_POLAR,
+              "host and device material ids");
+
 hipError_t launch_elem_stress(int mat, const BlockArgs& a, const double* u, const int* eorig, double* out, hipStream_t st)
 {
     const dim3 grid((unsigned)((a.ne + 255) / 256));
@@ -1076,6 +1081,9 @@ hipError_t launch_elem_stress(int mat, const BlockArgs& a, const double* u, cons
     case LINEAR: hipLaunchKernelGGL(k_elem_stress<LINEAR>, grid, dim3(256), 0, st, a, u, eorig, out); break;
     case STVK: hipLaunchKernelGGL(k_elem_stress<STVK>, grid, dim3(256), 0, st, a, u, eorig, out); break;
     case COROTATIONAL: hipLaunchKernelGGL(k_elem_stress<COROTATIONAL>, grid, dim3(256), 0, st, a, u, eorig, out); break;
+    case COROTATIONAL_POLAR:
+        hipLaunchKernelGGL(k_elem_stress<COROTATIONAL_POLAR>, grid, dim3(256), 0, st, a, u, eorig, out);
+        break;
     default: hipLaunchKernelGGL(k_elem_stress<NEO_HOOKEAN>, grid, dim3(256), 0, st, a, u, eorig, out); break;
     }
     return hipGetLastError();

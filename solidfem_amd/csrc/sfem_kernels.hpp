@@ -27,8 +27,10 @@ enum Scal { S_RZ = 0, S_RZ0, S_RR, S_RR0, S_PQ, S_ALPHA, S_BETA, S_ITER, S_NT, S
 
 // Element-block partition, all device pointers.
 // the material id of coarse multigrid levels of a Neo-Hookean context (the
-// safeguarded tangent, sfem_device.hpp); the C-ABI ids are 0 .. 3
+// safeguarded tangent, sfem_device.hpp); the C-ABI ids 0 .. 3 are the kernels'
+// ids, SFEM_COROTATIONAL_POLAR (16) is MAT_COROTATIONAL_POLAR
 constexpr int MAT_NEO_HOOKEAN_COARSE = 4;
+constexpr int MAT_COROTATIONAL_POLAR = 5;
 
 struct BlockArgs {
     int64_t ne;            // elements

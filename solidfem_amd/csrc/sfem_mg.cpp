@@ -382,7 +382,7 @@ int sfem_mg_add_level(sfem_ctx* f, const sfem_mesh* coarse, const int32_t* injec
     for (int64_t k = 0; k < nnz; k++)
         if (p_col[k] < 0 || p_col[k] >= nc) return fail(SFEM_EINVAL, "sfem_mg_add_level: prolongation column out of range");
     sfem_material mat{};
-    mat.material = f->material;
+    mat.material = f->material == MAT_COROTATIONAL_POLAR ? SFEM_COROTATIONAL_POLAR : f->material;
     mat.young = f->young;
     mat.poisson = f->poisson;
     mat.density = f->density;

@@ -492,6 +492,9 @@ hipError_t launch_mg_elem_matrix(int mat, int64_t ne, const int4* tets, const fl
     case LINEAR: hipLaunchKernelGGL(k_mg_elem_matrix<LINEAR>, g, bl, 0, st, ne, tets, u, dminv, cvol, Ke); break;
     case STVK: hipLaunchKernelGGL(k_mg_elem_matrix<STVK>, g, bl, 0, st, ne, tets, u, dminv, cvol, Ke); break;
     case COROTATIONAL: hipLaunchKernelGGL(k_mg_elem_matrix<COROTATIONAL>, g, bl, 0, st, ne, tets, u, dminv, cvol, Ke); break;
+    case COROTATIONAL_POLAR:
+        hipLaunchKernelGGL(k_mg_elem_matrix<COROTATIONAL_POLAR>, g, bl, 0, st, ne, tets, u, dminv, cvol, Ke);
+        break;
     case NEO_HOOKEAN_COARSE:
         hipLaunchKernelGGL(k_mg_elem_matrix<NEO_HOOKEAN_COARSE>, g, bl, 0, st, ne, tets, u, dminv, cvol, Ke);
         break;

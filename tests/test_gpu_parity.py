@@ -13,6 +13,7 @@ from solidfem_amd.mesh import BeamMesh, lame, uniform01
 
 pytestmark = pytest.mark.gpu
 MATS = [capi.LINEAR, capi.STVK, capi.COROTATIONAL, capi.NEO_HOOKEAN]
+MATS_ALL = MATS + [capi.COROTATIONAL_POLAR]     # + the polar corotational model (no reference counterpart)
 
 
 def relmax(a, b):
@@ -42,7 +43,7 @@ def setup(mat, grid=(10, 3, 3), L=(1.0, 0.25, 0.25), per_elem=False, jitter=0.0,
     return m, fixed, p, s
 
 
-@pytest.mark.parametrize("mat", MATS)
+@pytest.mark.parametrize("mat", MATS_ALL)
 @pytest.mark.parametrize("grid,per_elem", [((10, 3, 3), False), ((24, 6, 8), True)])
 def test_forces_and_hessvec(mat, grid, per_elem):
     print(f"case mat={mat} grid={grid} per_elem={per_elem}", flush=True)
@@ -99,7 +100,7 @@ def test_reference_mode_solve_cfg1(mat):
     assert err < (5e-2 if mat == capi.COROTATIONAL else 1e-4)
 
 
-@pytest.mark.parametrize("mat", [capi.LINEAR, capi.STVK, capi.NEO_HOOKEAN])
+@pytest.mark.parametrize("mat", [capi.LINEAR, capi.STVK, capi.NEO_HOOKEAN, capi.COROTATIONAL_POLAR])
 @pytest.mark.parametrize("precond", [capi.PC_NONE, capi.PC_JACOBI])
 def test_converged_solve_cfg1(mat, precond):
     m, fixed, p, s = setup(mat)
@@ -126,3 +127,42 @@ def test_quasi_static_nh_per_element_material():
     err = rel2(s.positions() - m.X, xo - m.X)
     print(f"quasi-static err {err:.3e}")
     assert err < 1e-6
+
+
+@pytest.mark.parametrize("precond", [capi.PC_JACOBI, capi.PC_MULTIGRID])
+def test_polar_corotational_torsion_converges(precond):
+    """Workload-3 physics at 32x8x8 (per-element E, nu; tip torsion, 4 load
+    steps) with the polar corotational model: Newton-PCG reaches 1e-8 with no
+    fallback and matches the FP64 oracle's Newton solve; the reference's QR
+    model on the same problem does not converge (its tangent is not the
+    derivative of its force)"""
+    from solidfem_amd import configs
+    from solidfem_amd.mesh import beam_hierarchy
+    print(f"case workload 3 at 32x8x8 polar precond={precond}", flush=True)
+    d = configs.build(3, (32, 8, 8), capi.COROTATIONAL_POLAR)
+    m, fixed = d["mesh"], d["fixed"]
+    s = configs.solver(d)
+    if precond == capi.PC_MULTIGRID:
+        s.set_multigrid(beam_hierarchy(m, young_e=d["young_e"], poisson_e=d["poisson_e"]))
+    mu, lam = lame(d["young_e"], d["poisson_e"])
+    p = O.OracleProblem(m.X, m.tets, fixed, O.COROTATIONAL_POLAR, mu=mu, lam=lam, density=1070.0,
+                        gravity=(0.0, 0.0, 0.0), fext=d["fdead"])
+    xo = m.X.copy()
+    nsteps = 4
+    for k in range(nsteps):
+        st = s.step(tol=1e-8, precond=precond, sim_type=capi.ST_QUASI_STATIC, force_step=1.0 / nsteps)
+        print(f"  step {k}: newton {st['newton_iters']} cg {st['cg_iters']} fallbacks {st['fallbacks']}", flush=True)
+        assert st["converged"] == 1 and st["fallbacks"] == 0
+        xo, _ = p.newton_solve(xo, (k + 1) / nsteps, tol=1e-10, cg_rtol=1e-8)
+    x = s.positions()
+    err = rel2(x - m.X, xo - m.X)
+    res = np.linalg.norm(p.gradients(x, 1.0)) / np.linalg.norm(p.external(x, 1.0))
+    twist = np.abs(x - m.X).max()
+    print(f"err {err:.3e} oracle-res {res:.3e} max displacement {twist:.3f} m", flush=True)
+    assert res < 1e-8 and err < 1e-6
+    # the reference's corotational model on the first load step
+    dq = configs.build(3, (32, 8, 8))
+    sq = configs.solver(dq)
+    stq = sq.step(capi.solve_params(tol=1e-8, precond=capi.PC_JACOBI, outer=30, sim_type=capi.ST_QUASI_STATIC,
+                                    force_step=1.0 / nsteps), allow_notconv=True)
+    print(f"QR model: converged {stq['converged']} rel residual {stq['res'] / stq['ext_norm']:.2e}", flush=True)

@@ -13,6 +13,8 @@ from solidfem_amd.mesh import BeamMesh, load_tet1, traction_nodal_forces, torsio
 
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 MATS = [O.LINEAR, O.STVK, O.COROTATIONAL, O.NEO_HOOKEAN]
+# + the corotational model with the polar rotation (no reference counterpart)
+MATS_ALL = MATS + [O.COROTATIONAL_POLAR]
 
 
 def rel(a, b):
@@ -195,13 +197,15 @@ def _rand_F(rng, scale=0.2):
     return np.eye(3) + scale * rng.standard_normal((3, 3))
 
 
-@pytest.mark.parametrize("mat", MATS)
+@pytest.mark.parametrize("mat", MATS_ALL)
 def test_stress_is_energy_derivative(oracle_lib, mat):
     """P = dPsi/dF by central differences (for COROT: the QR-based energy's
     derivative is not the reference's P, so only at the rest state)."""
     rng = np.random.default_rng(mat)
     mu, lam = 3.0e5, 7.0e5
     F = _rand_F(rng) if mat != O.COROTATIONAL else np.eye(3)
+    if mat == O.COROTATIONAL_POLAR:
+        F = _rot(rng.standard_normal(3)) @ _rand_F(rng)     # a large rotation as well
     P = O.stress(mat, mu, lam, F)
     h = 1e-6
     G = np.zeros((3, 3))
@@ -213,11 +217,13 @@ def test_stress_is_energy_derivative(oracle_lib, mat):
     assert np.abs(P - G).max() < 1e-6 * max(np.abs(P).max(), mu), (P, G)
 
 
-@pytest.mark.parametrize("mat", [O.LINEAR, O.STVK, O.NEO_HOOKEAN])
+@pytest.mark.parametrize("mat", [O.LINEAR, O.STVK, O.NEO_HOOKEAN, O.COROTATIONAL_POLAR])
 def test_dstress_is_stress_derivative(oracle_lib, mat):
     rng = np.random.default_rng(10 + mat)
     mu, lam = 3.0e5, 7.0e5
     F = _rand_F(rng)
+    if mat == O.COROTATIONAL_POLAR:
+        F = _rot(rng.standard_normal(3)) @ _rand_F(rng, 0.3)
     dF = rng.standard_normal((3, 3))
     h = 1e-6
     fd = (O.stress(mat, mu, lam, F + h * dF) - O.stress(mat, mu, lam, F - h * dF)) / (2 * h)
@@ -244,7 +250,7 @@ def test_material_special_cases(oracle_lib):
     assert np.abs(O.stress(O.LINEAR, mu, lam, R)).max() > 0.1
 
 
-@pytest.mark.parametrize("mat", MATS)
+@pytest.mark.parametrize("mat", MATS_ALL)
 def test_force_is_negative_energy_gradient(oracle_lib, mat):
     m = BeamMesh(3, 1, 2, 0.3, 0.1, 0.2)
     p = O.OracleProblem(m.X, m.tets, np.zeros(m.nn, np.uint8), mat, 1e5, 0.35)
@@ -263,7 +269,7 @@ def test_force_is_negative_energy_gradient(oracle_lib, mat):
         assert abs(-g - f.reshape(-1)[k]) < 1e-5 * np.abs(f).max()
 
 
-@pytest.mark.parametrize("mat", MATS)
+@pytest.mark.parametrize("mat", MATS_ALL)
 def test_matvec_invariants(oracle_lib, mat):
     m = BeamMesh(4, 2, 2, 0.4, 0.2, 0.2)
     fixed = np.zeros(m.nn, np.uint8)
@@ -282,6 +288,68 @@ def test_matvec_invariants(oracle_lib, mat):
         h = 1e-7
         fd = -(p.internal_forces(x + h * a) - p.internal_forces(x - h * a)) / (2 * h)
         assert rel(p.matvec(x, a), fd) < 1e-5
+
+
+def _rot(v):
+    """rotation by |v| about v (Rodrigues)"""
+    th = np.linalg.norm(v)
+    k = v / th
+    K = np.array([[0, -k[2], k[1]], [k[2], 0, -k[0]], [-k[1], k[0], 0]])
+    return np.eye(3) + np.sin(th) * K + (1 - np.cos(th)) * K @ K
+
+
+def test_polar_rotation_is_the_polar_factor(oracle_lib):
+    """With mu = 1/2, lam = 0 the polar corotational stress is F - R: R
+    against scipy's SVD-based polar decomposition, for stretches up to 2x
+    under rotations up to pi"""
+    from scipy.linalg import polar
+    rng = np.random.default_rng(41)
+    for k in range(20):
+        F = _rot(rng.standard_normal(3) * (0.5 + k / 8)) @ _rand_F(rng, 0.3)
+        assert np.linalg.det(F) > 0
+        R = F - O.stress(O.COROTATIONAL_POLAR, 0.5, 0.0, F)
+        Rs, Us = polar(F)
+        assert np.abs(R - Rs).max() < 1e-13, (k, R, Rs)
+
+
+def test_polar_corotational_special_cases(oracle_lib):
+    """At F = I the tangent is the linear one for every dF (the reference's
+    QR variant only for symmetric dF); rotations are stress-free; the stress is
+    frame-indifferent, P(QF) = Q P(F); an inverted element takes the tangent
+    at F = I"""
+    rng = np.random.default_rng(5)
+    mu, lam = 2.0, 5.0
+    dF = rng.standard_normal((3, 3))
+    lin = mu * (dF + dF.T) + lam * np.trace(dF) * np.eye(3)
+    assert np.abs(O.dstress(O.COROTATIONAL_POLAR, mu, lam, np.eye(3), dF) - lin).max() < 1e-12
+    Q = _rot(np.array([0.3, -1.9, 0.8]))
+    assert np.abs(O.stress(O.COROTATIONAL_POLAR, mu, lam, Q)).max() < 1e-12
+    F = _rand_F(rng, 0.3)
+    assert np.abs(O.stress(O.COROTATIONAL_POLAR, mu, lam, Q @ F) - Q @ O.stress(O.COROTATIONAL_POLAR, mu, lam, F)).max() < 1e-12
+    Finv = np.diag([1.1, 0.9, -0.2])
+    assert np.abs(O.dstress(O.COROTATIONAL_POLAR, mu, lam, Finv, dF) - lin).max() < 1e-12
+    # the reference's QR variant differs from the polar one under shear
+    Fs = np.array([[1.0, 0.4, 0.0], [0.0, 1.0, 0.0], [0.0, 0.0, 1.0]])
+    assert np.abs(O.stress(O.COROTATIONAL, mu, lam, Fs) - O.stress(O.COROTATIONAL_POLAR, mu, lam, Fs)).max() > 1e-2
+
+
+def test_polar_corotational_tangent_at_large_rotation(oracle_lib):
+    """a beam rotated by ~100 degrees and perturbed: K symmetric and equal to
+    -df/dx by central differences (the reference's QR tangent fails both)"""
+    m = BeamMesh(4, 2, 2, 0.4, 0.2, 0.2)
+    rng = np.random.default_rng(8)
+    x = m.X @ _rot(np.array([1.2, 0.9, -0.6])).T + 0.01 * rng.standard_normal(m.X.shape)
+    a, b = rng.standard_normal(m.X.shape), rng.standard_normal(m.X.shape)
+    h = 1e-7
+    for mat in (O.COROTATIONAL_POLAR, O.COROTATIONAL):
+        p = O.OracleProblem(m.X, m.tets, np.zeros(m.nn, np.uint8), mat, 1e5, 0.35)
+        Ka = p.matvec(x, a)
+        sym = abs(np.sum(b * Ka) - np.sum(a * p.matvec(x, b))) / abs(np.sum(b * Ka))
+        fd = -(p.internal_forces(x + h * a) - p.internal_forces(x - h * a)) / (2 * h)
+        if mat == O.COROTATIONAL_POLAR:
+            assert sym < 1e-10 and rel(Ka, fd) < 1e-5
+        else:
+            assert sym > 1e-6 and rel(Ka, fd) > 1e-3
 
 
 def test_geometry_and_mass(oracle_lib):
