@@ -4,6 +4,9 @@ timings and FP64 residuals.
 
     python -m solidfem_amd.configs 1 [jacobi|multigrid|multigrid_learned|learned|none] [nopredictor] [polar]
 
+(wall times are measured after the same solve on a 16x4x4 grid, which loads
+every kernel's code object first)
+
 All beams span [0, 1] x [0, 0.25] x [0, 0.25] m with the x = 0 face clamped
 and 6 tets per hex (the Models/box2.tet pattern, see mesh.py).
 
@@ -132,6 +135,10 @@ if __name__ == "__main__":
     pc = sys.argv[2] if len(sys.argv) > 2 else "jacobi"
     pred = 0 if "nopredictor" in sys.argv[3:] else 1
     mat = capi.COROTATIONAL_POLAR if "polar" in sys.argv[3:] else None
+    if cfg != 1:
+        # the same solve on a 16x4x4 grid first: the first launch of each
+        # kernel loads its code object, which would be timed with load step 1
+        run(cfg, PRECONDS[pc], grid=(16, 4, 4), predictor=pred, material=mat)
     out, d, s = run(cfg, PRECONDS[pc], log=lambda line: print(line, file=sys.stderr, flush=True), predictor=pred,
                     material=mat)
     out["precond"] = pc

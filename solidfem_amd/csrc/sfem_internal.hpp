@@ -167,6 +167,11 @@ struct sfem_ctx {
     bool mg_ready = false;     // eigenvalue estimates and coarsest inverse exist
     int64_t mg_evals = 0;      // element evaluations of one V-cycle
     DevBuf d_mg_r, d_mg_d, d_mg_q, d_mg_part, d_mg_norm;
+    // the coarsest level's inverse is computed on a second stream (default
+    // priority) beside the other levels' set-up: [0] coarse state ready,
+    // [1] inverse ready
+    hipStream_t mg_side = nullptr;
+    hipEvent_t mg_ev[2] = {};
 
     // sampled timing of the CG loops' Hessian-vector launches (sfem_time_hessvec)
     static constexpr int TM_POOL = 64;
@@ -180,7 +185,11 @@ struct sfem_ctx {
     {
         for (hipEvent_t e : tm_ev)
             if (e) (void)hipEventDestroy(e);
+        if (mg_side) (void)hipStreamSynchronize(mg_side);
         mg.clear();
+        for (hipEvent_t e : mg_ev)
+            if (e) (void)hipEventDestroy(e);
+        if (mg_side) (void)hipStreamDestroy(mg_side);
         if (stream && owns_stream) (void)hipStreamDestroy(stream);
     }
 };

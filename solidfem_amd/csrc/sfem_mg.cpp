@@ -179,6 +179,7 @@ bool mg_prepare(sfem_ctx* f, const float* minv0, bool full)
     // coarse states and block preconditioners, finest to coarsest
     const double* ua = f->d_u64.as<double>();
     const bool dense = nlev > 0 && f->mg[nlev - 1]->dense;
+    bool side = false;
     for (int l = 0; l < nlev; l++) {
         MgLevel& m = *f->mg[l];
         sfem_ctx* c = m.c;
@@ -186,13 +187,24 @@ bool mg_prepare(sfem_ctx* f, const float* minv0, bool full)
         ua = c->d_u64.as<double>();
         if (l == nlev - 1 && dense) {
             if (!full) continue;
-            // the coarsest operator, assembled and inverted
+            // the coarsest operator, assembled and inverted on the side stream
+            // once its state is injected; the main stream goes on with the
+            // other levels' estimates and waits for it before the first V-cycle
+            if (!f->mg_side) {
+                HIPCK(hipStreamCreateWithFlags(&f->mg_side, hipStreamNonBlocking));
+                for (hipEvent_t& e : f->mg_ev) HIPCK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+            }
+            hipStream_t ss = f->mg_side;
+            HIPCK(hipEventRecord(f->mg_ev[0], s));
+            HIPCK(hipStreamWaitEvent(ss, f->mg_ev[0], 0));
             const int n = 3 * (int)c->nn;
             HIPCK(launch_mg_elem_matrix(c->material, c->ne, m.d_tets4.as<int4>(), c->d_u32.as<float>(), m.d_dminv.as<float>(),
-                                        m.d_cvol.as<float>(), m.d_Ke.as<float>(), s));
+                                        m.d_cvol.as<float>(), m.d_Ke.as<float>(), ss));
             HIPCK(launch_mg_dense_assemble((int)c->nn, m.d_inc_ptr.as<int>(), m.d_inc.as<int2>(), m.d_tets4.as<int4>(),
-                                           m.d_Ke.as<float>(), c->d_fixed.as<uint8_t>(), m.d_A.as<float>(), s));
-            HIPCK(launch_mg_sweep_inverse(n, m.d_A.as<float>(), m.d_Ainv.as<float>(), s));
+                                           m.d_Ke.as<float>(), c->d_fixed.as<uint8_t>(), m.d_A.as<float>(), ss));
+            HIPCK(launch_mg_sweep_inverse(n, m.d_A.as<float>(), m.d_Ainv.as<float>(), ss));
+            HIPCK(hipEventRecord(f->mg_ev[1], ss));
+            side = true;
         } else {
             build_jacobi(c);
         }
@@ -218,6 +230,7 @@ bool mg_prepare(sfem_ctx* f, const float* minv0, bool full)
         HIPCK(launch_mg_sqnorm(3 * c->nn, v, f->d_mg_part.as<double>(), norms + 2 * l, s));
         HIPCK(launch_mg_sqnorm(3 * c->nn, y, f->d_mg_part.as<double>(), norms + 2 * l + 1, s));
     }
+    if (side) HIPCK(hipStreamWaitEvent(s, f->mg_ev[1], 0));
     std::vector<double> h(2 * nest);
     HIPCK(hipMemcpyAsync(h.data(), norms, h.size() * sizeof(double), hipMemcpyDeviceToHost, s));
     HIPCK(hipStreamSynchronize(s));
@@ -354,7 +367,6 @@ void dense_setup(MgLevel& m, const sfem_ctx* f, const sfem_mesh* mesh, const dou
     const size_t n = 3 * (size_t)nn;
     m.d_A.alloc(n * n * sizeof(float));
     m.d_Ainv.alloc(n * n * sizeof(float));
-    HIPCK(mg_sweep_init());
     m.dense = true;
 }
 
@@ -440,6 +452,7 @@ int sfem_mg_clear(sfem_ctx* f)
     if (!f) return fail(SFEM_EINVAL, "sfem_mg_clear: null context");
     return guard_call("sfem_mg_clear", [&]() -> int {
         HIPCK(hipStreamSynchronize(f->stream));
+        if (f->mg_side) HIPCK(hipStreamSynchronize(f->mg_side));
         f->mg.clear();
         f->mg_ready = false;
         return SFEM_OK;

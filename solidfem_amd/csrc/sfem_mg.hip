@@ -335,66 +335,111 @@ __global__ __launch_bounds__(MG_BLOCK) void k_mg_dense_assemble(int nn, const in
 // Ainv = A^-1 for a symmetric positive definite A by the sweep operator
 // (Goodnight, The American Statistician 33 (1979) 149-158): sweeping every
 // pivot k in turn (a_kk <- -1/d, a_ik <- a_ik/d, a_ij <- a_ij - a_ik a_kj/d,
-// d = a_kk) turns A into -A^-1.  One workgroup; the packed upper triangle
-// (symmetrised from A) lives in LDS, every thread owning up to MG_SWEEP_PER
-// entries (decoded once); column k is double-buffered in LDS: the owners of
-// column k+1's entries publish them while sweeping k, so each pivot costs one
-// barrier.
+// d = a_kk) turns A into -A^-1.  One workgroup.  The upper triangle
+// (symmetrised from A) is cut into chunks of 4 consecutive rows of one column
+// (column j has ceil((j+1)/4) chunks) and lives in registers: every thread
+// owns up to MG_SWEEP_CHUNKS chunks.  Only column k passes through LDS,
+// double-buffered: the owners of column k+1's entries publish them while
+// sweeping k, so each pivot costs one barrier and one float4 LDS read per
+// chunk.
 constexpr int MG_SWEEP_BLOCK = 1024;
-constexpr int MG_SWEEP_PER = 32;   // ceil(n (n + 1) / 2 / 1024) for n <= MG_DENSE_MAX
+constexpr int MG_SWEEP_CHUNKS = 9;
+constexpr int sweep_chunks(int n)
+{
+    int c = 0;
+    for (int m = 1; m <= n; m++) c += (m + 3) / 4;
+    return c;
+}
+static_assert(sweep_chunks(MG_DENSE_MAX) <= MG_SWEEP_CHUNKS * MG_SWEEP_BLOCK, "sweep chunks per thread");
 
 __global__ __launch_bounds__(MG_SWEEP_BLOCK) void k_mg_sweep_inverse(int n, const float* __restrict__ A,
                                                                      float* __restrict__ Ainv)
 {
-    extern __shared__ float P[];   // (i, j), i <= j, at j (j + 1) / 2 + i
-    __shared__ float col[2][MG_DENSE_MAX];
-    const int np = n * (n + 1) / 2;
+    __shared__ int cs[MG_DENSE_MAX + 1];                     // first chunk of column j
+    __shared__ __align__(16) float col[2][MG_DENSE_MAX + 5];  // + room for a float4 past row n-1
     const int tid = threadIdx.x;
-    uint32_t ij[MG_SWEEP_PER];     // i | j << 16; j = 0xffff: no entry
+    if (tid <= n) {
+        int c0 = 0;
+        for (int m = 0; m < tid; m++) c0 += (m + 4) >> 2;
+        cs[tid] = c0;
+    }
+    if (tid < 2 * (MG_DENSE_MAX + 5)) (&col[0][0])[tid] = 0.f;
+    __syncthreads();
+    const int nch = cs[n];
+    float v[MG_SWEEP_CHUNKS][4];
+    int jj[MG_SWEEP_CHUNKS], ii[MG_SWEEP_CHUNKS];   // column and first row; jj = -1: no chunk
 #pragma unroll
-    for (int q = 0; q < MG_SWEEP_PER; q++) {
-        const int e = tid + q * MG_SWEEP_BLOCK;
-        int i = 0, j = 0xffff;
-        if (e < np) {
-            j = (int)((sqrtf(8.f * (float)e + 1.f) - 1.f) * 0.5f);
-            while (j * (j + 1) / 2 > e) j--;
-            while ((j + 1) * (j + 2) / 2 <= e) j++;
-            i = e - j * (j + 1) / 2;
-            const float a = 0.5f * (A[(int64_t)i * n + j] + A[(int64_t)j * n + i]);
-            P[e] = a;
-            if (i == 0) col[0][j] = a;   // column 0 = row 0 of the upper triangle
+    for (int q = 0; q < MG_SWEEP_CHUNKS; q++) {
+        const int g = tid + q * MG_SWEEP_BLOCK;
+        jj[q] = -1;
+        ii[q] = 0;
+#pragma unroll
+        for (int r = 0; r < 4; r++) v[q][r] = 0.f;
+        if (g < nch) {
+            int lo = 0, hi = n - 1;   // the last column j with cs[j] <= g
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (cs[mid] <= g) lo = mid;
+                else hi = mid - 1;
+            }
+            const int j = lo, i0 = 4 * (g - cs[j]);
+            jj[q] = j;
+            ii[q] = i0;
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int i = i0 + r;
+                if (i <= j) v[q][r] = 0.5f * (A[(int64_t)i * n + j] + A[(int64_t)j * n + i]);
+            }
+            if (i0 == 0) col[0][j] = v[q][0];   // column 0 = row 0 of the upper triangle
         }
-        ij[q] = (uint32_t)i | ((uint32_t)j << 16);
     }
     __syncthreads();
     for (int k = 0; k < n; k++) {
         const float* c = col[k & 1];
         float* cn = col[(k + 1) & 1];
         const float rd = 1.f / c[k];
-#pragma unroll 8
-        for (int q = 0; q < MG_SWEEP_PER; q++) {
-            const int i = (int)(ij[q] & 0xffffu), j = (int)(ij[q] >> 16);
-            if (j == 0xffff) break;
-            const int e = tid + q * MG_SWEEP_BLOCK;
-            const float ci = c[i], cj = c[j];
-            const float t = ci * rd;
-            float nv = fmaf(-t, cj, P[e]);
-            const bool ik = i == k, jk = j == k;
-            if (ik || jk) nv = (ik && jk) ? -rd : (ik ? cj * rd : t);
-            P[e] = nv;
-            // publish column k + 1: entries (i, k+1) and (k+1, j)
-            if (j == k + 1) cn[i] = nv;
-            else if (i == k + 1) cn[j] = nv;
+#pragma unroll
+        for (int q = 0; q < MG_SWEEP_CHUNKS; q++) {
+            const int j = jj[q], i0 = ii[q];
+            if (j < 0) continue;
+            const float4 c4 = *reinterpret_cast<const float4*>(&c[i0]);
+            const float ci[4] = {c4.x, c4.y, c4.z, c4.w};
+            if (j == k) {
+                // column k: a_ik / d, and -1/d on the diagonal
+#pragma unroll
+                for (int r = 0; r < 4; r++) v[q][r] = i0 + r == k ? -rd : ci[r] * rd;
+            } else {
+                const float cj = c[j];
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const float t = ci[r] * rd;
+                    const float nv = fmaf(-t, cj, v[q][r]);
+                    v[q][r] = i0 + r == k ? cj * rd : nv;   // row k: a_kj / d
+                }
+            }
+            // publish column k + 1: entries (i, k+1), i <= k+1, and (k+1, j), j > k+1
+            if (j == k + 1) {
+#pragma unroll
+                for (int r = 0; r < 4; r++)
+                    if (i0 + r <= j) cn[i0 + r] = v[q][r];
+            } else if (j > k + 1 && (unsigned)(k + 1 - i0) < 4u) {
+                const int r = k + 1 - i0;
+                cn[j] = r == 0 ? v[q][0] : r == 1 ? v[q][1] : r == 2 ? v[q][2] : v[q][3];
+            }
         }
         __syncthreads();
     }
 #pragma unroll
-    for (int q = 0; q < MG_SWEEP_PER; q++) {
-        const int i = (int)(ij[q] & 0xffffu), j = (int)(ij[q] >> 16);
-        if (j == 0xffff) continue;
-        const float v = -P[tid + q * MG_SWEEP_BLOCK];
-        Ainv[(int64_t)i * n + j] = v;
-        Ainv[(int64_t)j * n + i] = v;
+    for (int q = 0; q < MG_SWEEP_CHUNKS; q++) {
+        const int j = jj[q], i0 = ii[q];
+        if (j < 0) continue;
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int i = i0 + r;
+            if (i > j) continue;
+            Ainv[(int64_t)i * n + j] = -v[q][r];
+            Ainv[(int64_t)j * n + i] = -v[q][r];
+        }
     }
 }
 
@@ -511,18 +556,10 @@ hipError_t launch_mg_dense_assemble(int nn, const int* inc_ptr, const int2* inc,
     return hipGetLastError();
 }
 
-static size_t sweep_lds(int n) { return (size_t)n * (n + 1) / 2 * sizeof(float); }
-
-hipError_t mg_sweep_init()
-{
-    return hipFuncSetAttribute(reinterpret_cast<const void*>(&k_mg_sweep_inverse),
-                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)sweep_lds(MG_DENSE_MAX));
-}
-
 hipError_t launch_mg_sweep_inverse(int n, const float* A, float* Ainv, hipStream_t st)
 {
     if (n > MG_DENSE_MAX) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_mg_sweep_inverse, dim3(1), dim3(MG_SWEEP_BLOCK), sweep_lds(n), st, n, A, Ainv);
+    hipLaunchKernelGGL(k_mg_sweep_inverse, dim3(1), dim3(MG_SWEEP_BLOCK), 0, st, n, A, Ainv);
     return hipGetLastError();
 }
 

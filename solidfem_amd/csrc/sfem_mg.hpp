@@ -33,7 +33,6 @@ hipError_t launch_mg_elem_matrix(int mat, int64_t ne, const int4* tets, const fl
                                  const float* cvol, float* Ke, hipStream_t st);
 hipError_t launch_mg_dense_assemble(int nn, const int* inc_ptr, const int2* inc, const int4* tets, const float* Ke,
                                     const uint8_t* fixed, float* A, hipStream_t st);
-hipError_t mg_sweep_init();   // opts the inverse kernel in to its LDS size (per device)
 hipError_t launch_mg_sweep_inverse(int n, const float* A, float* Ainv, hipStream_t st);
 hipError_t launch_mg_dense_solve(int n, const float* Ainv, const float* b, float* x, hipStream_t st);
 

@@ -76,18 +76,22 @@ def test_config3_corotational_reference_semantics():
 
 
 @pytest.mark.skipif(not FULL, reason="full-size workloads run with SFEM_FULL=1")
-@pytest.mark.parametrize("cfg", [1, 2, 3, 4, 5])
+@pytest.mark.parametrize("cfg", [1, 2, 3, "3p", 4, 5])
 def test_full_size_config(cfg):
     """Full grids.  The oracle checks the FP64 residual of the device solution;
-    for 1 it solves the problem itself, for 2 and 4 it continues Newton in FP64
-    from the device solution to 1e-10 (same equilibrium branch) and the
+    for 1 it solves the problem itself, for 2, 3p and 4 it continues Newton in
+    FP64 from the device solution to 1e-10 (same equilibrium branch) and the
     displacement error is measured against that.  3 uses the reference's
-    corotational tangent, with which Newton stalls (no 1e-8 claim); 5 is
-    checked by residual only (an FP64 CPU solve at nu = 0.495 takes hours)."""
+    corotational tangent, with which Newton stalls (no 1e-8 claim); 3p is
+    workload 3 with the polar corotational model; 5 is checked by residual only
+    (an FP64 CPU solve at nu = 0.495 takes hours)."""
     pc = configs.PRECONDS[FULL_PRECOND]
-    if pc in (capi.PC_MULTIGRID, capi.PC_MULTIGRID_LEARNED) and cfg in (1, 3):
-        pytest.skip("no multigrid hierarchy for this workload (too coarse / corotational)")
-    out, d, s = configs.run(cfg, pc, max_newton=60 if cfg != 3 else 12, log=lambda m: print(m, flush=True))
+    if pc in (capi.PC_MULTIGRID, capi.PC_MULTIGRID_LEARNED) and cfg == 1:
+        pytest.skip("no multigrid hierarchy for this workload (3 cells across)")
+    mat = capi.COROTATIONAL_POLAR if cfg == "3p" else None
+    cfg = 3 if cfg == "3p" else cfg
+    stall = cfg == 3 and mat is None
+    out, d, s = configs.run(cfg, pc, max_newton=12 if stall else 60, log=lambda m: print(m, flush=True), material=mat)
     out["precond"] = FULL_PRECOND
     x = s.positions()
     p = oracle_problem(d, nthreads=16)
@@ -96,7 +100,7 @@ def test_full_size_config(cfg):
     if cfg == 1:
         xo, st, _ = oracle_solution(d)
         out["err_vs_fp64_oracle"] = float(rel2(x - X, xo - X))
-    elif cfg in (2, 4):
+    elif cfg in (2, 4) or (cfg == 3 and not stall):
         # one FP64 Newton step per call, so that each step reports as it ends
         xo, tot = x, {"newton": 0, "cg": 0, "converged": 0}
         for k in range(10):
@@ -111,5 +115,5 @@ def test_full_size_config(cfg):
         out["oracle_polish"] = tot
     out.pop("steps")
     print("FULLCONFIG " + json.dumps(out), flush=True)
-    if cfg != 3:
+    if not stall:
         assert out["converged"] and out["oracle_rel_residual"] < 1e-8
