@@ -239,29 +239,35 @@ struct sfem_gnn {
 
 namespace sfem {
 
-// node graph: nodes sharing an element, ascending, no self loops
+// node graph: nodes sharing an element, ascending, no self loops (the
+// element neighbours of each node bucketed by node, then sorted per node)
 void ensure_graph(sfem_ctx* c)
 {
     if (c->d_adj_ptr.p) return;
-    std::vector<uint64_t> e;
-    e.reserve((size_t)c->ne * 12);
-    for (int64_t t = 0; t < c->ne; t++) {
+    const int64_t nn = c->nn, ne = c->ne;
+    std::vector<int64_t> off(nn + 1, 0);
+    for (int64_t t = 0; t < 4 * ne; t++) off[c->tets_h[t] + 1] += 3;
+    for (int64_t i = 0; i < nn; i++) off[i + 1] += off[i];
+    std::vector<int> nb(off[nn]);
+    std::vector<int64_t> fill(off.begin(), off.end() - 1);
+    for (int64_t t = 0; t < ne; t++) {
         const int32_t* v = &c->tets_h[4 * t];
         for (int a = 0; a < 4; a++)
             for (int b = 0; b < 4; b++)
-                if (a != b) e.push_back((uint64_t)(uint32_t)v[a] << 32 | (uint32_t)v[b]);
+                if (a != b) nb[fill[v[a]]++] = v[b];
     }
-    std::sort(e.begin(), e.end());
-    e.erase(std::unique(e.begin(), e.end()), e.end());
-    std::vector<int> ptr(c->nn + 1, 0), idx(e.size());
-    for (size_t k = 0; k < e.size(); k++) {
-        ptr[(e[k] >> 32) + 1]++;
-        idx[k] = (int)(e[k] & 0xffffffffu);
+    std::vector<int> ptr(nn + 1, 0), idx;
+    idx.reserve(nb.size());
+    for (int64_t i = 0; i < nn; i++) {
+        auto first = nb.begin() + off[i], last = nb.begin() + off[i + 1];
+        std::sort(first, last);
+        for (auto it = first; it != last; ++it)
+            if (*it != (int)i && (it == first || *it != *(it - 1))) idx.push_back(*it);
+        ptr[i + 1] = (int)idx.size();
     }
-    for (int64_t i = 0; i < c->nn; i++) ptr[i + 1] += ptr[i];
     c->d_adj_ptr.upload(ptr);
     c->d_adj_idx.upload(idx);
-    c->nadj = (int64_t)e.size();
+    c->nadj = (int64_t)idx.size();
 }
 
 void gnn_install(sfem_ctx* c)
@@ -411,7 +417,12 @@ int sfem_set_learned_preconditioner(sfem_ctx* c, sfem_gnn* g)
 {
     if (!c) return fail(SFEM_EINVAL, "sfem_set_learned_preconditioner: null context");
     c->gnn = g;
-    return SFEM_OK;
+    if (!g) return SFEM_OK;
+    // the node graph is part of the mesh set-up: built here, not in a solve
+    return guard_call("sfem_set_learned_preconditioner", [&]() -> int {
+        ensure_graph(c);
+        return SFEM_OK;
+    });
 }
 
 int sfem_collect_pairs(sfem_ctx* c, const double* load, int nsnap, int snap_every, double rtol, int* collected)
