@@ -228,7 +228,11 @@ int sfem_time_hessvec(sfem_ctx* ctx, int every, double* ms_avg, int64_t* samples
    (p_col = coarse node, p_val = weight); node values, the same for x, y, z.
    The level has the context's material model; its moduli are the context's
    uniform ones, or per coarse element young_e / poisson_e (required when the
-   context has per-element moduli, e.g. averages over each coarse element). */
+   context has per-element moduli, e.g. averages over each coarse element).
+   The coarsest level is solved directly (a dense inverse rebuilt once per load
+   step) when its dofs, 3 x nodes, are at most SFEM_MG_DENSE_MAX, else by
+   Chebyshev smoothing. */
+enum { SFEM_MG_DENSE_MAX = 1536 };
 int sfem_mg_add_level(sfem_ctx* ctx, const sfem_mesh* coarse, const int32_t* inject, const int64_t* p_ptr,
                       const int32_t* p_col, const double* p_val, const double* young_e, const double* poisson_e);
 int sfem_mg_clear(sfem_ctx* ctx);

@@ -18,6 +18,7 @@ COROTATIONAL_POLAR = 16     # the polar rotation and its consistent tangent (no 
 ST_STATIC, ST_QUASI_STATIC, ST_EXPLICIT, ST_IMPLICIT = 0, 1, 2, 3
 SOLVE_REFERENCE, SOLVE_CONVERGED = 0, 1
 PC_NONE, PC_JACOBI, PC_LEARNED, PC_MULTIGRID, PC_MULTIGRID_LEARNED = 0, 1, 2, 3, 4
+MG_DENSE_MAX = 1536         # dofs of a coarsest multigrid level solved directly (SFEM_MG_DENSE_MAX)
 ENOTCONV = -4
 
 EXPORTS = [

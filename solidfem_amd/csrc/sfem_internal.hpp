@@ -103,7 +103,8 @@ struct MgLevel {
     // in element order, node -> (element, corner) incidences, element
     // matrices, the dense operator and its inverse
     bool dense = false;
-    sfem::DevBuf d_tets4, d_dminv, d_cvol, d_inc_ptr, d_inc, d_Ke, d_A, d_Ainv;
+    // d_W = -A^-1 of the dense coarsest level; d_C, d_S: scratch of its blocked inverse
+    sfem::DevBuf d_tets4, d_dminv, d_cvol, d_inc_ptr, d_inc, d_Ke, d_A, d_W, d_C, d_S;
     ~MgLevel();
 };
 

@@ -26,6 +26,8 @@
 
 using namespace sfem;
 
+static_assert(MG_DENSE_MAX == SFEM_MG_DENSE_MAX, "dense coarsest limit of the C-ABI");
+
 MgLevel::~MgLevel() { delete c; }
 
 namespace {
@@ -123,7 +125,7 @@ void vcycle(sfem_ctx* f, int l)
     const int nlev = (int)f->mg.size();
     if (l == nlev) {
         const MgLevel* m = l > 0 ? f->mg[l - 1].get() : nullptr;
-        if (m && m->dense) HIPCK(launch_mg_dense_solve(3 * (int)c->nn, m->d_Ainv.as<float>(), L.b, L.x, c->stream));
+        if (m && m->dense) HIPCK(launch_mg_dense_solve(3 * (int)c->nn, m->d_W.as<float>(), L.b, L.x, c->stream));
         else smooth(L, MG_COARSE_DEGREE, true, cheb_band(L.lmax, MG_COARSE_RATIO));
         return;
     }
@@ -202,7 +204,7 @@ bool mg_prepare(sfem_ctx* f, const float* minv0, bool full)
                                         m.d_cvol.as<float>(), m.d_Ke.as<float>(), ss));
             HIPCK(launch_mg_dense_assemble((int)c->nn, m.d_inc_ptr.as<int>(), m.d_inc.as<int2>(), m.d_tets4.as<int4>(),
                                            m.d_Ke.as<float>(), c->d_fixed.as<uint8_t>(), m.d_A.as<float>(), ss));
-            HIPCK(launch_mg_sweep_inverse(n, m.d_A.as<float>(), m.d_Ainv.as<float>(), ss));
+            HIPCK(launch_mg_dense_inverse(n, m.d_A.as<float>(), m.d_W.as<float>(), m.d_C.as<float>(), m.d_S.as<float>(), ss));
             HIPCK(hipEventRecord(f->mg_ev[1], ss));
             side = true;
         } else {
@@ -366,7 +368,9 @@ void dense_setup(MgLevel& m, const sfem_ctx* f, const sfem_mesh* mesh, const dou
     m.d_Ke.alloc(144 * (size_t)ne * sizeof(float));
     const size_t n = 3 * (size_t)nn;
     m.d_A.alloc(n * n * sizeof(float));
-    m.d_Ainv.alloc(n * n * sizeof(float));
+    m.d_W.alloc(n * n * sizeof(float));
+    m.d_C.alloc(2 * n * 32 * sizeof(float));
+    m.d_S.alloc(32 * 32 * sizeof(float));
     m.dense = true;
 }
 
@@ -434,6 +438,17 @@ int sfem_mg_add_level(sfem_ctx* f, const sfem_mesh* coarse, const int32_t* injec
                 trow[slot] = (int)i;
                 tval[slot] = val[k];
             }
+        // only the coarsest level is solved directly: a level that now has one
+        // below it gives its dense set-up back
+        if (!f->mg.empty() && f->mg.back()->dense) {
+            MgLevel& prev = *f->mg.back();
+            HIPCK(hipStreamSynchronize(f->stream));
+            if (f->mg_side) HIPCK(hipStreamSynchronize(f->mg_side));
+            for (DevBuf* d : {&prev.d_tets4, &prev.d_dminv, &prev.d_cvol, &prev.d_inc_ptr, &prev.d_inc, &prev.d_Ke,
+                              &prev.d_A, &prev.d_W, &prev.d_C, &prev.d_S})
+                d->release();
+            prev.dense = false;
+        }
         if (3 * nc <= MG_DENSE_MAX) dense_setup(*m, f, coarse, young_e, poisson_e);
         m->p_ptr.upload(ptr);
         m->p_col.upload(col);

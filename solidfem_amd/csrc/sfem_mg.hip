@@ -332,132 +332,174 @@ __global__ __launch_bounds__(MG_BLOCK) void k_mg_dense_assemble(int nn, const in
     A[t] = s;
 }
 
-// Ainv = A^-1 for a symmetric positive definite A by the sweep operator
-// (Goodnight, The American Statistician 33 (1979) 149-158): sweeping every
-// pivot k in turn (a_kk <- -1/d, a_ik <- a_ik/d, a_ij <- a_ij - a_ik a_kj/d,
-// d = a_kk) turns A into -A^-1.  One workgroup.  The upper triangle
-// (symmetrised from A) is cut into chunks of 4 consecutive rows of one column
-// (column j has ceil((j+1)/4) chunks) and lives in registers: every thread
-// owns up to MG_SWEEP_CHUNKS chunks.  Only column k passes through LDS,
-// double-buffered: the owners of column k+1's entries publish them while
-// sweeping k, so each pivot costs one barrier and one float4 LDS read per
-// chunk.
-constexpr int MG_SWEEP_BLOCK = 1024;
-constexpr int MG_SWEEP_CHUNKS = 9;
-constexpr int sweep_chunks(int n)
-{
-    int c = 0;
-    for (int m = 1; m <= n; m++) c += (m + 3) / 4;
-    return c;
-}
-static_assert(sweep_chunks(MG_DENSE_MAX) <= MG_SWEEP_CHUNKS * MG_SWEEP_BLOCK, "sweep chunks per thread");
+// The coarsest operator's inverse by the sweep operator (Goodnight, The
+// American Statistician 33 (1979) 149-158): sweeping pivot k (d = a_kk;
+// a_kk <- -1/d, a_ik <- a_ik / d, a_ij <- a_ij - a_ik a_kj / d) for every k
+// turns a symmetric positive definite A into -A^-1.  Blocked in panels of
+// MG_TB pivots: with B the panel, S = the swept diagonal block (= -A_BB^-1),
+// C = the panel's columns A_iB before the sweep and G_i = -C_i S, sweeping B
+// sets A_BB <- S, A_iB <- G_i and A_ij <- A_ij - G_i . C_j (i, j outside B).
+// k_mg_sym symmetrises W = (A + A^T) / 2 and copies panel 0's columns to C;
+// per panel, k_mg_panel (one workgroup) sweeps the diagonal block (read from
+// C), k_mg_tiles updates every upper 32 x 32 tile, writes it and its mirror
+// (W stays exactly symmetric) and copies the next panel's columns as it
+// writes them (C is double-buffered, C[p & 1] for panel p, zero past n).
+// W ends as -A^-1 (k_mg_dense_solve negates).
+constexpr int MG_TB = 32;
 
-__global__ __launch_bounds__(MG_SWEEP_BLOCK) void k_mg_sweep_inverse(int n, const float* __restrict__ A,
-                                                                     float* __restrict__ Ainv)
+__global__ __launch_bounds__(MG_BLOCK) void k_mg_sym(int n, const float* __restrict__ A, float* __restrict__ W,
+                                                     float* __restrict__ C0)
 {
-    __shared__ int cs[MG_DENSE_MAX + 1];                     // first chunk of column j
-    __shared__ __align__(16) float col[2][MG_DENSE_MAX + 5];  // + room for a float4 past row n-1
+    const int64_t t = blockIdx.x * (int64_t)MG_BLOCK + threadIdx.x;
+    if (t < (int64_t)n * n) {
+        const int i = (int)(t / n), j = (int)(t % n);
+        const float v = 0.5f * (A[t] + A[(int64_t)j * n + i]);
+        W[t] = v;
+        if (j < MG_TB) C0[i * MG_TB + j] = v;
+    }
+    if (t < (int64_t)n * MG_TB && (int)(t % MG_TB) >= n) C0[t] = 0.f;   // n < MG_TB: zero past n
+}
+
+constexpr int MG_PANEL_BLOCK = 1024;
+
+__global__ __launch_bounds__(MG_PANEL_BLOCK) void k_mg_panel(int k0, int bp, const float* __restrict__ Cp,
+                                                             float* __restrict__ S)
+{
+    __shared__ float col[2][MG_TB];
     const int tid = threadIdx.x;
-    if (tid <= n) {
-        int c0 = 0;
-        for (int m = 0; m < tid; m++) c0 += (m + 4) >> 2;
-        cs[tid] = c0;
+    // thread -> entry (a, b), a <= b, of the block's upper triangle
+    constexpr int NU = MG_TB * (MG_TB + 1) / 2;
+    static_assert(NU <= MG_PANEL_BLOCK, "one entry per thread");
+    int a = 0, b = -1;
+    if (tid < NU) {
+        b = (int)((sqrtf(8.f * (float)tid + 1.f) - 1.f) * 0.5f);
+        while (b * (b + 1) / 2 > tid) b--;
+        while ((b + 1) * (b + 2) / 2 <= tid) b++;
+        a = tid - b * (b + 1) / 2;
     }
-    if (tid < 2 * (MG_DENSE_MAX + 5)) (&col[0][0])[tid] = 0.f;
-    __syncthreads();
-    const int nch = cs[n];
-    float v[MG_SWEEP_CHUNKS][4];
-    int jj[MG_SWEEP_CHUNKS], ii[MG_SWEEP_CHUNKS];   // column and first row; jj = -1: no chunk
-#pragma unroll
-    for (int q = 0; q < MG_SWEEP_CHUNKS; q++) {
-        const int g = tid + q * MG_SWEEP_BLOCK;
-        jj[q] = -1;
-        ii[q] = 0;
-#pragma unroll
-        for (int r = 0; r < 4; r++) v[q][r] = 0.f;
-        if (g < nch) {
-            int lo = 0, hi = n - 1;   // the last column j with cs[j] <= g
-            while (lo < hi) {
-                const int mid = (lo + hi + 1) >> 1;
-                if (cs[mid] <= g) lo = mid;
-                else hi = mid - 1;
-            }
-            const int j = lo, i0 = 4 * (g - cs[j]);
-            jj[q] = j;
-            ii[q] = i0;
-#pragma unroll
-            for (int r = 0; r < 4; r++) {
-                const int i = i0 + r;
-                if (i <= j) v[q][r] = 0.5f * (A[(int64_t)i * n + j] + A[(int64_t)j * n + i]);
-            }
-            if (i0 == 0) col[0][j] = v[q][0];   // column 0 = row 0 of the upper triangle
-        }
+    const bool own = b >= 0 && b < bp;
+    float v = 0.f;
+    if (own) {
+        v = Cp[(k0 + a) * MG_TB + b];   // C row k0 + a holds W row k0 + a at the panel's columns
+        if (a == 0) col[0][b] = v;       // column 0 = row 0 of the upper triangle
     }
     __syncthreads();
-    for (int k = 0; k < n; k++) {
+    for (int k = 0; k < bp; k++) {
         const float* c = col[k & 1];
         float* cn = col[(k + 1) & 1];
         const float rd = 1.f / c[k];
-#pragma unroll
-        for (int q = 0; q < MG_SWEEP_CHUNKS; q++) {
-            const int j = jj[q], i0 = ii[q];
-            if (j < 0) continue;
-            const float4 c4 = *reinterpret_cast<const float4*>(&c[i0]);
-            const float ci[4] = {c4.x, c4.y, c4.z, c4.w};
-            if (j == k) {
-                // column k: a_ik / d, and -1/d on the diagonal
-#pragma unroll
-                for (int r = 0; r < 4; r++) v[q][r] = i0 + r == k ? -rd : ci[r] * rd;
-            } else {
-                const float cj = c[j];
-#pragma unroll
-                for (int r = 0; r < 4; r++) {
-                    const float t = ci[r] * rd;
-                    const float nv = fmaf(-t, cj, v[q][r]);
-                    v[q][r] = i0 + r == k ? cj * rd : nv;   // row k: a_kj / d
-                }
-            }
-            // publish column k + 1: entries (i, k+1), i <= k+1, and (k+1, j), j > k+1
-            if (j == k + 1) {
-#pragma unroll
-                for (int r = 0; r < 4; r++)
-                    if (i0 + r <= j) cn[i0 + r] = v[q][r];
-            } else if (j > k + 1 && (unsigned)(k + 1 - i0) < 4u) {
-                const int r = k + 1 - i0;
-                cn[j] = r == 0 ? v[q][0] : r == 1 ? v[q][1] : r == 2 ? v[q][2] : v[q][3];
-            }
+        if (own) {
+            if (b == k) v = a == k ? -rd : c[a] * rd;
+            else if (a == k) v = c[b] * rd;
+            else v = fmaf(-(c[a] * rd), c[b], v);
+            // publish column k + 1: entries (a, k+1) and (k+1, b)
+            if (b == k + 1) cn[a] = v;
+            else if (a == k + 1) cn[b] = v;
         }
         __syncthreads();
     }
+    if (b >= 0) {
+        S[a * MG_TB + b] = v;   // zero outside the panel
+        S[b * MG_TB + a] = v;
+    }
+}
+
+__global__ __launch_bounds__(MG_BLOCK) void k_mg_tiles(int n, int p, int bp, float* __restrict__ W,
+                                                       const float* __restrict__ Cp, float* __restrict__ Cn,
+                                                       const float* __restrict__ S)
+{
+    __shared__ float sS[MG_TB][MG_TB + 1], sCI[MG_TB][MG_TB + 1], sCJ[MG_TB][MG_TB + 1];
+    __shared__ float sG[MG_TB][MG_TB + 1], sT[MG_TB][MG_TB + 1];
+    // upper tile (I, J), I <= J, enumerated column by column
+    const int t = blockIdx.x;
+    int J = (int)((sqrtf(8.f * (float)t + 1.f) - 1.f) * 0.5f);
+    while (J * (J + 1) / 2 > t) J--;
+    while ((J + 1) * (J + 2) / 2 <= t) J++;
+    const int I = t - J * (J + 1) / 2;
+    const int r0 = I * MG_TB, c0 = J * MG_TB, kn = (p + 1) * MG_TB;   // kn: the next panel's first column
+    const int tid = threadIdx.x;
+    for (int e = tid; e < MG_TB * MG_TB; e += MG_BLOCK) {
+        const int r = e / MG_TB, k = e % MG_TB;
+        sS[r][k] = S[e];
+        sCI[r][k] = r0 + r < n ? Cp[(r0 + r) * MG_TB + k] : 0.f;
+        sCJ[r][k] = c0 + r < n ? Cp[(c0 + r) * MG_TB + k] : 0.f;
+    }
+    __syncthreads();
+    // G of the tile's rows (I != p) or, for the panel's own row of tiles, of
+    // its columns (I == p)
+    const bool gcol = I == p;
+    if (!(I == p && J == p)) {
+        for (int e = tid; e < MG_TB * MG_TB; e += MG_BLOCK) {
+            const int r = e / MG_TB, k = e % MG_TB;
+            float g = 0.f;
+            for (int m = 0; m < MG_TB; m++) g = fmaf(gcol ? sCJ[r][m] : sCI[r][m], sS[m][k], g);
+            sG[r][k] = -g;
+        }
+    }
+    __syncthreads();
+    // thread -> row r of the tile, columns cq .. cq + 3
+    const int r = tid / 8, cq = (tid % 8) * 4;
+    const int i = r0 + r;
 #pragma unroll
-    for (int q = 0; q < MG_SWEEP_CHUNKS; q++) {
-        const int j = jj[q], i0 = ii[q];
-        if (j < 0) continue;
+    for (int q = 0; q < 4; q++) {
+        const int cc = cq + q, j = c0 + cc;
+        float v = 0.f;
+        if (i < n && j < n) {
+            if (I == p && J == p) v = sS[r][cc];
+            else if (J == p) v = sG[r][cc];
+            else if (I == p) v = sG[cc][r];
+            else {
+                float s = 0.f;
+                for (int k = 0; k < bp; k++) s = fmaf(sG[r][k], sCJ[cc][k], s);
+                v = W[(int64_t)i * n + j] - s;
+            }
+            if (I != J || j >= i) {
+                W[(int64_t)i * n + j] = v;
+                if (J == p + 1) Cn[i * MG_TB + (j - kn)] = v;
+            }
+        } else if (i < n && J == p + 1) {
+            Cn[i * MG_TB + (j - kn)] = 0.f;   // the next panel is the last and ends at n
+        }
+        sT[r][cc] = v;
+    }
+    __syncthreads();
+    // the mirror: W(j, i) = W(i, j); on a diagonal tile its lower half
 #pragma unroll
-        for (int r = 0; r < 4; r++) {
-            const int i = i0 + r;
-            if (i > j) continue;
-            Ainv[(int64_t)i * n + j] = -v[q][r];
-            Ainv[(int64_t)j * n + i] = -v[q][r];
+    for (int q = 0; q < 4; q++) {
+        const int cc = cq + q;   // row r of the mirrored tile is column r of (I, J)
+        if (I != J) {
+            const int jj = c0 + r, ii = r0 + cc;
+            if (jj < n && ii < n) {
+                W[(int64_t)jj * n + ii] = sT[cc][r];
+                if (I == p + 1) Cn[jj * MG_TB + (ii - kn)] = sT[cc][r];
+            }
+        } else if (cc < r) {
+            const int ii = r0 + r, jj = r0 + cc;
+            if (ii < n) {
+                W[(int64_t)ii * n + jj] = sT[cc][r];
+                if (I == p + 1) Cn[ii * MG_TB + (jj - kn)] = sT[cc][r];
+            }
         }
     }
 }
 
-// x = Ainv b: one workgroup per row, a fixed-order tree over its n products
-__global__ __launch_bounds__(MG_BLOCK) void k_mg_dense_solve(int n, const float* __restrict__ Ainv,
+// x = A^-1 b = -W b: one workgroup per row, each thread a strided partial
+// sum, then a fixed-order tree
+__global__ __launch_bounds__(MG_BLOCK) void k_mg_dense_solve(int n, const float* __restrict__ W,
                                                              const float* __restrict__ b, float* __restrict__ x)
 {
     __shared__ float red[MG_BLOCK / 64];
-    const int i = blockIdx.x, j = threadIdx.x;
-    float s = j < n ? Ainv[(int64_t)i * n + j] * b[j] : 0.f;
+    const int i = blockIdx.x, tid = threadIdx.x;
+    float s = 0.f;
+    for (int j = tid; j < n; j += MG_BLOCK) s = fmaf(W[(int64_t)i * n + j], b[j], s);
     for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
-    if ((j & 63) == 0) red[j >> 6] = s;
+    if ((tid & 63) == 0) red[tid >> 6] = s;
     __syncthreads();
-    if (j == 0) {
+    if (tid == 0) {
         float t = 0.f;
 #pragma unroll
         for (int w = 0; w < MG_BLOCK / 64; w++) t += red[w];
-        x[i] = t;
+        x[i] = -t;
     }
 }
 
@@ -556,17 +598,26 @@ hipError_t launch_mg_dense_assemble(int nn, const int* inc_ptr, const int2* inc,
     return hipGetLastError();
 }
 
-hipError_t launch_mg_sweep_inverse(int n, const float* A, float* Ainv, hipStream_t st)
+hipError_t launch_mg_dense_inverse(int n, const float* A, float* W, float* C, float* S, hipStream_t st)
 {
-    if (n > MG_DENSE_MAX) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_mg_sweep_inverse, dim3(1), dim3(MG_SWEEP_BLOCK), 0, st, n, A, Ainv);
+    if (n <= 0 || n > MG_DENSE_MAX) return hipErrorInvalidValue;
+    const int64_t nsym = (int64_t)n * n > (int64_t)n * MG_TB ? (int64_t)n * n : (int64_t)n * MG_TB;
+    hipLaunchKernelGGL(k_mg_sym, dim3(mg_grid(nsym)), dim3(MG_BLOCK), 0, st, n, A, W, C);
+    const int nt = (n + MG_TB - 1) / MG_TB;
+    for (int p = 0; p < nt; p++) {
+        const int k0 = p * MG_TB, bp = n - k0 < MG_TB ? n - k0 : MG_TB;
+        float* Cp = C + (size_t)(p & 1) * n * MG_TB;
+        float* Cn = C + (size_t)((p + 1) & 1) * n * MG_TB;
+        hipLaunchKernelGGL(k_mg_panel, dim3(1), dim3(MG_PANEL_BLOCK), 0, st, k0, bp, Cp, S);
+        hipLaunchKernelGGL(k_mg_tiles, dim3(nt * (nt + 1) / 2), dim3(MG_BLOCK), 0, st, n, p, bp, W, Cp, Cn, S);
+    }
     return hipGetLastError();
 }
 
-hipError_t launch_mg_dense_solve(int n, const float* Ainv, const float* b, float* x, hipStream_t st)
+hipError_t launch_mg_dense_solve(int n, const float* W, const float* b, float* x, hipStream_t st)
 {
-    if (n > MG_DENSE_MAX || n > MG_BLOCK) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_mg_dense_solve, dim3(n), dim3(MG_BLOCK), 0, st, n, Ainv, b, x);
+    if (n <= 0 || n > MG_DENSE_MAX) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_mg_dense_solve, dim3(n), dim3(MG_BLOCK), 0, st, n, W, b, x);
     return hipGetLastError();
 }
 

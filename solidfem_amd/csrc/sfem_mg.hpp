@@ -6,8 +6,8 @@
 namespace sfem {
 
 // the coarsest level is solved directly when its dofs (3 x nodes) are at most
-// this many: its dense inverse's packed upper triangle fits one workgroup's LDS
-constexpr int MG_DENSE_MAX = 255;
+// this many (a dense inverse of up to 9.4 MB, rebuilt once per load step)
+constexpr int MG_DENSE_MAX = 1536;
 
 // Chebyshev step: mode 0 first step from x = 0, 1 next step, 2 first step from
 // x after a coarse correction, 3 residual only (see k_mg_cheb)
@@ -33,7 +33,9 @@ hipError_t launch_mg_elem_matrix(int mat, int64_t ne, const int4* tets, const fl
                                  const float* cvol, float* Ke, hipStream_t st);
 hipError_t launch_mg_dense_assemble(int nn, const int* inc_ptr, const int2* inc, const int4* tets, const float* Ke,
                                     const uint8_t* fixed, float* A, hipStream_t st);
-hipError_t launch_mg_sweep_inverse(int n, const float* A, float* Ainv, hipStream_t st);
-hipError_t launch_mg_dense_solve(int n, const float* Ainv, const float* b, float* x, hipStream_t st);
+// W = -A^-1 (symmetric A); C: 2 x n x 32 and S: 32 x 32 floats of scratch
+hipError_t launch_mg_dense_inverse(int n, const float* A, float* W, float* C, float* S, hipStream_t st);
+// x = A^-1 b from W = -A^-1
+hipError_t launch_mg_dense_solve(int n, const float* W, const float* b, float* x, hipStream_t st);
 
 }  // namespace sfem

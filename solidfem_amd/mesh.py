@@ -181,10 +181,17 @@ def coarse_cell_mean(fine, values):
     return np.repeat(v.reshape(-1), 6)
 
 
-def beam_hierarchy(m, min_cells=2, fixed=None, young_e=None, poisson_e=None):
+# the library solves the coarsest multigrid level directly when it has at
+# most this many dofs (3 x nodes; MG_DENSE_MAX of the library, capi.MG_DENSE_MAX)
+DIRECT_DOFS = 1536
+
+
+def beam_hierarchy(m, min_cells=2, fixed=None, young_e=None, poisson_e=None, direct_dofs=DIRECT_DOFS):
     """Nested coarse levels of a beam mesh for the multigrid preconditioner:
     each level halves the hex grid in every direction while all three cell
-    counts stay even and the thinner cross-section keeps >= min_cells cells.
+    counts stay even and the thinner cross-section keeps >= min_cells cells,
+    and stops at the first level of at most direct_dofs dofs (3 x nodes),
+    which the library solves directly (None: no such stop).
     Coarse node (i, j, k) sits at fine node (2i, 2j, 2k) (its position, so a
     jittered mesh stays nested), and the prolongation interpolates node
     values trilinearly in the grid indices.  Per-element moduli (young_e,
@@ -239,4 +246,6 @@ def beam_hierarchy(m, min_cells=2, fixed=None, young_e=None, poisson_e=None):
             lv = lv + (fE, fnu)
         levels.append(lv)
         fine, ffix = c, cfix
+        if direct_dofs is not None and 3 * c.nn <= direct_dofs:
+            break
     return levels

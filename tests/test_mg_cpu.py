@@ -18,8 +18,16 @@ def dense_p(nf, nc, ptr, col, val):
 
 
 def test_levels_of_the_bench_beam():
+    """halving down to the first level the library solves directly (16x4x4:
+    1,275 dofs), or down to the thinnest cross-section without that stop"""
+    from solidfem_amd import capi, mesh
+    assert mesh.DIRECT_DOFS == capi.MG_DENSE_MAX
     m = BeamMesh(128, 32, 32)
     lv = beam_hierarchy(m)
+    assert [l[1].shape[0] for l in lv] == [98304, 12288, 1536]
+    assert [l[0].shape[0] for l in lv] == [65 * 17 * 17, 33 * 9 * 9, 17 * 5 * 5]
+    assert 3 * lv[-1][0].shape[0] <= capi.MG_DENSE_MAX < 3 * lv[-2][0].shape[0]
+    lv = beam_hierarchy(m, direct_dofs=None)
     assert [l[1].shape[0] for l in lv] == [98304, 12288, 1536, 192]
     assert [l[0].shape[0] for l in lv] == [65 * 17 * 17, 33 * 9 * 9, 17 * 5 * 5, 9 * 3 * 3]
 
