@@ -359,11 +359,14 @@ template <typename T> struct Tangent<T, NEO_HOOKEAN> {
 // The Neo-Hookean tangent on the coarse levels of the multigrid
 // preconditioner, whose states are injected from the level above and need not
 // be valid deformations of the coarse mesh: an element with J <= 0.1 (or not
-// finite) takes the tangent at F = I, and the coefficient mu - lam ln J of the
-// F^-T dF^T F^-T term is kept >= mu / 10 (it turns negative for
-// J > exp(mu / lam), J > 1.01 at nu = 0.495, where an expanded
-// near-incompressible element's tangent is indefinite).  A preconditioner
-// operator, not the element's tangent.
+// finite) takes the tangent at F = I, and the coefficient c = mu - lam ln J of
+// the F^-T dF^T F^-T term is kept in [mu / 10, mu].  Below: c turns negative
+// for J > exp(mu / lam) (J > 1.01 at nu = 0.495).  Above: for a compressed
+// element c grows like lam |ln J| (6 mu at J = 0.95, nu = 0.495), and since
+// c (F^-T dF^T F^-T) : dF = c tr((F^-1 dF)^2) is -c |F^-1 dF|^2 for a
+// rotation-like dF, the tangent is indefinite once c exceeds ~mu.  Both make
+// the coarse operator, solved exactly on the coarsest level, indefinite.  A
+// preconditioner operator, not the element's tangent.
 template <typename T> struct Tangent<T, NEO_HOOKEAN_COARSE> : Tangent<T, NEO_HOOKEAN> {
     __device__ __forceinline__ void init(const T* F, T m, T l)
     {
@@ -376,7 +379,7 @@ template <typename T> struct Tangent<T, NEO_HOOKEAN_COARSE> : Tangent<T, NEO_HOO
             J = T(1);
         }
         const T c = m - l * dlog(J);
-        this->c = c > T(0.1) * m ? c : T(0.1) * m;
+        this->c = c > T(0.1) * m ? (c < m ? c : m) : T(0.1) * m;
     }
 };
 

@@ -103,8 +103,9 @@ struct MgLevel {
     // in element order, node -> (element, corner) incidences, element
     // matrices, the dense operator and its inverse
     bool dense = false;
-    // d_W = -A^-1 of the dense coarsest level; d_C, d_S: scratch of its blocked inverse
-    sfem::DevBuf d_tets4, d_dminv, d_cvol, d_inc_ptr, d_inc, d_Ke, d_A, d_W, d_C, d_S;
+    // dense coarsest level: operator A, inverse Ainv (FP32), and the FP64
+    // working matrix W and scratch C, S of its blocked inverse
+    sfem::DevBuf d_tets4, d_dminv, d_cvol, d_inc_ptr, d_inc, d_Ke, d_A, d_Ainv, d_W, d_C, d_S;
     ~MgLevel();
 };
 

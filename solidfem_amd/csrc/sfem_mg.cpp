@@ -125,7 +125,7 @@ void vcycle(sfem_ctx* f, int l)
     const int nlev = (int)f->mg.size();
     if (l == nlev) {
         const MgLevel* m = l > 0 ? f->mg[l - 1].get() : nullptr;
-        if (m && m->dense) HIPCK(launch_mg_dense_solve(3 * (int)c->nn, m->d_W.as<float>(), L.b, L.x, c->stream));
+        if (m && m->dense) HIPCK(launch_mg_dense_solve(3 * (int)c->nn, m->d_Ainv.as<float>(), L.b, L.x, c->stream));
         else smooth(L, MG_COARSE_DEGREE, true, cheb_band(L.lmax, MG_COARSE_RATIO));
         return;
     }
@@ -191,7 +191,8 @@ bool mg_prepare(sfem_ctx* f, const float* minv0, bool full)
             if (!full) continue;
             // the coarsest operator, assembled and inverted on the side stream
             // once its state is injected; the main stream goes on with the
-            // other levels' estimates and waits for it before the first V-cycle
+            // other levels' estimates and waits for it before the norms are
+            // read back
             if (!f->mg_side) {
                 HIPCK(hipStreamCreateWithFlags(&f->mg_side, hipStreamNonBlocking));
                 for (hipEvent_t& e : f->mg_ev) HIPCK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
@@ -204,7 +205,8 @@ bool mg_prepare(sfem_ctx* f, const float* minv0, bool full)
                                         m.d_cvol.as<float>(), m.d_Ke.as<float>(), ss));
             HIPCK(launch_mg_dense_assemble((int)c->nn, m.d_inc_ptr.as<int>(), m.d_inc.as<int2>(), m.d_tets4.as<int4>(),
                                            m.d_Ke.as<float>(), c->d_fixed.as<uint8_t>(), m.d_A.as<float>(), ss));
-            HIPCK(launch_mg_dense_inverse(n, m.d_A.as<float>(), m.d_W.as<float>(), m.d_C.as<float>(), m.d_S.as<float>(), ss));
+            HIPCK(launch_mg_dense_inverse(n, m.d_A.as<float>(), m.d_Ainv.as<float>(), m.d_W.as<double>(),
+                                          m.d_C.as<double>(), m.d_S.as<double>(), ss));
             HIPCK(hipEventRecord(f->mg_ev[1], ss));
             side = true;
         } else {
@@ -368,9 +370,10 @@ void dense_setup(MgLevel& m, const sfem_ctx* f, const sfem_mesh* mesh, const dou
     m.d_Ke.alloc(144 * (size_t)ne * sizeof(float));
     const size_t n = 3 * (size_t)nn;
     m.d_A.alloc(n * n * sizeof(float));
-    m.d_W.alloc(n * n * sizeof(float));
-    m.d_C.alloc(2 * n * 32 * sizeof(float));
-    m.d_S.alloc(32 * 32 * sizeof(float));
+    m.d_Ainv.alloc(n * n * sizeof(float));
+    m.d_W.alloc(n * n * sizeof(double));
+    m.d_C.alloc(2 * n * 32 * sizeof(double));
+    m.d_S.alloc(32 * 32 * sizeof(double));
     m.dense = true;
 }
 
@@ -445,7 +448,7 @@ int sfem_mg_add_level(sfem_ctx* f, const sfem_mesh* coarse, const int32_t* injec
             HIPCK(hipStreamSynchronize(f->stream));
             if (f->mg_side) HIPCK(hipStreamSynchronize(f->mg_side));
             for (DevBuf* d : {&prev.d_tets4, &prev.d_dminv, &prev.d_cvol, &prev.d_inc_ptr, &prev.d_inc, &prev.d_Ke,
-                              &prev.d_A, &prev.d_W, &prev.d_C, &prev.d_S})
+                              &prev.d_A, &prev.d_Ainv, &prev.d_W, &prev.d_C, &prev.d_S})
                 d->release();
             prev.dense = false;
         }

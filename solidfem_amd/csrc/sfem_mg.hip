@@ -344,28 +344,37 @@ __global__ __launch_bounds__(MG_BLOCK) void k_mg_dense_assemble(int nn, const in
 // C), k_mg_tiles updates every upper 32 x 32 tile, writes it and its mirror
 // (W stays exactly symmetric) and copies the next panel's columns as it
 // writes them (C is double-buffered, C[p & 1] for panel p, zero past n).
-// W ends as -A^-1 (k_mg_dense_solve negates).
+// The sweep runs in FP64 (the coarse operators of nearly incompressible
+// material have condition numbers ~1e6, where an FP32 sweep loses percents);
+// k_mg_negate_f32 stores -W = A^-1 in FP32 for the V-cycle's product.
 constexpr int MG_TB = 32;
+typedef double MgT;   // arithmetic of the dense inverse
 
-__global__ __launch_bounds__(MG_BLOCK) void k_mg_sym(int n, const float* __restrict__ A, float* __restrict__ W,
-                                                     float* __restrict__ C0)
+__global__ __launch_bounds__(MG_BLOCK) void k_mg_sym(int n, const float* __restrict__ A, MgT* __restrict__ W,
+                                                     MgT* __restrict__ C0)
 {
     const int64_t t = blockIdx.x * (int64_t)MG_BLOCK + threadIdx.x;
     if (t < (int64_t)n * n) {
         const int i = (int)(t / n), j = (int)(t % n);
-        const float v = 0.5f * (A[t] + A[(int64_t)j * n + i]);
+        const MgT v = MgT(0.5) * ((MgT)A[t] + (MgT)A[(int64_t)j * n + i]);
         W[t] = v;
         if (j < MG_TB) C0[i * MG_TB + j] = v;
     }
-    if (t < (int64_t)n * MG_TB && (int)(t % MG_TB) >= n) C0[t] = 0.f;   // n < MG_TB: zero past n
+    if (t < (int64_t)n * MG_TB && (int)(t % MG_TB) >= n) C0[t] = MgT(0);   // n < MG_TB: zero past n
+}
+
+__global__ __launch_bounds__(MG_BLOCK) void k_mg_negate_f32(int64_t nn2, const MgT* __restrict__ W, float* __restrict__ Ainv)
+{
+    const int64_t t = blockIdx.x * (int64_t)MG_BLOCK + threadIdx.x;
+    if (t < nn2) Ainv[t] = (float)(-W[t]);
 }
 
 constexpr int MG_PANEL_BLOCK = 1024;
 
-__global__ __launch_bounds__(MG_PANEL_BLOCK) void k_mg_panel(int k0, int bp, const float* __restrict__ Cp,
-                                                             float* __restrict__ S)
+__global__ __launch_bounds__(MG_PANEL_BLOCK) void k_mg_panel(int k0, int bp, const MgT* __restrict__ Cp,
+                                                             MgT* __restrict__ S)
 {
-    __shared__ float col[2][MG_TB];
+    __shared__ MgT col[2][MG_TB];
     const int tid = threadIdx.x;
     // thread -> entry (a, b), a <= b, of the block's upper triangle
     constexpr int NU = MG_TB * (MG_TB + 1) / 2;
@@ -378,20 +387,20 @@ __global__ __launch_bounds__(MG_PANEL_BLOCK) void k_mg_panel(int k0, int bp, con
         a = tid - b * (b + 1) / 2;
     }
     const bool own = b >= 0 && b < bp;
-    float v = 0.f;
+    MgT v = 0;
     if (own) {
         v = Cp[(k0 + a) * MG_TB + b];   // C row k0 + a holds W row k0 + a at the panel's columns
         if (a == 0) col[0][b] = v;       // column 0 = row 0 of the upper triangle
     }
     __syncthreads();
     for (int k = 0; k < bp; k++) {
-        const float* c = col[k & 1];
-        float* cn = col[(k + 1) & 1];
-        const float rd = 1.f / c[k];
+        const MgT* c = col[k & 1];
+        MgT* cn = col[(k + 1) & 1];
+        const MgT rd = MgT(1) / c[k];
         if (own) {
             if (b == k) v = a == k ? -rd : c[a] * rd;
             else if (a == k) v = c[b] * rd;
-            else v = fmaf(-(c[a] * rd), c[b], v);
+            else v = fma(-(c[a] * rd), c[b], v);
             // publish column k + 1: entries (a, k+1) and (k+1, b)
             if (b == k + 1) cn[a] = v;
             else if (a == k + 1) cn[b] = v;
@@ -404,12 +413,12 @@ __global__ __launch_bounds__(MG_PANEL_BLOCK) void k_mg_panel(int k0, int bp, con
     }
 }
 
-__global__ __launch_bounds__(MG_BLOCK) void k_mg_tiles(int n, int p, int bp, float* __restrict__ W,
-                                                       const float* __restrict__ Cp, float* __restrict__ Cn,
-                                                       const float* __restrict__ S)
+__global__ __launch_bounds__(MG_BLOCK) void k_mg_tiles(int n, int p, int bp, MgT* __restrict__ W,
+                                                       const MgT* __restrict__ Cp, MgT* __restrict__ Cn,
+                                                       const MgT* __restrict__ S)
 {
-    __shared__ float sS[MG_TB][MG_TB + 1], sCI[MG_TB][MG_TB + 1], sCJ[MG_TB][MG_TB + 1];
-    __shared__ float sG[MG_TB][MG_TB + 1], sT[MG_TB][MG_TB + 1];
+    __shared__ MgT sS[MG_TB][MG_TB + 1], sCI[MG_TB][MG_TB + 1], sCJ[MG_TB][MG_TB + 1];
+    __shared__ MgT sG[MG_TB][MG_TB + 1], sT[MG_TB][MG_TB + 1];
     // upper tile (I, J), I <= J, enumerated column by column
     const int t = blockIdx.x;
     int J = (int)((sqrtf(8.f * (float)t + 1.f) - 1.f) * 0.5f);
@@ -421,8 +430,8 @@ __global__ __launch_bounds__(MG_BLOCK) void k_mg_tiles(int n, int p, int bp, flo
     for (int e = tid; e < MG_TB * MG_TB; e += MG_BLOCK) {
         const int r = e / MG_TB, k = e % MG_TB;
         sS[r][k] = S[e];
-        sCI[r][k] = r0 + r < n ? Cp[(r0 + r) * MG_TB + k] : 0.f;
-        sCJ[r][k] = c0 + r < n ? Cp[(c0 + r) * MG_TB + k] : 0.f;
+        sCI[r][k] = r0 + r < n ? Cp[(r0 + r) * MG_TB + k] : MgT(0);
+        sCJ[r][k] = c0 + r < n ? Cp[(c0 + r) * MG_TB + k] : MgT(0);
     }
     __syncthreads();
     // G of the tile's rows (I != p) or, for the panel's own row of tiles, of
@@ -431,8 +440,8 @@ __global__ __launch_bounds__(MG_BLOCK) void k_mg_tiles(int n, int p, int bp, flo
     if (!(I == p && J == p)) {
         for (int e = tid; e < MG_TB * MG_TB; e += MG_BLOCK) {
             const int r = e / MG_TB, k = e % MG_TB;
-            float g = 0.f;
-            for (int m = 0; m < MG_TB; m++) g = fmaf(gcol ? sCJ[r][m] : sCI[r][m], sS[m][k], g);
+            MgT g = 0;
+            for (int m = 0; m < MG_TB; m++) g = fma(gcol ? sCJ[r][m] : sCI[r][m], sS[m][k], g);
             sG[r][k] = -g;
         }
     }
@@ -443,14 +452,14 @@ __global__ __launch_bounds__(MG_BLOCK) void k_mg_tiles(int n, int p, int bp, flo
 #pragma unroll
     for (int q = 0; q < 4; q++) {
         const int cc = cq + q, j = c0 + cc;
-        float v = 0.f;
+        MgT v = 0;
         if (i < n && j < n) {
             if (I == p && J == p) v = sS[r][cc];
             else if (J == p) v = sG[r][cc];
             else if (I == p) v = sG[cc][r];
             else {
-                float s = 0.f;
-                for (int k = 0; k < bp; k++) s = fmaf(sG[r][k], sCJ[cc][k], s);
+                MgT s = 0;
+                for (int k = 0; k < bp; k++) s = fma(sG[r][k], sCJ[cc][k], s);
                 v = W[(int64_t)i * n + j] - s;
             }
             if (I != J || j >= i) {
@@ -458,7 +467,7 @@ __global__ __launch_bounds__(MG_BLOCK) void k_mg_tiles(int n, int p, int bp, flo
                 if (J == p + 1) Cn[i * MG_TB + (j - kn)] = v;
             }
         } else if (i < n && J == p + 1) {
-            Cn[i * MG_TB + (j - kn)] = 0.f;   // the next panel is the last and ends at n
+            Cn[i * MG_TB + (j - kn)] = MgT(0);   // the next panel is the last and ends at n
         }
         sT[r][cc] = v;
     }
@@ -483,15 +492,15 @@ __global__ __launch_bounds__(MG_BLOCK) void k_mg_tiles(int n, int p, int bp, flo
     }
 }
 
-// x = A^-1 b = -W b: one workgroup per row, each thread a strided partial
-// sum, then a fixed-order tree
-__global__ __launch_bounds__(MG_BLOCK) void k_mg_dense_solve(int n, const float* __restrict__ W,
+// x = Ainv b: one workgroup per row, each thread a strided partial sum, then
+// a fixed-order tree
+__global__ __launch_bounds__(MG_BLOCK) void k_mg_dense_solve(int n, const float* __restrict__ Ainv,
                                                              const float* __restrict__ b, float* __restrict__ x)
 {
     __shared__ float red[MG_BLOCK / 64];
     const int i = blockIdx.x, tid = threadIdx.x;
     float s = 0.f;
-    for (int j = tid; j < n; j += MG_BLOCK) s = fmaf(W[(int64_t)i * n + j], b[j], s);
+    for (int j = tid; j < n; j += MG_BLOCK) s = fmaf(Ainv[(int64_t)i * n + j], b[j], s);
     for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
     if ((tid & 63) == 0) red[tid >> 6] = s;
     __syncthreads();
@@ -499,7 +508,7 @@ __global__ __launch_bounds__(MG_BLOCK) void k_mg_dense_solve(int n, const float*
         float t = 0.f;
 #pragma unroll
         for (int w = 0; w < MG_BLOCK / 64; w++) t += red[w];
-        x[i] = -t;
+        x[i] = t;
     }
 }
 
@@ -598,7 +607,7 @@ hipError_t launch_mg_dense_assemble(int nn, const int* inc_ptr, const int2* inc,
     return hipGetLastError();
 }
 
-hipError_t launch_mg_dense_inverse(int n, const float* A, float* W, float* C, float* S, hipStream_t st)
+hipError_t launch_mg_dense_inverse(int n, const float* A, float* Ainv, double* W, double* C, double* S, hipStream_t st)
 {
     if (n <= 0 || n > MG_DENSE_MAX) return hipErrorInvalidValue;
     const int64_t nsym = (int64_t)n * n > (int64_t)n * MG_TB ? (int64_t)n * n : (int64_t)n * MG_TB;
@@ -606,18 +615,19 @@ hipError_t launch_mg_dense_inverse(int n, const float* A, float* W, float* C, fl
     const int nt = (n + MG_TB - 1) / MG_TB;
     for (int p = 0; p < nt; p++) {
         const int k0 = p * MG_TB, bp = n - k0 < MG_TB ? n - k0 : MG_TB;
-        float* Cp = C + (size_t)(p & 1) * n * MG_TB;
-        float* Cn = C + (size_t)((p + 1) & 1) * n * MG_TB;
+        double* Cp = C + (size_t)(p & 1) * n * MG_TB;
+        double* Cn = C + (size_t)((p + 1) & 1) * n * MG_TB;
         hipLaunchKernelGGL(k_mg_panel, dim3(1), dim3(MG_PANEL_BLOCK), 0, st, k0, bp, Cp, S);
         hipLaunchKernelGGL(k_mg_tiles, dim3(nt * (nt + 1) / 2), dim3(MG_BLOCK), 0, st, n, p, bp, W, Cp, Cn, S);
     }
+    hipLaunchKernelGGL(k_mg_negate_f32, dim3(mg_grid((int64_t)n * n)), dim3(MG_BLOCK), 0, st, (int64_t)n * n, W, Ainv);
     return hipGetLastError();
 }
 
-hipError_t launch_mg_dense_solve(int n, const float* W, const float* b, float* x, hipStream_t st)
+hipError_t launch_mg_dense_solve(int n, const float* Ainv, const float* b, float* x, hipStream_t st)
 {
     if (n <= 0 || n > MG_DENSE_MAX) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_mg_dense_solve, dim3(n), dim3(MG_BLOCK), 0, st, n, W, b, x);
+    hipLaunchKernelGGL(k_mg_dense_solve, dim3(n), dim3(MG_BLOCK), 0, st, n, Ainv, b, x);
     return hipGetLastError();
 }
 
