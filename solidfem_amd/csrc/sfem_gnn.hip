@@ -197,7 +197,19 @@ __global__ __launch_bounds__(256) void k_mean_fwd(int64_t N, const int* __restri
     const int c = (int)(t % 32);
     const int b = ptr[i], e = ptr[i + 1];
     float acc = 0.f;
-    for (int k = b; k < e; k++) acc += h[(int64_t)idx[k] * 32 + c];
+    // eight neighbours' ids, then their eight feature loads, in flight
+    // together; the sum keeps the neighbour order
+    for (int k0 = b; k0 < e; k0 += 8) {
+        int id[8];
+#pragma unroll
+        for (int q = 0; q < 8; q++) id[q] = k0 + q < e ? idx[k0 + q] : -1;
+        float v[8];
+#pragma unroll
+        for (int q = 0; q < 8; q++) v[q] = id[q] >= 0 ? h[(int64_t)id[q] * 32 + c] : 0.f;
+#pragma unroll
+        for (int q = 0; q < 8; q++)
+            if (k0 + q < e) acc += v[q];
+    }
     m[t] = e > b ? acc / (float)(e - b) : 0.f;
 }
 
