@@ -29,12 +29,14 @@ __device__ __forceinline__ short f2bf(float f)
 }
 
 // ------------------------------------------------------------------- GEMMs
-// Y[N,NO] = [X1 | X2] W + b, X1: N x K1, X2: N x (K - K1), W: K x NO bf16
-// row-major; Yact = relu(Y) when non-null.  One wave = 16 rows x NO columns.
+// Y[N,NO] = [X1 | X2] W + b (+ R when non-null), X1: N x K1, X2: N x (K - K1),
+// W: K x NO bf16 row-major; Y and Yact = relu(Y) each written when non-null.
+// K1 is a multiple of 4 (each lane loads 4 consecutive inputs of one row as
+// one float4).  One wave = 16 rows x NO columns.
 template <int K, int NO>
 __global__ __launch_bounds__(256) void k_gemm(int64_t N, const float* __restrict__ X1, int K1, const float* __restrict__ X2,
                                               const uint16_t* __restrict__ W, const float* __restrict__ bias,
-                                              float* __restrict__ Y, float* __restrict__ Yact)
+                                              const float* __restrict__ R, float* __restrict__ Y, float* __restrict__ Yact)
 {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int64_t row0 = ((int64_t)blockIdx.x * 4 + wave) * 16;
@@ -48,13 +50,15 @@ __global__ __launch_bounds__(256) void k_gemm(int64_t N, const float* __restrict
 #pragma unroll
     for (int k0 = 0; k0 < K; k0 += 16) {
         bf16x4 a;
-#pragma unroll
-        for (int q = 0; q < 4; q++) {
-            const int k = k0 + 4 * kq + q;
-            float v = 0.f;
-            if (ar < N) v = k < K1 ? X1[ar * K1 + k] : X2[ar * K2 + (k - K1)];
-            a[q] = f2bf(v);
-        }
+        const int k = k0 + 4 * kq;
+        float4 xv = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (ar < N)
+            xv = k < K1 ? *reinterpret_cast<const float4*>(X1 + ar * K1 + k)
+                        : *reinterpret_cast<const float4*>(X2 + ar * K2 + (k - K1));
+        a[0] = f2bf(xv.x);
+        a[1] = f2bf(xv.y);
+        a[2] = f2bf(xv.z);
+        a[3] = f2bf(xv.w);
 #pragma unroll
         for (int t = 0; t < NO / 16; t++) {
             bf16x4 b;
@@ -70,8 +74,9 @@ __global__ __launch_bounds__(256) void k_gemm(int64_t N, const float* __restrict
             const int64_t r = row0 + 4 * kq + q;
             const int j = t * 16 + i;
             if (r < N) {
-                const float v = acc[t][q] + (bias ? bias[j] : 0.f);
-                Y[r * NO + j] = v;
+                float v = acc[t][q] + (bias ? bias[j] : 0.f);
+                if (R) v = R[r * NO + j] + v;
+                if (Y) Y[r * NO + j] = v;
                 if (Yact) Yact[r * NO + j] = v > 0.f ? v : 0.f;
             }
         }
@@ -165,14 +170,6 @@ __global__ __launch_bounds__(256) void k_relu_bwd(int64_t n, const float* __rest
 {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i < n) dpre[i] = pre[i] > 0.f ? dpost[i] : 0.f;
-}
-
-// y = a + b
-__global__ __launch_bounds__(256) void k_add(int64_t n, const float* __restrict__ a, const float* __restrict__ b,
-                                             float* __restrict__ y)
-{
-    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (i < n) y[i] = a[i] + b[i];
 }
 
 // dU (N x 64) -> dh (+)= dU[:, :32], dm = dU[:, 32:]
@@ -457,12 +454,13 @@ __global__ __launch_bounds__(256) void k_to_bf16_pair(int K, int NO, const float
 static inline unsigned nblk(int64_t n, int bs = 256) { return (unsigned)((n + bs - 1) / bs); }
 
 hipError_t gnn_gemm(int K, int NO, int64_t N, const float* X1, int K1, const float* X2, const uint16_t* W, const float* bias,
-                    float* Y, float* Yact, hipStream_t s)
+                    float* Y, float* Yact, hipStream_t s, const float* R)
 {
+    if (K1 % 4 != 0 || (K1 < K && !X2)) return hipErrorInvalidValue;
     const dim3 g((unsigned)((N + 63) / 64)), b(256);
 #define SFEM_GEMM_CASE(KK, NN)                                                                            \
     if (K == KK && NO == NN) {                                                                             \
-        hipLaunchKernelGGL((k_gemm<KK, NN>), g, b, 0, s, N, X1, K1, X2, W, bias, Y, Yact);                 \
+        hipLaunchKernelGGL((k_gemm<KK, NN>), g, b, 0, s, N, X1, K1, X2, W, bias, R, Y, Yact);              \
         return hipGetLastError();                                                                          \
     }
     SFEM_GEMM_CASE(16, 32)
@@ -506,12 +504,6 @@ hipError_t gnn_bgrad(int NO, int64_t N, const float* dY, float* partial, int npa
 hipError_t gnn_relu_bwd(int64_t n, const float* pre, const float* dpost, float* dpre, hipStream_t s)
 {
     hipLaunchKernelGGL(k_relu_bwd, dim3(nblk(n)), dim3(256), 0, s, n, pre, dpost, dpre);
-    return hipGetLastError();
-}
-
-hipError_t gnn_add(int64_t n, const float* a, const float* b, float* y, hipStream_t s)
-{
-    hipLaunchKernelGGL(k_add, dim3(nblk(n)), dim3(256), 0, s, n, a, b, y);
     return hipGetLastError();
 }
 

@@ -5,16 +5,16 @@
 
 namespace sfem {
 
-// Y = [X1 | X2] W + b (W: K x NO bf16 row-major), optional relu copy
+// Y = [X1 | X2] W + b (+ R) (W: K x NO bf16 row-major), Y and its relu copy
+// Yact each optional; K1 a multiple of 4
 hipError_t gnn_gemm(int K, int NO, int64_t N, const float* X1, int K1, const float* X2, const uint16_t* W, const float* bias,
-                    float* Y, float* Yact, hipStream_t s);
+                    float* Y, float* Yact, hipStream_t s, const float* R = nullptr);
 // out (K x NO) = [X1 | X2]^T dY via nparts partials (partial: nparts*K*NO floats)
 hipError_t gnn_wgrad(int K, int NO, int64_t N, const float* X1, int K1, const float* X2, const float* dY, float* partial,
                      int nparts, float* out, hipStream_t s);
 // out (NO) = column sums of dY
 hipError_t gnn_bgrad(int NO, int64_t N, const float* dY, float* partial, int nparts, float* out, hipStream_t s);
 hipError_t gnn_relu_bwd(int64_t n, const float* pre, const float* dpost, float* dpre, hipStream_t s);
-hipError_t gnn_add(int64_t n, const float* a, const float* b, float* y, hipStream_t s);
 hipError_t gnn_split_concat(int64_t N, const float* dU, float* dh, float* dm, hipStream_t s);
 hipError_t gnn_mean_fwd(int64_t N, const int* ptr, const int* idx, const float* h, float* m, hipStream_t s);
 hipError_t gnn_mean_bwd(int64_t N, const int* ptr, const int* idx, const float* dm, float* dh, hipStream_t s);

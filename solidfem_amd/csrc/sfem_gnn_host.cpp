@@ -176,23 +176,27 @@ struct sfem_gnn {
     }
 
     // features of the context (Jacobi blocks in c->d_bd32 must be current)
-    void forward(sfem_ctx* c, hipStream_t s)
+    // train: also keep the pre-activations the backward pass reads
+    void forward(sfem_ctx* c, hipStream_t s, bool train = true)
     {
         const int64_t N = c->nn;
         workspace(N);
         const int* ap = c->d_adj_ptr.as<int>();
         const int* ai = c->d_adj_idx.as<int>();
         HIPCK(gnn_features(N, c->d_bd32.as<float>(), c->d_fixed.as<uint8_t>(), ap, ai, feat.as<float>(), cbase.as<float>(), s));
-        HIPCK(gnn_gemm(FIN, WIDTH, N, feat.as<float>(), FIN, nullptr, Wb(e1()), Bias(e1()), A1.as<float>(), H1.as<float>(), s));
+        HIPCK(gnn_gemm(FIN, WIDTH, N, feat.as<float>(), FIN, nullptr, Wb(e1()), Bias(e1()), train ? A1.as<float>() : nullptr,
+                       H1.as<float>(), s));
         HIPCK(gnn_gemm(WIDTH, WIDTH, N, H1.as<float>(), WIDTH, nullptr, Wb(e2()), Bias(e2()), h[0].as<float>(), nullptr, s));
         for (int r = 0; r < ROUNDS; r++) {
             HIPCK(gnn_mean_fwd(N, ap, ai, h[r].as<float>(), Mg[r].as<float>(), s));
             HIPCK(gnn_gemm(2 * WIDTH, WIDTH, N, h[r].as<float>(), WIDTH, Mg[r].as<float>(), Wb(r1(r)), Bias(r1(r)),
-                           B[r].as<float>(), C[r].as<float>(), s));
-            HIPCK(gnn_gemm(WIDTH, WIDTH, N, C[r].as<float>(), WIDTH, nullptr, Wb(r2(r)), Bias(r2(r)), dT1.as<float>(), nullptr, s));
-            HIPCK(gnn_add(N * WIDTH, h[r].as<float>(), dT1.as<float>(), h[r + 1].as<float>(), s));
+                           train ? B[r].as<float>() : nullptr, C[r].as<float>(), s));
+            // h_{r+1} = h_r + (C_r W + b): the residual add in the product's epilogue
+            HIPCK(gnn_gemm(WIDTH, WIDTH, N, C[r].as<float>(), WIDTH, nullptr, Wb(r2(r)), Bias(r2(r)), h[r + 1].as<float>(),
+                           nullptr, s, h[r].as<float>()));
         }
-        HIPCK(gnn_gemm(WIDTH, WIDTH, N, h[ROUNDS].as<float>(), WIDTH, nullptr, Wb(d1()), Bias(d1()), A3.as<float>(), H3.as<float>(), s));
+        HIPCK(gnn_gemm(WIDTH, WIDTH, N, h[ROUNDS].as<float>(), WIDTH, nullptr, Wb(d1()), Bias(d1()),
+                       train ? A3.as<float>() : nullptr, H3.as<float>(), s));
         HIPCK(gnn_gemm(WIDTH, FOUT, N, H3.as<float>(), WIDTH, nullptr, Wb(d2()), Bias(d2()), O.as<float>(), nullptr, s));
         HIPCK(gnn_blocks_fwd(N, O.as<float>(), cbase.as<float>(), BLOCK_EPS, minv.as<float>(), s));
     }
@@ -274,7 +278,7 @@ void gnn_install(sfem_ctx* c)
 {
     sfem_gnn* g = c->gnn;
     ensure_graph(c);
-    g->forward(c, c->stream);
+    g->forward(c, c->stream, false);
     if (!c->d_minv_learned.p || c->d_minv_learned.n < 9 * (size_t)c->nn * sizeof(float))
         c->d_minv_learned.alloc(9 * (size_t)c->nn * sizeof(float));
     HIPCK(hipMemcpyAsync(c->d_minv_learned.p, g->minv.p, 9 * (size_t)c->nn * sizeof(float), hipMemcpyDeviceToDevice,
