@@ -239,6 +239,13 @@ int sfem_mg_clear(sfem_ctx* ctx);
 /* levels (1 + coarse levels) and, when lmax is non-null, the largest-eigenvalue
    estimates of M A per level from the last set-up */
 int sfem_mg_info(sfem_ctx* ctx, int32_t* nlevels, double* lmax);
+/* The directly solved coarsest level at the current state (after a full
+   set-up): n = 3 x its nodes; when a / ainv are non-null (n x n each,
+   capacity >= n * n), the assembled operator A (FP32 element matrices; fixed
+   rows and columns identity) and the inverse the V-cycle applies, row-major,
+   dof 3 k + c of coarse node k.  SFEM_ESTATE when the coarsest level is not
+   solved directly. */
+int sfem_mg_coarse_dense(sfem_ctx* ctx, int64_t capacity, int32_t* n, double* a, double* ainv);
 /* z = V(r): one V-cycle (block-Jacobi smoothers) at the current state;
    r, z nn*3 in original order, fixed rows ignored / zero */
 int sfem_mg_apply(sfem_ctx* ctx, const double* r, double* z);

@@ -28,7 +28,7 @@ EXPORTS = [
     "sfem_gradients", "sfem_matvec",
     "sfem_block_diagonal", "sfem_solve", "sfem_step", "sfem_set_precond_blocks", "sfem_info",
     "sfem_reset", "sfem_profile_hessvec", "sfem_time_hessvec",
-    "sfem_mg_add_level", "sfem_mg_clear", "sfem_mg_info", "sfem_mg_apply", "sfem_mg_pcg",
+    "sfem_mg_add_level", "sfem_mg_clear", "sfem_mg_info", "sfem_mg_apply", "sfem_mg_pcg", "sfem_mg_coarse_dense",
     "sfem_gnn_create", "sfem_gnn_destroy", "sfem_gnn_num_params", "sfem_gnn_get_params", "sfem_gnn_set_params",
     "sfem_set_learned_preconditioner", "sfem_collect_pairs", "sfem_collect_rough_pairs", "sfem_num_pairs", "sfem_get_pair", "sfem_clear_pairs",
     "sfem_gnn_loss", "sfem_gnn_eval", "sfem_gnn_train", "sfem_gnn_inputs", "sfem_gnn_gemm_check",
@@ -112,6 +112,7 @@ def lib():
             "sfem_mg_add_level": ([V, C.POINTER(Mesh), _ip, C.POINTER(C.c_int64), _ip, _dp, _dp, _dp], C.c_int),
             "sfem_mg_clear": ([V], C.c_int),
             "sfem_mg_info": ([V, C.POINTER(C.c_int32), _dp], C.c_int),
+            "sfem_mg_coarse_dense": ([V, C.c_int64, C.POINTER(C.c_int32), _dp, _dp], C.c_int),
             "sfem_mg_apply": ([V, _dp, _dp], C.c_int),
             "sfem_mg_pcg": ([V, _dp, _dp, C.c_double, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32)], C.c_int),
             "sfem_gnn_create": ([C.c_uint64, C.POINTER(V)], C.c_int),
@@ -294,6 +295,15 @@ class Solver:
         lm = np.zeros(n.value)
         _check(lib().sfem_mg_info(self.h, C.byref(n), _p(lm)))
         return lm
+
+    def mg_coarse_dense(self):
+        """(A, Ainv) of the directly solved coarsest level at the current state"""
+        n = C.c_int32()
+        _check(lib().sfem_mg_coarse_dense(self.h, 0, C.byref(n), None, None))
+        a = np.zeros((n.value, n.value))
+        ai = np.zeros((n.value, n.value))
+        _check(lib().sfem_mg_coarse_dense(self.h, a.size, C.byref(n), _p(a), _p(ai)))
+        return a, ai
 
     def mg_apply(self, r):
         r = np.ascontiguousarray(r, dtype=np.float64).reshape(-1, 3)

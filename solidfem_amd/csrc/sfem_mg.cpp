@@ -488,6 +488,32 @@ int sfem_mg_info(sfem_ctx* f, int32_t* nlevels, double* lmax)
     return SFEM_OK;
 }
 
+int sfem_mg_coarse_dense(sfem_ctx* f, int64_t capacity, int32_t* n, double* a, double* ainv)
+{
+    if (!f || !n) return fail(SFEM_EINVAL, "sfem_mg_coarse_dense: null argument");
+    if (f->mg.empty() || !f->mg.back()->dense)
+        return fail(SFEM_ESTATE, "sfem_mg_coarse_dense: the coarsest level is not solved directly");
+    MgLevel& m = *f->mg.back();
+    const int nd = 3 * (int)m.c->nn;
+    *n = nd;
+    if (!a && !ainv) return SFEM_OK;
+    if (capacity < (int64_t)nd * nd) return fail(SFEM_EINVAL, "sfem_mg_coarse_dense: buffer smaller than n x n");
+    return guard_call("sfem_mg_coarse_dense", [&]() -> int {
+        build_jacobi(f);
+        if (!mg_prepare(f, f->d_minv.as<float>(), true))
+            return fail(SFEM_ESTATE, "multigrid: a level's operator is not finite at this state");
+        HIPCK(hipStreamSynchronize(f->stream));
+        std::vector<float> h((size_t)nd * nd);
+        for (int k = 0; k < 2; k++) {
+            double* out = k == 0 ? a : ainv;
+            if (!out) continue;
+            HIPCK(hipMemcpy(h.data(), (k == 0 ? m.d_A : m.d_Ainv).p, h.size() * sizeof(float), hipMemcpyDeviceToHost));
+            for (size_t i = 0; i < h.size(); i++) out[i] = h[i];
+        }
+        return SFEM_OK;
+    });
+}
+
 int sfem_mg_apply(sfem_ctx* f, const double* r, double* z)
 {
     if (!f || !r || !z) return fail(SFEM_EINVAL, "sfem_mg_apply: null argument");

@@ -204,3 +204,35 @@ def test_multigrid_errors():
         sp.add_mg_level(*lv[0])                                 # per-element moduli
     s.set_multigrid(lv)
     assert len(s.mg_info()) == 2
+
+
+@pytest.mark.parametrize("grid", [(16, 4, 4), (32, 8, 8)])
+def test_dense_coarsest_level(grid):
+    """The directly solved coarsest level (8x2x2: 243 dofs; 16x4x4: 1,275) at
+    rest, where its tangent is linear elasticity: the assembled operator
+    against the oracle's stiffness of the coarse mesh (probed column by
+    column), and the inverse the V-cycle applies against numpy's FP64 inverse
+    of that same assembled operator (pins the blocked FP64 sweep; its FP32
+    storage bounds the difference)"""
+    print(f"case {grid} dense coarsest level", flush=True)
+    m, fixed, f, s = beam(grid)
+    X0, tets, fx = beam_hierarchy(m)[-1][:3]
+    A, Ainv = s.mg_coarse_dense()
+    n = 3 * X0.shape[0]
+    assert A.shape == Ainv.shape == (n, n)
+    p = O.OracleProblem(X0, tets, fx, capi.NEO_HOOKEAN, 1e6, 0.45, 1070.0)
+    free = np.repeat(fx == 0, 3)
+    K = np.eye(n)
+    for j in np.nonzero(free)[0]:
+        e = np.zeros(n)
+        e[j] = 1.0
+        K[free, j] = p.matvec(X0, e.reshape(-1, 3)).reshape(-1)[free]
+    ref = np.linalg.inv(0.5 * (A + A.T))
+    # on the free dofs (the fixed ones are identity rows and columns, which
+    # would dominate the inverse's norm)
+    ff = np.ix_(free, free)
+    assert np.array_equal(Ainv[~free], np.eye(n)[~free])
+    ea, ei = rel2(A[ff], K[ff]), rel2(Ainv[ff], ref[ff])
+    print(f"  n {n}: operator vs oracle {ea:.2e}, inverse vs numpy {ei:.2e}", flush=True)
+    assert ea < 1e-5
+    assert np.array_equal(Ainv, Ainv.T) and ei < 1e-6
