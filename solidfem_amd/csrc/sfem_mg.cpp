@@ -178,7 +178,8 @@ bool mg_prepare(sfem_ctx* f, const float* minv0, bool full)
         f->d_mg_norm.alloc(2 * (size_t)(nlev + 1) * sizeof(double));
     f->mg_minv0 = minv0;
     full = full || !f->mg_ready;
-    // coarse states and block preconditioners, finest to coarsest
+    // coarse states, finest to coarsest; then the coarsest operator's inverse
+    // on the side stream; then the other levels' block preconditioners
     const double* ua = f->d_u64.as<double>();
     const bool dense = nlev > 0 && f->mg[nlev - 1]->dense;
     bool side = false;
@@ -187,32 +188,33 @@ bool mg_prepare(sfem_ctx* f, const float* minv0, bool full)
         sfem_ctx* c = m.c;
         HIPCK(launch_mg_inject(c->nn, m.inj.as<int>(), ua, c->d_u64.as<double>(), c->d_u32.as<float>(), s));
         ua = c->d_u64.as<double>();
-        if (l == nlev - 1 && dense) {
-            if (!full) continue;
-            // the coarsest operator, assembled and inverted on the side stream
-            // once its state is injected; the main stream goes on with the
-            // other levels' estimates and waits for it before the norms are
-            // read back
-            if (!f->mg_side) {
-                HIPCK(hipStreamCreateWithFlags(&f->mg_side, hipStreamNonBlocking));
-                for (hipEvent_t& e : f->mg_ev) HIPCK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-            }
-            hipStream_t ss = f->mg_side;
-            HIPCK(hipEventRecord(f->mg_ev[0], s));
-            HIPCK(hipStreamWaitEvent(ss, f->mg_ev[0], 0));
-            const int n = 3 * (int)c->nn;
-            HIPCK(launch_mg_elem_matrix(c->material, c->ne, m.d_tets4.as<int4>(), c->d_u32.as<float>(), m.d_dminv.as<float>(),
-                                        m.d_cvol.as<float>(), m.d_Ke.as<float>(), ss));
-            HIPCK(launch_mg_dense_assemble((int)c->nn, m.d_inc_ptr.as<int>(), m.d_inc.as<int2>(), m.d_tets4.as<int4>(),
-                                           m.d_Ke.as<float>(), c->d_fixed.as<uint8_t>(), m.d_A.as<float>(), ss));
-            HIPCK(launch_mg_dense_inverse(n, m.d_A.as<float>(), m.d_Ainv.as<float>(), m.d_W.as<double>(),
-                                          m.d_C.as<double>(), m.d_S.as<double>(), ss));
-            HIPCK(hipEventRecord(f->mg_ev[1], ss));
-            side = true;
-        } else {
-            build_jacobi(c);
-        }
     }
+    if (dense && full) {
+        // the coarsest operator, assembled and inverted on the side stream
+        // once its state is injected; the main stream goes on with the other
+        // levels' block preconditioners and estimates and waits for it before
+        // the norms are read back
+        MgLevel& m = *f->mg[nlev - 1];
+        sfem_ctx* c = m.c;
+        if (!f->mg_side) {
+            HIPCK(hipStreamCreateWithFlags(&f->mg_side, hipStreamNonBlocking));
+            for (hipEvent_t& e : f->mg_ev) HIPCK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        }
+        hipStream_t ss = f->mg_side;
+        HIPCK(hipEventRecord(f->mg_ev[0], s));
+        HIPCK(hipStreamWaitEvent(ss, f->mg_ev[0], 0));
+        const int n = 3 * (int)c->nn;
+        HIPCK(launch_mg_elem_matrix(c->material, c->ne, m.d_tets4.as<int4>(), c->d_u32.as<float>(), m.d_dminv.as<float>(),
+                                    m.d_cvol.as<float>(), m.d_Ke.as<float>(), ss));
+        HIPCK(launch_mg_dense_assemble((int)c->nn, m.d_inc_ptr.as<int>(), m.d_inc.as<int2>(), m.d_tets4.as<int4>(),
+                                       m.d_Ke.as<float>(), c->d_fixed.as<uint8_t>(), m.d_A.as<float>(), ss));
+        HIPCK(launch_mg_dense_inverse(n, m.d_A.as<float>(), m.d_Ainv.as<float>(), m.d_W.as<double>(),
+                                      m.d_C.as<double>(), m.d_S.as<double>(), ss));
+        HIPCK(hipEventRecord(f->mg_ev[1], ss));
+        side = true;
+    }
+    for (int l = 0; l < nlev; l++)
+        if (!(l == nlev - 1 && dense)) build_jacobi(f->mg[l]->c);
     // lmax(M A) per level: K unnormalised power steps from a hashed start
     // (the growth of K steps stays far inside FP32 range for lmax of a few
     // units), lmax ~ |v_K| / |v_{K-1}|
@@ -372,7 +374,7 @@ void dense_setup(MgLevel& m, const sfem_ctx* f, const sfem_mesh* mesh, const dou
     m.d_A.alloc(n * n * sizeof(float));
     m.d_Ainv.alloc(n * n * sizeof(float));
     m.d_W.alloc(n * n * sizeof(double));
-    m.d_C.alloc(2 * n * 32 * sizeof(double));
+    m.d_C.alloc(3 * n * 32 * sizeof(double));
     m.d_S.alloc(32 * 32 * sizeof(double));
     m.dense = true;
 }

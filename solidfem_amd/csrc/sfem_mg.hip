@@ -341,9 +341,10 @@ __global__ __launch_bounds__(MG_BLOCK) void k_mg_dense_assemble(int nn, const in
 // sets A_BB <- S, A_iB <- G_i and A_ij <- A_ij - G_i . C_j (i, j outside B).
 // k_mg_sym symmetrises W = (A + A^T) / 2 and copies panel 0's columns to C;
 // per panel, k_mg_panel (one workgroup) sweeps the diagonal block (read from
-// C), k_mg_tiles updates every upper 32 x 32 tile, writes it and its mirror
-// (W stays exactly symmetric) and copies the next panel's columns as it
-// writes them (C is double-buffered, C[p & 1] for panel p, zero past n).
+// C), k_mg_gmat forms G, k_mg_tiles updates every upper 32 x 32 tile, writes
+// it and its mirror (W stays exactly symmetric) and copies the next panel's
+// columns as it writes them (C is double-buffered, C[p & 1] for panel p,
+// zero past n).
 // The sweep runs in FP64 (the coarse operators of nearly incompressible
 // material have condition numbers ~1e6, where an FP32 sweep loses percents);
 // k_mg_negate_f32 stores -W = A^-1 in FP32 for the V-cycle's product.
@@ -413,12 +414,31 @@ __global__ __launch_bounds__(MG_PANEL_BLOCK) void k_mg_panel(int k0, int bp, con
     }
 }
 
+// G = -C S for 32 rows of C per workgroup (rows past n are not written)
+__global__ __launch_bounds__(MG_BLOCK) void k_mg_gmat(int n, const MgT* __restrict__ Cp, const MgT* __restrict__ S,
+                                                      MgT* __restrict__ G)
+{
+    __shared__ MgT sS[MG_TB][MG_TB + 1], sC[MG_TB][MG_TB + 1];
+    const int r0 = blockIdx.x * MG_TB, tid = threadIdx.x;
+    for (int e = tid; e < MG_TB * MG_TB; e += MG_BLOCK) {
+        const int r = e / MG_TB, k = e % MG_TB;
+        sS[r][k] = S[e];
+        sC[r][k] = r0 + r < n ? Cp[(r0 + r) * MG_TB + k] : MgT(0);
+    }
+    __syncthreads();
+    for (int e = tid; e < MG_TB * MG_TB; e += MG_BLOCK) {
+        const int r = e / MG_TB, k = e % MG_TB;
+        MgT g = 0;
+        for (int m = 0; m < MG_TB; m++) g = fma(sC[r][m], sS[m][k], g);
+        if (r0 + r < n) G[(r0 + r) * MG_TB + k] = -g;
+    }
+}
+
 __global__ __launch_bounds__(MG_BLOCK) void k_mg_tiles(int n, int p, int bp, MgT* __restrict__ W,
                                                        const MgT* __restrict__ Cp, MgT* __restrict__ Cn,
-                                                       const MgT* __restrict__ S)
+                                                       const MgT* __restrict__ S, const MgT* __restrict__ G)
 {
-    __shared__ MgT sS[MG_TB][MG_TB + 1], sCI[MG_TB][MG_TB + 1], sCJ[MG_TB][MG_TB + 1];
-    __shared__ MgT sG[MG_TB][MG_TB + 1], sT[MG_TB][MG_TB + 1];
+    __shared__ MgT sA[MG_TB][MG_TB + 1], sB[MG_TB][MG_TB + 1], sT[MG_TB][MG_TB + 1];
     // upper tile (I, J), I <= J, enumerated column by column
     const int t = blockIdx.x;
     int J = (int)((sqrtf(8.f * (float)t + 1.f) - 1.f) * 0.5f);
@@ -427,23 +447,14 @@ __global__ __launch_bounds__(MG_BLOCK) void k_mg_tiles(int n, int p, int bp, MgT
     const int I = t - J * (J + 1) / 2;
     const int r0 = I * MG_TB, c0 = J * MG_TB, kn = (p + 1) * MG_TB;   // kn: the next panel's first column
     const int tid = threadIdx.x;
+    // sA: the diagonal block S (panel tile), G of the columns (panel row of
+    // tiles) or G of the rows; sB: C of the columns (general tiles)
+    const bool diag = I == p && J == p, gcol = I == p;
     for (int e = tid; e < MG_TB * MG_TB; e += MG_BLOCK) {
         const int r = e / MG_TB, k = e % MG_TB;
-        sS[r][k] = S[e];
-        sCI[r][k] = r0 + r < n ? Cp[(r0 + r) * MG_TB + k] : MgT(0);
-        sCJ[r][k] = c0 + r < n ? Cp[(c0 + r) * MG_TB + k] : MgT(0);
-    }
-    __syncthreads();
-    // G of the tile's rows (I != p) or, for the panel's own row of tiles, of
-    // its columns (I == p)
-    const bool gcol = I == p;
-    if (!(I == p && J == p)) {
-        for (int e = tid; e < MG_TB * MG_TB; e += MG_BLOCK) {
-            const int r = e / MG_TB, k = e % MG_TB;
-            MgT g = 0;
-            for (int m = 0; m < MG_TB; m++) g = fma(gcol ? sCJ[r][m] : sCI[r][m], sS[m][k], g);
-            sG[r][k] = -g;
-        }
+        const int ra = gcol ? c0 + r : r0 + r;
+        sA[r][k] = diag ? S[e] : (ra < n ? G[ra * MG_TB + k] : MgT(0));
+        if (!gcol && J != p) sB[r][k] = c0 + r < n ? Cp[(c0 + r) * MG_TB + k] : MgT(0);
     }
     __syncthreads();
     // thread -> row r of the tile, columns cq .. cq + 3
@@ -454,12 +465,12 @@ __global__ __launch_bounds__(MG_BLOCK) void k_mg_tiles(int n, int p, int bp, MgT
         const int cc = cq + q, j = c0 + cc;
         MgT v = 0;
         if (i < n && j < n) {
-            if (I == p && J == p) v = sS[r][cc];
-            else if (J == p) v = sG[r][cc];
-            else if (I == p) v = sG[cc][r];
+            if (diag) v = sA[r][cc];
+            else if (J == p) v = sA[r][cc];
+            else if (I == p) v = sA[cc][r];
             else {
                 MgT s = 0;
-                for (int k = 0; k < bp; k++) s = fma(sG[r][k], sCJ[cc][k], s);
+                for (int k = 0; k < bp; k++) s = fma(sA[r][k], sB[cc][k], s);
                 v = W[(int64_t)i * n + j] - s;
             }
             if (I != J || j >= i) {
@@ -613,12 +624,14 @@ hipError_t launch_mg_dense_inverse(int n, const float* A, float* Ainv, double* W
     const int64_t nsym = (int64_t)n * n > (int64_t)n * MG_TB ? (int64_t)n * n : (int64_t)n * MG_TB;
     hipLaunchKernelGGL(k_mg_sym, dim3(mg_grid(nsym)), dim3(MG_BLOCK), 0, st, n, A, W, C);
     const int nt = (n + MG_TB - 1) / MG_TB;
+    double* G = C + (size_t)2 * n * MG_TB;
     for (int p = 0; p < nt; p++) {
         const int k0 = p * MG_TB, bp = n - k0 < MG_TB ? n - k0 : MG_TB;
         double* Cp = C + (size_t)(p & 1) * n * MG_TB;
         double* Cn = C + (size_t)((p + 1) & 1) * n * MG_TB;
         hipLaunchKernelGGL(k_mg_panel, dim3(1), dim3(MG_PANEL_BLOCK), 0, st, k0, bp, Cp, S);
-        hipLaunchKernelGGL(k_mg_tiles, dim3(nt * (nt + 1) / 2), dim3(MG_BLOCK), 0, st, n, p, bp, W, Cp, Cn, S);
+        hipLaunchKernelGGL(k_mg_gmat, dim3(nt), dim3(MG_BLOCK), 0, st, n, Cp, S, G);
+        hipLaunchKernelGGL(k_mg_tiles, dim3(nt * (nt + 1) / 2), dim3(MG_BLOCK), 0, st, n, p, bp, W, Cp, Cn, S, G);
     }
     hipLaunchKernelGGL(k_mg_negate_f32, dim3(mg_grid((int64_t)n * n)), dim3(MG_BLOCK), 0, st, (int64_t)n * n, W, Ainv);
     return hipGetLastError();

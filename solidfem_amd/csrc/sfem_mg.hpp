@@ -33,8 +33,8 @@ hipError_t launch_mg_elem_matrix(int mat, int64_t ne, const int4* tets, const fl
                                  const float* cvol, float* Ke, hipStream_t st);
 hipError_t launch_mg_dense_assemble(int nn, const int* inc_ptr, const int2* inc, const int4* tets, const float* Ke,
                                     const uint8_t* fixed, float* A, hipStream_t st);
-// Ainv = A^-1 (symmetric A) in FP32, computed in FP64 in W (n x n), C (2 x n x 32)
-// and S (32 x 32) of scratch
+// Ainv = A^-1 (symmetric A) in FP32, computed in FP64 in W (n x n), C (3 x n x 32:
+// two panel-column buffers and G) and S (32 x 32) of scratch
 hipError_t launch_mg_dense_inverse(int n, const float* A, float* Ainv, double* W, double* C, double* S, hipStream_t st);
 // x = Ainv b
 hipError_t launch_mg_dense_solve(int n, const float* Ainv, const float* b, float* x, hipStream_t st);
