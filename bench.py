@@ -197,8 +197,14 @@ def main():
             net.attach(s)
         elif args.precond in ("learned", "multigrid_learned"):
             raise SystemExit(f"no trained weights at {learned.WEIGHTS}")
+    hierarchy = None
     if args.precond in ("multigrid", "multigrid_learned", "all"):
-        s.set_multigrid(beam_hierarchy(m))
+        levels = beam_hierarchy(m)
+        s.set_multigrid(levels)
+        # grid sizes finest to coarsest (each level halves the hex grid); the last is solved directly
+        grids = [tuple(g // 2 ** k for g in grid) for k in range(len(levels) + 1)]
+        hierarchy = (" > ".join("x".join(map(str, g)) for g in grids)
+                     + f" (coarsest solved directly, {3 * levels[-1][0].shape[0]} dofs)")
     dt, runs, x, (ms_elem, n_timed) = leg(main_pc)
     evals = sum(r["hessvec_evals"] + r["force_evals"] for r in runs)
     cg = sum(r["cg_iters"] for r in runs)
@@ -213,6 +219,7 @@ def main():
             "final_rel_residual": runs_l[-1]["res"] / runs_l[-1]["ext_norm"],
             "speedup_vs_jacobi": dt / dtl,
             "displacement_rel_diff_vs_jacobi": float(np.linalg.norm(xl - x) / np.linalg.norm(x - m.X)),
+            **({"hierarchy": hierarchy} if pc in (capi.PC_MULTIGRID, capi.PC_MULTIGRID_LEARNED) else {}),
         }
 
     extra = {}
