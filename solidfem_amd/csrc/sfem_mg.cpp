@@ -99,15 +99,18 @@ Cheb cheb_band(double lmax, double ratio)
 
 // Chebyshev smoothing of K x = b (Saad, Iterative Methods for Sparse Linear
 // Systems, 2nd ed., Alg. 12.1, with the preconditioner M): from x = 0, or from
-// the current x when r already holds b - K x_old and q = K e for x = x_old + e
-void smooth(const LevelView& L, int degree, bool from_zero, const Cheb& ch)
+// the current x when r already holds b - K x_old and q = K e for x = x_old + e.
+// first_done: the first step from x = 0 was made by the restriction that
+// produced b.
+void smooth(const LevelView& L, int degree, bool from_zero, const Cheb& ch, bool first_done = false)
 {
     sfem_ctx* c = L.c;
     const int64_t nn = c->nn;
     const int2* nq = c->d_node_info.as<int2>();
     const float* part = c->d_partial.as<float>();
-    HIPCK(launch_mg_cheb(from_zero ? 0 : 2, nn, L.b, L.r, L.d, L.x, L.q, nq, part, L.minv, 0.f, (float)(1.0 / ch.theta),
-                         c->stream));
+    if (!first_done)
+        HIPCK(launch_mg_cheb(from_zero ? 0 : 2, nn, L.b, L.r, L.d, L.x, L.q, nq, part, L.minv, 0.f,
+                             (float)(1.0 / ch.theta), c->stream));
     double rho = 1.0 / ch.sigma;
     for (int i = 1; i < degree; i++) {
         matvec(L, L.d, L.q);
@@ -118,7 +121,14 @@ void smooth(const LevelView& L, int degree, bool from_zero, const Cheb& ch)
     }
 }
 
-void vcycle(sfem_ctx* f, int l)
+// the Chebyshev band a level smooths with
+Cheb level_band(sfem_ctx* f, int l, double lmax)
+{
+    return cheb_band(lmax, l == (int)f->mg.size() ? MG_COARSE_RATIO : MG_RATIO);
+}
+
+// first_done: the restriction into this level made its first smoothing step
+void vcycle(sfem_ctx* f, int l, bool first_done = false)
 {
     const LevelView L = level(f, l);
     sfem_ctx* c = L.c;
@@ -126,19 +136,23 @@ void vcycle(sfem_ctx* f, int l)
     if (l == nlev) {
         const MgLevel* m = l > 0 ? f->mg[l - 1].get() : nullptr;
         if (m && m->dense) HIPCK(launch_mg_dense_solve(3 * (int)c->nn, m->d_Ainv.as<float>(), L.b, L.x, c->stream));
-        else smooth(L, MG_COARSE_DEGREE, true, cheb_band(L.lmax, MG_COARSE_RATIO));
+        else smooth(L, MG_COARSE_DEGREE, true, level_band(f, l, L.lmax), first_done);
         return;
     }
-    const Cheb ch = cheb_band(L.lmax, MG_RATIO);
-    smooth(L, MG_DEGREE, true, ch);
+    const Cheb ch = level_band(f, l, L.lmax);
+    smooth(L, MG_DEGREE, true, ch, first_done);
     matvec(L, L.d, L.q);
     HIPCK(launch_mg_cheb(3, c->nn, L.b, L.r, L.d, L.x, L.q, c->d_node_info.as<int2>(), c->d_partial.as<float>(), L.minv,
                          0.f, 0.f, c->stream));
     MgLevel& m = *f->mg[l];
     const LevelView C = level(f, l + 1);
+    // a smoothed coarse level's first step is made by the restriction
+    const bool fuse = !(l + 1 == nlev && m.dense);
+    const float c2 = fuse ? (float)(1.0 / level_band(f, l + 1, C.lmax).theta) : 0.f;
     HIPCK(launch_mg_restrict(C.c->nn, m.t_ptr.as<int>(), m.t_row.as<int>(), m.t_val.as<float>(), L.r,
-                             C.c->d_fixed.as<uint8_t>(), const_cast<float*>(C.b), c->stream));
-    vcycle(f, l + 1);
+                             C.c->d_fixed.as<uint8_t>(), const_cast<float*>(C.b), c->stream, fuse ? C.minv : nullptr,
+                             C.r, C.d, C.x, c2));
+    vcycle(f, l + 1, fuse);
     HIPCK(launch_mg_prolong(c->nn, m.p_ptr.as<int>(), m.p_col.as<int>(), m.p_val.as<float>(), C.x,
                             c->d_fixed.as<uint8_t>(), L.d, L.x, c->stream));
     matvec(L, L.d, L.q);

@@ -143,9 +143,14 @@ __global__ __launch_bounds__(MG_BLOCK) void k_mg_prolong(int64_t nn, const int* 
 // bc = P^T r: rows of the transpose (coarse nodes) in increasing fine-node
 // order; 0 at fixed coarse nodes.  Four lanes per coarse node: each sums a
 // contiguous quarter of the row, then (s0 + s1) + (s2 + s3).
+// With minv non-null, the coarse level's first Chebyshev step from x = 0
+// follows on the new right-hand side (k_mg_cheb<0>: rc = bc, dc = c2 M bc,
+// xc = dc), which saves its launch.
 __global__ __launch_bounds__(MG_BLOCK) void k_mg_restrict(int64_t nnc, const int* __restrict__ ptr, const int* __restrict__ row,
                                                           const float* __restrict__ val, const float* __restrict__ r,
-                                                          const uint8_t* __restrict__ fixed, float* __restrict__ bc)
+                                                          const uint8_t* __restrict__ fixed, float* __restrict__ bc,
+                                                          const float* __restrict__ minv, float* __restrict__ rc,
+                                                          float* __restrict__ dc, float* __restrict__ xc, float c2)
 {
     const int64_t t = blockIdx.x * (int64_t)MG_BLOCK + threadIdx.x;
     const int64_t k = t >> 2;
@@ -164,8 +169,23 @@ __global__ __launch_bounds__(MG_BLOCK) void k_mg_restrict(int64_t nnc, const int
     }
     if (k < nnc && lane == 0) {
         const bool f = fixed[k] != 0;
+        float rv[3];
 #pragma unroll
-        for (int c = 0; c < 3; c++) bc[3 * k + c] = f ? 0.f : s[c];
+        for (int c = 0; c < 3; c++) {
+            rv[c] = f ? 0.f : s[c];
+            bc[3 * k + c] = rv[c];
+        }
+        if (minv) {
+            float mr[3];
+            block_apply(minv, k, rv, mr);
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                const float dv = c2 * mr[c];
+                rc[3 * k + c] = rv[c];
+                dc[3 * k + c] = dv;
+                xc[3 * k + c] = dv;
+            }
+        }
     }
 }
 
@@ -546,9 +566,11 @@ hipError_t launch_mg_prolong(int64_t nn, const int* ptr, const int* col, const f
 }
 
 hipError_t launch_mg_restrict(int64_t nnc, const int* ptr, const int* row, const float* val, const float* r,
-                              const uint8_t* fixed, float* bc, hipStream_t st)
+                              const uint8_t* fixed, float* bc, hipStream_t st, const float* minv, float* rc, float* dc,
+                              float* xc, float c2)
 {
-    hipLaunchKernelGGL(k_mg_restrict, dim3(mg_grid(4 * nnc)), dim3(MG_BLOCK), 0, st, nnc, ptr, row, val, r, fixed, bc);
+    hipLaunchKernelGGL(k_mg_restrict, dim3(mg_grid(4 * nnc)), dim3(MG_BLOCK), 0, st, nnc, ptr, row, val, r, fixed, bc, minv,
+                       rc, dc, xc, c2);
     return hipGetLastError();
 }
 

@@ -15,8 +15,11 @@ hipError_t launch_mg_cheb(int mode, int64_t nn, const float* b, float* r, float*
                           const int2* node_q, const float* partial, const float* minv, float c1, float c2, hipStream_t st);
 hipError_t launch_mg_prolong(int64_t nn, const int* ptr, const int* col, const float* val, const float* xc,
                              const uint8_t* fixed, float* e, float* x, hipStream_t st);
+// minv non-null: also the coarse level's first Chebyshev step from x = 0
+// (rc = bc, dc = c2 M bc, xc = dc)
 hipError_t launch_mg_restrict(int64_t nnc, const int* ptr, const int* row, const float* val, const float* r,
-                              const uint8_t* fixed, float* bc, hipStream_t st);
+                              const uint8_t* fixed, float* bc, hipStream_t st, const float* minv = nullptr,
+                              float* rc = nullptr, float* dc = nullptr, float* xc = nullptr, float c2 = 0.f);
 hipError_t launch_mg_inject(int64_t nnc, const int* inj, const double* ua, double* u64, float* u32, hipStream_t st);
 hipError_t launch_mg_power(int64_t nn, const float* q, const int2* node_q, const float* partial, const float* minv, float* y,
                            hipStream_t st);
