@@ -69,7 +69,8 @@ typedef struct {
     int64_t ne;
     const double* X0;       /* nn*3 rest positions (Node::pos0) */
     int64_t nn;
-    const uint8_t* fixed;   /* nn; 1 = Dirichlet node (Node::invMass == 0) */
+    const uint8_t* fixed;   /* nn; 1 = Dirichlet node (Node::invMass == 0); a node
+                               that no element references is held fixed as well */
 } sfem_mesh;
 
 typedef struct {

@@ -582,14 +582,24 @@ int sfem_create(const sfem_mesh* m, const sfem_material* mat, sfem_ctx** out)
             for (int k = 0; k < 4; k++) msum[t[k]] += lm;
         }
         std::vector<double> mass(nn, 0.0);
+        // a node that no element references has neither stiffness nor mass:
+        // it is held like a fixed node (its block would be singular)
         c->fixed_h.assign(m->fixed, m->fixed + nn);
+        {
+            std::vector<uint8_t> used(nn, 0);
+            for (int64_t k = 0; k < 4 * ne; k++) used[m->tets[k]] = 1;
+            for (int64_t i = 0; i < nn; i++)
+                if (!used[i]) c->fixed_h[i] = 1;
+        }
+        sfem_mesh mf = *m;
+        mf.fixed = c->fixed_h.data();
         c->tets_h.assign(m->tets, m->tets + 4 * ne);
         for (int64_t i = 0; i < nn; i++) {
-            if (m->fixed[i]) { c->nfixed++; continue; }
+            if (c->fixed_h[i]) { c->nfixed++; continue; }
             const double inv = msum[i] != 0 ? 1.0 / msum[i] : 1.0;   // Node::invMass default 1
             mass[i] = 1.0 / inv;
         }
-        build_partition(c, m, mu, lam, dminv, vol);
+        build_partition(c, &mf, mu, lam, dminv, vol);
         HIPCK(set_elem_lds_limits(c->blk));
         if (elem_lds_max(c->blk) > 160 * 1024) throw std::runtime_error("element block exceeds LDS");
         std::vector<double> X0(m->X0, m->X0 + 3 * nn);
