@@ -98,6 +98,10 @@ def cpu_baseline(m, fixed, fdead, x, budget_s=10.0):
                       f"({what}, {threads} OpenMP threads, {dt:.1f} s)"}
 
 
+MATERIALS = {"linear": capi.LINEAR, "stvk": capi.STVK, "corotational": capi.COROTATIONAL,
+             "neo-hookean": capi.NEO_HOOKEAN, "corotational-polar": capi.COROTATIONAL_POLAR}
+
+
 def hessvec_only(s, m, args):
     """Kernel-level timing at a smooth deformed state (bending-like field), no
     solve: the run that hardware-counter passes profile."""
@@ -109,7 +113,8 @@ def hessvec_only(s, m, args):
     ms_elem, ms_gather = s.profile_hessvec(args.hessvec_only)
     info = s.info()
     bytes_alg = hessvec_bytes(info["ne"], info["nn"])
-    out = {"kernel": "k_elem_block<0,NEO_HOOKEAN,256>", "reps": args.hessvec_only, "ms_elem": ms_elem,
+    out = {"kernel": f"k_elem_block<0,{args.material.upper().replace('-', '_')},256>", "material": args.material,
+           "reps": args.hessvec_only, "ms_elem": ms_elem,
            "ms_gather": ms_gather, "bytes_alg": bytes_alg, "gbs_alg": bytes_alg / (ms_elem * 1e-3) / 1e9}
     if args.calib_mib:
         import ctypes as C
@@ -141,6 +146,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--hessvec-only", type=int, default=0, metavar="REPS",
                     help="only time REPS Hessian-vector launches at a deformed state (for counter runs)")
+    ap.add_argument("--material", choices=list(MATERIALS), default="neo-hookean",
+                    help="with --hessvec-only: the material whose Hessian-vector kernel is timed")
     ap.add_argument("--calib-mib", type=int, default=0,
                     help="with --hessvec-only: also run one device-to-device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
@@ -150,7 +157,9 @@ def main():
         sys.exit(2)
     grid = tuple(int(v) for v in args.grid.split("x"))
     m, fixed, fdead = workload(grid)
-    s = capi.Solver(m.X, m.tets, fixed, capi.NEO_HOOKEAN, 1e6, 0.45, 1070.0, gravity=(0.0, 0.0, -9.8))
+    if args.material != "neo-hookean" and not args.hessvec_only:
+        ap.error("--material applies to --hessvec-only; the solve legs run the Neo-Hookean workload")
+    s = capi.Solver(m.X, m.tets, fixed, MATERIALS[args.material], 1e6, 0.45, 1070.0, gravity=(0.0, 0.0, -9.8))
     s.set_external_forces(fdead)
     if args.hessvec_only:
         hessvec_only(s, m, args)
