@@ -124,15 +124,18 @@ __device__ __forceinline__ void qr_rot(const T* F, T* R, T* S)
 // for det F > 0) by Newton's iteration X <- (g X + X^-T / g) / 2 with the
 // Frobenius-norm scaling g = (|X^-1|_F / |X|_F)^(1/2) (N. J. Higham, SIAM J.
 // Sci. Stat. Comput. 7 (1986) 1160-1174), until the update is below
-// 2^-20 (FP32) / 2^-45 (FP64) of |X|_F: the iteration converges
-// quadratically, so the returned R is then exact to the working precision.
+// 2^-14 (FP32) / 2^-45 (FP64) of |X|_F.  Near the solution the iteration
+// converges quadratically: the error of the new iterate is about
+// |update|^2 / (2 sigma_min(F)), so an FP32 update below 2^-14 |X|_F already
+// leaves the returned R exact to FP32 rounding (for sigma_min >= 0.05), one
+// iteration earlier than a test at the rounding level would stop.
 // S = sym(R^T F).  An inverted F (det < 0) converges to the orthogonal factor
 // with det R = -1.
 template <typename T>
 __device__ __forceinline__ void polar_rot(const T* F, T* R, T* S)
 {
     constexpr int maxit = sizeof(T) == 4 ? 12 : 20;
-    const T tol2 = sizeof(T) == 4 ? T(3.0 * 0x1p-40) : T(3.0 * 0x1p-90);
+    const T tol2 = sizeof(T) == 4 ? T(3.0 * 0x1p-28) : T(3.0 * 0x1p-90);
     T X[9];
 #pragma unroll
     for (int k = 0; k < 9; k++) X[k] = F[k];
