@@ -172,11 +172,11 @@ def main():
         def one_solve():
             s.reset()
             tot = {"newton_iters": 0, "cg_iters": 0, "hessvec_evals": 0, "force_evals": 0, "converged": 1,
-                   "fallbacks": 0}
+                   "fallbacks": 0, "rejects": 0}
             last = None
             for _ in range(args.load_steps):
                 st = s.step(params)
-                for k in ("newton_iters", "cg_iters", "hessvec_evals", "force_evals", "fallbacks"):
+                for k in ("newton_iters", "cg_iters", "hessvec_evals", "force_evals", "fallbacks", "rejects"):
                     tot[k] += st[k]
                 tot["converged"] &= st["converged"]
                 last = st

@@ -43,14 +43,18 @@ enum { SFEM_ST_STATIC = 0, SFEM_ST_QUASI_STATIC = 1, SFEM_ST_EXPLICIT = 2, SFEM_
      Newton steps, each an unpreconditioned CG from zero to |r.r| <= (1e-3f)^2
      |r0.r0| or inner_iterations; no residual test, no line search.
    CONVERGED: Newton to ||r||_2 <= tol * ||r_ext||_2 (FP64), PCG inner solves
-     to cg_rtol, FP64 back-tracking line search, block-Jacobi fallback.       */
+     to cg_rtol, FP64 back-tracking line search, block-Jacobi fallback; with
+     a learned or multigrid preconditioner Jacobi-PCG is also the judge: a
+     solve whose final FP64 residual misses the target is rerun from its start
+     state with Jacobi-PCG.                                                    */
 enum { SFEM_SOLVE_REFERENCE = 0, SFEM_SOLVE_CONVERGED = 1 };
 
 /* Preconditioners for the inner CG.  MULTIGRID: one V-cycle over the levels
    added by sfem_mg_add_level per CG iteration, Chebyshev smoothing on block
    Jacobi; MULTIGRID_LEARNED: the same with the learned blocks in the finest
-   level's smoother.  Both fall back to block-Jacobi PCG when they do not reach
-   the forcing tolerance. */
+   level's smoother.  A learned or multigrid PCG that breaks down (p.Kp <= 0,
+   r.z <= 0), stagnates, hits its iteration limit or meets a non-finite value
+   is redone with block-Jacobi PCG for that Newton step. */
 enum { SFEM_PC_NONE = 0, SFEM_PC_JACOBI = 1, SFEM_PC_LEARNED = 2, SFEM_PC_MULTIGRID = 3, SFEM_PC_MULTIGRID_LEARNED = 4 };
 
 enum {
@@ -99,24 +103,36 @@ typedef struct {
     int32_t predictor;         /* sfem_step, ST_QUASI_STATIC, CONVERGED only: 1 = start each
                                   load step from the secant extrapolation of the last two
                                   states (dropped if it inverts an element; the step is
-                                  redone from the last state if that solve fails); 0 = from
-                                  the last state, as Step does */
+                                  redone from the last state if an inner solve breaks down
+                                  there or that solve fails); 0 = from the last state, as
+                                  Step does */
     double time_step;          /* sfem_step, ST_EXPLICIT / ST_IMPLICIT: Step(dt)'s dt */
     int32_t substeps;          /* sfem_step, ST_EXPLICIT / ST_IMPLICIT: FemConfig::mNumSubsteps (>= 1) */
+    int32_t pc_newton_cap;     /* CONVERGED, learned / multigrid preconditioners: Newton steps the
+                                  preconditioned attempt gets before the judge reruns the solve
+                                  with Jacobi-PCG (0 = outer_iterations) */
 } sfem_solve_params;
 
 typedef struct {
     int32_t newton_iters;      /* Newton steps taken */
     int32_t cg_iters;          /* total CG iterations */
     int32_t converged;         /* CONVERGED mode: 1 if ||r|| <= max(tol ||r_ext||, abs_tol) */
-    int32_t fallbacks;         /* learned-PC steps rerun with Jacobi, and quasi-static steps
-                                  redone from the last state after the predicted start failed */
+    int32_t fallbacks;         /* linear solves of the learned / multigrid preconditioner redone
+                                  with block Jacobi (breakdown, stagnation, iteration limit or a
+                                  non-finite value), and solves rerun by the judge */
     double res0;               /* ||r||_2 at the start (FP64) */
     double res;                /* ||r||_2 at the end (FP64) */
     double ext_norm;           /* ||r_ext||_2 at the end (FP64) */
     double seconds;            /* device wall time of the solve */
     int64_t hessvec_evals;     /* element Hessian-vector evaluations (ne per apply) */
     int64_t force_evals;       /* element internal-force evaluations (ne per residual) */
+    int32_t reruns;            /* solves rerun by the judge: a learned / multigrid solve whose FP64
+                                  residual missed the target is redone from its start state with
+                                  Jacobi-PCG (counted in fallbacks too) */
+    int32_t rejects;           /* quasi-static load steps whose predicted start was abandoned (an
+                                  inner solve broke down there, or Newton did not improve on the
+                                  last state within 4 steps) and that were redone from the last
+                                  state */
 } sfem_solve_stats;
 
 /* FemPhysicsMatrixFree(tets, nodes, config) (Include/FemPhysicsMatrixFree.h:61-62,
@@ -156,6 +172,52 @@ int sfem_internal_forces(sfem_ctx* ctx, double* f);
    doubles row-major, original element order (FP64; the VTK heat map's cell
    data, FemIO.cpp:317-349) */
 int sfem_element_stress(sfem_ctx* ctx, double* P);
+
+/* Energies at the current state (FP64, device reductions in a fixed order):
+   out[0] = sum_e V_e Psi(F_e), FemPhysicsBase::ComputeElasticEnergy
+            (FemPhysicsBase.cpp:203-212, ElasticEnergy::ComputeElementEnergy
+            ElasticEnergy.cpp:40-112);
+   out[1] = the cable springs' potential, FemPhysicsMatrixFree::ComputeSpringEnergy
+            (FemPhysicsMatrixFree.cpp:1062-1121; not scaled by the load fraction,
+            as there);
+   out[2] = frac sum over free nodes of (m g + f_dead) . u, the work of the dead
+            loads (gravity, sfem_set_external_forces).
+   out[0] - out[2] is the total potential energy of a problem with dead loads
+   only, whose negative gradient is sfem_gradients' residual; the C++ class
+   builds FemPhysicsMatrixFree::ComputeEnergy (:882-916) from these. */
+int sfem_energy(sfem_ctx* ctx, double* out);
+
+/* ElasticEnergy::ComputeForceParamGrads (ElasticEnergy.cpp:284-315) and the
+   density term of FemPhysicsBase::GetForceParamGrads (FemPhysicsBase.cpp:162-187)
+   at the current state: fmu, flam = d f_int / d mu and d f_int / d lambda per node
+   (the internal force with (mu, lambda) = (1, 0) and (0, 1): P is linear in both),
+   frho = (sum of V/4 over the node's elements) g on free nodes; nn*3 doubles each,
+   all rows, original order.  The reference fills these for NEO_HOOKEAN only
+   (ComputeElementStress sets Pmu / Plambda for that material alone, zeros
+   otherwise); here every material gets its derivative. */
+int sfem_force_param_grads(sfem_ctx* ctx, double* fmu, double* flam, double* frho);
+
+/* Cable springs: AddCable (FemPhysicsBase.cpp:152-155) with a cable as
+   CreateCable builds it (FemBody.cpp:31-104): n >= 2 cable nodes, node k inside
+   element elem[k] (original element index) at barycentric weights bary[3k ..
+   3k+2] (the fourth is 1 - their sum), n - 1 springs of rest length
+   rest_length, stiffness and damping, actuation 1.  Their forces (the Bern
+   tension model of ComputeSpringForces, FemPhysicsBase.cpp:214-309, scaled by
+   the load fraction) enter the residual (ComputeGradients :544) and the
+   explicit substep (:377), and are counted in ||r_ext||; as in the reference
+   they add no stiffness to the matrix-free operator.  Free cable nodes
+   (elem < 0: points outside the mesh, which the reference turns into extra
+   massless degrees of freedom, FemPhysicsMatrixFree.cpp:265-294) are refused
+   with SFEM_EINVAL.  *id (nullable) receives the cable's index. */
+int sfem_add_cable(sfem_ctx* ctx, const int32_t* elem, const double* bary, int32_t n, double rest_length, double stiffness,
+                   double damping, int32_t* id);
+/* SetCableActuation (FemPhysicsBase.h:134): the rest length scale of one
+   cable; 0 switches its forces off; an index out of range is ignored */
+int sfem_set_cable_actuation(sfem_ctx* ctx, int32_t cable, double actuation);
+int sfem_num_cables(sfem_ctx* ctx, int32_t* n);
+/* GetCablePosition (FemPhysicsBase.h:133) for all nodes of one cable, n*3 */
+int sfem_get_cable_positions(sfem_ctx* ctx, int32_t cable, double* pos);
+int sfem_clear_cables(sfem_ctx* ctx);
 
 /* Node velocities (Node::vel) of the explicit integrator, nn*3 doubles */
 int sfem_get_velocities(sfem_ctx* ctx, double* v);
@@ -253,7 +315,8 @@ int sfem_mg_apply(sfem_ctx* ctx, const double* r, double* z);
 /* One linear solve K x = b at the current state by PCG with the V-cycle
    (block-Jacobi smoothers) to ||r|| <= rtol ||b|| in at most maxit iterations;
    b, x nn*3 in original order; status 1 converged, 0 iteration limit,
-   3 breakdown (p.Kp <= 0), 4 not finite */
+   3 breakdown (p.Kp <= 0 or r.z <= 0), 4 not finite, 5 stagnation (r.r not
+   halved within 150 iterations); at most 500 iterations */
 int sfem_mg_pcg(sfem_ctx* ctx, const double* b, double* x, double rtol, int32_t maxit, int32_t* iters, int32_t* status);
 
 /* ---------------------------------------------------------------------------

@@ -78,6 +78,9 @@ def lib():
             ("orc_cg", [P, _dp, _dp, _dp, C.c_int, C.c_float, _dp], C.c_int),
             ("orc_newton_cg_ref", [P, _dp, C.c_double, C.c_int, C.c_int, _ip], None),
             ("orc_newton_solve", [P, _dp, C.c_double, C.c_double, C.c_int, C.c_double, C.c_int, _dp], C.c_int),
+            ("orc_cable_forces", [C.c_int64, _ip, _dp, C.c_double, C.c_double, C.c_double, C.c_double, _dp, _dp,
+                                  C.c_double, _dp], None),
+            ("orc_cable_energy", [C.c_int64, _ip, _dp, C.c_double, C.c_double, C.c_double, _dp], C.c_double),
         ]:
             f = getattr(L, name)
             f.argtypes = args
@@ -173,6 +176,67 @@ def energy_density(mat, mu, lam, F):
     return lib().orc_energy_density(mat, mu, lam, _p(_f64(F).reshape(9)))
 
 
+class OracleCable:
+    """One cable for the oracle: n nodes inside elements elem at barycentric
+    weights bary (n, 3) of the mesh tets, springs of rest length `rest`
+    (CreateCable's layout, FemBody.cpp:31-104; AddCable's fields,
+    FemDataStructures.h:212-220)."""
+
+    def __init__(self, tets, elem, bary, rest, stiffness=1000.0, damping=0.0, actuation=1.0):
+        tets = np.asarray(tets).reshape(-1, 4)
+        self.elem = np.ascontiguousarray(elem, dtype=np.int32)
+        self.bary = _f64(bary).reshape(-1, 3)
+        self.node4 = np.ascontiguousarray(tets[self.elem], dtype=np.int32)
+        w3 = 1.0 - self.bary.sum(axis=1)
+        self.w4 = np.ascontiguousarray(np.column_stack([self.bary, w3]))
+        self.rest, self.stiffness, self.damping, self.actuation = float(rest), float(stiffness), float(damping), \
+            float(actuation)
+
+    def forces(self, x, frac=1.0, v=None):
+        """frac * the cable's node forces on the mesh nodes (nn, 3)"""
+        x = _f64(x).reshape(-1, 3)
+        f = np.zeros_like(x)
+        vv = None if v is None else _f64(v).reshape(-1, 3)
+        lib().orc_cable_forces(len(self.elem), _p(self.node4, _ip), _p(self.w4), self.rest, self.actuation,
+                               self.stiffness, self.damping, _p(x), _p(vv), float(frac), _p(f))
+        return f
+
+    def energy(self, x):
+        x = _f64(x).reshape(-1, 3)
+        return lib().orc_cable_energy(len(self.elem), _p(self.node4, _ip), _p(self.w4), self.rest, self.actuation,
+                                      self.stiffness, _p(x))
+
+    def positions(self, x):
+        x = _f64(x).reshape(-1, 3)
+        return np.einsum("nk,nkc->nc", self.w4, x[self.node4])
+
+
+def reference_total_energy(problem, x, frac, gravity, tris=None, pressure=0.0, cables=(), h=0.0):
+    """FemPhysicsMatrixFree::ComputeEnergy (FemPhysicsMatrixFree.cpp:882-916)
+    restated term by term, with its conventions: the elastic energy; minus
+    m frac g_y y over free nodes (the y components only, whatever the gravity
+    axis); minus (0.5 / h^2) m |x - x0|^2 when h > 0; plus |det[p1, p2, p3]|
+    times (float) 1/6 times the pressure for the triangles 0, 3, 6, ... of the traction surface
+    (the loop steps by 3 over the triangle count); plus the cable springs'
+    potential.  `gravity` is the gravity vector (FemPhysicsBase.cpp:30-33)."""
+    x = _f64(x).reshape(-1, 3)
+    val = problem.energy(x)
+    free = problem.fixed == 0
+    m = problem.mass[free]
+    val -= np.sum(m * frac * gravity[1] * x[free, 1])
+    if h:
+        val -= np.sum((0.5 / h / h) * m * np.sum((x[free] - problem.X0[free]) ** 2, axis=1))
+    if tris is not None and pressure:
+        t = np.asarray(tris).reshape(-1, 3)
+        for i in range(0, t.shape[0], 3):
+            p1, p2, p3 = x[t[i, 0]], x[t[i, 1]], x[t[i, 2]]
+            # (1.f / 6.f) is a float constant there
+            val += abs(float(np.float32(1.0) / np.float32(6.0)) * np.dot(p3, np.cross(p1, p2))) * pressure
+    for c in cables:
+        val += c.energy(x)
+    return float(val)
+
+
 class OracleProblem:
     """One mesh + material + loads, in the caller's original node order."""
 
@@ -189,8 +253,9 @@ class OracleProblem:
             # FemPhysicsBase.cpp:40-41
             lam = E * nu / (1.0 + nu) / (1.0 - 2.0 * nu)
             mu = 0.5 * E / (1.0 + nu)
-        self.mu = _f64(np.broadcast_to(mu, (self.ne,)))
-        self.lam = _f64(np.broadcast_to(lam, (self.ne,)))
+        # own, writable copies (the C side reads them through these pointers)
+        self.mu = np.array(np.broadcast_to(mu, (self.ne,)), dtype=np.float64)
+        self.lam = np.array(np.broadcast_to(lam, (self.ne,)), dtype=np.float64)
         self.material = int(material)
         self.DmInv = np.zeros((self.ne, 9))
         self.vol = np.zeros(self.ne)
@@ -234,6 +299,38 @@ class OracleProblem:
 
     def energy(self, x=None):
         return lib().orc_energy(C.byref(self.s), _p(self._x(x)))
+
+    def dead_work(self, x=None, frac=1.0):
+        """frac sum over free nodes of (m g + f_dead) . u, u = x - X0"""
+        u = self._x(x) - self.X0
+        free = self.fixed == 0
+        f = self.mass[:, None] * np.asarray(self.s.grav[:])[None, :]
+        if self.fext is not None:
+            f = f + self.fext
+        return float(frac * np.sum(f[free] * u[free]))
+
+    def force_param_grads(self, x=None):
+        """d f_int / d mu and d f_int / d lambda (nn, 3): the internal force with
+        (mu, lambda) = (1, 0) and (0, 1), since the stress is linear in both
+        (ElasticEnergy::ComputeForceParamGrads, ElasticEnergy.cpp:284-315)"""
+        x = self._x(x)
+        out = []
+        mu0, lam0 = self.mu.copy(), self.lam.copy()
+        try:
+            for a, b in ((1.0, 0.0), (0.0, 1.0)):
+                self.mu[:] = a
+                self.lam[:] = b
+                out.append(self.internal_forces(x))
+        finally:
+            self.mu[:] = mu0
+            self.lam[:] = lam0
+        return out[0], out[1]
+
+    def nodal_volumes(self):
+        """sum over each node's elements of V / 4 (signed rest volumes)"""
+        nv = np.zeros(self.nn)
+        np.add.at(nv, self.tets.reshape(-1), np.repeat(0.25 * self.vol, 4))
+        return nv
 
     def matvec(self, x, d):
         x = self._x(x)
@@ -317,6 +414,12 @@ def reflib():
             ("ref_newton_cg", [C.c_void_p, C.c_double], None),
             ("ref_set_force_fraction", [C.c_void_p, C.c_double], None),
             ("ref_force_fraction", [C.c_void_p], C.c_double),
+            ("ref_energy", [C.c_void_p], C.c_double),
+            ("ref_elastic_energy", [C.c_void_p], C.c_double),
+            ("ref_spring_energy", [C.c_void_p], C.c_double),
+            ("ref_add_cable", [C.c_void_p, _ip, _dp, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double], None),
+            ("ref_set_cable_actuation", [C.c_void_p, C.c_int, C.c_double], None),
+            ("ref_force_param_grads", [C.c_void_p, _dp, _dp, _dp], None),
         ]:
             f = getattr(L, name)
             f.argtypes = args
@@ -401,3 +504,27 @@ class RefProblem:
 
     def newton_cg(self, frac):
         reflib().ref_newton_cg(self.h, float(frac))
+
+    def energy(self):
+        return reflib().ref_energy(self.h)
+
+    def elastic_energy(self):
+        return reflib().ref_elastic_energy(self.h)
+
+    def spring_energy(self):
+        return reflib().ref_spring_energy(self.h)
+
+    def add_cable(self, elem, bary, rest, stiffness=1000.0, damping=0.0, actuation=1.0):
+        elem = np.ascontiguousarray(elem, dtype=np.int32)
+        bary = _f64(bary).reshape(-1, 3)
+        reflib().ref_add_cable(self.h, _p(elem, _ip), _p(bary), len(elem), float(rest), float(stiffness),
+                               float(damping), float(actuation))
+
+    def set_cable_actuation(self, cable, actuation):
+        reflib().ref_set_cable_actuation(self.h, int(cable), float(actuation))
+
+    def force_param_grads(self):
+        """GetForceParamGrads: (d f/d mu, d f/d lambda, d f/d rho) as (nn, 3), zero on fixed rows"""
+        a, b, c = (np.zeros_like(self.X0) for _ in range(3))
+        reflib().ref_force_param_grads(self.h, _p(a), _p(b), _p(c))
+        return a, b, c

@@ -190,4 +190,52 @@ double ref_force_fraction(void* h)
     return ((RefCtx*)h)->phys->GetForceFraction();
 }
 
+// FemPhysicsMatrixFree::ComputeEnergy (:882-916), FemPhysicsBase::ComputeElasticEnergy
+// (FemPhysicsBase.cpp:203-212) and ComputeSpringEnergy (:1113-1121)
+double ref_energy(void* h) { return ((RefCtx*)h)->phys->ComputeEnergy(); }
+double ref_elastic_energy(void* h) { return ((RefCtx*)h)->phys->ComputeElasticEnergy(); }
+double ref_spring_energy(void* h) { return ((RefCtx*)h)->phys->ComputeSpringEnergy(); }
+
+// AddCable (FemPhysicsBase.cpp:152-155, through the base class's public virtual)
+// with n cable nodes inside elements elem[] at barycentric weights bary[3 n]
+void ref_add_cable(void* h, const int32_t* elem, const double* bary, int n, double rest, double stiffness, double damping,
+                   double actuation)
+{
+    RefCtx* c = (RefCtx*)h;
+    Cable cb;
+    cb.mCableNodes.resize(n);
+    cb.mCablePositions.resize(n);
+    for (int i = 0; i < n; i++) {
+        cb.mCableNodes[i].elem = elem[i];
+        cb.mCableNodes[i].bary = Vector3R(bary[3 * i], bary[3 * i + 1], bary[3 * i + 2]);
+    }
+    cb.mActuation = actuation;
+    cb.mCableRestLength = rest;
+    cb.mCableStiffness = stiffness;
+    cb.mCableDamping = damping;
+    static_cast<FemPhysicsBase*>(c->phys)->AddCable(cb);
+}
+
+void ref_set_cable_actuation(void* h, int cable, double actuation)
+{
+    ((RefCtx*)h)->phys->SetCableActuation((uint32)cable, actuation);
+}
+
+// FemPhysicsBase::GetForceParamGrads (FemPhysicsBase.cpp:162-187): free-node
+// vectors in shuffled order, returned here as full original-order arrays
+void ref_force_param_grads(void* h, double* gmu, double* glam, double* grho)
+{
+    RefCtx* c = (RefCtx*)h;
+    EigenVector gm, gl, gr;
+    c->phys->GetForceParamGrads(gm, gl, gr);
+    const EigenVector* src[3] = {&gm, &gl, &gr};
+    double* dst[3] = {gmu, glam, grho};
+    for (int k = 0; k < 3; k++)
+        for (size_t i = 0; i < c->perm.size(); i++)
+            for (int q = 0; q < 3; q++) {
+                const uint32 sidx = c->perm[i];
+                dst[k][3 * i + q] = sidx >= c->nbc ? (*src[k])(3 * (sidx - c->nbc) + q) : 0.0;
+            }
+}
+
 }  // extern "C"

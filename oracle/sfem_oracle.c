@@ -778,3 +778,83 @@ int orc_newton_solve(const orc_problem* p, double* x, double frac, double tol, i
     free(r); free(rt); free(ext); free(dl); free(xt); free(B); free(Binv);
     return ok ? 0 : 1;
 }
+
+/* ------------------------------------------------------------ cable springs */
+/* One cable of n nodes, node j at sum_k w4[4j+k] x[node4[4j+k]] (the
+   barycentric interpolation of FemPhysicsBase::ComputeSpringForces,
+   FemPhysicsBase.cpp:222-245), springs j -> j+1 of rest length `rest`
+   scaled by `act`.  Adds frac * the back-interpolated node forces to f (nn*3,
+   all rows) following :247-300: Bern tension (unilateral, smoothed over
+   |err| <= eps = rest / 100) plus damping along the spring from the
+   interpolated velocities v (may be NULL: zero).  No forces when act == 0
+   (:216-217).  Restated, not copied; pinned by the compiled reference
+   (FemPhysicsMatrixFree::ComputeGradients with AddCable) and by
+   f = -frac dE/dx of orc_cable_energy. */
+static void cable_point(const int32_t* node4, const double* w4, int64_t j, const double* x, double* p)
+{
+    for (int q = 0; q < 3; q++) p[q] = 0.0;
+    for (int k = 0; k < 4; k++) {
+        const double wk = w4[4 * j + k];
+        for (int q = 0; q < 3; q++) p[q] += wk * x[3 * (int64_t)node4[4 * j + k] + q];
+    }
+}
+
+void orc_cable_forces(int64_t n, const int32_t* node4, const double* w4, double rest, double act, double k,
+                      double damp, const double* x, const double* v, double frac, double* f)
+{
+    if (n < 2 || act == 0.0) return;
+    double* fs = (double*)calloc((size_t)(3 * (n - 1)), sizeof(double));
+    const double eps = rest * 0.01;
+    for (int64_t j = 0; j + 1 < n; j++) {
+        double pa[3], pb[3], va[3] = {0, 0, 0}, vb[3] = {0, 0, 0};
+        cable_point(node4, w4, j, x, pa);
+        cable_point(node4, w4, j + 1, x, pb);
+        if (v) { cable_point(node4, w4, j, v, va); cable_point(node4, w4, j + 1, v, vb); }
+        double y[3] = {pb[0] - pa[0], pb[1] - pa[1], pb[2] - pa[2]};
+        double len = sqrt(y[0] * y[0] + y[1] * y[1] + y[2] * y[2]);
+        double dir[3] = {y[0] / len, y[1] / len, y[2] / len};
+        double err = len - rest * act;
+        double tension = -2.0 * k * err;
+        if (err < -eps) tension = 0.0;
+        else if (err >= -eps && err <= eps) tension = -k * (0.5 * err * err / eps + err + 0.5 * eps);
+        double dv = (vb[0] - va[0]) * dir[0] + (vb[1] - va[1]) * dir[1] + (vb[2] - va[2]) * dir[2];
+        for (int q = 0; q < 3; q++) fs[3 * j + q] = (tension - damp * dv) * dir[q];
+    }
+    for (int64_t j = 0; j < n; j++) {
+        double nf[3] = {0, 0, 0};
+        for (int q = 0; q < 3; q++) {
+            if (j + 1 < n) nf[q] -= fs[3 * j + q];
+            if (j > 0) nf[q] += fs[3 * (j - 1) + q];
+        }
+        for (int kk = 0; kk < 4; kk++) {
+            const double wk = w4[4 * j + kk];
+            for (int q = 0; q < 3; q++) f[3 * (int64_t)node4[4 * j + kk] + q] += wk * (frac * nf[q]);
+        }
+    }
+    free(fs);
+}
+
+/* FemPhysicsMatrixFree::ComputeSpringEnergy (FemPhysicsMatrixFree.cpp:1062-1111)
+   for one cable, restated: sum over springs of k (err^2 + eps^2 / 3), 0 below
+   -eps, the smoothed cubic k/2 (err^3 / (3 eps) + err^2 + eps err + eps^2 / 3)
+   on |err| <= eps; 0 when act == 0 */
+double orc_cable_energy(int64_t n, const int32_t* node4, const double* w4, double rest, double act, double k,
+                        const double* x)
+{
+    if (n < 2 || act == 0.0) return 0.0;
+    const double eps = rest * 0.01;
+    double e = 0.0;
+    for (int64_t j = 0; j + 1 < n; j++) {
+        double pa[3], pb[3];
+        cable_point(node4, w4, j, x, pa);
+        cable_point(node4, w4, j + 1, x, pb);
+        double y[3] = {pb[0] - pa[0], pb[1] - pa[1], pb[2] - pa[2]};
+        double len = sqrt(y[0] * y[0] + y[1] * y[1] + y[2] * y[2]);
+        double err = len - rest * act;
+        double pot = k * (err * err + eps * eps / 3.0);
+        if (err < -eps) pot = 0.0;
+        else if (err >= -eps && err <= eps) pot = 0.5 * k * (err * err * err / 3.0 / eps + err * err + eps * err + eps * eps / 3.0);
+        e += pot;
+    }
+    return e;
+}

@@ -25,7 +25,8 @@ EXPORTS = [
     "sfem_create", "sfem_destroy", "sfem_last_error", "sfem_set_positions", "sfem_get_positions",
     "sfem_set_load_fraction", "sfem_get_load_fraction", "sfem_set_pressure_surface",
     "sfem_set_external_forces", "sfem_internal_forces", "sfem_element_stress", "sfem_get_velocities",
-    "sfem_gradients", "sfem_matvec",
+    "sfem_gradients", "sfem_matvec", "sfem_energy", "sfem_force_param_grads",
+    "sfem_add_cable", "sfem_set_cable_actuation", "sfem_num_cables", "sfem_get_cable_positions", "sfem_clear_cables",
     "sfem_block_diagonal", "sfem_solve", "sfem_step", "sfem_set_precond_blocks", "sfem_info",
     "sfem_reset", "sfem_profile_hessvec", "sfem_time_hessvec",
     "sfem_mg_add_level", "sfem_mg_clear", "sfem_mg_info", "sfem_mg_apply", "sfem_mg_pcg", "sfem_mg_coarse_dense",
@@ -55,14 +56,14 @@ class SolveParams(C.Structure):
                 ("inner_iterations", C.c_int32), ("tol", C.c_double), ("cg_rtol", C.c_double),
                 ("line_search", C.c_int32), ("check_every", C.c_int32), ("sim_type", C.c_int32),
                 ("force_step", C.c_double), ("abs_tol", C.c_double), ("predictor", C.c_int32),
-                ("time_step", C.c_double), ("substeps", C.c_int32)]
+                ("time_step", C.c_double), ("substeps", C.c_int32), ("pc_newton_cap", C.c_int32)]
 
 
 class SolveStats(C.Structure):
     _fields_ = [("newton_iters", C.c_int32), ("cg_iters", C.c_int32), ("converged", C.c_int32),
                 ("fallbacks", C.c_int32), ("res0", C.c_double), ("res", C.c_double),
                 ("ext_norm", C.c_double), ("seconds", C.c_double), ("hessvec_evals", C.c_int64),
-                ("force_evals", C.c_int64)]
+                ("force_evals", C.c_int64), ("reruns", C.c_int32), ("rejects", C.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -100,6 +101,14 @@ def lib():
             "sfem_get_velocities": ([V, _dp], C.c_int),
             "sfem_gradients": ([V, _dp], C.c_int),
             "sfem_matvec": ([V, _dp, _dp], C.c_int),
+            "sfem_energy": ([V, _dp], C.c_int),
+            "sfem_force_param_grads": ([V, _dp, _dp, _dp], C.c_int),
+            "sfem_add_cable": ([V, _ip, _dp, C.c_int32, C.c_double, C.c_double, C.c_double, C.POINTER(C.c_int32)],
+                               C.c_int),
+            "sfem_set_cable_actuation": ([V, C.c_int32, C.c_double], C.c_int),
+            "sfem_num_cables": ([V, C.POINTER(C.c_int32)], C.c_int),
+            "sfem_get_cable_positions": ([V, C.c_int32, _dp], C.c_int),
+            "sfem_clear_cables": ([V], C.c_int),
             "sfem_block_diagonal": ([V, _dp], C.c_int),
             "sfem_solve": ([V, C.POINTER(SolveParams), C.POINTER(SolveStats)], C.c_int),
             "sfem_step": ([V, C.POINTER(SolveParams), C.POINTER(SolveStats)], C.c_int),
@@ -152,9 +161,9 @@ def _p(a, t=_dp):
 
 def solve_params(mode=SOLVE_CONVERGED, precond=PC_JACOBI, outer=50, inner=20000, tol=1e-8, cg_rtol=1e-1,
                  line_search=1, check_every=16, sim_type=ST_STATIC, force_step=0.1, abs_tol=0.0, predictor=1,
-                 time_step=0.016, substeps=1):
+                 time_step=0.016, substeps=1, pc_newton_cap=0):
     return SolveParams(mode, precond, outer, inner, tol, cg_rtol, line_search, check_every, sim_type, force_step,
-                       abs_tol, predictor, time_step, substeps)
+                       abs_tol, predictor, time_step, substeps, pc_newton_cap)
 
 
 class Solver:
@@ -257,6 +266,43 @@ class Solver:
         q = np.zeros_like(d)
         _check(lib().sfem_matvec(self.h, _p(d), _p(q)))
         return q
+
+    def energy(self):
+        """(elastic energy, cable spring energy, work of the dead loads) at the current state (sfem_energy)"""
+        out = np.zeros(3)
+        _check(lib().sfem_energy(self.h, _p(out)))
+        return tuple(out)
+
+    def force_param_grads(self):
+        """(d f/d mu, d f/d lambda, d (m g)/d rho), each (nn, 3), original order"""
+        a, b, c = (np.zeros((self.nn, 3)) for _ in range(3))
+        _check(lib().sfem_force_param_grads(self.h, _p(a), _p(b), _p(c)))
+        return a, b, c
+
+    def add_cable(self, elem, bary, rest_length, stiffness=1000.0, damping=0.0):
+        """a cable of len(elem) nodes embedded in elements elem at barycentric weights bary (n, 3) -> its index"""
+        elem = np.ascontiguousarray(elem, dtype=np.int32)
+        bary = np.ascontiguousarray(bary, dtype=np.float64).reshape(-1, 3)
+        i = C.c_int32()
+        _check(lib().sfem_add_cable(self.h, _p(elem, _ip), _p(bary), len(elem), float(rest_length), float(stiffness),
+                                    float(damping), C.byref(i)))
+        return i.value
+
+    def set_cable_actuation(self, cable, actuation):
+        _check(lib().sfem_set_cable_actuation(self.h, int(cable), float(actuation)))
+
+    def num_cables(self):
+        n = C.c_int32()
+        _check(lib().sfem_num_cables(self.h, C.byref(n)))
+        return n.value
+
+    def cable_positions(self, cable, n):
+        pos = np.zeros((n, 3))
+        _check(lib().sfem_get_cable_positions(self.h, int(cable), _p(pos)))
+        return pos
+
+    def clear_cables(self):
+        _check(lib().sfem_clear_cables(self.h))
 
     def block_diagonal(self):
         B = np.zeros((self.nn, 9))

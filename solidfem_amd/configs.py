@@ -116,15 +116,17 @@ def run(cfg, precond=capi.PC_JACOBI, tol=1e-8, grid=None, max_newton=50, log=Non
     t0 = time.perf_counter()
     for k in range(steps):
         st = s.step(p, allow_notconv=True)
-        out["steps"].append({k: st[k] for k in ("newton_iters", "cg_iters", "fallbacks", "converged", "res", "ext_norm",
+        out["steps"].append({k: st[k] for k in ("newton_iters", "cg_iters", "fallbacks", "rejects", "converged", "res",
+                                                "ext_norm",
                                                 "seconds")})
         if log:
             log(f"workload {cfg} load step {k + 1}/{steps}: newton {st['newton_iters']} cg {st['cg_iters']} "
-                f"fallbacks {st['fallbacks']} rel residual {st['res'] / st['ext_norm']:.2e} ({st['seconds']:.2f} s)")
+                f"fallbacks {st['fallbacks']} rejected predictions {st['rejects']} rel residual {st['res'] / st['ext_norm']:.2e} ({st['seconds']:.2f} s)")
     out["wall_s"] = time.perf_counter() - t0
     out["newton_iters"] = sum(x["newton_iters"] for x in out["steps"])
     out["cg_iters"] = sum(x["cg_iters"] for x in out["steps"])
     out["fallbacks"] = sum(x["fallbacks"] for x in out["steps"])
+    out["rejects"] = sum(x["rejects"] for x in out["steps"])
     out["converged"] = all(x["converged"] for x in out["steps"])
     out["final_rel_residual"] = out["steps"][-1]["res"] / out["steps"][-1]["ext_norm"]
     return out, d, s

@@ -48,6 +48,8 @@ FemPhysicsMatrixFree::FemPhysicsMatrixFree(const std::vector<Tet>& tetrahedra, c
     mNumBCs = k;
     for (size_t i = 0; i < nn; i++)
         if (!mFixed[i]) mReshuffleMap[i] = k++;
+    mShuffledToOriginal.resize(nn);
+    for (size_t i = 0; i < nn; i++) mShuffledToOriginal[mReshuffleMap[i]] = (uint32)i;
     // rest geometry kept on the host for the geometric getters (:126-139)
     const size_t ne = mTets.size();
     mVol.resize(ne);
@@ -66,6 +68,10 @@ FemPhysicsMatrixFree::FemPhysicsMatrixFree(const std::vector<Tet>& tetrahedra, c
         for (int q = 0; q < 9; q++) mDmInv[9 * e + q] = v[q] / det;
         mVol[e] = det / 6.0;
     }
+    // lumped masses rho V / 4 per corner (:141-146)
+    mMass.assign(nn, 0.0);
+    for (size_t e = 0; e < ne; e++)
+        for (int c = 0; c < 4; c++) mMass[mTets[e].idx[c]] += 0.25 * mDensity * mVol[e];
     Create(nullptr, nullptr);
 }
 
@@ -105,6 +111,13 @@ void FemPhysicsMatrixFree::Create(const double* youngE, const double* poissonE)
     mHostDirty = true;
     PushPositions();
     Check(sfem_set_load_fraction(mCtx, mForceFraction), "sfem_set_load_fraction");
+    // loads and cables carry over to the new context
+    if (!mPressureTris.empty()) {
+        std::vector<int32_t> t(mPressureTris.begin(), mPressureTris.end());
+        Check(sfem_set_pressure_surface(mCtx, t.data(), (int64_t)t.size() / 3, mAppliedPressure), "sfem_set_pressure_surface");
+    }
+    if (!mExternalForces.empty()) Check(sfem_set_external_forces(mCtx, mExternalForces.data()), "sfem_set_external_forces");
+    for (const Cable& cb : mCables) PushCable(cb);
     // the multigrid levels follow the new material: uniform moduli re-add them
     // without coarse moduli; per-element moduli need the caller's coarse means
     if (youngE) mMultigrid.clear();
@@ -280,12 +293,15 @@ void FemPhysicsMatrixFree::SetBoundaryConditionsSurface(const std::vector<uint32
 {
     std::vector<int32_t> t(triangleList.begin(), triangleList.end());
     Check(sfem_set_pressure_surface(mCtx, t.data(), (int64_t)t.size() / 3, pressure), "sfem_set_pressure_surface");
+    mPressureTris = triangleList;
+    mAppliedPressure = pressure;
 }
 
 void FemPhysicsMatrixFree::SetExternalForces(const std::vector<Vector3R>& forces)
 {
     if (forces.empty()) {
         Check(sfem_set_external_forces(mCtx, nullptr), "sfem_set_external_forces");
+        mExternalForces.clear();
         return;
     }
     if (forces.size() != mNodes.size()) throw std::runtime_error("SetExternalForces: one force per node expected");
@@ -293,6 +309,170 @@ void FemPhysicsMatrixFree::SetExternalForces(const std::vector<Vector3R>& forces
     for (size_t i = 0; i < forces.size(); i++)
         for (int c = 0; c < 3; c++) f[3 * i + c] = forces[i][c];
     Check(sfem_set_external_forces(mCtx, f.data()), "sfem_set_external_forces");
+    mExternalForces = f;
+}
+
+Vector3R FemPhysicsMatrixFree::GetVelocity(uint32 idx) const
+{
+    std::vector<double> v(3 * mNodes.size());
+    Check(sfem_get_velocities(mCtx, v.data()), "sfem_get_velocities");
+    return Vector3R(v[3 * idx], v[3 * idx + 1], v[3 * idx + 2]);
+}
+
+Vector3R FemPhysicsMatrixFree::GetTotalDisplacement(uint32 i) const
+{
+    PullPositions();
+    const Node& n = mNodes[mShuffledToOriginal[i + mNumBCs]];
+    return n.pos - n.pos0;
+}
+
+real FemPhysicsMatrixFree::ComputeElasticEnergy()
+{
+    PushPositions();
+    double e[3];
+    Check(sfem_energy(mCtx, e), "sfem_energy");
+    return e[0];
+}
+
+real FemPhysicsMatrixFree::ComputeSpringEnergy()
+{
+    PushPositions();
+    double e[3];
+    Check(sfem_energy(mCtx, e), "sfem_energy");
+    return e[1];
+}
+
+real FemPhysicsMatrixFree::ComputeEnergy()
+{
+    PushPositions();
+    Check(sfem_set_load_fraction(mCtx, mForceFraction), "sfem_set_load_fraction");
+    double e[3];
+    Check(sfem_energy(mCtx, e), "sfem_energy");
+    PullPositions();
+    real val = e[0];
+    const real invHSqr = mTimeStep != 0 ? 0.5f / mTimeStep / mTimeStep : 0;
+    for (uint32 s = mNumBCs; s < GetNumNodes(); s++) {
+        const Node& n = mNodes[mShuffledToOriginal[s]];
+        const real mass = mMass[mShuffledToOriginal[s]];
+        val -= mass * mForceFraction * mGravity.y * n.pos.y;
+        val -= invHSqr * mass * (n.pos - n.pos0).LengthSquared();
+    }
+    for (size_t i = 0; i < mPressureTris.size() / 3; i += 3) {
+        const Vector3R& p1 = mNodes[mPressureTris[3 * i]].pos;
+        const Vector3R& p2 = mNodes[mPressureTris[3 * i + 1]].pos;
+        const Vector3R& p3 = mNodes[mPressureTris[3 * i + 2]].pos;
+        const real vol = (1.f / 6.f) * triple(p1, p2, p3);
+        val += std::fabs(vol) * mAppliedPressure;
+    }
+    return val + e[1];
+}
+
+void FemPhysicsMatrixFree::GetForceParamGrads(std::vector<real>& gradMu, std::vector<real>& gradLambda,
+                                              std::vector<real>& gradRho)
+{
+    PushPositions();
+    const size_t nn = mNodes.size();
+    std::vector<double> fm(3 * nn), fl(3 * nn), fr(3 * nn);
+    Check(sfem_force_param_grads(mCtx, fm.data(), fl.data(), fr.data()), "sfem_force_param_grads");
+    const size_t nf = GetNumFreeNodes();
+    gradMu.assign(3 * nf, 0);
+    gradLambda.assign(3 * nf, 0);
+    gradRho.assign(3 * nf, 0);
+    for (size_t i = 0; i < nn; i++) {
+        if (mFixed[i]) continue;
+        const size_t s = mReshuffleMap[i] - mNumBCs;
+        for (int c = 0; c < 3; c++) {
+            gradMu[3 * s + c] = fm[3 * i + c];
+            gradLambda[3 * s + c] = fl[3 * i + c];
+            gradRho[3 * s + c] = fr[3 * i + c];
+        }
+    }
+}
+
+void FemPhysicsMatrixFree::GetHessian(std::vector<real>& K) const
+{
+    PushPositions();
+    const size_t nn = mNodes.size(), nf = GetNumFreeNodes(), n = 3 * nf;
+    // node adjacency (nodes sharing an element), then a greedy distance-2
+    // coloring: two nodes of one color share no neighbour, so one operator
+    // apply per color and component recovers every column of that color
+    std::vector<std::vector<uint32>> adj(nn);
+    for (const Tet& t : mTets)
+        for (int a = 0; a < 4; a++)
+            for (int b = 0; b < 4; b++) adj[t.idx[a]].push_back(t.idx[b]);
+    for (auto& a : adj) {
+        std::sort(a.begin(), a.end());
+        a.erase(std::unique(a.begin(), a.end()), a.end());
+    }
+    std::vector<int> color(nn, -1);
+    int ncolors = 0;
+    for (size_t i = 0; i < nn; i++) {
+        if (mFixed[i]) continue;
+        std::vector<char> used(ncolors + 1, 0);
+        for (uint32 j : adj[i])
+            for (uint32 k : adj[j])
+                if (color[k] >= 0) used[color[k]] = 1;
+        int c = 0;
+        while (c < ncolors && used[c]) c++;
+        color[i] = c;
+        ncolors = std::max(ncolors, c + 1);
+    }
+    K.assign(n * n, 0);
+    std::vector<double> d(3 * nn), q(3 * nn);
+    for (int c = 0; c < ncolors; c++)
+        for (int comp = 0; comp < 3; comp++) {
+            std::fill(d.begin(), d.end(), 0.0);
+            for (size_t j = 0; j < nn; j++)
+                if (color[j] == c) d[3 * j + comp] = 1.0;
+            Check(sfem_matvec(mCtx, d.data(), q.data()), "sfem_matvec");
+            // column (j, comp) of K: rows of the free nodes adjacent to j
+            for (size_t j = 0; j < nn; j++) {
+                if (color[j] != c) continue;
+                const size_t col = 3 * (mReshuffleMap[j] - mNumBCs) + comp;
+                for (uint32 i : adj[j]) {
+                    if (mFixed[i]) continue;
+                    const size_t row = 3 * (mReshuffleMap[i] - mNumBCs);
+                    for (int r = 0; r < 3; r++) K[(row + r) * n + col] = q[3 * (size_t)i + r];
+                }
+            }
+        }
+}
+
+void FemPhysicsMatrixFree::PushCable(const Cable& cable)
+{
+    const size_t n = cable.mCableNodes.size();
+    std::vector<int32_t> elem(n);
+    std::vector<double> bary(3 * n);
+    for (size_t i = 0; i < n; i++) {
+        elem[i] = cable.mCableNodes[i].elem;
+        for (int c = 0; c < 3; c++) bary[3 * i + c] = cable.mCableNodes[i].bary[c];
+    }
+    int32_t id = 0;
+    Check(sfem_add_cable(mCtx, elem.data(), bary.data(), (int32_t)n, cable.mCableRestLength, cable.mCableStiffness,
+                         cable.mCableDamping, &id),
+          "sfem_add_cable");
+    if (cable.mActuation != 1.0) Check(sfem_set_cable_actuation(mCtx, id, cable.mActuation), "sfem_set_cable_actuation");
+}
+
+void FemPhysicsMatrixFree::AddCable(const Cable& cable)
+{
+    PushCable(cable);
+    mCables.push_back(cable);
+}
+
+Vector3R FemPhysicsMatrixFree::GetCablePosition(uint32 cable, uint32 node) const
+{
+    PushPositions();
+    std::vector<double> pos(3 * mCables.at(cable).mCableNodes.size());
+    Check(sfem_get_cable_positions(mCtx, (int32_t)cable, pos.data()), "sfem_get_cable_positions");
+    return Vector3R(pos[3 * node], pos[3 * node + 1], pos[3 * node + 2]);
+}
+
+void FemPhysicsMatrixFree::SetCableActuation(uint32 cable, real actuation)
+{
+    if (cable >= mCables.size()) return;   // FemPhysicsBase.h:134
+    mCables[cable].mActuation = actuation;
+    Check(sfem_set_cable_actuation(mCtx, (int32_t)cable, actuation), "sfem_set_cable_actuation");
 }
 
 void FemPhysicsMatrixFree::SetPreconditionerBlocks(const std::vector<float>& minv9)
@@ -380,6 +560,7 @@ void FemPhysicsMatrixFree::Step(real dt)
         p.time_step = dt;
         p.substeps = std::max(1, mNumSteps);
         mForceFraction = 1;
+        mTimeStep = dt / p.substeps;
         const int rc = sfem_step(mCtx, &p, &mStats);
         if (rc != SFEM_OK && rc != SFEM_ENOTCONV) Check(rc, "sfem_step");
         mDeviceDirty = true;

@@ -223,6 +223,8 @@ public:
         d["cg_iters"] = s.cg_iters;
         d["converged"] = s.converged;
         d["fallbacks"] = s.fallbacks;
+        d["reruns"] = s.reruns;
+        d["rejects"] = s.rejects;
         d["res0"] = s.res0;
         d["res"] = s.res;
         d["ext_norm"] = s.ext_norm;

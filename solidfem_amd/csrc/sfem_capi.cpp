@@ -15,6 +15,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <numeric>
 #include <string>
@@ -272,6 +273,17 @@ void set_time_step(sfem_ctx* c, double h)
 // unpredicted start before it is abandoned
 constexpr int PRED_PATIENCE = 4;
 
+// SFEM_TRACE=1 in the environment: one line per Newton step of the converged
+// solve on stderr (residual, forcing, inner solves and their CG status)
+bool trace_on()
+{
+    static const bool on = [] {
+        const char* v = std::getenv("SFEM_TRACE");
+        return v && v[0] == '1';
+    }();
+    return on;
+}
+
 // ---------------------------------------------------------------- operations
 
 // residual r = f_int + ext on free rows (0 on fixed) at displacement u (device)
@@ -287,8 +299,64 @@ void residual(sfem_ctx* c, const double* u, double* r)
                           c->ntri ? c->d_ftri.as<double>() : nullptr, c->d_fixed.as<uint8_t>(),
                           dyn ? c->d_vel.as<double>() : nullptr, dyn ? c->d_u0step.as<double>() : nullptr,
                           dyn ? 1.0 / c->tstep : 0.0, c->d_ext64.as<double>(), c->stream));
+    // cable spring forces (ComputeGradients adds them, FemPhysicsMatrixFree.cpp:544)
+    if (c->cab.n)
+        HIPCK(launch_cable_forces(c->cab, c->d_X0.as<double>(), u, c->d_vel.as<double>(), c->d_cab_fs.as<double>(), c->frac,
+                                  c->d_fixed.as<uint8_t>(), c->d_ext64.as<double>(), 1, c->stream));
     double* part = reinterpret_cast<double*>(c->d_partial.as<char>() + (size_t)c->nrows_partial * 9 * sizeof(float));
     HIPCK(launch_force(c->material, c->blk, c->shr, u, r, part, c->d_ext64.as<double>(), 1, c->stream));
+}
+
+// The cables' device arrays (CableArgs) from c->cables: per cable node its
+// element's 4 nodes and 4 weights (w3 = 1 - w0 - w1 - w2, FemPhysicsBase.cpp:
+// 233-238); per spring j -> j+1 of one cable (rest length, actuation,
+// stiffness, damping), stiffness 0 after a cable's last node and for a cable
+// with zero actuation (no force, :216-217); the touched mesh nodes with their
+// attachments in cable-node order (the deterministic back-interpolation)
+void rebuild_cables(sfem_ctx* c)
+{
+    std::vector<int> node4;
+    std::vector<double> w4, spring;
+    for (const sfem_ctx::Cable& cb : c->cables) {
+        const size_t n = cb.elem.size();
+        for (size_t i = 0; i < n; i++) {
+            const int e = cb.elem[i];
+            for (int k = 0; k < 4; k++) node4.push_back(c->tets_h[4 * (size_t)e + k]);
+            const double w0 = cb.bary[3 * i], w1 = cb.bary[3 * i + 1], w2 = cb.bary[3 * i + 2];
+            w4.insert(w4.end(), {w0, w1, w2, 1.0 - w0 - w1 - w2});
+            const bool live = i + 1 < n && cb.actuation != 0.0;
+            spring.insert(spring.end(), {cb.rest, cb.actuation, live ? cb.stiffness : 0.0, cb.damping});
+        }
+    }
+    const int n = (int)(node4.size() / 4);
+    std::vector<std::vector<int2>> att(c->nn);
+    for (int j = 0; j < n; j++)
+        for (int k = 0; k < 4; k++) att[node4[4 * j + k]].push_back(make_int2(j, 4 * j + k));
+    std::vector<int> touched, tptr{0};
+    std::vector<int2> tent;
+    for (int64_t i = 0; i < c->nn; i++)
+        if (!att[i].empty()) {
+            touched.push_back((int)i);
+            tent.insert(tent.end(), att[i].begin(), att[i].end());
+            tptr.push_back((int)tent.size());
+        }
+    c->d_cab_node.upload(node4);
+    c->d_cab_w.upload(w4);
+    c->d_cab_spring.upload(spring);
+    c->d_cab_touched.upload(touched);
+    c->d_cab_tptr.upload(tptr);
+    c->d_cab_tent.upload(tent);
+    c->d_cab_fs.alloc(3 * (size_t)std::max(n, 1) * sizeof(double));
+    zero<double>(c->d_fcable, 3 * (size_t)c->nn, c->stream);
+    HIPCK(hipStreamSynchronize(c->stream));
+    c->cab.n = n;
+    c->cab.node = c->d_cab_node.as<int4>();
+    c->cab.w = c->d_cab_w.as<double>();
+    c->cab.spring = c->d_cab_spring.as<double>();
+    c->cab.ntouched = (int)touched.size();
+    c->cab.touched = c->d_cab_touched.as<int>();
+    c->cab.tptr = c->d_cab_tptr.as<int>();
+    c->cab.tent = c->d_cab_tent.as<int2>();
 }
 
 }  // namespace
@@ -436,7 +504,9 @@ int solve_converged(sfem_ctx* c, const sfem_solve_params* p, sfem_solve_stats* s
     Dots d0 = dots3(c, c->d_r64.as<double>(), c->d_ext64.as<double>(), nullptr);
     double rn = std::sqrt(d0.aa), en = std::sqrt(d0.bb);
     st->res0 = rn;
-    const int maxn = p->outer_iterations > 0 ? p->outer_iterations : 50;
+    const bool adaptive = pc_kind == SFEM_PC_LEARNED || multigrid;
+    int maxn = p->outer_iterations > 0 ? p->outer_iterations : 50;
+    if (adaptive && p->pc_newton_cap > 0) maxn = std::min(maxn, p->pc_newton_cap);
     const double eta_max = p->cg_rtol > 0 ? p->cg_rtol : 1e-1;
     double eta = eta_max, rn_prev = 0;
     const double abs_tol = p->abs_tol > 0 ? p->abs_tol : 0.0;
@@ -461,16 +531,31 @@ int solve_converged(sfem_ctx* c, const sfem_solve_params* p, sfem_solve_stats* s
         if (multigrid) {
             if (mg_prepare(c, minv, it == 0)) cg = mg_pcg(c, p->inner_iterations, eta * eta, &st->hessvec_evals);
         } else {
-            cg = pcg(c, minv, pc_kind == SFEM_PC_NONE ? 0 : 1, p->inner_iterations, 1, eta * eta, 1, p->check_every,
-                     &st->hessvec_evals);
+            // learned blocks: also stop at r.z <= 0 (blocks not positive definite)
+            cg = pcg(c, minv, pc_kind == SFEM_PC_NONE ? 0 : 1, p->inner_iterations, 1, eta * eta, learned ? 2 : 1,
+                     p->check_every, &st->hessvec_evals);
         }
         st->cg_iters += cg.iters;
-        if ((learned || multigrid) && cg.status != 1) {
-            // the learned / multigrid PCG did not reach its forcing: redo with block Jacobi
+        if (c->abort_after > 0 && cg.status == CG_BREAKDOWN) {
+            // a predicted start where the tangent (or its preconditioner) is not
+            // positive definite: abandon it at once, the step is redone from the
+            // last state (sfem_step)
+            if (trace_on()) std::fprintf(stderr, "sfem newton %d: predicted start rejected (breakdown)\n", it);
+            c->pred_aborted = true;
+            break;
+        }
+        if (trace_on())
+            std::fprintf(stderr, "sfem newton %d: |r| %.3e target %.3e eta %.2e pc %d: cg %d status %d\n", it, rn,
+                         target(en), eta, pc_kind, cg.iters, cg.status);
+        if ((learned || multigrid) && cg.status != CG_CONVERGED) {
+            // the learned / multigrid PCG did not reach its forcing (iteration
+            // limit, breakdown, stagnation or a non-finite value): redo this
+            // Newton step's linear solve with block Jacobi
             st->fallbacks++;
             build_jacobi(c);
             cg = pcg(c, c->d_minv.as<float>(), 1, p->inner_iterations, 1, eta * eta, 1, p->check_every, &st->hessvec_evals);
             st->cg_iters += cg.iters;
+            if (trace_on()) std::fprintf(stderr, "sfem   fallback jacobi: cg %d status %d\n", cg.iters, cg.status);
         }
         const double R0 = dots3(c, c->d_r64.as<double>(), nullptr, c->d_x32.as<float>()).da;
         double alpha = 1.0;
@@ -510,7 +595,10 @@ int solve_converged(sfem_ctx* c, const sfem_solve_params* p, sfem_solve_stats* s
         st->newton_iters++;
         if (!std::isfinite(rn)) break;
         // a predicted start that is not paying off (see sfem_step)
-        if (c->abort_after > 0 && st->newton_iters >= c->abort_after && rn > c->abort_rn && rn > target(en)) break;
+        if (c->abort_after > 0 && st->newton_iters >= c->abort_after && rn > c->abort_rn && rn > target(en)) {
+            c->pred_aborted = true;
+            break;
+        }
     }
     // u32 must match the accepted state
     HIPCK(launch_to_f32(n3, c->d_u64.as<double>(), c->d_u32.as<float>(), c->stream));
@@ -610,6 +698,7 @@ int sfem_create(const sfem_mesh* m, const sfem_material* mat, sfem_ctx** out)
         hipStream_t s = c->stream;
         zero<double>(c->d_u64, n3, s);
         zero<double>(c->d_uprev, n3, s);
+        zero<double>(c->d_ustart, n3, s);
         zero<double>(c->d_vel, n3, s);
         zero<double>(c->d_u0step, n3, s);
         zero<float>(c->d_mdiag32, (size_t)nn, s);
@@ -814,6 +903,140 @@ int sfem_matvec(sfem_ctx* c, const double* d, double* Kd)
     });
 }
 
+int sfem_energy(sfem_ctx* c, double* out)
+{
+    if (!c || !out) return fail(SFEM_EINVAL, "sfem_energy: null argument");
+    return guard_call("sfem_energy", [&]() -> int {
+        double* scal = c->d_scal.as<double>();
+        double* part = c->d_partials.as<double>();
+        HIPCK(launch_energy(c->material, c->blk, c->d_u64.as<double>(), scal, S_NT, part, c->stream));
+        if (c->cab.n) HIPCK(launch_cable_energy(c->cab, c->d_X0.as<double>(), c->d_u64.as<double>(), scal, S_NT2, part, c->stream));
+        else HIPCK(hipMemsetAsync(scal + S_NT2, 0, sizeof(double), c->stream));
+        HIPCK(launch_dead_work(c->nn, c->d_mass.as<double>(), c->has_fdead ? c->d_fdead.as<double>() : nullptr, c->gravity,
+                               c->frac, c->d_fixed.as<uint8_t>(), c->d_u64.as<double>(), scal, S_NT3, part, c->stream));
+        HIPCK(hipMemcpyAsync(out, scal + S_NT, 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPCK(hipStreamSynchronize(c->stream));
+        return SFEM_OK;
+    });
+}
+
+int sfem_force_param_grads(sfem_ctx* c, double* fmu, double* flam, double* frho)
+{
+    if (!c || !fmu || !flam || !frho) return fail(SFEM_EINVAL, "sfem_force_param_grads: null argument");
+    return guard_call("sfem_force_param_grads", [&]() -> int {
+        const size_t n3 = 3 * (size_t)c->nn;
+        double* part = reinterpret_cast<double*>(c->d_partial.as<char>() + (size_t)c->nrows_partial * 9 * sizeof(float));
+        BlockArgs a = c->blk;   // elastic forces only, whatever the time step
+        a.mdiag32 = nullptr;
+        a.mdiag64 = nullptr;
+        for (int mode = 1; mode <= 2; mode++) {
+            a.coef_mode = mode;
+            HIPCK(launch_force(c->material, a, c->shr, c->d_u64.as<double>(), c->d_f64tmp.as<double>(), part, nullptr, 0,
+                               c->stream));
+            HIPCK(hipMemcpyAsync(mode == 1 ? fmu : flam, c->d_f64tmp.p, n3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        }
+        // d(m g)/d rho = (sum of V / 4 over the node's elements) g on free nodes
+        // (FemPhysicsBase.cpp:171-186)
+        std::vector<double> X0(n3);
+        HIPCK(hipMemcpyAsync(X0.data(), c->d_X0.p, n3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPCK(hipStreamSynchronize(c->stream));
+        std::vector<double> nv(c->nn, 0.0);
+        for (int64_t e = 0; e < c->ne; e++) {
+            const int32_t* t = &c->tets_h[4 * e];
+            double D[9];
+            for (int k = 1; k < 4; k++)
+                for (int r = 0; r < 3; r++) D[3 * r + (k - 1)] = X0[3 * (size_t)t[k] + r] - X0[3 * (size_t)t[0] + r];
+            const double det = D[0] * (D[4] * D[8] - D[5] * D[7]) - D[1] * (D[3] * D[8] - D[5] * D[6]) +
+                               D[2] * (D[3] * D[7] - D[4] * D[6]);
+            for (int k = 0; k < 4; k++) nv[t[k]] += 0.25 * (det / 6.0);
+        }
+        for (int64_t i = 0; i < c->nn; i++)
+            for (int k = 0; k < 3; k++) frho[3 * i + k] = c->fixed_h[i] ? 0.0 : nv[i] * c->gravity[k];
+        return SFEM_OK;
+    });
+}
+
+int sfem_add_cable(sfem_ctx* c, const int32_t* elem, const double* bary, int32_t n, double rest_length, double stiffness,
+                   double damping, int32_t* id)
+{
+    if (!c || !elem || !bary || n < 2 || !(rest_length > 0)) return fail(SFEM_EINVAL, "sfem_add_cable: bad argument");
+    for (int32_t i = 0; i < n; i++) {
+        if (elem[i] < 0)
+            return fail(SFEM_EINVAL, "sfem_add_cable: free cable nodes (outside the mesh, elem < 0) are not supported: "
+                                     "they would add degrees of freedom with no elastic stiffness");
+        if (elem[i] >= c->ne) return fail(SFEM_EINVAL, "sfem_add_cable: element index out of range");
+    }
+    return guard_call("sfem_add_cable", [&]() -> int {
+        sfem_ctx::Cable cb;
+        cb.elem.assign(elem, elem + n);
+        cb.bary.assign(bary, bary + 3 * (size_t)n);
+        cb.rest = rest_length;
+        cb.stiffness = stiffness;
+        cb.damping = damping;
+        cb.actuation = 1.0;
+        c->cables.push_back(cb);
+        if (id) *id = (int32_t)c->cables.size() - 1;
+        rebuild_cables(c);
+        return SFEM_OK;
+    });
+}
+
+int sfem_set_cable_actuation(sfem_ctx* c, int32_t cable, double actuation)
+{
+    if (!c) return fail(SFEM_EINVAL, "sfem_set_cable_actuation: null context");
+    // SetCableActuation (FemPhysicsBase.h:134) ignores an index out of range
+    if (cable < 0 || (size_t)cable >= c->cables.size()) return SFEM_OK;
+    return guard_call("sfem_set_cable_actuation", [&]() -> int {
+        c->cables[cable].actuation = actuation;
+        rebuild_cables(c);
+        return SFEM_OK;
+    });
+}
+
+int sfem_num_cables(sfem_ctx* c, int32_t* n)
+{
+    if (!c || !n) return fail(SFEM_EINVAL, "sfem_num_cables: null argument");
+    *n = (int32_t)c->cables.size();
+    return SFEM_OK;
+}
+
+int sfem_get_cable_positions(sfem_ctx* c, int32_t cable, double* pos)
+{
+    if (!c || !pos || cable < 0 || (size_t)cable >= c->cables.size())
+        return fail(SFEM_EINVAL, "sfem_get_cable_positions: bad argument");
+    return guard_call("sfem_get_cable_positions", [&]() -> int {
+        const sfem_ctx::Cable& cb = c->cables[cable];
+        const size_t n3 = 3 * (size_t)c->nn;
+        std::vector<double> X0(n3), u(n3);
+        HIPCK(hipStreamSynchronize(c->stream));
+        HIPCK(hipMemcpy(X0.data(), c->d_X0.p, n3 * sizeof(double), hipMemcpyDeviceToHost));
+        HIPCK(hipMemcpy(u.data(), c->d_u64.p, n3 * sizeof(double), hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < cb.elem.size(); i++) {
+            const double w[4] = {cb.bary[3 * i], cb.bary[3 * i + 1], cb.bary[3 * i + 2],
+                                 1.0 - cb.bary[3 * i] - cb.bary[3 * i + 1] - cb.bary[3 * i + 2]};
+            for (int q = 0; q < 3; q++) {
+                double v = 0.0;
+                for (int k = 0; k < 4; k++) {
+                    const size_t nd = (size_t)c->tets_h[4 * (size_t)cb.elem[i] + k];
+                    v += w[k] * (X0[3 * nd + q] + u[3 * nd + q]);
+                }
+                pos[3 * i + q] = v;
+            }
+        }
+        return SFEM_OK;
+    });
+}
+
+int sfem_clear_cables(sfem_ctx* c)
+{
+    if (!c) return fail(SFEM_EINVAL, "sfem_clear_cables: null context");
+    return guard_call("sfem_clear_cables", [&]() -> int {
+        c->cables.clear();
+        rebuild_cables(c);
+        return SFEM_OK;
+    });
+}
+
 int sfem_block_diagonal(sfem_ctx* c, double* B)
 {
     if (!c || !B) return fail(SFEM_EINVAL, "sfem_block_diagonal: null argument");
@@ -902,7 +1125,36 @@ int sfem_solve(sfem_ctx* c, const sfem_solve_params* p, sfem_solve_stats* st)
     return guard_call("sfem_solve", [&]() -> int {
         HIPCK(hipStreamSynchronize(c->stream));
         auto t0 = std::chrono::steady_clock::now();
-        int rc = p->mode == SFEM_SOLVE_REFERENCE ? solve_reference(c, p, st) : solve_converged(c, p, st);
+        int rc;
+        if (p->mode == SFEM_SOLVE_REFERENCE) {
+            rc = solve_reference(c, p, st);
+        } else {
+            const bool adaptive = p->precond == SFEM_PC_LEARNED || p->precond == SFEM_PC_MULTIGRID ||
+                                  p->precond == SFEM_PC_MULTIGRID_LEARNED;
+            const size_t n3b = 3 * (size_t)c->nn * sizeof(double);
+            if (adaptive) HIPCK(hipMemcpyAsync(c->d_ustart.p, c->d_u64.p, n3b, hipMemcpyDeviceToDevice, c->stream));
+            c->pred_aborted = false;
+            rc = solve_converged(c, p, st);
+            if (adaptive && rc == SFEM_ENOTCONV && !c->pred_aborted) {
+                // the judge: the learned / multigrid solve's true FP64 residual missed
+                // the target; the solve reruns from its start state with Jacobi-PCG
+                const sfem_solve_stats first = *st;
+                HIPCK(hipMemcpyAsync(c->d_u64.p, c->d_ustart.p, n3b, hipMemcpyDeviceToDevice, c->stream));
+                HIPCK(launch_to_f32(3 * c->nn, c->d_u64.as<double>(), c->d_u32.as<float>(), c->stream));
+                sfem_solve_params pj = *p;
+                pj.precond = SFEM_PC_JACOBI;
+                pj.pc_newton_cap = 0;
+                std::memset(st, 0, sizeof *st);
+                rc = solve_converged(c, &pj, st);
+                st->newton_iters += first.newton_iters;
+                st->cg_iters += first.cg_iters;
+                st->fallbacks += first.fallbacks + 1;
+                st->reruns += first.reruns + 1;
+                st->hessvec_evals += first.hessvec_evals;
+                st->force_evals += first.force_evals;
+                st->res0 = first.res0;
+            }
+        }
         HIPCK(hipStreamSynchronize(c->stream));
         st->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         if (rc == SFEM_ENOTCONV)
@@ -979,7 +1231,9 @@ int sfem_step(sfem_ctx* c, const sfem_solve_params* p, sfem_solve_stats* st)
                 rc = sfem_solve(c, p, st);
                 st->newton_iters += first.newton_iters;
                 st->cg_iters += first.cg_iters;
-                st->fallbacks += first.fallbacks + 1;
+                st->fallbacks += first.fallbacks;
+                st->rejects += first.rejects + 1;
+                st->reruns += first.reruns;
                 st->seconds += first.seconds;
                 st->hessvec_evals += first.hessvec_evals;
                 st->force_evals += first.force_evals;
@@ -1006,8 +1260,13 @@ int sfem_step(sfem_ctx* c, const sfem_solve_params* p, sfem_solve_stats* st)
             a.mdiag32 = nullptr;
             a.mdiag64 = nullptr;
             for (int k = 0; k < p->substeps; k++) {
+                // SubStep (FemPhysicsMatrixFree.cpp:371-387): elastic plus cable spring forces
+                if (c->cab.n)
+                    HIPCK(launch_cable_forces(c->cab, c->d_X0.as<double>(), c->d_u64.as<double>(), c->d_vel.as<double>(),
+                                              c->d_cab_fs.as<double>(), c->frac, c->d_fixed.as<uint8_t>(),
+                                              c->d_fcable.as<double>(), 0, c->stream));
                 HIPCK(launch_force(c->material, a, c->shr, c->d_u64.as<double>(), c->d_f64tmp.as<double>(), part,
-                                   nullptr, 0, c->stream));
+                                   c->cab.n ? c->d_fcable.as<double>() : nullptr, 0, c->stream));
                 HIPCK(launch_explicit(c->nn, c->d_u64.as<double>(), c->d_vel.as<double>(), c->d_f64tmp.as<double>(),
                                       c->d_mass.as<double>(), c->d_fixed.as<uint8_t>(), c->gravity, h, c->d_u32.as<float>(),
                                       c->stream));
@@ -1047,6 +1306,8 @@ int sfem_step(sfem_ctx* c, const sfem_solve_params* p, sfem_solve_stats* st)
             tot.newton_iters += st->newton_iters;
             tot.cg_iters += st->cg_iters;
             tot.fallbacks += st->fallbacks;
+            tot.reruns += st->reruns;
+            tot.rejects += st->rejects;
             tot.seconds += st->seconds;
             tot.hessvec_evals += st->hessvec_evals;
             tot.force_evals += st->force_evals;

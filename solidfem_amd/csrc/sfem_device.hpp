@@ -425,6 +425,49 @@ __device__ __forceinline__ void stress(const T* F, T cmu, T clam, T* P)
     }
 }
 
+// Strain energy of one element, volume-scaled like the stresses (cmu = V mu,
+// clam = V lambda; Psi is linear in mu, lambda), ElasticEnergy::
+// ComputeElementEnergy (ElasticEnergy.cpp:40-112): LINEAR, COROTATIONAL (on
+// the stretch S of the reference's QR rotation) and STVK mu e:e + lam/2 tr(e)^2
+// with the Cauchy (LINEAR, COROTATIONAL) or Green (STVK) strain; NEO_HOOKEAN
+// mu/2 (I1 - 3) - mu log J + lam/2 (log J)^2; COROTATIONAL_POLAR the
+// corotational form on the polar stretch, mu |F - R|^2 + lam/2 tr(S - I)^2.
+template <typename T, int MAT>
+__device__ __forceinline__ T energy(const T* F, T cmu, T clam)
+{
+    if (MAT == NEO_HOOKEAN || MAT == NEO_HOOKEAN_COARSE) {
+        T I1 = T(0);
+#pragma unroll
+        for (int k = 0; k < 9; k++) I1 += F[k] * F[k];
+        const T J = F[0] * (F[4] * F[8] - F[5] * F[7]) - F[1] * (F[3] * F[8] - F[5] * F[6]) +
+                    F[2] * (F[3] * F[7] - F[4] * F[6]);
+        const T lJ = dlog(J);
+        return T(0.5) * cmu * (I1 - T(3)) - cmu * lJ + T(0.5) * clam * lJ * lJ;
+    }
+    T e[9];
+    if (MAT == STVK) {
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++)
+                e[3 * i + j] = T(0.5) * (F[i] * F[j] + F[3 + i] * F[3 + j] + F[6 + i] * F[6 + j] - (i == j ? T(1) : T(0)));
+    } else {
+        T R[9], S[9];
+        const T* U = F;
+        if (MAT == COROTATIONAL) { qr_rot(F, R, S); U = S; }
+        if (MAT == COROTATIONAL_POLAR) { polar_rot(F, R, S); U = S; }
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++) e[3 * i + j] = T(0.5) * (U[3 * i + j] + U[3 * j + i]) - (i == j ? T(1) : T(0));
+    }
+    T ee = T(0);
+#pragma unroll
+    for (int k = 0; k < 9; k++) ee += e[k] * e[k];
+    const T tr = e[0] + e[4] + e[8];
+    return cmu * ee + T(0.5) * clam * tr * tr;
+}
+
 // F = I + grad(u)
 template <typename T>
 __device__ __forceinline__ void defgrad(const T* g, const T u[4][3], T* F)

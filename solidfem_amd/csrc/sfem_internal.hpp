@@ -123,8 +123,11 @@ struct sfem_ctx {
     double frac_prev = 0.0;
     bool has_prev = false;
     // a predicted solve stops after abort_after Newton steps if ||r|| > abort_rn
+    // (pred_aborted records that it did: the judge does not rerun such a solve,
+    // the step is redone from the last state instead)
     int abort_after = 0;
     double abort_rn = 0.0;
+    bool pred_aborted = false;
     double pressure = 0.0;
     int64_t ntri = 0;
     bool has_fdead = false;
@@ -146,7 +149,7 @@ struct sfem_ctx {
     DevBuf d_partial;        // nrows_partial * 9 floats (also used as FP64 x3)
 
     // node data
-    DevBuf d_uprev, d_vel, d_u0step, d_mdiag32, d_mdiag64, d_eorig, d_X0, d_mass, d_fixed, d_fdead;
+    DevBuf d_ustart, d_uprev, d_vel, d_u0step, d_mdiag32, d_mdiag64, d_eorig, d_X0, d_mass, d_fixed, d_fdead;
     DevBuf d_u64, d_ut64, d_u32;
     DevBuf d_r64, d_ext64, d_f64tmp;
     DevBuf d_b32, d_x32, d_r32, d_z32, d_p32, d_q32, d_d32, d_minv, d_minv_learned, d_bd32;
@@ -154,6 +157,18 @@ struct sfem_ctx {
 
     // reductions
     DevBuf d_scal, d_partials, d_done;
+
+    // cable springs (FemPhysicsBase::mCables): the cables as added, and their
+    // concatenation on the device (CableArgs); d_cab_fs = spring forces,
+    // d_fcable = cable node forces on the mesh for the explicit integrator
+    struct Cable {
+        std::vector<int> elem;       // element (original index) of each cable node
+        std::vector<double> bary;    // 3 barycentric weights per cable node
+        double rest = 0.0, stiffness = 0.0, damping = 0.0, actuation = 1.0;
+    };
+    std::vector<Cable> cables;
+    sfem::CableArgs cab{};
+    DevBuf d_cab_node, d_cab_w, d_cab_spring, d_cab_touched, d_cab_tptr, d_cab_tent, d_cab_fs, d_fcable;
 
     // learned preconditioner: node graph (CSR, no self loops), network, pairs
     DevBuf d_adj_ptr, d_adj_idx;

@@ -158,6 +158,15 @@ __global__ __launch_bounds__(BE) void k_elem_block(BlockArgs a, const typename K
             for (int k = 0; k < 9; k++) g[k] = (T)a.dminv64[k * st + e];
             cmu = (T)a.coef64[e];
             clam = (T)a.coef64[st + e];
+            if (KIND == 1 && a.coef_mode) {
+                // d f / d mu or d f / d lambda: coefficients (V, 0) or (0, V),
+                // V = 1 / (6 det Dm^-1) (ElasticEnergy.cpp:284-315)
+                const T dg = g[0] * (g[4] * g[8] - g[5] * g[7]) - g[1] * (g[3] * g[8] - g[5] * g[6]) +
+                             g[2] * (g[3] * g[7] - g[4] * g[6]);
+                const T V = T(1) / (T(6) * dg);
+                cmu = a.coef_mode == 1 ? V : T(0);
+                clam = a.coef_mode == 2 ? V : T(0);
+            }
         }
         li = a.lidx[e];
         lp4 = a.lperm[e];
@@ -470,6 +479,8 @@ __global__ __launch_bounds__(256) void k_cg_init_finish(const double* __restrict
         scal[S_RR] = tot[1];
         scal[S_RR0] = tot[1];
         scal[S_ITER] = 0;
+        scal[S_RR_BEST] = tot[1];
+        scal[S_IT_BEST] = 0;
         *done = (tot[0] == 0.0) ? 1 : 0;    // IterativeSolvers.h:156-157
     }
 }
@@ -583,11 +594,14 @@ __global__ __launch_bounds__(CG_BLOCK) void k_cg_update(int64_t n3, float* __res
 
 // beta = rz' / rz (every workgroup); test; p = z + beta p, one node per step
 // test 0: the reference criterion |r.z| <= eps^2 |r0.z0| (IterativeSolvers.h:180-185)
-// test 1: true residual ||r||^2 <= tol^2 ||r0||^2
+// test 1: true residual ||r||^2 <= tol^2 ||r0||^2; with rz_guard, r.z <= 0
+//   is a breakdown (the preconditioner is not positive definite here), and
+//   with stall_window > 0 an r.r that has not halved within that many
+//   iterations of its last reference value stops the solve (stagnation)
 template <int PC>
 __global__ __launch_bounds__(CG_P_BLOCK) void k_cg_p(int64_t n3, const float* __restrict__ z, const float* __restrict__ r,
                                               float* __restrict__ p, double* scal, const double* __restrict__ part, int n_part,
-                                              int* done, int test_kind, double tol2)
+                                              int* done, int test_kind, double tol2, int rz_guard, int stall_window)
 {
     const int stop = vload(done);    // tested after the loads and the reduction are issued
     const int64_t nn = n3 / 3;
@@ -605,16 +619,28 @@ __global__ __launch_bounds__(CG_P_BLOCK) void k_cg_p(int64_t n3, const float* __
     const double rz1 = t[0], rr = t[1];
     const bool conv = test_kind == 0 ? (fabs(rz1) <= tol2 * fabs(scal[S_RZ0])) : (rr <= tol2 * scal[S_RR0]);
     const bool bad = !(rr == rr) || !(rz1 == rz1);
+    const bool indef = rz_guard && !conv && !bad && !(rz1 > 0.0);
     const double beta = rz1 / scal[S_RZ_PREV];
     if (blockIdx.x == 0 && threadIdx.x == 0) {
+        const double it = scal[S_ITER] + 1.0;
         scal[S_RZ] = rz1;
         scal[S_RR] = rr;
         scal[S_BETA] = beta;
-        scal[S_ITER] += 1.0;
-        if (conv) *done = 1;
-        else if (bad) *done = 4;
+        scal[S_ITER] = it;
+        if (conv) *done = CG_CONVERGED;
+        else if (bad) *done = CG_NOT_FINITE;
+        else if (indef) *done = CG_BREAKDOWN;
+        else if (stall_window > 0) {
+            // only this thread reads and writes the stagnation reference
+            if (rr <= 0.5 * scal[S_RR_BEST]) {
+                scal[S_RR_BEST] = rr;
+                scal[S_IT_BEST] = it;
+            } else if (it - scal[S_IT_BEST] >= (double)stall_window) {
+                *done = CG_STAGNATED;
+            }
+        }
     }
-    if (conv || bad) return;
+    if (conv || bad || indef) return;
     const float fb = (float)beta;
     for (; i < nn; i += stride) {
 #pragma unroll
@@ -705,6 +731,165 @@ __global__ __launch_bounds__(256) void k_elem_stress(BlockArgs a, const double* 
     double* o = out + 9 * (size_t)eorig[s];
 #pragma unroll
     for (int k = 0; k < 9; k++) o[k] = P[k] * invV;
+}
+
+// the element's displacement gradient input: Dm^-1 and the 4 corner
+// displacements of sorted element s (FP64)
+__device__ __forceinline__ void elem_state64(const BlockArgs& a, int64_t s, const double* __restrict__ u, double* g,
+                                             double uu[4][3])
+{
+    const size_t st = (size_t)a.ne_pad;
+#pragma unroll
+    for (int k = 0; k < 9; k++) g[k] = a.dminv64[k * st + s];
+    const uint2 li = a.lidx[s];
+    const int n0 = a.bnode_ptr[s / BLOCK_E];
+    const int l[4] = {(int)(li.x & 0xffffu), (int)(li.x >> 16), (int)(li.y & 0xffffu), (int)(li.y >> 16)};
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const uint32_t node = a.bnodes[n0 + l[k]] & NODE_MASK;
+#pragma unroll
+        for (int c = 0; c < 3; c++) uu[k][c] = u[3 * (size_t)node + c];
+    }
+}
+
+// per-workgroup partial of sum_e V_e Psi(F_e) (FP64; grid-stride over the
+// sorted elements, then a fixed tree: deterministic)
+template <int MAT>
+__global__ __launch_bounds__(256) void k_elem_energy(BlockArgs a, const double* __restrict__ u, double* __restrict__ part)
+{
+    double v[1] = {0.0};
+    const size_t st = (size_t)a.ne_pad;
+    for (int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; s < a.ne; s += (int64_t)gridDim.x * blockDim.x) {
+        double g[9], uu[4][3], F[9];
+        elem_state64(a, s, u, g, uu);
+        defgrad(g, uu, F);
+        v[0] += energy<double, MAT>(F, a.coef64[s], a.coef64[st + s]);
+    }
+    block_partials<1, 256>(v, part);
+}
+
+// per-workgroup partial of frac sum_free (m g + f_dead) . u
+__global__ __launch_bounds__(256) void k_dead_work(int64_t nn, const double* __restrict__ mass, const double* __restrict__ fdead,
+                                                   double gx, double gy, double gz, double frac,
+                                                   const uint8_t* __restrict__ fixed, const double* __restrict__ u,
+                                                   double* __restrict__ part)
+{
+    double v[1] = {0.0};
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nn; i += (int64_t)gridDim.x * blockDim.x) {
+        if (fixed[i]) continue;
+        double f[3] = {mass[i] * gx, mass[i] * gy, mass[i] * gz};
+        if (fdead) {
+#pragma unroll
+            for (int c = 0; c < 3; c++) f[c] += fdead[3 * i + c];
+        }
+        v[0] += frac * (f[0] * u[3 * i] + f[1] * u[3 * i + 1] + f[2] * u[3 * i + 2]);
+    }
+    block_partials<1, 256>(v, part);
+}
+
+// cable node j at x = X0 + u (and its velocity): the barycentric combination
+// of its element's corners (FemPhysicsBase.cpp:222-245)
+__device__ __forceinline__ void cable_point(const CableArgs& c, int j, const double* __restrict__ X0,
+                                            const double* __restrict__ u, const double* __restrict__ vel, double* p, double* v)
+{
+    const int4 nd = c.node[j];
+    const int ids[4] = {nd.x, nd.y, nd.z, nd.w};
+#pragma unroll
+    for (int q = 0; q < 3; q++) { p[q] = 0.0; v[q] = 0.0; }
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const double wk = c.w[4 * j + k];
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+            p[q] += wk * (X0[3 * (size_t)ids[k] + q] + u[3 * (size_t)ids[k] + q]);
+            if (vel) v[q] += wk * vel[3 * (size_t)ids[k] + q];
+        }
+    }
+}
+
+// spring j -> j+1 (FemPhysicsBase.cpp:247-268): tension of the Bern model
+// (unilateral, smoothed over |err| <= eps = rest / 100) plus damping along it
+__global__ __launch_bounds__(256) void k_cable_springs(CableArgs c, const double* __restrict__ X0, const double* __restrict__ u,
+                                                       const double* __restrict__ vel, double* __restrict__ fs)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= c.n) return;
+    const double L = c.spring[4 * j], act = c.spring[4 * j + 1], k = c.spring[4 * j + 2], damp = c.spring[4 * j + 3];
+    double f[3] = {0.0, 0.0, 0.0};
+    if (k != 0.0) {
+        double pa[3], va[3], pb[3], vb[3];
+        cable_point(c, j, X0, u, vel, pa, va);
+        cable_point(c, j + 1, X0, u, vel, pb, vb);
+        const double y[3] = {pb[0] - pa[0], pb[1] - pa[1], pb[2] - pa[2]};
+        const double len = sqrt(y[0] * y[0] + y[1] * y[1] + y[2] * y[2]);
+        const double inv = 1.0 / len;
+        const double dir[3] = {inv * y[0], inv * y[1], inv * y[2]};
+        const double eps = L * 0.01;
+        const double err = len - L * act;
+        double tension = -2.0 * k * err;
+        if (err < -eps) tension = 0.0;
+        else if (err >= -eps && err <= eps) tension = -k * (0.5 * err * err / eps + err + 0.5 * eps);
+        const double dv = (vb[0] - va[0]) * dir[0] + (vb[1] - va[1]) * dir[1] + (vb[2] - va[2]) * dir[2];
+        const double t = tension - damp * dv;
+#pragma unroll
+        for (int q = 0; q < 3; q++) f[q] = t * dir[q];
+    }
+#pragma unroll
+    for (int q = 0; q < 3; q++) fs[3 * (size_t)j + q] = f[q];
+}
+
+// node forces of the cable nodes (f_s[j-1] - f_s[j]) back-interpolated to the
+// touched mesh nodes, in the fixed order of each node's attachment list
+// (FemPhysicsBase.cpp:270-300); fixed nodes are skipped
+__global__ __launch_bounds__(256) void k_cable_gather(CableArgs c, const double* __restrict__ fs, double frac,
+                                                      const uint8_t* __restrict__ fixed, double* __restrict__ out,
+                                                      int accumulate)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= c.ntouched) return;
+    const int node = c.touched[t];
+    if (fixed[node]) return;
+    double acc[3] = {0.0, 0.0, 0.0};
+    for (int k = c.tptr[t]; k < c.tptr[t + 1]; k++) {
+        const int2 en = c.tent[k];
+        const int j = en.x;
+        const double wk = c.w[en.y];
+        double nf[3] = {-fs[3 * (size_t)j], -fs[3 * (size_t)j + 1], -fs[3 * (size_t)j + 2]};
+        if (j > 0) {
+#pragma unroll
+            for (int q = 0; q < 3; q++) nf[q] += fs[3 * (size_t)(j - 1) + q];
+        }
+#pragma unroll
+        for (int q = 0; q < 3; q++) acc[q] += wk * (frac * nf[q]);
+    }
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+        double* o = out + 3 * (size_t)node + q;
+        *o = accumulate ? *o + acc[q] : acc[q];
+    }
+}
+
+// spring potentials (FemPhysicsMatrixFree.cpp:1093-1108), per-workgroup partials
+__global__ __launch_bounds__(256) void k_cable_energy(CableArgs c, const double* __restrict__ X0, const double* __restrict__ u,
+                                                      double* __restrict__ part)
+{
+    double v[1] = {0.0};
+    for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < c.n; j += gridDim.x * blockDim.x) {
+        const double L = c.spring[4 * j], act = c.spring[4 * j + 1], k = c.spring[4 * j + 2];
+        if (k == 0.0) continue;
+        double pa[3], va[3], pb[3], vb[3];
+        cable_point(c, j, X0, u, nullptr, pa, va);
+        cable_point(c, j + 1, X0, u, nullptr, pb, vb);
+        const double y[3] = {pb[0] - pa[0], pb[1] - pa[1], pb[2] - pa[2]};
+        const double len = sqrt(y[0] * y[0] + y[1] * y[1] + y[2] * y[2]);
+        const double eps = L * 0.01;
+        const double err = len - L * act;
+        double pot = k * (err * err + eps * eps / 3.0);
+        if (err < -eps) pot = 0.0;
+        else if (err >= -eps && err <= eps) pot = 0.5 * k * (err * err * err / 3.0 / eps + err * err + eps * err + eps * eps / 3.0);
+        v[0] += pot;
+    }
+    block_partials<1, 256>(v, part);
 }
 
 // implicit dynamics: m / h^2 per free node (0 on fixed), FP32 and FP64
@@ -1014,8 +1199,13 @@ hipError_t launch_cg_iter_tail(int pc, int64_t n3, float* x, float* r, float* z,
     // k_cg_p re-reduces the update's g partials in every one of its workgroups:
     // fewer, larger workgroups read them fewer times
     const int gp = grid_for(n3 / 3, CG_P_BLOCK, CG_GRID);
-    if (pc) hipLaunchKernelGGL(k_cg_p<1>, dim3(gp), dim3(CG_P_BLOCK), 0, st, n3, z, r, p, scal, part, g, done, test_kind, tol2);
-    else hipLaunchKernelGGL(k_cg_p<0>, dim3(gp), dim3(CG_P_BLOCK), 0, st, n3, z, r, p, scal, part, g, done, test_kind, tol2);
+    // guard 2: also stop at r.z <= 0 (a learned preconditioner that is not
+    // positive definite at this state; its solve is redone with block Jacobi)
+    const int rz_guard = guard >= 2 && pc;
+    if (pc) hipLaunchKernelGGL(k_cg_p<1>, dim3(gp), dim3(CG_P_BLOCK), 0, st, n3, z, r, p, scal, part, g, done, test_kind, tol2,
+                               rz_guard, 0);
+    else hipLaunchKernelGGL(k_cg_p<0>, dim3(gp), dim3(CG_P_BLOCK), 0, st, n3, z, r, p, scal, part, g, done, test_kind, tol2,
+                            0, 0);
     return hipGetLastError();
 }
 
@@ -1033,11 +1223,11 @@ hipError_t launch_cg_update_only(int64_t n3, float* x, float* r, const float* p,
 }
 
 hipError_t launch_cg_p_only(int64_t n3, const float* z, const float* r, float* p, double* scal, const double* part,
-                            int* done, int test_kind, double tol2, hipStream_t st)
+                            int* done, int test_kind, double tol2, int rz_guard, int stall_window, hipStream_t st)
 {
     const int gp = grid_for(n3 / 3, CG_P_BLOCK, CG_GRID);
     hipLaunchKernelGGL(k_cg_p<1>, dim3(gp), dim3(CG_P_BLOCK), 0, st, n3, z, r, p, scal, part, cg_grid(n3 / 3), done,
-                       test_kind, tol2);
+                       test_kind, tol2, rz_guard, stall_window);
     return hipGetLastError();
 }
 
@@ -1086,6 +1276,51 @@ hipError_t launch_elem_stress(int mat, const BlockArgs& a, const double* u, cons
         break;
     default: hipLaunchKernelGGL(k_elem_stress<NEO_HOOKEAN>, grid, dim3(256), 0, st, a, u, eorig, out); break;
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_energy(int mat, const BlockArgs& a, const double* u, double* scal, int slot, double* part, hipStream_t st)
+{
+    const int g = grid_for(a.ne, 256, RED_GRID);
+    switch (mat) {
+    case LINEAR: hipLaunchKernelGGL(k_elem_energy<LINEAR>, dim3(g), dim3(256), 0, st, a, u, part); break;
+    case STVK: hipLaunchKernelGGL(k_elem_energy<STVK>, dim3(g), dim3(256), 0, st, a, u, part); break;
+    case COROTATIONAL: hipLaunchKernelGGL(k_elem_energy<COROTATIONAL>, dim3(g), dim3(256), 0, st, a, u, part); break;
+    case COROTATIONAL_POLAR:
+        hipLaunchKernelGGL(k_elem_energy<COROTATIONAL_POLAR>, dim3(g), dim3(256), 0, st, a, u, part);
+        break;
+    default: hipLaunchKernelGGL(k_elem_energy<NEO_HOOKEAN>, dim3(g), dim3(256), 0, st, a, u, part); break;
+    }
+    hipLaunchKernelGGL(k_finish<1>, dim3(1), dim3(256), 0, st, part, g, scal, slot);
+    return hipGetLastError();
+}
+
+hipError_t launch_dead_work(int64_t nn, const double* mass, const double* fdead, const double* g, double frac,
+                            const uint8_t* fixed, const double* u, double* scal, int slot, double* part, hipStream_t st)
+{
+    const int gr = grid_for(nn, 256, RED_GRID);
+    hipLaunchKernelGGL(k_dead_work, dim3(gr), dim3(256), 0, st, nn, mass, fdead, g[0], g[1], g[2], frac, fixed, u, part);
+    hipLaunchKernelGGL(k_finish<1>, dim3(1), dim3(256), 0, st, part, gr, scal, slot);
+    return hipGetLastError();
+}
+
+hipError_t launch_cable_forces(const CableArgs& c, const double* X0, const double* u, const double* vel, double* fs,
+                               double frac, const uint8_t* fixed, double* out, int accumulate, hipStream_t st)
+{
+    if (c.n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_cable_springs, dim3((c.n + 255) / 256), dim3(256), 0, st, c, X0, u, vel, fs);
+    if (c.ntouched)
+        hipLaunchKernelGGL(k_cable_gather, dim3((c.ntouched + 255) / 256), dim3(256), 0, st, c, (const double*)fs, frac,
+                           fixed, out, accumulate);
+    return hipGetLastError();
+}
+
+hipError_t launch_cable_energy(const CableArgs& c, const double* X0, const double* u, double* scal, int slot, double* part,
+                               hipStream_t st)
+{
+    const int g = grid_for(c.n > 0 ? c.n : 1, 256, RED_GRID);
+    hipLaunchKernelGGL(k_cable_energy, dim3(g), dim3(256), 0, st, c, X0, u, part);
+    hipLaunchKernelGGL(k_finish<1>, dim3(1), dim3(256), 0, st, part, g, scal, slot);
     return hipGetLastError();
 }
 

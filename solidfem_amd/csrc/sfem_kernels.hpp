@@ -23,7 +23,17 @@ constexpr int CG_BLOCK = SFEM_CG_BLOCK;
 constexpr int CG_P_BLOCK = SFEM_CG_P_BLOCK;
 
 // FP64 scalars kept on the device during a CG solve
-enum Scal { S_RZ = 0, S_RZ0, S_RR, S_RR0, S_PQ, S_ALPHA, S_BETA, S_ITER, S_NT, S_NT2, S_NT3, S_RZ_PREV, S_COUNT = 16 };
+// S_RR_BEST / S_IT_BEST: the stagnation test's reference r.r and the iteration
+// it was reached at (read and written by one thread of k_cg_p only)
+enum Scal {
+    S_RZ = 0, S_RZ0, S_RR, S_RR0, S_PQ, S_ALPHA, S_BETA, S_ITER, S_NT, S_NT2, S_NT3, S_RZ_PREV, S_RR_BEST, S_IT_BEST,
+    S_COUNT = 16
+};
+// CG status in the done flag: 0 running / iteration limit, 1 converged,
+// 3 breakdown (p.Kp <= 0, or r.z <= 0: operator or preconditioner not
+// positive definite), 4 not finite, 5 stagnation (r.r not halved within the
+// stagnation window)
+enum CgStatus { CG_RUNNING = 0, CG_CONVERGED = 1, CG_BREAKDOWN = 3, CG_NOT_FINITE = 4, CG_STAGNATED = 5 };
 
 // Element-block partition, all device pointers.
 // the material id of coarse multigrid levels of a Neo-Hookean context (the
@@ -58,6 +68,24 @@ struct BlockArgs {
     const double* mdiag64;
     const int2* node_q;
     const int* zero_flag;  // a device int that stays 0: the done flag of launches outside CG
+    // internal-force kernel only: 0 = the element's (V mu, V lambda); 1 = (V, 0)
+    // and 2 = (0, V), the force's derivatives in mu and in lambda (P is linear in both)
+    int coef_mode;
+};
+
+// Cable springs (FemPhysicsBase::ComputeSpringForces, FemPhysicsBase.cpp:214-309):
+// cable nodes embedded in elements by barycentric weights, one spring between
+// consecutive nodes of a cable; all cables concatenated
+struct CableArgs {
+    int n;                  // cable nodes
+    const int4* node;       // per cable node: the 4 mesh nodes of its element
+    const double* w;        // per cable node: 4 barycentric weights
+    const double* spring;   // per cable node j: spring j -> j+1 as (rest length, actuation, stiffness, damping);
+                            // stiffness 0 where j is the last node of its cable or the cable is not actuated
+    int ntouched;           // mesh nodes that carry cable forces
+    const int* touched;     // their ids
+    const int* tptr;        // ntouched+1 offsets into tent
+    const int2* tent;       // (cable node, index into w) per attachment, cable-node order
 };
 
 // nodes shared by several blocks: partial rows to sum
@@ -89,7 +117,8 @@ hipError_t launch_cg_init(int pc, int64_t n3, const float* b, float* x, float* r
 hipError_t launch_hessvec_cg(int mat, const BlockArgs& a, const float* u32, const float* d32, float* out, float* partial,
                              const int* done, double* dot_elem, hipStream_t st, hipEvent_t ev0 = nullptr,
                              hipEvent_t ev1 = nullptr);
-// alpha, x, r, z (summing shared nodes' partial slots), then beta and p
+// alpha, x, r, z (summing shared nodes' partial slots), then beta and p;
+// guard 1: stop at p.Kp <= 0, guard 2: also at r.z <= 0 (preconditioned)
 hipError_t launch_cg_iter_tail(int pc, int64_t n3, float* x, float* r, float* z, float* p, const float* q, const float* minv,
                                const int2* node_q, const float* partial,
                                double* scal, const double* pq_part, int n_pq, double* part, int* done, int test_kind,
@@ -97,8 +126,11 @@ hipError_t launch_cg_iter_tail(int pc, int64_t n3, float* x, float* r, float* z,
 hipError_t launch_cg_update_only(int64_t n3, float* x, float* r, const float* p, const float* q, const int2* node_q,
                                  const float* partial, double* scal, const double* pq_part, int n_pq, double* part,
                                  int* done, int guard, hipStream_t st);
+// rz_guard: stop with CG_BREAKDOWN at r.z <= 0; stall_window > 0: stop with
+// CG_STAGNATED when r.r has not fallen below half its last reference value
+// within that many iterations
 hipError_t launch_cg_p_only(int64_t n3, const float* z, const float* r, float* p, double* scal, const double* part,
-                            int* done, int test_kind, double tol2, hipStream_t st);
+                            int* done, int test_kind, double tol2, int rz_guard, int stall_window, hipStream_t st);
 hipError_t launch_cg_scalars(const double* part, int n, double* scal, int* done, hipStream_t st);
 hipError_t launch_dots3(int64_t n3, const double* a, const double* b, const float* d, double* scal, int slot, double* part,
                         hipStream_t st);
@@ -119,6 +151,19 @@ hipError_t launch_implicit_vel(int64_t nn, const double* u, const double* u0, co
                                hipStream_t st);
 hipError_t launch_pressure(int64_t ntri, const int* tris, const double* X0, const double* u, double pressure, double frac,
                            double* ftri, hipStream_t st);
+// sum_e V_e Psi(F_e) at displacement u (FP64) -> scal[slot]
+hipError_t launch_energy(int mat, const BlockArgs& a, const double* u, double* scal, int slot, double* part, hipStream_t st);
+// frac sum over free nodes of (m g + f_dead) . u -> scal[slot] (fdead may be null)
+hipError_t launch_dead_work(int64_t nn, const double* mass, const double* fdead, const double* g, double frac,
+                            const uint8_t* fixed, const double* u, double* scal, int slot, double* part, hipStream_t st);
+// spring forces f_s (3 per cable node, FP64) at x = X0 + u, velocities vel;
+// then out[node] (+)= frac * sum_j w_j (f_s[j-1] - f_s[j]) over the touched
+// free mesh nodes (accumulate: add to out, else overwrite those rows)
+hipError_t launch_cable_forces(const CableArgs& c, const double* X0, const double* u, const double* vel, double* fs,
+                               double frac, const uint8_t* fixed, double* out, int accumulate, hipStream_t st);
+// spring potentials (FemPhysicsMatrixFree::ComputeSpringEnergy, FemPhysicsMatrixFree.cpp:1062-1111) -> scal[slot]
+hipError_t launch_cable_energy(const CableArgs& c, const double* X0, const double* u, double* scal, int slot, double* part,
+                               hipStream_t st);
 hipError_t launch_invert_blocks(int64_t nn, const float* B, const uint8_t* fixed, float* Minv, hipStream_t st);
 hipError_t launch_identity_blocks(int64_t nn, const uint8_t* fixed, float* Minv, hipStream_t st);
 

@@ -38,11 +38,14 @@ constexpr double MG_RATIO = 10.0;       // smoothed band [lmax / ratio, lmax]
 constexpr double MG_COARSE_RATIO = 100.0;
 constexpr double MG_SAFETY = 1.15;      // the power iteration estimates lmax from below
 constexpr int MG_POWER_ITERS = 12;
-// a multigrid-PCG solve that has not met its forcing after this many
-// iterations stops (and the Newton driver redoes it with block Jacobi): one
-// iteration costs about six Jacobi-PCG iterations, and where the V-cycle
-// works it needs tens to a few hundred
-constexpr int MG_MAX_ITERS = 2500;
+// a multigrid-PCG solve stops (and the Newton driver redoes it with block
+// Jacobi) when it breaks down (p.Kp <= 0 or r.z <= 0: the V-cycle is not
+// positive definite at this state), when r.r has not halved within
+// MG_STALL_WINDOW iterations, or after MG_MAX_ITERS iterations.  Where the
+// V-cycle works it needs a few to a few tens of iterations per Newton step
+// and reduces r.r by ~10x per iteration.
+constexpr int MG_MAX_ITERS = 500;
+constexpr int MG_STALL_WINDOW = 150;
 
 struct LevelView {
     sfem_ctx* c;           // mesh, partition, state of the level
@@ -309,12 +312,11 @@ CgResult mg_pcg(sfem_ctx* c, int maxit, double tol2, int64_t* hv_evals)
                                     c->nblk, parts, done, 1, s));
         mg_vcycle(c);
         HIPCK(launch_mg_rz(c->nn, r, z, parts, s));
-        // the test is on the true residual norm (r.r), which stays a valid stop
-        // even where the V-cycle is not positive definite (tangents of
-        // near-incompressible material at strongly deformed Newton iterates);
-        // a V-cycle that stalls reaches MG_MAX_ITERS and the Newton driver
-        // falls back to block Jacobi
-        HIPCK(launch_cg_p_only(n3, z, r, p, scal, parts, done, 1, tol2, s));
+        // the test is on the true residual norm (r.r); a V-cycle that is not
+        // positive definite at this state (tangents of near-incompressible
+        // material at strongly deformed Newton iterates) shows as r.z <= 0 or
+        // stagnation, and the Newton driver falls back to block Jacobi
+        HIPCK(launch_cg_p_only(n3, z, r, p, scal, parts, done, 1, tol2, 1, MG_STALL_WINDOW, s));
         launched++;
     }
     if (!dflag) {

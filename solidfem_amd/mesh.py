@@ -249,3 +249,50 @@ def beam_hierarchy(m, min_cells=2, fixed=None, young_e=None, poisson_e=None, dir
         if direct_dofs is not None and 3 * c.nn <= direct_dofs:
             break
     return levels
+
+
+def _tet_volumes(a, b, c, d):
+    """signed volumes det[b - a, c - a, d - a] / 6 (rows of points)"""
+    return np.einsum("ij,ij->i", b - a, np.cross(c - a, d - a)) / 6.0
+
+
+def barycentric(p, X, tets):
+    """(ne, 4) barycentric weights of point p in every tetrahedron: sub-volume
+    ratios as ComputeBarycentric computes them (Source/FemBody.cpp:17-29)"""
+    x0, x1, x2, x3 = (X[tets[:, k]] for k in range(4))
+    P = np.broadcast_to(p, x0.shape)
+    vol = _tet_volumes(x0, x1, x2, x3)
+    return np.stack([_tet_volumes(P, x1, x2, x3), _tet_volumes(x0, P, x2, x3), _tet_volumes(x0, x1, P, x3),
+                     _tet_volumes(x0, x1, x2, P)], axis=1) / vol[:, None]
+
+
+def create_cable(X, tets, divs=10, length=1.0, offset=(0.0, 0.0, 0.0), direction=(1.0, 0.0, 0.0), stiffness=1000.0,
+                 use_free_cable=True, scale=1.0):
+    """CreateCable (Source/FemBody.cpp:31-104) restated: divs + 1 points
+    offset + i (length / divs) direction, each registered to the element with
+    the smallest score sum_j |100 X_j - p|^2 among those whose barycentric
+    weights all exceed -0.001 (else among all elements), at its barycentric
+    weights.  Returns dict(elem, bary (n, 3), rest_length, stiffness, free)
+    where `free` marks points further outside their element than 0.1 in a
+    weight (the reference's free cable nodes; sfem_add_cable refuses them)."""
+    X = np.asarray(X, dtype=np.float64).reshape(-1, 3)
+    tets = np.asarray(tets).reshape(-1, 4)
+    div = length / divs
+    d = np.asarray(direction, dtype=np.float64)
+    off = np.asarray(offset, dtype=np.float64)
+    elem, bary, free = [], [], []
+    Xs = scale * X
+    for i in range(divs + 1):
+        p = i * div * d + off
+        w = barycentric(p, Xs, tets)
+        score = np.sum((100.0 * X[tets] - p) ** 2, axis=(1, 2))
+        e = int(np.argmin(score))
+        positive = np.all(w > -0.001, axis=1)
+        if positive.any():
+            e = int(np.flatnonzero(positive)[np.argmin(score[positive])])
+        we = w[e]
+        elem.append(e)
+        bary.append(we[:3])
+        free.append(bool(use_free_cable and np.any(we < -0.1)))
+    return {"elem": np.array(elem, dtype=np.int32), "bary": np.array(bary), "rest_length": div / scale,
+            "stiffness": float(stiffness), "free": np.array(free)}

@@ -1,8 +1,10 @@
 """The five workloads (solidfem_amd/configs.py) on the GPU against the FP64
-oracle.  Default sizes are reduced grids with the full physics of each
-workload; SFEM_FULL=1 runs the full grids (minutes), checking the FP64
-residual of the device solution with the oracle and, where the CPU can solve
-it, the displacement error."""
+oracle: reduced grids with the full physics of each workload, and the full
+grids of workloads 2, 3 (polar corotational), 4 and 5 with the FP64 residual
+of the device solution checked by the oracle and the displacement error
+measured (FP64 Newton polish) or estimated (the FP64 residual's Newton
+correction).  SFEM_FULL=1 adds the remaining full-size combinations
+(minutes)."""
 import json
 import os
 
@@ -14,8 +16,6 @@ from tests.cfg_util import oracle_problem
 
 pytestmark = pytest.mark.gpu
 FULL = os.environ.get("SFEM_FULL") == "1"
-# preconditioner of the full-size runs: jacobi (default), multigrid, multigrid_learned
-FULL_PRECOND = os.environ.get("SFEM_FULL_PRECOND", "jacobi")
 
 
 def rel2(a, b):
@@ -75,24 +75,46 @@ def test_config3_corotational_reference_semantics():
     assert err < max(10 * spread, 1e-5)
 
 
-@pytest.mark.skipif(not FULL, reason="full-size workloads run with SFEM_FULL=1")
-@pytest.mark.parametrize("cfg", [1, 2, 3, "3p", 4, 5])
-def test_full_size_config(cfg):
+# full-size workloads run by default: (workload, preconditioner); "3p" is
+# workload 3 with the polar corotational model.  SFEM_FULL=1 adds the rest.
+FULL_DEFAULT = [(2, "jacobi"), (2, "multigrid_learned"), (4, "jacobi"), (4, "multigrid_learned"), ("3p", "multigrid"),
+                (5, "multigrid")]
+FULL_OPT_IN = [(1, "jacobi"), (3, "jacobi"), ("3p", "jacobi"), (5, "multigrid_learned"), (5, "jacobi")]
+
+
+def newton_correction_estimate(s, d, x, p):
+    """||K(u)^-1 r(u)|| / ||u||: the FP64 oracle's residual r at the device
+    state u, and the Newton correction it asks for, solved by the device's
+    multigrid PCG to 1e-3 relative at that state.  In Newton's quadratic
+    regime this is the distance to the FP64 solution on the same branch (to a
+    relative O(||correction||)), the error estimate where an FP64 CPU solve is
+    out of reach (the 6.3M-tet nu = 0.495 beam)."""
+    from solidfem_amd.mesh import beam_hierarchy
+    r = p.gradients(x, 1.0)
+    if s.mg_info().size <= 1:
+        s.set_multigrid(beam_hierarchy(d["mesh"], young_e=d["young_e"], poisson_e=d["poisson_e"]))
+    dx, it, status = s.mg_pcg(r, rtol=1e-3, maxit=400)
+    X = d["mesh"].X
+    return float(np.linalg.norm(dx) / np.linalg.norm(x - X)), it, status
+
+
+@pytest.mark.parametrize("cfg,pc", FULL_DEFAULT + (FULL_OPT_IN if FULL else []))
+def test_full_size_config(cfg, pc):
     """Full grids.  The oracle checks the FP64 residual of the device solution;
-    for 1 it solves the problem itself, for 2, 3p and 4 it continues Newton in
+    for 1 it solves the problem itself; for 2, 3p and 4 it continues Newton in
     FP64 from the device solution to 1e-10 (same equilibrium branch) and the
-    displacement error is measured against that.  3 uses the reference's
-    corotational tangent, with which Newton stalls (no 1e-8 claim); 3p is
-    workload 3 with the polar corotational model; 5 is checked by residual only
-    (an FP64 CPU solve at nu = 0.495 takes hours)."""
-    pc = configs.PRECONDS[FULL_PRECOND]
-    if pc in (capi.PC_MULTIGRID, capi.PC_MULTIGRID_LEARNED) and cfg == 1:
-        pytest.skip("no multigrid hierarchy for this workload (3 cells across)")
+    displacement error is measured against that; for every workload the
+    error is also estimated by the FP64 residual's Newton correction (the only
+    measure for 5: an FP64 CPU solve at nu = 0.495 takes hours), which the
+    oracle polish of 2, 3p and 4 validates.  3 uses the reference's
+    corotational tangent, with which Newton stalls (no 1e-8 claim)."""
+    print(f"case full-size workload {cfg} precond {pc}", flush=True)
+    pcv = configs.PRECONDS[pc]
     mat = capi.COROTATIONAL_POLAR if cfg == "3p" else None
     cfg = 3 if cfg == "3p" else cfg
     stall = cfg == 3 and mat is None
-    out, d, s = configs.run(cfg, pc, max_newton=12 if stall else 60, log=lambda m: print(m, flush=True), material=mat)
-    out["precond"] = FULL_PRECOND
+    out, d, s = configs.run(cfg, pcv, max_newton=12 if stall else 60, log=lambda m: print(m, flush=True), material=mat)
+    out["precond"] = pc
     x = s.positions()
     p = oracle_problem(d, nthreads=16)
     out["oracle_rel_residual"] = float(np.linalg.norm(p.gradients(x, 1.0)) / np.linalg.norm(p.external(x, 1.0)))
@@ -113,7 +135,19 @@ def test_full_size_config(cfg):
                 break
         out["err_vs_fp64_oracle"] = float(rel2(x - X, xo - X))
         out["oracle_polish"] = tot
+    if cfg != 1 and not stall:
+        est, it, status = newton_correction_estimate(s, d, x, p)
+        out["err_estimate_newton_correction"] = est
+        out["estimate_mg_pcg"] = {"iters": it, "status": status}
     out.pop("steps")
     print("FULLCONFIG " + json.dumps(out), flush=True)
     if not stall:
         assert out["converged"] and out["oracle_rel_residual"] < 1e-8
+        assert out["err_estimate_newton_correction"] < 1e-6
+        if "err_vs_fp64_oracle" in out:
+            assert out["err_vs_fp64_oracle"] < 1e-6
+            # the estimate agrees with the measured error within a factor of 3
+            # (where that error is above the polish's own resolution)
+            if cfg != 1 and out["err_vs_fp64_oracle"] > 1e-11:
+                e, m = out["err_estimate_newton_correction"], out["err_vs_fp64_oracle"]
+                assert m / 3 <= e <= 3 * m + 1e-12, (e, m)

@@ -61,13 +61,15 @@ def test_forces_and_hessvec(mat, grid, per_elem):
     q = s.matvec(d)
     # FP32 rounding of element contributions whose moduli differ by up to 100x
     # (per-element E in [1e5, 1e7]) cancels in the node sums: looser bound there
-    tol32 = 1e-4 if per_elem else 2e-5
-    assert rel2(q, p.matvec(x, d)) < tol32
+    tol32 = 2e-5 if per_elem else 2e-6
+    eq = rel2(q, p.matvec(x, d))
     assert np.all(q[fixed == 1] == 0)
     B = s.block_diagonal()
     Bo = p.block_diag(x)
     free = fixed == 0
-    assert rel2(B[free], Bo[free]) < tol32
+    eb = rel2(B[free], Bo[free])
+    print(f"  K.d rel err {eq:.2e}, block diagonal rel err {eb:.2e}", flush=True)
+    assert eq < tol32 and eb < tol32, (eq, eb)
 
 
 def test_hessvec_is_deterministic():
@@ -85,9 +87,12 @@ def test_hessvec_is_deterministic():
 @pytest.mark.parametrize("mat", MATS)
 def test_reference_mode_solve_cfg1(mat):
     """SolveNewtonCG semantics (10 steps, CG eps 1e-3f) against the compiled
-    reference's own result (golden vector).  FP32 inner solves: the
-    trajectories differ at the FP32 level, so the bound is looser than the
-    converged-solve bound."""
+    reference's own result (golden vector), to 3e-7 for LINEAR, STVK and
+    NEO_HOOKEAN (measured 1e-8 - 3e-8: the FP32 inner solves follow the FP64
+    trajectory closely).  The reference's corotational tangent is not the
+    Jacobian of its force and not symmetric, so its fixed-forcing CG amplifies
+    rounding: there the bound is 10x the spread of the FP64 oracle's own result
+    when its moduli change by +-2e-7 (about FP32 rounding), itself printed."""
     import os
     g = np.load(os.path.join(os.path.dirname(__file__), "golden", f"ref_cfg1_mat{mat}.npz"))
     m, fixed, p, s = setup(mat)
@@ -95,9 +100,23 @@ def test_reference_mode_solve_cfg1(mat):
     st = s.solve(capi.solve_params(mode=capi.SOLVE_REFERENCE, outer=10, inner=1000, check_every=8))
     u_ref = g["newton_x"] - m.X
     err = rel2(s.positions() - m.X, u_ref)
-    print(f"mat {mat}: reference-mode rel err {err:.3e}, cg {st['cg_iters']}")
+    bound = 3e-7
+    if mat == capi.COROTATIONAL:
+        def oracle_run(scale):
+            mu, lam = lame(1e6, 0.45)
+            q = O.OracleProblem(m.X, m.tets, fixed, mat, mu=float(mu) * scale, lam=float(lam) * scale, density=1070.0,
+                                tris=m.tip_triangles(), pressure=1e4, gravity=(0.0, 0.0, -9.8))
+            x, _ = q.newton_cg_ref(m.X.copy(), 1.0, 10, 1000)
+            return x
+        x1 = oracle_run(1.0)
+        spread = max(rel2(oracle_run(1.0 + sc) - m.X, x1 - m.X) for sc in (2e-7, -2e-7))
+        e_orc = rel2(x1 - m.X, u_ref)
+        print(f"  corotational: oracle vs compiled reference {e_orc:.2e}, oracle spread under +-2e-7 moduli "
+              f"{spread:.2e}", flush=True)
+        bound = 10 * max(spread, e_orc)
+    print(f"mat {mat}: reference-mode rel err {err:.3e} (bound {bound:.1e}), cg {st['cg_iters']}")
     assert st["newton_iters"] == 10
-    assert err < (5e-2 if mat == capi.COROTATIONAL else 1e-4)
+    assert err < bound, (err, bound)
 
 
 @pytest.mark.parametrize("mat", [capi.LINEAR, capi.STVK, capi.NEO_HOOKEAN, capi.COROTATIONAL_POLAR])
