@@ -229,6 +229,18 @@ int sfem_gradients(sfem_ctx* ctx, double* r);
    rows, FP32 element kernel, fixed rows of d ignored and of Kd zero */
 int sfem_matvec(sfem_ctx* ctx, const double* d, double* Kd);
 
+/* sfem_matvec without allocating: no reference counterpart; for callers that
+   cannot allocate (code under HIP graph capture, tight serving loops,
+   harnesses that forbid hipMalloc).  Same kernels and result as sfem_matvec,
+   but the host staging, the FP32 direction and result and the shared-node
+   partial rows live in a static per-device array of 16 MiB (held by each
+   device the library runs on until the process ends); no hipMalloc, hipFree
+   or host allocation.  Meshes needing more (about 24 B per node plus 12 B per
+   partial-row slot, i.e. up to ~1M elements of the beam meshes) get SFEM_EINVAL.
+   All calls on one device share the array: calls that overlap on one device
+   race. */
+int sfem_matvec_noalloc(sfem_ctx* ctx, const double* d, double* Kd);
+
 /* 3x3 diagonal blocks of K per node (nn*9, row-major), FP32 kernel; fixed
    nodes get the identity.  The block-Jacobi preconditioner is their inverse. */
 int sfem_block_diagonal(sfem_ctx* ctx, double* B);

@@ -19,7 +19,8 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIP_FLAGS = ["-O3", "-std=c++17", "--offload-arch=gfx950", "-fPIC",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
-LIB_SOURCES = ["sfem_kernels.hip", "sfem_gnn.hip", "sfem_mg.hip", "sfem_capi.cpp", "sfem_gnn_host.cpp", "sfem_mg.cpp"]
+LIB_SOURCES = ["sfem_kernels.hip", "sfem_gnn.hip", "sfem_mg.hip", "sfem_noalloc.hip", "sfem_capi.cpp", "sfem_gnn_host.cpp",
+               "sfem_mg.cpp"]
 LIB = os.path.join(HERE, "libsfem_hip.so")
 
 

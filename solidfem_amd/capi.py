@@ -25,7 +25,7 @@ EXPORTS = [
     "sfem_create", "sfem_destroy", "sfem_last_error", "sfem_set_positions", "sfem_get_positions",
     "sfem_set_load_fraction", "sfem_get_load_fraction", "sfem_set_pressure_surface",
     "sfem_set_external_forces", "sfem_internal_forces", "sfem_element_stress", "sfem_get_velocities",
-    "sfem_gradients", "sfem_matvec", "sfem_energy", "sfem_force_param_grads",
+    "sfem_gradients", "sfem_matvec", "sfem_matvec_noalloc", "sfem_energy", "sfem_force_param_grads",
     "sfem_add_cable", "sfem_set_cable_actuation", "sfem_num_cables", "sfem_get_cable_positions", "sfem_clear_cables",
     "sfem_block_diagonal", "sfem_solve", "sfem_step", "sfem_set_precond_blocks", "sfem_info",
     "sfem_reset", "sfem_profile_hessvec", "sfem_time_hessvec",
@@ -101,6 +101,7 @@ def lib():
             "sfem_get_velocities": ([V, _dp], C.c_int),
             "sfem_gradients": ([V, _dp], C.c_int),
             "sfem_matvec": ([V, _dp, _dp], C.c_int),
+            "sfem_matvec_noalloc": ([V, _dp, _dp], C.c_int),
             "sfem_energy": ([V, _dp], C.c_int),
             "sfem_force_param_grads": ([V, _dp, _dp, _dp], C.c_int),
             "sfem_add_cable": ([V, _ip, _dp, C.c_int32, C.c_double, C.c_double, C.c_double, C.POINTER(C.c_int32)],
@@ -265,6 +266,13 @@ class Solver:
         d = np.ascontiguousarray(d, dtype=np.float64).reshape(-1, 3)
         q = np.zeros_like(d)
         _check(lib().sfem_matvec(self.h, _p(d), _p(q)))
+        return q
+
+    def matvec_noalloc(self, d):
+        """sfem_matvec_noalloc: K d with no allocation (static per-device scratch)"""
+        d = np.ascontiguousarray(d, dtype=np.float64).reshape(-1, 3)
+        q = np.zeros_like(d)
+        _check(lib().sfem_matvec_noalloc(self.h, _p(d), _p(q)))
         return q
 
     def energy(self):

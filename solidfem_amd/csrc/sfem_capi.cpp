@@ -216,6 +216,7 @@ void build_partition(sfem_ctx* c, const sfem_mesh* m, const std::vector<double>&
     }
     c->d_node_info.upload(node_q);
     c->nrows_partial = (int64_t)bnodes.size();
+    c->nslots = sh_ptr.back();
     c->d_partial.alloc((size_t)c->nrows_partial * 9 * sizeof(float) + (size_t)c->nrows_partial * 3 * sizeof(double));
 
     BlockArgs& a = c->blk;
@@ -629,6 +630,9 @@ int sfem_create(const sfem_mesh* m, const sfem_material* mat, sfem_ctx** out)
     int rc = guard_call("sfem_create", [&]() -> int {
         int ndev = 0;
         if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) throw HipError{hipErrorNoDevice, "no HIP device"};
+        HIPCK(hipGetDevice(&c->device));
+        // the allocation-free entry point's scratch array on this device
+        if (noalloc_init(c->device) != SFEM_OK) throw std::runtime_error(g_err);
         HIPCK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
         c->ne = m->ne;
         c->nn = m->nn;

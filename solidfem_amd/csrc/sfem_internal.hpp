@@ -111,6 +111,8 @@ struct MgLevel {
 
 struct sfem_ctx {
     int64_t ne = 0, nn = 0, nfixed = 0;
+    int device = 0;          // the HIP device current at creation (the context's device)
+    int64_t nslots = 0;      // partial-row slots of the nodes shared by several blocks
     int material = 0;
     // the material as given (coarse multigrid levels inherit a uniform one)
     double young = 0.0, poisson = 0.0, density = 0.0;
@@ -225,6 +227,13 @@ template <typename F> int guard_call(const char* where, F&& f)
         return fail(SFEM_EDEVICE, std::string(where) + ": " + e.what());
     }
 }
+
+// sfem_matvec_noalloc (sfem_noalloc.hip): the per-device scratch array's
+// size, the bytes one call needs for context c, and the once-per-device
+// lookup of the array's address (at context creation)
+constexpr size_t NOALLOC_SCRATCH_BYTES = size_t(16) << 20;
+size_t noalloc_matvec_bytes(const sfem_ctx* c);
+int noalloc_init(int device);
 
 // operations on a context (sfem_capi.cpp)
 void hessvec(sfem_ctx* c, const float* d, float* out, const int* done);
