@@ -20,7 +20,7 @@ HIP_FLAGS = ["-O3", "-std=c++17", "--offload-arch=gfx950", "-fPIC",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
 LIB_SOURCES = ["sfem_kernels.hip", "sfem_gnn.hip", "sfem_mg.hip", "sfem_noalloc.hip", "sfem_capi.cpp", "sfem_gnn_host.cpp",
-               "sfem_mg.cpp"]
+               "sfem_mg.cpp", "sfem_host.cpp"]
 LIB = os.path.join(HERE, "libsfem_hip.so")
 
 
@@ -45,7 +45,7 @@ def build_pysolidfem():
     host = os.path.join(CSRC, "host")
     cmd = ["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-fvisibility=hidden",
            f"-I{pybind11.get_include()}", f"-I{sysconfig.get_paths()['include']}", f"-I{INCLUDE}", f"-I{host}",
-           os.path.join(host, "FemPhysicsMatrixFree.cpp"), os.path.join(host, "FemIO.cpp"),
+           os.path.join(host, "FemPhysicsMatrixFree.cpp"), os.path.join(host, "FemIO.cpp"), os.path.join(host, "FemBody.cpp"),
            os.path.join(host, "pysolidfem.cpp"),
            f"-L{HERE}", "-lsfem_hip", "-Wl,-rpath,$ORIGIN", "-o", out + ".tmp"]
     _run(cmd)

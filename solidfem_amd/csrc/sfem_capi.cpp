@@ -43,180 +43,30 @@ void build_partition(sfem_ctx* c, const sfem_mesh* m, const std::vector<double>&
                      const std::vector<double>& dminv, const std::vector<double>& vol)
 {
     const int64_t ne = m->ne, nn = m->nn;
-    const int BE = BLOCK_E;
-    // --- space-filling-curve order of elements (centroid Morton code, cubic cells)
-    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
-    for (int64_t i = 0; i < nn; i++)
-        for (int k = 0; k < 3; k++) {
-            lo[k] = std::min(lo[k], m->X0[3 * i + k]);
-            hi[k] = std::max(hi[k], m->X0[3 * i + k]);
-        }
-    double ext = std::max(hi[0] - lo[0], std::max(hi[1] - lo[1], hi[2] - lo[2]));
-    if (!(ext > 0)) ext = 1;
-    const double scale = (double)((1 << 21) - 1) / ext;
-    std::vector<std::pair<uint64_t, int64_t>> key(ne);
-    for (int64_t e = 0; e < ne; e++) {
-        double cc[3] = {0, 0, 0};
-        for (int v = 0; v < 4; v++)
-            for (int k = 0; k < 3; k++) cc[k] += 0.25 * m->X0[3 * (int64_t)m->tets[4 * e + v] + k];
-        uint64_t q[3];
-        for (int k = 0; k < 3; k++) q[k] = (uint64_t)std::max(0.0, std::min((cc[k] - lo[k]) * scale, (double)((1 << 21) - 1)));
-        key[e] = {spread21(q[0]) | spread21(q[1]) << 1 | spread21(q[2]) << 2, e};
-    }
-    std::sort(key.begin(), key.end());
-
-    const int nblk = (int)((ne + BE - 1) / BE);
-    const int64_t ne_pad = (int64_t)nblk * BE;
+    const Partition P = make_partition(m->tets, ne, m->X0, nn, m->fixed, mu, lam, dminv, vol, BLOCK_E);
+    const int nblk = P.nblk;
+    const int64_t ne_pad = P.ne_pad;
+    const int nmax = P.nmax;
+    const bool uniform = P.uniform;
     c->nblk = nblk;
-
-    std::vector<int> bnode_ptr(nblk + 1, 0);
-    std::vector<uint32_t> bnodes;
-    std::vector<uint16_t> lptr;
-    std::vector<uint32_t> lidx((size_t)ne_pad * 2, 0), lperm((size_t)ne_pad * 2, 0);
-    std::vector<float> dm32((size_t)9 * ne_pad, 0.f), cf32((size_t)2 * ne_pad, 0.f);
-    std::vector<double> dm64((size_t)9 * ne_pad, 0.0), cf64((size_t)2 * ne_pad, 0.0);
-    std::vector<int> node_blocks(nn, 0);
-    bnodes.reserve((size_t)ne);
-    lptr.reserve((size_t)ne);
-    int nmax = 0;
-    std::vector<int> ids;
-    std::vector<int> cnt;
-    for (int b = 0; b < nblk; b++) {
-        const int64_t e0 = (int64_t)b * BE;
-        const int64_t e1 = std::min<int64_t>(ne, e0 + BE);
-        ids.clear();
-        for (int64_t s = e0; s < e1; s++)
-            for (int v = 0; v < 4; v++) ids.push_back(m->tets[4 * key[s].second + v]);
-        std::sort(ids.begin(), ids.end());
-        ids.erase(std::unique(ids.begin(), ids.end()), ids.end());
-        const int nb = (int)ids.size();
-        nmax = std::max(nmax, nb);
-        bnode_ptr[b + 1] = bnode_ptr[b] + nb;
-        for (int id : ids) {
-            bnodes.push_back((uint32_t)id | (m->fixed[id] ? F_FIXED : 0u));
-            node_blocks[id]++;
-        }
-        // local ids and the per-node incidence lists (counting sort, element order)
-        cnt.assign(nb + 1, 0);
-        for (int64_t s = e0; s < e1; s++) {
-            const int64_t e = key[s].second;
-            int l[4];
-            for (int v = 0; v < 4; v++) {
-                l[v] = (int)(std::lower_bound(ids.begin(), ids.end(), m->tets[4 * e + v]) - ids.begin());
-                cnt[l[v] + 1]++;
-            }
-            lidx[2 * s] = (uint32_t)l[0] | (uint32_t)l[1] << 16;
-            lidx[2 * s + 1] = (uint32_t)l[2] | (uint32_t)l[3] << 16;
-            for (int k = 0; k < 9; k++) {
-                dm64[(size_t)k * ne_pad + s] = dminv[9 * e + k];
-                dm32[(size_t)k * ne_pad + s] = (float)dminv[9 * e + k];
-            }
-            const double cm = vol[e] * mu[e], cl = vol[e] * lam[e];
-            cf64[s] = cm; cf64[ne_pad + s] = cl;
-            cf32[s] = (float)cm; cf32[ne_pad + s] = (float)cl;
-        }
-        for (int j = 0; j < nb; j++) cnt[j + 1] += cnt[j];
-        for (int j = 0; j <= nb; j++) lptr.push_back((uint16_t)cnt[j]);
-        std::vector<int> fill(cnt.begin(), cnt.end() - 1);
-        for (int64_t s = e0; s < e1; s++) {
-            const uint32_t* li = &lidx[2 * s];
-            const int l[4] = {(int)(li[0] & 0xffff), (int)(li[0] >> 16), (int)(li[1] & 0xffff), (int)(li[1] >> 16)};
-            // corner v of element s -> its slot in the node-sorted incidence order
-            // (counting sort in element order: a node's slots list its elements in order)
-            uint16_t k[4];
-            for (int v = 0; v < 4; v++) k[v] = (uint16_t)fill[l[v]]++;
-            lperm[2 * s] = (uint32_t)k[0] | (uint32_t)k[1] << 16;
-            lperm[2 * s + 1] = (uint32_t)k[2] | (uint32_t)k[3] << 16;
-        }
-    }
-    // shared nodes: flag, and list their partial rows in block order
-    std::vector<int> sh_of(nn, -1);
-    std::vector<uint32_t> sh_node;
-    for (int64_t i = 0; i < nn; i++)
-        if (node_blocks[i] > 1) {
-            sh_of[i] = (int)sh_node.size();
-            sh_node.push_back((uint32_t)i | (m->fixed[i] ? F_FIXED : 0u));
-        }
-    std::vector<int> sh_ptr(sh_node.size() + 1, 0);
-    for (size_t r = 0; r < bnodes.size(); r++) {
-        const uint32_t id = bnodes[r] & 0x3fffffffu;
-        if (sh_of[id] >= 0) {
-            bnodes[r] |= F_SHARED;
-            sh_ptr[sh_of[id] + 1]++;
-        }
-    }
-    for (size_t i = 0; i < sh_node.size(); i++) sh_ptr[i + 1] += sh_ptr[i];
-    // each shared node's partial rows get contiguous slots, in block order
-    std::vector<int> bslot(bnodes.size(), -1);
-    std::vector<int> fill(sh_ptr.begin(), sh_ptr.end() - 1);
-    for (size_t r = 0; r < bnodes.size(); r++) {   // rows are in block order
-        const uint32_t id = bnodes[r] & 0x3fffffffu;
-        if (sh_of[id] >= 0) bslot[r] = fill[sh_of[id]]++;
-    }
-
-    // rest coordinates of every block row relative to the centre of the block's
-    // bounding box (FP64 difference, then FP32): |coordinate| <= half the block
-    // diameter, so the FP32 edge vectors the Hessian-vector kernel forms from
-    // them are accurate to ~2^-24 block diameter / edge, ~3e-7 relative here
-    std::vector<float> rel32(3 * bnodes.size());
-    for (int b = 0; b < nblk; b++) {
-        double blo[3] = {1e300, 1e300, 1e300}, bhi[3] = {-1e300, -1e300, -1e300};
-        for (int r = bnode_ptr[b]; r < bnode_ptr[b + 1]; r++) {
-            const int64_t id = bnodes[r] & 0x3fffffffu;
-            for (int k = 0; k < 3; k++) {
-                blo[k] = std::min(blo[k], m->X0[3 * id + k]);
-                bhi[k] = std::max(bhi[k], m->X0[3 * id + k]);
-            }
-        }
-        for (int r = bnode_ptr[b]; r < bnode_ptr[b + 1]; r++) {
-            const int64_t id = bnodes[r] & 0x3fffffffu;
-            for (int k = 0; k < 3; k++) rel32[3 * (size_t)r + k] = (float)(m->X0[3 * id + k] - 0.5 * (blo[k] + bhi[k]));
-        }
-    }
-    c->d_rel32.upload(rel32);
-    // Lame parameters for the Hessian-vector kernel: per element only when they vary
-    bool uniform = true;
-    for (int64_t e = 1; e < ne && uniform; e++) uniform = mu[e] == mu[0] && lam[e] == lam[0];
-    if (!uniform) {
-        std::vector<float> ml((size_t)2 * ne_pad, 0.f);
-        for (int64_t s2 = 0; s2 < ne; s2++) {
-            ml[s2] = (float)mu[key[s2].second];
-            ml[ne_pad + s2] = (float)lam[key[s2].second];
-        }
-        c->d_mat32.upload(ml);
-    }
-    // sorted element -> original element (per-element outputs in caller order)
-    std::vector<int> eorig(ne);
-    for (int64_t s2 = 0; s2 < ne; s2++) eorig[s2] = (int)key[s2].second;
-    c->d_eorig.upload(eorig);
-    c->d_bnode_ptr.upload(bnode_ptr);
-    c->d_bnodes.upload(bnodes);
-    c->d_lptr.upload(lptr);
-    c->d_lperm.upload(lperm);
-    c->d_lidx.upload(lidx);
-    c->d_dminv32.upload(dm32);
-    c->d_coef32.upload(cf32);
-    c->d_dminv64.upload(dm64);
-    c->d_coef64.upload(cf64);
-    c->d_sh_node.upload(sh_node);
-    c->d_sh_ptr.upload(sh_ptr);
-    c->d_bslot.upload(bslot);
-    // per node, for the CG update: a shared node's partial slot range
-    // {first, end}; {-1, 0} for a free node of one block; {-2, 0} for a fixed
-    // node and for a node that no element references (its row of K is 0);
-    // after the nn pairs, one more pair that stays 0 (BlockArgs::zero_flag)
-    std::vector<int> node_q(2 * nn + 2, 0);
-    for (int64_t i = 0; i < nn; i++) {
-        if (m->fixed[i] || node_blocks[i] == 0) node_q[2 * i] = -2;
-        else if (sh_of[i] < 0) node_q[2 * i] = -1;
-        else {
-            node_q[2 * i] = sh_ptr[sh_of[i]];
-            node_q[2 * i + 1] = sh_ptr[sh_of[i] + 1];
-        }
-    }
-    c->d_node_info.upload(node_q);
-    c->nrows_partial = (int64_t)bnodes.size();
-    c->nslots = sh_ptr.back();
+    c->d_rel32.upload(P.rel32);
+    if (!uniform) c->d_mat32.upload(P.mat32);
+    c->d_eorig.upload(P.eorig);
+    c->d_bnode_ptr.upload(P.bnode_ptr);
+    c->d_bnodes.upload(P.bnodes);
+    c->d_lptr.upload(P.lptr);
+    c->d_lperm.upload(P.lperm);
+    c->d_lidx.upload(P.lidx);
+    c->d_dminv32.upload(P.dm32);
+    c->d_coef32.upload(P.cf32);
+    c->d_dminv64.upload(P.dm64);
+    c->d_coef64.upload(P.cf64);
+    c->d_sh_node.upload(P.sh_node);
+    c->d_sh_ptr.upload(P.sh_ptr);
+    c->d_bslot.upload(P.bslot);
+    c->d_node_info.upload(P.node_q);
+    c->nrows_partial = (int64_t)P.bnodes.size();
+    c->nslots = P.sh_ptr.back();
     c->d_partial.alloc((size_t)c->nrows_partial * 9 * sizeof(float) + (size_t)c->nrows_partial * 3 * sizeof(double));
 
     BlockArgs& a = c->blk;
@@ -243,7 +93,7 @@ void build_partition(sfem_ctx* c, const sfem_mesh* m, const std::vector<double>&
     a.mu0 = (float)mu[0];
     a.lam0 = (float)lam[0];
     SharedArgs& s = c->shr;
-    s.nshared = (int)sh_node.size();
+    s.nshared = (int)P.sh_node.size();
     s.node = c->d_sh_node.as<uint32_t>();
     s.ptr = c->d_sh_ptr.as<int>();
 }
@@ -330,17 +180,11 @@ void rebuild_cables(sfem_ctx* c)
         }
     }
     const int n = (int)(node4.size() / 4);
-    std::vector<std::vector<int2>> att(c->nn);
-    for (int j = 0; j < n; j++)
-        for (int k = 0; k < 4; k++) att[node4[4 * j + k]].push_back(make_int2(j, 4 * j + k));
-    std::vector<int> touched, tptr{0};
-    std::vector<int2> tent;
-    for (int64_t i = 0; i < c->nn; i++)
-        if (!att[i].empty()) {
-            touched.push_back((int)i);
-            tent.insert(tent.end(), att[i].begin(), att[i].end());
-            tptr.push_back((int)tent.size());
-        }
+    std::vector<int> touched, tptr;
+    std::vector<std::pair<int, int>> att;
+    cable_attachments(node4, c->nn, &touched, &tptr, &att);
+    std::vector<int2> tent(att.size());
+    for (size_t k = 0; k < att.size(); k++) tent[k] = make_int2(att[k].first, att[k].second);
     c->d_cab_node.upload(node4);
     c->d_cab_w.upload(w4);
     c->d_cab_spring.upload(spring);
@@ -644,53 +488,18 @@ int sfem_create(const sfem_mesh* m, const sfem_material* mat, sfem_ctx** out)
         for (int k = 0; k < 3; k++) c->gravity[k] = mat->gravity[k];
         const int64_t ne = m->ne, nn = m->nn;
         // --- geometry (FemPhysicsMatrixFree.cpp:126-146): Dm^-1, V, lumped masses
-        std::vector<double> dminv(9 * ne), vol(ne), mu(ne), lam(ne), msum(nn, 0.0);
-        for (int64_t e = 0; e < ne; e++) {
-            const int32_t* t = m->tets + 4 * e;
-            double D[9];
-            for (int k = 1; k < 4; k++)
-                for (int r = 0; r < 3; r++) D[3 * r + (k - 1)] = m->X0[3 * (int64_t)t[k] + r] - m->X0[3 * (int64_t)t[0] + r];
-            double v[9];
-            v[0] = D[4] * D[8] - D[5] * D[7];
-            v[3] = -D[3] * D[8] + D[6] * D[5];
-            v[6] = D[3] * D[7] - D[6] * D[4];
-            v[1] = D[2] * D[7] - D[1] * D[8];
-            v[4] = D[0] * D[8] - D[2] * D[6];
-            v[7] = D[1] * D[6] - D[0] * D[7];
-            v[2] = D[1] * D[5] - D[2] * D[4];
-            v[5] = D[2] * D[3] - D[0] * D[5];
-            v[8] = D[0] * D[4] - D[1] * D[3];
-            const double det = D[0] * v[0] + D[1] * v[3] + D[2] * v[6];
-            if (det == 0.0) throw std::runtime_error("degenerate element " + std::to_string(e));
-            const double s = 1.0 / det;
-            for (int k = 0; k < 9; k++) dminv[9 * e + k] = v[k] * s;
-            vol[e] = det / 6.0;
-            const double E = mat->young_e ? mat->young_e[e] : mat->young;
-            const double nu = mat->poisson_e ? mat->poisson_e[e] : mat->poisson;
-            // FemPhysicsBase.cpp:40-41
-            lam[e] = E * nu / (1.0 + nu) / (1.0 - 2.0 * nu);
-            mu[e] = 0.5 * E / (1.0 + nu);
-            const double lm = 0.25 * vol[e] * mat->density;
-            for (int k = 0; k < 4; k++) msum[t[k]] += lm;
-        }
-        std::vector<double> mass(nn, 0.0);
-        // a node that no element references has neither stiffness nor mass:
-        // it is held like a fixed node (its block would be singular)
-        c->fixed_h.assign(m->fixed, m->fixed + nn);
-        {
-            std::vector<uint8_t> used(nn, 0);
-            for (int64_t k = 0; k < 4 * ne; k++) used[m->tets[k]] = 1;
-            for (int64_t i = 0; i < nn; i++)
-                if (!used[i]) c->fixed_h[i] = 1;
-        }
+        ElementGeometry g;
+        std::string err;
+        if (!element_geometry(m->tets, ne, m->X0, nn, m->fixed, mat->young, mat->poisson, mat->young_e, mat->poisson_e,
+                              mat->density, &g, &err))
+            throw std::runtime_error(err);
+        c->fixed_h = g.held;
+        c->nfixed = g.nheld;
         sfem_mesh mf = *m;
         mf.fixed = c->fixed_h.data();
         c->tets_h.assign(m->tets, m->tets + 4 * ne);
-        for (int64_t i = 0; i < nn; i++) {
-            if (c->fixed_h[i]) { c->nfixed++; continue; }
-            const double inv = msum[i] != 0 ? 1.0 / msum[i] : 1.0;   // Node::invMass default 1
-            mass[i] = 1.0 / inv;
-        }
+        const std::vector<double>& mass = g.mass;
+        const std::vector<double>&dminv = g.dminv, &vol = g.vol, &mu = g.mu, &lam = g.lam;
         build_partition(c, &mf, mu, lam, dminv, vol);
         HIPCK(set_elem_lds_limits(c->blk));
         if (elem_lds_max(c->blk) > 160 * 1024) throw std::runtime_error("element block exceeds LDS");
@@ -815,12 +624,8 @@ int sfem_set_pressure_surface(sfem_ctx* c, const int32_t* tris, int64_t ntri, do
         if (ntri == 0) return SFEM_OK;
         std::vector<int> t(tris, tris + 3 * ntri);
         // node -> incident triangles (CSR, triangle order)
-        std::vector<int> ptr(c->nn + 1, 0);
-        for (int64_t i = 0; i < 3 * ntri; i++) ptr[t[i] + 1]++;
-        for (int64_t i = 0; i < c->nn; i++) ptr[i + 1] += ptr[i];
-        std::vector<int> idx(ptr.back()), fill(ptr.begin(), ptr.end() - 1);
-        for (int64_t k = 0; k < ntri; k++)
-            for (int v = 0; v < 3; v++) idx[fill[t[3 * k + v]]++] = (int)k;
+        std::vector<int> ptr, idx;
+        node_triangle_csr(tris, ntri, c->nn, &ptr, &idx);
         c->d_tris.upload(t);
         c->d_pptr.upload(ptr);
         c->d_pidx.upload(idx);

@@ -1,0 +1,298 @@
+// sfem_host.cpp -- host-only set-up logic of libsfem_hip.so (see sfem_host.hpp).
+#include "sfem_host.hpp"
+
+#include <algorithm>
+#include <cmath>
+
+namespace sfem {
+
+namespace {
+
+// 21-bit-per-axis Morton code
+uint64_t spread21(uint64_t v)
+{
+    v &= 0x1fffff;
+    v = (v | v << 32) & 0x1f00000000ffffULL;
+    v = (v | v << 16) & 0x1f0000ff0000ffULL;
+    v = (v | v << 8) & 0x100f00f00f00f00fULL;
+    v = (v | v << 4) & 0x10c30c30c30c30c3ULL;
+    v = (v | v << 2) & 0x1249249249249249ULL;
+    return v;
+}
+
+}  // namespace
+
+bool element_geometry(const int32_t* tets, int64_t ne, const double* X0, int64_t nn, const uint8_t* fixed, double young,
+                      double poisson, const double* young_e, const double* poisson_e, double density, ElementGeometry* g,
+                      std::string* err)
+{
+    g->dminv.assign(9 * (size_t)ne, 0.0);
+    g->vol.assign(ne, 0.0);
+    g->mu.assign(ne, 0.0);
+    g->lam.assign(ne, 0.0);
+    std::vector<double> msum(nn, 0.0);
+    for (int64_t e = 0; e < ne; e++) {
+        const int32_t* t = tets + 4 * e;
+        for (int k = 0; k < 4; k++)
+            if (t[k] < 0 || t[k] >= nn) {
+                *err = "node index out of range";
+                return false;
+            }
+        double D[9];
+        for (int k = 1; k < 4; k++)
+            for (int r = 0; r < 3; r++) D[3 * r + (k - 1)] = X0[3 * (int64_t)t[k] + r] - X0[3 * (int64_t)t[0] + r];
+        // adjugate inverse (Math/matrix3.h:234-249)
+        double v[9];
+        v[0] = D[4] * D[8] - D[5] * D[7];
+        v[3] = -D[3] * D[8] + D[6] * D[5];
+        v[6] = D[3] * D[7] - D[6] * D[4];
+        v[1] = D[2] * D[7] - D[1] * D[8];
+        v[4] = D[0] * D[8] - D[2] * D[6];
+        v[7] = D[1] * D[6] - D[0] * D[7];
+        v[2] = D[1] * D[5] - D[2] * D[4];
+        v[5] = D[2] * D[3] - D[0] * D[5];
+        v[8] = D[0] * D[4] - D[1] * D[3];
+        const double det = D[0] * v[0] + D[1] * v[3] + D[2] * v[6];
+        if (det == 0.0) {
+            *err = "degenerate element " + std::to_string(e);
+            return false;
+        }
+        const double s = 1.0 / det;
+        for (int k = 0; k < 9; k++) g->dminv[9 * e + k] = v[k] * s;
+        g->vol[e] = det / 6.0;
+        const double E = young_e ? young_e[e] : young;
+        const double nu = poisson_e ? poisson_e[e] : poisson;
+        g->lam[e] = E * nu / (1.0 + nu) / (1.0 - 2.0 * nu);
+        g->mu[e] = 0.5 * E / (1.0 + nu);
+        const double lm = 0.25 * g->vol[e] * density;
+        for (int k = 0; k < 4; k++) msum[t[k]] += lm;
+    }
+    // a node that no element references has neither stiffness nor mass: it is
+    // held like a fixed node (its block would be singular)
+    g->held.assign(fixed, fixed + nn);
+    std::vector<uint8_t> used(nn, 0);
+    for (int64_t k = 0; k < 4 * ne; k++) used[tets[k]] = 1;
+    for (int64_t i = 0; i < nn; i++)
+        if (!used[i]) g->held[i] = 1;
+    g->mass.assign(nn, 0.0);
+    g->nheld = 0;
+    for (int64_t i = 0; i < nn; i++) {
+        if (g->held[i]) {
+            g->nheld++;
+            continue;
+        }
+        const double inv = msum[i] != 0 ? 1.0 / msum[i] : 1.0;   // Node::invMass default 1
+        g->mass[i] = 1.0 / inv;
+    }
+    return true;
+}
+
+Partition make_partition(const int32_t* tets, int64_t ne, const double* X0, int64_t nn, const uint8_t* held,
+                         const std::vector<double>& mu, const std::vector<double>& lam, const std::vector<double>& dminv,
+                         const std::vector<double>& vol, int BE)
+{
+    Partition P;
+    // --- space-filling-curve order of elements (centroid Morton code, cubic cells)
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    for (int64_t i = 0; i < nn; i++)
+        for (int k = 0; k < 3; k++) {
+            lo[k] = std::min(lo[k], X0[3 * i + k]);
+            hi[k] = std::max(hi[k], X0[3 * i + k]);
+        }
+    double ext = std::max(hi[0] - lo[0], std::max(hi[1] - lo[1], hi[2] - lo[2]));
+    if (!(ext > 0)) ext = 1;
+    const double scale = (double)((1 << 21) - 1) / ext;
+    std::vector<std::pair<uint64_t, int64_t>> key(ne);
+    for (int64_t e = 0; e < ne; e++) {
+        double cc[3] = {0, 0, 0};
+        for (int v = 0; v < 4; v++)
+            for (int k = 0; k < 3; k++) cc[k] += 0.25 * X0[3 * (int64_t)tets[4 * e + v] + k];
+        uint64_t q[3];
+        for (int k = 0; k < 3; k++) q[k] = (uint64_t)std::max(0.0, std::min((cc[k] - lo[k]) * scale, (double)((1 << 21) - 1)));
+        key[e] = {spread21(q[0]) | spread21(q[1]) << 1 | spread21(q[2]) << 2, e};
+    }
+    std::sort(key.begin(), key.end());
+
+    const int nblk = (int)((ne + BE - 1) / BE);
+    const int64_t ne_pad = (int64_t)nblk * BE;
+    P.nblk = nblk;
+    P.ne_pad = ne_pad;
+    P.bnode_ptr.assign(nblk + 1, 0);
+    P.lidx.assign((size_t)ne_pad * 2, 0);
+    P.lperm.assign((size_t)ne_pad * 2, 0);
+    P.dm32.assign((size_t)9 * ne_pad, 0.f);
+    P.cf32.assign((size_t)2 * ne_pad, 0.f);
+    P.dm64.assign((size_t)9 * ne_pad, 0.0);
+    P.cf64.assign((size_t)2 * ne_pad, 0.0);
+    std::vector<int> node_blocks(nn, 0);
+    P.bnodes.reserve((size_t)ne);
+    P.lptr.reserve((size_t)ne);
+    std::vector<int> ids, cnt;
+    for (int b = 0; b < nblk; b++) {
+        const int64_t e0 = (int64_t)b * BE;
+        const int64_t e1 = std::min<int64_t>(ne, e0 + BE);
+        ids.clear();
+        for (int64_t s = e0; s < e1; s++)
+            for (int v = 0; v < 4; v++) ids.push_back(tets[4 * key[s].second + v]);
+        std::sort(ids.begin(), ids.end());
+        ids.erase(std::unique(ids.begin(), ids.end()), ids.end());
+        const int nb = (int)ids.size();
+        P.nmax = std::max(P.nmax, nb);
+        P.bnode_ptr[b + 1] = P.bnode_ptr[b] + nb;
+        for (int id : ids) {
+            P.bnodes.push_back((uint32_t)id | (held[id] ? F_FIXED : 0u));
+            node_blocks[id]++;
+        }
+        // local ids and the per-node incidence lists (counting sort, element order)
+        cnt.assign(nb + 1, 0);
+        for (int64_t s = e0; s < e1; s++) {
+            const int64_t e = key[s].second;
+            int l[4];
+            for (int v = 0; v < 4; v++) {
+                l[v] = (int)(std::lower_bound(ids.begin(), ids.end(), tets[4 * e + v]) - ids.begin());
+                cnt[l[v] + 1]++;
+            }
+            P.lidx[2 * s] = (uint32_t)l[0] | (uint32_t)l[1] << 16;
+            P.lidx[2 * s + 1] = (uint32_t)l[2] | (uint32_t)l[3] << 16;
+            for (int k = 0; k < 9; k++) {
+                P.dm64[(size_t)k * ne_pad + s] = dminv[9 * e + k];
+                P.dm32[(size_t)k * ne_pad + s] = (float)dminv[9 * e + k];
+            }
+            const double cm = vol[e] * mu[e], cl = vol[e] * lam[e];
+            P.cf64[s] = cm;
+            P.cf64[ne_pad + s] = cl;
+            P.cf32[s] = (float)cm;
+            P.cf32[ne_pad + s] = (float)cl;
+        }
+        for (int j = 0; j < nb; j++) cnt[j + 1] += cnt[j];
+        for (int j = 0; j <= nb; j++) P.lptr.push_back((uint16_t)cnt[j]);
+        std::vector<int> fill(cnt.begin(), cnt.end() - 1);
+        for (int64_t s = e0; s < e1; s++) {
+            const uint32_t* li = &P.lidx[2 * s];
+            const int l[4] = {(int)(li[0] & 0xffff), (int)(li[0] >> 16), (int)(li[1] & 0xffff), (int)(li[1] >> 16)};
+            // corner v of element s -> its slot in the node-sorted incidence order
+            // (counting sort in element order: a node's slots list its elements in order)
+            uint16_t k[4];
+            for (int v = 0; v < 4; v++) k[v] = (uint16_t)fill[l[v]]++;
+            P.lperm[2 * s] = (uint32_t)k[0] | (uint32_t)k[1] << 16;
+            P.lperm[2 * s + 1] = (uint32_t)k[2] | (uint32_t)k[3] << 16;
+        }
+    }
+    // shared nodes: flag, and list their partial rows in block order
+    std::vector<int> sh_of(nn, -1);
+    for (int64_t i = 0; i < nn; i++)
+        if (node_blocks[i] > 1) {
+            sh_of[i] = (int)P.sh_node.size();
+            P.sh_node.push_back((uint32_t)i | (held[i] ? F_FIXED : 0u));
+        }
+    P.sh_ptr.assign(P.sh_node.size() + 1, 0);
+    for (size_t r = 0; r < P.bnodes.size(); r++) {
+        const uint32_t id = P.bnodes[r] & F_NODE_MASK;
+        if (sh_of[id] >= 0) {
+            P.bnodes[r] |= F_SHARED;
+            P.sh_ptr[sh_of[id] + 1]++;
+        }
+    }
+    for (size_t i = 0; i < P.sh_node.size(); i++) P.sh_ptr[i + 1] += P.sh_ptr[i];
+    // each shared node's partial rows get contiguous slots, in block order
+    P.bslot.assign(P.bnodes.size(), -1);
+    std::vector<int> fill(P.sh_ptr.begin(), P.sh_ptr.end() - 1);
+    for (size_t r = 0; r < P.bnodes.size(); r++) {   // rows are in block order
+        const uint32_t id = P.bnodes[r] & F_NODE_MASK;
+        if (sh_of[id] >= 0) P.bslot[r] = fill[sh_of[id]]++;
+    }
+    // rest coordinates of every block row relative to the centre of the block's
+    // bounding box (FP64 difference, then FP32): |coordinate| <= half the block
+    // diameter, so the FP32 edge vectors the Hessian-vector kernel forms from
+    // them are accurate to ~2^-24 block diameter / edge
+    P.rel32.assign(3 * P.bnodes.size(), 0.f);
+    for (int b = 0; b < nblk; b++) {
+        double blo[3] = {1e300, 1e300, 1e300}, bhi[3] = {-1e300, -1e300, -1e300};
+        for (int r = P.bnode_ptr[b]; r < P.bnode_ptr[b + 1]; r++) {
+            const int64_t id = P.bnodes[r] & F_NODE_MASK;
+            for (int k = 0; k < 3; k++) {
+                blo[k] = std::min(blo[k], X0[3 * id + k]);
+                bhi[k] = std::max(bhi[k], X0[3 * id + k]);
+            }
+        }
+        for (int r = P.bnode_ptr[b]; r < P.bnode_ptr[b + 1]; r++) {
+            const int64_t id = P.bnodes[r] & F_NODE_MASK;
+            for (int k = 0; k < 3; k++) P.rel32[3 * (size_t)r + k] = (float)(X0[3 * id + k] - 0.5 * (blo[k] + bhi[k]));
+        }
+    }
+    // Lame parameters for the Hessian-vector kernel: per element only when they vary
+    P.uniform = true;
+    for (int64_t e = 1; e < ne && P.uniform; e++) P.uniform = mu[e] == mu[0] && lam[e] == lam[0];
+    if (!P.uniform) {
+        P.mat32.assign((size_t)2 * ne_pad, 0.f);
+        for (int64_t s2 = 0; s2 < ne; s2++) {
+            P.mat32[s2] = (float)mu[key[s2].second];
+            P.mat32[ne_pad + s2] = (float)lam[key[s2].second];
+        }
+    }
+    P.eorig.resize(ne);
+    for (int64_t s2 = 0; s2 < ne; s2++) P.eorig[s2] = (int)key[s2].second;
+    // per node, for the CG update: a shared node's partial slot range {first,
+    // end}; {-1, 0} for a free node of one block; {-2, 0} for a held node and
+    // for a node that no element references (its row of K is 0); after the nn
+    // pairs, one more pair that stays 0 (a flag that is never set)
+    P.node_q.assign(2 * nn + 2, 0);
+    for (int64_t i = 0; i < nn; i++) {
+        if (held[i] || node_blocks[i] == 0) P.node_q[2 * i] = -2;
+        else if (sh_of[i] < 0) P.node_q[2 * i] = -1;
+        else {
+            P.node_q[2 * i] = P.sh_ptr[sh_of[i]];
+            P.node_q[2 * i + 1] = P.sh_ptr[sh_of[i] + 1];
+        }
+    }
+    return P;
+}
+
+void node_triangle_csr(const int32_t* tris, int64_t ntri, int64_t nn, std::vector<int>* ptr, std::vector<int>* idx)
+{
+    ptr->assign(nn + 1, 0);
+    for (int64_t i = 0; i < 3 * ntri; i++) (*ptr)[tris[i] + 1]++;
+    for (int64_t i = 0; i < nn; i++) (*ptr)[i + 1] += (*ptr)[i];
+    idx->assign(ptr->back(), 0);
+    std::vector<int> fill(ptr->begin(), ptr->end() - 1);
+    for (int64_t k = 0; k < ntri; k++)
+        for (int v = 0; v < 3; v++) (*idx)[fill[tris[3 * k + v]]++] = (int)k;
+}
+
+void transpose_prolongation(const int64_t* p_ptr, const int32_t* p_col, const double* p_val, int64_t na, int64_t nc,
+                            std::vector<int>* tptr, std::vector<int>* trow, std::vector<float>* tval)
+{
+    const int64_t nnz = p_ptr[na];
+    tptr->assign(nc + 1, 0);
+    trow->assign(nnz, 0);
+    tval->assign(nnz, 0.f);
+    for (int64_t k = 0; k < nnz; k++) (*tptr)[p_col[k] + 1]++;
+    for (int64_t j = 0; j < nc; j++) (*tptr)[j + 1] += (*tptr)[j];
+    std::vector<int> fill(tptr->begin(), tptr->end() - 1);
+    for (int64_t i = 0; i < na; i++)
+        for (int64_t k = p_ptr[i]; k < p_ptr[i + 1]; k++) {
+            const int slot = fill[p_col[k]]++;
+            (*trow)[slot] = (int)i;
+            (*tval)[slot] = (float)p_val[k];
+        }
+}
+
+void cable_attachments(const std::vector<int>& node4, int64_t nn, std::vector<int>* touched, std::vector<int>* tptr,
+                       std::vector<std::pair<int, int>>* tent)
+{
+    const int n = (int)(node4.size() / 4);
+    std::vector<std::vector<std::pair<int, int>>> att(nn);
+    for (int j = 0; j < n; j++)
+        for (int k = 0; k < 4; k++) att[node4[4 * j + k]].push_back({j, 4 * j + k});
+    touched->clear();
+    tent->clear();
+    tptr->assign(1, 0);
+    for (int64_t i = 0; i < nn; i++)
+        if (!att[i].empty()) {
+            touched->push_back((int)i);
+            tent->insert(tent->end(), att[i].begin(), att[i].end());
+            tptr->push_back((int)tent->size());
+        }
+}
+
+}  // namespace sfem

@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "sfem_capi.h"
+#include "sfem_host.hpp"
 #include "sfem_kernels.hpp"
 
 
@@ -58,21 +59,6 @@ struct DevBuf {   // owning device allocation
     }
     template <typename T> T* as() const { return static_cast<T*>(p); }
 };
-
-constexpr uint32_t F_SHARED = 0x80000000u;
-constexpr uint32_t F_FIXED = 0x40000000u;
-
-// 21-bit-per-axis Morton code
-inline uint64_t spread21(uint64_t v)
-{
-    v &= 0x1fffff;
-    v = (v | v << 32) & 0x1f00000000ffffULL;
-    v = (v | v << 16) & 0x1f0000ff0000ffULL;
-    v = (v | v << 8) & 0x100f00f00f00f00fULL;
-    v = (v | v << 4) & 0x10c30c30c30c30c3ULL;
-    v = (v | v << 2) & 0x1249249249249249ULL;
-    return v;
-}
 
 }  // namespace sfem
 

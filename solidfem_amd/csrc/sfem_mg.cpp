@@ -448,17 +448,9 @@ int sfem_mg_add_level(sfem_ctx* f, const sfem_mesh* coarse, const int32_t* injec
         }
         // transpose: for each coarse node its (fine row, weight) entries in
         // increasing fine-row order (the restriction's fixed summation order)
-        std::vector<int> tptr(nc + 1, 0), trow(nnz);
-        std::vector<float> tval(nnz);
-        for (int64_t k = 0; k < nnz; k++) tptr[col[k] + 1]++;
-        for (int64_t j = 0; j < nc; j++) tptr[j + 1] += tptr[j];
-        std::vector<int> fill(tptr.begin(), tptr.end() - 1);
-        for (int64_t i = 0; i < na; i++)
-            for (int k = ptr[i]; k < ptr[i + 1]; k++) {
-                const int slot = fill[col[k]]++;
-                trow[slot] = (int)i;
-                tval[slot] = val[k];
-            }
+        std::vector<int> tptr, trow;
+        std::vector<float> tval;
+        transpose_prolongation(p_ptr, p_col, p_val, na, nc, &tptr, &trow, &tval);
         // only the coarsest level is solved directly: a level that now has one
         // below it gives its dense set-up back
         if (!f->mg.empty() && f->mg.back()->dense) {
