@@ -45,10 +45,18 @@ def build(force=False):
         subprocess.check_call(["make", "-s", "-C", HERE, "all"])
 
 
+TEMPLATES_PATH = os.path.join(HERE, "_ref", "ref_templates")
+
+
 def build_ref():
+    """the compiled reference library and, when libsfem_hip.so is built, the
+    driver that instantiates the reference's solver templates on this
+    library's C++ class (oracle/ref_templates.cpp)"""
     ref_root = os.environ.get("SFEM_REFERENCE", "/root/reference")
     if os.path.isdir(os.path.join(ref_root, "Source")):
         subprocess.check_call(["make", "-s", "-C", HERE, "ref", f"REF={ref_root}"])
+        if os.path.exists(os.path.join(os.path.dirname(HERE), "solidfem_amd", "libsfem_hip.so")):
+            subprocess.check_call(["make", "-s", "-C", HERE, "templates", f"REF={ref_root}"])
     return os.path.exists(REF_PATH)
 
 

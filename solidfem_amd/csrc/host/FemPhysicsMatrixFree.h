@@ -18,10 +18,16 @@
 
 #include "sfem_capi.h"
 
-namespace FEM_SYSTEM {
-
+// The reference keeps `real`, `uint32` in the global namespace and the math
+// types in namespace Math, pulled into the global one by `using namespace
+// Math` (Include/FemDataStructures.h:52-57, Engine/Types.h, Math/matrix3.h:38);
+// the same arrangement here, so that templates written against it (the
+// global std::vector operators of Math/IterativeSolvers.h found from inside
+// FEM_SYSTEM) resolve the same way.
 typedef double real;
 typedef uint32_t uint32;
+
+namespace Math {
 
 // 3-vector with the reference's layout and operators (Math/vector3.h:40-275):
 // x, y, z aliasing v[3]; Vector3T(a) fills all three; u * v is the dot
@@ -172,6 +178,14 @@ template <typename R> Vector3T<R> operator*(const Matrix3T<R>& M, const Vector3T
 template <typename R> Matrix3T<R> operator*(double s, Matrix3T<R> M) { M.Scale(R(s)); return M; }
 template <typename R> Matrix3T<R> operator*(float s, Matrix3T<R> M) { M.Scale(R(s)); return M; }
 
+}  // namespace Math
+
+using namespace Math;
+
+namespace FEM_SYSTEM {
+
+using ::real;
+using ::uint32;
 typedef Vector3T<real> Vector3R;
 typedef Matrix3T<real> Matrix3R;
 
@@ -424,6 +438,11 @@ private:
     void Check(int rc, const char* what) const;
     void PushMultigridLevel(size_t k);
     void PushCable(const Cable& cable);
+
+    // a driver's access to the private solve entry points and the load
+    // fraction, as the reference grants (FemPhysicsMatrixFree.h:152-153)
+    friend class FemSystem;
+    friend class ElasticProblem;
 
     sfem_ctx* mCtx = nullptr;
     Config mConfig;
