@@ -11,8 +11,13 @@
 // or the enum value), gravity, zup, inner_iters, rel_tol, cg_tol, precond
 // ("none", "jacobi", "learned", "multigrid", "multigrid_learned"), force_step,
 // young_e / poisson_e (per element arrays).
-// Unlike the reference's FemBody::BuildBoundaryMesh, no uint16 node indices
-// are involved anywhere (meshes above 65,535 nodes work).
+// The rest of the reference's surface: module-level load_from_xml, and
+// save_to_obj, get_boundary_mesh, get_visual_mesh, get_hessian,
+// compute_force_param_grads with get_force_mu/lambda/rho_grad, and
+// set_cable_actuation (PySolidFEM/Module.cpp:43-111); added: add_cable,
+// get_cable_positions, compute_energy, build_boundary_mesh.  Unlike the
+// reference's FemBody::BuildBoundaryMesh, no uint16 node indices are involved
+// anywhere (meshes above 65,535 nodes keep their boundary).
 #include <pybind11/numpy.h>
 #include <pybind11/pybind11.h>
 #include <pybind11/stl.h>
@@ -24,6 +29,7 @@
 
 #include <fstream>
 
+#include "FemBody.h"
 #include "FemIO.h"
 #include "FemPhysicsMatrixFree.h"
 
@@ -71,6 +77,172 @@ void Prepare(std::vector<Node>& nodes, std::vector<Tet>& tets)
     }
 }
 
+// IO::LoadFromXmlFile (FemIO.cpp:583-847), the parts the matrix-free path
+// uses: a tet1 model (scale, swap, visual mesh path), its Dirichlet nodes,
+// material, simulation type / steps / gravity / pressure surface, solver
+// iterations and tolerance, and the first <cable>
+struct Scene {
+    std::vector<Node> nodes;
+    std::vector<Tet> tets;
+    std::vector<int> fixed, tris;
+    FemConfig config;
+    real pressure = 0, scale = 1;
+    std::string visualPath;
+    std::vector<CableDescriptor> cables;
+};
+
+std::vector<int> Ints(py::handle text)
+{
+    std::vector<int> v;
+    if (text.is_none()) return v;
+    std::string s = py::cast<std::string>(text);
+    for (char& ch : s)
+        if (ch == ',') ch = ' ';
+    std::istringstream in(s);
+    int k;
+    while (in >> k) v.push_back(k);
+    return v;
+}
+
+Scene LoadScene(const std::string& path)
+{
+    Scene sc;
+    py::object ET = py::module_::import("xml.etree.ElementTree");
+    py::object root = ET.attr("parse")(path).attr("getroot")();
+    if (py::cast<std::string>(root.attr("tag")) != "solidfem") throw std::invalid_argument(path + ": not a <solidfem> scene");
+    auto attr = [](py::handle el, const char* name) -> py::object { return el.attr("get")(name); };
+    auto num = [&](py::handle el, const char* name) { return std::stod(py::cast<std::string>(attr(el, name))); };
+    const std::string dir = path.substr(0, path.find_last_of("/\\") + 1);
+    FemConfig& c = sc.config;
+    c.mVerbose = false;
+    py::object model = root.attr("find")("model");
+    if (model.is_none()) throw std::invalid_argument(path + ": no <model>");
+    py::object mpath = attr(model, "path"), type = attr(model, "type");
+    if (mpath.is_none() || type.is_none() || py::cast<std::string>(type) != "tet1")
+        throw std::invalid_argument(path + ": only tet1 models are supported");
+    sc.scale = attr(model, "scale").is_none() ? 1 : num(model, "scale");
+    if (!attr(model, "visual").is_none()) sc.visualPath = dir + py::cast<std::string>(attr(model, "visual"));
+    if (!IO::LoadFromTet1File((dir + py::cast<std::string>(mpath)).c_str(), sc.scale, Vector3R(), sc.nodes, sc.tets))
+        throw std::invalid_argument(path + ": cannot read the model");
+    if (!attr(model, "swap").is_none()) {
+        const std::string sw = py::cast<std::string>(attr(model, "swap"));
+        const int a = sw == "yz" ? 1 : 0, b = sw == "xy" ? 1 : 2;
+        if (sw == "xy" || sw == "xz" || sw == "yz")
+            for (Node& n : sc.nodes) {
+                std::swap(n.pos[a], n.pos[b]);
+                n.pos0 = n.pos;
+            }
+    }
+    py::object dir_el = model.attr("find")("dirichlet");
+    if (!dir_el.is_none()) sc.fixed = Ints(dir_el.attr("text"));
+    py::object mat = root.attr("find")("material");
+    if (!mat.is_none()) {
+        if (!attr(mat, "young").is_none()) c.mYoungsModulus = num(mat, "young");
+        if (!attr(mat, "poisson").is_none()) c.mPoissonRatio = num(mat, "poisson");
+        if (!attr(mat, "density").is_none()) c.mDensity = num(mat, "density");
+    }
+    py::object sim = root.attr("find")("simulation");
+    if (!sim.is_none()) {
+        const std::string st = attr(sim, "type").is_none() ? "" : py::cast<std::string>(attr(sim, "type"));
+        if (st == "quasi-static") c.mSimType = ST_QUASI_STATIC;
+        else if (st == "explicit") c.mSimType = ST_EXPLICIT;
+        if (!attr(sim, "method").is_none() && py::cast<std::string>(attr(sim, "method")) == "mixed")
+            throw std::invalid_argument("the mixed (pressure) formulation is not on the matrix-free path");
+        if (!attr(sim, "steps").is_none()) {
+            const int n = std::stoi(py::cast<std::string>(attr(sim, "steps")));
+            if (c.mSimType < ST_EXPLICIT) c.mForceApplicationStep = real(1) / n;
+            else c.mNumSubsteps = n;
+        }
+        if (!attr(sim, "gravity").is_none()) c.mGravity = num(sim, "gravity");
+        py::object pr = sim.attr("find")("pressure");
+        if (!pr.is_none()) {
+            if (!attr(pr, "p").is_none()) sc.pressure = num(pr, "p");
+            sc.tris = Ints(pr.attr("text"));
+        }
+    }
+    py::object sol = root.attr("find")("solver");
+    if (!sol.is_none()) {
+        if (!attr(sol, "iterations").is_none()) c.mOuterIterations = std::stoi(py::cast<std::string>(attr(sol, "iterations")));
+        if (!attr(sol, "tolerance").is_none()) c.mAbsNewtonRsidualThreshold = num(sol, "tolerance");
+    }
+    py::object cab = root.attr("find")("cable");
+    if (!cab.is_none()) {
+        // FemIO.cpp:801-845
+        CableDescriptor d;
+        if (!attr(cab, "divs").is_none()) d.divs = std::stoi(py::cast<std::string>(attr(cab, "divs")));
+        if (!attr(cab, "len").is_none()) d.length = num(cab, "len");
+        if (!attr(cab, "offset").is_none()) {
+            std::string o = py::cast<std::string>(attr(cab, "offset"));
+            for (char& ch : o)
+                if (ch == ',') ch = ' ';
+            std::istringstream in(o);
+            real x = 0, y = 0, z = 0;
+            in >> x >> y >> z;
+            d.offset = Vector3R(x, y, z);
+        }
+        if (!attr(cab, "axis").is_none()) {
+            const std::string ax = py::cast<std::string>(attr(cab, "axis"));
+            if (!ax.empty() && ax[0] == 'y') d.dir = Vector3R(0, 1, 0);
+            else if (!ax.empty() && ax[0] == 'z') d.dir = Vector3R(0, 0, 1);
+        }
+        if (!attr(cab, "stiffness").is_none()) d.stiffness = num(cab, "stiffness");
+        sc.cables.push_back(d);
+    }
+    for (int i : sc.fixed) {
+        if (i < 0 || (size_t)i >= sc.nodes.size()) throw std::invalid_argument("fixed node index out of range");
+        sc.nodes[i].invMass = 0;
+    }
+    return sc;
+}
+
+// PyLoadFromXml (PySolidFEM/Module.cpp:43-87): (nodes (nn, 3), tets (ne, 4), fixed node list)
+py::tuple PyLoadFromXml(const std::string& path)
+{
+    Scene sc = LoadScene(path);
+    py::array_t<double> nodes({(py::ssize_t)sc.nodes.size(), (py::ssize_t)3});
+    double* pn = nodes.mutable_data();
+    for (size_t i = 0; i < sc.nodes.size(); i++)
+        for (int c = 0; c < 3; c++) pn[3 * i + c] = sc.nodes[i].pos[c];
+    py::array_t<int> tets({(py::ssize_t)sc.tets.size(), (py::ssize_t)4});
+    int* pt = tets.mutable_data();
+    for (size_t e = 0; e < sc.tets.size(); e++)
+        for (int k = 0; k < 4; k++) pt[4 * e + k] = (int)sc.tets[e].idx[k];
+    return py::make_tuple(nodes, tets, py::cast(sc.fixed));
+}
+
+// a Mesh as the reference's bindings return it (PyNonlinearFEM.cpp:196-238):
+// (vertices (n, 3), flat triangle index list, boundary node list), 32-bit
+// indices throughout (the reference's uint16 truncation, FemBody.cpp:263,
+// 310-313, is not reproduced)
+py::tuple MeshTuple(const Mesh& mesh, const std::vector<int>& boundaryNodes)
+{
+    py::array_t<double> verts({(py::ssize_t)mesh.vertices.size(), (py::ssize_t)3});
+    double* pv = verts.mutable_data();
+    for (size_t i = 0; i < mesh.vertices.size(); i++)
+        for (int c = 0; c < 3; c++) pv[3 * i + c] = mesh.vertices[i][c];
+    return py::make_tuple(verts, py::cast(mesh.indices), py::cast(boundaryNodes));
+}
+
+// FemBody::BuildBoundaryMesh on a mesh (tets (ne, 4), nodes (nn, 3)), without
+// a physics context: the mesh tuple above at the given positions
+py::tuple PyBuildBoundaryMesh(py::array_t<int, py::array::c_style | py::array::forcecast> tets,
+                              py::array_t<double, py::array::c_style | py::array::forcecast> nodes)
+{
+    if (nodes.ndim() != 2 || nodes.shape(1) != 3 || tets.size() % 4) throw std::invalid_argument("tets (ne, 4), nodes (nn, 3)");
+    std::vector<Node> n(nodes.shape(0));
+    for (size_t i = 0; i < n.size(); i++) n[i].pos = n[i].pos0 = Vector3R(nodes.data()[3 * i], nodes.data()[3 * i + 1], nodes.data()[3 * i + 2]);
+    std::vector<Tet> t(tets.size() / 4);
+    for (size_t e = 0; e < t.size(); e++)
+        for (int k = 0; k < 4; k++) {
+            const int v = tets.data()[4 * e + k];
+            if (v < 0 || (size_t)v >= n.size()) throw std::invalid_argument("tet index out of range");
+            t[e].idx[k] = (uint32)v;
+        }
+    FemBody body;
+    body.BuildBoundaryMesh(t, n);
+    return MeshTuple(body.GetBoundaryMesh(), body.GetBoundaryNodes());
+}
+
 class PyNonlinearFEM {
 public:
     PyNonlinearFEM(py::array_t<int, py::array::c_style | py::array::forcecast> tets,
@@ -99,6 +271,7 @@ public:
                 t[e].idx[k] = (uint32)pt[4 * e + k];
             }
         Prepare(n, t);
+        mBody.BuildBoundaryMesh(t, n);
         FemConfig cfg = ParseConfig(config);
         mPhys.reset(new FemPhysicsMatrixFree(t, n, cfg));
         if (config.contains("young_e") || config.contains("poisson_e")) {
@@ -125,8 +298,85 @@ public:
         IO::ExportToVTKHeatMap(out, *mPhys);
     }
 
-    // Step (PyNonlinearFEM.cpp:126-132)
-    void Step(real dt) { mPhys->Step(dt); }
+    // Step (PyNonlinearFEM.cpp:126-132): the physics step, then the boundary
+    // and visual meshes follow (SimulateStatic below does not update them, as
+    // in the reference)
+    void Step(real dt)
+    {
+        mPhys->Step(dt);
+        mBody.UpdateBoundaryMesh(*mPhys);
+        if (mBody.HasVisualMesh()) mBody.UpdateVisualMesh();
+    }
+
+    // SaveToOBJ (:186-194): the visual mesh when there is one, else the boundary mesh
+    void SaveToOBJ(const std::string& path) const
+    {
+        FILE* f = std::fopen(path.c_str(), "wt");
+        if (!f) throw std::runtime_error("cannot write " + path);
+        ExportMeshToOBJ(f, mBody.HasVisualMesh() ? mBody.GetVisualMesh() : mBody.GetBoundaryMesh());
+        std::fclose(f);
+    }
+
+    // GetBoundaryMesh, GetVisualMesh (:196-238)
+    py::tuple GetBoundaryMesh() const { return MeshTuple(mBody.GetBoundaryMesh(), mBody.GetBoundaryNodes()); }
+    py::tuple GetVisualMesh() const { return MeshTuple(mBody.GetVisualMesh(), mBody.GetBoundaryNodes()); }
+
+    // GetHessian (PyNonlinearFEM.h:67): dense, free dofs in shuffled order
+    py::array_t<double> GetHessian() const
+    {
+        std::vector<real> K;
+        mPhys->GetHessian(K);
+        const py::ssize_t n = 3 * (py::ssize_t)mPhys->GetNumFreeNodes();
+        py::array_t<double> out({n, n});
+        std::copy(K.begin(), K.end(), out.mutable_data());
+        return out;
+    }
+
+    // ComputeForceParamGrads and its getters (PyNonlinearFEM.h:68-71); the
+    // reference's get_force_rho_grad returns the lambda gradient, here the
+    // density gradient
+    void ComputeForceParamGrads() { mPhys->GetForceParamGrads(mGradMu, mGradLambda, mGradRho); }
+    py::array_t<double> GetForceMuGrad() const { return py::array_t<double>(mGradMu.size(), mGradMu.data()); }
+    py::array_t<double> GetForceLambdaGrad() const { return py::array_t<double>(mGradLambda.size(), mGradLambda.data()); }
+    py::array_t<double> GetForceRhoGrad() const { return py::array_t<double>(mGradRho.size(), mGradRho.data()); }
+
+    // SetCableActuation (PyNonlinearFEM.h:74); AddCable takes a cable of n
+    // nodes in elements elem at barycentric weights bary (n, 3)
+    void SetCableActuation(uint32 cable, real actuation) { mPhys->SetCableActuation(cable, actuation); }
+    int AddCable(py::array_t<int, py::array::c_style | py::array::forcecast> elem,
+                 py::array_t<double, py::array::c_style | py::array::forcecast> bary, real rest, real stiffness, real damping)
+    {
+        if (bary.size() != 3 * elem.size()) throw std::invalid_argument("add_cable: bary must be n x 3");
+        Cable cb;
+        cb.mCableNodes.resize(elem.size());
+        for (py::ssize_t i = 0; i < elem.size(); i++) {
+            cb.mCableNodes[i].elem = elem.data()[i];
+            cb.mCableNodes[i].bary = Vector3R(bary.data()[3 * i], bary.data()[3 * i + 1], bary.data()[3 * i + 2]);
+        }
+        cb.mCablePositions.resize(elem.size());
+        cb.mCableRestLength = rest;
+        cb.mCableStiffness = stiffness;
+        cb.mCableDamping = damping;
+        AddCreatedCable(cb);
+        return (int)mPhys->GetNumCables() - 1;
+    }
+    int NumCables() const { return (int)mPhys->GetNumCables(); }
+    py::array_t<double> GetCablePositions(uint32 cable) const
+    {
+        if (cable >= mPhys->GetNumCables()) throw std::out_of_range("no such cable");
+        const size_t n = mPhys->GetCable(cable).mCableNodes.size();
+        py::array_t<double> out({(py::ssize_t)n, (py::ssize_t)3});
+        double* p = out.mutable_data();
+        for (size_t i = 0; i < n; i++) {
+            const Vector3R v = mPhys->GetCablePosition(cable, (uint32)i);
+            for (int c = 0; c < 3; c++) p[3 * i + c] = v[c];
+        }
+        return out;
+    }
+
+    // FemPhysicsMatrixFree::ComputeEnergy (FemPhysicsMatrixFree.cpp:882-916)
+    real ComputeEnergy() { return mPhys->ComputeEnergy(); }
+    real ComputeElasticEnergy() { return mPhys->ComputeElasticEnergy(); }
 
     // SimulateStatic (:134-143): step until the load fraction reaches 1
     void SimulateStatic()
@@ -238,87 +488,10 @@ public:
     bool CheckForInversion() const { return mPhys->CheckForInversion(0); }
 
 private:
-    static std::vector<int> Ints(py::handle text)
-    {
-        std::vector<int> v;
-        if (text.is_none()) return v;
-        std::string s = py::cast<std::string>(text);
-        for (char& ch : s)
-            if (ch == ',') ch = ' ';
-        std::istringstream in(s);
-        int k;
-        while (in >> k) v.push_back(k);
-        return v;
-    }
-
     void InitFromXml(const std::string& path, py::dict* config)
     {
-        py::object ET = py::module_::import("xml.etree.ElementTree");
-        py::object root = ET.attr("parse")(path).attr("getroot")();
-        if (py::cast<std::string>(root.attr("tag")) != "solidfem") throw std::invalid_argument(path + ": not a <solidfem> scene");
-        auto attr = [](py::handle el, const char* name) -> py::object { return el.attr("get")(name); };
-        const std::string dir = path.substr(0, path.find_last_of("/\\") + 1);
-        std::vector<Node> nodes;
-        std::vector<Tet> tets;
-        std::vector<int> fixed, tris;
-        FemConfig c;
-        c.mVerbose = false;
-        py::object model = root.attr("find")("model");
-        if (model.is_none()) throw std::invalid_argument(path + ": no <model>");
-        py::object mpath = attr(model, "path"), type = attr(model, "type");
-        if (mpath.is_none() || type.is_none() || py::cast<std::string>(type) != "tet1")
-            throw std::invalid_argument(path + ": only tet1 models are supported");
-        const real scale = attr(model, "scale").is_none() ? 1 : std::stod(py::cast<std::string>(attr(model, "scale")));
-        if (!IO::LoadFromTet1File((dir + py::cast<std::string>(mpath)).c_str(), scale, Vector3R(), nodes, tets))
-            throw std::invalid_argument(path + ": cannot read the model");
-        if (!attr(model, "swap").is_none()) {
-            const std::string sw = py::cast<std::string>(attr(model, "swap"));
-            const int a = sw == "yz" ? 1 : 0, b = sw == "xy" ? 1 : 2;
-            if (sw == "xy" || sw == "xz" || sw == "yz")
-                for (Node& n : nodes) {
-                    std::swap(n.pos[a], n.pos[b]);
-                    n.pos0 = n.pos;
-                }
-        }
-        py::object dir_el = model.attr("find")("dirichlet");
-        if (!dir_el.is_none()) fixed = Ints(dir_el.attr("text"));
-        py::object mat = root.attr("find")("material");
-        if (!mat.is_none()) {
-            if (!attr(mat, "young").is_none()) c.mYoungsModulus = std::stod(py::cast<std::string>(attr(mat, "young")));
-            if (!attr(mat, "poisson").is_none()) c.mPoissonRatio = std::stod(py::cast<std::string>(attr(mat, "poisson")));
-            if (!attr(mat, "density").is_none()) c.mDensity = std::stod(py::cast<std::string>(attr(mat, "density")));
-        }
-        real pressure = 0;
-        py::object sim = root.attr("find")("simulation");
-        if (!sim.is_none()) {
-            const std::string st = attr(sim, "type").is_none() ? "" : py::cast<std::string>(attr(sim, "type"));
-            if (st == "quasi-static") c.mSimType = ST_QUASI_STATIC;
-            else if (st == "explicit") c.mSimType = ST_EXPLICIT;
-            if (!attr(sim, "method").is_none() && py::cast<std::string>(attr(sim, "method")) == "mixed")
-                throw std::invalid_argument("the mixed (pressure) formulation is not on the matrix-free path");
-            if (!attr(sim, "steps").is_none()) {
-                const int n = std::stoi(py::cast<std::string>(attr(sim, "steps")));
-                if (c.mSimType < ST_EXPLICIT) c.mForceApplicationStep = real(1) / n;
-                else c.mNumSubsteps = n;
-            }
-            if (!attr(sim, "gravity").is_none()) c.mGravity = std::stod(py::cast<std::string>(attr(sim, "gravity")));
-            py::object pr = sim.attr("find")("pressure");
-            if (!pr.is_none()) {
-                if (!attr(pr, "p").is_none()) pressure = std::stod(py::cast<std::string>(attr(pr, "p")));
-                tris = Ints(pr.attr("text"));
-            }
-        }
-        py::object sol = root.attr("find")("solver");
-        if (!sol.is_none()) {
-            if (!attr(sol, "iterations").is_none()) c.mOuterIterations = std::stoi(py::cast<std::string>(attr(sol, "iterations")));
-            if (!attr(sol, "tolerance").is_none())
-                c.mAbsNewtonRsidualThreshold = std::stod(py::cast<std::string>(attr(sol, "tolerance")));
-        }
-        if (!root.attr("find")("cable").is_none()) throw std::invalid_argument("cables are not on the matrix-free path");
-        for (int i : fixed) {
-            if (i < 0 || (size_t)i >= nodes.size()) throw std::invalid_argument("fixed node index out of range");
-            nodes[i].invMass = 0;
-        }
+        Scene sc = LoadScene(path);
+        FemConfig c = sc.config;
         if (config) {
             c = ParseConfig(*config);
         } else {
@@ -327,9 +500,25 @@ private:
             c.mInnerIterations = 100000;
             c.mCustomConfig = &mCfg;
         }
-        Prepare(nodes, tets);
-        mPhys.reset(new FemPhysicsMatrixFree(tets, nodes, c));
-        if (!tris.empty()) mPhys->SetBoundaryConditionsSurface(std::vector<uint32>(tris.begin(), tris.end()), pressure);
+        Prepare(sc.nodes, sc.tets);
+        mBody.BuildBoundaryMesh(sc.tets, sc.nodes);
+        mPhys.reset(new FemPhysicsMatrixFree(sc.tets, sc.nodes, c));
+        if (!sc.tris.empty())
+            mPhys->SetBoundaryConditionsSurface(std::vector<uint32>(sc.tris.begin(), sc.tris.end()), sc.pressure);
+        // the scene's visual mesh (FemBody::LoadFromXml, FemBody.cpp:106-125) and
+        // cables (PyNonlinearFEM.cpp:88-91)
+        if (!sc.visualPath.empty() && !mBody.LoadVisualMesh(sc.visualPath.c_str(), (float)sc.scale))
+            throw std::invalid_argument(path + ": cannot read the visual mesh " + sc.visualPath);
+        for (const CableDescriptor& d : sc.cables) AddCreatedCable(CreateCable(d, sc.nodes, sc.tets));
+    }
+
+    void AddCreatedCable(const Cable& cb)
+    {
+        for (const SpringNode& n : cb.mCableNodes)
+            if (n.elem < 0)
+                throw std::invalid_argument("cable: a point lies outside the mesh (a free cable node, which the "
+                                            "matrix-free path does not support)");
+        mPhys->AddCable(cb);
     }
 
     FemConfig ParseConfig(py::dict config)
@@ -373,6 +562,8 @@ private:
 
     FemPhysicsMatrixFree::Config mCfg;
     std::unique_ptr<FemPhysicsMatrixFree> mPhys;
+    FemBody mBody;
+    std::vector<real> mGradMu, mGradLambda, mGradRho;
 };
 
 }  // namespace
@@ -380,6 +571,8 @@ private:
 PYBIND11_MODULE(pysolidfem, m)
 {
     m.doc() = "SolidFEM-compatible bindings of the MI355X matrix-free Newton-CG solver";
+    m.def("load_from_xml", &PyLoadFromXml);
+    m.def("build_boundary_mesh", &PyBuildBoundaryMesh);
     py::class_<PyNonlinearFEM>(m, "NonlinearFEM")
         .def(py::init<py::array_t<int, py::array::c_style | py::array::forcecast>,
                       py::array_t<double, py::array::c_style | py::array::forcecast>,
@@ -404,5 +597,20 @@ PYBIND11_MODULE(pysolidfem, m)
         .def("num_multigrid_levels", &PyNonlinearFEM::NumMultigridLevels)
         .def("last_stats", &PyNonlinearFEM::LastStats)
         .def("get_force_fraction", &PyNonlinearFEM::GetForceFraction)
-        .def("check_for_inversion", &PyNonlinearFEM::CheckForInversion);
+        .def("check_for_inversion", &PyNonlinearFEM::CheckForInversion)
+        .def("save_to_obj", &PyNonlinearFEM::SaveToOBJ)
+        .def("get_boundary_mesh", &PyNonlinearFEM::GetBoundaryMesh)
+        .def("get_visual_mesh", &PyNonlinearFEM::GetVisualMesh)
+        .def("get_hessian", &PyNonlinearFEM::GetHessian)
+        .def("compute_force_param_grads", &PyNonlinearFEM::ComputeForceParamGrads)
+        .def("get_force_mu_grad", &PyNonlinearFEM::GetForceMuGrad)
+        .def("get_force_lambda_grad", &PyNonlinearFEM::GetForceLambdaGrad)
+        .def("get_force_rho_grad", &PyNonlinearFEM::GetForceRhoGrad)
+        .def("set_cable_actuation", &PyNonlinearFEM::SetCableActuation)
+        .def("add_cable", &PyNonlinearFEM::AddCable, py::arg("elem"), py::arg("bary"), py::arg("rest_length"),
+             py::arg("stiffness") = 1000.0, py::arg("damping") = 0.0)
+        .def("num_cables", &PyNonlinearFEM::NumCables)
+        .def("get_cable_positions", &PyNonlinearFEM::GetCablePositions)
+        .def("compute_energy", &PyNonlinearFEM::ComputeEnergy)
+        .def("compute_elastic_energy", &PyNonlinearFEM::ComputeElasticEnergy);
 }
