@@ -62,28 +62,26 @@ def hessvec_bytes(ne, nn, per_element=False):
     return ne * (16 + 36 + 4 + (8 if per_element else 0)) + nn * 36
 
 
-def cpu_baseline(m, fixed, fdead, x, budget_s=10.0):
-    """Hessian-vector products of the same mesh at the solved state on this
-    host's cores: the compiled reference's own MatrixVectorMultiply
-    (oracle/_ref, built from the reference sources; OpenMP element loop, serial
-    scatter) when that library is present, else the FP64 restatement with the
-    same loop structure (oracle/sfem_oracle.c)."""
-    from oracle import oracle as O
-    cores = min(os.cpu_count() or 1, 16)
-    os.environ.setdefault("OMP_NUM_THREADS", str(cores))
-    d = np.random.default_rng(0).standard_normal(m.X.shape)
-    if O.ref_available():
-        kind = "reference"
-        ref = O.RefProblem(m.X, m.tets, fixed, O.NEO_HOOKEAN, 1e6, 0.45, 1070.0, -9.8, zup=True)
-        ref.set_positions(x)
-        fn = lambda: ref.matvec(d)   # noqa: E731
-        what = "FemPhysicsMatrixFree::MatrixVectorMultiply compiled from the reference sources"
-    else:
-        kind = "port"
-        p = O.OracleProblem(m.X, m.tets, fixed, O.NEO_HOOKEAN, 1e6, 0.45, 1070.0, gravity=(0, 0, -9.8),
-                            fext=fdead, nthreads=cores)
-        fn = lambda: p.matvec(x, d)   # noqa: E731
-        what = "oracle/sfem_oracle.c orc_matvec"
+def host_info():
+    """nproc, the threads the CPU legs use (OMP_NUM_THREADS where the host sets
+    it, e.g. the CPU share of a GPU box, else every core) and lscpu's model"""
+    nproc = os.cpu_count() or 1
+    threads = int(os.environ.get("OMP_NUM_THREADS", nproc))
+    info = {"nproc": nproc, "threads": threads}
+    try:
+        import subprocess
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            if k.strip() in ("Model name", "Socket(s)", "Core(s) per socket", "Thread(s) per core", "CPU(s)"):
+                info["lscpu " + k.strip()] = v.strip()
+    except Exception:   # lscpu absent: nproc alone
+        pass
+    return info
+
+
+def _rate(fn, budget_s):
+    """calls of fn per second over about budget_s seconds (after one warm call)"""
     fn()
     n, t0 = 0, time.perf_counter()
     while True:
@@ -91,11 +89,77 @@ def cpu_baseline(m, fixed, fdead, x, budget_s=10.0):
         n += 1
         dt = time.perf_counter() - t0
         if dt > budget_s or n >= 1000:
-            break
-    threads = int(os.environ.get("OMP_NUM_THREADS", cores))
-    return {"value": n * m.ne / dt, "unit": "elem-evals/s", "cores": threads, "kind": kind,
-            "sample": f"{n} Hessian-vector products of the {m.ne:,}-tet NH beam at its solved state "
-                      f"({what}, {threads} OpenMP threads, {dt:.1f} s)"}
+            return n / dt, n, dt
+
+
+def cpu_baseline(m, fixed, fdead, x, gpu_runs, budget_s=8.0):
+    """The CPU path on this host's cores, on bounded samples of the same
+    workload: Hessian-vector products at the solved state through the
+    compiled reference's own MatrixVectorMultiply (oracle/_ref, built from the
+    reference sources; OpenMP element loop, serial scatter) when that library
+    is present, and through the FP64 restatement with the same loop structure
+    (oracle/sfem_oracle.c) always; plus the restatement's Jacobi-PCG iteration
+    and residual, which extrapolate its converged Newton-CG over the GPU
+    solve's iteration counts (labelled as extrapolated)."""
+    from oracle import oracle as O
+    host = host_info()
+    os.environ.setdefault("OMP_NUM_THREADS", str(host["threads"]))
+    threads = host["threads"]
+    d = np.random.default_rng(0).standard_normal(m.X.shape)
+    p = O.OracleProblem(m.X, m.tets, fixed, O.NEO_HOOKEAN, 1e6, 0.45, 1070.0, gravity=(0, 0, -9.8),
+                        fext=fdead, nthreads=threads, incidence=True)
+    port_rate, n_port, dt_port = _rate(lambda: p.matvec(x, d), budget_s)
+    out = {"unit": "elem-evals/s", "cores": threads, "host": host,
+           "port_value": port_rate * m.ne,
+           "port_sample": f"{n_port} Hessian-vector products of the {m.ne:,}-tet NH beam at its solved state "
+                          f"(oracle/sfem_oracle.c orc_matvec, {threads} OpenMP threads, {dt_port:.1f} s)"}
+    if O.ref_available():
+        ref = O.RefProblem(m.X, m.tets, fixed, O.NEO_HOOKEAN, 1e6, 0.45, 1070.0, -9.8, zup=True)
+        ref.set_positions(x)
+        ref_rate, n_ref, dt_ref = _rate(lambda: ref.matvec(d), budget_s)
+        out.update(value=ref_rate * m.ne, kind="reference",
+                   sample=f"{n_ref} Hessian-vector products of the {m.ne:,}-tet NH beam at its solved state "
+                          f"(FemPhysicsMatrixFree::MatrixVectorMultiply compiled from the reference sources, "
+                          f"{threads} OpenMP threads, {dt_ref:.1f} s)")
+    else:
+        out.update(value=out["port_value"], kind="port", sample=out["port_sample"])
+    # the restatement's converged Jacobi-PCG: one PCG iteration (operator,
+    # block-Jacobi apply, vector updates) and one FP64 residual, timed
+    B = p.block_diag(x)
+    Binv = np.linalg.inv(B).reshape(-1, 9)
+    b = p.gradients(x, 1.0)
+    cg_rate, n_cg, dt_cg = _rate(lambda: p.cg(x, b, 10, 1e-30, Binv), budget_s / 2)
+    res_rate, n_res, _ = _rate(lambda: p.gradients(x, 1.0), budget_s / 4)
+    cg_total = sum(r["cg_iters"] for r in gpu_runs) / len(gpu_runs)
+    res_total = sum(r["force_evals"] for r in gpu_runs) / len(gpu_runs) / m.ne
+    t_iter, t_res = 1.0 / (10 * cg_rate), 1.0 / res_rate
+    out["newton_cg_extrapolated_s"] = cg_total * t_iter + res_total * t_res
+    out["newton_cg_extrapolation"] = (
+        f"EXTRAPOLATED: the GPU Jacobi-PCG solve's {cg_total:.0f} CG iterations x {1e3 * t_iter:.2f} ms "
+        f"(oracle block-Jacobi PCG iteration, {threads} threads) + {res_total:.0f} FP64 residuals x "
+        f"{1e3 * t_res:.2f} ms; the reference's own SolveNewtonCG (unpreconditioned CG, fixed 1e-3 "
+        f"forcing, no residual test) does not reach 1e-8 on this beam")
+    return out
+
+
+def cpu_full_newton(grid, load_steps, threads):
+    """The FP64 restatement's converged Newton with block-Jacobi PCG, run in
+    full on the CPU (minutes): the CPU counterpart of the GPU Jacobi-PCG leg"""
+    from oracle import oracle as O
+    m, fixed, fdead = workload(grid)
+    p = O.OracleProblem(m.X, m.tets, fixed, O.NEO_HOOKEAN, 1e6, 0.45, 1070.0, gravity=(0, 0, -9.8),
+                        fext=fdead, nthreads=threads, incidence=True)
+    x = m.X.copy()
+    tot = {"newton": 0, "cg": 0}
+    t0 = time.perf_counter()
+    for k in range(load_steps):
+        x, st = p.newton_solve(x, (k + 1) / load_steps, tol=1e-8, cg_rtol=0.1, max_newton=60, max_cg=1000000)
+        tot["newton"] += st["newton"]
+        tot["cg"] += st["cg"]
+        print(f"cpu load step {k + 1}/{load_steps}: newton {st['newton']} cg {st['cg']} "
+              f"rel residual {st['res'] / st['ext']:.2e} ({time.perf_counter() - t0:.1f} s)", file=sys.stderr, flush=True)
+    return {"grid": "x".join(map(str, grid)), "tets": m.ne, "load_steps": load_steps, "threads": threads,
+            "newton_cg_wall_s": time.perf_counter() - t0, **tot, "host": host_info()}
 
 
 MATERIALS = {"linear": capi.LINEAR, "stvk": capi.STVK, "corotational": capi.COROTATIONAL,
@@ -150,7 +214,14 @@ def main():
                     help="with --hessvec-only: the material whose Hessian-vector kernel is timed")
     ap.add_argument("--calib-mib", type=int, default=0,
                     help="with --hessvec-only: also run one device-to-device copy of this many MiB (counter calibration)")
+    ap.add_argument("--cpu-full-newton", action="store_true",
+                    help="only run the CPU FP64 restatement's converged Jacobi-PCG Newton on --grid in full "
+                         "(no GPU), and print its wall time as one JSON line")
     args = ap.parse_args()
+    if args.cpu_full_newton:
+        print(json.dumps(cpu_full_newton(tuple(int(v) for v in args.grid.split("x")), args.load_steps,
+                                         host_info()["threads"])), flush=True)
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU sharding is design-only.",
               file=sys.stderr)
@@ -288,7 +359,7 @@ def main():
     }
     line.update(extra)
     if not args.no_cpu_baseline:
-        line["cpu_baseline"] = cpu_baseline(m, fixed, fdead, x)
+        line["cpu_baseline"] = cpu_baseline(m, fixed, fdead, x, runs)
     print(json.dumps(line), flush=True)
 
 
