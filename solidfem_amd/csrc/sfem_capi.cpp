@@ -231,6 +231,11 @@ void build_jacobi(sfem_ctx* c)
 }
 
 // PCG on K x = b (b already in d_b32); x in d_x32.
+// Two launches per iteration: the element kernel, which from the second
+// iteration on also makes the previous iteration's convergence test and
+// direction update (p = z + beta p, into the other buffer of d_p32 / d_p32b),
+// then the update of x, r and z.  After the last launch, one direction kernel
+// makes the test of the last update.
 CgResult pcg(sfem_ctx* c, const float* minv, int pc, int maxit, int test_kind, double tol2, int guard, int check_every,
              int64_t* hv_evals, CgHook hook, void* user)
 {
@@ -239,8 +244,22 @@ CgResult pcg(sfem_ctx* c, const float* minv, int pc, int maxit, int test_kind, d
     double* parts = c->d_partials.as<double>();          // CG update partials (2 x CG_GRID)
     double* pq_elem = parts + 2 * CG_GRID;               // p.q per element block (nblk)
     int* done = c->d_done.as<int>();
+    float* pbuf[2] = {c->d_p32.as<float>(), c->d_p32b.as<float>()};
+    int cur = 0;
     HIPCK(launch_cg_init(pc, n3, c->d_b32.as<float>(), c->d_x32.as<float>(), c->d_r32.as<float>(), c->d_z32.as<float>(),
-                         c->d_p32.as<float>(), minv, c->d_fixed.as<uint8_t>(), scal, parts, done, c->stream));
+                         pbuf[0], minv, c->d_fixed.as<uint8_t>(), scal, parts, done, c->stream, pbuf[1]));
+    // guard 2: also stop at r.z <= 0 (a learned preconditioner that is not
+    // positive definite at this state; its solve is redone with block Jacobi)
+    const int rz_guard = guard >= 2 && pc;
+    PFuse pf{};
+    pf.z = pc ? c->d_z32.as<float>() : c->d_r32.as<float>();
+    pf.rz_part = parts;
+    pf.n_rz = cg_grid(c->nn);
+    pf.scal = scal;
+    pf.test_kind = test_kind;
+    pf.tol2 = tol2;
+    pf.rz_guard = rz_guard;
+    pf.stall_window = 0;
     if (check_every < 1) check_every = 1;
     int launched = 0;
     int dflag = 0;
@@ -251,15 +270,25 @@ CgResult pcg(sfem_ctx* c, const float* minv, int pc, int maxit, int test_kind, d
             const bool timed = c->tm_every > 0 && c->tm_counter++ % c->tm_every == 0 && c->tm_pending < sfem_ctx::TM_POOL;
             hipEvent_t ev0 = timed ? c->tm_ev[2 * c->tm_pending] : nullptr;
             hipEvent_t ev1 = timed ? c->tm_ev[2 * c->tm_pending + 1] : nullptr;
-            HIPCK(launch_hessvec_cg(c->material, c->blk, c->d_u32.as<float>(), c->d_p32.as<float>(), c->d_q32.as<float>(),
-                                    c->d_partial.as<float>(), done, pq_elem, c->stream, ev0, ev1));
+            if (launched + i == 0) {
+                HIPCK(launch_hessvec_cg(c->material, c->blk, c->d_u32.as<float>(), pbuf[cur], c->d_q32.as<float>(),
+                                        c->d_partial.as<float>(), done, pq_elem, c->stream, ev0, ev1));
+            } else {
+                pf.p_old = pbuf[cur];
+                pf.p_new = pbuf[cur ^ 1];
+                cur ^= 1;
+                HIPCK(launch_hessvec_cg_fused(c->material, c->blk, c->d_u32.as<float>(), pf, c->d_q32.as<float>(),
+                                              c->d_partial.as<float>(), done, pq_elem, c->stream, ev0, ev1));
+            }
             if (timed) c->tm_iter[c->tm_pending++] = launched + i;
-            HIPCK(launch_cg_iter_tail(pc, n3, c->d_x32.as<float>(), c->d_r32.as<float>(), c->d_z32.as<float>(),
-                                      c->d_p32.as<float>(), c->d_q32.as<float>(), minv, c->d_node_info.as<int2>(),
-                                      c->d_partial.as<float>(), scal, pq_elem, c->nblk, parts, done,
-                                      test_kind, tol2, guard, c->stream));
+            HIPCK(launch_cg_update(pc, n3, c->d_x32.as<float>(), c->d_r32.as<float>(), c->d_z32.as<float>(), pbuf[cur],
+                                   c->d_q32.as<float>(), minv, c->d_node_info.as<int2>(), c->d_partial.as<float>(), scal,
+                                   pq_elem, c->nblk, parts, done, guard, c->stream));
         }
         launched += k;
+        if (launched >= maxit)
+            HIPCK(launch_cg_test(pc, n3, pf.z, c->d_r32.as<float>(), pbuf[cur], scal, parts, done, test_kind, tol2, rz_guard, 0,
+                                 c->stream));
         HIPCK(hipMemcpyAsync(&dflag, done, sizeof(int), hipMemcpyDeviceToHost, c->stream));
         HIPCK(hipStreamSynchronize(c->stream));
         if (c->tm_pending) {
@@ -521,7 +550,8 @@ int sfem_create(const sfem_mesh* m, const sfem_material* mat, sfem_ctx** out)
         zero<double>(c->d_r64, n3, s);
         zero<double>(c->d_ext64, n3, s);
         zero<double>(c->d_f64tmp, n3, s);
-        for (DevBuf* b : {&c->d_b32, &c->d_x32, &c->d_r32, &c->d_z32, &c->d_p32, &c->d_q32, &c->d_d32}) zero<float>(*b, n3, s);
+        for (DevBuf* b : {&c->d_b32, &c->d_x32, &c->d_r32, &c->d_z32, &c->d_p32, &c->d_p32b, &c->d_q32, &c->d_d32})
+            zero<float>(*b, n3, s);
         zero<float>(c->d_minv, 9 * (size_t)nn, s);
         zero<float>(c->d_bd32, 9 * (size_t)nn, s);
         zero<double>(c->d_scal, S_COUNT, s);

@@ -341,15 +341,18 @@ struct Recorder {
     const float* e;
 };
 
+// runs between CG launches (recorder_hook): its partials have a buffer of
+// their own, since the CG's own partials carry over to the next launch
 void record_pair(sfem_ctx* c, const float* e, const float* x, const float* r)
 {
     const int64_t n3 = 3 * c->nn;
     auto buf = std::make_unique<DevBuf>();
     buf->alloc(n3 * sizeof(float));
     HIPCK(hipMemcpyAsync(buf->p, r, n3 * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
-    HIPCK(gnn_err_dot(n3, e, x, r, c->d_partials.as<double>(), 256, c->stream));
     double part[256];
-    HIPCK(hipMemcpyAsync(part, c->d_partials.p, sizeof part, hipMemcpyDeviceToHost, c->stream));
+    if (!c->pairs.d_part.p) c->pairs.d_part.alloc(sizeof part);
+    HIPCK(gnn_err_dot(n3, e, x, r, c->pairs.d_part.as<double>(), 256, c->stream));
+    HIPCK(hipMemcpyAsync(part, c->pairs.d_part.p, sizeof part, hipMemcpyDeviceToHost, c->stream));
     HIPCK(hipStreamSynchronize(c->stream));
     double er = 0;
     for (double v : part) er += v;

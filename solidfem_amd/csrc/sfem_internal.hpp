@@ -73,6 +73,7 @@ struct SfemPairs {
     int64_t n = 0;                                  // number of pairs
     std::vector<std::unique_ptr<sfem::DevBuf>> r;   // FP32 residual vectors (nn*3)
     std::vector<double> er;                         // r . e per pair (FP64)
+    sfem::DevBuf d_part;                            // the recorder's partial sums
 };
 
 // a coarse level of the multigrid preconditioner: its own context (mesh,
@@ -141,6 +142,7 @@ struct sfem_ctx {
     DevBuf d_u64, d_ut64, d_u32;
     DevBuf d_r64, d_ext64, d_f64tmp;
     DevBuf d_b32, d_x32, d_r32, d_z32, d_p32, d_q32, d_d32, d_minv, d_minv_learned, d_bd32;
+    DevBuf d_p32b;           // the second CG direction buffer (the direction update writes the other one)
     DevBuf d_tris, d_pptr, d_pidx, d_ftri;
 
     // reductions

@@ -92,18 +92,53 @@ size_t lds_layout_bytes(int BE, int nmax)
            sizeof(uint16_t) * ((size_t)nmax + 1) + 16;
 }
 
+// the CG scalars and status of one convergence test (one thread of the grid):
+// the stagnation reference is read and written by that thread only
+__device__ __forceinline__ void cg_test_record(double* scal, int* done, double rz1, double rr, double beta, bool conv,
+                                               bool bad, bool indef, int stall_window)
+{
+    const double it = scal[S_ITER] + 1.0;
+    scal[S_RZ] = rz1;
+    scal[S_RR] = rr;
+    scal[S_BETA] = beta;
+    scal[S_ITER] = it;
+    if (conv) *done = CG_CONVERGED;
+    else if (bad) *done = CG_NOT_FINITE;
+    else if (indef) *done = CG_BREAKDOWN;
+    else if (stall_window > 0) {
+        if (rr <= 0.5 * scal[S_RR_BEST]) {
+            scal[S_RR_BEST] = rr;
+            scal[S_IT_BEST] = it;
+        } else if (it - scal[S_IT_BEST] >= (double)stall_window) {
+            *done = CG_STAGNATED;
+        }
+    }
+}
+
+// workgroups are dispatched to the 8 XCDs round robin (workgroup w to XCD
+// w % 8); each XCD gets a contiguous range of the space-filling-curve blocks,
+// so that the node rows that neighbouring blocks share are fetched into its
+// own L2 once
+__device__ __forceinline__ int xcd_block(int w, int nblk)
+{
+    const int per = nblk >> 3, rem = nblk & 7, xcd = w & 7, loc = w >> 3;
+    return xcd < rem ? xcd * (per + 1) + loc : rem * (per + 1) + (xcd - rem) * per + loc;
+}
+
 // DYN: implicit dynamics (mdiag non-null); a separate instance so that static
-// solves carry none of its code
-template <int KIND, int MAT, int BE, bool DYN>
+// solves carry none of its code.  PF (KIND 0): the CG direction update of the
+// previous iteration is made here (PFuse); d is not read.
+template <int KIND, int MAT, int BE, bool DYN, bool PF>
 __global__ __launch_bounds__(BE) void k_elem_block(BlockArgs a, const typename KindTraits<KIND>::T* __restrict__ u,
                                                     const typename KindTraits<KIND>::T* __restrict__ d,
                                                     typename KindTraits<KIND>::T* __restrict__ out,
                                                     typename KindTraits<KIND>::T* __restrict__ partial,
                                                     const double* __restrict__ ext, int mask_fixed,
-                                                    const int* __restrict__ done, double* __restrict__ dot_part,
+                                                    int* __restrict__ done, double* __restrict__ dot_part,
                                                     const typename KindTraits<KIND>::T* __restrict__ mdiag,
-                                                    const int2* __restrict__ node_q)
+                                                    const int2* __restrict__ node_q, PFuse pf)
 {
+    static_assert(!PF || KIND == 0, "direction update only in the Hessian-vector kernel");
     typedef typename KindTraits<KIND>::T T;
     typedef typename Vec4<T>::V V4;
     constexpr int W = KindTraits<KIND>::W;
@@ -124,8 +159,19 @@ __global__ __launch_bounds__(BE) void k_elem_block(BlockArgs a, const typename K
     int* s_slot = reinterpret_cast<int*>(s_flag + a.nmax);                        // nmax
     uint16_t* s_ptr = reinterpret_cast<uint16_t*>(s_slot + a.nmax);               // nmax+1
 
-    const int b = blockIdx.x;
+    const int b = xcd_block(blockIdx.x, a.nblk);
     const int tid = threadIdx.x;
+    // PF: the update's partials, issued before every other load
+    constexpr int PU = 4;
+    double rzv[2][PU];
+    double rz0 = 0.0, rr0 = 0.0, rz_prev = 0.0;
+    if (PF) {
+        partials_issue<2, BE, PU>(pf.rz_part, pf.n_rz, pf.n_rz, rzv);
+        // written by the previous kernels, read before the reduction needs them
+        rz0 = pf.scal[S_RZ0];
+        rr0 = pf.scal[S_RR0];
+        rz_prev = pf.scal[S_RZ_PREV];
+    }
     const int n0 = vload(a.bnode_ptr + b);
     const int nb = vload(a.bnode_ptr + b + 1) - n0;
 
@@ -176,34 +222,81 @@ __global__ __launch_bounds__(BE) void k_elem_block(BlockArgs a, const typename K
     const uint16_t* lp = a.lptr + (n0 + b);
     const uint16_t lp0 = tid <= nb ? lp[tid] : uint16_t(0);
     if (stop) return;
-    // stage block nodes (rows, flags, partial slots); all of a row's loads are
-    // issued before its LDS stores
-    for (int j = tid; j < nb; j += BE) {
-        const uint32_t fl = a.bnodes[n0 + j];
-        const uint32_t node = fl & NODE_MASK;
-        const T* up = u + 3 * (size_t)node;
-        V4 uv, dv;
-        uv.x = up[0]; uv.y = up[1]; uv.z = up[2]; uv.w = T(0);
-        if (HAS_D) {
-            const T* dp = d + 3 * (size_t)node;
-            dv.x = dp[0]; dv.y = dp[1]; dv.z = dp[2]; dv.w = T(0);
+    // stage block nodes (rows, flags, partial slots): a row's loads, then its
+    // LDS stores; the first round of loads is issued before the PF test
+    struct Row {
+        uint32_t fl, node;
+        int slot, first;
+        V4 uv, dv, xv;
+        T zr[3], po[3], md;
+    };
+    // every load of a row is issued before any of its values is tested
+    auto load_row = [&](int j, Row& w) {
+        w.fl = a.bnodes[n0 + j];
+        w.node = w.fl & NODE_MASK;
+        const T* up = u + 3 * (size_t)w.node;
+        w.uv.x = up[0]; w.uv.y = up[1]; w.uv.z = up[2]; w.uv.w = T(0);
+        if (PF) {
+            const float* zp = pf.z + 3 * (size_t)w.node;
+            const float* pp = pf.p_old + 3 * (size_t)w.node;
+#pragma unroll
+            for (int c = 0; c < 3; c++) { w.zr[c] = zp[c]; w.po[c] = pp[c]; }
+        } else if (HAS_D) {
+            const T* dp = d + 3 * (size_t)w.node;
+            w.dv.x = dp[0]; w.dv.y = dp[1]; w.dv.z = dp[2]; w.dv.w = T(0);
         }
-        const int slot = (fl & FLAG_SHARED) ? a.bslot[n0 + j] : -1;
-        // dynamics: the row of the node's first block carries m / h^2 (the
-        // node's partial rows add up to one mass term)
-        T md = T(0);
-        if (DYN && (!(fl & FLAG_SHARED) || node_q[node].x == slot)) md = mdiag[node];
-        V4 xv;
         if (HAS_X) {
             const float* xp = a.rel32 + 3 * (size_t)(n0 + j);
-            xv.x = (T)xp[0]; xv.y = (T)xp[1]; xv.z = (T)xp[2]; xv.w = T(0);
+            w.xv.x = (T)xp[0]; w.xv.y = (T)xp[1]; w.xv.z = (T)xp[2]; w.xv.w = T(0);
         }
-        su4[j] = uv;
-        if (HAS_D) sd4[j] = dv;
-        if (HAS_X) sx4[j] = xv;
-        if (DYN) s_md[j] = md;
-        s_flag[j] = fl;
-        s_slot[j] = slot;
+        // unconditional loads (bslot is -1 on unshared rows): a load under a
+        // branch would be waited for where the branch joins
+        w.slot = a.bslot[n0 + j];
+        w.first = (DYN || PF) ? node_q[w.node].x : -1;
+        w.md = DYN ? mdiag[w.node] : T(0);
+    };
+    auto store_row = [&](int j, Row& w, float beta) {
+        // the row's owner: its only block, or the first block of a shared node
+        // (dynamics: that row carries m / h^2, so the node's partial rows add
+        // up to one mass term; PF: that block stores p)
+        const bool owner = w.slot < 0 || w.first == w.slot;
+        if (PF) {
+            w.dv.x = w.zr[0] + beta * w.po[0]; w.dv.y = w.zr[1] + beta * w.po[1]; w.dv.z = w.zr[2] + beta * w.po[2];
+            w.dv.w = T(0);
+            if (owner) {
+                float* pn = pf.p_new + 3 * (size_t)w.node;
+                pn[0] = w.dv.x; pn[1] = w.dv.y; pn[2] = w.dv.z;
+            }
+        }
+        su4[j] = w.uv;
+        if (HAS_D) sd4[j] = w.dv;
+        if (HAS_X) sx4[j] = w.xv;
+        if (DYN) s_md[j] = owner ? w.md : T(0);
+        s_flag[j] = w.fl;
+        s_slot[j] = w.slot;
+    };
+    Row r0;
+    if (tid < nb) load_row(tid, r0);
+    float beta = 0.f;
+    if (PF) {
+        // beta and the test of the previous iteration's residual, as k_cg_p
+        // makes them (every workgroup reaches the same decision)
+        double t[2];
+        partials_finish<2, BE, PU>(pf.rz_part, pf.n_rz, pf.n_rz, rzv, t);
+        const double rz1 = t[0], rr = t[1];
+        const bool conv = pf.test_kind == 0 ? (fabs(rz1) <= pf.tol2 * fabs(rz0)) : (rr <= pf.tol2 * rr0);
+        const bool bad = !(rr == rr) || !(rz1 == rz1);
+        const bool indef = pf.rz_guard && !conv && !bad && !(rz1 > 0.0);
+        const double beta_d = rz1 / rz_prev;
+        if (blockIdx.x == 0 && tid == 0) cg_test_record(pf.scal, done, rz1, rr, beta_d, conv, bad, indef, pf.stall_window);
+        if (conv || bad || indef) return;
+        beta = (float)beta_d;
+    }
+    if (tid < nb) store_row(tid, r0, beta);
+    for (int j = tid + BE; j < nb; j += BE) {
+        Row w;
+        load_row(j, w);
+        store_row(j, w, beta);
     }
     if (tid <= nb) s_ptr[tid] = lp0;
     for (int j = tid + BE; j <= nb; j += BE) s_ptr[j] = lp[j];
@@ -440,12 +533,12 @@ __global__ __launch_bounds__(256) void k_shared_gather(SharedArgs s, const T* __
 // itself in a fixed order: no atomics, no extra launch.  Scalars that a kernel
 // reads are never written by the same kernel (S_RZ_PREV holds the previous r.z).
 
-// x = 0, r = b, z = M r, p = z; partials of r.z, r.r
+// x = 0, r = b, z = M r, p = z (and p2 = z); partials of r.z, r.r
 template <int PC>
 __global__ __launch_bounds__(CG_BLOCK) void k_cg_init(int64_t n3, const float* __restrict__ b, float* __restrict__ x,
                                                  float* __restrict__ r, float* __restrict__ z, float* __restrict__ p,
-                                                 const float* __restrict__ minv, const uint8_t* __restrict__ fixed,
-                                                 double* __restrict__ part)
+                                                 float* __restrict__ p2, const float* __restrict__ minv,
+                                                 const uint8_t* __restrict__ fixed, double* __restrict__ part)
 {
     const int64_t nn = n3 / 3;
     double v[2] = {0.0, 0.0};
@@ -463,6 +556,9 @@ __global__ __launch_bounds__(CG_BLOCK) void k_cg_init(int64_t n3, const float* _
         }
 #pragma unroll
         for (int c = 0; c < 3; c++) { z[3 * i + c] = zv[c]; p[3 * i + c] = zv[c]; }
+        if (p2)
+#pragma unroll
+            for (int c = 0; c < 3; c++) p2[3 * i + c] = zv[c];
         v[0] += (double)rv[0] * zv[0] + (double)rv[1] * zv[1] + (double)rv[2] * zv[2];
         v[1] += (double)rv[0] * rv[0] + (double)rv[1] * rv[1] + (double)rv[2] * rv[2];
     }
@@ -548,10 +644,11 @@ __global__ __launch_bounds__(CG_BLOCK) void k_cg_update(int64_t n3, float* __res
     double pq_v[1][16], pq[1];
     if (i < nn) load_rows(i);
     partials_issue<1, CG_BLOCK, 16>(pq_part, n_pq, n_pq, pq_v);
+    // r.z of the previous kernel: loaded with the first round, not after the reduction
+    const double rz = scal[S_RZ];
     if (i < nn) load_partial_rows();
     partials_finish<1, CG_BLOCK, 16>(pq_part, n_pq, n_pq, pq_v, pq);
     if (stop) return;
-    const double rz = scal[S_RZ];
     if (guard && !(pq[0] > 0.0)) {          // breakdown: not positive definite
         if (blockIdx.x == 0 && threadIdx.x == 0) *done = 3;
         return;
@@ -613,33 +710,17 @@ __global__ __launch_bounds__(CG_P_BLOCK) void k_cg_p(int64_t n3, const float* __
 #pragma unroll
         for (int c = 0; c < 3; c++) { zv[c] = zz[3 * i + c]; pv[c] = p[3 * i + c]; }
     }
+    // the scalars of the previous kernels: loaded with the first round
+    const double rz0 = scal[S_RZ0], rr0 = scal[S_RR0], rz_prev = scal[S_RZ_PREV];
     double t[2];
     sum_partials<2, CG_P_BLOCK>(part, n_part, n_part, t);
     if (stop) return;
     const double rz1 = t[0], rr = t[1];
-    const bool conv = test_kind == 0 ? (fabs(rz1) <= tol2 * fabs(scal[S_RZ0])) : (rr <= tol2 * scal[S_RR0]);
+    const bool conv = test_kind == 0 ? (fabs(rz1) <= tol2 * fabs(rz0)) : (rr <= tol2 * rr0);
     const bool bad = !(rr == rr) || !(rz1 == rz1);
     const bool indef = rz_guard && !conv && !bad && !(rz1 > 0.0);
-    const double beta = rz1 / scal[S_RZ_PREV];
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        const double it = scal[S_ITER] + 1.0;
-        scal[S_RZ] = rz1;
-        scal[S_RR] = rr;
-        scal[S_BETA] = beta;
-        scal[S_ITER] = it;
-        if (conv) *done = CG_CONVERGED;
-        else if (bad) *done = CG_NOT_FINITE;
-        else if (indef) *done = CG_BREAKDOWN;
-        else if (stall_window > 0) {
-            // only this thread reads and writes the stagnation reference
-            if (rr <= 0.5 * scal[S_RR_BEST]) {
-                scal[S_RR_BEST] = rr;
-                scal[S_IT_BEST] = it;
-            } else if (it - scal[S_IT_BEST] >= (double)stall_window) {
-                *done = CG_STAGNATED;
-            }
-        }
-    }
+    const double beta = rz1 / rz_prev;
+    if (blockIdx.x == 0 && threadIdx.x == 0) cg_test_record(scal, done, rz1, rr, beta, conv, bad, indef, stall_window);
     if (conv || bad || indef) return;
     const float fb = (float)beta;
     for (; i < nn; i += stride) {
@@ -1024,25 +1105,26 @@ static int grid_for(int64_t n, int bs, int cap)
     return (int)(g > cap ? cap : g);
 }
 
-template <int KIND, int BE>
+template <int KIND, int BE, bool PF = false>
 static hipError_t launch_elem_mat(int mat, const BlockArgs& a, const void* u, const void* d, void* out, void* partial,
-                                  const double* ext, int mask_fixed, const int* done, double* dot, size_t lds, hipStream_t s,
-                                  hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr)
+                                  const double* ext, int mask_fixed, const int* done_in, double* dot, size_t lds, hipStream_t s,
+                                  hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr, const PFuse& pf = PFuse{})
 {
     typedef typename KindTraits<KIND>::T T;
     dim3 grid(a.nblk), block(BE);
     const T* md = (const T*)(sizeof(T) == 8 ? (const void*)a.mdiag64 : (const void*)a.mdiag32);
     const int2* nq = a.node_q;
-    if (!done) done = a.zero_flag;
+    // only PF launches write the flag (the test of the previous iteration)
+    int* done = const_cast<int*>(done_in ? done_in : a.zero_flag);
     // with events: hipExtLaunchKernelGGL records them at the dispatch's own start
     // and end, so their interval is the kernel's execution time
 #define SFEM_ELEM_DYN(M, D)                                                                                          \
     if (ev0)                                                                                                         \
-        hipExtLaunchKernelGGL((k_elem_block<KIND, M, BE, D>), grid, block, (uint32_t)lds, s, ev0, ev1, 0u, a,       \
-                              (const T*)u, (const T*)d, (T*)out, (T*)partial, ext, mask_fixed, done, dot, md, nq);   \
+        hipExtLaunchKernelGGL((k_elem_block<KIND, M, BE, D, PF>), grid, block, (uint32_t)lds, s, ev0, ev1, 0u, a,   \
+                              (const T*)u, (const T*)d, (T*)out, (T*)partial, ext, mask_fixed, done, dot, md, nq, pf); \
     else                                                                                                             \
-        hipLaunchKernelGGL((k_elem_block<KIND, M, BE, D>), grid, block, lds, s, a, (const T*)u, (const T*)d, (T*)out, \
-                           (T*)partial, ext, mask_fixed, done, dot, md, nq)
+        hipLaunchKernelGGL((k_elem_block<KIND, M, BE, D, PF>), grid, block, lds, s, a, (const T*)u, (const T*)d,     \
+                           (T*)out, (T*)partial, ext, mask_fixed, done, dot, md, nq, pf)
 #define SFEM_ELEM_CASE(M)                                                                                            \
     if (md) {                                                                                                        \
         SFEM_ELEM_DYN(M, true);                                                                                      \
@@ -1140,13 +1222,19 @@ hipError_t set_elem_lds_limits(const BlockArgs& a)
         int kind;
     };
     const Fn fns[] = {
-#define SFEM_FN(K, M) {(const void*)k_elem_block<K, M, BLOCK_E, false>, K}, {(const void*)k_elem_block<K, M, BLOCK_E, true>, K}
+#define SFEM_FN(K, M) {(const void*)k_elem_block<K, M, BLOCK_E, false, false>, K}, \
+                      {(const void*)k_elem_block<K, M, BLOCK_E, true, false>, K}
+#define SFEM_FN_PF(M) {(const void*)k_elem_block<0, M, BLOCK_E, false, true>, 0}, \
+                      {(const void*)k_elem_block<0, M, BLOCK_E, true, true>, 0}
         SFEM_FN(0, LINEAR), SFEM_FN(0, STVK), SFEM_FN(0, COROTATIONAL), SFEM_FN(0, NEO_HOOKEAN),
         SFEM_FN(0, NEO_HOOKEAN_COARSE), SFEM_FN(0, COROTATIONAL_POLAR),
+        SFEM_FN_PF(LINEAR), SFEM_FN_PF(STVK), SFEM_FN_PF(COROTATIONAL), SFEM_FN_PF(NEO_HOOKEAN),
+        SFEM_FN_PF(NEO_HOOKEAN_COARSE), SFEM_FN_PF(COROTATIONAL_POLAR),
         SFEM_FN(1, LINEAR), SFEM_FN(1, STVK), SFEM_FN(1, COROTATIONAL), SFEM_FN(1, NEO_HOOKEAN),
         SFEM_FN(1, COROTATIONAL_POLAR),
         SFEM_FN(2, LINEAR), SFEM_FN(2, STVK), SFEM_FN(2, COROTATIONAL), SFEM_FN(2, NEO_HOOKEAN),
         SFEM_FN(2, NEO_HOOKEAN_COARSE), SFEM_FN(2, COROTATIONAL_POLAR),
+#undef SFEM_FN_PF
 #undef SFEM_FN
     };
     const size_t need[] = {lds_bytes<0>(a), lds_bytes<1>(a), lds_bytes<2>(a)};
@@ -1172,11 +1260,11 @@ hipError_t set_elem_lds_limits(const BlockArgs& a)
 int cg_grid(int64_t nn) { return grid_for(nn, CG_BLOCK, CG_GRID); }
 
 hipError_t launch_cg_init(int pc, int64_t n3, const float* b, float* x, float* r, float* z, float* p, const float* minv,
-                          const uint8_t* fixed, double* scal, double* part, int* done, hipStream_t st)
+                          const uint8_t* fixed, double* scal, double* part, int* done, hipStream_t st, float* p2)
 {
     const int g = cg_grid(n3 / 3);
-    if (pc) hipLaunchKernelGGL(k_cg_init<1>, dim3(g), dim3(CG_BLOCK), 0, st, n3, b, x, r, z, p, minv, fixed, part);
-    else hipLaunchKernelGGL(k_cg_init<0>, dim3(g), dim3(CG_BLOCK), 0, st, n3, b, x, r, z, p, minv, fixed, part);
+    if (pc) hipLaunchKernelGGL(k_cg_init<1>, dim3(g), dim3(CG_BLOCK), 0, st, n3, b, x, r, z, p, p2, minv, fixed, part);
+    else hipLaunchKernelGGL(k_cg_init<0>, dim3(g), dim3(CG_BLOCK), 0, st, n3, b, x, r, z, p, p2, minv, fixed, part);
     hipLaunchKernelGGL(k_cg_init_finish, dim3(1), dim3(256), 0, st, part, g, scal, done);
     return hipGetLastError();
 }
@@ -1188,24 +1276,31 @@ hipError_t launch_hessvec_cg(int mat, const BlockArgs& a, const float* u32, cons
                                        ev0, ev1);
 }
 
-hipError_t launch_cg_iter_tail(int pc, int64_t n3, float* x, float* r, float* z, float* p, const float* q, const float* minv,
-                               const int2* node_q, const float* partial,
-                               double* scal, const double* pq_part, int n_pq, double* part, int* done, int test_kind,
-                               double tol2, int guard, hipStream_t st)
+hipError_t launch_hessvec_cg_fused(int mat, const BlockArgs& a, const float* u32, const PFuse& pf, float* out,
+                                   float* partial, int* done, double* dot_elem, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1)
+{
+    return launch_elem_mat<0, BLOCK_E, true>(mat, a, u32, nullptr, out, partial, nullptr, 1, done, dot_elem, lds_bytes<0>(a),
+                                             st, ev0, ev1, pf);
+}
+
+hipError_t launch_cg_test(int pc, int64_t n3, const float* z, const float* r, float* p, double* scal, const double* part,
+                          int* done, int test_kind, double tol2, int rz_guard, int stall_window, hipStream_t st)
+{
+    const int gp = grid_for(n3 / 3, CG_P_BLOCK, CG_GRID);
+    if (pc) hipLaunchKernelGGL(k_cg_p<1>, dim3(gp), dim3(CG_P_BLOCK), 0, st, n3, z, r, p, scal, part, cg_grid(n3 / 3), done,
+                               test_kind, tol2, rz_guard, stall_window);
+    else hipLaunchKernelGGL(k_cg_p<0>, dim3(gp), dim3(CG_P_BLOCK), 0, st, n3, z, r, p, scal, part, cg_grid(n3 / 3), done,
+                            test_kind, tol2, 0, stall_window);
+    return hipGetLastError();
+}
+
+hipError_t launch_cg_update(int pc, int64_t n3, float* x, float* r, float* z, const float* p, const float* q, const float* minv,
+                            const int2* node_q, const float* partial, double* scal, const double* pq_part, int n_pq,
+                            double* part, int* done, int guard, hipStream_t st)
 {
     const int g = cg_grid(n3 / 3);
     if (pc) hipLaunchKernelGGL(k_cg_update<1>, dim3(g), dim3(CG_BLOCK), 0, st, n3, x, r, z, p, q, minv, node_q, partial, scal, pq_part, n_pq, part, done, guard);
     else hipLaunchKernelGGL(k_cg_update<0>, dim3(g), dim3(CG_BLOCK), 0, st, n3, x, r, z, p, q, minv, node_q, partial, scal, pq_part, n_pq, part, done, guard);
-    // k_cg_p re-reduces the update's g partials in every one of its workgroups:
-    // fewer, larger workgroups read them fewer times
-    const int gp = grid_for(n3 / 3, CG_P_BLOCK, CG_GRID);
-    // guard 2: also stop at r.z <= 0 (a learned preconditioner that is not
-    // positive definite at this state; its solve is redone with block Jacobi)
-    const int rz_guard = guard >= 2 && pc;
-    if (pc) hipLaunchKernelGGL(k_cg_p<1>, dim3(gp), dim3(CG_P_BLOCK), 0, st, n3, z, r, p, scal, part, g, done, test_kind, tol2,
-                               rz_guard, 0);
-    else hipLaunchKernelGGL(k_cg_p<0>, dim3(gp), dim3(CG_P_BLOCK), 0, st, n3, z, r, p, scal, part, g, done, test_kind, tol2,
-                            0, 0);
     return hipGetLastError();
 }
 

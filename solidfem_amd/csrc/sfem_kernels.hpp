@@ -6,7 +6,10 @@
 namespace sfem {
 
 // elements per workgroup of the element-block kernels
-constexpr int BLOCK_E = 256;
+#ifndef SFEM_BLOCK_E
+#define SFEM_BLOCK_E 256
+#endif
+constexpr int BLOCK_E = SFEM_BLOCK_E;
 // cap on workgroups of the reduction kernels (their partials buffer size)
 constexpr int RED_GRID = 2048;   // reduction partials buffer holds 3 * RED_GRID doubles
 // workgroups of the CG vector kernels; each of them re-reduces the previous
@@ -109,20 +112,43 @@ hipError_t launch_blockdiag(int mat, const BlockArgs& a, const SharedArgs& s, co
                             hipStream_t st);
 size_t elem_lds_max(const BlockArgs& a);
 hipError_t set_elem_lds_limits(const BlockArgs& a);
+// p2 (may be null): a second direction buffer that also gets p = z
 hipError_t launch_cg_init(int pc, int64_t n3, const float* b, float* x, float* r, float* z, float* p, const float* minv,
-                          const uint8_t* fixed, double* scal, double* part, int* done, hipStream_t st);
+                          const uint8_t* fixed, double* scal, double* part, int* done, hipStream_t st, float* p2 = nullptr);
 // CG operator apply: element blocks only (single-block rows -> out, shared
 // rows -> partial slots) plus one d.(K d) partial per element block
 // with events ev0/ev1: they record the dispatch's start and end (timing)
 hipError_t launch_hessvec_cg(int mat, const BlockArgs& a, const float* u32, const float* d32, float* out, float* partial,
                              const int* done, double* dot_elem, hipStream_t st, hipEvent_t ev0 = nullptr,
                              hipEvent_t ev1 = nullptr);
-// alpha, x, r, z (summing shared nodes' partial slots), then beta and p;
-// guard 1: stop at p.Kp <= 0, guard 2: also at r.z <= 0 (preconditioned)
-hipError_t launch_cg_iter_tail(int pc, int64_t n3, float* x, float* r, float* z, float* p, const float* q, const float* minv,
-                               const int2* node_q, const float* partial,
-                               double* scal, const double* pq_part, int n_pq, double* part, int* done, int test_kind,
-                               double tol2, int guard, hipStream_t st);
+// The CG direction update folded into the next Hessian-vector apply: every
+// block reduces the update's r.z, r.r partials (the fixed order of the
+// separate direction kernel), makes the convergence test, forms
+// p = z + beta p_old for its staged rows and applies K to that p; each row's
+// owner (its only block, or the first block of a shared node) stores p into
+// p_new, the other buffer of a pair, while other blocks still read p_old.
+struct PFuse {
+    const float* z;          // z, or r without a preconditioner
+    const float* p_old;
+    float* p_new;
+    const double* rz_part;   // 2 x n_rz partials (r.z, then r.r) of the update
+    int n_rz;
+    double* scal;
+    int test_kind;           // as launch_cg_p_only
+    double tol2;
+    int rz_guard, stall_window;
+};
+hipError_t launch_hessvec_cg_fused(int mat, const BlockArgs& a, const float* u32, const PFuse& pf, float* out,
+                                   float* partial, int* done, double* dot_elem, hipStream_t st, hipEvent_t ev0 = nullptr,
+                                   hipEvent_t ev1 = nullptr);
+// alpha, x, r and z (pc: z = M r), r.z and r.r partials (2 x cg_grid); guard
+// 1: stop at p.Kp <= 0
+hipError_t launch_cg_update(int pc, int64_t n3, float* x, float* r, float* z, const float* p, const float* q, const float* minv,
+                            const int2* node_q, const float* partial, double* scal, const double* pq_part, int n_pq,
+                            double* part, int* done, int guard, hipStream_t st);
+// beta, the convergence test and p from r.z, r.r partials (2 x cg_grid)
+hipError_t launch_cg_test(int pc, int64_t n3, const float* z, const float* r, float* p, double* scal, const double* part,
+                          int* done, int test_kind, double tol2, int rz_guard, int stall_window, hipStream_t st);
 hipError_t launch_cg_update_only(int64_t n3, float* x, float* r, const float* p, const float* q, const int2* node_q,
                                  const float* partial, double* scal, const double* pq_part, int n_pq, double* part,
                                  int* done, int guard, hipStream_t st);

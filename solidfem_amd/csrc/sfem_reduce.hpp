@@ -15,25 +15,53 @@ __device__ __forceinline__ int vload(const int* f)
     return __hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
+// one 64-bit DPP move of a double (two 32-bit moves); lanes whose source is
+// outside the pattern, and rows that row_mask leaves out, read 0
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ double dpp_f64(double v)
+{
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, ROW_MASK, 0xf, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, ROW_MASK, 0xf, true);
+    return __hiloint2double(hi, lo);
+}
+
+// the sum of v over the wave's 64 lanes, returned to every lane, in a fixed
+// order: row_shr 1, 2, 4, 8 leave each 16-lane row's sum in its lane 15,
+// row_bcast 15 and 31 add the rows into lane 63 (VALU moves: no LDS round trip)
+__device__ __forceinline__ double wave_sum(double v)
+{
+    v += dpp_f64<0x111, 0xf>(v);    // row_shr:1
+    v += dpp_f64<0x112, 0xf>(v);    // row_shr:2
+    v += dpp_f64<0x114, 0xf>(v);    // row_shr:4
+    v += dpp_f64<0x118, 0xf>(v);    // row_shr:8
+    v += dpp_f64<0x142, 0xa>(v);    // row_bcast:15 into rows 1, 3
+    v += dpp_f64<0x143, 0xc>(v);    // row_bcast:31 into rows 2, 3
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), 63);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), 63);
+    return __hiloint2double(hi, lo);
+}
+
 // BS: the workgroup size the kernel is launched with (a constant, so that no
-// dispatch-packet load and no wait on it precede the reduction's loads)
+// dispatch-packet load and no wait on it precede the reduction's loads).
+// Thread 0 ends with the workgroup's sums: each wave's sum, then the waves'
+// sums in wave order.
 template <int NV, int BS>
 __device__ __forceinline__ void block_sum(double (&v)[NV], double* lds /* >= NV*16 */)
 {
     static_assert(BS % 64 == 0 && BS <= 1024, "whole waves");
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = BS >> 6;
 #pragma unroll
-    for (int q = 0; q < NV; q++)
-        for (int off = 32; off > 0; off >>= 1) v[q] += __shfl_down(v[q], off, 64);
+    for (int q = 0; q < NV; q++) v[q] = wave_sum(v[q]);
+    if (nw == 1) return;
     if (lane == 0)
 #pragma unroll
         for (int q = 0; q < NV; q++) lds[q * 16 + w] = v[q];
     __syncthreads();
-    if (w == 0) {
+    if (threadIdx.x == 0) {
 #pragma unroll
         for (int q = 0; q < NV; q++) {
-            double t = lane < nw ? lds[q * 16 + lane] : 0.0;
-            for (int off = 8; off > 0; off >>= 1) t += __shfl_down(t, off, 64);
+            double t = lds[q * 16];
+            for (int k = 1; k < nw; k++) t += lds[q * 16 + k];
             v[q] = t;
         }
     }
