@@ -55,11 +55,13 @@ def workload(grid=(128, 32, 32)):
     return m, fixed, fdead
 
 
-def hessvec_bytes(ne, nn, per_element=False):
+def hessvec_bytes(ne, nn, per_element=False, direction_update=False):
     """Algorithmic bytes of one Hessian-vector apply: per element 16 B node ids +
     36 B Dm^-1 + 4 B volume (+ 8 B mu, lambda when per element), per node 36 B
-    (read u, read d, write Kd), FP32."""
-    return ne * (16 + 36 + 4 + (8 if per_element else 0)) + nn * 36
+    (read u, read d, write Kd), FP32.  direction_update: the CG loop's element
+    kernel, which also forms d = z + beta p_old (per node 12 B more read, z,
+    and 12 B written, the new p)."""
+    return ne * (16 + 36 + 4 + (8 if per_element else 0)) + nn * (36 + (24 if direction_update else 0))
 
 
 def host_info():
@@ -318,13 +320,20 @@ def main():
             extra["multigrid_pcg"] = side_leg(capi.PC_MULTIGRID)
 
     info = s.info()
-    bytes_alg = hessvec_bytes(info["ne"], info["nn"])
+    # the timed launches of the Jacobi-PCG loop are the element kernel with the
+    # direction update (every iteration but the first of each solve)
+    bytes_alg = hessvec_bytes(info["ne"], info["nn"], direction_update=n_timed > 0)
     timing = "in-solve"
     if n_timed == 0:
         # the multigrid loops launch the kernel without events: time it back to back
         ms_elem, _ = s.profile_hessvec(200)
         timing = "back-to-back"
+        bytes_alg = hessvec_bytes(info["ne"], info["nn"])
     achieved = bytes_alg / (ms_elem * 1e-3) / 1e9
+    # the whole stand-alone apply (element blocks + shared-node completion),
+    # back to back at the solution state
+    ms_e2, ms_g2 = s.profile_hessvec(200)
+    bytes_apply = hessvec_bytes(info["ne"], info["nn"])
     line = {
         "metric": "Newton-CG time to 1e-8 rel. residual (learned-PCG vs Jacobi-PCG); elem-evals/s",
         "value": evals / dt,
@@ -353,9 +362,17 @@ def main():
         "tip_deflection_m": float((x - m.X)[:, 2].min()),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(grid),
-                     "kernel": "k_elem_block<0,NEO_HOOKEAN,256> (Hessian-vector element blocks)",
+                     "traffic_source": "profile-derived: rocprofv3 FETCH_SIZE + WRITE_SIZE of the element kernel "
+                                       "(profiles/hessvec_nh786k_counters.md), not measured in this run",
+                     "kernel": ("k_elem_block<0,NEO_HOOKEAN,256,PF> (Hessian-vector element blocks with the CG "
+                                "direction update)" if n_timed else
+                                "k_elem_block<0,NEO_HOOKEAN,256> (Hessian-vector element blocks)"),
                      "ms_per_launch": ms_elem, "timed_launches": n_timed, "timing": timing,
-                     "bytes_per_launch": bytes_alg},
+                     "bytes_per_launch": bytes_alg,
+                     "whole_apply": {"ms_elem": ms_e2, "ms_shared_gather": ms_g2, "bytes": bytes_apply,
+                                     "achieved_gbs": bytes_apply / ((ms_e2 + ms_g2) * 1e-3) / 1e9,
+                                     "frac": bytes_apply / ((ms_e2 + ms_g2) * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                                     "timing": "back-to-back at the final state"}},
     }
     line.update(extra)
     if not args.no_cpu_baseline:
