@@ -362,8 +362,8 @@ def main():
         "tip_deflection_m": float((x - m.X)[:, 2].min()),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(grid),
-                     "traffic_source": "profile-derived: rocprofv3 FETCH_SIZE + WRITE_SIZE of the element kernel "
-                                       "(profiles/hessvec_nh786k_counters.md), not measured in this run",
+                     "traffic_source": "profile-derived: rocprofv3 FETCH_SIZE + WRITE_SIZE of the stand-alone element "
+                                       "kernel (profiles/hessvec_nh786k_counters.md), not measured in this run",
                      "kernel": ("k_elem_block<0,NEO_HOOKEAN,256,PF> (Hessian-vector element blocks with the CG "
                                 "direction update)" if n_timed else
                                 "k_elem_block<0,NEO_HOOKEAN,256> (Hessian-vector element blocks)"),

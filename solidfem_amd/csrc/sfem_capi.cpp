@@ -493,7 +493,7 @@ int sfem_create(const sfem_mesh* m, const sfem_material* mat, sfem_ctx** out)
 {
     if (!m || !mat || !out || !m->tets || !m->X0 || !m->fixed || m->ne <= 0 || m->nn <= 0)
         return fail(SFEM_EINVAL, "sfem_create: null or empty mesh");
-    if (m->nn >= (1 << 30)) return fail(SFEM_EINVAL, "sfem_create: more than 2^30 nodes");
+    if (m->nn >= (1 << 29)) return fail(SFEM_EINVAL, "sfem_create: more than 2^29 nodes");
     if ((mat->material < SFEM_LINEAR || mat->material > SFEM_NEO_HOOKEAN) && mat->material != SFEM_COROTATIONAL_POLAR)
         return fail(SFEM_EINVAL, "sfem_create: unknown material");
     for (int64_t i = 0; i < 4 * m->ne; i++)

@@ -200,6 +200,9 @@ Partition make_partition(const int32_t* tets, int64_t ne, const double* X0, int6
     for (size_t r = 0; r < P.bnodes.size(); r++) {   // rows are in block order
         const uint32_t id = P.bnodes[r] & F_NODE_MASK;
         if (sh_of[id] >= 0) P.bslot[r] = fill[sh_of[id]]++;
+        // the row that stores a node's per-node values: its only block's, or
+        // its first block's (the one with the node's first partial slot)
+        if (sh_of[id] < 0 || P.bslot[r] == P.sh_ptr[sh_of[id]]) P.bnodes[r] |= F_OWNER;
     }
     // rest coordinates of every block row relative to the centre of the block's
     // bounding box (FP64 difference, then FP32): |coordinate| <= half the block

@@ -14,7 +14,8 @@ namespace sfem {
 
 constexpr uint32_t F_SHARED = 0x80000000u;
 constexpr uint32_t F_FIXED = 0x40000000u;
-constexpr uint32_t F_NODE_MASK = 0x3fffffffu;
+constexpr uint32_t F_OWNER = 0x20000000u;   // the node's only block, or the first of its blocks
+constexpr uint32_t F_NODE_MASK = 0x1fffffffu;
 
 // per element Dm^-1 (9, row-major), signed rest volume det(Dm) / 6 and Lame
 // parameters (FemPhysicsMatrixFree.cpp:126-146, FemPhysicsBase.cpp:40-41);
@@ -41,7 +42,7 @@ struct Partition {
     int nblk = 0, nmax = 0;
     int64_t ne_pad = 0;
     std::vector<int> bnode_ptr;        // nblk + 1
-    std::vector<uint32_t> bnodes;      // node id | F_SHARED | F_FIXED per block row
+    std::vector<uint32_t> bnodes;      // node id | F_SHARED | F_FIXED | F_OWNER per block row
     std::vector<uint16_t> lptr;        // per block nb + 1 incidence offsets
     std::vector<uint32_t> lidx, lperm; // 2 x ne_pad: 4 x 16-bit local ids / incidence slots
     std::vector<float> dm32, cf32;     // 9 x ne_pad, 2 x ne_pad (V mu, V lambda), block order

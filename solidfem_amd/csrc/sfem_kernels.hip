@@ -29,9 +29,10 @@
 
 namespace sfem {
 
-constexpr uint32_t NODE_MASK = 0x3fffffffu;
+constexpr uint32_t NODE_MASK = 0x1fffffffu;
 constexpr uint32_t FLAG_SHARED = 0x80000000u;
 constexpr uint32_t FLAG_FIXED = 0x40000000u;
+constexpr uint32_t FLAG_OWNER = 0x20000000u;   // the node's only block, or the first of its blocks
 
 // one workgroup: scal[slot + q] = sum of the NV partial arrays
 template <int NV>
@@ -135,8 +136,7 @@ __global__ __launch_bounds__(BE) void k_elem_block(BlockArgs a, const typename K
                                                     typename KindTraits<KIND>::T* __restrict__ partial,
                                                     const double* __restrict__ ext, int mask_fixed,
                                                     int* __restrict__ done, double* __restrict__ dot_part,
-                                                    const typename KindTraits<KIND>::T* __restrict__ mdiag,
-                                                    const int2* __restrict__ node_q, PFuse pf)
+                                                    const typename KindTraits<KIND>::T* __restrict__ mdiag, PFuse pf)
 {
     static_assert(!PF || KIND == 0, "direction update only in the Hessian-vector kernel");
     typedef typename KindTraits<KIND>::T T;
@@ -226,7 +226,7 @@ __global__ __launch_bounds__(BE) void k_elem_block(BlockArgs a, const typename K
     // LDS stores; the first round of loads is issued before the PF test
     struct Row {
         uint32_t fl, node;
-        int slot, first;
+        int slot;
         V4 uv, dv, xv;
         T zr[3], po[3], md;
     };
@@ -252,14 +252,13 @@ __global__ __launch_bounds__(BE) void k_elem_block(BlockArgs a, const typename K
         // unconditional loads (bslot is -1 on unshared rows): a load under a
         // branch would be waited for where the branch joins
         w.slot = a.bslot[n0 + j];
-        w.first = (DYN || PF) ? node_q[w.node].x : -1;
         w.md = DYN ? mdiag[w.node] : T(0);
     };
     auto store_row = [&](int j, Row& w, float beta) {
         // the row's owner: its only block, or the first block of a shared node
         // (dynamics: that row carries m / h^2, so the node's partial rows add
         // up to one mass term; PF: that block stores p)
-        const bool owner = w.slot < 0 || w.first == w.slot;
+        const bool owner = (w.fl & FLAG_OWNER) != 0;
         if (PF) {
             w.dv.x = w.zr[0] + beta * w.po[0]; w.dv.y = w.zr[1] + beta * w.po[1]; w.dv.z = w.zr[2] + beta * w.po[2];
             w.dv.w = T(0);
@@ -1113,7 +1112,6 @@ static hipError_t launch_elem_mat(int mat, const BlockArgs& a, const void* u, co
     typedef typename KindTraits<KIND>::T T;
     dim3 grid(a.nblk), block(BE);
     const T* md = (const T*)(sizeof(T) == 8 ? (const void*)a.mdiag64 : (const void*)a.mdiag32);
-    const int2* nq = a.node_q;
     // only PF launches write the flag (the test of the previous iteration)
     int* done = const_cast<int*>(done_in ? done_in : a.zero_flag);
     // with events: hipExtLaunchKernelGGL records them at the dispatch's own start
@@ -1121,10 +1119,10 @@ static hipError_t launch_elem_mat(int mat, const BlockArgs& a, const void* u, co
 #define SFEM_ELEM_DYN(M, D)                                                                                          \
     if (ev0)                                                                                                         \
         hipExtLaunchKernelGGL((k_elem_block<KIND, M, BE, D, PF>), grid, block, (uint32_t)lds, s, ev0, ev1, 0u, a,   \
-                              (const T*)u, (const T*)d, (T*)out, (T*)partial, ext, mask_fixed, done, dot, md, nq, pf); \
+                              (const T*)u, (const T*)d, (T*)out, (T*)partial, ext, mask_fixed, done, dot, md, pf);     \
     else                                                                                                             \
         hipLaunchKernelGGL((k_elem_block<KIND, M, BE, D, PF>), grid, block, lds, s, a, (const T*)u, (const T*)d,     \
-                           (T*)out, (T*)partial, ext, mask_fixed, done, dot, md, nq, pf)
+                           (T*)out, (T*)partial, ext, mask_fixed, done, dot, md, pf)
 #define SFEM_ELEM_CASE(M)                                                                                            \
     if (md) {                                                                                                        \
         SFEM_ELEM_DYN(M, true);                                                                                      \

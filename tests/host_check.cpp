@@ -174,6 +174,23 @@ static void check_mesh(const Mesh& m, bool vary_moduli, const char* name)
             }
         }
         for (int u : slot_rows) CHECK(u == 1);
+        // one owner row per referenced node: the row of its only block, or of
+        // the first of its blocks (the one holding its first partial slot)
+        std::vector<int> owners(nn, 0), first_block(nn, 1 << 30);
+        for (int b = 0; b < P.nblk; b++)
+            for (int r = P.bnode_ptr[b]; r < P.bnode_ptr[b + 1]; r++) {
+                const int node = (int)(P.bnodes[r] & F_NODE_MASK);
+                first_block[node] = std::min(first_block[node], b);
+            }
+        for (int b = 0; b < P.nblk; b++)
+            for (int r = P.bnode_ptr[b]; r < P.bnode_ptr[b + 1]; r++) {
+                const int node = (int)(P.bnodes[r] & F_NODE_MASK);
+                if (P.bnodes[r] & F_OWNER) {
+                    owners[node]++;
+                    CHECK(b == first_block[node]);
+                }
+            }
+        for (int64_t i = 0; i < nn; i++) CHECK(owners[i] == (used[i] ? 1 : 0));
         for (int64_t i = 0; i < nn; i++) {
             if (g.held[i] || !used[i]) CHECK(P.node_q[2 * i] == -2);
             else if (node_blocks[i].size() == 1) CHECK(P.node_q[2 * i] == -1);
