@@ -178,9 +178,9 @@ def hessvec_only(s, m, args):
     s.set_positions(x)
     ms_elem, ms_gather = s.profile_hessvec(args.hessvec_only)
     info = s.info()
-    bytes_alg = hessvec_bytes(info["ne"], info["nn"])
+    bytes_alg = hessvec_bytes(info["ne"], info["nn"], per_element=args.per_element_moduli)
     out = {"kernel": f"k_elem_block<0,{args.material.upper().replace('-', '_')},256>", "material": args.material,
-           "reps": args.hessvec_only, "ms_elem": ms_elem,
+           "per_element_moduli": args.per_element_moduli, "reps": args.hessvec_only, "ms_elem": ms_elem,
            "ms_gather": ms_gather, "bytes_alg": bytes_alg, "gbs_alg": bytes_alg / (ms_elem * 1e-3) / 1e9}
     if args.calib_mib:
         import ctypes as C
@@ -216,6 +216,9 @@ def main():
                     help="with --hessvec-only: the material whose Hessian-vector kernel is timed")
     ap.add_argument("--calib-mib", type=int, default=0,
                     help="with --hessvec-only: also run one device-to-device copy of this many MiB (counter calibration)")
+    ap.add_argument("--per-element-moduli", action="store_true",
+                    help="with --hessvec-only: E and nu per element, drawn as in the per-element-material workload "
+                         "(E log-uniform in [1e5, 1e7] Pa, nu uniform in [0.30, 0.48])")
     ap.add_argument("--cpu-full-newton", action="store_true",
                     help="only run the CPU FP64 restatement's converged Jacobi-PCG Newton on --grid in full "
                          "(no GPU), and print its wall time as one JSON line")
@@ -232,7 +235,13 @@ def main():
     m, fixed, fdead = workload(grid)
     if args.material != "neo-hookean" and not args.hessvec_only:
         ap.error("--material applies to --hessvec-only; the solve legs run the Neo-Hookean workload")
-    s = capi.Solver(m.X, m.tets, fixed, MATERIALS[args.material], 1e6, 0.45, 1070.0, gravity=(0.0, 0.0, -9.8))
+    per_e = {}
+    if args.per_element_moduli:
+        if not args.hessvec_only:
+            ap.error("--per-element-moduli applies to --hessvec-only")
+        from solidfem_amd.mesh import uniform01
+        per_e = dict(young_e=10 ** (5 + 2 * uniform01(0, 1, m.ne)), poisson_e=0.30 + 0.18 * uniform01(0, 2, m.ne))
+    s = capi.Solver(m.X, m.tets, fixed, MATERIALS[args.material], 1e6, 0.45, 1070.0, gravity=(0.0, 0.0, -9.8), **per_e)
     s.set_external_forces(fdead)
     if args.hessvec_only:
         hessvec_only(s, m, args)
