@@ -280,6 +280,86 @@ void transpose_prolongation(const int64_t* p_ptr, const int32_t* p_col, const do
         }
 }
 
+bool check_level_transfer(int64_t na, int64_t nc, const int32_t* inject, const int64_t* p_ptr, const int32_t* p_col,
+                          std::string* err)
+{
+    if (nc <= 0 || nc >= na) {
+        *err = "a coarse level has fewer nodes than the one above";
+        return false;
+    }
+    for (int64_t k = 0; k < nc; k++)
+        if (inject[k] < 0 || inject[k] >= na) {
+            *err = "injection index out of range";
+            return false;
+        }
+    if (p_ptr[0] != 0) {
+        *err = "p_ptr[0] != 0";
+        return false;
+    }
+    for (int64_t i = 0; i < na; i++)
+        if (p_ptr[i + 1] < p_ptr[i]) {
+            *err = "p_ptr not monotone";
+            return false;
+        }
+    const int64_t nnz = p_ptr[na];
+    if (nnz >= (int64_t)1 << 31) {
+        *err = "prolongation too large";
+        return false;
+    }
+    for (int64_t k = 0; k < nnz; k++)
+        if (p_col[k] < 0 || p_col[k] >= nc) {
+            *err = "prolongation column out of range";
+            return false;
+        }
+    return true;
+}
+
+DenseLevelData dense_level_data(const int32_t* tets, int64_t ne, const double* X0, int64_t nn, double young,
+                                double poisson, const double* young_e, const double* poisson_e)
+{
+    DenseLevelData o;
+    o.tets4.assign(4 * ne, 0);
+    o.dminv.assign(9 * ne, 0.f);
+    o.cvol.assign(2 * ne, 0.f);
+    std::vector<int> count(nn + 1, 0);
+    for (int64_t e = 0; e < ne; e++) {
+        const int32_t* t = tets + 4 * e;
+        double D[9], v[9];
+        for (int k = 1; k < 4; k++)
+            for (int r = 0; r < 3; r++) D[3 * r + (k - 1)] = X0[3 * (int64_t)t[k] + r] - X0[3 * (int64_t)t[0] + r];
+        v[0] = D[4] * D[8] - D[5] * D[7];
+        v[3] = -D[3] * D[8] + D[6] * D[5];
+        v[6] = D[3] * D[7] - D[6] * D[4];
+        v[1] = D[2] * D[7] - D[1] * D[8];
+        v[4] = D[0] * D[8] - D[2] * D[6];
+        v[7] = D[1] * D[6] - D[0] * D[7];
+        v[2] = D[1] * D[5] - D[2] * D[4];
+        v[5] = D[2] * D[3] - D[0] * D[5];
+        v[8] = D[0] * D[4] - D[1] * D[3];
+        const double det = D[0] * v[0] + D[1] * v[3] + D[2] * v[6];
+        const double E = young_e ? young_e[e] : young, nu = poisson_e ? poisson_e[e] : poisson;
+        const double lam = E * nu / (1.0 + nu) / (1.0 - 2.0 * nu), mu = 0.5 * E / (1.0 + nu);
+        for (int k = 0; k < 9; k++) o.dminv[9 * e + k] = (float)(v[k] / det);
+        o.cvol[2 * e] = (float)(det / 6.0 * mu);
+        o.cvol[2 * e + 1] = (float)(det / 6.0 * lam);
+        for (int a = 0; a < 4; a++) {
+            o.tets4[4 * e + a] = t[a];
+            count[t[a] + 1]++;
+        }
+    }
+    o.inc_ptr.assign(nn + 1, 0);
+    for (int64_t i = 0; i < nn; i++) o.inc_ptr[i + 1] = o.inc_ptr[i] + count[i + 1];
+    o.inc.assign(2 * (size_t)o.inc_ptr[nn], 0);
+    std::vector<int> fill(o.inc_ptr.begin(), o.inc_ptr.end() - 1);
+    for (int64_t e = 0; e < ne; e++)
+        for (int a = 0; a < 4; a++) {
+            const int slot = fill[tets[4 * e + a]]++;
+            o.inc[2 * (size_t)slot] = (int)e;
+            o.inc[2 * (size_t)slot + 1] = a;
+        }
+    return o;
+}
+
 void cable_attachments(const std::vector<int>& node4, int64_t nn, std::vector<int>* touched, std::vector<int>* tptr,
                        std::vector<std::pair<int, int>>* tent)
 {

@@ -68,6 +68,25 @@ void node_triangle_csr(const int32_t* tris, int64_t ntri, int64_t nn, std::vecto
 void transpose_prolongation(const int64_t* p_ptr, const int32_t* p_col, const double* p_val, int64_t na, int64_t nc,
                             std::vector<int>* tptr, std::vector<int>* trow, std::vector<float>* tval);
 
+// the transfer arguments of a multigrid level below a level of na nodes:
+// nc coarse nodes (0 < nc < na), inject[nc] in [0, na), the prolongation as
+// CSR over the na rows (p_ptr[0] = 0, monotone, nnz < 2^31, columns in
+// [0, nc)); false and *err set otherwise
+bool check_level_transfer(int64_t na, int64_t nc, const int32_t* inject, const int64_t* p_ptr, const int32_t* p_col,
+                          std::string* err);
+
+// the element data of a directly solved (dense) level, element order: its
+// corners, Dm^-1 (FP32, row-major) and (V mu, V lambda) (FP32) as the context
+// computes them at set-up (FemPhysicsMatrixFree.cpp:126-146,
+// FemPhysicsBase.cpp:40-41), and per node its (element, corner) incidences
+// in element order (CSR: inc_ptr[nn + 1], inc = 2 ints per incidence)
+struct DenseLevelData {
+    std::vector<int> tets4, inc_ptr, inc;
+    std::vector<float> dminv, cvol;
+};
+DenseLevelData dense_level_data(const int32_t* tets, int64_t ne, const double* X0, int64_t nn, double young,
+                                double poisson, const double* young_e, const double* poisson_e);
+
 // per mesh node (of nn) the cable attachments (cable node j, weight index
 // 4 j + k) of cable nodes whose element corners are node4[4 j + k]; the
 // touched nodes in increasing order with CSR offsets

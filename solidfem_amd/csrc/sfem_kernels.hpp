@@ -14,7 +14,10 @@ constexpr int BLOCK_E = SFEM_BLOCK_E;
 constexpr int RED_GRID = 2048;   // reduction partials buffer holds 3 * RED_GRID doubles
 // workgroups of the CG vector kernels; each of them re-reduces the previous
 // kernel's partials, so this stays small
-constexpr int CG_GRID = 1024;
+#ifndef SFEM_CG_GRID
+#define SFEM_CG_GRID 1024
+#endif
+constexpr int CG_GRID = SFEM_CG_GRID;
 // threads per workgroup of the CG vector kernels (one node per thread)
 #ifndef SFEM_CG_BLOCK
 #define SFEM_CG_BLOCK 256

@@ -338,53 +338,16 @@ CgResult mg_pcg(sfem_ctx* c, int maxit, double tol2, int64_t* hv_evals)
 
 namespace {
 
-// the element data of a small level for its direct solve (element order):
-// Dm^-1 and V mu, V lambda as the context computes them at set-up
-// (FemPhysicsMatrixFree.cpp:126-146, FemPhysicsBase.cpp:40-41)
+// the element data of a small level for its direct solve (dense_level_data)
 void dense_setup(MgLevel& m, const sfem_ctx* f, const sfem_mesh* mesh, const double* young_e, const double* poisson_e)
 {
     const int64_t ne = mesh->ne, nn = mesh->nn;
-    std::vector<int> tets4(4 * ne);
-    std::vector<float> dminv(9 * ne), cvol(2 * ne);
-    std::vector<std::vector<std::pair<int, int>>> inc(nn);
-    for (int64_t e = 0; e < ne; e++) {
-        const int32_t* t = mesh->tets + 4 * e;
-        double D[9], v[9];
-        for (int k = 1; k < 4; k++)
-            for (int r = 0; r < 3; r++) D[3 * r + (k - 1)] = mesh->X0[3 * (int64_t)t[k] + r] - mesh->X0[3 * (int64_t)t[0] + r];
-        v[0] = D[4] * D[8] - D[5] * D[7];
-        v[3] = -D[3] * D[8] + D[6] * D[5];
-        v[6] = D[3] * D[7] - D[6] * D[4];
-        v[1] = D[2] * D[7] - D[1] * D[8];
-        v[4] = D[0] * D[8] - D[2] * D[6];
-        v[7] = D[1] * D[6] - D[0] * D[7];
-        v[2] = D[1] * D[5] - D[2] * D[4];
-        v[5] = D[2] * D[3] - D[0] * D[5];
-        v[8] = D[0] * D[4] - D[1] * D[3];
-        const double det = D[0] * v[0] + D[1] * v[3] + D[2] * v[6];
-        const double E = young_e ? young_e[e] : f->young, nu = poisson_e ? poisson_e[e] : f->poisson;
-        const double lam = E * nu / (1.0 + nu) / (1.0 - 2.0 * nu), mu = 0.5 * E / (1.0 + nu);
-        for (int k = 0; k < 9; k++) dminv[9 * e + k] = (float)(v[k] / det);
-        cvol[2 * e] = (float)(det / 6.0 * mu);
-        cvol[2 * e + 1] = (float)(det / 6.0 * lam);
-        for (int a = 0; a < 4; a++) {
-            tets4[4 * e + a] = t[a];
-            inc[t[a]].push_back({(int)e, a});
-        }
-    }
-    std::vector<int> inc_ptr(nn + 1, 0), incv;
-    for (int64_t i = 0; i < nn; i++) {
-        inc_ptr[i + 1] = inc_ptr[i] + (int)inc[i].size();
-        for (auto& p : inc[i]) {
-            incv.push_back(p.first);
-            incv.push_back(p.second);
-        }
-    }
-    m.d_tets4.upload(tets4);
-    m.d_dminv.upload(dminv);
-    m.d_cvol.upload(cvol);
-    m.d_inc_ptr.upload(inc_ptr);
-    m.d_inc.upload(incv);
+    const DenseLevelData d = dense_level_data(mesh->tets, ne, mesh->X0, nn, f->young, f->poisson, young_e, poisson_e);
+    m.d_tets4.upload(d.tets4);
+    m.d_dminv.upload(d.dminv);
+    m.d_cvol.upload(d.cvol);
+    m.d_inc_ptr.upload(d.inc_ptr);
+    m.d_inc.upload(d.inc);
     m.d_Ke.alloc(144 * (size_t)ne * sizeof(float));
     const size_t n = 3 * (size_t)nn;
     m.d_A.alloc(n * n * sizeof(float));
@@ -408,16 +371,9 @@ int sfem_mg_add_level(sfem_ctx* f, const sfem_mesh* coarse, const int32_t* injec
         return fail(SFEM_EINVAL, "sfem_mg_add_level: this context has per-element moduli; give the level's own");
     sfem_ctx* above = f->mg.empty() ? f : f->mg.back()->c;
     const int64_t na = above->nn, nc = coarse->nn;
-    if (nc <= 0 || nc >= na) return fail(SFEM_EINVAL, "sfem_mg_add_level: a coarse level has fewer nodes than the one above");
-    for (int64_t k = 0; k < nc; k++)
-        if (inject[k] < 0 || inject[k] >= na) return fail(SFEM_EINVAL, "sfem_mg_add_level: injection index out of range");
-    if (p_ptr[0] != 0) return fail(SFEM_EINVAL, "sfem_mg_add_level: p_ptr[0] != 0");
-    for (int64_t i = 0; i < na; i++)
-        if (p_ptr[i + 1] < p_ptr[i]) return fail(SFEM_EINVAL, "sfem_mg_add_level: p_ptr not monotone");
+    std::string err;
+    if (!check_level_transfer(na, nc, inject, p_ptr, p_col, &err)) return fail(SFEM_EINVAL, "sfem_mg_add_level: " + err);
     const int64_t nnz = p_ptr[na];
-    if (nnz >= (int64_t)1 << 31) return fail(SFEM_EINVAL, "sfem_mg_add_level: prolongation too large");
-    for (int64_t k = 0; k < nnz; k++)
-        if (p_col[k] < 0 || p_col[k] >= nc) return fail(SFEM_EINVAL, "sfem_mg_add_level: prolongation column out of range");
     sfem_material mat{};
     mat.material = f->material == MAT_COROTATIONAL_POLAR ? SFEM_COROTATIONAL_POLAR : f->material;
     mat.young = f->young;

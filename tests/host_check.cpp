@@ -3,7 +3,8 @@
 // AddressSanitizer + UndefinedBehaviorSanitizer by tests/test_host_sanitized_cpu.py.
 // Checks the invariants the device kernels rely on, on structured, jittered
 // and irregular meshes (nodes no element references, a single element,
-// several blocks, varying moduli); exits non-zero on the first violation.
+// several blocks, varying moduli), the CSR tables, and the multigrid level
+// set-up; exits non-zero on the first violation.
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -245,6 +246,64 @@ static void check_csr_helpers()
     std::printf("csr helpers ok\n");
 }
 
+// the directly solved multigrid level's element data and incidence lists,
+// and the validation of a level's transfer arguments
+static void check_multigrid_setup()
+{
+    const Mesh m = beam(3, 2, 2, 0.15, 11);
+    std::vector<double> ye(m.ne()), pe(m.ne());
+    for (int64_t e = 0; e < m.ne(); e++) {
+        ye[e] = 1e5 * (1 + e % 7);
+        pe[e] = 0.3 + 0.01 * (e % 5);
+    }
+    const DenseLevelData d = dense_level_data(m.tets.data(), m.ne(), m.X.data(), m.nn(), 1e6, 0.45, ye.data(), pe.data());
+    CHECK(d.tets4 == m.tets);
+    CHECK((int64_t)d.inc_ptr.size() == m.nn() + 1 && d.inc_ptr.back() == 4 * m.ne());
+    std::vector<int> seen(4 * m.ne(), 0);
+    for (int64_t n = 0; n < m.nn(); n++)
+        for (int k = d.inc_ptr[n]; k < d.inc_ptr[n + 1]; k++) {
+            const int e = d.inc[2 * k], a = d.inc[2 * k + 1];
+            CHECK(e >= 0 && e < m.ne() && a >= 0 && a < 4 && m.tets[4 * e + a] == n);
+            if (k > d.inc_ptr[n]) CHECK(d.inc[2 * (k - 1)] <= e);    // element order
+            seen[4 * e + a]++;
+        }
+    for (int u : seen) CHECK(u == 1);
+    for (int64_t e = 0; e < m.ne(); e++) {
+        const int32_t* t = m.tets.data() + 4 * e;
+        double D[9];
+        for (int k = 1; k < 4; k++)
+            for (int r = 0; r < 3; r++) D[3 * r + (k - 1)] = m.X[3 * t[k] + r] - m.X[3 * t[0] + r];
+        // Dm^-1 Dm = I (FP32 storage)
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) {
+                double v = 0;
+                for (int k = 0; k < 3; k++) v += d.dminv[9 * e + 3 * i + k] * D[3 * k + j];
+                CHECK(std::fabs(v - (i == j)) < 1e-5);
+            }
+        const double det = D[0] * (D[4] * D[8] - D[5] * D[7]) - D[1] * (D[3] * D[8] - D[5] * D[6]) +
+                           D[2] * (D[3] * D[7] - D[4] * D[6]);
+        const double mu = 0.5 * ye[e] / (1 + pe[e]), lam = ye[e] * pe[e] / (1 + pe[e]) / (1 - 2 * pe[e]);
+        CHECK(std::fabs(d.cvol[2 * e] - det / 6 * mu) <= 1e-6 * det / 6 * mu);
+        CHECK(std::fabs(d.cvol[2 * e + 1] - det / 6 * lam) <= 1e-6 * det / 6 * lam);
+    }
+    // transfer validation: a valid 5 -> 3 injection and prolongation, then each defect
+    const int32_t inj[] = {0, 2, 4};
+    const int64_t pp[] = {0, 1, 3, 4, 6, 7};
+    const int32_t pc[] = {0, 0, 1, 1, 1, 2, 2};
+    std::string err;
+    CHECK(check_level_transfer(5, 3, inj, pp, pc, &err));
+    CHECK(!check_level_transfer(3, 3, inj, pp, pc, &err) && err.find("fewer") != std::string::npos);
+    const int32_t bad_inj[] = {0, 2, 5};
+    CHECK(!check_level_transfer(5, 3, bad_inj, pp, pc, &err) && err.find("injection") != std::string::npos);
+    const int64_t pp_start[] = {1, 1, 3, 4, 6, 7};
+    CHECK(!check_level_transfer(5, 3, inj, pp_start, pc, &err) && err.find("p_ptr[0]") != std::string::npos);
+    const int64_t pp_down[] = {0, 3, 2, 4, 6, 7};
+    CHECK(!check_level_transfer(5, 3, inj, pp_down, pc, &err) && err.find("monotone") != std::string::npos);
+    const int32_t pc_bad[] = {0, 0, 1, 3, 1, 2, 2};
+    CHECK(!check_level_transfer(5, 3, inj, pp, pc_bad, &err) && err.find("column") != std::string::npos);
+    std::printf("multigrid set-up ok\n");
+}
+
 static void check_errors()
 {
     Mesh m = beam(1, 1, 1, 0.0, 1);
@@ -291,6 +350,7 @@ int main()
         for (int k = 0; k < 4; k++) s.tets.push_back(perm[r.tets[4 * e + k]]);
     check_mesh(s, true, "shuffled nodes and elements");
     check_csr_helpers();
+    check_multigrid_setup();
     check_errors();
     if (failures) {
         std::fprintf(stderr, "%d check(s) failed\n", failures);
