@@ -541,7 +541,7 @@ __global__ __launch_bounds__(CG_BLOCK) void k_cg_init(int64_t n3, const float* _
 {
     const int64_t nn = n3 / 3;
     double v[2] = {0.0, 0.0};
-    for (int64_t i = blockIdx.x * (int64_t)CG_BLOCK + threadIdx.x; i < nn; i += (int64_t)gridDim.x * CG_BLOCK) {
+    for (int64_t i = xcd_block(blockIdx.x, gridDim.x) * (int64_t)CG_BLOCK + threadIdx.x; i < nn; i += (int64_t)gridDim.x * CG_BLOCK) {
         float rv[3], zv[3];
 #pragma unroll
         for (int c = 0; c < 3; c++) { rv[c] = fixed[i] ? 0.f : b[3 * i + c]; x[3 * i + c] = 0.f; r[3 * i + c] = rv[c]; }
@@ -602,7 +602,8 @@ __global__ __launch_bounds__(CG_BLOCK) void k_cg_update(int64_t n3, float* __res
     const int stop = vload(done);
     const int64_t nn = n3 / 3;
     const int64_t stride = (int64_t)gridDim.x * CG_BLOCK;
-    int64_t i = blockIdx.x * (int64_t)CG_BLOCK + threadIdx.x;
+    // each XCD updates a contiguous range of nodes (its workgroups' first rounds)
+    int64_t i = xcd_block(blockIdx.x, gridDim.x) * (int64_t)CG_BLOCK + threadIdx.x;
     float pv[3], qv[3], rv[3], xv[3], M[9];
     int2 nq = make_int2(-2, 0);
     auto load_rows = [&](int64_t n) {
