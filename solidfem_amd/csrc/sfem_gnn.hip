@@ -1,12 +1,14 @@
 // sfem_gnn.hip -- kernels of the learned block preconditioner: a message-passing
 // network over the mesh node graph (3 rounds, width 32) whose MLP layers are
-// BF16 x BF16 -> FP32 GEMMs on the CDNA matrix cores (v_mfma_f32_16x16x16_bf16),
-// its backward pass, the SPD block construction, the training loss and Adam.
+// BF16 x BF16 -> FP32 GEMMs on the matrix cores (gfx950's double-K
+// v_mfma_f32_16x16x32_bf16 where the reduction length is a multiple of 32,
+// v_mfma_f32_16x16x16_bf16 for 16), its backward pass, the SPD block
+// construction, the training loss and Adam.
 //
-// MFMA 16x16x16 operand layout (one wave, lane l, item q = 0..3):
-//   A[i][k]: i = l % 16, k = 4 (l / 16) + q
-//   B[k][j]: j = l % 16, k = 4 (l / 16) + q
-//   D[i][j]: j = l % 16, i = 4 (l / 16) + q
+// MFMA operand layouts (one wave, lane l):
+//   16x16x16, item q = 0..3:  A[i][k]: i = l % 16, k = 4 (l / 16) + q;  B[k][j]: j = l % 16, k = 4 (l / 16) + q
+//   16x16x32, item q = 0..7:  A[i][k]: i = l % 16, k = 8 (l / 16) + q;  B[k][j]: j = l % 16, k = 8 (l / 16) + q
+//   both:                     D[i][j]: j = l % 16, i = 4 (l / 16) + q (q = 0..3)
 // Every reduction over nodes writes per-workgroup partials that one kernel
 // sums in index order, so training is bitwise reproducible run to run.
 #include <hip/hip_runtime.h>
@@ -18,6 +20,7 @@
 namespace sfem {
 
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
 // FP32 -> BF16, round to nearest even (finite inputs)
@@ -28,16 +31,37 @@ __device__ __forceinline__ short f2bf(float f)
     return (short)(u >> 16);
 }
 
+__device__ __forceinline__ __bf16 bits_bf16(short h) { return __builtin_bit_cast(__bf16, h); }
+
+// one step of the reduction: D += A B over 32 (KS = 32) or 16 (KS = 16) values
+// of k; a8 / b8 hold the lane's 8 (or first 4) items
+template <int KS>
+__device__ __forceinline__ floatx4 mfma_step(const short (&a8)[8], const short (&b8)[8], floatx4 acc)
+{
+    if constexpr (KS == 32) {
+        bf16x8 a, b;
+#pragma unroll
+        for (int q = 0; q < 8; q++) { a[q] = bits_bf16(a8[q]); b[q] = bits_bf16(b8[q]); }
+        return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc, 0, 0, 0);
+    } else {
+        bf16x4 a, b;
+#pragma unroll
+        for (int q = 0; q < 4; q++) { a[q] = a8[q]; b[q] = b8[q]; }
+        return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+    }
+}
+
 // ------------------------------------------------------------------- GEMMs
 // Y[N,NO] = [X1 | X2] W + b (+ R when non-null), X1: N x K1, X2: N x (K - K1),
 // W: K x NO bf16 row-major; Y and Yact = relu(Y) each written when non-null.
 // K1 is a multiple of 4 (each lane loads 4 consecutive inputs of one row as
-// one float4).  One wave = 16 rows x NO columns.
+// one float4, from X1 or from X2).  One wave = 16 rows x NO columns.
 template <int K, int NO>
 __global__ __launch_bounds__(256) void k_gemm(int64_t N, const float* __restrict__ X1, int K1, const float* __restrict__ X2,
                                               const uint16_t* __restrict__ W, const float* __restrict__ bias,
                                               const float* __restrict__ R, float* __restrict__ Y, float* __restrict__ Yact)
 {
+    constexpr int KS = K % 32 == 0 ? 32 : 16, PQ = KS / 4;   // k values per step, per lane
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int64_t row0 = ((int64_t)blockIdx.x * 4 + wave) * 16;
     if (row0 >= N) return;
@@ -48,23 +72,26 @@ __global__ __launch_bounds__(256) void k_gemm(int64_t N, const float* __restrict
     const int64_t ar = row0 + i;
     const int K2 = K - K1;
 #pragma unroll
-    for (int k0 = 0; k0 < K; k0 += 16) {
-        bf16x4 a;
-        const int k = k0 + 4 * kq;
-        float4 xv = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (ar < N)
-            xv = k < K1 ? *reinterpret_cast<const float4*>(X1 + ar * K1 + k)
-                        : *reinterpret_cast<const float4*>(X2 + ar * K2 + (k - K1));
-        a[0] = f2bf(xv.x);
-        a[1] = f2bf(xv.y);
-        a[2] = f2bf(xv.z);
-        a[3] = f2bf(xv.w);
+    for (int k0 = 0; k0 < K; k0 += KS) {
+        short a[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+        for (int h = 0; h < PQ / 4; h++) {
+            const int k = k0 + PQ * kq + 4 * h;
+            float4 xv = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (ar < N)
+                xv = k < K1 ? *reinterpret_cast<const float4*>(X1 + ar * K1 + k)
+                            : *reinterpret_cast<const float4*>(X2 + ar * K2 + (k - K1));
+            a[4 * h] = f2bf(xv.x);
+            a[4 * h + 1] = f2bf(xv.y);
+            a[4 * h + 2] = f2bf(xv.z);
+            a[4 * h + 3] = f2bf(xv.w);
+        }
 #pragma unroll
         for (int t = 0; t < NO / 16; t++) {
-            bf16x4 b;
+            short b[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 #pragma unroll
-            for (int q = 0; q < 4; q++) b[q] = (short)W[(k0 + 4 * kq + q) * NO + t * 16 + i];
-            acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[t], 0, 0, 0);
+            for (int q = 0; q < PQ; q++) b[q] = (short)W[(k0 + PQ * kq + q) * NO + t * 16 + i];
+            acc[t] = mfma_step<KS>(a, b, acc[t]);
         }
     }
 #pragma unroll
@@ -88,6 +115,7 @@ __global__ __launch_bounds__(256) void k_gemm_wgrad(int64_t N, const float* __re
                                                     const float* __restrict__ dY, int64_t rows_per_block,
                                                     float* __restrict__ partial)
 {
+    // the reduction runs over rows: 32 per MFMA step (rows_per_block is a multiple of 32)
     constexpr int TN = NO / 16, NT = (K / 16) * TN, PER = (NT + 3) / 4;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int i = lane & 15, kq = lane >> 4;
@@ -97,16 +125,16 @@ __global__ __launch_bounds__(256) void k_gemm_wgrad(int64_t N, const float* __re
     floatx4 acc[PER];
 #pragma unroll
     for (int tt = 0; tt < PER; tt++) acc[tt] = floatx4{0.f, 0.f, 0.f, 0.f};
-    for (int64_t r0 = rbeg; r0 < rend; r0 += 16) {
+    for (int64_t r0 = rbeg; r0 < rend; r0 += 32) {
 #pragma unroll
         for (int tt = 0; tt < PER; tt++) {
             const int t = wave + 4 * tt;
             if (t < NT) {
                 const int kt = t / TN, nt = t % TN;
-                bf16x4 a, b;
+                short a[8], b[8];
 #pragma unroll
-                for (int q = 0; q < 4; q++) {
-                    const int64_t r = r0 + 4 * kq + q;
+                for (int q = 0; q < 8; q++) {
+                    const int64_t r = r0 + 8 * kq + q;
                     const int k = kt * 16 + i;
                     float xv = 0.f, yv = 0.f;
                     if (r < rend) {
@@ -116,7 +144,7 @@ __global__ __launch_bounds__(256) void k_gemm_wgrad(int64_t N, const float* __re
                     a[q] = f2bf(xv);
                     b[q] = f2bf(yv);
                 }
-                acc[tt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[tt], 0, 0, 0);
+                acc[tt] = mfma_step<32>(a, b, acc[tt]);
             }
         }
     }
@@ -475,7 +503,7 @@ hipError_t gnn_gemm(int K, int NO, int64_t N, const float* X1, int K1, const flo
 hipError_t gnn_wgrad(int K, int NO, int64_t N, const float* X1, int K1, const float* X2, const float* dY, float* partial,
                      int nparts, float* out, hipStream_t s)
 {
-    const int64_t rpb = ((N + nparts - 1) / nparts + 15) / 16 * 16;
+    const int64_t rpb = ((N + nparts - 1) / nparts + 31) / 32 * 32;
     const dim3 g((unsigned)nparts), b(256);
 #define SFEM_WG_CASE(KK, NN)                                                                              \
     if (K == KK && NO == NN) {                                                                             \
