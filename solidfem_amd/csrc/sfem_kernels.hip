@@ -526,11 +526,16 @@ __global__ __launch_bounds__(256) void k_shared_gather(SharedArgs s, const T* __
 }
 
 // ---------------------------------------------------------------- CG kernels
-// Per iteration: hessvec (element blocks + shared gather, which also leave
-// p.q partials), k_cg_update (alpha, x, r, z; r.z and r.r partials), k_cg_p
-// (beta, convergence test, p).  Each workgroup reduces the partials it needs
-// itself in a fixed order: no atomics, no extra launch.  Scalars that a kernel
-// reads are never written by the same kernel (S_RZ_PREV holds the previous r.z).
+// Per iteration two launches: the element kernel (K p and the p.q partials;
+// from the second iteration on it first makes the previous iteration's test
+// and direction update, beta and p = z + beta p, PFuse) and k_cg_update
+// (alpha, x, r, z, summing the shared nodes' partial rows; r.z and r.r
+// partials).  k_cg_p makes the test and the direction update on its own:
+// after the last launch of a solve, and in the multigrid PCG, where the
+// V-cycle runs between the update and the direction.  Each workgroup reduces
+// the partials it needs itself in a fixed order: no atomics, no extra launch.
+// Scalars that a kernel reads are never written by the same kernel
+// (S_RZ_PREV holds the previous r.z).
 
 // x = 0, r = b, z = M r, p = z (and p2 = z); partials of r.z, r.r
 template <int PC>
