@@ -185,3 +185,31 @@ def test_polar_corotational_torsion_converges(precond):
     stq = sq.step(capi.solve_params(tol=1e-8, precond=capi.PC_JACOBI, outer=30, sim_type=capi.ST_QUASI_STATIC,
                                     force_step=1.0 / nsteps), allow_notconv=True)
     print(f"QR model: converged {stq['converged']} rel residual {stq['res'] / stq['ext_norm']:.2e}", flush=True)
+
+
+@pytest.mark.parametrize("precond", [capi.PC_NONE, capi.PC_JACOBI])
+@pytest.mark.parametrize("mode,inner", [(capi.SOLVE_CONVERGED, 20000), (capi.SOLVE_REFERENCE, 100),
+                                        (capi.SOLVE_CONVERGED, 7)])
+def test_cg_result_independent_of_check_interval(precond, mode, inner):
+    """The CG loop makes each iteration's convergence test inside the next
+    operator apply, and the last test in a separate direction kernel when the
+    iteration cap ends the loop; how often the host reads the done flag does
+    not change what the device computes: the same Newton state, bitwise, for
+    converged solves, reference-mode solves (fixed CG eps) and solves cut off
+    by a 7-iteration cap, for check intervals 1, 5 and 32."""
+    if mode == capi.SOLVE_REFERENCE and precond != capi.PC_NONE:
+        pytest.skip("reference mode is unpreconditioned")
+    print(f"case check interval mode={mode} inner={inner} precond={precond}", flush=True)
+    m, fixed, p, s = setup(capi.NEO_HOOKEAN)
+    out = []
+    for every in (1, 5, 32):
+        s.reset()
+        s.set_load_fraction(1.0)
+        st = s.solve(capi.solve_params(mode=mode, precond=precond, inner=inner, outer=10 if mode == capi.SOLVE_REFERENCE else 50,
+                                       check_every=every, tol=1e-8), allow_notconv=True)
+        out.append((s.positions(), st["cg_iters"], st["newton_iters"]))
+    x0, cg0, nw0 = out[0]
+    print(f"  cg {[o[1] for o in out]} newton {[o[2] for o in out]}", flush=True)
+    for x, cg, nw in out[1:]:
+        assert cg == cg0 and nw == nw0
+        assert np.array_equal(x, x0)
