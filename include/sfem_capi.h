@@ -330,6 +330,13 @@ int sfem_mg_apply(sfem_ctx* ctx, const double* r, double* z);
    3 breakdown (p.Kp <= 0 or r.z <= 0), 4 not finite, 5 stagnation (r.r not
    halved within 150 iterations); at most 500 iterations */
 int sfem_mg_pcg(sfem_ctx* ctx, const double* b, double* x, double rtol, int32_t maxit, int32_t* iters, int32_t* status);
+/* The V-cycle's launch schedule (no reference counterpart; for tests and
+   measurement): fused = 1 (the default) folds each level's Chebyshev steps
+   and its prolongation into the element kernels' row staging and the
+   residual into the restriction (6 launches per smoothed level instead of
+   10); fused = 0 launches one kernel per step.  Both give bitwise the same
+   V-cycle. */
+int sfem_mg_set_schedule(sfem_ctx* ctx, int fused);
 
 /* ---------------------------------------------------------------------------
  * Learned block preconditioner (no reference counterpart).  A message-passing

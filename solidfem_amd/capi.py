@@ -30,6 +30,7 @@ EXPORTS = [
     "sfem_block_diagonal", "sfem_solve", "sfem_step", "sfem_set_precond_blocks", "sfem_info",
     "sfem_reset", "sfem_profile_hessvec", "sfem_time_hessvec",
     "sfem_mg_add_level", "sfem_mg_clear", "sfem_mg_info", "sfem_mg_apply", "sfem_mg_pcg", "sfem_mg_coarse_dense",
+    "sfem_mg_set_schedule",
     "sfem_gnn_create", "sfem_gnn_destroy", "sfem_gnn_num_params", "sfem_gnn_get_params", "sfem_gnn_set_params",
     "sfem_set_learned_preconditioner", "sfem_collect_pairs", "sfem_collect_rough_pairs", "sfem_num_pairs", "sfem_get_pair", "sfem_clear_pairs",
     "sfem_gnn_loss", "sfem_gnn_eval", "sfem_gnn_train", "sfem_gnn_inputs", "sfem_gnn_gemm_check",
@@ -125,6 +126,7 @@ def lib():
             "sfem_mg_coarse_dense": ([V, C.c_int64, C.POINTER(C.c_int32), _dp, _dp], C.c_int),
             "sfem_mg_apply": ([V, _dp, _dp], C.c_int),
             "sfem_mg_pcg": ([V, _dp, _dp, C.c_double, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32)], C.c_int),
+            "sfem_mg_set_schedule": ([V, C.c_int], C.c_int),
             "sfem_gnn_create": ([C.c_uint64, C.POINTER(V)], C.c_int),
             "sfem_gnn_destroy": ([V], None),
             "sfem_gnn_num_params": ([V, C.POINTER(C.c_int64)], C.c_int),
@@ -365,6 +367,11 @@ class Solver:
         _check(lib().sfem_mg_apply(self.h, _p(r), _p(z)))
         return z
 
+    def mg_set_schedule(self, fused):
+        """the V-cycle's launch schedule: True folds the smoothing steps into the
+        element kernels (the default), False launches one kernel per step"""
+        _check(lib().sfem_mg_set_schedule(self.h, 1 if fused else 0))
+
     def mg_pcg(self, b, rtol=1e-6, maxit=500):
         """-> (x, iterations, status) of one multigrid-PCG solve K x = b"""
         b = np.ascontiguousarray(b, dtype=np.float64).reshape(-1, 3)
@@ -424,11 +431,15 @@ class LearnedPreconditioner:
         _check(lib().sfem_gnn_set_params(self.h, _p(w, _fp)))
 
     def attach(self, solver):
+        """the context keeps a pointer to the network: the solver object holds
+        a reference to it until detach, so the network outlives its use"""
         _check(lib().sfem_set_learned_preconditioner(solver.h, self.h))
+        solver._learned = self
 
     @staticmethod
     def detach(solver):
         _check(lib().sfem_set_learned_preconditioner(solver.h, None))
+        solver._learned = None
 
     @staticmethod
     def collect(solver, load, nsnap=8, snap_every=10, rtol=1e-6):

@@ -174,6 +174,18 @@ struct sfem_ctx {
     bool mg_ready = false;     // eigenvalue estimates and coarsest inverse exist
     int64_t mg_evals = 0;      // element evaluations of one V-cycle
     DevBuf d_mg_r, d_mg_d, d_mg_q, d_mg_part, d_mg_norm;
+    // the V-cycle's launch schedule: the Chebyshev steps and the prolongation
+    // folded into the element kernels (true), or one kernel per step; the
+    // results are bitwise the same
+    bool mg_fused = true;
+    // this context as a multigrid level under the folded schedule: a second
+    // residual and direction, and a second operator image (rows and partial
+    // slots)
+    DevBuf d_mgw_r2, d_mgw_d2, d_mgw_qb, d_mgw_pb;
+    // the multigrid PCG's done-flag copies (pinned host memory, two slots) and
+    // the events after them
+    int* h_flag = nullptr;
+    hipEvent_t flag_ev[2] = {};
     // the coarsest level's inverse is computed on a second stream (default
     // priority) beside the other levels' set-up: [0] coarse state ready,
     // [1] inverse ready
@@ -192,6 +204,10 @@ struct sfem_ctx {
     {
         for (hipEvent_t e : tm_ev)
             if (e) (void)hipEventDestroy(e);
+        if (stream) (void)hipStreamSynchronize(stream);
+        for (hipEvent_t e : flag_ev)
+            if (e) (void)hipEventDestroy(e);
+        if (h_flag) (void)hipHostFree(h_flag);
         if (mg_side) (void)hipStreamSynchronize(mg_side);
         mg.clear();
         for (hipEvent_t e : mg_ev)

@@ -25,6 +25,7 @@
 
 #include "sfem_device.hpp"
 #include "sfem_kernels.hpp"
+#include "sfem_mgdev.hpp"
 #include "sfem_reduce.hpp"
 
 namespace sfem {
@@ -127,18 +128,25 @@ __device__ __forceinline__ int xcd_block(int w, int nblk)
 }
 
 // DYN: implicit dynamics (mdiag non-null); a separate instance so that static
-// solves carry none of its code.  PF (KIND 0): the CG direction update of the
-// previous iteration is made here (PFuse); d is not read.
-template <int KIND, int MAT, int BE, bool DYN, bool PF>
+// solves carry none of its code.  SG (KIND 0): how the block forms the rows
+// it applies K to -- 0 the given d; 1 (PF) the CG direction update of the
+// previous iteration (PFuse), d is not read; MG_STAGE_CHEB and
+// MG_STAGE_PROLONG a multigrid smoothing step or prolongation (MgStage).
+template <int KIND, int MAT, int BE, bool DYN, int SG>
 __global__ __launch_bounds__(BE) void k_elem_block(BlockArgs a, const typename KindTraits<KIND>::T* __restrict__ u,
                                                     const typename KindTraits<KIND>::T* __restrict__ d,
                                                     typename KindTraits<KIND>::T* __restrict__ out,
                                                     typename KindTraits<KIND>::T* __restrict__ partial,
                                                     const double* __restrict__ ext, int mask_fixed,
                                                     int* __restrict__ done, double* __restrict__ dot_part,
-                                                    const typename KindTraits<KIND>::T* __restrict__ mdiag, PFuse pf)
+                                                    const typename KindTraits<KIND>::T* __restrict__ mdiag, PFuse pf,
+                                                    MgStage mg)
 {
-    static_assert(!PF || KIND == 0, "direction update only in the Hessian-vector kernel");
+    constexpr bool PF = SG == 1;
+    constexpr bool MGC = SG == MG_STAGE_CHEB;
+    constexpr bool MGP = SG == MG_STAGE_PROLONG;
+    static_assert(SG == 0 || KIND == 0, "staged directions only in the Hessian-vector kernel");
+    static_assert(!(DYN && (MGC || MGP)), "multigrid staging is static only");
     typedef typename KindTraits<KIND>::T T;
     typedef typename Vec4<T>::V V4;
     constexpr int W = KindTraits<KIND>::W;
@@ -229,6 +237,11 @@ __global__ __launch_bounds__(BE) void k_elem_block(BlockArgs a, const typename K
         int slot;
         V4 uv, dv, xv;
         T zr[3], po[3], md;
+        // multigrid staging: the node's slot range, its r, d, M, x and the
+        // rows of the previous images (MGC); its prolongation range (MGP)
+        int2 nq;
+        float ro[3], dold[3], mb[9], qa[3], xo[3];
+        int pp0, pp1;
     };
     // every load of a row is issued before any of its values is tested
     auto load_row = [&](int j, Row& w) {
@@ -241,6 +254,21 @@ __global__ __launch_bounds__(BE) void k_elem_block(BlockArgs a, const typename K
             const float* pp = pf.p_old + 3 * (size_t)w.node;
 #pragma unroll
             for (int c = 0; c < 3; c++) { w.zr[c] = zp[c]; w.po[c] = pp[c]; }
+        } else if (MGC) {
+            const size_t n3 = 3 * (size_t)w.node;
+            w.nq = a.node_q[w.node];
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                w.ro[c] = mg.r_old[n3 + c]; w.dold[c] = mg.d_old[n3 + c]; w.qa[c] = mg.qa[n3 + c];
+                w.xo[c] = mg.x[n3 + c];
+            }
+#pragma unroll
+            for (int k = 0; k < 9; k++) w.mb[k] = mg.minv[9 * (size_t)w.node + k];
+        } else if (MGP) {
+            w.pp0 = mg.p_ptr[w.node];
+            w.pp1 = mg.p_ptr[w.node + 1];
+#pragma unroll
+            for (int c = 0; c < 3; c++) w.xo[c] = mg.x[3 * (size_t)w.node + c];
         } else if (HAS_D) {
             const T* dp = d + 3 * (size_t)w.node;
             w.dv.x = dp[0]; w.dv.y = dp[1]; w.dv.z = dp[2]; w.dv.w = T(0);
@@ -266,6 +294,40 @@ __global__ __launch_bounds__(BE) void k_elem_block(BlockArgs a, const typename K
                 float* pn = pf.p_new + 3 * (size_t)w.node;
                 pn[0] = w.dv.x; pn[1] = w.dv.y; pn[2] = w.dv.z;
             }
+        }
+        if (MGC) {
+            // the k_mg_cheb step of this node (same expressions): complete the
+            // previous image, r, then d = c1 d_old + c2 M r and x += d
+            float qv[3], r1[3], mr[3], dn[3];
+            complete_row(w.nq, w.qa, mg.pa, qv);
+#pragma unroll
+            for (int c = 0; c < 3; c++) r1[c] = w.ro[c] - qv[c];
+            block_mul(w.mb, r1, mr);
+#pragma unroll
+            for (int c = 0; c < 3; c++) dn[c] = cheb_dir(mg.has_dold != 0, mg.c1, w.dold[c], mg.c2, mr[c]);
+            if (owner) {
+                const size_t n3 = 3 * (size_t)w.node;
+#pragma unroll
+                for (int c = 0; c < 3; c++) {
+                    mg.r_new[n3 + c] = r1[c];
+                    mg.d_new[n3 + c] = dn[c];
+                    mg.x[n3 + c] = w.xo[c] + dn[c];
+                }
+            }
+            w.dv.x = dn[0]; w.dv.y = dn[1]; w.dv.z = dn[2]; w.dv.w = T(0);
+        } else if (MGP) {
+            // the k_mg_prolong row of this node: e = P xc, 0 at a fixed node
+            float ev[3] = {0.f, 0.f, 0.f};
+            gather_rows(w.pp0, w.pp1, mg.p_col, mg.p_val, mg.xc, ev);
+            const bool f = (w.fl & FLAG_FIXED) != 0;
+#pragma unroll
+            for (int c = 0; c < 3; c++) ev[c] = f ? 0.f : ev[c];
+            if (owner) {
+                const size_t n3 = 3 * (size_t)w.node;
+#pragma unroll
+                for (int c = 0; c < 3; c++) mg.x[n3 + c] = w.xo[c] + ev[c];
+            }
+            w.dv.x = ev[0]; w.dv.y = ev[1]; w.dv.z = ev[2]; w.dv.w = T(0);
         }
         su4[j] = w.uv;
         if (HAS_D) sd4[j] = w.dv;
@@ -592,13 +654,17 @@ __global__ __launch_bounds__(256) void k_cg_init_finish(const double* __restrict
 // of the node's contiguous partial slots for shared nodes, 0 for fixed nodes
 // and nodes of no element (node_q: {first slot, end slot} of a shared node;
 // {-1, 0} free node of one block, {-2, 0} fixed or of no element)
+// PC 2 (multigrid PCG): z = zscale M r into z and z2 -- the V-cycle's first
+// Chebyshev step from x = 0 on the new residual (k_mg_cheb<0>'s direction);
+// no partials (the V-cycle's last step makes them)
 template <int PC>
 __global__ __launch_bounds__(CG_BLOCK) void k_cg_update(int64_t n3, float* __restrict__ x, float* __restrict__ r,
                                                    float* __restrict__ z, const float* __restrict__ p,
                                                    const float* __restrict__ q, const float* __restrict__ minv,
                                                    const int2* __restrict__ node_q, const float* __restrict__ partial,
                                                    double* scal, const double* __restrict__ pq_part, int n_pq,
-                                                   double* __restrict__ part, int* done, int guard)
+                                                   double* __restrict__ part, int* done, int guard, float zscale,
+                                                   float* __restrict__ z2)
 {
     // the done flag is read with the node loads and the p.q reduction and
     // tested before anything is stored (a vector load: as a scalar load its
@@ -675,7 +741,16 @@ __global__ __launch_bounds__(CG_BLOCK) void k_cg_update(int64_t n3, float* __res
             x[3 * i + c] = xv[c];
             r[3 * i + c] = rv[c];
         }
-        if (PC) {
+        if (PC == 2) {
+            float mr[3];
+            block_mul(M, rv, mr);
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                zv[c] = cheb_dir(false, 0.f, 0.f, zscale, mr[c]);
+                z[3 * i + c] = zv[c];
+                z2[3 * i + c] = zv[c];
+            }
+        } else if (PC) {
 #pragma unroll
             for (int c = 0; c < 3; c++) zv[c] = M[3 * c] * rv[0] + M[3 * c + 1] * rv[1] + M[3 * c + 2] * rv[2];
 #pragma unroll
@@ -684,14 +759,16 @@ __global__ __launch_bounds__(CG_BLOCK) void k_cg_update(int64_t n3, float* __res
 #pragma unroll
             for (int c = 0; c < 3; c++) zv[c] = rv[c];
         }
-        v[0] += (double)rv[0] * zv[0] + (double)rv[1] * zv[1] + (double)rv[2] * zv[2];
-        v[1] += (double)rv[0] * rv[0] + (double)rv[1] * rv[1] + (double)rv[2] * rv[2];
+        if (PC != 2) {
+            v[0] += (double)rv[0] * zv[0] + (double)rv[1] * zv[1] + (double)rv[2] * zv[2];
+            v[1] += (double)rv[0] * rv[0] + (double)rv[1] * rv[1] + (double)rv[2] * rv[2];
+        }
         if (i + stride < nn) {
             load_rows(i + stride);
             load_partial_rows();
         }
     }
-    block_partials<2, CG_BLOCK>(v, part);
+    if (PC != 2) block_partials<2, CG_BLOCK>(v, part);
 }
 
 // beta = rz' / rz (every workgroup); test; p = z + beta p, one node per step
@@ -1110,10 +1187,11 @@ static int grid_for(int64_t n, int bs, int cap)
     return (int)(g > cap ? cap : g);
 }
 
-template <int KIND, int BE, bool PF = false>
+template <int KIND, int BE, int SG = 0>
 static hipError_t launch_elem_mat(int mat, const BlockArgs& a, const void* u, const void* d, void* out, void* partial,
                                   const double* ext, int mask_fixed, const int* done_in, double* dot, size_t lds, hipStream_t s,
-                                  hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr, const PFuse& pf = PFuse{})
+                                  hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr, const PFuse& pf = PFuse{},
+                                  const MgStage& mg = MgStage{})
 {
     typedef typename KindTraits<KIND>::T T;
     dim3 grid(a.nblk), block(BE);
@@ -1124,13 +1202,17 @@ static hipError_t launch_elem_mat(int mat, const BlockArgs& a, const void* u, co
     // and end, so their interval is the kernel's execution time
 #define SFEM_ELEM_DYN(M, D)                                                                                          \
     if (ev0)                                                                                                         \
-        hipExtLaunchKernelGGL((k_elem_block<KIND, M, BE, D, PF>), grid, block, (uint32_t)lds, s, ev0, ev1, 0u, a,   \
-                              (const T*)u, (const T*)d, (T*)out, (T*)partial, ext, mask_fixed, done, dot, md, pf);     \
+        hipExtLaunchKernelGGL((k_elem_block<KIND, M, BE, D, SG>), grid, block, (uint32_t)lds, s, ev0, ev1, 0u, a,   \
+                              (const T*)u, (const T*)d, (T*)out, (T*)partial, ext, mask_fixed, done, dot, md, pf, mg); \
     else                                                                                                             \
-        hipLaunchKernelGGL((k_elem_block<KIND, M, BE, D, PF>), grid, block, lds, s, a, (const T*)u, (const T*)d,     \
-                           (T*)out, (T*)partial, ext, mask_fixed, done, dot, md, pf)
+        hipLaunchKernelGGL((k_elem_block<KIND, M, BE, D, SG>), grid, block, lds, s, a, (const T*)u, (const T*)d,     \
+                           (T*)out, (T*)partial, ext, mask_fixed, done, dot, md, pf, mg)
+    // the multigrid stages have no dynamics instance (static solves only)
 #define SFEM_ELEM_CASE(M)                                                                                            \
-    if (md) {                                                                                                        \
+    if constexpr (SG >= MG_STAGE_CHEB) {                                                                             \
+        if (md) return hipErrorInvalidValue;                                                                         \
+        SFEM_ELEM_DYN(M, false);                                                                                     \
+    } else if (md) {                                                                                                 \
         SFEM_ELEM_DYN(M, true);                                                                                      \
     } else {                                                                                                         \
         SFEM_ELEM_DYN(M, false);                                                                                     \
@@ -1226,18 +1308,23 @@ hipError_t set_elem_lds_limits(const BlockArgs& a)
         int kind;
     };
     const Fn fns[] = {
-#define SFEM_FN(K, M) {(const void*)k_elem_block<K, M, BLOCK_E, false, false>, K}, \
-                      {(const void*)k_elem_block<K, M, BLOCK_E, true, false>, K}
-#define SFEM_FN_PF(M) {(const void*)k_elem_block<0, M, BLOCK_E, false, true>, 0}, \
-                      {(const void*)k_elem_block<0, M, BLOCK_E, true, true>, 0}
+#define SFEM_FN(K, M) {(const void*)k_elem_block<K, M, BLOCK_E, false, 0>, K}, \
+                      {(const void*)k_elem_block<K, M, BLOCK_E, true, 0>, K}
+#define SFEM_FN_PF(M) {(const void*)k_elem_block<0, M, BLOCK_E, false, 1>, 0}, \
+                      {(const void*)k_elem_block<0, M, BLOCK_E, true, 1>, 0}
+#define SFEM_FN_MG(M) {(const void*)k_elem_block<0, M, BLOCK_E, false, MG_STAGE_CHEB>, 0}, \
+                      {(const void*)k_elem_block<0, M, BLOCK_E, false, MG_STAGE_PROLONG>, 0}
         SFEM_FN(0, LINEAR), SFEM_FN(0, STVK), SFEM_FN(0, COROTATIONAL), SFEM_FN(0, NEO_HOOKEAN),
         SFEM_FN(0, NEO_HOOKEAN_COARSE), SFEM_FN(0, COROTATIONAL_POLAR),
         SFEM_FN_PF(LINEAR), SFEM_FN_PF(STVK), SFEM_FN_PF(COROTATIONAL), SFEM_FN_PF(NEO_HOOKEAN),
         SFEM_FN_PF(NEO_HOOKEAN_COARSE), SFEM_FN_PF(COROTATIONAL_POLAR),
+        SFEM_FN_MG(LINEAR), SFEM_FN_MG(STVK), SFEM_FN_MG(COROTATIONAL), SFEM_FN_MG(NEO_HOOKEAN),
+        SFEM_FN_MG(NEO_HOOKEAN_COARSE), SFEM_FN_MG(COROTATIONAL_POLAR),
         SFEM_FN(1, LINEAR), SFEM_FN(1, STVK), SFEM_FN(1, COROTATIONAL), SFEM_FN(1, NEO_HOOKEAN),
         SFEM_FN(1, COROTATIONAL_POLAR),
         SFEM_FN(2, LINEAR), SFEM_FN(2, STVK), SFEM_FN(2, COROTATIONAL), SFEM_FN(2, NEO_HOOKEAN),
         SFEM_FN(2, NEO_HOOKEAN_COARSE), SFEM_FN(2, COROTATIONAL_POLAR),
+#undef SFEM_FN_MG
 #undef SFEM_FN_PF
 #undef SFEM_FN
     };
@@ -1283,8 +1370,20 @@ hipError_t launch_hessvec_cg(int mat, const BlockArgs& a, const float* u32, cons
 hipError_t launch_hessvec_cg_fused(int mat, const BlockArgs& a, const float* u32, const PFuse& pf, float* out,
                                    float* partial, int* done, double* dot_elem, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1)
 {
-    return launch_elem_mat<0, BLOCK_E, true>(mat, a, u32, nullptr, out, partial, nullptr, 1, done, dot_elem, lds_bytes<0>(a),
-                                             st, ev0, ev1, pf);
+    return launch_elem_mat<0, BLOCK_E, 1>(mat, a, u32, nullptr, out, partial, nullptr, 1, done, dot_elem, lds_bytes<0>(a),
+                                          st, ev0, ev1, pf);
+}
+
+hipError_t launch_hessvec_mg(int mat, const BlockArgs& a, const float* u32, int stage, const MgStage& mg, float* out,
+                             float* partial, const int* done, hipStream_t st)
+{
+    if (stage == MG_STAGE_CHEB)
+        return launch_elem_mat<0, BLOCK_E, MG_STAGE_CHEB>(mat, a, u32, nullptr, out, partial, nullptr, 1, done, nullptr,
+                                                          lds_bytes<0>(a), st, nullptr, nullptr, PFuse{}, mg);
+    if (stage == MG_STAGE_PROLONG)
+        return launch_elem_mat<0, BLOCK_E, MG_STAGE_PROLONG>(mat, a, u32, nullptr, out, partial, nullptr, 1, done, nullptr,
+                                                             lds_bytes<0>(a), st, nullptr, nullptr, PFuse{}, mg);
+    return hipErrorInvalidValue;
 }
 
 hipError_t launch_cg_test(int pc, int64_t n3, const float* z, const float* r, float* p, double* scal, const double* part,
@@ -1303,8 +1402,8 @@ hipError_t launch_cg_update(int pc, int64_t n3, float* x, float* r, float* z, co
                             double* part, int* done, int guard, hipStream_t st)
 {
     const int g = cg_grid(n3 / 3);
-    if (pc) hipLaunchKernelGGL(k_cg_update<1>, dim3(g), dim3(CG_BLOCK), 0, st, n3, x, r, z, p, q, minv, node_q, partial, scal, pq_part, n_pq, part, done, guard);
-    else hipLaunchKernelGGL(k_cg_update<0>, dim3(g), dim3(CG_BLOCK), 0, st, n3, x, r, z, p, q, minv, node_q, partial, scal, pq_part, n_pq, part, done, guard);
+    if (pc) hipLaunchKernelGGL(k_cg_update<1>, dim3(g), dim3(CG_BLOCK), 0, st, n3, x, r, z, p, q, minv, node_q, partial, scal, pq_part, n_pq, part, done, guard, 1.f, nullptr);
+    else hipLaunchKernelGGL(k_cg_update<0>, dim3(g), dim3(CG_BLOCK), 0, st, n3, x, r, z, p, q, minv, node_q, partial, scal, pq_part, n_pq, part, done, guard, 1.f, nullptr);
     return hipGetLastError();
 }
 
@@ -1317,7 +1416,17 @@ hipError_t launch_cg_update_only(int64_t n3, float* x, float* r, const float* p,
 {
     const int g = cg_grid(n3 / 3);
     hipLaunchKernelGGL(k_cg_update<0>, dim3(g), dim3(CG_BLOCK), 0, st, n3, x, r, nullptr, p, q, nullptr, node_q, partial,
-                       scal, pq_part, n_pq, part, done, guard);
+                       scal, pq_part, n_pq, part, done, guard, 1.f, nullptr);
+    return hipGetLastError();
+}
+
+hipError_t launch_cg_update_mg(int64_t n3, float* x, float* r, float* z, float* z2, const float* p, const float* q,
+                               const float* minv, float c2, const int2* node_q, const float* partial, double* scal,
+                               const double* pq_part, int n_pq, int* done, int guard, hipStream_t st)
+{
+    const int g = cg_grid(n3 / 3);
+    hipLaunchKernelGGL(k_cg_update<2>, dim3(g), dim3(CG_BLOCK), 0, st, n3, x, r, z, p, q, minv, node_q, partial, scal,
+                       pq_part, n_pq, nullptr, done, guard, c2, z2);
     return hipGetLastError();
 }
 

@@ -144,6 +144,37 @@ struct PFuse {
 hipError_t launch_hessvec_cg_fused(int mat, const BlockArgs& a, const float* u32, const PFuse& pf, float* out,
                                    float* partial, int* done, double* dot_elem, hipStream_t st, hipEvent_t ev0 = nullptr,
                                    hipEvent_t ev1 = nullptr);
+// A multigrid smoothing step folded into the level's next operator apply:
+// every block forms the direction d of each staged row itself and applies K
+// to it; the row's owner (its only block, or the first block of a shared
+// node) stores the node's new vectors.  Reads and writes go to different
+// buffers wherever another block may still read the old value.
+//   MG_STAGE_CHEB: q = the node's complete image of the previous apply (qa
+//     rows and pa partial slots by node_q); r = r_old - q; d = c1 d_old +
+//     c2 M r (c2 M r without has_dold); the owner stores r_new, d_new and
+//     x += d.
+//   MG_STAGE_PROLONG: d = e = P xc at the row (0 at fixed rows); the owner
+//     stores x += e.
+enum MgStageKind { MG_STAGE_CHEB = 2, MG_STAGE_PROLONG = 3 };
+struct MgStage {
+    const float* r_old;
+    float* r_new;
+    const float* d_old;
+    float* d_new;
+    float* x;
+    const float* minv;
+    const float* qa;
+    const float* pa;
+    float c1, c2;
+    int has_dold;
+    const int* p_ptr;
+    const int* p_col;
+    const float* p_val;
+    const float* xc;
+};
+// done: the PCG's flag (the block returns at once when it is set), or null
+hipError_t launch_hessvec_mg(int mat, const BlockArgs& a, const float* u32, int stage, const MgStage& mg, float* out,
+                             float* partial, const int* done, hipStream_t st);
 // alpha, x, r and z (pc: z = M r), r.z and r.r partials (2 x cg_grid); guard
 // 1: stop at p.Kp <= 0
 hipError_t launch_cg_update(int pc, int64_t n3, float* x, float* r, float* z, const float* p, const float* q, const float* minv,
@@ -155,6 +186,11 @@ hipError_t launch_cg_test(int pc, int64_t n3, const float* z, const float* r, fl
 hipError_t launch_cg_update_only(int64_t n3, float* x, float* r, const float* p, const float* q, const int2* node_q,
                                  const float* partial, double* scal, const double* pq_part, int n_pq, double* part,
                                  int* done, int guard, hipStream_t st);
+// the update of the multigrid PCG with the V-cycle's first Chebyshev step on
+// the new residual: alpha, x, r, then z = c2 M r into z and z2 (no partials)
+hipError_t launch_cg_update_mg(int64_t n3, float* x, float* r, float* z, float* z2, const float* p, const float* q,
+                               const float* minv, float c2, const int2* node_q, const float* partial, double* scal,
+                               const double* pq_part, int n_pq, int* done, int guard, hipStream_t st);
 // rz_guard: stop with CG_BREAKDOWN at r.z <= 0; stall_window > 0: stop with
 // CG_STAGNATED when r.r has not fallen below half its last reference value
 // within that many iterations

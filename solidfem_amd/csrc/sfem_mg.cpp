@@ -53,6 +53,9 @@ struct LevelView {
     const float* b;        // right-hand side
     float *x, *r, *d, *q;  // solution, residual, Chebyshev direction, operator image
     double lmax;
+    // the folded schedule: partial slots of q (pa), second residual and
+    // direction, the image of one more apply (qb rows, pb partial slots)
+    float *pa, *r2, *d2, *qb, *pb;
 };
 
 LevelView level(sfem_ctx* f, int l)
@@ -79,14 +82,30 @@ LevelView level(sfem_ctx* f, int l)
         L.q = c->d_q32.as<float>();
         L.lmax = m.lmax;
     }
+    sfem_ctx* c = L.c;
+    L.pa = c->d_partial.as<float>();
+    L.r2 = c->d_mgw_r2.as<float>();
+    L.d2 = c->d_mgw_d2.as<float>();
+    L.qb = c->d_mgw_qb.as<float>();
+    L.pb = c->d_mgw_pb.as<float>();
     return L;
 }
 
-// out = K_l v (element rows and partial rows; operator_row assembles them)
-void matvec(const LevelView& L, const float* v, float* out)
+// the folded schedule's work vectors of one level context
+void alloc_fused(sfem_ctx* c)
+{
+    if (c->d_mgw_r2.p) return;
+    const size_t n3 = 3 * (size_t)c->nn * sizeof(float), ns = 3 * (size_t)c->nslots * sizeof(float);
+    for (DevBuf* b : {&c->d_mgw_r2, &c->d_mgw_d2, &c->d_mgw_qb}) b->alloc(n3);
+    c->d_mgw_pb.alloc(ns);
+}
+
+// out = K_l v (element rows and partial rows; operator_row assembles them);
+// done: the PCG's flag, or a flag that stays 0
+void matvec(const LevelView& L, const float* v, float* out, const int* done)
 {
     sfem_ctx* c = L.c;
-    HIPCK(launch_hessvec_cg(c->material, c->blk, c->d_u32.as<float>(), v, out, c->d_partial.as<float>(), nullptr, nullptr,
+    HIPCK(launch_hessvec_cg(c->material, c->blk, c->d_u32.as<float>(), v, out, c->d_partial.as<float>(), done, nullptr,
                             c->stream, nullptr, nullptr));
 }
 
@@ -105,7 +124,7 @@ Cheb cheb_band(double lmax, double ratio)
 // the current x when r already holds b - K x_old and q = K e for x = x_old + e.
 // first_done: the first step from x = 0 was made by the restriction that
 // produced b.
-void smooth(const LevelView& L, int degree, bool from_zero, const Cheb& ch, bool first_done = false)
+void smooth(const LevelView& L, int degree, bool from_zero, const Cheb& ch, const int* done, bool first_done = false)
 {
     sfem_ctx* c = L.c;
     const int64_t nn = c->nn;
@@ -113,13 +132,13 @@ void smooth(const LevelView& L, int degree, bool from_zero, const Cheb& ch, bool
     const float* part = c->d_partial.as<float>();
     if (!first_done)
         HIPCK(launch_mg_cheb(from_zero ? 0 : 2, nn, L.b, L.r, L.d, L.x, L.q, nq, part, L.minv, 0.f,
-                             (float)(1.0 / ch.theta), c->stream));
+                             (float)(1.0 / ch.theta), done, c->stream));
     double rho = 1.0 / ch.sigma;
     for (int i = 1; i < degree; i++) {
-        matvec(L, L.d, L.q);
+        matvec(L, L.d, L.q, done);
         const double rho1 = 1.0 / (2.0 * ch.sigma - rho);
         HIPCK(launch_mg_cheb(1, nn, L.b, L.r, L.d, L.x, L.q, nq, part, L.minv, (float)(rho1 * rho),
-                             (float)(2.0 * rho1 / ch.delta), c->stream));
+                             (float)(2.0 * rho1 / ch.delta), done, c->stream));
         rho = rho1;
     }
 }
@@ -130,36 +149,153 @@ Cheb level_band(sfem_ctx* f, int l, double lmax)
     return cheb_band(lmax, l == (int)f->mg.size() ? MG_COARSE_RATIO : MG_RATIO);
 }
 
-// first_done: the restriction into this level made its first smoothing step
-void vcycle(sfem_ctx* f, int l, bool first_done = false)
+// One V-cycle from level l, one kernel per step.  first_done: the restriction
+// into this level made its first smoothing step.  done: the PCG's flag, or a
+// flag that stays 0.
+void vcycle(sfem_ctx* f, int l, const int* done, bool first_done = false)
 {
     const LevelView L = level(f, l);
     sfem_ctx* c = L.c;
     const int nlev = (int)f->mg.size();
     if (l == nlev) {
         const MgLevel* m = l > 0 ? f->mg[l - 1].get() : nullptr;
-        if (m && m->dense) HIPCK(launch_mg_dense_solve(3 * (int)c->nn, m->d_Ainv.as<float>(), L.b, L.x, c->stream));
-        else smooth(L, MG_COARSE_DEGREE, true, level_band(f, l, L.lmax), first_done);
+        if (m && m->dense) HIPCK(launch_mg_dense_solve(3 * (int)c->nn, m->d_Ainv.as<float>(), L.b, L.x, done, c->stream));
+        else smooth(L, MG_COARSE_DEGREE, true, level_band(f, l, L.lmax), done, first_done);
         return;
     }
     const Cheb ch = level_band(f, l, L.lmax);
-    smooth(L, MG_DEGREE, true, ch, first_done);
-    matvec(L, L.d, L.q);
+    smooth(L, MG_DEGREE, true, ch, done, first_done);
+    matvec(L, L.d, L.q, done);
     HIPCK(launch_mg_cheb(3, c->nn, L.b, L.r, L.d, L.x, L.q, c->d_node_info.as<int2>(), c->d_partial.as<float>(), L.minv,
-                         0.f, 0.f, c->stream));
+                         0.f, 0.f, done, c->stream));
     MgLevel& m = *f->mg[l];
     const LevelView C = level(f, l + 1);
     // a smoothed coarse level's first step is made by the restriction
     const bool fuse = !(l + 1 == nlev && m.dense);
     const float c2 = fuse ? (float)(1.0 / level_band(f, l + 1, C.lmax).theta) : 0.f;
     HIPCK(launch_mg_restrict(C.c->nn, m.t_ptr.as<int>(), m.t_row.as<int>(), m.t_val.as<float>(), L.r,
-                             C.c->d_fixed.as<uint8_t>(), const_cast<float*>(C.b), c->stream, fuse ? C.minv : nullptr,
+                             C.c->d_fixed.as<uint8_t>(), const_cast<float*>(C.b), done, c->stream, fuse ? C.minv : nullptr,
                              C.r, C.d, C.x, c2));
-    vcycle(f, l + 1, fuse);
+    vcycle(f, l + 1, done, fuse);
     HIPCK(launch_mg_prolong(c->nn, m.p_ptr.as<int>(), m.p_col.as<int>(), m.p_val.as<float>(), C.x,
-                            c->d_fixed.as<uint8_t>(), L.d, L.x, c->stream));
-    matvec(L, L.d, L.q);
-    smooth(L, MG_DEGREE, false, ch);
+                            c->d_fixed.as<uint8_t>(), L.d, L.x, done, c->stream));
+    matvec(L, L.d, L.q, done);
+    smooth(L, MG_DEGREE, false, ch, done);
+}
+
+// The same V-cycle with steps folded into neighbouring kernels, bitwise the
+// same result (every node value is the same FP32 expression):
+//  - on every level the prolongation into it is made by the staging of the
+//    level's next operator apply (MG_STAGE_PROLONG: e = P x_c, x += e, K e);
+//  - on the coarse levels, where each kernel costs about one workgroup's
+//    latency chain, both Chebyshev steps that follow an apply are made by the
+//    staging of the next apply (MG_STAGE_CHEB); on the finest level, whose
+//    element kernel is VALU-bound, the extra staging work costs more than the
+//    launch it saves, and they stay separate kernels;
+//  - the finest level's first step is made by the PCG update
+//    (launch_cg_update_mg), and its last step also writes the PCG's r.z, r.r
+//    partials (rz_part).
+// A smoothed level (degree 2; q_k: the image of the k-th apply in rows and
+// partial slots), on the coarse levels:
+//   [first step: by the restriction into this level]
+//   q1 = K d                                              matvec
+//   r' = b - q1, d' = c1 d + c2 M r', x += d'; q2 = K d'   CHEB stage
+//   r' -= q2                                              k_mg_cheb<3>
+//   b_c = P^T r'                                          restriction (+ the coarse first step)
+//   e = P x_c, x += e; q3 = K e                           PROLONG stage
+//   r'' = r' - q3, d = c2 M r'', x += d; q4 = K d          CHEB stage
+//   r = r'' - q4, d = c1 d + c2 M r, x += d               k_mg_cheb<1>
+void vcycle_fused(sfem_ctx* f, int l, const int* done, bool first_done, double* rz_part)
+{
+    static_assert(MG_DEGREE == 2, "the folded schedule is written for degree-2 smoothing");
+    const LevelView L = level(f, l);
+    sfem_ctx* c = L.c;
+    const int nlev = (int)f->mg.size();
+    const int2* nq = c->d_node_info.as<int2>();
+    if (l == nlev) {
+        const MgLevel* m = l > 0 ? f->mg[l - 1].get() : nullptr;
+        if (m && m->dense) HIPCK(launch_mg_dense_solve(3 * (int)c->nn, m->d_Ainv.as<float>(), L.b, L.x, done, c->stream));
+        else smooth(L, MG_COARSE_DEGREE, true, level_band(f, l, L.lmax), done, first_done);
+        return;
+    }
+    const bool fold = l > 0;
+    const Cheb ch = level_band(f, l, L.lmax);
+    // the coefficients of smooth(): the first step c2 = 1 / theta, the second from rho
+    const double rho = 1.0 / ch.sigma, rho1 = 1.0 / (2.0 * ch.sigma - rho);
+    const float c1 = (float)(rho1 * rho), c2 = (float)(2.0 * rho1 / ch.delta), c2first = (float)(1.0 / ch.theta);
+    const float* u32 = c->d_u32.as<float>();
+    if (!first_done)
+        HIPCK(launch_mg_cheb(0, c->nn, L.b, L.r, L.d, L.x, L.q, nq, L.pa, L.minv, 0.f, c2first, done, c->stream));
+    matvec(L, L.d, L.q, done);
+    // q2: the image of the pre-smoothing's second direction
+    const float* q2 = L.q;
+    const float* p2 = L.pa;
+    if (fold) {
+        MgStage st{};
+        st.r_old = L.b;
+        st.r_new = L.r;
+        st.d_old = L.d;
+        st.d_new = L.d2;
+        st.x = L.x;
+        st.minv = L.minv;
+        st.qa = L.q;
+        st.pa = L.pa;
+        st.has_dold = 1;
+        st.c1 = c1;
+        st.c2 = c2;
+        HIPCK(launch_hessvec_mg(c->material, c->blk, u32, MG_STAGE_CHEB, st, L.qb, L.pb, done, c->stream));
+        q2 = L.qb;
+        p2 = L.pb;
+    } else {
+        // r from b: the first step may have been the PCG update's, which does
+        // not copy b to r
+        HIPCK(launch_mg_cheb(4, c->nn, L.b, L.r, L.d, L.x, L.q, nq, L.pa, L.minv, c1, c2, done, c->stream));
+        matvec(L, L.d, L.q, done);
+    }
+    HIPCK(launch_mg_cheb(3, c->nn, L.b, L.r, L.d, L.x, q2, nq, p2, L.minv, 0.f, 0.f, done, c->stream));
+    MgLevel& m = *f->mg[l];
+    const LevelView C = level(f, l + 1);
+    const bool cfirst = !(l + 1 == nlev && m.dense);
+    const float cc2 = cfirst ? (float)(1.0 / level_band(f, l + 1, C.lmax).theta) : 0.f;
+    HIPCK(launch_mg_restrict(C.c->nn, m.t_ptr.as<int>(), m.t_row.as<int>(), m.t_val.as<float>(), L.r,
+                             C.c->d_fixed.as<uint8_t>(), const_cast<float*>(C.b), done, c->stream,
+                             cfirst ? C.minv : nullptr, C.r, C.d, C.x, cc2));
+    vcycle_fused(f, l + 1, done, cfirst, nullptr);
+    MgStage pr{};
+    pr.x = L.x;
+    pr.p_ptr = m.p_ptr.as<int>();
+    pr.p_col = m.p_col.as<int>();
+    pr.p_val = m.p_val.as<float>();
+    pr.xc = C.x;
+    HIPCK(launch_hessvec_mg(c->material, c->blk, u32, MG_STAGE_PROLONG, pr, L.q, L.pa, done, c->stream));
+    float* rl = L.r;
+    float* q4 = L.q;
+    float* p4 = L.pa;
+    if (fold) {
+        MgStage s1{};
+        s1.r_old = L.r;
+        s1.r_new = L.r2;
+        s1.d_old = L.d2;   // not read (a first step); no block writes it here
+        s1.d_new = L.d;
+        s1.x = L.x;
+        s1.minv = L.minv;
+        s1.qa = L.q;
+        s1.pa = L.pa;
+        s1.has_dold = 0;
+        s1.c1 = 0.f;
+        s1.c2 = c2first;
+        HIPCK(launch_hessvec_mg(c->material, c->blk, u32, MG_STAGE_CHEB, s1, L.qb, L.pb, done, c->stream));
+        rl = L.r2;
+        q4 = L.qb;
+        p4 = L.pb;
+    } else {
+        HIPCK(launch_mg_cheb(2, c->nn, L.b, L.r, L.d, L.x, L.q, nq, L.pa, L.minv, 0.f, c2first, done, c->stream));
+        matvec(L, L.d, L.q, done);
+    }
+    if (rz_part)
+        HIPCK(launch_mg_cheb_rz(c->nn, L.b, rl, L.d, L.x, q4, nq, p4, L.minv, c1, c2, rz_part, done, c->stream));
+    else
+        HIPCK(launch_mg_cheb(1, c->nn, L.b, rl, L.d, L.x, q4, nq, p4, L.minv, c1, c2, done, c->stream));
 }
 
 // element evaluations of one V-cycle (matvecs per level x its elements)
@@ -191,6 +327,10 @@ bool mg_prepare(sfem_ctx* f, const float* minv0, bool full)
         f->d_mg_part.alloc(1024 * sizeof(double));
     }
     const int nlev = (int)f->mg.size();
+    if (f->mg_fused) {
+        alloc_fused(f);
+        for (auto& m : f->mg) alloc_fused(m->c);
+    }
     if (!f->d_mg_norm.p || f->d_mg_norm.n < 2 * (size_t)(nlev + 1) * sizeof(double))
         f->d_mg_norm.alloc(2 * (size_t)(nlev + 1) * sizeof(double));
     f->mg_minv0 = minv0;
@@ -246,7 +386,7 @@ bool mg_prepare(sfem_ctx* f, const float* minv0, bool full)
         float* y = L.r;
         HIPCK(launch_mg_start(c->nn, c->d_fixed.as<uint8_t>(), v, s));
         for (int it = 0; it < MG_POWER_ITERS; it++) {
-            matvec(L, v, L.q);
+            matvec(L, v, L.q, c->blk.zero_flag);
             HIPCK(launch_mg_power(c->nn, L.q, c->d_node_info.as<int2>(), c->d_partial.as<float>(), L.minv, y, s));
             std::swap(v, y);
         }
@@ -275,12 +415,19 @@ bool mg_prepare(sfem_ctx* f, const float* minv0, bool full)
     return true;
 }
 
-void mg_vcycle(sfem_ctx* f) { vcycle(f, 0); }
+void mg_vcycle(sfem_ctx* f)
+{
+    if (f->mg_fused) vcycle_fused(f, 0, f->blk.zero_flag, false, nullptr);
+    else vcycle(f, 0, f->blk.zero_flag);
+}
 
 // PCG with one V-cycle per iteration: the element kernel (q = K p, p.q
 // partials), the CG update without z, z = V(r), r.z partials, then beta,
-// the convergence test and p.  The host reads the done flag every iteration:
-// an iteration past convergence would cost a whole V-cycle.
+// the convergence test and p.  The host keeps one iteration enqueued beyond
+// the done flag it waits for (a copy of the flag into pinned memory and an
+// event after every iteration), so the device never waits for the host; the
+// iteration enqueued past the end of a solve finds the flag set and every
+// kernel of it returns at once.
 CgResult mg_pcg(sfem_ctx* c, int maxit, double tol2, int64_t* hv_evals)
 {
     maxit = std::min(maxit, MG_MAX_ITERS);
@@ -290,38 +437,73 @@ CgResult mg_pcg(sfem_ctx* c, int maxit, double tol2, int64_t* hv_evals)
     double* parts = c->d_partials.as<double>();
     double* pq_elem = parts + 2 * CG_GRID;
     int* done = c->d_done.as<int>();
+    const int* zero = c->blk.zero_flag;
     float* x = c->d_x32.as<float>();
     float* r = c->d_r32.as<float>();
     float* z = c->d_z32.as<float>();
     float* p = c->d_p32.as<float>();
     float* q = c->d_q32.as<float>();
+    if (!c->h_flag) {
+        HIPCK(hipHostMalloc((void**)&c->h_flag, 2 * sizeof(int), hipHostMallocDefault));
+        for (hipEvent_t& e : c->flag_ev) HIPCK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    }
     HIPCK(launch_cg_init(0, n3, c->d_b32.as<float>(), x, r, z, p, c->d_minv.as<float>(), c->d_fixed.as<uint8_t>(), scal,
                          parts, done, s));
-    mg_vcycle(c);
-    HIPCK(launch_mg_rz(c->nn, r, z, parts, s));
+    // the folded schedule: the update makes the V-cycle's first step and the
+    // V-cycle's last step the r.z partials.  The first V-cycle runs whatever
+    // the flag says (its partials set the flag again).
+    const bool fused = c->mg_fused;
+    const float c2first = (float)(1.0 / level_band(c, 0, c->mg_lmax0).theta);
+    if (fused) {
+        vcycle_fused(c, 0, zero, false, parts);
+    } else {
+        vcycle(c, 0, zero);
+        HIPCK(launch_mg_rz(c->nn, r, z, parts, zero, s));
+    }
     HIPCK(launch_cg_scalars(parts, cg_grid(c->nn), scal, done, s));
     HIPCK(launch_mg_copy(n3, z, p, s));
-    int dflag = 0, launched = 0;
-    while (launched < maxit) {
-        HIPCK(hipMemcpyAsync(&dflag, done, sizeof(int), hipMemcpyDeviceToHost, s));
-        HIPCK(hipStreamSynchronize(s));
-        if (dflag) break;
+    auto iteration = [&]() {
         HIPCK(launch_hessvec_cg(c->material, c->blk, c->d_u32.as<float>(), p, q, c->d_partial.as<float>(), done, pq_elem, s,
                                 nullptr, nullptr));
-        HIPCK(launch_cg_update_only(n3, x, r, p, q, c->d_node_info.as<int2>(), c->d_partial.as<float>(), scal, pq_elem,
-                                    c->nblk, parts, done, 1, s));
-        mg_vcycle(c);
-        HIPCK(launch_mg_rz(c->nn, r, z, parts, s));
+        if (fused) {
+            HIPCK(launch_cg_update_mg(n3, x, r, z, c->d_mg_d.as<float>(), p, q, c->mg_minv0, c2first,
+                                      c->d_node_info.as<int2>(), c->d_partial.as<float>(), scal, pq_elem, c->nblk, done, 1,
+                                      s));
+            vcycle_fused(c, 0, done, true, parts);
+        } else {
+            HIPCK(launch_cg_update_only(n3, x, r, p, q, c->d_node_info.as<int2>(), c->d_partial.as<float>(), scal, pq_elem,
+                                        c->nblk, parts, done, 1, s));
+            vcycle(c, 0, done);
+            HIPCK(launch_mg_rz(c->nn, r, z, parts, done, s));
+        }
         // the test is on the true residual norm (r.r); a V-cycle that is not
         // positive definite at this state (tangents of near-incompressible
         // material at strongly deformed Newton iterates) shows as r.z <= 0 or
         // stagnation, and the Newton driver falls back to block Jacobi
         HIPCK(launch_cg_p_only(n3, z, r, p, scal, parts, done, 1, tol2, 1, MG_STALL_WINDOW, s));
-        launched++;
+    };
+    // check k: the flag after k iterations, into slot k & 1 (check k + 2 is
+    // recorded only after check k has been read)
+    auto check = [&](int k) {
+        HIPCK(hipMemcpyAsync(c->h_flag + (k & 1), done, sizeof(int), hipMemcpyDeviceToHost, s));
+        HIPCK(hipEventRecord(c->flag_ev[k & 1], s));
+    };
+    int dflag = 0, launched = 0;
+    check(0);
+    if (maxit > 0) {
+        iteration();
+        launched = 1;
+        check(1);
     }
-    if (!dflag) {
-        HIPCK(hipMemcpyAsync(&dflag, done, sizeof(int), hipMemcpyDeviceToHost, s));
-        HIPCK(hipStreamSynchronize(s));
+    for (int k = 0;; k++) {
+        HIPCK(hipEventSynchronize(c->flag_ev[k & 1]));
+        dflag = c->h_flag[k & 1];
+        if (dflag || k >= launched) break;
+        if (launched < maxit) {
+            iteration();
+            launched++;
+            check(launched);
+        }
     }
     double hs[S_COUNT];
     HIPCK(hipMemcpy(hs, scal, sizeof hs, hipMemcpyDeviceToHost));
@@ -441,6 +623,14 @@ int sfem_mg_clear(sfem_ctx* f)
         f->mg_ready = false;
         return SFEM_OK;
     });
+}
+
+int sfem_mg_set_schedule(sfem_ctx* f, int fused)
+{
+    if (!f) return fail(SFEM_EINVAL, "sfem_mg_set_schedule: null context");
+    if (fused != 0 && fused != 1) return fail(SFEM_EINVAL, "sfem_mg_set_schedule: fused must be 0 or 1");
+    f->mg_fused = fused != 0;
+    return SFEM_OK;
 }
 
 int sfem_mg_info(sfem_ctx* f, int32_t* nlevels, double* lmax)

@@ -15,6 +15,7 @@
 #include "sfem_device.hpp"
 #include "sfem_kernels.hpp"
 #include "sfem_mg.hpp"
+#include "sfem_mgdev.hpp"
 #include "sfem_reduce.hpp"
 
 namespace sfem {
@@ -23,54 +24,24 @@ constexpr int MG_BLOCK = 256;
 
 static int mg_grid(int64_t n) { return (int)((n + MG_BLOCK - 1) / MG_BLOCK); }
 
-// node_q: {first slot, end slot} of a shared node; {-1, 0} node of one block;
-// {-2, 0} fixed node or node of no element
-__device__ __forceinline__ void operator_row(int64_t n, const float* __restrict__ q, const int2* __restrict__ node_q,
-                                             const float* __restrict__ partial, float (&qv)[3])
-{
-    const int2 nq = node_q[n];
-#pragma unroll
-    for (int c = 0; c < 3; c++) qv[c] = q[3 * n + c];
-#pragma unroll
-    for (int c = 0; c < 3; c++) qv[c] = nq.x == -1 ? qv[c] : 0.f;
-    if (nq.x >= 0) {
-        const int k1 = nq.y;
-        for (int k = nq.x; k < k1; k += 4) {
-            float t[4][3];
-#pragma unroll
-            for (int j = 0; j < 4; j++)
-#pragma unroll
-                for (int c = 0; c < 3; c++) t[j][c] = k + j < k1 ? partial[3 * (size_t)(k + j) + c] : 0.f;
-#pragma unroll
-            for (int j = 0; j < 4; j++)
-#pragma unroll
-                for (int c = 0; c < 3; c++) qv[c] += t[j][c];
-        }
-    }
-}
-
-__device__ __forceinline__ void block_apply(const float* __restrict__ minv, int64_t n, const float (&r)[3], float (&z)[3])
-{
-    float M[9];
-#pragma unroll
-    for (int k = 0; k < 9; k++) M[k] = minv[9 * n + k];
-#pragma unroll
-    for (int c = 0; c < 3; c++) z[c] = M[3 * c] * r[0] + M[3 * c + 1] * r[1] + M[3 * c + 2] * r[2];
-}
-
 // One Chebyshev step (q = the level operator applied to the previous d, or to
 // the coarse correction e for MODE 2):
 //   MODE 0: r = b;      d = c2 M r;         x = d        first step from x = 0
 //   MODE 1: r -= q;     d = c1 d + c2 M r;  x += d       next step
 //   MODE 2: r -= q;     d = c2 M r;         x += d       first step from x (after x += e)
 //   MODE 3: r -= q                                       residual after the last step
+//   MODE 4: r = b - q;  d = c1 d + c2 M r;  x += d       second step, r not yet copied from b
+// done: the PCG's flag (a flag that stays 0 outside PCG), read with the first
+// loads; once the solve has stopped, the V-cycles enqueued after it return at
+// once (every multigrid kernel tests it before its first store)
 template <int MODE>
 __global__ __launch_bounds__(MG_BLOCK) void k_mg_cheb(int64_t nn, const float* __restrict__ b, float* __restrict__ r,
                                                       float* __restrict__ d, float* __restrict__ x,
                                                       const float* __restrict__ q, const int2* __restrict__ node_q,
                                                       const float* __restrict__ partial, const float* __restrict__ minv,
-                                                      float c1, float c2)
+                                                      float c1, float c2, const int* __restrict__ done)
 {
+    const int stop = vload(done);
     const int64_t i = blockIdx.x * (int64_t)MG_BLOCK + threadIdx.x;
     if (i >= nn) return;
     float rv[3];
@@ -80,9 +51,11 @@ __global__ __launch_bounds__(MG_BLOCK) void k_mg_cheb(int64_t nn, const float* _
     } else {
         float qv[3];
         operator_row(i, q, node_q, partial, qv);
+        const float* rin = MODE == 4 ? b : r;
 #pragma unroll
-        for (int c = 0; c < 3; c++) rv[c] = r[3 * i + c] - qv[c];
+        for (int c = 0; c < 3; c++) rv[c] = rin[3 * i + c] - qv[c];
     }
+    if (stop) return;
 #pragma unroll
     for (int c = 0; c < 3; c++) r[3 * i + c] = rv[c];
     if (MODE == 3) return;
@@ -90,47 +63,63 @@ __global__ __launch_bounds__(MG_BLOCK) void k_mg_cheb(int64_t nn, const float* _
     block_apply(minv, i, rv, mr);
 #pragma unroll
     for (int c = 0; c < 3; c++) {
-        const float dv = MODE == 1 ? c1 * d[3 * i + c] + c2 * mr[c] : c2 * mr[c];
+        constexpr bool next = MODE == 1 || MODE == 4;
+        const float dv = cheb_dir(next, c1, next ? d[3 * i + c] : 0.f, c2, mr[c]);
         d[3 * i + c] = dv;
         x[3 * i + c] = MODE == 0 ? dv : x[3 * i + c] + dv;
     }
 }
 
-// acc += sum over entries j0 .. j1-1 (in order) of val[j] * v[idx[j]] (3-vectors);
-// eight entries' index loads, then their row loads, are in flight at once
-__device__ __forceinline__ void gather_rows(int j0, int j1, const int* __restrict__ idx, const float* __restrict__ val,
-                                            const float* __restrict__ v, float (&acc)[3])
+// The finest level's last Chebyshev step (k_mg_cheb<1>) and the PCG's r.z and
+// r.r partials of its result z = x against b (the PCG residual), in the CG
+// update's layout: CG_BLOCK threads, cg_grid(nn) workgroups, grid-stride
+// (k_mg_rz's order)
+__global__ __launch_bounds__(CG_BLOCK) void k_mg_cheb_rz(int64_t nn, const float* __restrict__ b, float* __restrict__ r,
+                                                         float* __restrict__ d, float* __restrict__ x,
+                                                         const float* __restrict__ q, const int2* __restrict__ node_q,
+                                                         const float* __restrict__ partial, const float* __restrict__ minv,
+                                                         float c1, float c2, double* __restrict__ part,
+                                                         const int* __restrict__ done)
 {
-    for (int j = j0; j < j1; j += 8) {
-        int id[8];
-        float w[8], t[8][3];
+    const int stop = vload(done);
+    double v[2] = {0.0, 0.0};
+    for (int64_t i = blockIdx.x * (int64_t)CG_BLOCK + threadIdx.x; i < nn; i += (int64_t)gridDim.x * CG_BLOCK) {
+        float qv[3], rv[3], mr[3];
+        operator_row(i, q, node_q, partial, qv);
+        if (stop) break;
 #pragma unroll
-        for (int q = 0; q < 8; q++) {
-            const bool ok = j + q < j1;
-            id[q] = ok ? idx[j + q] : -1;
-            w[q] = ok ? val[j + q] : 0.f;
+        for (int c = 0; c < 3; c++) {
+            rv[c] = r[3 * i + c] - qv[c];
+            r[3 * i + c] = rv[c];
         }
+        block_apply(minv, i, rv, mr);
 #pragma unroll
-        for (int q = 0; q < 8; q++)
-#pragma unroll
-            for (int c = 0; c < 3; c++) t[q][c] = id[q] >= 0 ? v[3 * (size_t)id[q] + c] : 0.f;
-#pragma unroll
-        for (int q = 0; q < 8; q++)
-#pragma unroll
-            for (int c = 0; c < 3; c++) acc[c] += w[q] * t[q][c];
+        for (int c = 0; c < 3; c++) {
+            const float dv = cheb_dir(true, c1, d[3 * i + c], c2, mr[c]);
+            d[3 * i + c] = dv;
+            const float xv = x[3 * i + c] + dv;
+            x[3 * i + c] = xv;
+            const double rb = b[3 * i + c];
+            v[0] = fma(rb, (double)xv, v[0]);
+            v[1] = fma(rb, rb, v[1]);
+        }
     }
+    if (stop) return;
+    block_partials<2, CG_BLOCK>(v, part);
 }
 
 // e = P xc on the level above (0 at its fixed nodes); x += e
 __global__ __launch_bounds__(MG_BLOCK) void k_mg_prolong(int64_t nn, const int* __restrict__ ptr, const int* __restrict__ col,
                                                          const float* __restrict__ val, const float* __restrict__ xc,
                                                          const uint8_t* __restrict__ fixed, float* __restrict__ e,
-                                                         float* __restrict__ x)
+                                                         float* __restrict__ x, const int* __restrict__ done)
 {
+    const int stop = vload(done);
     const int64_t i = blockIdx.x * (int64_t)MG_BLOCK + threadIdx.x;
     if (i >= nn) return;
     float ev[3] = {0.f, 0.f, 0.f};
     gather_rows(ptr[i], ptr[i + 1], col, val, xc, ev);
+    if (stop) return;
     const bool f = fixed[i] != 0;
 #pragma unroll
     for (int c = 0; c < 3; c++) {
@@ -150,8 +139,10 @@ __global__ __launch_bounds__(MG_BLOCK) void k_mg_restrict(int64_t nnc, const int
                                                           const float* __restrict__ val, const float* __restrict__ r,
                                                           const uint8_t* __restrict__ fixed, float* __restrict__ bc,
                                                           const float* __restrict__ minv, float* __restrict__ rc,
-                                                          float* __restrict__ dc, float* __restrict__ xc, float c2)
+                                                          float* __restrict__ dc, float* __restrict__ xc, float c2,
+                                                          const int* __restrict__ done)
 {
+    const int stop = vload(done);
     const int64_t t = blockIdx.x * (int64_t)MG_BLOCK + threadIdx.x;
     const int64_t k = t >> 2;
     const int lane = (int)(t & 3);
@@ -162,6 +153,7 @@ __global__ __launch_bounds__(MG_BLOCK) void k_mg_restrict(int64_t nnc, const int
         const int a = p0 + lane * cs, e = a + cs < p1 ? a + cs : p1;
         gather_rows(a, e, row, val, r, s);
     }
+    if (stop) return;
 #pragma unroll
     for (int c = 0; c < 3; c++) {
         s[c] += __shfl_xor(s[c], 1, 64);
@@ -180,7 +172,7 @@ __global__ __launch_bounds__(MG_BLOCK) void k_mg_restrict(int64_t nnc, const int
             block_apply(minv, k, rv, mr);
 #pragma unroll
             for (int c = 0; c < 3; c++) {
-                const float dv = c2 * mr[c];
+                const float dv = cheb_dir(false, 0.f, 0.f, c2, mr[c]);
                 rc[3 * k + c] = rv[c];
                 dc[3 * k + c] = dv;
                 xc[3 * k + c] = dv;
@@ -254,15 +246,16 @@ __global__ __launch_bounds__(256) void k_mg_finish(const double* __restrict__ pa
 
 // r.z and r.r partials in the CG update's layout (2 x gridDim.x)
 __global__ __launch_bounds__(CG_BLOCK) void k_mg_rz(int64_t nn, const float* __restrict__ r, const float* __restrict__ z,
-                                                    double* __restrict__ part)
+                                                    double* __restrict__ part, const int* __restrict__ done)
 {
+    if (vload(done)) return;
     double v[2] = {0.0, 0.0};
     for (int64_t i = blockIdx.x * (int64_t)CG_BLOCK + threadIdx.x; i < nn; i += (int64_t)gridDim.x * CG_BLOCK) {
 #pragma unroll
         for (int c = 0; c < 3; c++) {
             const double rv = r[3 * i + c];
-            v[0] += rv * z[3 * i + c];
-            v[1] += rv * rv;
+            v[0] = fma(rv, (double)z[3 * i + c], v[0]);
+            v[1] = fma(rv, rv, v[1]);
         }
     }
     block_partials<2, CG_BLOCK>(v, part);
@@ -526,12 +519,15 @@ __global__ __launch_bounds__(MG_BLOCK) void k_mg_tiles(int n, int p, int bp, MgT
 // x = Ainv b: one workgroup per row, each thread a strided partial sum, then
 // a fixed-order tree
 __global__ __launch_bounds__(MG_BLOCK) void k_mg_dense_solve(int n, const float* __restrict__ Ainv,
-                                                             const float* __restrict__ b, float* __restrict__ x)
+                                                             const float* __restrict__ b, float* __restrict__ x,
+                                                             const int* __restrict__ done)
 {
     __shared__ float red[MG_BLOCK / 64];
+    const int stop = vload(done);
     const int i = blockIdx.x, tid = threadIdx.x;
     float s = 0.f;
     for (int j = tid; j < n; j += MG_BLOCK) s = fmaf(Ainv[(int64_t)i * n + j], b[j], s);
+    if (stop) return;
     for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
     if ((tid & 63) == 0) red[tid >> 6] = s;
     __syncthreads();
@@ -546,31 +542,42 @@ __global__ __launch_bounds__(MG_BLOCK) void k_mg_dense_solve(int n, const float*
 // ------------------------------------------------------------------ launchers
 
 hipError_t launch_mg_cheb(int mode, int64_t nn, const float* b, float* r, float* d, float* x, const float* q,
-                          const int2* node_q, const float* partial, const float* minv, float c1, float c2, hipStream_t st)
+                          const int2* node_q, const float* partial, const float* minv, float c1, float c2, const int* done,
+                          hipStream_t st)
 {
     const dim3 g(mg_grid(nn)), bl(MG_BLOCK);
     switch (mode) {
-    case 0: hipLaunchKernelGGL(k_mg_cheb<0>, g, bl, 0, st, nn, b, r, d, x, q, node_q, partial, minv, c1, c2); break;
-    case 1: hipLaunchKernelGGL(k_mg_cheb<1>, g, bl, 0, st, nn, b, r, d, x, q, node_q, partial, minv, c1, c2); break;
-    case 2: hipLaunchKernelGGL(k_mg_cheb<2>, g, bl, 0, st, nn, b, r, d, x, q, node_q, partial, minv, c1, c2); break;
-    default: hipLaunchKernelGGL(k_mg_cheb<3>, g, bl, 0, st, nn, b, r, d, x, q, node_q, partial, minv, c1, c2); break;
+    case 0: hipLaunchKernelGGL(k_mg_cheb<0>, g, bl, 0, st, nn, b, r, d, x, q, node_q, partial, minv, c1, c2, done); break;
+    case 1: hipLaunchKernelGGL(k_mg_cheb<1>, g, bl, 0, st, nn, b, r, d, x, q, node_q, partial, minv, c1, c2, done); break;
+    case 2: hipLaunchKernelGGL(k_mg_cheb<2>, g, bl, 0, st, nn, b, r, d, x, q, node_q, partial, minv, c1, c2, done); break;
+    case 4: hipLaunchKernelGGL(k_mg_cheb<4>, g, bl, 0, st, nn, b, r, d, x, q, node_q, partial, minv, c1, c2, done); break;
+    default: hipLaunchKernelGGL(k_mg_cheb<3>, g, bl, 0, st, nn, b, r, d, x, q, node_q, partial, minv, c1, c2, done); break;
     }
     return hipGetLastError();
 }
 
 hipError_t launch_mg_prolong(int64_t nn, const int* ptr, const int* col, const float* val, const float* xc,
-                             const uint8_t* fixed, float* e, float* x, hipStream_t st)
+                             const uint8_t* fixed, float* e, float* x, const int* done, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_mg_prolong, dim3(mg_grid(nn)), dim3(MG_BLOCK), 0, st, nn, ptr, col, val, xc, fixed, e, x);
+    hipLaunchKernelGGL(k_mg_prolong, dim3(mg_grid(nn)), dim3(MG_BLOCK), 0, st, nn, ptr, col, val, xc, fixed, e, x, done);
     return hipGetLastError();
 }
 
 hipError_t launch_mg_restrict(int64_t nnc, const int* ptr, const int* row, const float* val, const float* r,
-                              const uint8_t* fixed, float* bc, hipStream_t st, const float* minv, float* rc, float* dc,
-                              float* xc, float c2)
+                              const uint8_t* fixed, float* bc, const int* done, hipStream_t st, const float* minv, float* rc,
+                              float* dc, float* xc, float c2)
 {
     hipLaunchKernelGGL(k_mg_restrict, dim3(mg_grid(4 * nnc)), dim3(MG_BLOCK), 0, st, nnc, ptr, row, val, r, fixed, bc, minv,
-                       rc, dc, xc, c2);
+                       rc, dc, xc, c2, done);
+    return hipGetLastError();
+}
+
+hipError_t launch_mg_cheb_rz(int64_t nn, const float* b, float* r, float* d, float* x, const float* q, const int2* node_q,
+                             const float* partial, const float* minv, float c1, float c2, double* part, const int* done,
+                             hipStream_t st)
+{
+    hipLaunchKernelGGL(k_mg_cheb_rz, dim3(cg_grid(nn)), dim3(CG_BLOCK), 0, st, nn, b, r, d, x, q, node_q, partial, minv, c1,
+                       c2, part, done);
     return hipGetLastError();
 }
 
@@ -607,9 +614,9 @@ hipError_t launch_mg_sqnorm(int64_t n3, const float* a, double* part, double* ou
     return hipGetLastError();
 }
 
-hipError_t launch_mg_rz(int64_t nn, const float* r, const float* z, double* part, hipStream_t st)
+hipError_t launch_mg_rz(int64_t nn, const float* r, const float* z, double* part, const int* done, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_mg_rz, dim3(cg_grid(nn)), dim3(CG_BLOCK), 0, st, nn, r, z, part);
+    hipLaunchKernelGGL(k_mg_rz, dim3(cg_grid(nn)), dim3(CG_BLOCK), 0, st, nn, r, z, part, done);
     return hipGetLastError();
 }
 
@@ -659,10 +666,10 @@ hipError_t launch_mg_dense_inverse(int n, const float* A, float* Ainv, double* W
     return hipGetLastError();
 }
 
-hipError_t launch_mg_dense_solve(int n, const float* Ainv, const float* b, float* x, hipStream_t st)
+hipError_t launch_mg_dense_solve(int n, const float* Ainv, const float* b, float* x, const int* done, hipStream_t st)
 {
     if (n <= 0 || n > MG_DENSE_MAX) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_mg_dense_solve, dim3(n), dim3(MG_BLOCK), 0, st, n, Ainv, b, x);
+    hipLaunchKernelGGL(k_mg_dense_solve, dim3(n), dim3(MG_BLOCK), 0, st, n, Ainv, b, x, done);
     return hipGetLastError();
 }
 
