@@ -66,6 +66,38 @@ def test_vcycle_symmetric_positive_deterministic():
         assert np.vdot(r, s.mg_apply(r)) > 0
 
 
+@pytest.mark.parametrize("grid", [(32, 8, 8), (64, 16, 16)])
+def test_vcycle_matches_fp64_model(grid):
+    """the device V-cycle at rest (every level's tangent is linear elasticity)
+    against the FP64 model of oracle/mg_model.py given the device's
+    eigenvalue estimates and its directly solved coarsest operator (assembled
+    from FP32 element matrices: its inverse amplifies their rounding by the
+    coarse condition number, ~1e5, which an exact coarse inverse would show as
+    a 1e-3 difference): equal to FP32 accuracy, on one and two smoothed levels
+    above the 16x4x4 level"""
+    import torch
+
+    from oracle import mg_model as MM
+    print(f"case {grid} V-cycle vs model", flush=True)
+    m, fixed, f, s = beam(grid)
+    s.set_positions(m.X)
+    rng = np.random.default_rng(4)
+    r = rng.standard_normal(m.X.shape) * (1 - fixed)[:, None]
+    z = s.mg_apply(r)
+    lm = s.mg_info()
+    _, ainv = s.mg_coarse_dense()
+    mu, lam = lame(1e6, 0.45)
+    H = MM.Hierarchy(m, fixed, beam_hierarchy(m), mu, lam)
+    fc = H.free[-1]
+    H.Ac_inv = torch.tensor(ainv[np.ix_(fc, fc)])
+    zm = H.vcycle(torch.tensor(r.reshape(-1)[H.free[0]])[:, None], H.jacobi, list(lm[:-1]))
+    zd = z.reshape(-1)[H.free[0]]
+    err = np.linalg.norm(zd - zm.numpy().ravel()) / np.linalg.norm(zm.numpy())
+    print(f"levels {len(lm)} lmax {lm} rel err {err:.2e}", flush=True)
+    assert len(lm) == H.nlev and lm[-1] == 0
+    assert err < 1e-6
+
+
 # multigrid-PCG iterations of the two load steps of the 32x8x8 nu = 0.495
 # jittered beam, about twice the measured counts (2,123 with block-Jacobi
 # smoothing, 1,984 with the network as the finest smoother)
