@@ -102,10 +102,11 @@ typedef struct {
                                   (FemConfig::mAbsNewtonRsidualThreshold); 0 = off */
     int32_t predictor;         /* sfem_step, ST_QUASI_STATIC, CONVERGED only: 1 = start each
                                   load step from the secant extrapolation of the last two
-                                  states (dropped if it inverts an element; the step is
-                                  redone from the last state if an inner solve breaks down
-                                  there or that solve fails); 0 = from the last state, as
-                                  Step does */
+                                  states, 2 = from the quadratic one through the last three
+                                  (the secant while there are two); dropped if it inverts an
+                                  element; the step is redone from the last state if an inner
+                                  solve breaks down there or that solve fails; 0 = from the
+                                  last state, as Step does */
     double time_step;          /* sfem_step, ST_EXPLICIT / ST_IMPLICIT: Step(dt)'s dt */
     int32_t substeps;          /* sfem_step, ST_EXPLICIT / ST_IMPLICIT: FemConfig::mNumSubsteps (>= 1) */
     int32_t pc_newton_cap;     /* CONVERGED, learned / multigrid preconditioners: Newton steps the

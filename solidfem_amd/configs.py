@@ -2,7 +2,7 @@
 generator, and a runner that solves one on the device and reports iterations,
 timings and FP64 residuals.
 
-    python -m solidfem_amd.configs 1 [jacobi|multigrid|multigrid_learned|learned|none] [nopredictor] [polar]
+    python -m solidfem_amd.configs 1 [jacobi|multigrid|multigrid_learned|learned|none] [nopredictor|secant] [polar]
 
 (wall times are measured after the same solve on a 16x4x4 grid, which loads
 every kernel's code object first)
@@ -93,7 +93,7 @@ PRECONDS = {"none": capi.PC_NONE, "jacobi": capi.PC_JACOBI, "learned": capi.PC_L
             "multigrid": capi.PC_MULTIGRID, "multigrid_learned": capi.PC_MULTIGRID_LEARNED}
 
 
-def run(cfg, precond=capi.PC_JACOBI, tol=1e-8, grid=None, max_newton=50, log=None, predictor=1, material=None):
+def run(cfg, precond=capi.PC_JACOBI, tol=1e-8, grid=None, max_newton=50, log=None, predictor=2, material=None):
     """Solves workload `cfg`; `log(str)` (optional) gets one line per load step.
     The multigrid preconditioners get the beam's nested hierarchy, the learned
     ones the shipped weights."""
@@ -135,7 +135,7 @@ def run(cfg, precond=capi.PC_JACOBI, tol=1e-8, grid=None, max_newton=50, log=Non
 if __name__ == "__main__":
     cfg = int(sys.argv[1])
     pc = sys.argv[2] if len(sys.argv) > 2 else "jacobi"
-    pred = 0 if "nopredictor" in sys.argv[3:] else 1
+    pred = 0 if "nopredictor" in sys.argv[3:] else (1 if "secant" in sys.argv[3:] else 2)
     mat = capi.COROTATIONAL_POLAR if "polar" in sys.argv[3:] else None
     if cfg != 1:
         # the same solve on a 16x4x4 grid first: the first launch of each

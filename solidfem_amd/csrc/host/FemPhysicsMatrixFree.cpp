@@ -544,7 +544,7 @@ sfem_solve_params FemPhysicsMatrixFree::Params() const
     p.check_every = mConfig.mCheckEvery;
     p.sim_type = mSimType;
     p.force_step = mForceStep;
-    p.predictor = !reference && mConfig.mPredictor;
+    p.predictor = reference ? 0 : mConfig.mPredictor;
     return p;
 }
 

@@ -328,7 +328,9 @@ public:
     //   SolveNewtonCG.  NST_NEWTON / NST_NEWTON_LS (assembled direct solves in
     //   the reference) run the converged matrix-free Newton-PCG without / with
     //   line search.  mPredictor (converged modes, quasi-static steps) starts
-    //   each load step from the secant extrapolation of the last two states.
+    //   each load step from an extrapolation of the last states: 2 quadratic
+    //   through the last three (the secant one if it fails), 1 the secant of
+    //   the last two, 0 none (the last state, as Step does).
     struct Config {
         NodalForcesType mNodalForcesComputation = NFT_DIRECT;
         NonlinearSolverType mSolver = NST_NEWTON;
@@ -338,7 +340,7 @@ public:
         real mCGTolerance = 0.1;
         Preconditioner mPreconditioner = PC_JACOBI;
         int mCheckEvery = 16;
-        bool mPredictor = true;
+        int mPredictor = 2;
     };
 
     FemPhysicsMatrixFree(const std::vector<Tet>& tetrahedra, const std::vector<Node>& allNodes, const FemConfig& config);

@@ -546,7 +546,7 @@ private:
         if (config.contains("rel_tol")) mCfg.mTolerance = py::cast<real>(config["rel_tol"]);
         if (config.contains("cg_tol")) mCfg.mCGTolerance = py::cast<real>(config["cg_tol"]);
         if (config.contains("verbose")) c.mVerbose = py::cast<bool>(config["verbose"]);
-        if (config.contains("predictor")) mCfg.mPredictor = py::cast<bool>(config["predictor"]);
+        if (config.contains("predictor")) mCfg.mPredictor = py::cast<int>(config["predictor"]);
         if (config.contains("precond")) {
             std::string p = py::cast<std::string>(config["precond"]);
             if (p == "none") mCfg.mPreconditioner = FemPhysicsMatrixFree::PC_NONE;

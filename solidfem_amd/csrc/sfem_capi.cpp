@@ -540,6 +540,7 @@ int sfem_create(const sfem_mesh* m, const sfem_material* mat, sfem_ctx** out)
         hipStream_t s = c->stream;
         zero<double>(c->d_u64, n3, s);
         zero<double>(c->d_uprev, n3, s);
+        zero<double>(c->d_uprev2, n3, s);
         zero<double>(c->d_ustart, n3, s);
         zero<double>(c->d_vel, n3, s);
         zero<double>(c->d_u0step, n3, s);
@@ -609,7 +610,7 @@ int sfem_set_positions(sfem_ctx* c, const double* x)
         HIPCK(hipMemcpy(c->d_u64.p, u.data(), n3 * sizeof(double), hipMemcpyHostToDevice));
         HIPCK(launch_to_f32((int64_t)n3, c->d_u64.as<double>(), c->d_u32.as<float>(), c->stream));
         HIPCK(hipStreamSynchronize(c->stream));
-        c->has_prev = false;
+        c->has_prev = c->has_prev2 = false;
         return SFEM_OK;
     });
 }
@@ -632,7 +633,7 @@ int sfem_set_load_fraction(sfem_ctx* c, double frac)
 {
     if (!c) return fail(SFEM_EINVAL, "sfem_set_load_fraction: null context");
     c->frac = frac;
-    c->has_prev = false;
+    c->has_prev = c->has_prev2 = false;
     return SFEM_OK;
 }
 
@@ -900,7 +901,7 @@ int sfem_reset(sfem_ctx* c)
         HIPCK(hipMemsetAsync(c->d_vel.p, 0, 3 * c->nn * sizeof(double), c->stream));
         HIPCK(hipStreamSynchronize(c->stream));
         c->frac = 0.0;
-        c->has_prev = false;
+        c->has_prev = c->has_prev2 = false;
         set_time_step(c, 0.0);
         return SFEM_OK;
     });
@@ -1018,16 +1019,20 @@ int sfem_step(sfem_ctx* c, const sfem_solve_params* p, sfem_solve_stats* st)
         if (c->frac < 1) {
             const double f_old = c->frac, f_new = std::min(1.0, c->frac + p->force_step);
             int extra = 0;
-            bool predicted = false;
+            bool predicted = false, quadratic = false;
             int rc = guard_call("sfem_step", [&]() -> int {
                 const int64_t n3 = 3 * c->nn;
                 double* u = c->d_u64.as<double>();
                 double* r = c->d_r64.as<double>();
-                const bool pred = p->mode == SFEM_SOLVE_CONVERGED && p->predictor && c->has_prev && f_old > c->frac_prev;
+                const int order = p->mode == SFEM_SOLVE_CONVERGED ? p->predictor : 0;
+                const bool pred = order > 0 && c->has_prev && f_old > c->frac_prev;
+                const bool quad = order >= 2 && pred && c->has_prev2 && c->frac_prev > c->frac_prev2;
                 c->frac = f_new;
                 if (pred) {
-                    // secant predictor u + s (u - u_prev), dropped if it inverts an
-                    // element (non-finite residual).  Its residual norm is usually
+                    // predictor: the secant u + s (u - u_prev) of the last two
+                    // states, or the quadratic through the last three (Lagrange
+                    // in the load fraction), dropped if it inverts an element
+                    // (non-finite residual).  Its residual norm is usually
                     // larger than at u (the stiff modes it misses dominate ||r||)
                     // while its error in the soft modes that Newton-CG converges
                     // slowest is much smaller, so ||r|| is no test of it at the
@@ -1035,20 +1040,36 @@ int sfem_step(sfem_ctx* c, const sfem_solve_params* p, sfem_solve_stats* st)
                     // PRED_PATIENCE steps, else the step is redone from u.
                     residual(c, u, r);
                     c->abort_rn = std::sqrt(dots3(c, r, nullptr, nullptr).aa);
-                    HIPCK(launch_extrapolate(n3, u, c->d_uprev.as<double>(), (f_new - f_old) / (f_old - c->frac_prev),
-                                             c->d_u32.as<float>(), c->stream));
+                    if (quad) {
+                        const double f0 = c->frac_prev2, f1 = c->frac_prev, f2 = f_old, f = f_new;
+                        const double l0 = (f - f1) * (f - f2) / ((f0 - f1) * (f0 - f2));
+                        const double l1 = (f - f0) * (f - f2) / ((f1 - f0) * (f1 - f2));
+                        const double l2 = (f - f0) * (f - f1) / ((f2 - f0) * (f2 - f1));
+                        HIPCK(launch_extrapolate2(n3, u, c->d_uprev.as<double>(), c->d_uprev2.as<double>(), l2, l1, l0,
+                                                  c->d_u32.as<float>(), c->stream));
+                    } else {
+                        HIPCK(hipMemcpyAsync(c->d_uprev2.p, c->d_uprev.p, n3 * sizeof(double), hipMemcpyDeviceToDevice,
+                                             c->stream));
+                        HIPCK(launch_extrapolate(n3, u, c->d_uprev.as<double>(), (f_new - f_old) / (f_old - c->frac_prev),
+                                                 c->d_u32.as<float>(), c->stream));
+                    }
                     residual(c, u, r);
                     const double r1 = dots3(c, r, nullptr, nullptr).aa;
                     extra = 2;
                     predicted = std::isfinite(r1);
+                    quadratic = quad && predicted;
                     c->abort_after = predicted ? PRED_PATIENCE : 0;
                     if (!predicted) {
                         HIPCK(hipMemcpyAsync(u, c->d_uprev.p, n3 * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
                         HIPCK(launch_to_f32(n3, u, c->d_u32.as<float>(), c->stream));
                     }
                 } else {
+                    HIPCK(hipMemcpyAsync(c->d_uprev2.p, c->d_uprev.p, n3 * sizeof(double), hipMemcpyDeviceToDevice,
+                                         c->stream));
                     HIPCK(hipMemcpyAsync(c->d_uprev.p, u, n3 * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
                 }
+                c->frac_prev2 = c->frac_prev;
+                c->has_prev2 = c->has_prev;
                 c->frac_prev = f_old;
                 c->has_prev = true;
                 return SFEM_OK;
@@ -1056,6 +1077,45 @@ int sfem_step(sfem_ctx* c, const sfem_solve_params* p, sfem_solve_stats* st)
             if (rc != SFEM_OK) return rc;
             rc = sfem_solve(c, p, st);
             c->abort_after = 0;
+            // adds the work of an abandoned attempt to the stats of the next one
+            auto add_attempt = [](sfem_solve_stats* st, const sfem_solve_stats& first) {
+                st->newton_iters += first.newton_iters;
+                st->cg_iters += first.cg_iters;
+                st->fallbacks += first.fallbacks;
+                st->rejects += first.rejects + 1;
+                st->reruns += first.reruns;
+                st->seconds += first.seconds;
+                st->hessvec_evals += first.hessvec_evals;
+                st->force_evals += first.force_evals;
+                st->res0 = first.res0;
+            };
+            if (quadratic && rc == SFEM_ENOTCONV) {
+                // the quadratic prediction failed (it can overshoot into a state
+                // where the tangent is not positive definite): try the secant one
+                // u + s (u - u_prev) from the same history before the last state
+                const sfem_solve_stats first = *st;
+                bool finite = false;
+                rc = guard_call("sfem_step", [&]() -> int {
+                    const int64_t n3 = 3 * c->nn;
+                    const double sc = (f_new - c->frac_prev) / (c->frac_prev - c->frac_prev2);
+                    HIPCK(launch_lincomb(n3, c->d_uprev.as<double>(), c->d_uprev2.as<double>(), 1.0 + sc, -sc,
+                                         c->d_u64.as<double>(), c->d_u32.as<float>(), c->stream));
+                    residual(c, c->d_u64.as<double>(), c->d_r64.as<double>());
+                    finite = std::isfinite(dots3(c, c->d_r64.as<double>(), nullptr, nullptr).aa);
+                    return SFEM_OK;
+                });
+                if (rc != SFEM_OK) return rc;
+                extra++;
+                if (finite) {
+                    c->abort_after = PRED_PATIENCE;
+                    rc = sfem_solve(c, p, st);
+                    c->abort_after = 0;
+                    add_attempt(st, first);
+                } else {
+                    *st = first;
+                    rc = SFEM_ENOTCONV;
+                }
+            }
             if (predicted && rc == SFEM_ENOTCONV) {
                 // the solve from the predicted state failed: redo the step from the
                 // last state (what Step does), adding up the work of both attempts
@@ -1068,15 +1128,7 @@ int sfem_step(sfem_ctx* c, const sfem_solve_params* p, sfem_solve_stats* st)
                 });
                 if (rc != SFEM_OK) return rc;
                 rc = sfem_solve(c, p, st);
-                st->newton_iters += first.newton_iters;
-                st->cg_iters += first.cg_iters;
-                st->fallbacks += first.fallbacks;
-                st->rejects += first.rejects + 1;
-                st->reruns += first.reruns;
-                st->seconds += first.seconds;
-                st->hessvec_evals += first.hessvec_evals;
-                st->force_evals += first.force_evals;
-                st->res0 = first.res0;
+                add_attempt(st, first);
             }
             st->force_evals += (int64_t)extra * c->ne;
             return rc;
@@ -1092,7 +1144,7 @@ int sfem_step(sfem_ctx* c, const sfem_solve_params* p, sfem_solve_stats* st)
             HIPCK(hipStreamSynchronize(c->stream));
             auto t0 = std::chrono::steady_clock::now();
             c->frac = 1.0;
-            c->has_prev = false;
+            c->has_prev = c->has_prev2 = false;
             const double h = p->time_step / p->substeps;
             double* part = reinterpret_cast<double*>(c->d_partial.as<char>() + (size_t)c->nrows_partial * 9 * sizeof(float));
             BlockArgs a = c->blk;   // elastic forces only (ElasticEnergy::ComputeForces)
@@ -1127,7 +1179,7 @@ int sfem_step(sfem_ctx* c, const sfem_solve_params* p, sfem_solve_stats* st)
             HIPCK(hipMemcpyAsync(c->d_u0step.p, c->d_u64.p, n3 * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
             set_time_step(c, h);
             c->frac = 1.0;
-            c->has_prev = false;
+            c->has_prev = c->has_prev2 = false;
             return SFEM_OK;
         });
         if (rc != SFEM_OK) return rc;

@@ -108,9 +108,12 @@ struct sfem_ctx {
     double frac = 0.0;
     double tstep = 0.0;   // implicit dynamics substep h; 0 = static (FemPhysicsMatrixFree::mTimeStep)
     // load-step predictor history: the converged displacement of the previous
-    // quasi-static step (d_uprev) and its load fraction
+    // quasi-static step (d_uprev) and its load fraction,
     double frac_prev = 0.0;
     bool has_prev = false;
+    // the state before it (d_uprev2, frac_prev2): the quadratic predictor
+    double frac_prev2 = 0.0;
+    bool has_prev2 = false;
     // a predicted solve stops after abort_after Newton steps if ||r|| > abort_rn
     // (pred_aborted records that it did: the judge does not rerun such a solve,
     // the step is redone from the last state instead)
@@ -138,7 +141,7 @@ struct sfem_ctx {
     DevBuf d_partial;        // nrows_partial * 9 floats (also used as FP64 x3)
 
     // node data
-    DevBuf d_ustart, d_uprev, d_vel, d_u0step, d_mdiag32, d_mdiag64, d_eorig, d_X0, d_mass, d_fixed, d_fdead;
+    DevBuf d_ustart, d_uprev, d_uprev2, d_vel, d_u0step, d_mdiag32, d_mdiag64, d_eorig, d_X0, d_mass, d_fixed, d_fdead;
     DevBuf d_u64, d_ut64, d_u32;
     DevBuf d_r64, d_ext64, d_f64tmp;
     DevBuf d_b32, d_x32, d_r32, d_z32, d_p32, d_q32, d_d32, d_minv, d_minv_learned, d_bd32;

@@ -1089,6 +1089,33 @@ __global__ __launch_bounds__(256) void k_extrapolate(int64_t n3, double* __restr
     }
 }
 
+// u <- a u + b uprev + c uprev2 (the quadratic load-step predictor); the
+// history shifts: uprev2 <- uprev, uprev <- u
+__global__ __launch_bounds__(256) void k_extrapolate2(int64_t n3, double* __restrict__ u, double* __restrict__ uprev,
+                                                      double* __restrict__ uprev2, double a, double b, double c,
+                                                      float* __restrict__ u32)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n3; i += (int64_t)gridDim.x * blockDim.x) {
+        const double x2 = u[i], x1 = uprev[i], x0 = uprev2[i];
+        const double v = a * x2 + b * x1 + c * x0;
+        uprev2[i] = x1;
+        uprev[i] = x2;
+        u[i] = v;
+        u32[i] = (float)v;
+    }
+}
+
+// u = a x + b y (FP64) and its FP32 copy
+__global__ __launch_bounds__(256) void k_lincomb(int64_t n3, const double* __restrict__ x, const double* __restrict__ y,
+                                                 double a, double b, double* __restrict__ u, float* __restrict__ u32)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n3; i += (int64_t)gridDim.x * blockDim.x) {
+        const double v = a * x[i] + b * y[i];
+        u[i] = v;
+        u32[i] = (float)v;
+    }
+}
+
 __global__ __launch_bounds__(256) void k_to_f32(int64_t n, const double* __restrict__ a, float* __restrict__ b)
 {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
@@ -1549,6 +1576,20 @@ hipError_t launch_implicit_vel(int64_t nn, const double* u, const double* u0, co
 hipError_t launch_extrapolate(int64_t n3, double* u, double* uprev, double sc, float* u32, hipStream_t st)
 {
     hipLaunchKernelGGL(k_extrapolate, dim3(grid_for(n3, 256, 8192)), dim3(256), 0, st, n3, u, uprev, sc, u32);
+    return hipGetLastError();
+}
+
+hipError_t launch_lincomb(int64_t n3, const double* x, const double* y, double a, double b, double* u, float* u32,
+                          hipStream_t st)
+{
+    hipLaunchKernelGGL(k_lincomb, dim3(grid_for(n3, 256, 8192)), dim3(256), 0, st, n3, x, y, a, b, u, u32);
+    return hipGetLastError();
+}
+
+hipError_t launch_extrapolate2(int64_t n3, double* u, double* uprev, double* uprev2, double a, double b, double c, float* u32,
+                               hipStream_t st)
+{
+    hipLaunchKernelGGL(k_extrapolate2, dim3(grid_for(n3, 256, 8192)), dim3(256), 0, st, n3, u, uprev, uprev2, a, b, c, u32);
     return hipGetLastError();
 }
 

@@ -205,6 +205,12 @@ hipError_t launch_explicit(int64_t nn, double* u, double* vel, const double* f, 
 // P per element at displacement u, 9 doubles at the element's original index
 hipError_t launch_elem_stress(int mat, const BlockArgs& a, const double* u, const int* eorig, double* out, hipStream_t st);
 hipError_t launch_extrapolate(int64_t n3, double* u, double* uprev, double sc, float* u32, hipStream_t st);
+// u = a x + b y and its FP32 copy u32
+hipError_t launch_lincomb(int64_t n3, const double* x, const double* y, double a, double b, double* u, float* u32,
+                          hipStream_t st);
+// u <- a u + b uprev + c uprev2, then uprev2 <- uprev, uprev <- u (old)
+hipError_t launch_extrapolate2(int64_t n3, double* u, double* uprev, double* uprev2, double a, double b, double c, float* u32,
+                               hipStream_t st);
 hipError_t launch_to_f32(int64_t n, const double* a, float* b, hipStream_t st);
 // ext = frac (m g + f_dead) + pressure (+ m/h v + m/h^2 u0 with vel, u0: implicit dynamics)
 hipError_t launch_external(int64_t nn, const double* mass, const double* fdead, const double* g, double frac, const int* pptr,
