@@ -1089,10 +1089,12 @@ int sfem_step(sfem_ctx* c, const sfem_solve_params* p, sfem_solve_stats* st)
                 st->force_evals += first.force_evals;
                 st->res0 = first.res0;
             };
-            if (quadratic && rc == SFEM_ENOTCONV) {
-                // the quadratic prediction failed (it can overshoot into a state
-                // where the tangent is not positive definite): try the secant one
-                // u + s (u - u_prev) from the same history before the last state
+            if (quadratic && rc == SFEM_ENOTCONV && c->pred_aborted) {
+                // the quadratic prediction was abandoned (it can overshoot into a
+                // state where the tangent is not positive definite): try the secant
+                // one u + s (u - u_prev) from the same history before the last
+                // state.  An attempt that used up its Newton steps is not retried
+                // from another prediction (Newton stalls there from any start).
                 const sfem_solve_stats first = *st;
                 bool finite = false;
                 rc = guard_call("sfem_step", [&]() -> int {
