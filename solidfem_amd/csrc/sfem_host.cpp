@@ -87,6 +87,81 @@ bool element_geometry(const int32_t* tets, int64_t ne, const double* X0, int64_t
     return true;
 }
 
+// LDS bank-aware order of a block's nodes.  The Hessian-vector kernel stages
+// one 16-byte row per block node in LDS and each lane (one element) gathers
+// the rows of its four corners; rows whose index agrees modulo ROW_CLASSES
+// fall in the same LDS banks, and distinct such rows read by the lanes of one
+// group of GROUP_LANES in one instruction are served one after the other.
+// The sorted order puts them there at random.  This order colours the block's
+// nodes with the ROW_CLASSES residues (each residue as often as the block has
+// rows for it) so that the distinct nodes a lane group reads at one corner get
+// distinct residues wherever the greedy colouring finds one (most constrained
+// node first); a node's residue class is then filled in sorted order.
+// -> order[new local id] = index into the sorted ids.  Large blocks (more than
+// MAX_ORDERED nodes, unstructured meshes) keep the sorted order.
+#ifndef SFEM_ROW_CLASSES
+#define SFEM_ROW_CLASSES 8
+#endif
+#ifndef SFEM_GROUP_LANES
+#define SFEM_GROUP_LANES 8
+#endif
+static std::vector<int> bank_order(const std::vector<int>& lsorted, int64_t nel, int nb)
+{
+    constexpr int R = SFEM_ROW_CLASSES, L = SFEM_GROUP_LANES, MAX_ORDERED = 1024;
+    std::vector<int> order(nb);
+    for (int i = 0; i < nb; i++) order[i] = i;
+    if (nb > MAX_ORDERED || nb <= R) return order;
+    // w[a * nb + b]: how often nodes a and b are read by the same corner of
+    // one lane group
+    std::vector<uint16_t> w((size_t)nb * nb, 0);
+    std::vector<int> set;
+    for (int64_t g0 = 0; g0 < nel; g0 += L)
+        for (int v = 0; v < 4; v++) {
+            set.clear();
+            for (int64_t s = g0; s < std::min<int64_t>(nel, g0 + L); s++) set.push_back(lsorted[4 * s + v]);
+            std::sort(set.begin(), set.end());
+            set.erase(std::unique(set.begin(), set.end()), set.end());
+            for (size_t i = 0; i < set.size(); i++)
+                for (size_t j = i + 1; j < set.size(); j++) {
+                    w[(size_t)set[i] * nb + set[j]]++;
+                    w[(size_t)set[j] * nb + set[i]]++;
+                }
+        }
+    std::vector<int> colour(nb, -1), cap(R), deg(nb, 0);
+    for (int c = 0; c < R; c++) cap[c] = (nb - c + R - 1) / R;
+    for (int a = 0; a < nb; a++)
+        for (int b = 0; b < nb; b++) deg[a] += w[(size_t)a * nb + b];
+    // cost[a * R + c]: weight of a's coloured neighbours of colour c
+    std::vector<int> cost((size_t)nb * R, 0), sat(nb, 0);
+    for (int step = 0; step < nb; step++) {
+        // the uncoloured node with the most distinct neighbour colours, then the largest weight
+        int a = -1;
+        for (int i = 0; i < nb; i++)
+            if (colour[i] < 0 && (a < 0 || sat[i] > sat[a] || (sat[i] == sat[a] && deg[i] > deg[a]))) a = i;
+        int best = -1;
+        for (int c = 0; c < R; c++)
+            if (cap[c] > 0 && (best < 0 || cost[(size_t)a * R + c] < cost[(size_t)a * R + best] ||
+                               (cost[(size_t)a * R + c] == cost[(size_t)a * R + best] && cap[c] > cap[best])))
+                best = c;
+        colour[a] = best;
+        cap[best]--;
+        for (int b = 0; b < nb; b++) {
+            const int wab = w[(size_t)a * nb + b];
+            if (!wab || colour[b] >= 0) continue;
+            if (cost[(size_t)b * R + best] == 0) sat[b]++;
+            cost[(size_t)b * R + best] += wab;
+        }
+    }
+    // residue class c holds local ids c, c + R, c + 2R, ...
+    std::vector<int> next(R);
+    for (int c = 0; c < R; c++) next[c] = c;
+    for (int i = 0; i < nb; i++) {
+        order[next[colour[i]]] = i;
+        next[colour[i]] += R;
+    }
+    return order;
+}
+
 Partition make_partition(const int32_t* tets, int64_t ne, const double* X0, int64_t nn, const uint8_t* held,
                          const std::vector<double>& mu, const std::vector<double>& lam, const std::vector<double>& dminv,
                          const std::vector<double>& vol, int BE)
@@ -127,7 +202,7 @@ Partition make_partition(const int32_t* tets, int64_t ne, const double* X0, int6
     std::vector<int> node_blocks(nn, 0);
     P.bnodes.reserve((size_t)ne);
     P.lptr.reserve((size_t)ne);
-    std::vector<int> ids, cnt;
+    std::vector<int> ids, cnt, lsorted, lid;
     for (int b = 0; b < nblk; b++) {
         const int64_t e0 = (int64_t)b * BE;
         const int64_t e1 = std::min<int64_t>(ne, e0 + BE);
@@ -139,7 +214,17 @@ Partition make_partition(const int32_t* tets, int64_t ne, const double* X0, int6
         const int nb = (int)ids.size();
         P.nmax = std::max(P.nmax, nb);
         P.bnode_ptr[b + 1] = P.bnode_ptr[b] + nb;
-        for (int id : ids) {
+        // the block's rows in LDS bank-aware order: local id of sorted index i = lid[i]
+        lsorted.assign((size_t)4 * (e1 - e0), 0);
+        for (int64_t s = e0; s < e1; s++)
+            for (int v = 0; v < 4; v++)
+                lsorted[4 * (s - e0) + v] =
+                    (int)(std::lower_bound(ids.begin(), ids.end(), tets[4 * key[s].second + v]) - ids.begin());
+        const std::vector<int> order = bank_order(lsorted, e1 - e0, nb);
+        lid.assign(nb, 0);
+        for (int j = 0; j < nb; j++) lid[order[j]] = j;
+        for (int j = 0; j < nb; j++) {
+            const int id = ids[order[j]];
             P.bnodes.push_back((uint32_t)id | (held[id] ? F_FIXED : 0u));
             node_blocks[id]++;
         }
@@ -149,7 +234,7 @@ Partition make_partition(const int32_t* tets, int64_t ne, const double* X0, int6
             const int64_t e = key[s].second;
             int l[4];
             for (int v = 0; v < 4; v++) {
-                l[v] = (int)(std::lower_bound(ids.begin(), ids.end(), tets[4 * e + v]) - ids.begin());
+                l[v] = lid[lsorted[4 * (s - e0) + v]];
                 cnt[l[v] + 1]++;
             }
             P.lidx[2 * s] = (uint32_t)l[0] | (uint32_t)l[1] << 16;

@@ -119,7 +119,11 @@ static void check_mesh(const Mesh& m, bool vary_moduli, const char* name)
         for (int b = 0; b < P.nblk; b++) {
             const int n0 = P.bnode_ptr[b], nb = P.bnode_ptr[b + 1] - n0;
             CHECK(nb > 0 && nb <= P.nmax && nb < 65536);
-            for (int r = 0; r + 1 < nb; r++) CHECK((P.bnodes[n0 + r] & F_NODE_MASK) < (P.bnodes[n0 + r + 1] & F_NODE_MASK));
+            // the block's rows name distinct nodes (in an LDS bank-aware order, not sorted)
+            std::vector<int> rows_sorted(nb);
+            for (int r = 0; r < nb; r++) rows_sorted[r] = (int)(P.bnodes[n0 + r] & F_NODE_MASK);
+            std::sort(rows_sorted.begin(), rows_sorted.end());
+            for (int r = 0; r + 1 < nb; r++) CHECK(rows_sorted[r] < rows_sorted[r + 1]);
             // incidence offsets: this block's lptr segment is at n0 + b
             const uint16_t* lp = &P.lptr[n0 + b];
             CHECK(lp[0] == 0);
