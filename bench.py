@@ -34,15 +34,15 @@ TIME_EVERY = 16
 COUNTERS = os.path.join(ROOT, "profiles", "hessvec_nh786k_counters.md")
 
 
-def measured_traffic(grid):
-    """HBM bytes per launch of the Hessian-vector kernel on the 128x32x32 beam
+def measured_traffic(grid, key):
+    """HBM bytes per launch of a Hessian-vector kernel on the 128x32x32 beam
     from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (its line
-    `traffic_bytes_per_launch: N`); None for other grids or without it."""
+    `key: N`); None for other grids or without it."""
     if grid != (128, 32, 32) or not os.path.exists(COUNTERS):
         return None
     with open(COUNTERS) as f:
         for line in f:
-            if line.startswith("traffic_bytes_per_launch:"):
+            if line.startswith(key + ":"):
                 return float(line.split(":")[1])
     return None
 
@@ -219,6 +219,9 @@ def main():
     ap.add_argument("--per-element-moduli", action="store_true",
                     help="with --hessvec-only: E and nu per element, drawn as in the per-element-material workload "
                          "(E log-uniform in [1e5, 1e7] Pa, nu uniform in [0.30, 0.48])")
+    ap.add_argument("--tangent-cache", type=int, choices=[0, 1], default=1,
+                    help="Neo-Hookean tangent state cached per Newton step (1, the default) or rebuilt in every "
+                         "Hessian-vector product (0)")
     ap.add_argument("--cpu-full-newton", action="store_true",
                     help="only run the CPU FP64 restatement's converged Jacobi-PCG Newton on --grid in full "
                          "(no GPU), and print its wall time as one JSON line")
@@ -243,6 +246,7 @@ def main():
         per_e = dict(young_e=10 ** (5 + 2 * uniform01(0, 1, m.ne)), poisson_e=0.30 + 0.18 * uniform01(0, 2, m.ne))
     s = capi.Solver(m.X, m.tets, fixed, MATERIALS[args.material], 1e6, 0.45, 1070.0, gravity=(0.0, 0.0, -9.8), **per_e)
     s.set_external_forces(fdead)
+    s.set_tangent_cache(args.tangent_cache)
     if args.hessvec_only:
         hessvec_only(s, m, args)
         return
@@ -361,7 +365,7 @@ def main():
                    "E": 1e6, "nu": 0.45, "loads": "gravity -z + dead tip traction 1e4 Pa -z",
                    "load_steps": args.load_steps, "tol": 1e-8,
                    "precond": "jacobi" if args.precond in ("both", "all") else args.precond,
-                   "parallelism": "none"},
+                   "tangent_cache": args.tangent_cache, "parallelism": "none"},
         "newton_solve_s": dt / args.steps,
         "newton_iters": runs[-1]["newton_iters"],
         "cg_iters": runs[-1]["cg_iters"],
@@ -370,11 +374,17 @@ def main():
         "final_rel_residual": runs[-1]["res"] / runs[-1]["ext_norm"],
         "tip_deflection_m": float((x - m.X)[:, 2].min()),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(grid),
-                     "traffic_source": "profile-derived: rocprofv3 FETCH_SIZE + WRITE_SIZE of the stand-alone element "
-                                       "kernel (profiles/hessvec_nh786k_counters.md), not measured in this run",
-                     "kernel": ("k_elem_block<0,NEO_HOOKEAN,256,PF> (Hessian-vector element blocks with the CG "
-                                "direction update)" if n_timed else
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic(grid, "traffic_bytes_per_launch_cg_lin" if args.tangent_cache
+                                                 else "traffic_bytes_per_launch_cg"),
+                     "traffic_source": "profile-derived: rocprofv3 FETCH_SIZE + WRITE_SIZE of this kernel in a "
+                                       "one-load-step Jacobi-PCG solve (profiles/hessvec_nh786k_counters.md), not "
+                                       "measured in this run",
+                     "kernel": (("k_elem_block<0,NEO_HOOKEAN_LIN,256,PF> (Hessian-vector element blocks on the "
+                                 "Neo-Hookean tangent state cached per Newton step, with the CG direction update)"
+                                 if args.tangent_cache else
+                                 "k_elem_block<0,NEO_HOOKEAN,256,PF> (Hessian-vector element blocks with the CG "
+                                 "direction update)") if n_timed else
                                 "k_elem_block<0,NEO_HOOKEAN,256> (Hessian-vector element blocks)"),
                      "ms_per_launch": ms_elem, "timed_launches": n_timed, "timing": timing,
                      "bytes_per_launch": bytes_alg,

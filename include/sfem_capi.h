@@ -272,6 +272,20 @@ int sfem_set_precond_blocks(sfem_ctx* ctx, const float* minv9);
    a host round trip (the state FemPhysicsMatrixFree's constructor leaves). */
 int sfem_reset(sfem_ctx* ctx);
 
+/* Neo-Hookean static solves (no reference counterpart; the reference rebuilds
+   F, F^-1 and ln J of every element in every product,
+   FemPhysicsMatrixFree.cpp:417-470): mode 1 (the default) builds the
+   tangent's state per element (Dm^-1 F^-1 and V (mu - lam ln J), 40 B) once
+   per Newton step and makes every Hessian-vector product of that step's
+   block-Jacobi, learned-block or unpreconditioned PCG (and of the reference
+   semantics' CG) from it, on meshes of up to 1.6M elements (a state of up to
+   64 MiB; beyond the 256 MB Infinity Cache its reads cost more than the
+   arithmetic they save); mode 0 rebuilds it in every product; mode 2 as 1,
+   and sfem_matvec builds it at each call too (for tests).  They agree to
+   FP32 rounding.  The multigrid PCG and the other stand-alone calls always
+   use the state itself. */
+int sfem_set_tangent_cache(sfem_ctx* ctx, int mode);
+
 /* Instrumentation: average device time (ms) of one launch of the Hessian-vector
    element-block kernel and of its shared-node gather kernel over `reps`
    back-to-back launches at the current state, timed with HIP events on the

@@ -31,6 +31,7 @@ EXPORTS = [
     "sfem_reset", "sfem_profile_hessvec", "sfem_time_hessvec",
     "sfem_mg_add_level", "sfem_mg_clear", "sfem_mg_info", "sfem_mg_apply", "sfem_mg_pcg", "sfem_mg_coarse_dense",
     "sfem_mg_set_schedule",
+    "sfem_set_tangent_cache",
     "sfem_gnn_create", "sfem_gnn_destroy", "sfem_gnn_num_params", "sfem_gnn_get_params", "sfem_gnn_set_params",
     "sfem_set_learned_preconditioner", "sfem_collect_pairs", "sfem_collect_rough_pairs", "sfem_num_pairs", "sfem_get_pair", "sfem_clear_pairs",
     "sfem_gnn_loss", "sfem_gnn_eval", "sfem_gnn_train", "sfem_gnn_inputs", "sfem_gnn_gemm_check",
@@ -127,6 +128,7 @@ def lib():
             "sfem_mg_apply": ([V, _dp, _dp], C.c_int),
             "sfem_mg_pcg": ([V, _dp, _dp, C.c_double, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32)], C.c_int),
             "sfem_mg_set_schedule": ([V, C.c_int], C.c_int),
+            "sfem_set_tangent_cache": ([V, C.c_int], C.c_int),
             "sfem_gnn_create": ([C.c_uint64, C.POINTER(V)], C.c_int),
             "sfem_gnn_destroy": ([V], None),
             "sfem_gnn_num_params": ([V, C.POINTER(C.c_int64)], C.c_int),
@@ -371,6 +373,13 @@ class Solver:
         """the V-cycle's launch schedule: True folds the smoothing steps into the
         element kernels (the default), False launches one kernel per step"""
         _check(lib().sfem_mg_set_schedule(self.h, 1 if fused else 0))
+
+    def set_tangent_cache(self, mode):
+        """Neo-Hookean static solves: 1 (the default) builds the tangent's
+        state per element once per Newton step and makes that step's products
+        from it; 0 rebuilds it in every product; 2 as 1, and matvec() builds
+        it at each call too"""
+        _check(lib().sfem_set_tangent_cache(self.h, int(mode)))
 
     def mg_pcg(self, b, rtol=1e-6, maxit=500):
         """-> (x, iterations, status) of one multigrid-PCG solve K x = b"""

@@ -90,6 +90,7 @@ void build_partition(sfem_ctx* c, const sfem_mesh* m, const std::vector<double>&
     a.node_q = c->d_node_info.as<int2>();
     a.zero_flag = c->d_node_info.as<int>() + 2 * nn;
     a.mat32 = uniform ? nullptr : c->d_mat32.as<float>();
+    a.lin32 = nullptr;
     a.mu0 = (float)mu[0];
     a.lam0 = (float)lam[0];
     SharedArgs& s = c->shr;
@@ -119,6 +120,39 @@ void set_time_step(sfem_ctx* c, double h)
         c->blk.mdiag64 = nullptr;
     }
 }
+
+// The Neo-Hookean tangent's state per element (E = Dm^-1 F^-1, V c) for the
+// linear solves of one Newton step: built from u32 when they begin, read by
+// every Hessian-vector product they make, dropped when they end, before u32
+// changes, and by the destructor on every other way out.  The CG loops use it
+// while it stays small beside the 256 MB Infinity Cache (their fused
+// direction-update-and-product kernel: 17.4 instead of 18.1 us per launch on
+// the 786,432-tet beam, 130 instead of 116 us on the 6.3M-tet one, where the
+// 40 B per element come from HBM); the multigrid PCG, the stand-alone
+// products and all other kernels read the state itself (a plain product is
+// latency-bound, and the 40 B per element cost it more than the arithmetic
+// they save: 15.2 against 13.7 us on the 786,432-tet beam).
+constexpr size_t LIN_STATE_MAX_BYTES = size_t(64) << 20;
+
+class LinState {
+public:
+    explicit LinState(sfem_ctx* c) : c_(c) {}
+    ~LinState() { c_->blk.lin32 = nullptr; }
+    LinState(const LinState&) = delete;
+    LinState& operator=(const LinState&) = delete;
+    void begin()
+    {
+        const size_t bytes = 10 * (size_t)c_->blk.ne_pad * sizeof(float);
+        if (!c_->lin_cache || c_->material != SFEM_NEO_HOOKEAN || c_->blk.mdiag32 || bytes > LIN_STATE_MAX_BYTES) return;
+        if (c_->d_lin.n < bytes) c_->d_lin.alloc(bytes);
+        HIPCK(launch_lin_state(c_->blk, c_->d_u32.as<float>(), c_->d_lin.as<float>(), c_->stream));
+        c_->blk.lin32 = c_->d_lin.as<float>();
+    }
+    void end() { c_->blk.lin32 = nullptr; }
+
+private:
+    sfem_ctx* c_;
+};
 
 // Newton steps a predicted load step gets to fall below the residual of the
 // unpredicted start before it is abandoned
@@ -332,13 +366,16 @@ int solve_reference(sfem_ctx* c, const sfem_solve_params* p, sfem_solve_stats* s
     HIPCK(launch_identity_blocks(c->nn, c->d_fixed.as<uint8_t>(), c->d_minv.as<float>(), c->stream));
     const float eps = 1e-3f;
     const float e2 = eps * eps;   // float product, as the reference evaluates eps * eps
+    LinState lin(c);
     for (int it = 0; it < p->outer_iterations; it++) {
         residual(c, c->d_u64.as<double>(), c->d_r64.as<double>());
         st->force_evals += c->ne;
         if (it == 0) st->res0 = std::sqrt(dots3(c, c->d_r64.as<double>(), nullptr, nullptr).aa);
         HIPCK(launch_to_f32(n3, c->d_r64.as<double>(), c->d_b32.as<float>(), c->stream));
+        lin.begin();
         CgResult cg = pcg(c, c->d_minv.as<float>(), 0, p->inner_iterations, 0, (double)e2, 0, p->check_every,
                           &st->hessvec_evals);
+        lin.end();
         st->cg_iters += cg.iters;
         HIPCK(launch_axpy_state(n3, c->d_u64.as<double>(), c->d_x32.as<float>(), 1.0, c->d_ut64.as<double>(),
                                 c->d_u32.as<float>(), c->stream));
@@ -385,6 +422,7 @@ int solve_converged(sfem_ctx* c, const sfem_solve_params* p, sfem_solve_stats* s
     double eta = eta_max, rn_prev = 0;
     const double abs_tol = p->abs_tol > 0 ? p->abs_tol : 0.0;
     auto target = [&](double e) { return std::max(p->tol * e, abs_tol); };
+    LinState lin(c);
     for (int it = 0; it < maxn; it++) {
         if (rn <= target(en)) break;
         if (it > 0) {
@@ -394,6 +432,9 @@ int solve_converged(sfem_ctx* c, const sfem_solve_params* p, sfem_solve_stats* s
         }
         eta = std::max(eta, std::max(0.5 * target(en) / rn, 1e-6));
         const bool learned = pc_kind == SFEM_PC_LEARNED || pc_kind == SFEM_PC_MULTIGRID_LEARNED;
+        // the cached state pays in the CG loop's fused kernel (direction update
+        // and product); the V-cycle's plain products are faster without it
+        if (!multigrid) lin.begin();
         if (pc_kind == SFEM_PC_JACOBI || multigrid) build_jacobi(c);
         if (learned && c->gnn) {
             if (!multigrid) build_jacobi(c);   // network input and the blocks it corrects
@@ -431,6 +472,7 @@ int solve_converged(sfem_ctx* c, const sfem_solve_params* p, sfem_solve_stats* s
             st->cg_iters += cg.iters;
             if (trace_on()) std::fprintf(stderr, "sfem   fallback jacobi: cg %d status %d\n", cg.iters, cg.status);
         }
+        lin.end();
         const double R0 = dots3(c, c->d_r64.as<double>(), nullptr, c->d_x32.as<float>()).da;
         double alpha = 1.0;
         Dots dt{0, 0, 0};
@@ -735,6 +777,8 @@ int sfem_matvec(sfem_ctx* c, const double* d, double* Kd)
         for (int64_t i = 0; i < c->nn; i++)
             for (int k = 0; k < 3; k++) d32[3 * i + k] = c->fixed_h[i] ? 0.f : (float)d[3 * i + k];
         HIPCK(hipMemcpyAsync(c->d_d32.p, d32.data(), n3 * sizeof(float), hipMemcpyHostToDevice, c->stream));
+        LinState lin(c);
+        if (c->lin_cache == 2) lin.begin();
         hessvec(c, c->d_d32.as<float>(), c->d_q32.as<float>(), nullptr);
         HIPCK(hipMemcpyAsync(o.data(), c->d_q32.p, n3 * sizeof(float), hipMemcpyDeviceToHost, c->stream));
         HIPCK(hipStreamSynchronize(c->stream));
@@ -941,6 +985,14 @@ int sfem_profile_hessvec(sfem_ctx* c, int reps, double* ms_elem, double* ms_gath
         (void)hipEventDestroy(e2);
         return SFEM_OK;
     });
+}
+
+int sfem_set_tangent_cache(sfem_ctx* c, int mode)
+{
+    if (!c) return fail(SFEM_EINVAL, "sfem_set_tangent_cache: null context");
+    if (mode < 0 || mode > 2) return fail(SFEM_EINVAL, "sfem_set_tangent_cache: mode must be 0, 1 or 2");
+    c->lin_cache = mode;
+    return SFEM_OK;
 }
 
 int sfem_set_precond_blocks(sfem_ctx* c, const float* minv9)

@@ -181,6 +181,12 @@ struct sfem_ctx {
     // folded into the element kernels (true), or one kernel per step; the
     // results are bitwise the same
     bool mg_fused = true;
+    // Neo-Hookean static solves: the tangent's state per element at each
+    // Newton step's displacements (BlockArgs::lin32), built when its linear
+    // solves begin; sfem_set_tangent_cache: 0 off, 1 in solves, 2 also in
+    // sfem_matvec (built at each call)
+    int lin_cache = 1;
+    DevBuf d_lin;
     // this context as a multigrid level under the folded schedule: a second
     // residual and direction, and a second operator image (rows and partial
     // slots)

@@ -62,6 +62,11 @@ template <> struct KindTraits<1> {
 template <> struct KindTraits<2> {
     typedef float T; static constexpr int W = 9; static constexpr bool HAS_D = false; static constexpr bool HAS_X = false;
 };
+// KIND 3: the Neo-Hookean tangent's state per element (nh_lin_state) into
+// out in the BlockArgs::lin32 layout; no corner or node phase
+template <> struct KindTraits<3> {
+    typedef float T; static constexpr int W = 3; static constexpr bool HAS_D = false; static constexpr bool HAS_X = true;
+};
 
 // LDS layout (vec4 = 4 x T, one ds_read/ds_write_b128 per row):
 //   corner outputs: CW vec4 per incidence slot k, in the block's node-sorted
@@ -145,7 +150,10 @@ __global__ __launch_bounds__(BE) void k_elem_block(BlockArgs a, const typename K
     constexpr bool PF = SG == 1;
     constexpr bool MGC = SG == MG_STAGE_CHEB;
     constexpr bool MGP = SG == MG_STAGE_PROLONG;
+    // LC: the Hessian-vector product on the cached Neo-Hookean state (u not read)
+    constexpr bool LC = MAT == NEO_HOOKEAN_LIN;
     static_assert(SG == 0 || KIND == 0, "staged directions only in the Hessian-vector kernel");
+    static_assert(!LC || (KIND == 0 && !DYN), "cached state: static Hessian-vector kernel only");
     static_assert(!(DYN && (MGC || MGP)), "multigrid staging is static only");
     typedef typename KindTraits<KIND>::T T;
     typedef typename Vec4<T>::V V4;
@@ -189,6 +197,7 @@ __global__ __launch_bounds__(BE) void k_elem_block(BlockArgs a, const typename K
     const bool live = e < a.ne;
     T g[9];
     T cmu = T(0), clam = T(0);
+    T lE[9], lc = T(0);
     uint2 li = make_uint2(0, 0), lp4 = make_uint2(0, 0);
     if (live && HAS_X) {
         if (a.mat32) {
@@ -247,8 +256,10 @@ __global__ __launch_bounds__(BE) void k_elem_block(BlockArgs a, const typename K
     auto load_row = [&](int j, Row& w) {
         w.fl = a.bnodes[n0 + j];
         w.node = w.fl & NODE_MASK;
-        const T* up = u + 3 * (size_t)w.node;
-        w.uv.x = up[0]; w.uv.y = up[1]; w.uv.z = up[2]; w.uv.w = T(0);
+        if (!LC) {
+            const T* up = u + 3 * (size_t)w.node;
+            w.uv.x = up[0]; w.uv.y = up[1]; w.uv.z = up[2]; w.uv.w = T(0);
+        }
         if (PF) {
             const float* zp = pf.z + 3 * (size_t)w.node;
             const float* pp = pf.p_old + 3 * (size_t)w.node;
@@ -329,7 +340,7 @@ __global__ __launch_bounds__(BE) void k_elem_block(BlockArgs a, const typename K
             }
             w.dv.x = ev[0]; w.dv.y = ev[1]; w.dv.z = ev[2]; w.dv.w = T(0);
         }
-        su4[j] = w.uv;
+        if (!LC) su4[j] = w.uv;
         if (HAS_D) sd4[j] = w.dv;
         if (HAS_X) sx4[j] = w.xv;
         if (DYN) s_md[j] = owner ? w.md : T(0);
@@ -352,6 +363,18 @@ __global__ __launch_bounds__(BE) void k_elem_block(BlockArgs a, const typename K
         if (blockIdx.x == 0 && tid == 0) cg_test_record(pf.scal, done, rz1, rr, beta_d, conv, bad, indef, pf.stall_window);
         if (conv || bad || indef) return;
         beta = (float)beta_d;
+    }
+    // LC: the element's cached state, issued after the first row loads and the
+    // PF reduction (loads complete in order: the row stores and the reduction
+    // do not wait for these) and unconditionally (in bounds for every thread
+    // of the block; a load under a branch is waited for where the branch
+    // joins); the block's states are contiguous, one 256 B run per wave and
+    // value
+    if (LC) {
+        const float* lb = a.lin32 + (size_t)b * 10 * BE + tid;
+#pragma unroll
+        for (int k = 0; k < 9; k++) lE[k] = lb[k * BE];
+        lc = lb[9 * BE];
     }
     if (tid < nb) store_row(tid, r0, beta);
     for (int j = tid + BE; j < nb; j += BE) {
@@ -384,34 +407,14 @@ __global__ __launch_bounds__(BE) void k_elem_block(BlockArgs a, const typename K
             cmu *= V;
             clam *= V;
         }
-        T uu[4][3];
+        if constexpr (LC) {
+            T dd[4][3], o[4][3];
 #pragma unroll
-        for (int k = 0; k < 4; k++) {
-            const V4 v = su4[l[k]];
-            uu[k][0] = v.x; uu[k][1] = v.y; uu[k][2] = v.z;
-        }
-        T F[9];
-        defgrad(g, uu, F);
-        if (KIND == 0 || KIND == 1) {
-            T o[4][3];
-            if (KIND == 0) {
-                T dd[4][3];
-#pragma unroll
-                for (int k = 0; k < 4; k++) {
-                    const V4 v = sd4[l[k]];
-                    dd[k][0] = v.x; dd[k][1] = v.y; dd[k][2] = v.z;
-                }
-                T dF[9], dP[9];
-                grad4(g, dd, dF);
-                Tangent<T, MAT> tg;
-                tg.init(F, cmu, clam);
-                tg.apply(dF, dP);
-                corners(dP, g, T(1), o);
-            } else {
-                T P[9];
-                stress<T, MAT>(F, cmu, clam, P);
-                corners(P, g, T(-1), o);
+            for (int k = 0; k < 4; k++) {
+                const V4 v = sd4[l[k]];
+                dd[k][0] = v.x; dd[k][1] = v.y; dd[k][2] = v.z;
             }
+            nh_lin_corners(g, lE, cmu, clam, lc, dd, o);
             const int sl[4] = {(int)(lp4.x & 0xffffu), (int)(lp4.x >> 16), (int)(lp4.y & 0xffffu), (int)(lp4.y >> 16)};
 #pragma unroll
             for (int k = 0; k < 4; k++) {
@@ -420,40 +423,85 @@ __global__ __launch_bounds__(BE) void k_elem_block(BlockArgs a, const typename K
                 sf4[sl[k]] = v;
             }
         } else {
-            // K_aa(:,k) = V dP(e_k (x) g_a) g_a,  g_a = gradient of shape function a
-            Tangent<T, MAT> tg;
-            tg.init(F, cmu, clam);
-            const int sl[4] = {(int)(lp4.x & 0xffffu), (int)(lp4.x >> 16), (int)(lp4.y & 0xffffu), (int)(lp4.y >> 16)};
-            T ga[4][3];
+            T uu[4][3];
 #pragma unroll
-            for (int c = 0; c < 3; c++) {
-                ga[1][c] = g[c]; ga[2][c] = g[3 + c]; ga[3][c] = g[6 + c];
-                ga[0][c] = -(g[c] + g[3 + c] + g[6 + c]);
+            for (int k = 0; k < 4; k++) {
+                const V4 v = su4[l[k]];
+                uu[k][0] = v.x; uu[k][1] = v.y; uu[k][2] = v.z;
             }
+            T F[9];
+            defgrad(g, uu, F);
+            if constexpr (KIND == 3) {
+                T E[9], cc;
+                nh_lin_state(g, F, cmu, clam, E, cc);
+                T* ob = out + (size_t)b * 10 * BE + tid;
 #pragma unroll
-            for (int aa = 0; aa < 4; aa++) {
-                T blk[12];
+                for (int k = 0; k < 9; k++) ob[k * BE] = E[k];
+                ob[9 * BE] = cc;
+            } else if (KIND == 0 || KIND == 1) {
+                T o[4][3];
+                if (KIND == 0) {
+                    T dd[4][3];
 #pragma unroll
-                for (int k = 0; k < 3; k++) {
+                    for (int k = 0; k < 4; k++) {
+                        const V4 v = sd4[l[k]];
+                        dd[k][0] = v.x; dd[k][1] = v.y; dd[k][2] = v.z;
+                    }
                     T dF[9], dP[9];
-#pragma unroll
-                    for (int q = 0; q < 9; q++) dF[q] = T(0);
-                    dF[3 * k] = ga[aa][0]; dF[3 * k + 1] = ga[aa][1]; dF[3 * k + 2] = ga[aa][2];
+                    grad4(g, dd, dF);
+                    Tangent<T, MAT> tg;
+                    tg.init(F, cmu, clam);
                     tg.apply(dF, dP);
-#pragma unroll
-                    for (int c = 0; c < 3; c++)
-                        blk[3 * c + k] = dP[3 * c] * ga[aa][0] + dP[3 * c + 1] * ga[aa][1] + dP[3 * c + 2] * ga[aa][2];
+                    corners(dP, g, T(1), o);
+                } else {
+                    T P[9];
+                    stress<T, MAT>(F, cmu, clam, P);
+                    corners(P, g, T(-1), o);
                 }
-                blk[9] = blk[10] = blk[11] = T(0);
+                const int sl[4] = {(int)(lp4.x & 0xffffu), (int)(lp4.x >> 16), (int)(lp4.y & 0xffffu), (int)(lp4.y >> 16)};
 #pragma unroll
-                for (int jv = 0; jv < CW; jv++) {
+                for (int k = 0; k < 4; k++) {
                     V4 v;
-                    v.x = blk[4 * jv]; v.y = blk[4 * jv + 1]; v.z = blk[4 * jv + 2]; v.w = blk[4 * jv + 3];
-                    sf4[sl[aa] * CW + jv] = v;
+                    v.x = o[k][0]; v.y = o[k][1]; v.z = o[k][2]; v.w = T(0);
+                    sf4[sl[k]] = v;
+                }
+            } else {
+                // K_aa(:,k) = V dP(e_k (x) g_a) g_a,  g_a = gradient of shape function a
+                Tangent<T, MAT> tg;
+                tg.init(F, cmu, clam);
+                const int sl[4] = {(int)(lp4.x & 0xffffu), (int)(lp4.x >> 16), (int)(lp4.y & 0xffffu), (int)(lp4.y >> 16)};
+                T ga[4][3];
+#pragma unroll
+                for (int c = 0; c < 3; c++) {
+                    ga[1][c] = g[c]; ga[2][c] = g[3 + c]; ga[3][c] = g[6 + c];
+                    ga[0][c] = -(g[c] + g[3 + c] + g[6 + c]);
+                }
+#pragma unroll
+                for (int aa = 0; aa < 4; aa++) {
+                    T blk[12];
+#pragma unroll
+                    for (int k = 0; k < 3; k++) {
+                        T dF[9], dP[9];
+#pragma unroll
+                        for (int q = 0; q < 9; q++) dF[q] = T(0);
+                        dF[3 * k] = ga[aa][0]; dF[3 * k + 1] = ga[aa][1]; dF[3 * k + 2] = ga[aa][2];
+                        tg.apply(dF, dP);
+#pragma unroll
+                        for (int c = 0; c < 3; c++)
+                            blk[3 * c + k] = dP[3 * c] * ga[aa][0] + dP[3 * c + 1] * ga[aa][1] + dP[3 * c + 2] * ga[aa][2];
+                    }
+                    blk[9] = blk[10] = blk[11] = T(0);
+#pragma unroll
+                    for (int jv = 0; jv < CW; jv++) {
+                        V4 v;
+                        v.x = blk[4 * jv]; v.y = blk[4 * jv + 1]; v.z = blk[4 * jv + 2]; v.w = blk[4 * jv + 3];
+                        sf4[sl[aa] * CW + jv] = v;
+                    }
                 }
             }
         }
     }
+    if constexpr (KIND == 3) return;
     __syncthreads();
 
     // node phase: two adjacent lanes per node split its contiguous incidence
@@ -1257,7 +1305,16 @@ static hipError_t launch_elem_mat(int mat, const BlockArgs& a, const void* u, co
             SFEM_ELEM_CASE(NEO_HOOKEAN);
         }
         break;
-    default: SFEM_ELEM_CASE(NEO_HOOKEAN); break;
+    default:
+        // the static Hessian-vector kernels read the cached state when it is set
+        if constexpr (KIND == 0) {
+            if (a.lin32 && !md) {
+                SFEM_ELEM_DYN(NEO_HOOKEAN_LIN, false);
+                break;
+            }
+        }
+        SFEM_ELEM_CASE(NEO_HOOKEAN);
+        break;
     }
 #undef SFEM_ELEM_CASE
 #undef SFEM_ELEM_DYN
@@ -1317,6 +1374,14 @@ hipError_t launch_blockdiag(int mat, const BlockArgs& a, const SharedArgs& s, co
     return launch_shared<float, 9>(s, partial, out, nullptr, 1, nullptr, st);
 }
 
+hipError_t launch_lin_state(const BlockArgs& a, const float* u32, float* lin, hipStream_t st)
+{
+    hipLaunchKernelGGL((k_elem_block<3, NEO_HOOKEAN, BLOCK_E, false, 0>), dim3(a.nblk), dim3(BLOCK_E), lds_bytes<3>(a), st, a,
+                       u32, nullptr, lin, nullptr, nullptr, 1, const_cast<int*>(a.zero_flag), nullptr, nullptr, PFuse{},
+                       MgStage{});
+    return hipGetLastError();
+}
+
 size_t elem_lds_max(const BlockArgs& a)
 {
     size_t m = lds_bytes<0>(a);
@@ -1351,11 +1416,16 @@ hipError_t set_elem_lds_limits(const BlockArgs& a)
         SFEM_FN(1, COROTATIONAL_POLAR),
         SFEM_FN(2, LINEAR), SFEM_FN(2, STVK), SFEM_FN(2, COROTATIONAL), SFEM_FN(2, NEO_HOOKEAN),
         SFEM_FN(2, NEO_HOOKEAN_COARSE), SFEM_FN(2, COROTATIONAL_POLAR),
+        {(const void*)k_elem_block<0, NEO_HOOKEAN_LIN, BLOCK_E, false, 0>, 0},
+        {(const void*)k_elem_block<0, NEO_HOOKEAN_LIN, BLOCK_E, false, 1>, 0},
+        {(const void*)k_elem_block<0, NEO_HOOKEAN_LIN, BLOCK_E, false, MG_STAGE_CHEB>, 0},
+        {(const void*)k_elem_block<0, NEO_HOOKEAN_LIN, BLOCK_E, false, MG_STAGE_PROLONG>, 0},
+        {(const void*)k_elem_block<3, NEO_HOOKEAN, BLOCK_E, false, 0>, 3},
 #undef SFEM_FN_MG
 #undef SFEM_FN_PF
 #undef SFEM_FN
     };
-    const size_t need[] = {lds_bytes<0>(a), lds_bytes<1>(a), lds_bytes<2>(a)};
+    const size_t need[] = {lds_bytes<0>(a), lds_bytes<1>(a), lds_bytes<2>(a), lds_bytes<3>(a)};
     if (elem_lds_max(a) <= 64 * 1024) return hipSuccess;
     int dev = 0, optin = 0;
     hipError_t e = hipGetDevice(&dev);

@@ -67,6 +67,12 @@ struct BlockArgs {
     // coordinates of the block rows, relative to the block's centre
     const float* rel32;    // 3 per block row
     const float* mat32;    // 2 x ne_pad: mu, lambda per element; null when uniform
+    // Neo-Hookean Hessian-vector kernel: 10 x ne_pad floats, the tangent's
+    // state at the current displacements (E = Dm^-1 F^-1 row-major, then
+    // V (mu - lam ln J)), value k of element b * BLOCK_E + t at
+    // (b * 10 + k) * BLOCK_E + t; set only while the linear solves of one
+    // Newton step run, null otherwise
+    const float* lin32;
     float mu0, lam0;       // uniform mu, lambda
     // implicit dynamics: m / h^2 per node (null when static) and each node's
     // partial-slot range (the first slot's block adds the mass term)
@@ -113,6 +119,9 @@ hipError_t launch_force(int mat, const BlockArgs& a, const SharedArgs& s, const 
                         const double* ext, int mask_fixed, hipStream_t st);
 hipError_t launch_blockdiag(int mat, const BlockArgs& a, const SharedArgs& s, const float* u32, float* out, float* partial,
                             hipStream_t st);
+// the Neo-Hookean tangent's state at the displacements u32 into lin
+// (10 x ne_pad floats, BlockArgs::lin32)
+hipError_t launch_lin_state(const BlockArgs& a, const float* u32, float* lin, hipStream_t st);
 size_t elem_lds_max(const BlockArgs& a);
 hipError_t set_elem_lds_limits(const BlockArgs& a);
 // p2 (may be null): a second direction buffer that also gets p = z
