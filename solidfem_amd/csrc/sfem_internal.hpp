@@ -191,9 +191,11 @@ struct sfem_ctx {
     // residual and direction, and a second operator image (rows and partial
     // slots)
     DevBuf d_mgw_r2, d_mgw_d2, d_mgw_qb, d_mgw_pb;
-    // the multigrid PCG's done-flag copies (pinned host memory, two slots) and
+    // the multigrid PCG's test outcomes (pinned host memory mapped into the
+    // device, two slots: h_slot the host's address, d_slot the device's) and
     // the events after them
-    int* h_flag = nullptr;
+    sfem::CgSlot* h_slot = nullptr;
+    sfem::CgSlot* d_slot = nullptr;
     hipEvent_t flag_ev[2] = {};
     // the coarsest level's inverse is computed on a second stream (default
     // priority) beside the other levels' set-up: [0] coarse state ready,
@@ -216,7 +218,7 @@ struct sfem_ctx {
         if (stream) (void)hipStreamSynchronize(stream);
         for (hipEvent_t e : flag_ev)
             if (e) (void)hipEventDestroy(e);
-        if (h_flag) (void)hipHostFree(h_flag);
+        if (h_slot) (void)hipHostFree(h_slot);
         if (mg_side) (void)hipStreamSynchronize(mg_side);
         mg.clear();
         for (hipEvent_t e : mg_ev)

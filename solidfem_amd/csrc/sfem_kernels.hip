@@ -101,25 +101,29 @@ size_t lds_layout_bytes(int BE, int nmax)
 
 // the CG scalars and status of one convergence test (one thread of the grid):
 // the stagnation reference is read and written by that thread only
-__device__ __forceinline__ void cg_test_record(double* scal, int* done, double rz1, double rr, double beta, bool conv,
-                                               bool bad, bool indef, int stall_window)
+// (returns the flag's new value)
+__device__ __forceinline__ int cg_test_record(double* scal, int* done, double rz1, double rr, double beta, bool conv,
+                                              bool bad, bool indef, int stall_window)
 {
     const double it = scal[S_ITER] + 1.0;
     scal[S_RZ] = rz1;
     scal[S_RR] = rr;
     scal[S_BETA] = beta;
     scal[S_ITER] = it;
-    if (conv) *done = CG_CONVERGED;
-    else if (bad) *done = CG_NOT_FINITE;
-    else if (indef) *done = CG_BREAKDOWN;
+    int st = CG_RUNNING;
+    if (conv) st = CG_CONVERGED;
+    else if (bad) st = CG_NOT_FINITE;
+    else if (indef) st = CG_BREAKDOWN;
     else if (stall_window > 0) {
         if (rr <= 0.5 * scal[S_RR_BEST]) {
             scal[S_RR_BEST] = rr;
             scal[S_IT_BEST] = it;
         } else if (it - scal[S_IT_BEST] >= (double)stall_window) {
-            *done = CG_STAGNATED;
+            st = CG_STAGNATED;
         }
     }
+    if (st != CG_RUNNING) *done = st;
+    return st;
 }
 
 // workgroups are dispatched to the 8 XCDs round robin (workgroup w to XCD
@@ -828,7 +832,8 @@ __global__ __launch_bounds__(CG_BLOCK) void k_cg_update(int64_t n3, float* __res
 template <int PC>
 __global__ __launch_bounds__(CG_P_BLOCK) void k_cg_p(int64_t n3, const float* __restrict__ z, const float* __restrict__ r,
                                               float* __restrict__ p, double* scal, const double* __restrict__ part, int n_part,
-                                              int* done, int test_kind, double tol2, int rz_guard, int stall_window)
+                                              int* done, int test_kind, double tol2, int rz_guard, int stall_window,
+                                              CgSlot* slot)
 {
     const int stop = vload(done);    // tested after the loads and the reduction are issued
     const int64_t nn = n3 / 3;
@@ -844,13 +849,26 @@ __global__ __launch_bounds__(CG_P_BLOCK) void k_cg_p(int64_t n3, const float* __
     const double rz0 = scal[S_RZ0], rr0 = scal[S_RR0], rz_prev = scal[S_RZ_PREV];
     double t[2];
     sum_partials<2, CG_P_BLOCK>(part, n_part, n_part, t);
-    if (stop) return;
+    const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+    if (stop) {
+        if (slot && lead) {
+            slot->rr_rel = 0.0;
+            slot->done = stop;
+        }
+        return;
+    }
     const double rz1 = t[0], rr = t[1];
     const bool conv = test_kind == 0 ? (fabs(rz1) <= tol2 * fabs(rz0)) : (rr <= tol2 * rr0);
     const bool bad = !(rr == rr) || !(rz1 == rz1);
     const bool indef = rz_guard && !conv && !bad && !(rz1 > 0.0);
     const double beta = rz1 / rz_prev;
-    if (blockIdx.x == 0 && threadIdx.x == 0) cg_test_record(scal, done, rz1, rr, beta, conv, bad, indef, stall_window);
+    if (lead) {
+        const int st = cg_test_record(scal, done, rz1, rr, beta, conv, bad, indef, stall_window);
+        if (slot) {
+            slot->rr_rel = rr / rr0;
+            slot->done = st;
+        }
+    }
     if (conv || bad || indef) return;
     const float fb = (float)beta;
     for (; i < nn; i += stride) {
@@ -1488,9 +1506,9 @@ hipError_t launch_cg_test(int pc, int64_t n3, const float* z, const float* r, fl
 {
     const int gp = grid_for(n3 / 3, CG_P_BLOCK, CG_GRID);
     if (pc) hipLaunchKernelGGL(k_cg_p<1>, dim3(gp), dim3(CG_P_BLOCK), 0, st, n3, z, r, p, scal, part, cg_grid(n3 / 3), done,
-                               test_kind, tol2, rz_guard, stall_window);
+                               test_kind, tol2, rz_guard, stall_window, nullptr);
     else hipLaunchKernelGGL(k_cg_p<0>, dim3(gp), dim3(CG_P_BLOCK), 0, st, n3, z, r, p, scal, part, cg_grid(n3 / 3), done,
-                            test_kind, tol2, 0, stall_window);
+                            test_kind, tol2, 0, stall_window, nullptr);
     return hipGetLastError();
 }
 
@@ -1528,11 +1546,12 @@ hipError_t launch_cg_update_mg(int64_t n3, float* x, float* r, float* z, float* 
 }
 
 hipError_t launch_cg_p_only(int64_t n3, const float* z, const float* r, float* p, double* scal, const double* part,
-                            int* done, int test_kind, double tol2, int rz_guard, int stall_window, hipStream_t st)
+                            int* done, int test_kind, double tol2, int rz_guard, int stall_window, hipStream_t st,
+                            CgSlot* slot)
 {
     const int gp = grid_for(n3 / 3, CG_P_BLOCK, CG_GRID);
     hipLaunchKernelGGL(k_cg_p<1>, dim3(gp), dim3(CG_P_BLOCK), 0, st, n3, z, r, p, scal, part, cg_grid(n3 / 3), done,
-                       test_kind, tol2, rz_guard, stall_window);
+                       test_kind, tol2, rz_guard, stall_window, slot);
     return hipGetLastError();
 }
 

@@ -200,11 +200,21 @@ hipError_t launch_cg_update_only(int64_t n3, float* x, float* r, const float* p,
 hipError_t launch_cg_update_mg(int64_t n3, float* x, float* r, float* z, float* z2, const float* p, const float* q,
                                const float* minv, float c2, const int2* node_q, const float* partial, double* scal,
                                const double* pq_part, int n_pq, int* done, int guard, hipStream_t st);
+// The outcome of one CG test, written by the direction kernel into pinned
+// host memory (a device pointer to it), so the host reads it after the
+// kernel's completion event without a copy: the flag after the test and
+// r.r / r0.r0 (0 once the flag was already set)
+struct CgSlot {
+    int done;
+    int pad;
+    double rr_rel;
+};
 // rz_guard: stop with CG_BREAKDOWN at r.z <= 0; stall_window > 0: stop with
 // CG_STAGNATED when r.r has not fallen below half its last reference value
-// within that many iterations
+// within that many iterations; slot (may be null): the test's outcome
 hipError_t launch_cg_p_only(int64_t n3, const float* z, const float* r, float* p, double* scal, const double* part,
-                            int* done, int test_kind, double tol2, int rz_guard, int stall_window, hipStream_t st);
+                            int* done, int test_kind, double tol2, int rz_guard, int stall_window, hipStream_t st,
+                            CgSlot* slot = nullptr);
 hipError_t launch_cg_scalars(const double* part, int n, double* scal, int* done, hipStream_t st);
 hipError_t launch_dots3(int64_t n3, const double* a, const double* b, const float* d, double* scal, int slot, double* part,
                         hipStream_t st);

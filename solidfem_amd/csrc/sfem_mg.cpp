@@ -443,8 +443,9 @@ CgResult mg_pcg(sfem_ctx* c, int maxit, double tol2, int64_t* hv_evals)
     float* z = c->d_z32.as<float>();
     float* p = c->d_p32.as<float>();
     float* q = c->d_q32.as<float>();
-    if (!c->h_flag) {
-        HIPCK(hipHostMalloc((void**)&c->h_flag, 2 * sizeof(int), hipHostMallocDefault));
+    if (!c->h_slot) {
+        HIPCK(hipHostMalloc((void**)&c->h_slot, 2 * sizeof(CgSlot), hipHostMallocMapped | hipHostMallocCoherent));
+        HIPCK(hipHostGetDevicePointer((void**)&c->d_slot, c->h_slot, 0));
         for (hipEvent_t& e : c->flag_ev) HIPCK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     }
     HIPCK(launch_cg_init(0, n3, c->d_b32.as<float>(), x, r, z, p, c->d_minv.as<float>(), c->d_fixed.as<uint8_t>(), scal,
@@ -462,6 +463,7 @@ CgResult mg_pcg(sfem_ctx* c, int maxit, double tol2, int64_t* hv_evals)
     }
     HIPCK(launch_cg_scalars(parts, cg_grid(c->nn), scal, done, s));
     HIPCK(launch_mg_copy(n3, z, p, s));
+    int dflag = 0, launched = 0;   // iterations enqueued
     auto iteration = [&]() {
         HIPCK(launch_hessvec_cg(c->material, c->blk, c->d_u32.as<float>(), p, q, c->d_partial.as<float>(), done, pq_elem, s,
                                 nullptr, nullptr));
@@ -480,30 +482,40 @@ CgResult mg_pcg(sfem_ctx* c, int maxit, double tol2, int64_t* hv_evals)
         // positive definite at this state (tangents of near-incompressible
         // material at strongly deformed Newton iterates) shows as r.z <= 0 or
         // stagnation, and the Newton driver falls back to block Jacobi
-        HIPCK(launch_cg_p_only(n3, z, r, p, scal, parts, done, 1, tol2, 1, MG_STALL_WINDOW, s));
+        HIPCK(launch_cg_p_only(n3, z, r, p, scal, parts, done, 1, tol2, 1, MG_STALL_WINDOW, s,
+                               c->d_slot + ((launched + 1) & 1)));
     };
-    // check k: the flag after k iterations, into slot k & 1 (check k + 2 is
-    // recorded only after check k has been read)
-    auto check = [&](int k) {
-        HIPCK(hipMemcpyAsync(c->h_flag + (k & 1), done, sizeof(int), hipMemcpyDeviceToHost, s));
-        HIPCK(hipEventRecord(c->flag_ev[k & 1], s));
-    };
-    int dflag = 0, launched = 0;
-    check(0);
-    if (maxit > 0) {
+    // The host keeps one iteration enqueued beyond the one whose outcome it
+    // waits for (the device never waits for the host), except where the last
+    // two outcomes predict that the iteration in flight converges: then it
+    // waits for that one first, which saves the iteration past the end of the
+    // solve (~70 us of early returns) when the prediction holds and costs one
+    // host round trip when it does not.  Outcome k (after k iterations) is in
+    // slot k & 1: the iteration's direction kernel writes it, or, for k = 0, a
+    // copy of the flag; slot k & 1 is reused by outcome k + 2 only after the
+    // host has read outcome k.
+    auto record = [&](int k) { HIPCK(hipEventRecord(c->flag_ev[k & 1], s)); };
+    HIPCK(hipMemcpyAsync(&c->h_slot[0].done, done, sizeof(int), hipMemcpyDeviceToHost, s));
+    record(0);
+    auto launch = [&]() {
         iteration();
-        launched = 1;
-        check(1);
-    }
+        launched++;
+        record(launched);
+    };
+    if (maxit > 0) launch();
+    double rr_prev = 0.0;
     for (int k = 0;; k++) {
         HIPCK(hipEventSynchronize(c->flag_ev[k & 1]));
-        dflag = c->h_flag[k & 1];
-        if (dflag || k >= launched) break;
-        if (launched < maxit) {
-            iteration();
-            launched++;
-            check(launched);
+        dflag = c->h_slot[k & 1].done;
+        const double rr = k > 0 ? c->h_slot[k & 1].rr_rel : 0.0;
+        if (dflag) break;
+        if (launched == k) {    // nothing in flight: the prediction held the next one back, or maxit
+            if (launched >= maxit) break;
+            launch();
         }
+        const bool ends = k > 1 && rr > 0.0 && rr_prev > 0.0 && rr * std::min(1.0, rr / rr_prev) <= tol2;
+        if (launched == k + 1 && launched < maxit && !ends) launch();
+        rr_prev = rr;
     }
     double hs[S_COUNT];
     HIPCK(hipMemcpy(hs, scal, sizeof hs, hipMemcpyDeviceToHost));
