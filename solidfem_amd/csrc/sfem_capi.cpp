@@ -444,7 +444,7 @@ int solve_converged(sfem_ctx* c, const sfem_solve_params* p, sfem_solve_stats* s
         const float* minv = learned ? c->d_minv_learned.as<float>() : c->d_minv.as<float>();
         CgResult cg{0, 0, 0.0, 0.0};
         if (multigrid) {
-            if (mg_prepare(c, minv, it == 0)) cg = mg_pcg(c, p->inner_iterations, eta * eta, &st->hessvec_evals);
+            if (mg_prepare(c, minv, it == 0, true)) cg = mg_pcg(c, p->inner_iterations, eta * eta, &st->hessvec_evals);
         } else {
             // learned blocks: also stop at r.z <= 0 (blocks not positive definite)
             cg = pcg(c, minv, pc_kind == SFEM_PC_NONE ? 0 : 1, p->inner_iterations, 1, eta * eta, learned ? 2 : 1,
@@ -947,6 +947,9 @@ int sfem_reset(sfem_ctx* c)
         c->frac = 0.0;
         c->has_prev = c->has_prev2 = false;
         set_time_step(c, 0.0);
+        // the multigrid set-up starts again as on a new context
+        c->mg_eig_ok = false;
+        for (auto& m : c->mg) m->c->mg_eig_ok = false;
         return SFEM_OK;
     });
 }

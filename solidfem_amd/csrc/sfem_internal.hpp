@@ -187,6 +187,12 @@ struct sfem_ctx {
     // sfem_matvec (built at each call)
     int lin_cache = 1;
     DevBuf d_lin;
+    // this context as a smoothed multigrid level: the last vector of its
+    // power iteration and that vector's norm, the next full set-up's start
+    // (mg_eig_ok false: the hashed start)
+    DevBuf d_mg_eig;
+    double mg_eig_norm = 0.0;
+    bool mg_eig_ok = false;
     // this context as a multigrid level under the folded schedule: a second
     // residual and direction, and a second operator image (rows and partial
     // slots)
@@ -277,9 +283,12 @@ void ensure_graph(sfem_ctx* c);
 // K x = b (b in d_b32, x in d_x32) with one V-cycle per iteration.  Without
 // `full`, the Chebyshev bands and the coarsest level's inverse of the last
 // full set-up are kept (the Newton steps of one load step share them); the
-// states and block preconditioners are always the current ones.  False when
-// a level's operator is not finite at this state (no V-cycle possible).
-bool mg_prepare(sfem_ctx* c, const float* minv0, bool full);
+// states and block preconditioners are always the current ones.  `warm`:
+// the eigenvalue estimates start from the previous warm set-up's vectors
+// (Newton solves; the test entry points start from the hashed vector).
+// False when a level's operator is not finite at this state (no V-cycle
+// possible).
+bool mg_prepare(sfem_ctx* c, const float* minv0, bool full, bool warm = false);
 CgResult mg_pcg(sfem_ctx* c, int maxit, double tol2, int64_t* hv_evals);
 // z = V(r) on the finest level (r in d_r32, z in d_z32), after mg_prepare
 void mg_vcycle(sfem_ctx* c);

@@ -38,6 +38,7 @@ constexpr double MG_RATIO = 10.0;       // smoothed band [lmax / ratio, lmax]
 constexpr double MG_COARSE_RATIO = 100.0;
 constexpr double MG_SAFETY = 1.15;      // the power iteration estimates lmax from below
 constexpr int MG_POWER_ITERS = 12;
+constexpr int MG_POWER_ITERS_WARM = 4;  // from the previous full set-up's last vector
 // a multigrid-PCG solve stops (and the Newton driver redoes it with block
 // Jacobi) when it breaks down (p.Kp <= 0 or r.z <= 0: the V-cycle is not
 // positive definite at this state), when r.r has not halved within
@@ -316,7 +317,7 @@ int64_t vcycle_evals(sfem_ctx* f)
 
 namespace sfem {
 
-bool mg_prepare(sfem_ctx* f, const float* minv0, bool full)
+bool mg_prepare(sfem_ctx* f, const float* minv0, bool full, bool warm_start)
 {
     hipStream_t s = f->stream;
     const size_t n3 = 3 * (size_t)f->nn;
@@ -372,9 +373,13 @@ bool mg_prepare(sfem_ctx* f, const float* minv0, bool full)
     }
     for (int l = 0; l < nlev; l++)
         if (!(l == nlev - 1 && dense)) build_jacobi(f->mg[l]->c);
-    // lmax(M A) per level: K unnormalised power steps from a hashed start
-    // (the growth of K steps stays far inside FP32 range for lmax of a few
-    // units), lmax ~ |v_K| / |v_{K-1}|
+    // lmax(M A) per level: K unnormalised power steps (the growth of K steps
+    // stays far inside FP32 range for lmax of a few units), lmax ~
+    // |v_K| / |v_{K-1}|: 12 from a hashed start, or, with warm_start (the
+    // Newton solves), 4 from the previous warm set-up's last vector scaled to
+    // unit norm (the tangents of consecutive load steps differ little, so it
+    // is already close to the top eigenvector; sfem_reset and changes of the
+    // hierarchy forget it)
     f->mg_evals = vcycle_evals(f);
     if (!full) return true;
     double* norms = f->d_mg_norm.as<double>();
@@ -382,16 +387,23 @@ bool mg_prepare(sfem_ctx* f, const float* minv0, bool full)
     for (int l = 0; l < nest; l++) {
         const LevelView L = level(f, l);
         sfem_ctx* c = L.c;
+        const int64_t n3c = 3 * c->nn;
         float* v = L.d;
         float* y = L.r;
-        HIPCK(launch_mg_start(c->nn, c->d_fixed.as<uint8_t>(), v, s));
-        for (int it = 0; it < MG_POWER_ITERS; it++) {
+        const bool warm = warm_start && c->mg_eig_ok;
+        if (warm) HIPCK(launch_mg_scale(n3c, c->d_mg_eig.as<float>(), (float)(1.0 / c->mg_eig_norm), v, s));
+        else HIPCK(launch_mg_start(c->nn, c->d_fixed.as<uint8_t>(), v, s));
+        for (int it = 0; it < (warm ? MG_POWER_ITERS_WARM : MG_POWER_ITERS); it++) {
             matvec(L, v, L.q, c->blk.zero_flag);
             HIPCK(launch_mg_power(c->nn, L.q, c->d_node_info.as<int2>(), c->d_partial.as<float>(), L.minv, y, s));
             std::swap(v, y);
         }
-        HIPCK(launch_mg_sqnorm(3 * c->nn, v, f->d_mg_part.as<double>(), norms + 2 * l, s));
-        HIPCK(launch_mg_sqnorm(3 * c->nn, y, f->d_mg_part.as<double>(), norms + 2 * l + 1, s));
+        HIPCK(launch_mg_sqnorm(n3c, v, f->d_mg_part.as<double>(), norms + 2 * l, s));
+        HIPCK(launch_mg_sqnorm(n3c, y, f->d_mg_part.as<double>(), norms + 2 * l + 1, s));
+        if (warm_start) {
+            if (c->d_mg_eig.n < (size_t)n3c * sizeof(float)) c->d_mg_eig.alloc((size_t)n3c * sizeof(float));
+            HIPCK(launch_mg_copy(n3c, v, c->d_mg_eig.as<float>(), s));
+        }
     }
     if (side) HIPCK(hipStreamWaitEvent(s, f->mg_ev[1], 0));
     std::vector<double> h(2 * nest);
@@ -404,12 +416,18 @@ bool mg_prepare(sfem_ctx* f, const float* minv0, bool full)
         lm[l] = h[2 * l + 1] > 0 ? std::sqrt(h[2 * l] / h[2 * l + 1]) : 0.0;
         if (!(lm[l] > 0) || !std::isfinite(lm[l])) {
             f->mg_ready = false;
+            for (int j = 0; j < nest; j++) level(f, j).c->mg_eig_ok = false;
             return false;
         }
     }
     for (int l = 0; l < nest; l++) {
         if (l == 0) f->mg_lmax0 = lm[l];
         else f->mg[l - 1]->lmax = lm[l];
+        if (warm_start) {
+            sfem_ctx* c = level(f, l).c;
+            c->mg_eig_norm = std::sqrt(h[2 * l]);
+            c->mg_eig_ok = c->mg_eig_norm > 0 && std::isfinite(c->mg_eig_norm);
+        }
     }
     f->mg_ready = true;
     return true;
@@ -621,6 +639,7 @@ int sfem_mg_add_level(sfem_ctx* f, const sfem_mesh* coarse, const int32_t* injec
         m->t_val.upload(tval);
         f->mg.push_back(std::move(m));
         f->mg_ready = false;
+        f->mg_eig_ok = false;
         return SFEM_OK;
     });
 }
@@ -633,6 +652,7 @@ int sfem_mg_clear(sfem_ctx* f)
         if (f->mg_side) HIPCK(hipStreamSynchronize(f->mg_side));
         f->mg.clear();
         f->mg_ready = false;
+        f->mg_eig_ok = false;
         return SFEM_OK;
     });
 }

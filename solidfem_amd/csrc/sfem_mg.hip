@@ -267,6 +267,13 @@ __global__ __launch_bounds__(MG_BLOCK) void k_mg_copy(int64_t n, const float* __
     if (i < n) b[i] = a[i];
 }
 
+__global__ __launch_bounds__(MG_BLOCK) void k_mg_scale(int64_t n, const float* __restrict__ a, float sc,
+                                                       float* __restrict__ b)
+{
+    const int64_t i = blockIdx.x * (int64_t)MG_BLOCK + threadIdx.x;
+    if (i < n) b[i] = sc * a[i];
+}
+
 // ------------------------------------------- direct solve on the coarsest level
 // Its dense operator (n = 3 nn dofs, n <= MG_DENSE_MAX) at the current state:
 // element matrices, a fixed-order gather into the dense matrix, and an
@@ -676,6 +683,12 @@ hipError_t launch_mg_dense_solve(int n, const float* Ainv, const float* b, float
 hipError_t launch_mg_copy(int64_t n, const float* a, float* b, hipStream_t st)
 {
     hipLaunchKernelGGL(k_mg_copy, dim3(mg_grid(n)), dim3(MG_BLOCK), 0, st, n, a, b);
+    return hipGetLastError();
+}
+
+hipError_t launch_mg_scale(int64_t n, const float* a, float sc, float* b, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_mg_scale, dim3(mg_grid(n)), dim3(MG_BLOCK), 0, st, n, a, sc, b);
     return hipGetLastError();
 }
 

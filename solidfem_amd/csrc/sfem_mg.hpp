@@ -38,6 +38,8 @@ hipError_t launch_mg_sqnorm(int64_t n3, const float* a, double* part, double* ou
 // r.z, r.r partials in the CG update's layout (2 x cg_grid(nn))
 hipError_t launch_mg_rz(int64_t nn, const float* r, const float* z, double* part, const int* done, hipStream_t st);
 hipError_t launch_mg_copy(int64_t n, const float* a, float* b, hipStream_t st);
+// b = sc a
+hipError_t launch_mg_scale(int64_t n, const float* a, float sc, float* b, hipStream_t st);
 // dense direct solve of the coarsest level: element matrices (12 x 12 per
 // element), the dense operator (n = 3 nn), its inverse, x = Ainv b
 hipError_t launch_mg_elem_matrix(int mat, int64_t ne, const int4* tets, const float* u, const float* dminv,
