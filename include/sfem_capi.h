@@ -93,7 +93,8 @@ typedef struct {
     int32_t outer_iterations;  /* FemConfig::mOuterIterations; CONVERGED: max Newton steps */
     int32_t inner_iterations;  /* FemConfig::mInnerIterations; max CG iterations per step */
     double tol;                /* CONVERGED: relative FP64 residual target (e.g. 1e-8) */
-    double cg_rtol;            /* CONVERGED: inner forcing ||r_cg|| <= cg_rtol ||b|| */
+    double cg_rtol;            /* CONVERGED: cap of the Eisenstat-Walker forcing ||r_cg|| <= eta ||b||;
+                                  <= 0: 0.3 with the multigrid preconditioners, else 0.1 */
     int32_t line_search;       /* CONVERGED: 1 = FP64 back-tracking on ||r|| */
     int32_t check_every;       /* CG iterations between host convergence checks (>=1) */
     int32_t sim_type;          /* sfem_step only: SFEM_ST_STATIC | SFEM_ST_QUASI_STATIC */

@@ -164,7 +164,7 @@ def _p(a, t=_dp):
     return None if a is None else a.ctypes.data_as(t)
 
 
-def solve_params(mode=SOLVE_CONVERGED, precond=PC_JACOBI, outer=50, inner=20000, tol=1e-8, cg_rtol=1e-1,
+def solve_params(mode=SOLVE_CONVERGED, precond=PC_JACOBI, outer=50, inner=20000, tol=1e-8, cg_rtol=0.0,
                  line_search=1, check_every=16, sim_type=ST_STATIC, force_step=0.1, abs_tol=0.0, predictor=2,
                  time_step=0.016, substeps=1, pc_newton_cap=0):
     return SolveParams(mode, precond, outer, inner, tol, cg_rtol, line_search, check_every, sim_type, force_step,

@@ -93,7 +93,8 @@ PRECONDS = {"none": capi.PC_NONE, "jacobi": capi.PC_JACOBI, "learned": capi.PC_L
             "multigrid": capi.PC_MULTIGRID, "multigrid_learned": capi.PC_MULTIGRID_LEARNED}
 
 
-def run(cfg, precond=capi.PC_JACOBI, tol=1e-8, grid=None, max_newton=50, log=None, predictor=2, material=None):
+def run(cfg, precond=capi.PC_JACOBI, tol=1e-8, grid=None, max_newton=50, log=None, predictor=2, material=None,
+        cg_rtol=0.0):
     """Solves workload `cfg`; `log(str)` (optional) gets one line per load step.
     The multigrid preconditioners get the beam's nested hierarchy, the learned
     ones the shipped weights."""
@@ -108,7 +109,7 @@ def run(cfg, precond=capi.PC_JACOBI, tol=1e-8, grid=None, max_newton=50, log=Non
         net.attach(s)
         s._net = net   # keep the network alive with the solver
     steps = d["load_steps"]
-    p = capi.solve_params(tol=tol, precond=precond, cg_rtol=0.1, outer=max_newton, inner=200000, check_every=32,
+    p = capi.solve_params(tol=tol, precond=precond, cg_rtol=cg_rtol, outer=max_newton, inner=200000, check_every=32,
                           sim_type=capi.ST_QUASI_STATIC if steps > 1 else capi.ST_STATIC, force_step=1.0 / steps,
                           predictor=predictor)
     out = {"config": cfg, "description": DESCRIPTIONS[cfg], "tets": d["mesh"].ne, "nodes": d["mesh"].nn,

@@ -418,7 +418,10 @@ int solve_converged(sfem_ctx* c, const sfem_solve_params* p, sfem_solve_stats* s
     const bool adaptive = pc_kind == SFEM_PC_LEARNED || multigrid;
     int maxn = p->outer_iterations > 0 ? p->outer_iterations : 50;
     if (adaptive && p->pc_newton_cap > 0) maxn = std::min(maxn, p->pc_newton_cap);
-    const double eta_max = p->cg_rtol > 0 ? p->cg_rtol : 1e-1;
+    // the forcing cap: 0.3 for the multigrid PCG, whose few iterations per Newton
+    // step make a looser cap pay, 0.1 for the Jacobi-type PCGs (with 0.3 the
+    // nu = 0.495 beam's first load step no longer converged in 50 Newton steps)
+    const double eta_max = p->cg_rtol > 0 ? p->cg_rtol : (multigrid ? 0.3 : 0.1);
     double eta = eta_max, rn_prev = 0;
     const double abs_tol = p->abs_tol > 0 ? p->abs_tol : 0.0;
     auto target = [&](double e) { return std::max(p->tol * e, abs_tol); };
@@ -473,6 +476,12 @@ int solve_converged(sfem_ctx* c, const sfem_solve_params* p, sfem_solve_stats* s
             if (trace_on()) std::fprintf(stderr, "sfem   fallback jacobi: cg %d status %d\n", cg.iters, cg.status);
         }
         lin.end();
+        if (cg.status == CG_BREAKDOWN && cg.iters == 0) {
+            // non-positive curvature along the first direction: no direction
+            // at all, and every further Newton step would repeat this one
+            if (trace_on()) std::fprintf(stderr, "sfem newton %d: no direction (breakdown at the first CG step)\n", it);
+            break;
+        }
         const double R0 = dots3(c, c->d_r64.as<double>(), nullptr, c->d_x32.as<float>()).da;
         double alpha = 1.0;
         Dots dt{0, 0, 0};

@@ -218,15 +218,15 @@ def test_learned_smoother_gain_in_multigrid():
     workload 2 at full size (64x16x16 Neo-Hookean beam, follower pressure, 10
     load steps to 1e-8) with the shipped smoother weights in the finest level's
     Chebyshev smoother against the same V-cycle with block-Jacobi smoothing,
-    under both load-step predictors.  From secant-predicted starts the network
-    saves ~6% of the PCG iterations; from the quadratic predictor's (the
-    default), whose Newton steps need fewer and easier linear solves, ~1%.  On
-    the 786k-tet bench beam it needs ~6% more; tests/test_mg_bound_cpu.py
-    shows why no per-node block network gains much more in this V-cycle."""
+    under both load-step predictors: the network saves ~2.5% of the PCG
+    iterations (770 vs 790 from secant-predicted starts, 707 vs 726 from the
+    quadratic predictor's, the default; asserted: at least 1.5%).  On the
+    786k-tet bench beam it needs ~3% more; tests/test_mg_bound_cpu.py shows
+    why no per-node block network gains much more in this V-cycle."""
     from solidfem_amd import configs, learned
     if not os.path.exists(learned.SMOOTHER_WEIGHTS):
         pytest.skip("no trained smoother weights")
-    for pred, gain in ((1, 0.97), (2, 1.0)):
+    for pred, gain in ((1, 0.985), (2, 0.985)):
         res = {}
         for pc in (capi.PC_MULTIGRID, capi.PC_MULTIGRID_LEARNED):
             out, d, s = configs.run(2, pc, predictor=pred)
