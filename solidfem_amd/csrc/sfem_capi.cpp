@@ -482,7 +482,12 @@ int solve_converged(sfem_ctx* c, const sfem_solve_params* p, sfem_solve_stats* s
             if (trace_on()) std::fprintf(stderr, "sfem newton %d: no direction (breakdown at the first CG step)\n", it);
             break;
         }
-        const double R0 = dots3(c, c->d_r64.as<double>(), nullptr, c->d_x32.as<float>()).da;
+        // the slope R(0) = dx . r(u): its dots are read with the first trial's
+        // (one host round trip fewer per Newton step)
+        HIPCK(launch_dots3(3 * c->nn, c->d_r64.as<double>(), nullptr, c->d_x32.as<float>(), c->d_scal.as<double>(), S_SL,
+                           c->d_partials.as<double>(), c->stream));
+        double R0 = 0.0;
+        bool have_r0 = false;
         double alpha = 1.0;
         Dots dt{0, 0, 0};
         bool finite = false;
@@ -492,7 +497,16 @@ int solve_converged(sfem_ctx* c, const sfem_solve_params* p, sfem_solve_stats* s
                                     c->d_u32.as<float>(), c->stream));
             residual(c, c->d_ut64.as<double>(), c->d_f64tmp.as<double>());
             st->force_evals += c->ne;
-            dt = dots3(c, c->d_f64tmp.as<double>(), c->d_ext64.as<double>(), c->d_x32.as<float>());
+            HIPCK(launch_dots3(3 * c->nn, c->d_f64tmp.as<double>(), c->d_ext64.as<double>(), c->d_x32.as<float>(),
+                               c->d_scal.as<double>(), S_NT, c->d_partials.as<double>(), c->stream));
+            double h[S_COUNT];
+            HIPCK(hipMemcpyAsync(h, c->d_scal.p, sizeof h, hipMemcpyDeviceToHost, c->stream));
+            HIPCK(hipStreamSynchronize(c->stream));
+            dt = Dots{h[S_NT], h[S_NT + 1], h[S_NT + 2]};
+            if (!have_r0) {
+                R0 = h[S_SL + 2];
+                have_r0 = true;
+            }
             finite = std::isfinite(dt.da) && std::isfinite(dt.aa);
         };
         for (int ls = 0; ls < ntrial; ls++) {

@@ -30,10 +30,11 @@ constexpr int CG_P_BLOCK = SFEM_CG_P_BLOCK;
 
 // FP64 scalars kept on the device during a CG solve
 // S_RR_BEST / S_IT_BEST: the stagnation test's reference r.r and the iteration
-// it was reached at (read and written by one thread of k_cg_p only)
+// it was reached at (read and written by one thread of k_cg_p only); S_SL..+2:
+// the Newton line search's slope at its start (three dots3 slots)
 enum Scal {
     S_RZ = 0, S_RZ0, S_RR, S_RR0, S_PQ, S_ALPHA, S_BETA, S_ITER, S_NT, S_NT2, S_NT3, S_RZ_PREV, S_RR_BEST, S_IT_BEST,
-    S_COUNT = 16
+    S_SL, S_COUNT = 20
 };
 // CG status in the done flag: 0 running / iteration limit, 1 converged,
 // 3 breakdown (p.Kp <= 0, or r.z <= 0: operator or preconditioner not
