@@ -622,7 +622,7 @@ int sfem_create(const sfem_mesh* m, const sfem_material* mat, sfem_ctx** out)
         zero<float>(c->d_bd32, 9 * (size_t)nn, s);
         zero<double>(c->d_scal, S_COUNT, s);
         zero<double>(c->d_partials, std::max<size_t>(3 * (size_t)RED_GRID, 2 * (size_t)CG_GRID + c->nblk), s);
-        zero<int>(c->d_done, 4, s);
+        zero<int>(c->d_done, 4, s);   // [0] the CG flag; [1] stays 0 (the multigrid PCG copies [0..1])
         HIPCK(hipStreamSynchronize(s));
         return SFEM_OK;
     });

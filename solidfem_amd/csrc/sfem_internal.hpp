@@ -266,7 +266,7 @@ Dots dots3(sfem_ctx* c, const double* a, const double* b, const float* d);
 struct CgResult {
     int iters;
     int status;   // 1 converged, 0 max iterations, 3 breakdown, 4 NaN
-    double rr0, rr;
+    double rr0, rr;   // r0.r0 and the last r.r (pcg; the multigrid PCG leaves them 0)
 };
 // on_check(iters) runs after every check_every iterations while not done
 typedef void (*CgHook)(sfem_ctx* c, int iters, void* user);

@@ -846,13 +846,14 @@ __global__ __launch_bounds__(CG_P_BLOCK) void k_cg_p(int64_t n3, const float* __
         for (int c = 0; c < 3; c++) { zv[c] = zz[3 * i + c]; pv[c] = p[3 * i + c]; }
     }
     // the scalars of the previous kernels: loaded with the first round
-    const double rz0 = scal[S_RZ0], rr0 = scal[S_RR0], rz_prev = scal[S_RZ_PREV];
+    const double rz0 = scal[S_RZ0], rr0 = scal[S_RR0], rz_prev = scal[S_RZ_PREV], it_prev = scal[S_ITER];
     double t[2];
     sum_partials<2, CG_P_BLOCK>(part, n_part, n_part, t);
     const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
     if (stop) {
         if (slot && lead) {
             slot->rr_rel = 0.0;
+            slot->iter = (int)it_prev;
             slot->done = stop;
         }
         return;
@@ -866,6 +867,7 @@ __global__ __launch_bounds__(CG_P_BLOCK) void k_cg_p(int64_t n3, const float* __
         const int st = cg_test_record(scal, done, rz1, rr, beta, conv, bad, indef, stall_window);
         if (slot) {
             slot->rr_rel = rr / rr0;
+            slot->iter = (int)it_prev + 1;
             slot->done = st;
         }
     }

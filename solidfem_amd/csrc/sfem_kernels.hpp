@@ -203,11 +203,11 @@ hipError_t launch_cg_update_mg(int64_t n3, float* x, float* r, float* z, float* 
                                const double* pq_part, int n_pq, int* done, int guard, hipStream_t st);
 // The outcome of one CG test, written by the direction kernel into pinned
 // host memory (a device pointer to it), so the host reads it after the
-// kernel's completion event without a copy: the flag after the test and
-// r.r / r0.r0 (0 once the flag was already set)
+// kernel's completion event without a copy: the flag after the test, the
+// iterations counted (S_ITER) and r.r / r0.r0 (0 once the flag was already set)
 struct CgSlot {
     int done;
-    int pad;
+    int iter;
     double rr_rel;
 };
 // rz_guard: stop with CG_BREAKDOWN at r.z <= 0; stall_window > 0: stop with

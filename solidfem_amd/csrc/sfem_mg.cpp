@@ -513,7 +513,10 @@ CgResult mg_pcg(sfem_ctx* c, int maxit, double tol2, int64_t* hv_evals)
     // copy of the flag; slot k & 1 is reused by outcome k + 2 only after the
     // host has read outcome k.
     auto record = [&](int k) { HIPCK(hipEventRecord(c->flag_ev[k & 1], s)); };
-    HIPCK(hipMemcpyAsync(&c->h_slot[0].done, done, sizeof(int), hipMemcpyDeviceToHost, s));
+    // outcome 0: the flag and 0 iterations (the flag word after it, which
+    // stays 0), copied in stream order (an iteration of the previous solve may
+    // still write this slot)
+    HIPCK(hipMemcpyAsync(&c->h_slot[0], done, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
     record(0);
     auto launch = [&]() {
         iteration();
@@ -522,9 +525,11 @@ CgResult mg_pcg(sfem_ctx* c, int maxit, double tol2, int64_t* hv_evals)
     };
     if (maxit > 0) launch();
     double rr_prev = 0.0;
+    int iters = 0;
     for (int k = 0;; k++) {
         HIPCK(hipEventSynchronize(c->flag_ev[k & 1]));
         dflag = c->h_slot[k & 1].done;
+        iters = c->h_slot[k & 1].iter;
         const double rr = k > 0 ? c->h_slot[k & 1].rr_rel : 0.0;
         if (dflag) break;
         if (launched == k) {    // nothing in flight: the prediction held the next one back, or maxit
@@ -535,13 +540,12 @@ CgResult mg_pcg(sfem_ctx* c, int maxit, double tol2, int64_t* hv_evals)
         if (launched == k + 1 && launched < maxit && !ends) launch();
         rr_prev = rr;
     }
-    double hs[S_COUNT];
-    HIPCK(hipMemcpy(hs, scal, sizeof hs, hipMemcpyDeviceToHost));
+    // no read-back: the outcomes carry the status and the iteration count, and
+    // what the caller launches next queues behind an iteration still in flight
     CgResult res;
-    res.iters = (int)hs[S_ITER];
+    res.iters = iters;
     res.status = dflag;
-    res.rr0 = hs[S_RR0];
-    res.rr = hs[S_RR];
+    res.rr0 = res.rr = 0.0;
     if (hv_evals) *hv_evals += (int64_t)res.iters * (c->ne + c->mg_evals) + c->mg_evals;
     return res;
 }
