@@ -162,6 +162,60 @@ static std::vector<int> bank_order(const std::vector<int>& lsorted, int64_t nel,
     return order;
 }
 
+// extra cycles of one lane group's read of rows l[0..n) (distinct rows of one
+// residue class are served one after the other)
+static int group_conflicts(const int* l, int n)
+{
+    constexpr int R = SFEM_ROW_CLASSES;
+    int cnt[R] = {0}, best = 0;
+    for (int i = 0; i < n; i++) {
+        bool dup = false;
+        for (int k = 0; k < i; k++) dup = dup || l[k] == l[i];
+        if (!dup) best = std::max(best, ++cnt[l[i] % R]);
+    }
+    return best > 0 ? best - 1 : 0;
+}
+
+// The elements of one block in lane groups: each group of GROUP_LANES
+// consecutive elements reads its four corners' staged rows one corner per
+// instruction, so a group's rows at one corner should fall in distinct
+// residue classes.  Greedy: each lane takes, among the next WINDOW unplaced
+// elements, the first that adds the fewest conflicts.  lid4: the elements'
+// local ids (4 per element, in their current order) -> perm[new position] =
+// current position.
+static std::vector<int> lane_groups(const std::vector<int>& lid4, int n)
+{
+    constexpr int L = SFEM_GROUP_LANES, WINDOW = 64;
+    std::vector<int> perm, rest(n);
+    perm.reserve(n);
+    for (int i = 0; i < n; i++) rest[i] = i;
+    while (!rest.empty()) {
+        int grp[L], m = 0;
+        while (m < L && !rest.empty()) {
+            int best = 0, bc = 1 << 30;
+            for (int c = 0; c < (int)rest.size() && c < WINDOW; c++) {
+                int add = 0;
+                for (int v = 0; v < 4; v++) {
+                    int l[L];
+                    for (int i = 0; i < m; i++) l[i] = lid4[4 * grp[i] + v];
+                    const int before = group_conflicts(l, m);
+                    l[m] = lid4[4 * rest[c] + v];
+                    add += group_conflicts(l, m + 1) - before;
+                }
+                if (add < bc) {
+                    bc = add;
+                    best = c;
+                    if (add == 0) break;
+                }
+            }
+            grp[m++] = rest[best];
+            perm.push_back(rest[best]);
+            rest.erase(rest.begin() + best);
+        }
+    }
+    return perm;
+}
+
 Partition make_partition(const int32_t* tets, int64_t ne, const double* X0, int64_t nn, const uint8_t* held,
                          const std::vector<double>& mu, const std::vector<double>& lam, const std::vector<double>& dminv,
                          const std::vector<double>& vol, int BE)
@@ -223,6 +277,19 @@ Partition make_partition(const int32_t* tets, int64_t ne, const double* X0, int6
         const std::vector<int> order = bank_order(lsorted, e1 - e0, nb);
         lid.assign(nb, 0);
         for (int j = 0; j < nb; j++) lid[order[j]] = j;
+        // the block's elements in conflict-free lane groups where possible
+        {
+            const int n = (int)(e1 - e0);
+            std::vector<int> lid4(4 * (size_t)n);
+            for (int i = 0; i < 4 * n; i++) lid4[i] = lid[lsorted[i]];
+            const std::vector<int> perm = lane_groups(lid4, n);
+            const std::vector<std::pair<uint64_t, int64_t>> kb(key.begin() + e0, key.begin() + e1);
+            const std::vector<int> lsb(lsorted);
+            for (int i = 0; i < n; i++) {
+                key[e0 + i] = kb[perm[i]];
+                for (int v = 0; v < 4; v++) lsorted[4 * i + v] = lsb[4 * perm[i] + v];
+            }
+        }
         for (int j = 0; j < nb; j++) {
             const int id = ids[order[j]];
             P.bnodes.push_back((uint32_t)id | (held[id] ? F_FIXED : 0u));
@@ -251,16 +318,36 @@ Partition make_partition(const int32_t* tets, int64_t ne, const double* X0, int6
         }
         for (int j = 0; j < nb; j++) cnt[j + 1] += cnt[j];
         for (int j = 0; j <= nb; j++) P.lptr.push_back((uint16_t)cnt[j]);
-        std::vector<int> fill(cnt.begin(), cnt.end() - 1);
-        for (int64_t s = e0; s < e1; s++) {
-            const uint32_t* li = &P.lidx[2 * s];
-            const int l[4] = {(int)(li[0] & 0xffff), (int)(li[0] >> 16), (int)(li[1] & 0xffff), (int)(li[1] >> 16)};
-            // corner v of element s -> its slot in the node-sorted incidence order
-            // (counting sort in element order: a node's slots list its elements in order)
-            uint16_t k[4];
-            for (int v = 0; v < 4; v++) k[v] = (uint16_t)fill[l[v]]++;
-            P.lperm[2 * s] = (uint32_t)k[0] | (uint32_t)k[1] << 16;
-            P.lperm[2 * s + 1] = (uint32_t)k[2] | (uint32_t)k[3] << 16;
+        // corner v of element s -> a slot of its node's range in the node-sorted
+        // incidence order.  The corner writes of one lane group (one corner per
+        // instruction) are served together: each takes the first free slot of
+        // its node whose residue class the group has not written at that corner
+        // yet, else the first free slot.
+        std::vector<std::vector<int>> free_slots(nb);
+        for (int j = 0; j < nb; j++)
+            for (int q = cnt[j]; q < cnt[j + 1]; q++) free_slots[j].push_back(q);
+        for (int64_t g0 = e0; g0 < e1; g0 += SFEM_GROUP_LANES) {
+            const int64_t g1 = std::min<int64_t>(e1, g0 + SFEM_GROUP_LANES);
+            uint16_t k[SFEM_GROUP_LANES][4];
+            for (int v = 0; v < 4; v++) {
+                unsigned used = 0;
+                for (int64_t s = g0; s < g1; s++) {
+                    const uint32_t w = P.lidx[2 * s + v / 2];
+                    const int l = (int)(v % 2 ? w >> 16 : w & 0xffff);
+                    std::vector<int>& F = free_slots[l];
+                    size_t pick = 0;
+                    while (pick < F.size() && (used >> (F[pick] % SFEM_ROW_CLASSES) & 1u)) pick++;
+                    if (pick == F.size()) pick = 0;
+                    const int q = F[pick];
+                    F.erase(F.begin() + (long)pick);
+                    used |= 1u << (q % SFEM_ROW_CLASSES);
+                    k[s - g0][v] = (uint16_t)q;
+                }
+            }
+            for (int64_t s = g0; s < g1; s++) {
+                P.lperm[2 * s] = (uint32_t)k[s - g0][0] | (uint32_t)k[s - g0][1] << 16;
+                P.lperm[2 * s + 1] = (uint32_t)k[s - g0][2] | (uint32_t)k[s - g0][3] << 16;
+            }
         }
     }
     // shared nodes: flag, and list their partial rows in block order
