@@ -133,7 +133,13 @@ def test_loss_and_gradient_match_reference():
     l_ref, g_ref = ref.loss(feat.astype(np.float64), cb.astype(np.float64), R, er, lambda z: p.matvec(m.X, z))
     print(f"loss gpu {loss:.8f} ref {l_ref:.8f}; grad rel err {rel2(grad, g_ref):.3e}")
     assert abs(loss - l_ref) < 1e-4 * abs(l_ref)
-    assert rel2(grad, g_ref) < 1e-6, rel2(grad, g_ref)
+    # the reference rounds its FP64 values to BF16 where the device rounds
+    # FP32 ones: a value within FP32 rounding of a BF16 rounding boundary (a
+    # chance of ~2^-16 per value, over ~1e5 values) rounds the other way and
+    # moves the gradients of the layers before it by ~1e-5 (measured 4.8e-8
+    # without such a flip, 1.4e-5 with one, concentrated in the first round's
+    # layers); without the BF16 emulation the two differ by 2.5e-2
+    assert rel2(grad, g_ref) < 1e-4, rel2(grad, g_ref)
     cos = np.dot(grad, g_ref) / np.linalg.norm(grad) / np.linalg.norm(g_ref)
     assert cos > 0.999
     # every layer receives gradient
