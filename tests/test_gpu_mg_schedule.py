@@ -109,6 +109,31 @@ def test_folded_schedule_newton_solve_bitwise_equal(precond):
     assert np.array_equal(out[0][1], out[1][1])
 
 
+def test_pcg_outcomes_iteration_limit_and_convergence():
+    """the multigrid PCG's host loop reads each iteration's outcome from the
+    slot its direction kernel writes (flag, iteration count) and may hold the
+    next iteration back when convergence is predicted: an iteration limit of
+    k returns k iterations and status 0 (the same iterate as the first k
+    iterations of a longer solve, bitwise), and a converged solve reports the
+    count at which it converged, whatever the limit beyond it"""
+    m, fixed, s = setup((32, 8, 8))
+    s.set_positions(m.X)
+    rng = np.random.default_rng(5)
+    r = rng.standard_normal(m.X.shape) * (1 - fixed)[:, None]
+    xc, itc, stc = s.mg_pcg(r, rtol=1e-6, maxit=200)
+    print(f"converged: {itc} iterations, status {stc}", flush=True)
+    assert stc == 1 and 2 < itc < 200
+    x2, it2, st2 = s.mg_pcg(r, rtol=1e-6, maxit=itc + 1)
+    assert (it2, st2) == (itc, 1) and np.array_equal(x2, xc)
+    for k in (1, 2, itc - 1):
+        xk, itk, stk = s.mg_pcg(r, rtol=1e-6, maxit=k)
+        print(f"maxit {k}: {itk} iterations, status {stk}", flush=True)
+        assert (itk, stk) == (k, 0)
+        assert np.linalg.norm(xk - xc) > 0
+    xa, _, _ = s.mg_pcg(r, rtol=1e-6, maxit=itc - 1)
+    assert np.array_equal(xa, xk)
+
+
 def test_schedule_argument_checked():
     m, fixed, s = setup((16, 4, 4))
     with pytest.raises(capi.SfemError):
