@@ -264,6 +264,14 @@ void build_jacobi(sfem_ctx* c)
     HIPCK(launch_invert_blocks(c->nn, c->d_bd32.as<float>(), c->d_fixed.as<uint8_t>(), c->d_minv.as<float>(), c->stream));
 }
 
+void ensure_cg_slots(sfem_ctx* c)
+{
+    if (c->h_slot) return;
+    HIPCK(hipHostMalloc((void**)&c->h_slot, 2 * sizeof(CgSlot), hipHostMallocMapped | hipHostMallocCoherent));
+    HIPCK(hipHostGetDevicePointer((void**)&c->d_slot, c->h_slot, 0));
+    for (hipEvent_t& e : c->flag_ev) HIPCK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+}
+
 // PCG on K x = b (b already in d_b32); x in d_x32.
 // Two launches per iteration: the element kernel, which from the second
 // iteration on also makes the previous iteration's convergence test and
@@ -298,8 +306,10 @@ CgResult pcg(sfem_ctx* c, const float* minv, int pc, int maxit, int test_kind, d
     int launched = 0;
     int dflag = 0;
     double hs[S_COUNT];
-    while (launched < maxit) {
+    // one batch of up to check_every iterations; its tests write `slot`
+    auto batch = [&](CgSlot* slot) {
         const int k = std::min(check_every, maxit - launched);
+        pf.slot = slot;
         for (int i = 0; i < k; i++) {
             const bool timed = c->tm_every > 0 && c->tm_counter++ % c->tm_every == 0 && c->tm_pending < sfem_ctx::TM_POOL;
             hipEvent_t ev0 = timed ? c->tm_ev[2 * c->tm_pending] : nullptr;
@@ -322,25 +332,83 @@ CgResult pcg(sfem_ctx* c, const float* minv, int pc, int maxit, int test_kind, d
         launched += k;
         if (launched >= maxit)
             HIPCK(launch_cg_test(pc, n3, pf.z, c->d_r32.as<float>(), pbuf[cur], scal, parts, done, test_kind, tol2, rz_guard, 0,
-                                 c->stream));
-        HIPCK(hipMemcpyAsync(&dflag, done, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-        HIPCK(hipStreamSynchronize(c->stream));
-        if (c->tm_pending) {
-            // launches after the converged iteration return at once: keep those that ran
-            double it_done = 1e300;
-            if (dflag) HIPCK(hipMemcpy(&it_done, scal + S_ITER, sizeof(double), hipMemcpyDeviceToHost));
-            for (int k = 0; k < c->tm_pending; k++) {
+                                 c->stream, slot));
+    };
+    // the sampled launch timings of the launches before iteration `end`
+    // (all of them complete), those at or after `upto` dropped (launches
+    // past the converged iteration return at once)
+    auto take_timings = [&](int end, double upto) {
+        int kept = 0;
+        for (int k = 0; k < c->tm_pending; k++) {
+            if (c->tm_iter[k] < end) {
                 float ms = 0.f;
                 HIPCK(hipEventElapsedTime(&ms, c->tm_ev[2 * k], c->tm_ev[2 * k + 1]));
-                if ((double)c->tm_iter[k] < it_done) {
+                if ((double)c->tm_iter[k] < upto) {
                     c->tm_ms += ms;
                     c->tm_samples++;
                 }
+            } else {
+                std::swap(c->tm_ev[2 * kept], c->tm_ev[2 * k]);
+                std::swap(c->tm_ev[2 * kept + 1], c->tm_ev[2 * k + 1]);
+                c->tm_iter[kept++] = c->tm_iter[k];
             }
-            c->tm_pending = 0;
         }
-        if (dflag) break;
-        if (hook) hook(c, launched, user);
+        c->tm_pending = kept;
+    };
+    if (hook) {
+        // the caller acts between batches: one batch at a time
+        while (launched < maxit) {
+            batch(nullptr);
+            HIPCK(hipMemcpyAsync(&dflag, done, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+            HIPCK(hipStreamSynchronize(c->stream));
+            if (c->tm_pending) {
+                double it_done = 1e300;
+                if (dflag) HIPCK(hipMemcpy(&it_done, scal + S_ITER, sizeof(double), hipMemcpyDeviceToHost));
+                take_timings(launched, it_done);
+            }
+            if (dflag) break;
+            hook(c, launched, user);
+        }
+    } else if (maxit > 0) {
+        // One batch stays enqueued beyond the one whose outcome the host waits
+        // for (the device never waits for the host), except where the last two
+        // outcomes predict that the batch in flight converges: then the host
+        // waits for it first, which saves a batch of early returns at the end
+        // of the solve when the prediction holds and costs one host round trip
+        // when it does not.  Batch j's tests write slot j & 1 (reset by the
+        // host before the batch is enqueued: batch j - 2, its last writer, has
+        // completed), and an event follows it.
+        ensure_cg_slots(c);
+        int nenq = 0, bend[2] = {0, 0};
+        auto enqueue = [&]() {
+            CgSlot& hs_ = c->h_slot[nenq & 1];
+            hs_.done = 0;
+            hs_.iter = 0;
+            hs_.rr_rel = 0.0;
+            batch(c->d_slot + (nenq & 1));
+            HIPCK(hipEventRecord(c->flag_ev[nenq & 1], c->stream));
+            bend[nenq & 1] = launched;
+            nenq++;
+        };
+        enqueue();
+        double rr_prev = 0.0;
+        bool hold = false;
+        for (int j = 0;; j++) {
+            if (nenq == j + 1 && launched < maxit && !hold) enqueue();
+            HIPCK(hipEventSynchronize(c->flag_ev[j & 1]));
+            const CgSlot out = c->h_slot[j & 1];
+            dflag = out.done;
+            if (c->tm_pending) take_timings(bend[j & 1], dflag ? (double)out.iter : 1e300);
+            if (dflag) break;
+            if (nenq == j + 1) {      // nothing in flight: held back, or the iteration limit
+                if (launched >= maxit) break;
+                enqueue();
+            }
+            const double rr = out.rr_rel;
+            hold = test_kind == 1 && rr > 0.0 && rr_prev > 0.0 && rr * std::min(1.0, rr / rr_prev) <= tol2;
+            rr_prev = rr;
+        }
+        c->tm_pending = 0;   // launches past the end of the solve
     }
     HIPCK(hipMemcpy(hs, scal, sizeof hs, hipMemcpyDeviceToHost));
     CgResult res;

@@ -290,6 +290,8 @@ void ensure_graph(sfem_ctx* c);
 // possible).
 bool mg_prepare(sfem_ctx* c, const float* minv0, bool full, bool warm = false);
 CgResult mg_pcg(sfem_ctx* c, int maxit, double tol2, int64_t* hv_evals);
+// the CG test-outcome slots (mapped pinned memory) and their events, once per context
+void ensure_cg_slots(sfem_ctx* c);
 // z = V(r) on the finest level (r in d_r32, z in d_z32), after mg_prepare
 void mg_vcycle(sfem_ctx* c);
 

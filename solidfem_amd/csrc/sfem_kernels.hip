@@ -101,9 +101,10 @@ size_t lds_layout_bytes(int BE, int nmax)
 
 // the CG scalars and status of one convergence test (one thread of the grid):
 // the stagnation reference is read and written by that thread only
-// (returns the flag's new value)
+// (returns the flag's new value; the outcome also into slot when given)
 __device__ __forceinline__ int cg_test_record(double* scal, int* done, double rz1, double rr, double beta, bool conv,
-                                              bool bad, bool indef, int stall_window)
+                                              bool bad, bool indef, int stall_window, CgSlot* slot = nullptr,
+                                              double rr0 = 1.0)
 {
     const double it = scal[S_ITER] + 1.0;
     scal[S_RZ] = rz1;
@@ -123,6 +124,11 @@ __device__ __forceinline__ int cg_test_record(double* scal, int* done, double rz
         }
     }
     if (st != CG_RUNNING) *done = st;
+    if (slot) {
+        slot->rr_rel = rr / rr0;
+        slot->iter = (int)it;
+        slot->done = st;
+    }
     return st;
 }
 
@@ -242,7 +248,14 @@ __global__ __launch_bounds__(BE) void k_elem_block(BlockArgs a, const typename K
     // stored to LDS after the node staging
     const uint16_t* lp = a.lptr + (n0 + b);
     const uint16_t lp0 = tid <= nb ? lp[tid] : uint16_t(0);
-    if (stop) return;
+    if (stop) {
+        if (PF && pf.slot && blockIdx.x == 0 && tid == 0) {
+            pf.slot->rr_rel = 0.0;
+            pf.slot->iter = (int)pf.scal[S_ITER];
+            pf.slot->done = stop;
+        }
+        return;
+    }
     // stage block nodes (rows, flags, partial slots): a row's loads, then its
     // LDS stores; the first round of loads is issued before the PF test
     struct Row {
@@ -364,7 +377,8 @@ __global__ __launch_bounds__(BE) void k_elem_block(BlockArgs a, const typename K
         const bool bad = !(rr == rr) || !(rz1 == rz1);
         const bool indef = pf.rz_guard && !conv && !bad && !(rz1 > 0.0);
         const double beta_d = rz1 / rz_prev;
-        if (blockIdx.x == 0 && tid == 0) cg_test_record(pf.scal, done, rz1, rr, beta_d, conv, bad, indef, pf.stall_window);
+        if (blockIdx.x == 0 && tid == 0)
+            cg_test_record(pf.scal, done, rz1, rr, beta_d, conv, bad, indef, pf.stall_window, pf.slot, rr0);
         if (conv || bad || indef) return;
         beta = (float)beta_d;
     }
@@ -863,14 +877,7 @@ __global__ __launch_bounds__(CG_P_BLOCK) void k_cg_p(int64_t n3, const float* __
     const bool bad = !(rr == rr) || !(rz1 == rz1);
     const bool indef = rz_guard && !conv && !bad && !(rz1 > 0.0);
     const double beta = rz1 / rz_prev;
-    if (lead) {
-        const int st = cg_test_record(scal, done, rz1, rr, beta, conv, bad, indef, stall_window);
-        if (slot) {
-            slot->rr_rel = rr / rr0;
-            slot->iter = (int)it_prev + 1;
-            slot->done = st;
-        }
-    }
+    if (lead) cg_test_record(scal, done, rz1, rr, beta, conv, bad, indef, stall_window, slot, rr0);
     if (conv || bad || indef) return;
     const float fb = (float)beta;
     for (; i < nn; i += stride) {
@@ -1504,13 +1511,14 @@ hipError_t launch_hessvec_mg(int mat, const BlockArgs& a, const float* u32, int 
 }
 
 hipError_t launch_cg_test(int pc, int64_t n3, const float* z, const float* r, float* p, double* scal, const double* part,
-                          int* done, int test_kind, double tol2, int rz_guard, int stall_window, hipStream_t st)
+                          int* done, int test_kind, double tol2, int rz_guard, int stall_window, hipStream_t st,
+                          CgSlot* slot)
 {
     const int gp = grid_for(n3 / 3, CG_P_BLOCK, CG_GRID);
     if (pc) hipLaunchKernelGGL(k_cg_p<1>, dim3(gp), dim3(CG_P_BLOCK), 0, st, n3, z, r, p, scal, part, cg_grid(n3 / 3), done,
-                               test_kind, tol2, rz_guard, stall_window, nullptr);
+                               test_kind, tol2, rz_guard, stall_window, slot);
     else hipLaunchKernelGGL(k_cg_p<0>, dim3(gp), dim3(CG_P_BLOCK), 0, st, n3, z, r, p, scal, part, cg_grid(n3 / 3), done,
-                            test_kind, tol2, 0, stall_window, nullptr);
+                            test_kind, tol2, 0, stall_window, slot);
     return hipGetLastError();
 }
 

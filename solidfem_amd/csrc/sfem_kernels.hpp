@@ -140,6 +140,16 @@ hipError_t launch_hessvec_cg(int mat, const BlockArgs& a, const float* u32, cons
 // p = z + beta p_old for its staged rows and applies K to that p; each row's
 // owner (its only block, or the first block of a shared node) stores p into
 // p_new, the other buffer of a pair, while other blocks still read p_old.
+// The outcome of a CG test, written by the kernel that makes it (the
+// direction kernel, or the element kernel with the direction update) into
+// pinned host memory (a device pointer to it), so the host reads it after the
+// kernel's completion event without a copy: the flag after the test, the
+// iterations counted (S_ITER) and r.r / r0.r0 (0 once the flag was already set)
+struct CgSlot {
+    int done;
+    int iter;
+    double rr_rel;
+};
 struct PFuse {
     const float* z;          // z, or r without a preconditioner
     const float* p_old;
@@ -150,6 +160,7 @@ struct PFuse {
     int test_kind;           // as launch_cg_p_only
     double tol2;
     int rz_guard, stall_window;
+    CgSlot* slot;            // the test's outcome (may be null)
 };
 hipError_t launch_hessvec_cg_fused(int mat, const BlockArgs& a, const float* u32, const PFuse& pf, float* out,
                                    float* partial, int* done, double* dot_elem, hipStream_t st, hipEvent_t ev0 = nullptr,
@@ -192,7 +203,8 @@ hipError_t launch_cg_update(int pc, int64_t n3, float* x, float* r, float* z, co
                             double* part, int* done, int guard, hipStream_t st);
 // beta, the convergence test and p from r.z, r.r partials (2 x cg_grid)
 hipError_t launch_cg_test(int pc, int64_t n3, const float* z, const float* r, float* p, double* scal, const double* part,
-                          int* done, int test_kind, double tol2, int rz_guard, int stall_window, hipStream_t st);
+                          int* done, int test_kind, double tol2, int rz_guard, int stall_window, hipStream_t st,
+                          CgSlot* slot = nullptr);
 hipError_t launch_cg_update_only(int64_t n3, float* x, float* r, const float* p, const float* q, const int2* node_q,
                                  const float* partial, double* scal, const double* pq_part, int n_pq, double* part,
                                  int* done, int guard, hipStream_t st);
@@ -201,15 +213,6 @@ hipError_t launch_cg_update_only(int64_t n3, float* x, float* r, const float* p,
 hipError_t launch_cg_update_mg(int64_t n3, float* x, float* r, float* z, float* z2, const float* p, const float* q,
                                const float* minv, float c2, const int2* node_q, const float* partial, double* scal,
                                const double* pq_part, int n_pq, int* done, int guard, hipStream_t st);
-// The outcome of one CG test, written by the direction kernel into pinned
-// host memory (a device pointer to it), so the host reads it after the
-// kernel's completion event without a copy: the flag after the test, the
-// iterations counted (S_ITER) and r.r / r0.r0 (0 once the flag was already set)
-struct CgSlot {
-    int done;
-    int iter;
-    double rr_rel;
-};
 // rz_guard: stop with CG_BREAKDOWN at r.z <= 0; stall_window > 0: stop with
 // CG_STAGNATED when r.r has not fallen below half its last reference value
 // within that many iterations; slot (may be null): the test's outcome

@@ -461,11 +461,7 @@ CgResult mg_pcg(sfem_ctx* c, int maxit, double tol2, int64_t* hv_evals)
     float* z = c->d_z32.as<float>();
     float* p = c->d_p32.as<float>();
     float* q = c->d_q32.as<float>();
-    if (!c->h_slot) {
-        HIPCK(hipHostMalloc((void**)&c->h_slot, 2 * sizeof(CgSlot), hipHostMallocMapped | hipHostMallocCoherent));
-        HIPCK(hipHostGetDevicePointer((void**)&c->d_slot, c->h_slot, 0));
-        for (hipEvent_t& e : c->flag_ev) HIPCK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-    }
+    ensure_cg_slots(c);
     HIPCK(launch_cg_init(0, n3, c->d_b32.as<float>(), x, r, z, p, c->d_minv.as<float>(), c->d_fixed.as<uint8_t>(), scal,
                          parts, done, s));
     // the folded schedule: the update makes the V-cycle's first step and the
