@@ -253,7 +253,7 @@ def main():
     def leg(pc):
         """warmup + timed solves with preconditioner pc -> (wall s, runs, final positions)"""
         params = capi.solve_params(tol=1e-8, precond=pc, cg_rtol=0.0, sim_type=capi.ST_QUASI_STATIC,
-                                   force_step=1.0 / args.load_steps, inner=100000, check_every=32)
+                                   force_step=1.0 / args.load_steps, inner=100000, check_every=16)
 
         def one_solve():
             s.reset()

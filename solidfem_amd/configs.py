@@ -109,7 +109,7 @@ def run(cfg, precond=capi.PC_JACOBI, tol=1e-8, grid=None, max_newton=50, log=Non
         net.attach(s)
         s._net = net   # keep the network alive with the solver
     steps = d["load_steps"]
-    p = capi.solve_params(tol=tol, precond=precond, cg_rtol=cg_rtol, outer=max_newton, inner=200000, check_every=32,
+    p = capi.solve_params(tol=tol, precond=precond, cg_rtol=cg_rtol, outer=max_newton, inner=200000, check_every=16,
                           sim_type=capi.ST_QUASI_STATIC if steps > 1 else capi.ST_STATIC, force_step=1.0 / steps,
                           predictor=predictor)
     out = {"config": cfg, "description": DESCRIPTIONS[cfg], "tets": d["mesh"].ne, "nodes": d["mesh"].nn,
