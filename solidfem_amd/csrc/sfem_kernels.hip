@@ -73,7 +73,8 @@ template <> struct KindTraits<3> {
 //       incidence order, sf4[k * CW + j]  (CW = 1 for W = 3, 3 for W = 9); an
 //       element writes its corner c to the slot lperm[element] gives it, so a
 //       node's incidences are the contiguous slots [s_ptr[j], s_ptr[j+1])
-//   staged node rows: su4[nmax] (and sd4[nmax], sx4[nmax]) = (x, y, z, 0);
+//   staged node rows: su4[nmax] (not on the cached state; and sd4[nmax],
+//   sx4[nmax]) = (x, y, z, 0);
 //   sx4: rest coordinates relative to the block's centre
 //   per row m / h^2 for dynamics s_md[nmax] (T), node flags s_flag[nmax]
 //   (bnodes entry) and partial slots s_slot[nmax]
@@ -88,13 +89,15 @@ __host__ __device__ constexpr int corner_vec4(int W_)
     return (W_ + 3) / 4;
 }
 
+// lc: the kernel on the cached Neo-Hookean state (no staged displacement rows)
 template <int KIND>
-size_t lds_layout_bytes(int BE, int nmax)
+size_t lds_layout_bytes(int BE, int nmax, bool lc = false)
 {
     typedef typename KindTraits<KIND>::T T;
     constexpr int W = KindTraits<KIND>::W;
     const size_t v4 = 4 * sizeof(T);
-    const size_t rows = (size_t)nmax * (1 + (KindTraits<KIND>::HAS_D ? 1 : 0) + (KindTraits<KIND>::HAS_X ? 1 : 0));
+    const size_t rows =
+        (size_t)nmax * ((lc ? 0 : 1) + (KindTraits<KIND>::HAS_D ? 1 : 0) + (KindTraits<KIND>::HAS_X ? 1 : 0));
     return v4 * ((size_t)4 * corner_vec4<KIND>(W) * BE + rows) + (2 * sizeof(uint32_t) + sizeof(T)) * (size_t)nmax +
            sizeof(uint16_t) * ((size_t)nmax + 1) + 16;
 }
@@ -177,8 +180,8 @@ __global__ __launch_bounds__(BE) void k_elem_block(BlockArgs a, const typename K
     const int stop = vload(done);
     extern __shared__ __align__(32) unsigned char smem_raw[];
     V4* sf4 = reinterpret_cast<V4*>(smem_raw);        // 4*CW*BE corner rows
-    V4* su4 = sf4 + 4 * CW * BE;                      // nmax
-    V4* sd4 = su4 + a.nmax;                           // nmax (HAS_D)
+    V4* su4 = sf4 + 4 * CW * BE;                      // nmax (not LC)
+    V4* sd4 = su4 + (LC ? 0 : a.nmax);                // nmax (HAS_D)
     V4* sx4 = sd4 + (HAS_D ? a.nmax : 0);             // nmax (HAS_X)
     T* s_md = reinterpret_cast<T*>(sx4 + (HAS_X ? a.nmax : 0));                  // nmax (mdiag)
     uint32_t* s_flag = reinterpret_cast<uint32_t*>(s_md + a.nmax);                // nmax
@@ -1336,6 +1339,8 @@ static hipError_t launch_elem_mat(int mat, const BlockArgs& a, const void* u, co
         // the static Hessian-vector kernels read the cached state when it is set
         if constexpr (KIND == 0) {
             if (a.lin32 && !md) {
+                // the cached state replaces the staged displacement rows
+                lds = lds_layout_bytes<0>(BE, a.nmax, true);
                 SFEM_ELEM_DYN(NEO_HOOKEAN_LIN, false);
                 break;
             }
