@@ -627,6 +627,8 @@ int sfem_create(const sfem_mesh* m, const sfem_material* mat, sfem_ctx** out)
     if (!m || !mat || !out || !m->tets || !m->X0 || !m->fixed || m->ne <= 0 || m->nn <= 0)
         return fail(SFEM_EINVAL, "sfem_create: null or empty mesh");
     if (m->nn >= (1 << 29)) return fail(SFEM_EINVAL, "sfem_create: more than 2^29 nodes");
+    // partial slots (at most 4 per element) share the node ids' 29 bits in the element kernel's LDS
+    if (m->ne >= (1 << 27)) return fail(SFEM_EINVAL, "sfem_create: more than 2^27 elements");
     if ((mat->material < SFEM_LINEAR || mat->material > SFEM_NEO_HOOKEAN) && mat->material != SFEM_COROTATIONAL_POLAR)
         return fail(SFEM_EINVAL, "sfem_create: unknown material");
     for (int64_t i = 0; i < 4 * m->ne; i++)

@@ -76,8 +76,9 @@ template <> struct KindTraits<3> {
 //   staged node rows: su4[nmax] (not on the cached state; and sd4[nmax],
 //   sx4[nmax]) = (x, y, z, 0);
 //   sx4: rest coordinates relative to the block's centre
-//   per row m / h^2 for dynamics s_md[nmax] (T), node flags s_flag[nmax]
-//   (bnodes entry) and partial slots s_slot[nmax]
+//   per row m / h^2 for dynamics s_md[nmax] (T, dynamics instances only),
+//   the row's output s_out[nmax]: its partial slot | FLAG_SHARED for a shared
+//   node, else its bnodes entry (node id and flags)
 //   incidence offsets s_ptr[nmax+1]
 template <typename T> struct Vec4;
 template <> struct Vec4<float> { typedef float4 V; };
@@ -89,16 +90,17 @@ __host__ __device__ constexpr int corner_vec4(int W_)
     return (W_ + 3) / 4;
 }
 
-// lc: the kernel on the cached Neo-Hookean state (no staged displacement rows)
+// lc: the kernel on the cached Neo-Hookean state (no staged displacement
+// rows); dyn: the implicit-dynamics instances (m / h^2 per row)
 template <int KIND>
-size_t lds_layout_bytes(int BE, int nmax, bool lc = false)
+size_t lds_layout_bytes(int BE, int nmax, bool lc = false, bool dyn = false)
 {
     typedef typename KindTraits<KIND>::T T;
     constexpr int W = KindTraits<KIND>::W;
     const size_t v4 = 4 * sizeof(T);
     const size_t rows =
         (size_t)nmax * ((lc ? 0 : 1) + (KindTraits<KIND>::HAS_D ? 1 : 0) + (KindTraits<KIND>::HAS_X ? 1 : 0));
-    return v4 * ((size_t)4 * corner_vec4<KIND>(W) * BE + rows) + (2 * sizeof(uint32_t) + sizeof(T)) * (size_t)nmax +
+    return v4 * ((size_t)4 * corner_vec4<KIND>(W) * BE + rows) + (sizeof(uint32_t) + (dyn ? sizeof(T) : 0)) * (size_t)nmax +
            sizeof(uint16_t) * ((size_t)nmax + 1) + 16;
 }
 
@@ -183,10 +185,9 @@ __global__ __launch_bounds__(BE) void k_elem_block(BlockArgs a, const typename K
     V4* su4 = sf4 + 4 * CW * BE;                      // nmax (not LC)
     V4* sd4 = su4 + (LC ? 0 : a.nmax);                // nmax (HAS_D)
     V4* sx4 = sd4 + (HAS_D ? a.nmax : 0);             // nmax (HAS_X)
-    T* s_md = reinterpret_cast<T*>(sx4 + (HAS_X ? a.nmax : 0));                  // nmax (mdiag)
-    uint32_t* s_flag = reinterpret_cast<uint32_t*>(s_md + a.nmax);                // nmax
-    int* s_slot = reinterpret_cast<int*>(s_flag + a.nmax);                        // nmax
-    uint16_t* s_ptr = reinterpret_cast<uint16_t*>(s_slot + a.nmax);               // nmax+1
+    T* s_md = reinterpret_cast<T*>(sx4 + (HAS_X ? a.nmax : 0));                  // nmax (DYN)
+    uint32_t* s_out = reinterpret_cast<uint32_t*>(s_md + (DYN ? a.nmax : 0));     // nmax
+    uint16_t* s_ptr = reinterpret_cast<uint16_t*>(s_out + a.nmax);                // nmax+1
 
     const int b = xcd_block(blockIdx.x, a.nblk);
     const int tid = threadIdx.x;
@@ -364,8 +365,7 @@ __global__ __launch_bounds__(BE) void k_elem_block(BlockArgs a, const typename K
         if (HAS_D) sd4[j] = w.dv;
         if (HAS_X) sx4[j] = w.xv;
         if (DYN) s_md[j] = owner ? w.md : T(0);
-        s_flag[j] = w.fl;
-        s_slot[j] = w.slot;
+        s_out[j] = (w.fl & FLAG_SHARED) ? ((uint32_t)w.slot | FLAG_SHARED) : w.fl;
     };
     Row r0;
     if (tid < nb) load_row(tid, r0);
@@ -573,10 +573,10 @@ __global__ __launch_bounds__(BE) void k_elem_block(BlockArgs a, const typename K
             }
         }
         if (has && half == 0) {
-            const uint32_t fl = s_flag[j];
-            const uint32_t node = fl & NODE_MASK;
+            const uint32_t fl = s_out[j];
+            const uint32_t node = fl & NODE_MASK;     // the partial slot of a shared row
             if (fl & FLAG_SHARED) {
-                T* pp = partial + (size_t)W * s_slot[j];
+                T* pp = partial + (size_t)W * node;
 #pragma unroll
                 for (int q = 0; q < W; q++) pp[q] = acc[q];
             } else {
@@ -1303,14 +1303,18 @@ static hipError_t launch_elem_mat(int mat, const BlockArgs& a, const void* u, co
     const T* md = (const T*)(sizeof(T) == 8 ? (const void*)a.mdiag64 : (const void*)a.mdiag32);
     // only PF launches write the flag (the test of the previous iteration)
     int* done = const_cast<int*>(done_in ? done_in : a.zero_flag);
+    // the dynamics instances also stage m / h^2 per row (lds: the static layout)
+    const size_t dyn_lds = (size_t)a.nmax * sizeof(T);
     // with events: hipExtLaunchKernelGGL records them at the dispatch's own start
     // and end, so their interval is the kernel's execution time
 #define SFEM_ELEM_DYN(M, D)                                                                                          \
     if (ev0)                                                                                                         \
-        hipExtLaunchKernelGGL((k_elem_block<KIND, M, BE, D, SG>), grid, block, (uint32_t)lds, s, ev0, ev1, 0u, a,   \
+        hipExtLaunchKernelGGL((k_elem_block<KIND, M, BE, D, SG>), grid, block, (uint32_t)(lds + (D ? dyn_lds : 0)), \
+                              s, ev0, ev1, 0u, a,                                                                    \
                               (const T*)u, (const T*)d, (T*)out, (T*)partial, ext, mask_fixed, done, dot, md, pf, mg); \
     else                                                                                                             \
-        hipLaunchKernelGGL((k_elem_block<KIND, M, BE, D, SG>), grid, block, lds, s, a, (const T*)u, (const T*)d,     \
+        hipLaunchKernelGGL((k_elem_block<KIND, M, BE, D, SG>), grid, block, lds + (D ? dyn_lds : 0), s, a,          \
+                           (const T*)u, (const T*)d,                                                                  \
                            (T*)out, (T*)partial, ext, mask_fixed, done, dot, md, pf, mg)
     // the multigrid stages have no dynamics instance (static solves only)
 #define SFEM_ELEM_CASE(M)                                                                                            \
@@ -1340,7 +1344,7 @@ static hipError_t launch_elem_mat(int mat, const BlockArgs& a, const void* u, co
         if constexpr (KIND == 0) {
             if (a.lin32 && !md) {
                 // the cached state replaces the staged displacement rows
-                lds = lds_layout_bytes<0>(BE, a.nmax, true);
+                lds = lds_layout_bytes<0>(BE, a.nmax, true, false);
                 SFEM_ELEM_DYN(NEO_HOOKEAN_LIN, false);
                 break;
             }
@@ -1353,11 +1357,11 @@ static hipError_t launch_elem_mat(int mat, const BlockArgs& a, const void* u, co
     return hipGetLastError();
 }
 
+// the static layout (launch_elem_mat adds the dynamics rows); dyn: with them
 template <int KIND>
-static size_t lds_bytes(const BlockArgs& a)
+static size_t lds_bytes(const BlockArgs& a, bool dyn = false)
 {
-    typedef typename KindTraits<KIND>::T T;
-    return lds_layout_bytes<KIND>(BLOCK_E, a.nmax);
+    return lds_layout_bytes<KIND>(BLOCK_E, a.nmax, false, dyn);
 }
 
 template <typename T, int W>
@@ -1416,9 +1420,9 @@ hipError_t launch_lin_state(const BlockArgs& a, const float* u32, float* lin, hi
 
 size_t elem_lds_max(const BlockArgs& a)
 {
-    size_t m = lds_bytes<0>(a);
-    if (lds_bytes<1>(a) > m) m = lds_bytes<1>(a);
-    if (lds_bytes<2>(a) > m) m = lds_bytes<2>(a);
+    size_t m = lds_bytes<0>(a, true);
+    if (lds_bytes<1>(a, true) > m) m = lds_bytes<1>(a, true);
+    if (lds_bytes<2>(a, true) > m) m = lds_bytes<2>(a, true);
     return m;
 }
 
@@ -1457,7 +1461,7 @@ hipError_t set_elem_lds_limits(const BlockArgs& a)
 #undef SFEM_FN_PF
 #undef SFEM_FN
     };
-    const size_t need[] = {lds_bytes<0>(a), lds_bytes<1>(a), lds_bytes<2>(a), lds_bytes<3>(a)};
+    const size_t need[] = {lds_bytes<0>(a, true), lds_bytes<1>(a, true), lds_bytes<2>(a, true), lds_bytes<3>(a, true)};
     if (elem_lds_max(a) <= 64 * 1024) return hipSuccess;
     int dev = 0, optin = 0;
     hipError_t e = hipGetDevice(&dev);
