@@ -764,18 +764,24 @@ __global__ __launch_bounds__(CG_BLOCK) void k_cg_update(int64_t n3, float* __res
 #pragma unroll
         for (int c = 0; c < 3; c++) qv[c] = nq.x == -1 ? qv[c] : 0.f;
         if (nq.x >= 0) {
-            // up to 4 partial rows in flight per round trip, summed in slot order
+            // up to 4 partial rows in flight per round trip, summed in slot order;
+            // the loads are unconditional (an index past the node's last slot
+            // reads that slot again and is not added): a load under a branch is
+            // waited for where the branch joins
             const int k1 = nq.y;
             for (int k = nq.x; k < k1; k += 4) {
                 float t[4][3];
 #pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    const size_t kk = (size_t)(k + j < k1 ? k + j : k1 - 1);
+#pragma unroll
+                    for (int c = 0; c < 3; c++) t[j][c] = partial[3 * kk + c];
+                }
+#pragma unroll
                 for (int j = 0; j < 4; j++)
+                    if (k + j < k1)
 #pragma unroll
-                    for (int c = 0; c < 3; c++) t[j][c] = k + j < k1 ? partial[3 * (size_t)(k + j) + c] : 0.f;
-#pragma unroll
-                for (int j = 0; j < 4; j++)
-#pragma unroll
-                    for (int c = 0; c < 3; c++) qv[c] += t[j][c];
+                        for (int c = 0; c < 3; c++) qv[c] += t[j][c];
             }
         }
     };
